@@ -1,0 +1,31 @@
+"""annongpu_amd -- MI355X-native variational Monte Carlo path of ANNonGPU.
+
+Drop-in for the reference's pyANNonGPU on the VMC path
+(``import annongpu_amd as pyANNonGPU``): wave functions PsiRBM / PsiDeep /
+PsiCNN, Pauli-sum Operators, MonteCarloSpins / ExactSummationSpins ensembles,
+ExpectationValue, TDVP (+ the SR solve) and the free functions of
+pyANNonGPU/main.cpp.template:494-540, all computed by hand-written gfx950 HIP
+kernels behind the C ABI in include/annongpu.h.
+"""
+from .pauli import PauliExpression, sigma_x, sigma_y, sigma_z, identity
+from .api import (
+    Spins, PsiRBM, PsiDeep, PsiCNN, Operator, MonteCarloSpins, ExactSummationSpins,
+    ExpectationValue, TDVP, log_psi_s, psi_O_k, psi_O_k_vector, log_psi, psi_vector,
+    log_psi_vector, apply_operator, local_energies, activation_function, setDevice,
+    start_profiling, stop_profiling, profiler_query, profiler_reset,
+)
+from .constructors import (
+    new_RBM, new_deep_neural_network, new_convolutional_network, seeded_rbm, seeded_rbm_deep,
+    seeded_cnn, deep_connections,
+)
+from .json_numpy import NumpyEncoder, NumpyDecoder
+from . import hamiltonians
+
+__all__ = [
+    "PauliExpression", "sigma_x", "sigma_y", "sigma_z", "identity", "Spins", "PsiRBM", "PsiDeep", "PsiCNN",
+    "Operator", "MonteCarloSpins", "ExactSummationSpins", "ExpectationValue", "TDVP", "log_psi_s", "psi_O_k",
+    "psi_O_k_vector", "log_psi", "psi_vector", "log_psi_vector", "apply_operator", "local_energies",
+    "activation_function", "setDevice", "start_profiling", "stop_profiling", "profiler_query", "profiler_reset",
+    "new_RBM", "new_deep_neural_network", "new_convolutional_network", "seeded_rbm", "seeded_rbm_deep",
+    "seeded_cnn", "deep_connections", "NumpyEncoder", "NumpyDecoder", "hamiltonians",
+]

@@ -1,0 +1,744 @@
+"""Python classes and functions with the names and argument order of the
+reference's pyANNonGPU bindings (pyANNonGPU/main.cpp.template:65-541) and
+wrappers (pyANNonGPU/PsiRBM.py, PsiDeep.py, PsiCNN.py), implemented on the
+C ABI of libannongpu.so.
+
+The reference's ``gpu`` flags are accepted and ignored: every computation runs
+on the MI355X; there is no CPU execution path in this package.
+"""
+import ctypes
+import json
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib, check, ptr, ref, as_c128, c2, handle
+from .json_numpy import NumpyEncoder, NumpyDecoder
+from .pauli import PauliExpression
+
+MAX_WORDS = 4
+
+
+def _words_for(n):
+    return max(1, (int(n) + 63) // 64)
+
+
+# ----------------------------------------------------------------- basis
+
+class Spins:
+    """Spin configuration (include/basis/Spins.h:20-414): bit i set <=> s_i = +1.
+
+    ``Spins(configuration, num_spins)`` masks to num_spins bits, like the
+    reference's Spins_t<1>(configuration, num_spins) (Spins.h:278-285); any
+    number of sites up to 256 is accepted here.
+    """
+
+    __slots__ = ("configuration",)
+
+    def __init__(self, configuration=0, num_spins=256):
+        c = int(configuration)
+        if num_spins < 256:
+            c &= (1 << int(num_spins)) - 1
+        self.configuration = c
+
+    @staticmethod
+    def enumerate(index):
+        return Spins(int(index))
+
+    @staticmethod
+    def from_array(arr):
+        c = 0
+        for i, s in enumerate(arr):
+            if s > 0:
+                c |= 1 << i
+        return Spins(c)
+
+    def roll(self, shift, N):
+        shift %= N
+        mask = (1 << N) - 1
+        c = self.configuration & mask
+        return Spins(((c << shift) | (c >> (N - shift))) & mask, N)
+
+    def array(self, num_spins):
+        return np.array([1.0 if (self.configuration >> i) & 1 else -1.0 for i in range(num_spins)])
+
+    def flip(self, position):
+        return Spins(self.configuration ^ (1 << position))
+
+    def words(self, n=MAX_WORDS):
+        return np.array([(self.configuration >> (64 * w)) & 0xFFFFFFFFFFFFFFFF for w in range(n)],
+                        dtype=np.uint64)
+
+    def __eq__(self, other):
+        return isinstance(other, Spins) and other.configuration == self.configuration
+
+    def __hash__(self):
+        return hash(self.configuration)
+
+    def __repr__(self):
+        return f"Spins({self.configuration:#x})"
+
+
+def _conf_words(spins):
+    if isinstance(spins, Spins):
+        return spins.words()
+    if isinstance(spins, (int, np.integer)):
+        return Spins(int(spins)).words()
+    a = np.ascontiguousarray(spins, dtype=np.uint64).ravel()
+    out = np.zeros(MAX_WORDS, dtype=np.uint64)
+    out[:a.size] = a
+    return out
+
+
+# ---------------------------------------------------------- wave functions
+
+class _Psi:
+    _h = None
+
+    def __del__(self):
+        if self._h is not None and self._h.value:
+            try:
+                lib.ann_psi_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _info(self):
+        p, n, s = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib.ann_psi_info(self._h, ref(p), ref(n), ref(s)))
+        return p.value, n.value, s.value
+
+    @property
+    def num_params(self):
+        return self._info()[0]
+
+    @property
+    def params(self):
+        out = np.zeros(self.num_params, dtype=np.complex128)
+        check(lib.ann_psi_get_params(self._h, ptr(out)))
+        return out
+
+    @params.setter
+    def params(self, value):
+        v = as_c128(value).ravel()
+        if v.size != self.num_params:
+            raise ValueError(f"expected {self.num_params} parameters, got {v.size}")
+        check(lib.ann_psi_set_params(self._h, ptr(v)))
+        self._on_params(v)
+
+    def _on_params(self, v):
+        pass
+
+    @property
+    def log_prefactor(self):
+        out = np.zeros(2)
+        check(lib.ann_psi_get_log_prefactor(self._h, ptr(out)))
+        return complex(out[0], out[1])
+
+    @log_prefactor.setter
+    def log_prefactor(self, value):
+        check(lib.ann_psi_set_log_prefactor(self._h, ptr(c2(value))))
+
+    @property
+    def precision(self):
+        return self._precision
+
+    @precision.setter
+    def precision(self, value):
+        code = {"fp64": _lib.FP64, "fp32": _lib.FP32}[value]
+        check(lib.ann_psi_set_precision(self._h, code))
+        self._precision = value
+
+    @property
+    def symmetric(self):
+        return False
+
+    @property
+    def gpu(self):
+        return True
+
+    def copy(self):
+        h = handle()
+        check(lib.ann_psi_copy(self._h, ref(h)))
+        c = object.__new__(type(self))
+        c.__dict__.update(self.__dict__)
+        c._h = h
+        return c
+
+    def __pos__(self):
+        return self.copy()
+
+    # reference: psi._vector(ensemble) / psi.vector property -> bound method
+    def _vector(self, ensemble):
+        return psi_vector(self, ensemble)
+
+    @property
+    def vector(self):
+        return self._vector
+
+    def norm(self, exact_summation):
+        out = ctypes.c_double()
+        check(lib.ann_psi_norm(self._h, exact_summation._h, ref(out)))
+        return out.value
+
+    def normalize(self, exact_summation):
+        """PsiRBM.py:39-40: log_prefactor -= log(norm)"""
+        self.log_prefactor -= np.log(self.norm(exact_summation))
+
+    def calibrate(self, ensemble):
+        """PsiRBM.py:43-49"""
+        if type(ensemble).__name__.startswith("ExactSummation"):
+            self.normalize(ensemble)
+            self.log_prefactor -= 1j * log_psi(self, ensemble).imag
+        else:
+            self.log_prefactor = 0
+            self.log_prefactor -= log_psi(self, ensemble)
+
+
+class PsiRBM(_Psi):
+    """PsiRBM(W, final_weight, log_prefactor, gpu) -- include/quantum_state/PsiRBM.hpp:377-395.
+
+    log psi(s) = log_prefactor + final_weight * sum_j lc(theta_j), theta_j = sum_i W_ij s_i.
+    """
+
+    def __init__(self, W, final_weight=1.0, log_prefactor=0.0, gpu=True):
+        W = as_c128(W)
+        if W.ndim != 2:
+            raise ValueError("W must be a 2d array [N, M]")
+        self._N, self._M = W.shape
+        self._final_weight = complex(final_weight)
+        self._precision = "fp64"
+        self._h = handle()
+        check(lib.ann_psi_create_rbm(ref(self._h), self._N, self._M, ptr(W), ptr(c2(final_weight)),
+                                     ptr(c2(log_prefactor))))
+
+    @property
+    def N(self):
+        return self._N
+
+    @property
+    def M(self):
+        return self._M
+
+    @property
+    def num_sites(self):
+        return self._N
+
+    @property
+    def W(self):
+        return self.params.reshape(self._N, self._M)
+
+    @property
+    def final_weight(self):
+        return self._final_weight
+
+    def to_json(self):
+        obj = dict(type="PsiRBM", W=self.W, final_weight=self.final_weight,
+                   log_prefactor_re=self.log_prefactor.real, log_prefactor_im=self.log_prefactor.imag)
+        return json.loads(json.dumps(obj, cls=NumpyEncoder))
+
+    @staticmethod
+    def from_json(json_obj, gpu=True):
+        obj = json.loads(json.dumps(json_obj, cls=NumpyEncoder), cls=NumpyDecoder)
+        return PsiRBM(obj["W"], obj["final_weight"], obj["log_prefactor_re"] + 1j * obj["log_prefactor_im"], gpu)
+
+
+class PsiDeep(_Psi):
+    """PsiDeep(num_sites, a, b_list, connections_list, W_list, final_weights, log_prefactor, gpu)
+    -- include/quantum_state/PsiDeep.hpp:504-578.
+
+    Parameter order [a | b_1 | W_1 | b_2 | W_2 ...] (source/quantum_state/PsiDeep.cu:245-270).
+    As in the reference, log psi does not contain the a.s term (PsiDeep.hpp:260-262)
+    although O_k reports s_i for the a parameters.
+    """
+
+    def __init__(self, num_sites, a, b, connections, W, final_weights, log_prefactor=0.0, gpu=True):
+        self._num_sites = int(num_sites)
+        a = as_c128(a).ravel()
+        self._N = a.size
+        self._layers = []
+        sizes, conns, c_ptrs, w_ptrs, b_ptrs = [], [], [], [], []
+        keep = []
+        for bl, cl, wl in zip(b, connections, W):
+            bl = as_c128(bl).ravel()
+            cl = np.ascontiguousarray(cl, dtype=np.uint32)
+            wl = as_c128(wl)
+            if cl.shape != wl.shape or wl.shape[1] != bl.size:
+                raise ValueError("connections / W / b shapes do not match")
+            sizes.append(bl.size)
+            conns.append(wl.shape[0])
+            keep += [bl, cl, wl]
+            c_ptrs.append(cl.ctypes.data)
+            w_ptrs.append(wl.ctypes.data)
+            b_ptrs.append(bl.ctypes.data)
+            self._layers.append((bl.size, wl.shape[0], cl.copy()))
+        L = len(sizes)
+        fw = as_c128(final_weights).ravel()
+        sizes_a = np.array(sizes, dtype=np.int32)
+        conns_a = np.array(conns, dtype=np.int32)
+        cp = (ctypes.c_void_p * L)(*c_ptrs)
+        wp = (ctypes.c_void_p * L)(*w_ptrs)
+        bp = (ctypes.c_void_p * L)(*b_ptrs)
+        self._final_weights = fw.copy()
+        self._precision = "fp64"
+        self._h = handle()
+        check(lib.ann_psi_create_deep(ref(self._h), self._num_sites, self._N, ptr(a), L, ptr(sizes_a), ptr(conns_a),
+                                      ctypes.cast(cp, ctypes.c_void_p), ctypes.cast(wp, ctypes.c_void_p),
+                                      ctypes.cast(bp, ctypes.c_void_p), ptr(fw), ptr(c2(log_prefactor))))
+        del keep
+
+    @property
+    def N(self):
+        return self._N
+
+    @property
+    def num_sites(self):
+        return self._num_sites
+
+    def _split(self):
+        p = self.params
+        a = p[:self._N]
+        o = self._N
+        bs, Ws = [], []
+        for size, conn, _ in self._layers:
+            bs.append(p[o:o + size])
+            o += size
+            Ws.append(p[o:o + conn * size].reshape(conn, size))
+            o += conn * size
+        return a, bs, Ws
+
+    @property
+    def a(self):
+        return self._split()[0]
+
+    input_weights = a
+
+    @property
+    def b(self):
+        return self._split()[1]
+
+    @property
+    def W(self):
+        return self._split()[2]
+
+    @property
+    def connections(self):
+        return [c.copy() for _, _, c in self._layers]
+
+    @property
+    def final_weights(self):
+        return self._final_weights.copy()
+
+    @property
+    def first_layer_params_slice(self):
+        return slice(self._N, self._N + self.W[0].size + self.b[0].size)
+
+    def to_json(self):
+        obj = dict(type="PsiDeep", num_sites=self.num_sites, a=self.a, b=self.b, connections=self.connections,
+                   W=self.W, final_weights=self.final_weights, log_prefactor_re=self.log_prefactor.real,
+                   log_prefactor_im=self.log_prefactor.imag)
+        return json.loads(json.dumps(obj, cls=NumpyEncoder))
+
+    @staticmethod
+    def from_json(json_obj, gpu=True):
+        obj = json.loads(json.dumps(json_obj, cls=NumpyEncoder), cls=NumpyDecoder)
+        return PsiDeep(obj["num_sites"], obj["a"], obj["b"], obj["connections"], obj["W"], obj["final_weights"],
+                       obj["log_prefactor_re"] + 1j * obj["log_prefactor_im"], gpu)
+
+
+class PsiCNN(_Psi):
+    """PsiCNN(extent, num_channels_list, connectivity_list, symmetry_classes, params,
+    final_factor, log_prefactor, gpu) -- include/quantum_state/PsiCNN.hpp:308-340 (dim = 3).
+
+    Extents of fewer than three dimensions are padded with leading 1s.
+    """
+
+    def __init__(self, extent, num_channels_list, connectivity_list, symmetry_classes, params, final_factor,
+                 log_prefactor=0.0, gpu=True):
+        extent = [int(e) for e in extent]
+        pad = 3 - len(extent)
+        if pad < 0:
+            raise ValueError("at most 3 dimensions")
+        self._extent = [1] * pad + extent
+        self._channels = np.array(num_channels_list, dtype=np.int32).ravel()
+        conn = np.array(connectivity_list, dtype=np.int32).reshape(len(self._channels), -1)
+        conn = np.concatenate([np.ones((conn.shape[0], pad), dtype=np.int32), conn], axis=1)
+        self._connectivity = np.ascontiguousarray(conn)
+        N = int(np.prod(self._extent))
+        if symmetry_classes is None:
+            symmetry_classes = np.zeros(N)
+        self._sym = np.ascontiguousarray(symmetry_classes, dtype=np.uint32).ravel()
+        self._final_factor = float(final_factor)
+        params = as_c128(params).ravel()
+        self._precision = "fp64"
+        ext = np.array(self._extent, dtype=np.int32)
+        self._h = handle()
+        check(lib.ann_psi_create_cnn(ref(self._h), ptr(ext), len(self._channels), ptr(self._channels),
+                                     ptr(self._connectivity), ptr(self._sym), params.size, ptr(params),
+                                     self._final_factor, ptr(c2(log_prefactor))))
+
+    @property
+    def N(self):
+        return int(np.prod(self._extent))
+
+    num_sites = N
+
+    @property
+    def extent(self):
+        return list(self._extent)
+
+    @property
+    def dim(self):
+        return 3
+
+    @property
+    def num_channels_list(self):
+        return self._channels.copy()
+
+    @property
+    def connectivity_list(self):
+        return self._connectivity.copy()
+
+    @property
+    def symmetry_classes(self):
+        return self._sym.copy()
+
+    @property
+    def num_symmetry_classes(self):
+        return len(set(self._sym.tolist()))
+
+    @property
+    def final_factor(self):
+        return self._final_factor
+
+    def init_gradient(self, num_steps):
+        """Kept for interface compatibility (PsiCNN.cpp:250-254); storage is managed internally."""
+
+    def channel_link(self, layer, i, j):
+        """PsiCNN.py:67-83"""
+        ns = self.num_symmetry_classes
+        offset = 0
+        for l in range(layer):
+            prev = self._channels[l - 1] if l > 0 else 1
+            offset += self._channels[l] * prev * ns * int(np.prod(self._connectivity[l]))
+        c = int(np.prod(self._connectivity[layer]))
+        offset += (i * self._channels[layer] + j) * ns * c
+        return self.params[offset:offset + ns * c]
+
+    def to_json(self):
+        obj = dict(type="PsiCNN", extent=self.extent, num_channels_list=self.num_channels_list,
+                   connectivity_list=self.connectivity_list, symmetry_classes=self.symmetry_classes,
+                   params=self.params, final_factor=self.final_factor, log_prefactor_re=self.log_prefactor.real,
+                   log_prefactor_im=self.log_prefactor.imag)
+        return json.loads(json.dumps(obj, cls=NumpyEncoder))
+
+    @staticmethod
+    def from_json(json_obj, gpu=True):
+        obj = json.loads(json.dumps(json_obj, cls=NumpyEncoder), cls=NumpyDecoder)
+        return PsiCNN(obj["extent"], obj["num_channels_list"], obj["connectivity_list"], obj["symmetry_classes"],
+                      obj["params"], obj["final_factor"], obj["log_prefactor_re"] + 1j * obj["log_prefactor_im"],
+                      gpu)
+
+
+# ------------------------------------------------------------------ operator
+
+class Operator:
+    """Operator(PauliExpression, gpu) -- source/operator/Operator.cpp:18-39."""
+
+    def __init__(self, expr, gpu=True):
+        if not isinstance(expr, PauliExpression):
+            raise TypeError("Operator expects an annongpu_amd.PauliExpression")
+        terms = list(expr)
+        n = len(terms)
+        words = MAX_WORDS
+        a = np.zeros((max(n, 1), words), dtype=np.uint64)
+        b = np.zeros((max(n, 1), words), dtype=np.uint64)
+        c = np.zeros(max(n, 1), dtype=np.complex128)
+        for k, ((ma, mb), coef) in enumerate(terms):
+            if max(ma.bit_length(), mb.bit_length()) > 64 * words:
+                raise ValueError("Pauli string beyond 256 sites")
+            for w in range(words):
+                a[k, w] = (ma >> (64 * w)) & 0xFFFFFFFFFFFFFFFF
+                b[k, w] = (mb >> (64 * w)) & 0xFFFFFFFFFFFFFFFF
+            c[k] = coef
+        self.expr = expr
+        self._h = handle()
+        check(lib.ann_operator_create(ref(self._h), n, words, ptr(a), ptr(b), ptr(c)))
+        self.num_strings = n
+
+    def __del__(self):
+        if self._h is not None and self._h.value:
+            try:
+                lib.ann_operator_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+# ----------------------------------------------------------------- ensembles
+
+class _Ensemble:
+    _h = None
+
+    def __del__(self):
+        if self._h is not None and self._h.value:
+            try:
+                lib.ann_ensemble_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def gpu(self):
+        return True
+
+    @property
+    def num_steps(self):
+        n = ctypes.c_uint64()
+        check(lib.ann_ensemble_num_steps(self._h, ref(n)))
+        return n.value
+
+    def last_samples(self):
+        """Configurations of the last sampling call, shape [num_steps, words] (uint64)."""
+        cnt, words = ctypes.c_int64(), ctypes.c_int32()
+        check(lib.ann_ensemble_last_samples(self._h, None, 0, ref(cnt), ref(words)))
+        out = np.zeros((cnt.value, words.value), dtype=np.uint64)
+        check(lib.ann_ensemble_last_samples(self._h, ptr(out), out.size, ref(cnt), ref(words)))
+        return out
+
+
+class MonteCarloSpins(_Ensemble):
+    """MonteCarloSpins(num_samples, num_sweeps, num_thermalization_sweeps, num_markov_chains, gpu)
+    -- include/ensembles/MonteCarlo.hpp:249-259.
+
+    Additive keywords: ``move`` ('flip' or 'exchange'), ``total_z`` (2 S_z of the
+    sector used by exchange moves), ``seed`` (Philox key), ``persistent`` (keep the
+    chains between calls instead of re-initialising and re-thermalising them).
+    """
+
+    def __init__(self, num_samples, num_sweeps, num_thermalization_sweeps, num_markov_chains, gpu=True,
+                 move="flip", total_z=0, seed=0, persistent=False):
+        mv = {"flip": _lib.MOVE_FLIP, "exchange": _lib.MOVE_EXCHANGE}[move]
+        self.move = move
+        self._h = handle()
+        check(lib.ann_monte_carlo_create(ref(self._h), int(num_samples), int(num_sweeps),
+                                         int(num_thermalization_sweeps), int(num_markov_chains), mv, int(total_z),
+                                         int(seed) & 0xFFFFFFFFFFFFFFFF))
+        self.persistent = persistent
+
+    @property
+    def persistent(self):
+        return self._persistent
+
+    @persistent.setter
+    def persistent(self, value):
+        check(lib.ann_monte_carlo_set_persistent(self._h, 1 if value else 0))
+        self._persistent = bool(value)
+
+    @property
+    def acceptance_rate(self):
+        a, r = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib.ann_ensemble_acceptance(self._h, ref(a), ref(r)))
+        tot = a.value + r.value
+        return a.value / tot if tot else float("nan")
+
+
+class ExactSummationSpins(_Ensemble):
+    """ExactSummationSpins(num_sites, gpu) -- source/ensembles/ExactSummation.cu:16-25."""
+
+    def __init__(self, num_sites, gpu=True):
+        self._h = handle()
+        check(lib.ann_exact_summation_create(ref(self._h), int(num_sites)))
+        self.num_sites = int(num_sites)
+
+
+# --------------------------------------------------------- network functions
+
+def log_psi_s(psi, spins):
+    """PsiVector.cu.template:101-133"""
+    c = _conf_words(spins)
+    out = np.zeros(2)
+    check(lib.ann_log_psi_s(psi._h, ptr(c), ptr(out)))
+    return complex(out[0], out[1])
+
+
+def psi_O_k(psi, spins):
+    """PsiOkVector.cu.template:40-74"""
+    c = _conf_words(spins)
+    out = np.zeros(psi.num_params, dtype=np.complex128)
+    check(lib.ann_psi_O_k(psi._h, ptr(c), ptr(out)))
+    return out
+
+
+def psi_O_k_vector(psi, ensemble):
+    """PsiOkVector.cu.template:8-37 (unweighted sum over the ensemble)"""
+    out = np.zeros(psi.num_params, dtype=np.complex128)
+    check(lib.ann_psi_O_k_vector(psi._h, ensemble._h, ptr(out)))
+    return out
+
+
+def log_psi(psi, ensemble):
+    """PsiVector.cu.template:73-99 (sum_s w_s log psi(s))"""
+    out = np.zeros(2)
+    check(lib.ann_log_psi(psi._h, ensemble._h, ptr(out)))
+    return complex(out[0], out[1])
+
+
+def psi_vector(psi, ensemble):
+    out = np.zeros(ensemble.num_steps, dtype=np.complex128)
+    check(lib.ann_psi_vector(psi._h, ensemble._h, ptr(out)))
+    return out
+
+
+def log_psi_vector(psi, ensemble):
+    out = np.zeros(ensemble.num_steps, dtype=np.complex128)
+    check(lib.ann_log_psi_vector(psi._h, ensemble._h, ptr(out)))
+    return out
+
+
+def apply_operator(psi, op, ensemble):
+    """ApplyOperator.cu.template:9-44: psi(s) * E_loc(s) = (H psi)(s)"""
+    out = np.zeros(ensemble.num_steps, dtype=np.complex128)
+    check(lib.ann_apply_operator(psi._h, op._h, ensemble._h, ptr(out)))
+    return out
+
+
+def local_energies(op, psi, ensemble):
+    """Additive: E_loc(s) for every sample of one sampling call."""
+    out = np.zeros(ensemble.num_steps, dtype=np.complex128)
+    check(lib.ann_local_energies(op._h, psi._h, ensemble._h, ptr(out), out.size))
+    return out
+
+
+def activation_function(x, layer=0):
+    """main.cpp.template:534-536"""
+    out = np.zeros(2)
+    check(lib.ann_activation_function(ptr(c2(x)), int(layer), ptr(out)))
+    return complex(out[0], out[1])
+
+
+def setDevice(device):
+    check(lib.ann_set_device(int(device)))
+
+
+def start_profiling():
+    check(lib.ann_profiler_start())
+
+
+def stop_profiling():
+    check(lib.ann_profiler_stop())
+
+
+def profiler_query(name):
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    check(lib.ann_profiler_query(name.encode(), ref(ms), ref(n)))
+    return ms.value, n.value
+
+
+def profiler_reset():
+    check(lib.ann_profiler_reset())
+
+
+class ExpectationValue:
+    """ExpectationValue(gpu) -- network_functions/ExpectationValue.hpp:21-42"""
+
+    def __init__(self, gpu=True):
+        pass
+
+    def __call__(self, operator_, psi, ensemble):
+        if isinstance(operator_, (list, tuple)):
+            return np.array([self(op, psi, ensemble) for op in operator_])
+        out = np.zeros(2)
+        check(lib.ann_expectation_value(operator_._h, psi._h, ensemble._h, ptr(out)))
+        return complex(out[0], out[1])
+
+    def fluctuation(self, operator_, psi, ensemble):
+        sigma = ctypes.c_double()
+        out = np.zeros(2)
+        check(lib.ann_expectation_fluctuation(operator_._h, psi._h, ensemble._h, ref(sigma), ptr(out)))
+        return sigma.value, complex(out[0], out[1])
+
+    def gradient(self, operator_, psi, ensemble):
+        g = np.zeros(psi.num_params, dtype=np.complex128)
+        out = np.zeros(2)
+        check(lib.ann_expectation_gradient(operator_._h, psi._h, ensemble._h, ptr(g), ptr(out)))
+        return g, complex(out[0], out[1])
+
+
+class TDVP:
+    """TDVP(num_params, gpu) -- network_functions/TDVP.hpp:16-101.
+
+    Additive: ``solve(shift, method='cholesky'|'cg')`` for (S + shift I) x = F.
+    """
+
+    def __init__(self, num_params, gpu=True):
+        self.num_params = int(num_params)
+        self._h = handle()
+        check(lib.ann_tdvp_create(ref(self._h), self.num_params))
+
+    def __del__(self):
+        if self._h is not None and self._h.value:
+            try:
+                lib.ann_tdvp_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def eval(self, op, psi, ensemble):
+        check(lib.ann_tdvp_eval(self._h, op._h, psi._h, ensemble._h, 1))
+
+    def eval_F(self, op, psi, ensemble):
+        check(lib.ann_tdvp_eval(self._h, op._h, psi._h, ensemble._h, 0))
+
+    @property
+    def S_matrix(self):
+        P = self.num_params
+        out = np.zeros((P, P), dtype=np.complex128)
+        check(lib.ann_tdvp_get_S(self._h, ptr(out)))
+        return out
+
+    @property
+    def F_vector(self):
+        out = np.zeros(self.num_params, dtype=np.complex128)
+        check(lib.ann_tdvp_get_F(self._h, ptr(out)))
+        return out
+
+    @property
+    def O_k_vector(self):
+        out = np.zeros(self.num_params, dtype=np.complex128)
+        check(lib.ann_tdvp_get_O(self._h, ptr(out)))
+        return out
+
+    def _energy(self):
+        e = np.zeros(2)
+        e2, var = ctypes.c_double(), ctypes.c_double()
+        check(lib.ann_tdvp_get_energy(self._h, ptr(e), ref(e2), ref(var)))
+        return complex(e[0], e[1]), e2.value, var.value
+
+    @property
+    def E_local(self):
+        return self._energy()[0]
+
+    @property
+    def E2_local(self):
+        return self._energy()[1]
+
+    @property
+    def var_H(self):
+        return self._energy()[2]
+
+    def S_dot_vector(self, vec, ensemble=None):
+        v = as_c128(vec).ravel()
+        out = np.zeros(self.num_params, dtype=np.complex128)
+        check(lib.ann_tdvp_S_dot_vector(self._h, ptr(v), ptr(out)))
+        return out
+
+    def solve(self, shift=1e-3, method="cholesky", rtol=1e-6, max_iter=500, raise_on_failure=True):
+        m = {"cholesky": _lib.SOLVE_CHOLESKY, "cg": _lib.SOLVE_CG}[method]
+        x = np.zeros(self.num_params, dtype=np.complex128)
+        it, res = ctypes.c_int32(), ctypes.c_double()
+        rc = lib.ann_tdvp_solve(self._h, float(shift), m, float(rtol), int(max_iter), ptr(x), ref(it), ref(res))
+        self.last_solve = dict(iterations=it.value, residual=res.value)
+        if rc != 0 and (raise_on_failure or rc != _lib.ANN_E_NUMERIC):
+            check(rc)
+        return x

@@ -1,0 +1,754 @@
+// capi.cpp -- the extern "C" boundary declared in include/annongpu.h.
+//
+// Every function catches C++ exceptions and turns them into a status code plus a
+// thread-local message (the reference throws std::runtime_error from CUDA_CHECK,
+// include/types.h:139-151, surfaced by pybind11 as RuntimeError).
+#include "annongpu.h"
+#include "runtime.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+struct ann_psi_s {
+    ann::Psi impl;
+};
+struct ann_operator_s {
+    ann::Operator impl;
+};
+struct ann_ensemble_s {
+    ann::Ensemble impl;
+};
+struct ann_tdvp_s {
+    ann::Tdvp impl;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return ANN_OK;
+    } catch (const ann::HipError& e) {
+        g_error = e.what();
+        return ANN_E_DEVICE;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return ANN_E_INVALID;
+    } catch (const std::logic_error& e) {
+        g_error = e.what();
+        return ANN_E_UNSUPPORTED;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return ANN_E_DEVICE;
+    }
+}
+
+using ann::cd;
+
+inline const cd* C(const double* p) { return reinterpret_cast<const cd*>(p); }
+inline cd* C(double* p) { return reinterpret_cast<cd*>(p); }
+
+void need(const void* p, const char* what) {
+    if (!p) throw std::invalid_argument(std::string("null pointer: ") + what);
+}
+
+ann::Samples& single_sample(ann::Psi& psi, const uint64_t* conf, ann::Samples& s) {
+    psi.sync_device();
+    s.count = 1;
+    s.words = psi.words;
+    s.Mpad = psi.Mpad;
+    s.precision = psi.precision;
+    s.confs.upload(conf, psi.words);
+    s.w.ensure(1);
+    s.log_psi.ensure(1);
+    if (psi.precision == 0)
+        s.th64.ensure(psi.Mpad);
+    else
+        s.th32.ensure(psi.Mpad);
+    ann::compute_theta(psi, s.confs.p, 1, s, 0);
+    const ann::cplx<double> lp = {psi.log_prefactor.real(), psi.log_prefactor.imag()};
+    if (psi.precision == 0)
+        ann::launch_net_log_psi<double>(psi.view64(), s.th64.p, 1, lp, s.log_psi.p, nullptr,
+                                        ann::Runtime::get().stream());
+    else
+        ann::launch_net_log_psi<float>(psi.view32(), s.th32.p, 1, lp, s.log_psi.p, nullptr,
+                                       ann::Runtime::get().stream());
+    return s;
+}
+
+void require_dense(const ann::Psi& psi) {
+    if (!psi.dense()) throw std::logic_error("this wave function kind is not available on the GPU path yet");
+}
+
+std::vector<cd> host_log_psi(ann::Samples& s) {
+    std::vector<cd> lp(s.count);
+    ann::Runtime::get().sync();
+    s.log_psi.download(reinterpret_cast<ann::cplx<double>*>(lp.data()), s.count);
+    return lp;
+}
+
+std::vector<double> host_weights(ann::Samples& s) {
+    std::vector<double> w(s.count);
+    s.w.download(w.data(), s.count);
+    return w;
+}
+
+// sum_s coef_s O_s for the last samples of a dense net, in parameter order
+template <typename T>
+void ok_sums(ann::Psi& psi, ann::Samples& s, const ann::cplx<double>* coef, int conj_g, ann::cplx<double>* out) {
+    using namespace ann;
+    hipStream_t st = Runtime::get().stream();
+    DenseView<T> net;
+    if constexpr (sizeof(T) == 8)
+        net = psi.view64();
+    else
+        net = psi.view32();
+    DevBuf<cplx<T>> g;
+    g.ensure(size_t(s.count) * psi.Mpad);
+    const cplx<T>* th;
+    if constexpr (sizeof(T) == 8)
+        th = s.th64.p;
+    else
+        th = s.th32.p;
+    launch_g_factor<T>(net, th, s.count, g.p, st);
+    DevBuf<cplx<double>> Yw, Yb, Ya;
+    Yw.ensure(size_t(psi.N) * psi.Mpad);
+    Yb.ensure(psi.Mpad);
+    Ya.ensure(psi.N);
+    launch_spin_weighted_sums<T>(net, s.confs.p, s.words, g.p, conj_g, coef, s.count, Yw.p, Yb.p, Ya.p, st);
+    launch_pack_y(net.layout, net.N, net.M, net.Mpad, net.C, net.conn, Ya.p, Yb.p, Yw.p, nullptr, 0.0,
+                  psi.num_params, out, st);
+    Runtime::get().sync();
+}
+
+void structured_sum(ann::Psi& psi, ann::Samples& s, const ann::cplx<double>* coef, int conj_g, cd* host_out) {
+    ann::DevBuf<ann::cplx<double>> out;
+    out.ensure(psi.num_params);
+    if (psi.precision == 0)
+        ok_sums<double>(psi, s, coef, conj_g, out.p);
+    else
+        ok_sums<float>(psi, s, coef, conj_g, out.p);
+    out.download(reinterpret_cast<ann::cplx<double>*>(host_out), psi.num_params);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ann_last_error(void) { return g_error.c_str(); }
+
+int ann_set_device(int device) {
+    return guard([&] { ann::Runtime::get().set_device(device); });
+}
+
+int ann_profiler_start(void) {
+    return guard([&] { ann::Runtime::get().profiling = true; });
+}
+int ann_profiler_stop(void) {
+    return guard([&] {
+        ann::Runtime::get().sync();
+        ann::Runtime::get().profiling = false;
+    });
+}
+int ann_profiler_query(const char* name, double* ms, int64_t* launches) {
+    return guard([&] {
+        need(name, "name");
+        ann::Runtime::get().sync();
+        auto& t = ann::Runtime::get().totals;
+        auto it = t.find(name);
+        if (ms) *ms = it == t.end() ? 0.0 : it->second.first;
+        if (launches) *launches = it == t.end() ? 0 : it->second.second;
+    });
+}
+int ann_profiler_reset(void) {
+    return guard([&] {
+        ann::Runtime::get().sync();
+        ann::Runtime::get().totals.clear();
+    });
+}
+
+// ------------------------------------------------------------- wave functions
+
+int ann_psi_create_rbm(ann_psi* out, int32_t N, int32_t M, const double* W, const double* final_weight,
+                       const double* log_prefactor) {
+    return guard([&] {
+        need(out, "out");
+        need(W, "W");
+        if (N <= 0 || M <= 0) throw std::invalid_argument("N and M must be positive");
+        auto p = new ann_psi_s;
+        try {
+            auto& q = p->impl;
+            q.kind = ann::PSI_RBM;
+            q.num_sites = N;
+            q.N = N;
+            q.M = M;
+            q.W.assign(C(W), C(W) + size_t(N) * M);
+            q.f = {final_weight ? *C(final_weight) : cd(1.0)};
+            q.log_prefactor = log_prefactor ? *C(log_prefactor) : cd(0.0);
+            q.finalize();
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+int ann_psi_create_deep(ann_psi* out, int32_t num_sites, int32_t N, const double* a, int32_t num_layers,
+                        const int32_t* sizes, const int32_t* conns, const uint32_t* const* connections,
+                        const double* const* weights, const double* const* biases, const double* final_weights,
+                        const double* log_prefactor) {
+    return guard([&] {
+        need(out, "out");
+        need(a, "a");
+        need(sizes, "sizes");
+        need(conns, "conns");
+        if (num_layers < 1 || num_layers > 4) throw std::invalid_argument("PsiDeep supports 1..4 hidden layers");
+        if (N != num_sites) throw std::invalid_argument("only the spin basis (N == num_sites) is supported");
+        auto p = new ann_psi_s;
+        try {
+            auto& q = p->impl;
+            q.kind = ann::PSI_DEEP;
+            q.num_sites = num_sites;
+            q.N = N;
+            q.a.assign(C(a), C(a) + N);
+            q.num_layers = num_layers;
+            int prev = N;
+            for (int l = 0; l < num_layers; l++) {
+                if (sizes[l] <= 0 || conns[l] <= 0) throw std::invalid_argument("empty layer");
+                need(connections[l], "connections");
+                need(weights[l], "weights");
+                need(biases[l], "biases");
+                q.sizes.push_back(sizes[l]);
+                q.conns.push_back(conns[l]);
+                const size_t n = size_t(sizes[l]) * conns[l];
+                q.conn_l.emplace_back(connections[l], connections[l] + n);
+                for (uint32_t c : q.conn_l.back())
+                    if (int(c) >= prev) throw std::invalid_argument("connection index out of range");
+                q.w_l.emplace_back(C(weights[l]), C(weights[l]) + n);
+                q.b_l.emplace_back(C(biases[l]), C(biases[l]) + sizes[l]);
+                prev = sizes[l];
+            }
+            need(final_weights, "final_weights");
+            q.f.assign(C(final_weights), C(final_weights) + sizes[num_layers - 1]);
+            q.log_prefactor = log_prefactor ? *C(log_prefactor) : cd(0.0);
+            q.finalize();
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+int ann_psi_create_cnn(ann_psi* out, const int32_t* extent, int32_t num_layers, const int32_t* channels,
+                       const int32_t* connectivity, const uint32_t* symmetry_classes, int32_t num_params,
+                       const double* params, double final_factor, const double* log_prefactor) {
+    return guard([&] {
+        need(out, "out");
+        need(extent, "extent");
+        need(channels, "channels");
+        need(connectivity, "connectivity");
+        need(symmetry_classes, "symmetry_classes");
+        need(params, "params");
+        if (num_layers < 1 || num_layers > 3) throw std::invalid_argument("PsiCNN supports 1..3 layers");
+        auto p = new ann_psi_s;
+        try {
+            auto& q = p->impl;
+            q.kind = ann::PSI_CNN;
+            int n = 1;
+            for (int d = 0; d < 3; d++) {
+                if (extent[d] <= 0) throw std::invalid_argument("extent must be positive");
+                q.extent[d] = extent[d];
+                n *= extent[d];
+            }
+            q.num_sites = n;
+            q.N = n;
+            q.cnn_layers = num_layers;
+            q.channels.assign(channels, channels + num_layers);
+            q.connectivity.assign(connectivity, connectivity + 3 * num_layers);
+            for (int l = 0; l < num_layers; l++)
+                for (int d = 0; d < 3; d++)
+                    if (connectivity[3 * l + d] <= 0 || connectivity[3 * l + d] > extent[d])
+                        throw std::invalid_argument("connectivity must lie in 1..extent");
+            q.sym.assign(symmetry_classes, symmetry_classes + n);
+            uint32_t mx = 0;
+            for (uint32_t c : q.sym) mx = std::max(mx, c);
+            q.num_sym = int(mx) + 1;
+            int expect = 0, prev = 1;
+            for (int l = 0; l < num_layers; l++) {
+                expect += prev * channels[l] * q.num_sym * connectivity[3 * l] * connectivity[3 * l + 1] *
+                          connectivity[3 * l + 2];
+                prev = channels[l];
+            }
+            if (expect != num_params) throw std::invalid_argument("PsiCNN parameter count mismatch");
+            q.params.assign(C(params), C(params) + num_params);
+            q.final_factor = final_factor;
+            q.log_prefactor = log_prefactor ? *C(log_prefactor) : cd(0.0);
+            q.finalize();
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+int ann_psi_copy(ann_psi src, ann_psi* out) {
+    return guard([&] {
+        need(src, "src");
+        need(out, "out");
+        auto q = src->impl.clone();
+        auto p = new ann_psi_s;
+        p->impl = std::move(*q);
+        *out = p;
+    });
+}
+
+int ann_psi_destroy(ann_psi psi) {
+    return guard([&] { delete psi; });
+}
+
+int ann_psi_info(ann_psi psi, int32_t* num_params, int32_t* N, int32_t* num_sites) {
+    return guard([&] {
+        need(psi, "psi");
+        if (num_params) *num_params = psi->impl.num_params;
+        if (N) *N = psi->impl.N;
+        if (num_sites) *num_sites = psi->impl.num_sites;
+    });
+}
+
+int ann_psi_get_params(ann_psi psi, double* params) {
+    return guard([&] {
+        need(psi, "psi");
+        need(params, "params");
+        auto p = psi->impl.get_params();
+        std::memcpy(params, p.data(), sizeof(cd) * p.size());
+    });
+}
+
+int ann_psi_set_params(ann_psi psi, const double* params) {
+    return guard([&] {
+        need(psi, "psi");
+        need(params, "params");
+        psi->impl.set_params(C(params));
+    });
+}
+
+int ann_psi_get_log_prefactor(ann_psi psi, double* value2) {
+    return guard([&] {
+        need(psi, "psi");
+        *C(value2) = psi->impl.log_prefactor;
+    });
+}
+
+int ann_psi_set_log_prefactor(ann_psi psi, const double* value2) {
+    return guard([&] {
+        need(psi, "psi");
+        psi->impl.log_prefactor = *C(value2);
+    });
+}
+
+int ann_psi_set_precision(ann_psi psi, int32_t precision) {
+    return guard([&] {
+        need(psi, "psi");
+        if (precision != ANN_FP64 && precision != ANN_FP32) throw std::invalid_argument("unknown precision");
+        psi->impl.precision = precision;
+    });
+}
+
+// ------------------------------------------------------------------ operator
+
+int ann_operator_create(ann_operator* out, int32_t num_strings, int32_t words, const uint64_t* a, const uint64_t* b,
+                        const double* coefficients) {
+    return guard([&] {
+        need(out, "out");
+        if (num_strings < 0 || words < 1 || words > ann::kMaxWords)
+            throw std::invalid_argument("bad operator dimensions");
+        if (num_strings > 0) {
+            need(a, "a");
+            need(b, "b");
+            need(coefficients, "coefficients");
+        }
+        auto p = new ann_operator_s;
+        auto& o = p->impl;
+        o.num_strings = num_strings;
+        o.a.assign(size_t(num_strings) * ann::kMaxWords, 0);
+        o.b.assign(size_t(num_strings) * ann::kMaxWords, 0);
+        for (int n = 0; n < num_strings; n++)
+            for (int w = 0; w < words; w++) {
+                o.a[size_t(n) * ann::kMaxWords + w] = a[size_t(n) * words + w];
+                o.b[size_t(n) * ann::kMaxWords + w] = b[size_t(n) * words + w];
+            }
+        o.coef.assign(C(coefficients), C(coefficients) + num_strings);
+        *out = p;
+    });
+}
+
+int ann_operator_destroy(ann_operator op) {
+    return guard([&] { delete op; });
+}
+
+// ----------------------------------------------------------------- ensembles
+
+int ann_monte_carlo_create(ann_ensemble* out, uint32_t num_samples, uint32_t num_sweeps,
+                           uint32_t num_thermalization_sweeps, uint32_t num_markov_chains, int32_t move,
+                           int32_t total_z, uint64_t seed) {
+    return guard([&] {
+        need(out, "out");
+        if (num_markov_chains == 0) throw std::invalid_argument("num_markov_chains must be positive");
+        if (move != ANN_MOVE_FLIP && move != ANN_MOVE_EXCHANGE) throw std::invalid_argument("unknown move");
+        auto p = new ann_ensemble_s;
+        auto& e = p->impl;
+        e.kind = ann::ENS_MC;
+        e.num_samples = num_samples;
+        e.num_sweeps = num_sweeps;
+        e.num_therm = num_thermalization_sweeps;
+        e.num_chains = num_markov_chains;
+        e.move = move;
+        e.total_z = total_z;
+        e.seed = seed;
+        *out = p;
+    });
+}
+
+int ann_monte_carlo_set_persistent(ann_ensemble ens, int32_t persistent) {
+    return guard([&] {
+        need(ens, "ens");
+        ens->impl.persistent = persistent != 0;
+        if (!persistent) ens->impl.chains_valid = false;
+    });
+}
+
+int ann_exact_summation_create(ann_ensemble* out, uint32_t num_sites) {
+    return guard([&] {
+        need(out, "out");
+        if (num_sites == 0 || num_sites > 26) throw std::invalid_argument("exact summation needs 1..26 sites");
+        auto p = new ann_ensemble_s;
+        p->impl.kind = ann::ENS_EXACT;
+        p->impl.num_sites = num_sites;
+        *out = p;
+    });
+}
+
+int ann_ensemble_destroy(ann_ensemble ens) {
+    return guard([&] { delete ens; });
+}
+
+int ann_ensemble_num_steps(ann_ensemble ens, uint64_t* n) {
+    return guard([&] {
+        need(ens, "ens");
+        *n = ens->impl.num_steps();
+    });
+}
+
+int ann_ensemble_acceptance(ann_ensemble ens, uint64_t* accepted, uint64_t* rejected) {
+    return guard([&] {
+        need(ens, "ens");
+        if (accepted) *accepted = ens->impl.accepted;
+        if (rejected) *rejected = ens->impl.rejected;
+    });
+}
+
+int ann_ensemble_last_samples(ann_ensemble ens, uint64_t* confs, int64_t capacity, int64_t* count, int32_t* words) {
+    return guard([&] {
+        need(ens, "ens");
+        auto& s = ens->impl.samples;
+        if (count) *count = s.count;
+        if (words) *words = s.words;
+        if (confs) {
+            if (capacity < s.count * s.words) throw std::invalid_argument("buffer too small");
+            s.confs.download(confs, size_t(s.count) * s.words);
+        }
+    });
+}
+
+// --------------------------------------------------------- network functions
+
+int ann_log_psi_s(ann_psi psi, const uint64_t* conf, double* out2) {
+    return guard([&] {
+        need(psi, "psi");
+        need(conf, "conf");
+        require_dense(psi->impl);
+        ann::Samples s;
+        single_sample(psi->impl, conf, s);
+        auto lp = host_log_psi(s);
+        *C(out2) = lp[0];
+    });
+}
+
+int ann_psi_O_k(ann_psi psi, const uint64_t* conf, double* out) {
+    return guard([&] {
+        need(psi, "psi");
+        need(conf, "conf");
+        auto& q = psi->impl;
+        require_dense(q);
+        ann::Samples s;
+        single_sample(q, conf, s);
+        ann::DevBuf<ann::cplx<double>> coef;
+        const ann::cplx<double> one = {1.0, 0.0};
+        coef.upload(&one, 1);
+        structured_sum(q, s, coef.p, 0, C(out));
+    });
+}
+
+int ann_psi_O_k_vector(ann_psi psi, ann_ensemble ens, double* out) {
+    return guard([&] {
+        need(psi, "psi");
+        need(ens, "ens");
+        auto& q = psi->impl;
+        ann::sample(q, ens->impl);
+        auto& s = ens->impl.samples;
+        ann::DevBuf<ann::cplx<double>> coef;
+        std::vector<ann::cplx<double>> ones(s.count, {1.0, 0.0});
+        coef.upload(ones.data(), ones.size());
+        structured_sum(q, s, coef.p, 0, C(out));
+    });
+}
+
+int ann_log_psi(ann_psi psi, ann_ensemble ens, double* out2) {
+    return guard([&] {
+        need(psi, "psi");
+        need(ens, "ens");
+        ann::sample(psi->impl, ens->impl);
+        auto& s = ens->impl.samples;
+        auto lp = host_log_psi(s);
+        auto w = host_weights(s);
+        cd acc = 0.0;
+        for (int64_t i = 0; i < s.count; i++) acc += w[i] * lp[i];
+        *C(out2) = acc;
+    });
+}
+
+int ann_psi_vector(ann_psi psi, ann_ensemble ens, double* out) {
+    return guard([&] {
+        need(psi, "psi");
+        need(ens, "ens");
+        ann::sample(psi->impl, ens->impl);
+        auto lp = host_log_psi(ens->impl.samples);
+        for (size_t i = 0; i < lp.size(); i++) C(out)[i] = std::exp(lp[i]);
+    });
+}
+
+int ann_log_psi_vector(ann_psi psi, ann_ensemble ens, double* out) {
+    return guard([&] {
+        need(psi, "psi");
+        need(ens, "ens");
+        ann::sample(psi->impl, ens->impl);
+        auto lp = host_log_psi(ens->impl.samples);
+        std::memcpy(out, lp.data(), sizeof(cd) * lp.size());
+    });
+}
+
+int ann_psi_norm(ann_psi psi, ann_ensemble ens, double* out) {
+    return guard([&] {
+        need(psi, "psi");
+        need(ens, "ens");
+        if (ens->impl.kind != ann::ENS_EXACT) throw std::invalid_argument("psi_norm needs an ExactSummation");
+        ann::sample(psi->impl, ens->impl);
+        auto lp = host_log_psi(ens->impl.samples);
+        double acc = 0.0;
+        for (const cd& z : lp) acc += std::exp(2.0 * z.real());
+        *out = std::sqrt(acc);
+    });
+}
+
+int ann_apply_operator(ann_psi psi, ann_operator op, ann_ensemble ens, double* out) {
+    return guard([&] {
+        need(psi, "psi");
+        need(op, "op");
+        need(ens, "ens");
+        ann::sample(psi->impl, ens->impl);
+        auto& s = ens->impl.samples;
+        ann::local_energies(psi->impl, op->impl, s);
+        auto lp = host_log_psi(s);
+        std::vector<cd> e(s.count);
+        s.e_loc.download(reinterpret_cast<ann::cplx<double>*>(e.data()), s.count);
+        for (int64_t i = 0; i < s.count; i++) C(out)[i] = std::exp(lp[i]) * e[i];
+    });
+}
+
+int ann_activation_function(const double* x2, int32_t layer, double* out2) {
+    return guard([&] {
+        const ann::cplx<double> z = {x2[0], x2[1]};
+        const ann::cplx<double> r = ann::act(z, layer);
+        out2[0] = r.re;
+        out2[1] = r.im;
+    });
+}
+
+static void expectation_core(ann::Operator& op, ann::Psi& psi, ann::Ensemble& ens, double h[4]) {
+    ann::sample(psi, ens);
+    auto& s = ens.samples;
+    ann::local_energies(psi, op, s);
+    ann::DevBuf<double> sums;
+    sums.ensure(4);
+    ann::launch_weighted_sums(s.e_loc.p, s.w.p, s.count, sums.p, ann::Runtime::get().stream());
+    ann::Runtime::get().sync();
+    sums.download(h, 4);
+}
+
+int ann_expectation_value(ann_operator op, ann_psi psi, ann_ensemble ens, double* out2) {
+    return guard([&] {
+        need(op, "op");
+        need(psi, "psi");
+        need(ens, "ens");
+        double h[4];
+        expectation_core(op->impl, psi->impl, ens->impl, h);
+        out2[0] = h[0];
+        out2[1] = h[1];
+    });
+}
+
+int ann_expectation_fluctuation(ann_operator op, ann_psi psi, ann_ensemble ens, double* sigma, double* value2) {
+    return guard([&] {
+        need(op, "op");
+        need(psi, "psi");
+        need(ens, "ens");
+        double h[4];
+        expectation_core(op->impl, psi->impl, ens->impl, h);
+        // ExpectationValue.cu.template:409-412
+        *sigma = std::sqrt(h[2] - (h[0] * h[0] + h[1] * h[1]));
+        value2[0] = h[0];
+        value2[1] = h[1];
+    });
+}
+
+int ann_expectation_gradient(ann_operator op, ann_psi psi, ann_ensemble ens, double* gradient, double* value2) {
+    return guard([&] {
+        need(op, "op");
+        need(psi, "psi");
+        need(ens, "ens");
+        need(gradient, "gradient");
+        auto& q = psi->impl;
+        double h[4];
+        expectation_core(op->impl, q, ens->impl, h);
+        auto& s = ens->impl.samples;
+        const cd E(h[0], h[1]);
+        ann::DevBuf<ann::cplx<double>> coef;
+        coef.ensure(s.count);
+        std::vector<cd> g1(q.num_params), g2(q.num_params);
+        hipStream_t st = ann::Runtime::get().stream();
+        ann::launch_weight_coef(s.w.p, s.e_loc.p, s.count, coef.p, st);
+        structured_sum(q, s, coef.p, 1, g1.data());
+        ann::launch_weight_coef(s.w.p, nullptr, s.count, coef.p, st);
+        structured_sum(q, s, coef.p, 1, g2.data());
+        // ExpectationValue.cu.template:467-469
+        for (int k = 0; k < q.num_params; k++) C(gradient)[k] = g1[k] - E * g2[k];
+        value2[0] = h[0];
+        value2[1] = h[1];
+    });
+}
+
+int ann_local_energies(ann_operator op, ann_psi psi, ann_ensemble ens, double* e_loc, int64_t capacity) {
+    return guard([&] {
+        need(op, "op");
+        need(psi, "psi");
+        need(ens, "ens");
+        ann::sample(psi->impl, ens->impl);
+        auto& s = ens->impl.samples;
+        if (capacity < s.count) throw std::invalid_argument("buffer too small");
+        ann::local_energies(psi->impl, op->impl, s);
+        ann::Runtime::get().sync();
+        s.e_loc.download(reinterpret_cast<ann::cplx<double>*>(e_loc), s.count);
+    });
+}
+
+// ---------------------------------------------------------------------- TDVP
+
+int ann_tdvp_create(ann_tdvp* out, uint32_t num_params) {
+    return guard([&] {
+        need(out, "out");
+        auto p = new ann_tdvp_s;
+        p->impl.P = num_params;
+        *out = p;
+    });
+}
+
+int ann_tdvp_destroy(ann_tdvp tdvp) {
+    return guard([&] { delete tdvp; });
+}
+
+int ann_tdvp_eval(ann_tdvp tdvp, ann_operator op, ann_psi psi, ann_ensemble ens, int32_t with_S) {
+    return guard([&] {
+        need(tdvp, "tdvp");
+        need(op, "op");
+        need(psi, "psi");
+        need(ens, "ens");
+        ann::tdvp_eval(tdvp->impl, op->impl, psi->impl, ens->impl, with_S != 0);
+    });
+}
+
+int ann_tdvp_get_S(ann_tdvp tdvp, double* S) {
+    return guard([&] {
+        need(tdvp, "tdvp");
+        if (!tdvp->impl.has_S) throw std::logic_error("no S matrix: call eval first");
+        tdvp->impl.S.download(reinterpret_cast<ann::cplx<double>*>(S), size_t(tdvp->impl.P) * tdvp->impl.P);
+    });
+}
+
+int ann_tdvp_get_F(ann_tdvp tdvp, double* F) {
+    return guard([&] {
+        need(tdvp, "tdvp");
+        std::memcpy(F, tdvp->impl.F.data(), sizeof(cd) * tdvp->impl.F.size());
+    });
+}
+
+int ann_tdvp_get_O(ann_tdvp tdvp, double* O) {
+    return guard([&] {
+        need(tdvp, "tdvp");
+        std::memcpy(O, tdvp->impl.O.data(), sizeof(cd) * tdvp->impl.O.size());
+    });
+}
+
+int ann_tdvp_get_energy(ann_tdvp tdvp, double* E2c, double* E2_local, double* var_H) {
+    return guard([&] {
+        need(tdvp, "tdvp");
+        auto& t = tdvp->impl;
+        if (E2c) *C(E2c) = t.E;
+        if (E2_local) *E2_local = t.E2;
+        if (var_H) *var_H = t.E2 - std::norm(t.E);   // TDVP.hpp:69-71
+    });
+}
+
+int ann_tdvp_S_dot_vector(ann_tdvp tdvp, const double* v, double* out) {
+    return guard([&] {
+        need(tdvp, "tdvp");
+        auto& t = tdvp->impl;
+        ann::DevBuf<ann::cplx<double>> dv, dy;
+        dv.upload(reinterpret_cast<const ann::cplx<double>*>(v), t.P);
+        dy.ensure(t.P);
+        ann::tdvp_S_dot(t, dv.p, 0.0, dy.p);
+        ann::Runtime::get().sync();
+        dy.download(reinterpret_cast<ann::cplx<double>*>(out), t.P);
+    });
+}
+
+int ann_tdvp_solve(ann_tdvp tdvp, double shift, int32_t method, double rtol, int32_t max_iter, double* x,
+                   int32_t* iterations, double* residual) {
+    int rc = ANN_OK;
+    const int g = guard([&] {
+        need(tdvp, "tdvp");
+        need(x, "x");
+        int it = 0;
+        double res = 0.0;
+        const int r = ann::tdvp_solve(tdvp->impl, shift, method, rtol, max_iter, C(x), &it, &res);
+        if (iterations) *iterations = it;
+        if (residual) *residual = res;
+        if (r == 1) {
+            g_error = "S + shift is not positive definite";
+            rc = ANN_E_NUMERIC;
+        } else if (r == 2) {
+            g_error = "CG did not reach the requested tolerance";
+            rc = ANN_E_NUMERIC;
+        }
+    });
+    return g != ANN_OK ? g : rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+// common.hpp -- shared device/host helpers for the gfx950 VMC kernels.
+//
+// Complex values are stored interleaved (re, im) in the real type T (float or
+// double).  All kernels are written for 64-lane CDNA wavefronts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace ann {
+
+constexpr int kWave = 64;
+constexpr int kMaxWords = 4;   // configurations of up to 256 sites
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define ANN_HIP_CHECK(expr)                                                               \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            throw ::ann::HipError(std::string(#expr) + ": " + hipGetErrorString(_e) +     \
+                                  " (" __FILE__ ":" + std::to_string(__LINE__) + ")");   \
+    } while (0)
+
+#define ANN_LAUNCH_CHECK() ANN_HIP_CHECK(hipGetLastError())
+
+template <typename T>
+struct cplx {
+    T re, im;
+};
+
+template <typename T>
+__host__ __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__host__ __device__ inline cplx<T> cadd(cplx<T> a, cplx<T> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+template <typename T>
+__host__ __device__ inline cplx<T> csub(cplx<T> a, cplx<T> b) {
+    return {a.re - b.re, a.im - b.im};
+}
+template <typename T>
+__host__ __device__ inline cplx<T> cscale(cplx<T> a, T s) {
+    return {a.re * s, a.im * s};
+}
+template <typename T>
+__host__ __device__ inline cplx<T> cconj(cplx<T> a) {
+    return {a.re, -a.im};
+}
+
+__device__ inline float dexp(float x) { return __expf(x); }
+__device__ inline double dexp(double x) { return exp(x); }
+__device__ inline float dsin(float x) { return __sinf(x); }
+__device__ inline double dsin(double x) { return sin(x); }
+__device__ inline float dcos(float x) { return __cosf(x); }
+__device__ inline double dcos(double x) { return cos(x); }
+
+template <typename T>
+__device__ inline cplx<T> cexp(cplx<T> z) {
+    const T m = dexp(z.re);
+    T s, c;
+    if constexpr (sizeof(T) == 4) {
+        sincosf(z.im, &s, &c);
+    } else {
+        sincos(z.im, &s, &c);
+    }
+    return {m * c, m * s};
+}
+
+// Activations of the reference (include/quantum_state/psi_functions.hpp:40-48,
+// 102-116): truncated Taylor polynomials of log cosh and tanh.
+//   layer 0 : lc(z) = z^2/2 - z^4/12 + z^6/45,   lc'(z) = z - z^3/3 + 2 z^5/15
+//   layer>=1: lc(z) = z - z^3/3 + 2 z^5/15,      lc'(z) = 1 - z^2 + 2 z^4/3
+// Evaluated in Horner form on u = z^2.
+template <typename T>
+__host__ __device__ inline cplx<T> logcosh0(cplx<T> z) {
+    const cplx<T> u = cmul(z, z);
+    // u * (1/2 + u * (-1/12 + u / 45))
+    cplx<T> p = {T(-1.0 / 12.0) + u.re * T(1.0 / 45.0), u.im * T(1.0 / 45.0)};
+    p = cmul(u, p);
+    p.re += T(0.5);
+    return cmul(u, p);
+}
+template <typename T>
+__host__ __device__ inline cplx<T> tanh0(cplx<T> z) {
+    const cplx<T> u = cmul(z, z);
+    // z * (1 - u / 3 + 2 u^2 / 15) = z * (1 + u * (-1/3 + 2u/15))
+    cplx<T> p = {T(-1.0 / 3.0) + u.re * T(2.0 / 15.0), u.im * T(2.0 / 15.0)};
+    p = cmul(u, p);
+    p.re += T(1.0);
+    return cmul(z, p);
+}
+template <typename T>
+__host__ __device__ inline cplx<T> dtanh1(cplx<T> z) {
+    const cplx<T> u = cmul(z, z);
+    // 1 - u + 2 u^2 / 3
+    cplx<T> p = {T(-1.0) + u.re * T(2.0 / 3.0), u.im * T(2.0 / 3.0)};
+    p = cmul(u, p);
+    p.re += T(1.0);
+    return p;
+}
+template <typename T>
+__host__ __device__ inline cplx<T> act(cplx<T> z, int layer) {
+    return layer == 0 ? logcosh0(z) : tanh0(z);
+}
+template <typename T>
+__host__ __device__ inline cplx<T> dact(cplx<T> z, int layer) {
+    return layer == 0 ? tanh0(z) : dtanh1(z);
+}
+
+// ---------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11).  Counter = (chain, proposal lo, proposal hi, stream).
+struct u32x4 {
+    uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline u32x4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                        uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+        const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+        c1 = uint32_t(p1);
+        c3 = uint32_t(p0);
+        c0 = n0;
+        c2 = n2;
+    }
+    return {c0, c1, c2, c3};
+}
+
+// -------------------------------------------------------- configurations
+struct Conf {
+    uint64_t w[kMaxWords];
+};
+
+__device__ inline uint64_t word_sel(const Conf& c, int k) {
+    // branch-free select keeps the words in registers (no scratch indexing)
+    uint64_t r = c.w[0];
+    r = k == 1 ? c.w[1] : r;
+    r = k == 2 ? c.w[2] : r;
+    r = k == 3 ? c.w[3] : r;
+    return r;
+}
+__device__ inline int spin_bit(const Conf& c, int i) { return int((word_sel(c, i >> 6) >> (i & 63)) & 1u); }
+__device__ inline void flip_bit(Conf& c, int i) {
+    const uint64_t m = uint64_t(1) << (i & 63);
+    const int k = i >> 6;
+    c.w[0] ^= k == 0 ? m : 0;
+    c.w[1] ^= k == 1 ? m : 0;
+    c.w[2] ^= k == 2 ? m : 0;
+    c.w[3] ^= k == 3 ? m : 0;
+}
+
+// index of the rank-th set bit of mask (words of the configuration)
+__device__ inline int select_bit(const uint64_t* m, int words, uint32_t rank) {
+    for (int w = 0; w < words; w++) {
+        const uint32_t c = uint32_t(__popcll(m[w]));
+        if (rank < c) {
+            uint64_t x = m[w];
+            for (uint32_t r = 0; r < rank; r++) x &= x - 1;
+            return w * 64 + __ffsll((unsigned long long)x) - 1;
+        }
+        rank -= c;
+    }
+    return -1;
+}
+
+// ------------------------------------------------------ wave reductions
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+}  // namespace ann

@@ -1,0 +1,730 @@
+// dense_net.hip -- gfx950 kernels for PsiRBM / one-hidden-layer PsiDeep.
+//
+// Layout: one 64-lane wavefront per Markov chain (or per sample); lane l owns the
+// hidden units j = l + 64u, u < U.  The angles theta_j = b_j + sum_v Wd[v][j] s_v
+// of a chain live in VGPRs for the whole sweep; a proposal flipping sites i (and
+// k) updates them with one (two) rows of Wd, the log-cosh sum is a wavefront
+// reduction, and the Metropolis decision is made wave-uniform.
+//
+// Reference behaviour restated here:
+//   log psi      include/quantum_state/PsiDeep.hpp:138-266, PsiRBM.hpp:69-119
+//   update       include/quantum_state/PsiDeep.hpp:268-343, PsiRBM.hpp:121-157
+//   Metropolis   include/ensembles/MonteCarlo.hpp:133-177
+//   local energy include/operator/Operator.hpp:36-121
+//   O_k          include/quantum_state/PsiDeep.hpp:345-445, PsiRBM.hpp:159-176
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <type_traits>
+
+namespace ann {
+
+namespace {
+
+constexpr int kBlock = 256;            // 4 wavefronts
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+template <typename T>
+__device__ inline T uniform(T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readfirstlane(int(b & 0xffffffff));
+        const int hi = __builtin_amdgcn_readfirstlane(int(b >> 32));
+        return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    }
+}
+
+__device__ inline uint64_t uniform_u64(uint64_t v) {
+    const int lo = __builtin_amdgcn_readfirstlane(int(v & 0xffffffffu));
+    const int hi = __builtin_amdgcn_readfirstlane(int(v >> 32));
+    return (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo);
+}
+
+__device__ inline Conf load_conf(const uint64_t* confs, int64_t s, int words) {
+    Conf c;
+#pragma unroll
+    for (int w = 0; w < kMaxWords; w++) c.w[w] = w < words ? uniform_u64(confs[s * words + w]) : 0;
+    return c;
+}
+
+// Re(lc(z)) for the layer-0 polynomial, expanded in u = z^2 = a + i b:
+// Re(u/2 - u^2/12 + u^3/45) = a(1/2 - a/12 + a^2/45) + b^2 (1/12 - a/15)
+template <typename T>
+__device__ inline T logcosh0_re(cplx<T> z) {
+    const T a = z.re * z.re - z.im * z.im;
+    const T h = z.re * z.im;            // b = 2h
+    const T b2 = T(4) * h * h;
+    T p = a * T(1.0 / 45.0) - T(1.0 / 12.0);
+    p = p * a + T(0.5);
+    return p * a + b2 * (T(1.0 / 12.0) - a * T(1.0 / 15.0));
+}
+
+// Sum_j Re(f_j lc(theta_j)) over this lane's units
+template <typename T, int U, bool REALF>
+__device__ inline T lane_re_sum(const cplx<T> (&th)[U], const cplx<T> (&f)[U]) {
+    T s = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        if constexpr (REALF) {
+            s += f[u].re * logcosh0_re(th[u]);
+        } else {
+            const cplx<T> l = logcosh0(th[u]);
+            s += f[u].re * l.re - f[u].im * l.im;
+        }
+    }
+    return s;
+}
+
+template <typename T, int U>
+__device__ inline cplx<T> lane_sum(const cplx<T> (&th)[U], const cplx<T> (&f)[U]) {
+    cplx<T> s = {0, 0};
+#pragma unroll
+    for (int u = 0; u < U; u++) s = cadd(s, cmul(f[u], logcosh0(th[u])));
+    return s;
+}
+
+template <typename T>
+__device__ inline cplx<T> wave_csum(cplx<T> v) {
+    return {uniform(wave_sum(v.re)), uniform(wave_sum(v.im))};
+}
+
+// ----------------------------------------------------------------- kernels
+
+__global__ void enumerate_kernel(uint64_t* confs, int64_t first, int64_t count, int words) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    confs[i * words] = uint64_t(first + i);
+    for (int w = 1; w < words; w++) confs[i * words + w] = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void theta_valu_kernel(DenseView<T> net, const uint64_t* confs,
+                                                            int words, int64_t count, cplx<T>* theta) {
+    const int64_t s = blockIdx.x;
+    if (s >= count) return;
+    const Conf c = load_conf(confs, s, words);
+    for (int j = threadIdx.x; j < net.Mpad; j += blockDim.x) {
+        cplx<T> t = net.b[j];
+        for (int v = 0; v < net.N; v++) {
+            const cplx<T> w = net.Wd[int64_t(v) * net.Mpad + j];
+            const T sv = spin_bit(c, v) ? T(1) : T(-1);
+            t.re += sv * w.re;
+            t.im += sv * w.im;
+        }
+        theta[s * net.Mpad + j] = t;
+    }
+}
+
+// theta on MFMA.  One wavefront computes 16 samples x 64 real columns (4 tiles
+// of v_mfma_f32_16x16x16_bf16).  A fragment: lane l holds row l%16, k = 4(l/16)+0..3
+// (spins, exact in bf16); B fragment: k = 4(l/16)+0..3, column l%16 (pre-packed).
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64) void theta_mfma_kernel(DenseView<float> net, const short4_t* wfrag,
+                                                         int KT, int NT, int nsplit,
+                                                         const uint64_t* confs, int words,
+                                                         int64_t count, float* theta) {
+    const int lane = threadIdx.x;
+    const int64_t row0 = int64_t(blockIdx.x) * 16;
+    const int nt0 = blockIdx.y * 4;
+    const int64_t my_row = row0 + (lane & 15);
+    Conf c = {{0, 0, 0, 0}};
+    if (my_row < count) {
+        for (int w = 0; w < words; w++) c.w[w] = confs[my_row * words + w];
+    }
+    float4_t acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t] = {0.f, 0.f, 0.f, 0.f};
+    const int kq = 4 * (lane >> 4);
+    for (int kt = 0; kt < KT; kt++) {
+        short4_t a;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int v = kt * 16 + kq + e;
+            const short one = v < net.N ? short(0x3F80) : short(0);
+            a[e] = (v < net.N && spin_bit(c, v)) ? one : short(v < net.N ? 0xBF80 : 0);
+        }
+        for (int sp = 0; sp < nsplit; sp++) {
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const short4_t b = wfrag[((int64_t(sp) * KT + kt) * NT + nt0 + t) * 64 + lane];
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    const int ncol = 2 * net.Mpad;
+    const float* bflat = reinterpret_cast<const float*>(net.b);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int col = (nt0 + t) * 16 + (lane & 15);
+        const float bias = bflat[col];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t row = row0 + 4 * (lane >> 4) + r;
+            if (row < count) theta[row * ncol + col] = acc[t][r] + bias;
+        }
+    }
+}
+
+template <typename T, int U>
+__global__ __launch_bounds__(kBlock) void net_log_psi_kernel(DenseView<T> net, const cplx<T>* theta,
+                                                             int64_t count, cplx<double> lp,
+                                                             cplx<double>* log_psi, double* weight) {
+    const int lane = threadIdx.x & 63;
+    const int64_t s = int64_t(blockIdx.x) * kWavesPerBlock + uniform_int(threadIdx.x >> 6);
+    if (s >= count) return;
+    cplx<T> th[U], f[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        th[u] = theta[s * net.Mpad + lane + 64 * u];
+        f[u] = net.f[lane + 64 * u];
+    }
+    const cplx<T> L = wave_csum(lane_sum<T, U>(th, f));
+    if (lane == 0) {
+        const cplx<double> r = {lp.re + double(L.re), lp.im + double(L.im)};
+        log_psi[s] = r;
+        if (weight) weight[s] = exp(2.0 * r.re);
+    }
+}
+
+// Metropolis sweeps (MonteCarlo.hpp:133-177) with single flips or S_z-conserving
+// exchanges.  The proposal for chain c at global proposal index t uses
+// Philox((c, t_lo, t_hi, 0), seed): x -> site i = x % N, y -> partner rank,
+// z -> acceptance uniform (z + 0.5) 2^-32.
+template <typename T, int U, int MOVE, bool REALF>
+__global__ __launch_bounds__(kBlock) void mc_sweep_kernel(DenseView<T> net, uint64_t* chain_confs,
+                                                          cplx<T>* theta, int num_chains, int words,
+                                                          int64_t num_proposals, uint64_t counter,
+                                                          uint32_t key0, uint32_t key1,
+                                                          uint32_t* accepted) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * kWavesPerBlock + uniform_int(threadIdx.x >> 6);
+    if (c >= num_chains) return;
+    const int N = net.N;
+    Conf conf = load_conf(chain_confs, c, words);
+    cplx<T> th[U], f[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        th[u] = theta[int64_t(c) * net.Mpad + lane + 64 * u];
+        f[u] = net.f[lane + 64 * u];
+    }
+    T cur = uniform(wave_sum(lane_re_sum<T, U, REALF>(th, f)));
+    uint64_t last_mask = N % 64 ? (uint64_t(1) << (N % 64)) - 1 : ~uint64_t(0);
+    uint32_t acc = 0;
+    for (int64_t q = 0; q < num_proposals; q++) {
+        const uint64_t t = counter + uint64_t(q);
+        const u32x4 r = philox(uint32_t(c), uint32_t(t), uint32_t(t >> 32), 0u, key0, key1);
+        const int i = int(r.x % uint32_t(N));
+        const int si_bit = spin_bit(conf, i);
+        int k = -1;
+        if constexpr (MOVE == MOVE_EXCHANGE) {
+            uint64_t opp[kMaxWords];
+            uint32_t n_opp = 0;
+#pragma unroll
+            for (int w = 0; w < kMaxWords; w++) {
+                uint64_t m = si_bit ? ~conf.w[w] : conf.w[w];
+                m = w < words ? m : 0;
+                m = w == words - 1 ? (m & last_mask) : m;
+                opp[w] = m;
+                n_opp += uint32_t(__popcll(m));
+            }
+            if (n_opp == 0) continue;
+            k = select_bit(opp, words, r.y % n_opp);
+        }
+        const T ci = si_bit ? T(-2) : T(2);   // theta' - theta = -2 s_i W_i
+        cplx<T> tn[U];
+        const cplx<T>* wi = net.Wd + int64_t(i) * net.Mpad + lane;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const cplx<T> w = wi[64 * u];
+            tn[u].re = th[u].re + ci * w.re;
+            tn[u].im = th[u].im + ci * w.im;
+        }
+        if constexpr (MOVE == MOVE_EXCHANGE) {
+            const T ck = -ci;              // s_k = -s_i
+            const cplx<T>* wk = net.Wd + int64_t(k) * net.Mpad + lane;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const cplx<T> w = wk[64 * u];
+                tn[u].re += ck * w.re;
+                tn[u].im += ck * w.im;
+            }
+        }
+        const T nxt = uniform(wave_sum(lane_re_sum<T, U, REALF>(tn, f)));
+        const T d2 = T(2) * (nxt - cur);
+        const T uu = (T(r.z) + T(0.5)) * T(2.3283064365386963e-10);
+        if (d2 > T(0) || uu <= dexp(d2)) {
+#pragma unroll
+            for (int u = 0; u < U; u++) th[u] = tn[u];
+            cur = nxt;
+            flip_bit(conf, i);
+            if (k >= 0) flip_bit(conf, k);
+            acc++;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) theta[int64_t(c) * net.Mpad + lane + 64 * u] = th[u];
+    if (lane == 0) {
+        for (int w = 0; w < words; w++) chain_confs[int64_t(c) * words + w] = conf.w[w];
+        accepted[c] += acc;
+    }
+}
+
+// E_loc(s) = sum_groups coef_g(s) * psi(s')/psi(s)  (Operator.hpp:36-121)
+template <typename T, int U>
+__global__ __launch_bounds__(kBlock) void local_energy_kernel(DenseView<T> net, OpView op,
+                                                              const uint64_t* confs,
+                                                              const cplx<T>* theta, int64_t count,
+                                                              cplx<double>* e_loc) {
+    const int lane = threadIdx.x & 63;
+    const int64_t s = int64_t(blockIdx.x) * kWavesPerBlock + uniform_int(threadIdx.x >> 6);
+    if (s >= count) return;
+    const int words = op.words;
+    const Conf conf = load_conf(confs, s, words);
+    cplx<T> th[U], f[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        th[u] = theta[s * net.Mpad + lane + 64 * u];
+        f[u] = net.f[lane + 64 * u];
+    }
+    const cplx<T> L0 = wave_csum(lane_sum<T, U>(th, f));
+    double e_re = 0.0, e_im = 0.0;
+    for (int g = 0; g < op.num_groups; g++) {
+        double c_re = 0.0, c_im = 0.0;
+        const int b0 = op.group_begin[g], b1 = op.group_begin[g + 1];
+        for (int n = b0; n < b1; n++) {
+            int par = 0;
+            for (int w = 0; w < words; w++) par += __popcll(~conf.w[w] & op.string_zy[int64_t(n) * words + w]);
+            const double sg = (par & 1) ? -1.0 : 1.0;
+            c_re += sg * op.string_coef[2 * n];
+            c_im += sg * op.string_coef[2 * n + 1];
+        }
+        if (c_re == 0.0 && c_im == 0.0) continue;
+        bool diag = true;
+        for (int w = 0; w < words; w++) diag = diag && op.group_flip[int64_t(g) * words + w] == 0;
+        if (diag) {
+            e_re += c_re;
+            e_im += c_im;
+            continue;
+        }
+        cplx<T> tn[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) tn[u] = th[u];
+        for (int w = 0; w < words; w++) {
+            uint64_t m = op.group_flip[int64_t(g) * words + w];
+            while (m) {
+                const int i = w * 64 + __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const T ci = spin_bit(conf, i) ? T(-2) : T(2);
+                const cplx<T>* wi = net.Wd + int64_t(i) * net.Mpad + lane;
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const cplx<T> wv = wi[64 * u];
+                    tn[u].re += ci * wv.re;
+                    tn[u].im += ci * wv.im;
+                }
+            }
+        }
+        const cplx<T> L1 = wave_csum(lane_sum<T, U>(tn, f));
+        const double d_re = double(L1.re) - double(L0.re);
+        const double d_im = double(L1.im) - double(L0.im);
+        const double m = exp(d_re);
+        double sn, cs;
+        sincos(d_im, &sn, &cs);
+        const double r_re = m * cs, r_im = m * sn;
+        e_re += c_re * r_re - c_im * r_im;
+        e_im += c_re * r_im + c_im * r_re;
+    }
+    if (lane == 0) e_loc[s] = {e_re, e_im};
+}
+
+template <typename T>
+__global__ void g_factor_kernel(DenseView<T> net, const cplx<T>* theta, int64_t total, cplx<T>* g) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int j = int(e % net.Mpad);
+    g[e] = cmul(net.f[j], tanh0(theta[e]));
+}
+
+// O_k rows in the reference parameter order (PsiDeep.cu:245-270 / PsiRBM W order)
+template <typename T>
+__global__ void materialize_ok_kernel(DenseView<T> net, const uint64_t* confs, int words,
+                                      const cplx<T>* g, int64_t count, const cplx<double>* center,
+                                      cplx<T>* O) {
+    const int kblocks = (net.num_params + int(blockDim.x) - 1) / int(blockDim.x);
+    const int64_t s = int64_t(blockIdx.x) / kblocks;
+    const int k = int(int64_t(blockIdx.x) % kblocks) * blockDim.x + threadIdx.x;
+    if (k >= net.num_params || s >= count) return;
+    const Conf c = load_conf(confs, s, words);
+    const cplx<T>* gs = g + s * net.Mpad;
+    cplx<T> o;
+    if (net.layout == LAYOUT_RBM) {
+        const int i = k / net.M, j = k % net.M;
+        o = cscale(gs[j], spin_bit(c, i) ? T(1) : T(-1));
+    } else {
+        if (k < net.N) {
+            o = {spin_bit(c, k) ? T(1) : T(-1), T(0)};
+        } else if (k < net.N + net.M) {
+            o = gs[k - net.N];
+        } else {
+            const int q = k - net.N - net.M;
+            const int j = q % net.M;
+            o = cscale(gs[j], spin_bit(c, int(net.conn[q])) ? T(1) : T(-1));
+        }
+    }
+    if (center) {
+        o.re -= T(center[k].re);
+        o.im -= T(center[k].im);
+    }
+    O[s * net.num_params + k] = o;
+}
+
+// Partial sums over a chunk of samples: P[chunk][v][j] = sum_s coef_s spin_sv X_sj
+// for v < N, row N holds sum_s coef_s X_sj.  Each wavefront owns 16 rows v.
+constexpr int kSwsRows = 16;
+constexpr int kSwsMaxChunks = 4096;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void spin_weighted_kernel(DenseView<T> net, const uint64_t* confs,
+                                                               int words, const cplx<T>* g, int conj_g,
+                                                               const cplx<double>* coef, int64_t count,
+                                                               int64_t chunk, cplx<double>* partial) {
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform_int(threadIdx.x >> 6);
+    const int j = blockIdx.x * 64 + lane;
+    const int v0 = (blockIdx.z * kWavesPerBlock + wave) * kSwsRows;
+    const int64_t s0 = int64_t(blockIdx.y) * chunk;
+    const int64_t s1 = s0 + chunk < count ? s0 + chunk : count;
+    if (v0 > net.N) return;
+    double ar[kSwsRows], ai[kSwsRows];
+#pragma unroll
+    for (int r = 0; r < kSwsRows; r++) ar[r] = ai[r] = 0.0;
+    for (int64_t s = s0; s < s1; s++) {
+        const Conf c = load_conf(confs, s, words);
+        const cplx<double> cf = coef[s];
+        const cplx<T> gv = g[s * net.Mpad + j];
+        const double gr = double(gv.re), gi = conj_g ? -double(gv.im) : double(gv.im);
+        const double xr = cf.re * gr - cf.im * gi;
+        const double xi = cf.re * gi + cf.im * gr;
+#pragma unroll
+        for (int r = 0; r < kSwsRows; r++) {
+            const int v = v0 + r;
+            const double sv = v < net.N ? (spin_bit(c, v) ? 1.0 : -1.0) : 1.0;
+            ar[r] += sv * xr;
+            ai[r] += sv * xi;
+        }
+    }
+    const int rows = net.N + 1;
+#pragma unroll
+    for (int r = 0; r < kSwsRows; r++) {
+        const int v = v0 + r;
+        if (v < rows) partial[(int64_t(blockIdx.y) * rows + v) * net.Mpad + j] = {ar[r], ai[r]};
+    }
+}
+
+__global__ void sum_chunks_kernel(const cplx<double>* partial, int chunks, int64_t n, cplx<double>* out) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double re = 0, im = 0;
+    for (int c = 0; c < chunks; c++) {
+        re += partial[c * n + e].re;
+        im += partial[c * n + e].im;
+    }
+    out[e] = {re, im};
+}
+
+// Ya[v] = sum_s coef_s spin_sv
+__global__ __launch_bounds__(kBlock) void spin_coef_sums_kernel(const uint64_t* confs, int words, int N,
+                                                                const cplx<double>* coef, int64_t count,
+                                                                cplx<double>* Ya) {
+    const int v = blockIdx.x;
+    __shared__ double sr[kBlock], si[kBlock];
+    double re = 0, im = 0;
+    for (int64_t s = threadIdx.x; s < count; s += blockDim.x) {
+        const uint64_t wv = confs[s * words + (v >> 6)];
+        const double sv = ((wv >> (v & 63)) & 1) ? 1.0 : -1.0;
+        re += sv * coef[s].re;
+        im += sv * coef[s].im;
+    }
+    sr[threadIdx.x] = re;
+    si[threadIdx.x] = im;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (int(threadIdx.x) < o) {
+            sr[threadIdx.x] += sr[threadIdx.x + o];
+            si[threadIdx.x] += si[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Ya[v] = {sr[0], si[0]};
+}
+
+// u_s = sum_i s_i va_i + sum_j g_sj (vb_j + SV_sj)
+template <typename T, int U>
+__global__ __launch_bounds__(kBlock) void structured_Ov_kernel(DenseView<T> net, const uint64_t* confs,
+                                                               int words, const cplx<T>* g,
+                                                               const cplx<T>* SV, const cplx<double>* vb,
+                                                               const cplx<double>* va, int64_t count,
+                                                               cplx<double>* u_out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t s = int64_t(blockIdx.x) * kWavesPerBlock + uniform_int(threadIdx.x >> 6);
+    if (s >= count) return;
+    const Conf c = load_conf(confs, s, words);
+    double re = 0, im = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int j = lane + 64 * u;
+        if (j < net.M) {
+            const cplx<T> gv = g[s * net.Mpad + j];
+            const cplx<T> sv = SV[s * net.Mpad + j];
+            double xr = double(sv.re), xi = double(sv.im);
+            if (vb) {
+                xr += vb[j].re;
+                xi += vb[j].im;
+            }
+            re += double(gv.re) * xr - double(gv.im) * xi;
+            im += double(gv.re) * xi + double(gv.im) * xr;
+        }
+    }
+    if (va) {
+        for (int i = lane; i < net.N; i += 64) {
+            const double si = spin_bit(c, i) ? 1.0 : -1.0;
+            re += si * va[i].re;
+            im += si * va[i].im;
+        }
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (lane == 0) u_out[s] = {re, im};
+}
+
+template <int U>
+struct UTag {
+    static constexpr int value = U;
+};
+
+// dispatch on the number of hidden units per lane
+template <typename F>
+void dispatch_u(int U, F&& fn) {
+    switch (U) {
+        case 1: fn(UTag<1>{}); break;
+        case 2: fn(UTag<2>{}); break;
+        case 4: fn(UTag<4>{}); break;
+        case 7: fn(UTag<7>{}); break;
+        case 8: fn(UTag<8>{}); break;
+        case 16: fn(UTag<16>{}); break;
+        default: throw HipError("unsupported hidden width (units per lane " + std::to_string(U) + ")");
+    }
+}
+
+inline unsigned grid_waves(int64_t n) { return unsigned((n + kWavesPerBlock - 1) / kWavesPerBlock); }
+
+uint16_t to_bf16(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    const uint32_t round = 0x7FFFu + ((b >> 16) & 1u);
+    return uint16_t((b + round) >> 16);
+}
+double from_bf16(uint16_t h) {
+    const uint32_t b = uint32_t(h) << 16;
+    float x;
+    std::memcpy(&x, &b, 4);
+    return double(x);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- wrappers
+
+void launch_enumerate_confs(uint64_t* confs, int64_t first_index, int64_t count, int words,
+                            hipStream_t stream) {
+    if (count <= 0) return;
+    enumerate_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(confs, first_index, count, words);
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_theta_valu(const DenseView<T>& net, const uint64_t* confs, int words, int64_t count,
+                       cplx<T>* theta, hipStream_t stream) {
+    if (count <= 0) return;
+    theta_valu_kernel<T><<<unsigned(count), kBlock, 0, stream>>>(net, confs, words, count, theta);
+    ANN_LAUNCH_CHECK();
+}
+
+void pack_theta_fragments(const double* Wd, int N, int Mpad, int nsplit, std::vector<uint16_t>& out,
+                          int& npad) {
+    npad = (N + 15) / 16 * 16;
+    const int KT = npad / 16, ncol = 2 * Mpad, NT = ncol / 16;
+    out.assign(size_t(nsplit) * KT * NT * 64 * 4, 0);
+    for (int v = 0; v < N; v++) {
+        for (int col = 0; col < ncol; col++) {
+            double r = Wd[size_t(v) * ncol + col];
+            const int kt = v / 16, kk = v % 16, nt = col / 16, cc = col % 16;
+            const int lane = (kk / 4) * 16 + cc, e = kk % 4;
+            for (int sp = 0; sp < nsplit; sp++) {
+                const uint16_t h = to_bf16(float(r));
+                r -= from_bf16(h);
+                out[((((size_t(sp) * KT + kt) * NT + nt) * 64 + lane) * 4) + e] = h;
+            }
+        }
+    }
+}
+
+void launch_theta_mfma(const DenseView<float>& net, const uint16_t* wfrag, int npad, const uint64_t* confs,
+                       int words, int64_t count, cplx<float>* theta, hipStream_t stream) {
+    if (count <= 0) return;
+    const int KT = npad / 16, NT = 2 * net.Mpad / 16;
+    dim3 grid(unsigned((count + 15) / 16), unsigned(NT / 4));
+    // nsplit is recovered from the fragment buffer size by the caller convention: 3 terms
+    theta_mfma_kernel<<<grid, 64, 0, stream>>>(net, reinterpret_cast<const short4_t*>(wfrag), KT, NT, 3,
+                                               confs, words, count, reinterpret_cast<float*>(theta));
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_net_log_psi(const DenseView<T>& net, const cplx<T>* theta, int64_t count, cplx<double> lp,
+                        cplx<double>* log_psi, double* weight, hipStream_t stream) {
+    if (count <= 0) return;
+    dispatch_u(net.Mpad / 64, [&](auto tag) {
+        constexpr int U = std::decay_t<decltype(tag)>::value;
+        net_log_psi_kernel<T, U><<<grid_waves(count), kBlock, 0, stream>>>(net, theta, count, lp, log_psi,
+                                                                            weight);
+    });
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_mc_sweep(const DenseView<T>& net, uint64_t* chain_confs, cplx<T>* theta, int num_chains,
+                     int words, int move, int64_t num_proposals, uint64_t counter, uint64_t seed,
+                     uint32_t* accepted, hipStream_t stream) {
+    if (num_chains <= 0 || num_proposals <= 0) return;
+    const uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
+    dispatch_u(net.Mpad / 64, [&](auto tag) {
+        constexpr int U = std::decay_t<decltype(tag)>::value;
+        const unsigned grid = grid_waves(num_chains);
+        if (move == MOVE_EXCHANGE) {
+            if (net.real_f)
+                mc_sweep_kernel<T, U, MOVE_EXCHANGE, true><<<grid, kBlock, 0, stream>>>(
+                    net, chain_confs, theta, num_chains, words, num_proposals, counter, k0, k1, accepted);
+            else
+                mc_sweep_kernel<T, U, MOVE_EXCHANGE, false><<<grid, kBlock, 0, stream>>>(
+                    net, chain_confs, theta, num_chains, words, num_proposals, counter, k0, k1, accepted);
+        } else {
+            if (net.real_f)
+                mc_sweep_kernel<T, U, MOVE_FLIP, true><<<grid, kBlock, 0, stream>>>(
+                    net, chain_confs, theta, num_chains, words, num_proposals, counter, k0, k1, accepted);
+            else
+                mc_sweep_kernel<T, U, MOVE_FLIP, false><<<grid, kBlock, 0, stream>>>(
+                    net, chain_confs, theta, num_chains, words, num_proposals, counter, k0, k1, accepted);
+        }
+    });
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_local_energy(const DenseView<T>& net, const OpView& op, const uint64_t* confs,
+                         const cplx<T>* theta, int64_t count, cplx<double>* e_loc, hipStream_t stream) {
+    if (count <= 0) return;
+    dispatch_u(net.Mpad / 64, [&](auto tag) {
+        constexpr int U = std::decay_t<decltype(tag)>::value;
+        local_energy_kernel<T, U><<<grid_waves(count), kBlock, 0, stream>>>(net, op, confs, theta, count,
+                                                                             e_loc);
+    });
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_g_factor(const DenseView<T>& net, const cplx<T>* theta, int64_t count, cplx<T>* g,
+                     hipStream_t stream) {
+    const int64_t total = count * net.Mpad;
+    if (total <= 0) return;
+    g_factor_kernel<T><<<unsigned((total + 255) / 256), 256, 0, stream>>>(net, theta, total, g);
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_materialize_ok(const DenseView<T>& net, const uint64_t* confs, int words, const cplx<T>* g,
+                           int64_t count, const cplx<double>* center, cplx<T>* O, hipStream_t stream) {
+    if (count <= 0) return;
+    const int64_t blocks = count * ((net.num_params + 255) / 256);
+    if (blocks > 2147483647) throw HipError("too many samples for O_k materialisation");
+    materialize_ok_kernel<T><<<unsigned(blocks), 256, 0, stream>>>(net, confs, words, g, count, center, O);
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_spin_weighted_sums(const DenseView<T>& net, const uint64_t* confs, int words, const cplx<T>* g,
+                               int conj_g, const cplx<double>* coef, int64_t count, cplx<double>* Yw,
+                               cplx<double>* Yb, cplx<double>* Ya, hipStream_t stream) {
+    if (count <= 0) return;
+    int64_t chunk = 256;
+    while ((count + chunk - 1) / chunk > kSwsMaxChunks) chunk *= 2;
+    const int chunks = int((count + chunk - 1) / chunk);
+    const int rows = net.N + 1;
+    const int64_t n = int64_t(rows) * net.Mpad;
+    cplx<double>* partial = nullptr;
+    ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(cplx<double>) * n * chunks, stream));
+    dim3 grid(unsigned(net.Mpad / 64), unsigned(chunks),
+              unsigned((rows + kSwsRows * kWavesPerBlock - 1) / (kSwsRows * kWavesPerBlock)));
+    spin_weighted_kernel<T><<<grid, kBlock, 0, stream>>>(net, confs, words, g, conj_g, coef, count, chunk,
+                                                         partial);
+    ANN_LAUNCH_CHECK();
+    cplx<double>* full = nullptr;
+    ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&full), sizeof(cplx<double>) * n, stream));
+    sum_chunks_kernel<<<unsigned((n + 255) / 256), 256, 0, stream>>>(partial, chunks, n, full);
+    ANN_LAUNCH_CHECK();
+    // rows 0..N-1 -> Yw, row N -> Yb
+    if (Yw)
+        ANN_HIP_CHECK(hipMemcpyAsync(Yw, full, sizeof(cplx<double>) * net.N * net.Mpad, hipMemcpyDeviceToDevice,
+                                     stream));
+    if (Yb)
+        ANN_HIP_CHECK(hipMemcpyAsync(Yb, full + int64_t(net.N) * net.Mpad, sizeof(cplx<double>) * net.Mpad,
+                                     hipMemcpyDeviceToDevice, stream));
+    if (Ya) {
+        spin_coef_sums_kernel<<<unsigned(net.N), kBlock, 0, stream>>>(confs, words, net.N, coef, count, Ya);
+        ANN_LAUNCH_CHECK();
+    }
+    ANN_HIP_CHECK(hipFreeAsync(partial, stream));
+    ANN_HIP_CHECK(hipFreeAsync(full, stream));
+}
+
+template <typename T>
+void launch_structured_Ov(const DenseView<T>& net, const uint64_t* confs, int words, const cplx<T>* g,
+                          const cplx<T>* SV, const cplx<double>* vb, const cplx<double>* va, int64_t count,
+                          cplx<double>* u, hipStream_t stream) {
+    if (count <= 0) return;
+    dispatch_u(net.Mpad / 64, [&](auto tag) {
+        constexpr int U = std::decay_t<decltype(tag)>::value;
+        structured_Ov_kernel<T, U><<<grid_waves(count), kBlock, 0, stream>>>(net, confs, words, g, SV, vb, va,
+                                                                              count, u);
+    });
+    ANN_LAUNCH_CHECK();
+}
+
+#define ANN_INSTANTIATE(T)                                                                                   \
+    template void launch_theta_valu<T>(const DenseView<T>&, const uint64_t*, int, int64_t, cplx<T>*,        \
+                                       hipStream_t);                                                         \
+    template void launch_net_log_psi<T>(const DenseView<T>&, const cplx<T>*, int64_t, cplx<double>,        \
+                                        cplx<double>*, double*, hipStream_t);                                \
+    template void launch_mc_sweep<T>(const DenseView<T>&, uint64_t*, cplx<T>*, int, int, int, int64_t,      \
+                                     uint64_t, uint64_t, uint32_t*, hipStream_t);                            \
+    template void launch_local_energy<T>(const DenseView<T>&, const OpView&, const uint64_t*, const cplx<T>*, \
+                                         int64_t, cplx<double>*, hipStream_t);                               \
+    template void launch_g_factor<T>(const DenseView<T>&, const cplx<T>*, int64_t, cplx<T>*, hipStream_t);  \
+    template void launch_materialize_ok<T>(const DenseView<T>&, const uint64_t*, int, const cplx<T>*,       \
+                                           int64_t, const cplx<double>*, cplx<T>*, hipStream_t);             \
+    template void launch_spin_weighted_sums<T>(const DenseView<T>&, const uint64_t*, int, const cplx<T>*,   \
+                                               int, const cplx<double>*, int64_t, cplx<double>*,             \
+                                               cplx<double>*, cplx<double>*, hipStream_t);                   \
+    template void launch_structured_Ov<T>(const DenseView<T>&, const uint64_t*, int, const cplx<T>*,        \
+                                          const cplx<T>*, const cplx<double>*, const cplx<double>*, int64_t, \
+                                          cplx<double>*, hipStream_t);
+
+ANN_INSTANTIATE(float)
+ANN_INSTANTIATE(double)
+
+}  // namespace ann

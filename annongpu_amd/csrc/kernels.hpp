@@ -1,0 +1,121 @@
+// kernels.hpp -- device views and launch wrappers of the gfx950 VMC kernels.
+//
+// A "dense net" is PsiRBM (include/quantum_state/PsiRBM.hpp:193-218) or a PsiDeep
+// with one hidden layer (include/quantum_state/PsiDeep.hpp:70-135).  On the device
+// both are stored as a dense visible x hidden matrix Wd[v][j] (the gather table of
+// PsiDeep folded in), hidden units padded to Mpad = 64 * U so that one wavefront
+// owns one chain and every lane owns U hidden units.
+#pragma once
+
+#include "common.hpp"
+
+#include <vector>
+
+namespace ann {
+
+enum MoveKind : int { MOVE_FLIP = 0, MOVE_EXCHANGE = 1 };
+enum ParamLayout : int { LAYOUT_RBM = 0, LAYOUT_DEEP1 = 1 };
+
+template <typename T>
+struct DenseView {
+    int N;             // visible units (sites)
+    int M;             // hidden units
+    int Mpad;          // 64 * U
+    int C;             // connectivity (PsiDeep lhs_connectivity; N for PsiRBM)
+    int layout;        // ParamLayout
+    int num_params;
+    int real_f;        // all final weights real -> cheaper real-part kernels
+    const cplx<T>* Wd;     // [N][Mpad]
+    const cplx<T>* b;      // [Mpad] (zero for PsiRBM)
+    const cplx<T>* f;      // [Mpad] final weights (zero in the padding)
+    const uint32_t* conn;  // [C*M] PsiDeep connections (nullptr for PsiRBM)
+};
+
+// Pauli-sum operator compiled for the device: strings grouped by flip mask so
+// that X and Y strings that reach the same s' share one wave-function ratio.
+struct OpView {
+    int num_groups;
+    int words;
+    const uint64_t* group_flip;      // [num_groups][words]
+    const int* group_begin;          // [num_groups + 1] ranges into the string arrays
+    const uint64_t* string_zy;       // [num_strings][words]  Z|Y mask (b bits)
+    const double* string_coef;       // [num_strings][2] coefficient * (-i)^{#Y}
+};
+
+// ---------------------------------------------------------------- launches
+// Every wrapper launches on `stream` and throws HipError on a launch failure.
+
+void launch_enumerate_confs(uint64_t* confs, int64_t first_index, int64_t count, int words,
+                            hipStream_t stream);
+
+template <typename T>
+void launch_theta_valu(const DenseView<T>& net, const uint64_t* confs, int words, int64_t count,
+                       cplx<T>* theta, hipStream_t stream);
+
+// theta = S * Wd + b on MFMA: spins are exact in bf16, Wd is split into three bf16
+// terms (hi, mid, lo) and accumulated in fp32.  wfrag is the fragment-ordered
+// split matrix built by pack_theta_fragments().
+void launch_theta_mfma(const DenseView<float>& net, const uint16_t* wfrag, int npad,
+                       const uint64_t* confs, int words, int64_t count, cplx<float>* theta,
+                       hipStream_t stream);
+void pack_theta_fragments(const double* Wd_interleaved, int N, int Mpad, int nsplit,
+                          std::vector<uint16_t>& out, int& npad);
+
+template <typename T>
+void launch_net_log_psi(const DenseView<T>& net, const cplx<T>* theta, int64_t count,
+                        cplx<double> log_prefactor, cplx<double>* log_psi, double* weight_exact,
+                        hipStream_t stream);
+
+template <typename T>
+void launch_mc_sweep(const DenseView<T>& net, uint64_t* chain_confs, cplx<T>* theta, int num_chains,
+                     int words, int move, int64_t num_proposals, uint64_t counter, uint64_t seed,
+                     uint32_t* accepted, hipStream_t stream);
+
+template <typename T>
+void launch_local_energy(const DenseView<T>& net, const OpView& op, const uint64_t* confs,
+                         const cplx<T>* theta, int64_t count, cplx<double>* e_loc,
+                         hipStream_t stream);
+
+template <typename T>
+void launch_g_factor(const DenseView<T>& net, const cplx<T>* theta, int64_t count, cplx<T>* g,
+                     hipStream_t stream);
+
+template <typename T>
+void launch_materialize_ok(const DenseView<T>& net, const uint64_t* confs, int words,
+                           const cplx<T>* g, int64_t count, const cplx<double>* center,
+                           cplx<T>* O, hipStream_t stream);
+
+// Y[v][j] += sum_s coef_s * spin_sv * X_sj  (and Yb[j] = sum_s coef_s X_sj,
+// Ya[v] = sum_s coef_s spin_sv), fp64 accumulation.  X_sj = g_sj or conj(g_sj).
+template <typename T>
+void launch_spin_weighted_sums(const DenseView<T>& net, const uint64_t* confs, int words,
+                               const cplx<T>* g, int conj_g, const cplx<double>* coef,
+                               int64_t count, cplx<double>* Yw, cplx<double>* Yb,
+                               cplx<double>* Ya, hipStream_t stream);
+
+// u_s = coef_s * (O_s . v - shift_u) for the structured O of a dense net; V is the
+// reindexed W part of v ([N][Mpad]), vb the bias part, va the visible part.
+template <typename T>
+void launch_structured_Ov(const DenseView<T>& net, const uint64_t* confs, int words,
+                          const cplx<T>* g, const cplx<T>* SV, const cplx<double>* vb,
+                          const cplx<double>* va, int64_t count, cplx<double>* u,
+                          hipStream_t stream);
+
+// weighted fp64 reductions
+void launch_weighted_sums(const cplx<double>* e_loc, const double* w, int64_t count,
+                          double* out4 /* E.re, E.im, E2, sum w */, hipStream_t stream);
+
+// dense Hermitian Gram S = O^H diag(w) O  (O row-major [count][P], complex T)
+template <typename T>
+void launch_gram(const cplx<T>* O, const double* w, int64_t count, int P, cplx<double>* S,
+                 hipStream_t stream);
+
+// y = S x for dense S in fp64 (complex)
+void launch_dense_matvec(const cplx<double>* S, const cplx<double>* x, int P, double shift,
+                         cplx<double>* y, hipStream_t stream);
+
+// in-place blocked Cholesky of the lower triangle (fp64 complex); info != 0 on failure
+void cholesky_lower(cplx<double>* A, int P, int* d_info, hipStream_t stream);
+void cholesky_solve_inplace(const cplx<double>* L, int P, cplx<double>* x, hipStream_t stream);
+
+}  // namespace ann

@@ -1,0 +1,35 @@
+// ops.hpp -- launch wrappers of sampler_ops.hip (chain initialisation, SR vector
+// plumbing, fp64 vector kernels).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace ann {
+
+void launch_init_chains(uint64_t* confs, int num_chains, int words, int N, int move, int total_z,
+                        uint32_t call_index, uint64_t seed, hipStream_t stream);
+
+template <typename T>
+void launch_unpack_v(const DenseView<T>& net, const cplx<double>* v, cplx<double>* va, cplx<double>* vb,
+                     cplx<T>* V, hipStream_t stream);
+
+void launch_pack_y(int layout, int N, int M, int Mpad, int C, const uint32_t* conn, const cplx<double>* Ya,
+                   const cplx<double>* Yb, const cplx<double>* Yw, const cplx<double>* v, double shift, int P,
+                   cplx<double>* y, hipStream_t stream);
+
+void launch_pack_fragments(const cplx<float>* V, int N, int Mpad, int npad, uint16_t* out, hipStream_t stream);
+
+void launch_cdot(const cplx<double>* a, const cplx<double>* b, int64_t n, int conj_a, double* out2,
+                 hipStream_t stream);
+void launch_center_scale(cplx<double>* u, const double* w, const double* c2, int64_t n, hipStream_t stream);
+void launch_cg_step(cplx<double>* x, cplx<double>* r, const cplx<double>* p, const cplx<double>* Ap, double alpha,
+                    int64_t n, hipStream_t stream);
+void launch_cg_dir(cplx<double>* p, const cplx<double>* r, double beta, int64_t n, hipStream_t stream);
+void launch_rank1(cplx<double>* S, const cplx<double>* o, int P, double alpha, hipStream_t stream);
+void launch_add_diag(cplx<double>* S, int P, double shift, hipStream_t stream);
+template <typename Tin, typename Tout>
+void launch_convert(const cplx<Tin>* a, cplx<Tout>* b, int64_t n, hipStream_t stream);
+void launch_fill(double* a, double v, int64_t n, hipStream_t stream);
+void launch_weight_coef(const double* w, const cplx<double>* e, int64_t n, cplx<double>* coef, hipStream_t stream);
+
+}  // namespace ann

@@ -1,0 +1,710 @@
+// runtime.cpp -- host runtime: parameter upload, sampling driver, local energies,
+// TDVP statistics and the SR solve.  Every device call goes through the kernels
+// in dense_net.hip / linalg.hip / sampler_ops.hip on one HIP stream.
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace ann {
+
+// ------------------------------------------------------------------ runtime
+
+Runtime& Runtime::get() {
+    static Runtime rt;
+    return rt;
+}
+
+hipStream_t Runtime::stream() {
+    if (!stream_) {
+        ANN_HIP_CHECK(hipSetDevice(device_));
+        ANN_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    }
+    return stream_;
+}
+
+void Runtime::set_device(int device) {
+    if (stream_) {
+        ANN_HIP_CHECK(hipStreamSynchronize(stream_));
+        collect();
+        for (auto e : pool_) (void)hipEventDestroy(e);
+        pool_.clear();
+        ANN_HIP_CHECK(hipStreamDestroy(stream_));
+        stream_ = nullptr;
+    }
+    device_ = device;
+    ANN_HIP_CHECK(hipSetDevice(device_));
+}
+
+void Runtime::sync() {
+    ANN_HIP_CHECK(hipStreamSynchronize(stream()));
+    collect();
+}
+
+hipEvent_t Runtime::event() {
+    if (!pool_.empty()) {
+        hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    ANN_HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+void Runtime::begin(const char* name) {
+    open_name_ = name;
+    open_a_ = event();
+    ANN_HIP_CHECK(hipEventRecord(open_a_, stream()));
+}
+
+void Runtime::end() {
+    hipEvent_t b = event();
+    ANN_HIP_CHECK(hipEventRecord(b, stream()));
+    pending_.push_back({open_name_, open_a_, b});
+    open_a_ = nullptr;
+}
+
+void Runtime::collect() {
+    for (auto& p : pending_) {
+        ANN_HIP_CHECK(hipEventSynchronize(p.b));
+        float ms = 0.f;
+        ANN_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+        auto& t = totals[p.name];
+        t.first += ms;
+        t.second += 1;
+        pool_.push_back(p.a);
+        pool_.push_back(p.b);
+    }
+    pending_.clear();
+}
+
+static hipStream_t S() { return Runtime::get().stream(); }
+
+// --------------------------------------------------------------------- Psi
+
+static int units_per_lane(int M) {
+    const int need = (M + 63) / 64;
+    for (int u : {1, 2, 4, 7, 8, 16})
+        if (need <= u) return u;
+    throw std::invalid_argument("hidden layer wider than 1024 units is not supported");
+}
+
+void Psi::finalize() {
+    words = std::max(1, (num_sites + 63) / 64);
+    if (num_sites > 64 * kMaxWords) throw std::invalid_argument("at most 256 sites are supported");
+    if (kind == PSI_RBM) {
+        layout = LAYOUT_RBM;
+        C = N;
+        num_params = N * M;
+        Mpad = 64 * units_per_lane(M);
+    } else if (kind == PSI_DEEP) {
+        num_params = N;
+        for (int l = 0; l < num_layers; l++) num_params += sizes[l] + conns[l] * sizes[l];
+        if (num_layers == 1) {
+            layout = LAYOUT_DEEP1;
+            M = sizes[0];
+            C = conns[0];
+            W = w_l[0];
+            b = b_l[0];
+            conn = conn_l[0];
+            Mpad = 64 * units_per_lane(M);
+            for (uint32_t c : conn)
+                if (int(c) >= N) throw std::invalid_argument("PsiDeep connection index out of range");
+        }
+    } else {
+        num_params = int(params.size());
+    }
+    dirty = true;
+}
+
+void Psi::sync_device() {
+    if (!dirty) return;
+    if (!dense()) {
+        dirty = false;
+        return;
+    }
+    std::vector<cd> Wd(size_t(N) * Mpad, 0.0), bd(Mpad, 0.0), fd(Mpad, 0.0);
+    if (kind == PSI_RBM) {
+        for (int v = 0; v < N; v++)
+            for (int j = 0; j < M; j++) Wd[size_t(v) * Mpad + j] = W[size_t(v) * M + j];
+        for (int j = 0; j < M; j++) fd[j] = f[0];
+    } else {
+        for (int i = 0; i < C; i++)
+            for (int j = 0; j < M; j++) Wd[size_t(conn[size_t(i) * M + j]) * Mpad + j] += W[size_t(i) * M + j];
+        for (int j = 0; j < M; j++) {
+            bd[j] = b[j];
+            fd[j] = f[j];
+        }
+    }
+    real_f = true;
+    for (int j = 0; j < M; j++) real_f = real_f && fd[j].imag() == 0.0;
+    auto up64 = [](DevBuf<cplx<double>>& d, const std::vector<cd>& h) {
+        d.upload(reinterpret_cast<const cplx<double>*>(h.data()), h.size());
+    };
+    auto up32 = [](DevBuf<cplx<float>>& d, const std::vector<cd>& h) {
+        std::vector<cplx<float>> t(h.size());
+        for (size_t i = 0; i < h.size(); i++) t[i] = {float(h[i].real()), float(h[i].imag())};
+        d.upload(t.data(), t.size());
+    };
+    up64(Wd64, Wd);
+    up64(b64, bd);
+    up64(f64, fd);
+    up32(Wd32, Wd);
+    up32(b32, bd);
+    up32(f32, fd);
+    std::vector<cd> z(Mpad, 0.0);
+    up64(zero64, z);
+    up32(zero32, z);
+    if (kind == PSI_DEEP) conn_d.upload(conn.data(), conn.size());
+    std::vector<uint16_t> frag;
+    pack_theta_fragments(reinterpret_cast<const double*>(Wd.data()), N, Mpad, 3, frag, npad);
+    wfrag.upload(frag.data(), frag.size());
+    dirty = false;
+}
+
+DenseView<double> Psi::view64() {
+    sync_device();
+    return {N, M, Mpad, C, layout, num_params, real_f ? 1 : 0, Wd64.p, b64.p, f64.p,
+            kind == PSI_DEEP ? conn_d.p : nullptr};
+}
+
+DenseView<float> Psi::view32() {
+    sync_device();
+    return {N, M, Mpad, C, layout, num_params, real_f ? 1 : 0, Wd32.p, b32.p, f32.p,
+            kind == PSI_DEEP ? conn_d.p : nullptr};
+}
+
+std::vector<cd> Psi::get_params() const {
+    std::vector<cd> p;
+    p.reserve(num_params);
+    if (kind == PSI_RBM) {
+        p = W;
+    } else if (kind == PSI_DEEP) {
+        p.insert(p.end(), a.begin(), a.end());
+        for (int l = 0; l < num_layers; l++) {
+            p.insert(p.end(), b_l[l].begin(), b_l[l].end());
+            p.insert(p.end(), w_l[l].begin(), w_l[l].end());
+        }
+    } else {
+        p = params;
+    }
+    return p;
+}
+
+void Psi::set_params(const cd* p) {
+    if (kind == PSI_RBM) {
+        W.assign(p, p + num_params);
+    } else if (kind == PSI_DEEP) {
+        size_t o = 0;
+        a.assign(p, p + N);
+        o += N;
+        for (int l = 0; l < num_layers; l++) {
+            b_l[l].assign(p + o, p + o + sizes[l]);
+            o += sizes[l];
+            w_l[l].assign(p + o, p + o + size_t(conns[l]) * sizes[l]);
+            o += size_t(conns[l]) * sizes[l];
+        }
+        if (num_layers == 1) {
+            W = w_l[0];
+            b = b_l[0];
+        }
+    } else {
+        params.assign(p, p + num_params);
+    }
+    dirty = true;
+}
+
+std::unique_ptr<Psi> Psi::clone() const {
+    auto q = std::make_unique<Psi>();
+    q->kind = kind;
+    q->num_sites = num_sites;
+    q->N = N;
+    q->num_params = num_params;
+    q->words = words;
+    q->log_prefactor = log_prefactor;
+    q->precision = precision;
+    q->M = M;
+    q->C = C;
+    q->Mpad = Mpad;
+    q->layout = layout;
+    q->W = W;
+    q->a = a;
+    q->b = b;
+    q->f = f;
+    q->conn = conn;
+    q->num_layers = num_layers;
+    q->sizes = sizes;
+    q->conns = conns;
+    q->conn_l = conn_l;
+    q->w_l = w_l;
+    q->b_l = b_l;
+    std::copy(extent, extent + 3, q->extent);
+    q->cnn_layers = cnn_layers;
+    q->channels = channels;
+    q->connectivity = connectivity;
+    q->sym = sym;
+    q->num_sym = num_sym;
+    q->params = params;
+    q->final_factor = final_factor;
+    q->dirty = true;
+    return q;
+}
+
+// ---------------------------------------------------------------- operator
+
+int Operator::max_site() const {
+    int m = -1;
+    for (int n = 0; n < num_strings; n++)
+        for (int w = 0; w < kMaxWords; w++) {
+            const uint64_t x = a[size_t(n) * kMaxWords + w] | b[size_t(n) * kMaxWords + w];
+            if (x) m = std::max(m, w * 64 + 63 - __builtin_clzll(x));
+        }
+    return m;
+}
+
+const DeviceOp& Operator::device(int words) {
+    auto it = compiled.find(words);
+    if (it != compiled.end()) return *it->second;
+    auto d = std::make_unique<DeviceOp>();
+    d->words = words;
+    // group strings by flip mask (X|Y bits = a ^ b), first-occurrence order
+    std::vector<std::vector<int>> groups;
+    std::vector<std::vector<uint64_t>> masks;
+    for (int n = 0; n < num_strings; n++) {
+        std::vector<uint64_t> m(words);
+        for (int w = 0; w < words; w++) m[w] = a[size_t(n) * kMaxWords + w] ^ b[size_t(n) * kMaxWords + w];
+        size_t g = 0;
+        for (; g < masks.size(); g++)
+            if (masks[g] == m) break;
+        if (g == masks.size()) {
+            masks.push_back(m);
+            groups.emplace_back();
+        }
+        groups[g].push_back(n);
+    }
+    std::vector<uint64_t> flip, zy;
+    std::vector<int> begin;
+    std::vector<double> coef;
+    for (size_t g = 0; g < groups.size(); g++) {
+        begin.push_back(int(coef.size() / 2));
+        flip.insert(flip.end(), masks[g].begin(), masks[g].end());
+        for (int n : groups[g]) {
+            int ny = 0;
+            for (int w = 0; w < kMaxWords; w++)
+                ny += __builtin_popcountll(~a[size_t(n) * kMaxWords + w] & b[size_t(n) * kMaxWords + w]);
+            // (-i)^{#Y}  (PauliString.hpp:193-208)
+            cd ph = 1.0;
+            if ((ny & 3) > 1) ph = -ph;
+            if (ny & 1) ph *= cd(0.0, -1.0);
+            const cd c = coef[n] * ph;
+            coef.push_back(c.real());
+            coef.push_back(c.imag());
+            for (int w = 0; w < words; w++) zy.push_back(b[size_t(n) * kMaxWords + w]);
+        }
+    }
+    begin.push_back(int(coef.size() / 2));
+    d->num_groups = int(groups.size());
+    d->group_flip.upload(flip.data(), flip.size());
+    d->group_begin.upload(begin.data(), begin.size());
+    d->string_zy.upload(zy.data(), zy.size());
+    d->string_coef.upload(coef.data(), coef.size());
+    auto& ref = *d;
+    compiled[words] = std::move(d);
+    return ref;
+}
+
+// ------------------------------------------------------------------ sampling
+
+void compute_theta(Psi& psi, const uint64_t* confs, int64_t count, Samples& out, int64_t offset) {
+    ScopedTimer timer("theta");
+    if (psi.precision == 0) {
+        launch_theta_valu<double>(psi.view64(), confs, psi.words, count, out.th64.p + offset * psi.Mpad, S());
+    } else {
+        auto v = psi.view32();
+        launch_theta_mfma(v, psi.wfrag.p, psi.npad, confs, psi.words, count, out.th32.p + offset * psi.Mpad, S());
+    }
+}
+
+static void require_dense(const Psi& psi) {
+    if (!psi.dense())
+        throw std::logic_error("this wave function kind is not available on the GPU path yet");
+}
+
+static void prepare_samples(Psi& psi, Samples& s, int64_t count) {
+    s.count = count;
+    s.words = psi.words;
+    s.Mpad = psi.Mpad;
+    s.precision = psi.precision;
+    s.confs.ensure(size_t(count) * psi.words);
+    s.w.ensure(count);
+    s.log_psi.ensure(count);
+    if (psi.precision == 0)
+        s.th64.ensure(size_t(count) * psi.Mpad);
+    else
+        s.th32.ensure(size_t(count) * psi.Mpad);
+}
+
+void sample(Psi& psi, Ensemble& ens) {
+    require_dense(psi);
+    psi.sync_device();
+    Samples& s = ens.samples;
+    const int words = psi.words;
+    if (ens.kind == ENS_EXACT) {
+        if (int(ens.num_sites) != psi.num_sites)
+            throw std::invalid_argument("ExactSummation num_sites does not match the wave function");
+        if (ens.num_sites > 26) throw std::invalid_argument("exact summation is limited to 26 sites");
+        const int64_t count = int64_t(1) << ens.num_sites;
+        prepare_samples(psi, s, count);
+        launch_enumerate_confs(s.confs.p, 0, count, words, S());
+        compute_theta(psi, s.confs.p, count, s, 0);
+        if (psi.precision == 0)
+            launch_net_log_psi<double>(psi.view64(), s.th64.p, count, {psi.log_prefactor.real(), psi.log_prefactor.imag()},
+                                       s.log_psi.p, s.w.p, S());
+        else
+            launch_net_log_psi<float>(psi.view32(), s.th32.p, count, {psi.log_prefactor.real(), psi.log_prefactor.imag()},
+                                      s.log_psi.p, s.w.p, S());
+        return;
+    }
+    const int B = int(ens.num_chains);
+    if (B <= 0) throw std::invalid_argument("num_markov_chains must be positive");
+    const int64_t steps = ens.num_samples / ens.num_chains;
+    if (steps <= 0) throw std::invalid_argument("num_samples must be at least num_markov_chains");
+    const int64_t count = steps * B;
+    prepare_samples(psi, s, count);
+    const int N = psi.N;
+    const bool reinit = !ens.persistent || !ens.chains_valid || ens.chain_words != words || ens.chain_N != N;
+    ens.chains.ensure(size_t(B) * words);
+    ens.acc.ensure(B);
+    ANN_HIP_CHECK(hipMemsetAsync(ens.acc.p, 0, sizeof(uint32_t) * B, S()));
+    if (reinit) {
+        if (ens.move == MOVE_EXCHANGE && ((N + ens.total_z) % 2 != 0 || std::abs(ens.total_z) > N))
+            throw std::invalid_argument("total_z incompatible with the number of sites");
+        launch_init_chains(ens.chains.p, B, words, N, ens.move, ens.total_z, ens.call_index, ens.seed, S());
+        ens.call_index++;
+    }
+    // fresh angles of the chains
+    Samples chain_view;
+    if (psi.precision == 0) {
+        ens.chain_th64.ensure(size_t(B) * psi.Mpad);
+        chain_view.th64.p = ens.chain_th64.p;
+    } else {
+        ens.chain_th32.ensure(size_t(B) * psi.Mpad);
+        chain_view.th32.p = ens.chain_th32.p;
+    }
+    compute_theta(psi, ens.chains.p, B, chain_view, 0);
+    const int64_t therm = reinit ? int64_t(ens.num_therm) * N : 0;
+    for (int64_t step = 0; step < steps; step++) {
+        const int64_t nprop = (step == 0 ? therm : 0) + int64_t(ens.num_sweeps) * N;
+        {
+            ScopedTimer timer("mc_sweep");
+            if (psi.precision == 0)
+                launch_mc_sweep<double>(psi.view64(), ens.chains.p, ens.chain_th64.p, B, words, ens.move, nprop,
+                                        ens.counter, ens.seed, ens.acc.p, S());
+            else
+                launch_mc_sweep<float>(psi.view32(), ens.chains.p, ens.chain_th32.p, B, words, ens.move, nprop,
+                                       ens.counter, ens.seed, ens.acc.p, S());
+        }
+        ens.counter += uint64_t(nprop);
+        ANN_HIP_CHECK(hipMemcpyAsync(s.confs.p + step * B * words, ens.chains.p, sizeof(uint64_t) * B * words,
+                                     hipMemcpyDeviceToDevice, S()));
+        compute_theta(psi, ens.chains.p, B, s, step * B);
+        // restart the next sweep from exact angles
+        if (psi.precision == 0)
+            ANN_HIP_CHECK(hipMemcpyAsync(ens.chain_th64.p, s.th64.p + step * B * psi.Mpad,
+                                         sizeof(cplx<double>) * B * psi.Mpad, hipMemcpyDeviceToDevice, S()));
+        else
+            ANN_HIP_CHECK(hipMemcpyAsync(ens.chain_th32.p, s.th32.p + step * B * psi.Mpad,
+                                         sizeof(cplx<float>) * B * psi.Mpad, hipMemcpyDeviceToDevice, S()));
+    }
+    chain_view.th64.p = nullptr;
+    chain_view.th32.p = nullptr;
+    ens.chains_valid = true;
+    ens.chain_words = words;
+    ens.chain_N = N;
+    launch_fill(s.w.p, 1.0 / double(ens.num_samples), count, S());
+    if (psi.precision == 0)
+        launch_net_log_psi<double>(psi.view64(), s.th64.p, count, {psi.log_prefactor.real(), psi.log_prefactor.imag()},
+                                   s.log_psi.p, nullptr, S());
+    else
+        launch_net_log_psi<float>(psi.view32(), s.th32.p, count, {psi.log_prefactor.real(), psi.log_prefactor.imag()},
+                                  s.log_psi.p, nullptr, S());
+    std::vector<uint32_t> acc(B);
+    ANN_HIP_CHECK(hipMemcpyAsync(acc.data(), ens.acc.p, sizeof(uint32_t) * B, hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    uint64_t total_acc = 0;
+    for (uint32_t x : acc) total_acc += x;
+    const uint64_t total = uint64_t(B) * uint64_t(therm + steps * int64_t(ens.num_sweeps) * N);
+    ens.accepted = total_acc;
+    ens.rejected = total - total_acc;
+}
+
+void local_energies(Psi& psi, Operator& op, Samples& s) {
+    if (op.max_site() >= psi.num_sites) throw std::invalid_argument("operator acts on sites beyond num_sites");
+    const DeviceOp& d = op.device(psi.words);
+    s.e_loc.ensure(s.count);
+    ScopedTimer timer("local_energy");
+    if (psi.precision == 0)
+        launch_local_energy<double>(psi.view64(), d.view(), s.confs.p, s.th64.p, s.count, s.e_loc.p, S());
+    else
+        launch_local_energy<float>(psi.view32(), d.view(), s.confs.p, s.th32.p, s.count, s.e_loc.p, S());
+}
+
+// ----------------------------------------------------------------------- TDVP
+
+namespace {
+
+template <typename T>
+struct Prec;
+template <>
+struct Prec<double> {
+    static DevBuf<cplx<double>>& th(Samples& s) { return s.th64; }
+    static DevBuf<cplx<double>>& g(Tdvp& t) { return t.g64; }
+    static DenseView<double> view(Psi& p) { return p.view64(); }
+};
+template <>
+struct Prec<float> {
+    static DevBuf<cplx<float>>& th(Samples& s) { return s.th32; }
+    static DevBuf<cplx<float>>& g(Tdvp& t) { return t.g32; }
+    static DenseView<float> view(Psi& p) { return p.view32(); }
+};
+
+// weighted sums of the structured O, packed in parameter order: out = sum_s coef_s O_s (or conj O_s)
+template <typename T>
+void structured_sums(const DenseView<T>& net, const uint64_t* confs, int words, const cplx<T>* g, int conj_g,
+                     const cplx<double>* coef, int64_t count, cplx<double>* out, double shift,
+                     const cplx<double>* v) {
+    DevBuf<cplx<double>> Yw, Yb, Ya;
+    Yw.ensure(size_t(net.N) * net.Mpad);
+    Yb.ensure(net.Mpad);
+    Ya.ensure(net.N);
+    {
+        ScopedTimer timer("spin_sums");
+        launch_spin_weighted_sums<T>(net, confs, words, g, conj_g, coef, count, Yw.p, Yb.p, Ya.p, S());
+        launch_pack_y(net.layout, net.N, net.M, net.Mpad, net.C, net.conn, Ya.p, Yb.p, Yw.p, v, shift,
+                      net.num_params, out, S());
+    }
+    Runtime::get().sync();
+}
+
+template <typename T>
+void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
+    sample(psi, ens);
+    Samples& s = ens.samples;
+    local_energies(psi, op, s);
+    const int64_t n = s.count;
+    const int P = psi.num_params;
+    DevBuf<double> sums;
+    sums.ensure(4);
+    launch_weighted_sums(s.e_loc.p, s.w.p, n, sums.p, S());
+    double h[4];
+    ANN_HIP_CHECK(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    t.E = cd(h[0], h[1]);
+    t.E2 = h[2];
+    t.wsum = h[3];
+    auto net = Prec<T>::view(psi);
+    auto& g = Prec<T>::g(t);
+    g.ensure(size_t(n) * psi.Mpad);
+    {
+        ScopedTimer timer("o_k");
+        launch_g_factor<T>(net, Prec<T>::th(s).p, n, g.p, S());
+    }
+    DevBuf<cplx<double>> coef;
+    coef.ensure(n);
+    t.Obar.ensure(P);
+    t.Fd.ensure(P);
+    launch_weight_coef(s.w.p, nullptr, n, coef.p, S());
+    structured_sums<T>(net, s.confs.p, s.words, g.p, 0, coef.p, n, t.Obar.p, 0.0, nullptr);
+    launch_weight_coef(s.w.p, s.e_loc.p, n, coef.p, S());
+    structured_sums<T>(net, s.confs.p, s.words, g.p, 1, coef.p, n, t.Fd.p, 0.0, nullptr);
+    t.O.resize(P);
+    t.F.resize(P);
+    t.Obar.download(reinterpret_cast<cplx<double>*>(t.O.data()), P);
+    t.Fd.download(reinterpret_cast<cplx<double>*>(t.F.data()), P);
+    for (int k = 0; k < P; k++) t.F[k] -= t.E * std::conj(t.O[k]);   // TDVP.cu.template:300
+    t.Fd.upload(reinterpret_cast<const cplx<double>*>(t.F.data()), P);
+    // keep the structured O for S.v / CG
+    t.precision = psi.precision;
+    t.words = s.words;
+    t.N = psi.N;
+    t.M = psi.M;
+    t.Mpad = psi.Mpad;
+    t.C = psi.C;
+    t.layout = psi.layout;
+    t.npad = psi.npad;
+    t.count = n;
+    if (psi.kind == PSI_DEEP) t.conn.upload(psi.conn.data(), psi.conn.size());
+    t.confs.ensure(size_t(n) * s.words);
+    ANN_HIP_CHECK(hipMemcpyAsync(t.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice, S()));
+    t.w.ensure(n);
+    ANN_HIP_CHECK(hipMemcpyAsync(t.w.p, s.w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, S()));
+    t.has_samples = true;
+    t.has_S = false;
+    if (with_S) {
+        DevBuf<cplx<T>> O;
+        O.ensure(size_t(n) * P);
+        {
+            ScopedTimer timer("o_k");
+            launch_materialize_ok<T>(net, s.confs.p, s.words, g.p, n, t.Obar.p, O.p, S());
+        }
+        t.S.ensure(size_t(P) * P);
+        {
+            ScopedTimer timer("gram");
+            launch_gram<T>(O.p, s.w.p, n, P, t.S.p, S());
+        }
+        // reference S = sum w conj(O) O - conj(<O>) <O>; with centred O this needs
+        // (1 - sum w) conj(<O>) <O> added back (zero for normalised weights)
+        if (std::abs(1.0 - t.wsum) > 0.0) launch_rank1(t.S.p, t.Obar.p, P, 1.0 - t.wsum, S());
+        Runtime::get().sync();
+        t.has_S = true;
+    }
+    Runtime::get().sync();
+}
+
+template <typename T>
+void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y) {
+    const int P = int(t.P);
+    DenseView<T> net{};
+    net.N = t.N;
+    net.M = t.M;
+    net.Mpad = t.Mpad;
+    net.C = t.C;
+    net.layout = t.layout;
+    net.num_params = P;
+    net.conn = t.layout == LAYOUT_DEEP1 ? t.conn.p : nullptr;
+    DevBuf<cplx<double>> va, vb, u, coef, c;
+    DevBuf<cplx<T>> V, SV, zero;
+    va.ensure(t.N);
+    vb.ensure(t.Mpad);
+    V.ensure(size_t(t.N) * t.Mpad);
+    SV.ensure(size_t(t.count) * t.Mpad);
+    u.ensure(t.count);
+    c.ensure(1);
+    zero.ensure(t.Mpad);
+    ANN_HIP_CHECK(hipMemsetAsync(zero.p, 0, sizeof(cplx<T>) * t.Mpad, S()));
+    auto& g = Prec<T>::g(t);
+    {
+        ScopedTimer timer("sr_matvec");
+        launch_unpack_v<T>(net, v, va.p, vb.p, V.p, S());
+        DenseView<T> vnet = net;
+        vnet.Wd = V.p;
+        vnet.b = zero.p;
+        if constexpr (sizeof(T) == 4) {
+            DevBuf<uint16_t> frag;
+            frag.ensure(size_t(3) * (t.npad / 16) * (2 * t.Mpad / 16) * 64 * 4);
+            launch_pack_fragments(V.p, t.N, t.Mpad, t.npad, frag.p, S());
+            launch_theta_mfma(vnet, frag.p, t.npad, t.confs.p, t.words, t.count, SV.p, S());
+            launch_structured_Ov<T>(net, t.confs.p, t.words, g.p, SV.p, t.layout == LAYOUT_DEEP1 ? vb.p : nullptr,
+                                    t.layout == LAYOUT_DEEP1 ? va.p : nullptr, t.count, u.p, S());
+            Runtime::get().sync();
+        } else {
+            launch_theta_valu<T>(vnet, t.confs.p, t.words, t.count, SV.p, S());
+            launch_structured_Ov<T>(net, t.confs.p, t.words, g.p, SV.p, t.layout == LAYOUT_DEEP1 ? vb.p : nullptr,
+                                    t.layout == LAYOUT_DEEP1 ? va.p : nullptr, t.count, u.p, S());
+        }
+        launch_cdot(t.Obar.p, v, P, 0, reinterpret_cast<double*>(c.p), S());
+        launch_center_scale(u.p, t.w.p, reinterpret_cast<const double*>(c.p), t.count, S());
+    }
+    structured_sums<T>(net, t.confs.p, t.words, g.p, 1, u.p, t.count, y, shift, v);
+}
+
+}  // namespace
+
+void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
+    require_dense(psi);
+    if (uint32_t(psi.num_params) != t.P) throw std::invalid_argument("TDVP num_params does not match the wave function");
+    if (psi.precision == 0)
+        tdvp_eval_t<double>(t, op, psi, ens, with_S);
+    else
+        tdvp_eval_t<float>(t, op, psi, ens, with_S);
+}
+
+void tdvp_S_dot(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y) {
+    if (!t.has_samples) throw std::logic_error("TDVP.eval must run before S_dot_vector");
+    if (t.precision == 0)
+        tdvp_S_dot_t<double>(t, v, shift, y);
+    else
+        tdvp_S_dot_t<float>(t, v, shift, y);
+}
+
+int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x_out, int* iters, double* res) {
+    const int P = int(t.P);
+    DevBuf<cplx<double>> x;
+    x.ensure(P);
+    if (method == 0) {
+        if (!t.has_S) throw std::logic_error("Cholesky solve needs TDVP.eval with the S matrix");
+        DevBuf<cplx<double>> A;
+        A.ensure(size_t(P) * P);
+        DevBuf<int> info;
+        info.ensure(1);
+        ANN_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), S()));
+        ANN_HIP_CHECK(hipMemcpyAsync(A.p, t.S.p, sizeof(cplx<double>) * P * P, hipMemcpyDeviceToDevice, S()));
+        {
+            ScopedTimer timer("cholesky");
+            launch_add_diag(A.p, P, shift, S());
+            cholesky_lower(A.p, P, info.p, S());
+            ANN_HIP_CHECK(hipMemcpyAsync(x.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
+            cholesky_solve_inplace(A.p, P, x.p, S());
+        }
+        int hinfo = 0;
+        ANN_HIP_CHECK(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, S()));
+        Runtime::get().sync();
+        if (hinfo != 0) return 1;
+        x.download(reinterpret_cast<cplx<double>*>(x_out), P);
+        // residual || (S + shift) x - F || / || F ||
+        DevBuf<cplx<double>> y;
+        y.ensure(P);
+        launch_dense_matvec(t.S.p, x.p, P, shift, y.p, S());
+        std::vector<cd> hy(P);
+        y.download(reinterpret_cast<cplx<double>*>(hy.data()), P);
+        double num = 0, den = 0;
+        for (int k = 0; k < P; k++) {
+            num += std::norm(hy[k] - t.F[k]);
+            den += std::norm(t.F[k]);
+        }
+        *res = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
+        *iters = 1;
+        return 0;
+    }
+    // conjugate gradients, fp64 vectors and dot products on the device
+    if (!t.has_samples) throw std::logic_error("CG solve needs TDVP.eval first");
+    DevBuf<cplx<double>> r, p, Ap;
+    DevBuf<double> dots;
+    r.ensure(P);
+    p.ensure(P);
+    Ap.ensure(P);
+    dots.ensure(2);
+    ANN_HIP_CHECK(hipMemsetAsync(x.p, 0, sizeof(cplx<double>) * P, S()));
+    ANN_HIP_CHECK(hipMemcpyAsync(r.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
+    ANN_HIP_CHECK(hipMemcpyAsync(p.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
+    auto dot = [&](const cplx<double>* a, const cplx<double>* b) {
+        launch_cdot(a, b, P, 1, dots.p, S());
+        double h[2];
+        ANN_HIP_CHECK(hipMemcpyAsync(h, dots.p, sizeof(h), hipMemcpyDeviceToHost, S()));
+        Runtime::get().sync();
+        return cd(h[0], h[1]);
+    };
+    double rr = dot(r.p, r.p).real();
+    const double bnorm = std::sqrt(rr);
+    int it = 0;
+    while (it < max_iter && std::sqrt(rr) > rtol * bnorm) {
+        tdvp_S_dot(t, p.p, shift, Ap.p);
+        const double pAp = dot(p.p, Ap.p).real();
+        if (!(pAp > 0.0)) break;
+        const double alpha = rr / pAp;
+        launch_cg_step(x.p, r.p, p.p, Ap.p, alpha, P, S());
+        const double rr_new = dot(r.p, r.p).real();
+        launch_cg_dir(p.p, r.p, rr_new / rr, P, S());
+        rr = rr_new;
+        it++;
+    }
+    x.download(reinterpret_cast<cplx<double>*>(x_out), P);
+    *iters = it;
+    *res = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
+    return std::sqrt(rr) <= rtol * bnorm ? 0 : 2;
+}
+
+}  // namespace ann

@@ -1,0 +1,218 @@
+// runtime.hpp -- host-side objects behind the C ABI: device buffers, the stream,
+// per-kernel hipEvent timers, wave functions, operators, ensembles and TDVP state.
+#pragma once
+
+#include "ops.hpp"
+
+#include <complex>
+#include <map>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace ann {
+
+using cd = std::complex<double>;
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        release();
+        p = o.p;
+        n = o.n;
+        o.p = nullptr;
+        o.n = 0;
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void ensure(size_t m) {
+        if (m <= n && p) return;
+        release();
+        ANN_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (m > 0 ? m : 1)));
+        n = m;
+    }
+    void upload(const T* h, size_t m) {
+        ensure(m);
+        if (m) ANN_HIP_CHECK(hipMemcpy(p, h, sizeof(T) * m, hipMemcpyHostToDevice));
+    }
+    void download(T* h, size_t m) const {
+        if (m) ANN_HIP_CHECK(hipMemcpy(h, p, sizeof(T) * m, hipMemcpyDeviceToHost));
+    }
+};
+
+class Runtime {
+public:
+    static Runtime& get();
+    hipStream_t stream();
+    void set_device(int device);
+    void sync();
+    // per-kernel-class device timers (hipEvents on the library stream)
+    bool profiling = false;
+    void begin(const char* name);
+    void end();
+    void collect();
+    std::map<std::string, std::pair<double, int64_t>> totals;
+
+private:
+    hipStream_t stream_ = nullptr;
+    int device_ = 0;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    std::string open_name_;
+    hipEvent_t open_a_ = nullptr;
+    hipEvent_t event();
+};
+
+struct ScopedTimer {
+    explicit ScopedTimer(const char* name) {
+        if (Runtime::get().profiling) {
+            Runtime::get().begin(name);
+            on = true;
+        }
+    }
+    ~ScopedTimer() {
+        if (on) Runtime::get().end();
+    }
+    bool on = false;
+};
+
+enum PsiKind : int { PSI_RBM = 0, PSI_DEEP = 1, PSI_CNN = 2 };
+
+struct Psi {
+    int kind = PSI_RBM;
+    int num_sites = 0, N = 0, num_params = 0, words = 1;
+    cd log_prefactor = 0.0;
+    int precision = 0;  // ANN_FP64 / ANN_FP32
+    // dense nets (PsiRBM, PsiDeep with one hidden layer)
+    int M = 0, C = 0, Mpad = 0, layout = LAYOUT_RBM;
+    std::vector<cd> W, a, b, f;
+    std::vector<uint32_t> conn;
+    // multi-layer PsiDeep (host description)
+    int num_layers = 1;
+    std::vector<int> sizes, conns;
+    std::vector<std::vector<uint32_t>> conn_l;
+    std::vector<std::vector<cd>> w_l, b_l;
+    // PsiCNN
+    int extent[3] = {1, 1, 1};
+    int cnn_layers = 0;
+    std::vector<int> channels;
+    std::vector<int> connectivity;  // [L][3]
+    std::vector<uint32_t> sym;
+    int num_sym = 1;
+    std::vector<cd> params;  // PsiCNN parameters
+    double final_factor = 1.0;
+
+    // device state
+    bool dirty = true;
+    bool real_f = true;
+    DevBuf<cplx<double>> Wd64, b64, f64;
+    DevBuf<cplx<float>> Wd32, b32, f32;
+    DevBuf<uint32_t> conn_d;
+    DevBuf<uint16_t> wfrag;
+    int npad = 0;
+    DevBuf<cplx<double>> zero64;
+    DevBuf<cplx<float>> zero32;
+
+    bool dense() const { return kind == PSI_RBM || (kind == PSI_DEEP && num_layers == 1); }
+    void finalize();           // derived sizes after construction
+    void sync_device();        // upload parameters when dirty
+    DenseView<double> view64();
+    DenseView<float> view32();
+    std::vector<cd> get_params() const;
+    void set_params(const cd* p);
+    std::unique_ptr<Psi> clone() const;
+};
+
+struct DeviceOp {
+    int words = 0;
+    int num_groups = 0;
+    DevBuf<uint64_t> group_flip, string_zy;
+    DevBuf<int> group_begin;
+    DevBuf<double> string_coef;
+    OpView view() const {
+        return {num_groups, words, group_flip.p, group_begin.p, string_zy.p, string_coef.p};
+    }
+};
+
+struct Operator {
+    int num_strings = 0;
+    std::vector<uint64_t> a, b;  // [n][kMaxWords]
+    std::vector<cd> coef;
+    std::map<int, std::unique_ptr<DeviceOp>> compiled;
+    const DeviceOp& device(int words);
+    int max_site() const;
+};
+
+struct Samples {
+    int64_t count = 0;
+    int words = 0, Mpad = 0, precision = 0;
+    DevBuf<uint64_t> confs;
+    DevBuf<double> w;
+    DevBuf<cplx<double>> log_psi;
+    DevBuf<cplx<float>> th32;
+    DevBuf<cplx<double>> th64;
+    DevBuf<cplx<double>> e_loc;
+};
+
+enum EnsembleKind : int { ENS_MC = 0, ENS_EXACT = 1 };
+
+struct Ensemble {
+    int kind = ENS_MC;
+    uint32_t num_sites = 0;
+    uint32_t num_samples = 0, num_sweeps = 0, num_therm = 0, num_chains = 0;
+    int move = MOVE_FLIP, total_z = 0;
+    uint64_t seed = 0, counter = 0;
+    uint32_t call_index = 0;
+    bool persistent = false, chains_valid = false;
+    int chain_words = 0, chain_N = 0;
+    DevBuf<uint64_t> chains;
+    DevBuf<uint32_t> acc;
+    DevBuf<cplx<float>> chain_th32;
+    DevBuf<cplx<double>> chain_th64;
+    uint64_t accepted = 0, rejected = 0;
+    Samples samples;
+    uint64_t num_steps() const { return kind == ENS_EXACT ? (uint64_t(1) << num_sites) : num_samples; }
+};
+
+struct Tdvp {
+    uint32_t P = 0;
+    cd E = 0.0;
+    double E2 = 0.0, wsum = 0.0;
+    std::vector<cd> O, F;
+    bool has_S = false, has_samples = false;
+    DevBuf<cplx<double>> S, Obar, Fd;
+    // structured O of the last evaluation (dense nets)
+    int precision = 0, words = 0, N = 0, M = 0, Mpad = 0, C = 0, layout = 0, npad = 0;
+    int64_t count = 0;
+    DevBuf<uint32_t> conn;
+    DevBuf<uint64_t> confs;
+    DevBuf<double> w;
+    DevBuf<cplx<float>> g32;
+    DevBuf<cplx<double>> g64;
+};
+
+// core operations
+void sample(Psi& psi, Ensemble& ens);
+void compute_theta(Psi& psi, const uint64_t* confs, int64_t count, Samples& out, int64_t offset);
+void local_energies(Psi& psi, Operator& op, Samples& s);
+void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S);
+void tdvp_S_dot(Tdvp& t, const cplx<double>* v_dev, double shift, cplx<double>* y_dev);
+int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x, int* iters, double* res);
+
+}  // namespace ann

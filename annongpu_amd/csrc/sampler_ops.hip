@@ -1,0 +1,298 @@
+// sampler_ops.hip -- chain initialisation, SR vector plumbing and small fp64
+// vector kernels used by the host runtime.
+#include "ops.hpp"
+
+namespace ann {
+
+namespace {
+
+// Random initial configurations on the Philox stream (Init_Policy.hpp:15-25 draws
+// uniform bits; exchange sampling starts from a uniform configuration of the
+// requested S_z sector).
+__global__ void init_chains_kernel(uint64_t* confs, int num_chains, int words, int N, int move, int total_z,
+                                   uint32_t call_index, uint32_t k0, uint32_t k1) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= num_chains) return;
+    uint64_t w[kMaxWords] = {0, 0, 0, 0};
+    if (move == MOVE_FLIP) {
+        for (int blk = 0; blk < (words + 1) / 2; blk++) {
+            const u32x4 r = philox(uint32_t(c), uint32_t(blk), call_index, 1u, k0, k1);
+            w[2 * blk] = uint64_t(r.x) | (uint64_t(r.y) << 32);
+            if (2 * blk + 1 < words) w[2 * blk + 1] = uint64_t(r.z) | (uint64_t(r.w) << 32);
+        }
+        if (N % 64) w[N / 64] &= (uint64_t(1) << (N % 64)) - 1;
+        for (int k = (N + 63) / 64; k < kMaxWords; k++) w[k] = 0;
+    } else {
+        const int n_up = (N + total_z) / 2;
+        uint64_t down[kMaxWords] = {0, 0, 0, 0};
+        for (int i = 0; i < N; i++) down[i >> 6] |= uint64_t(1) << (i & 63);
+        u32x4 r = {0, 0, 0, 0};
+        for (int k = 0; k < n_up; k++) {
+            if ((k & 3) == 0) r = philox(uint32_t(c), uint32_t(k >> 2), call_index, 2u, k0, k1);
+            const uint32_t x = (k & 3) == 0 ? r.x : (k & 3) == 1 ? r.y : (k & 3) == 2 ? r.z : r.w;
+            const int pos = select_bit(down, words, x % uint32_t(N - k));
+            down[pos >> 6] &= ~(uint64_t(1) << (pos & 63));
+            w[pos >> 6] |= uint64_t(1) << (pos & 63);
+        }
+    }
+    for (int k = 0; k < words; k++) confs[int64_t(c) * words + k] = w[k];
+}
+
+// v (reference parameter order) -> va [N], vb [Mpad], V [N][Mpad] (W part re-indexed
+// onto visible units: V[conn(i,j)][j] += v_W(i,j))
+template <typename T>
+__global__ void unpack_v_kernel(DenseView<T> net, const cplx<double>* v, cplx<double>* va, cplx<double>* vb,
+                                cplx<T>* V) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= net.Mpad) return;
+    const bool real_unit = j < net.M;
+    if (net.layout == LAYOUT_RBM) {
+        for (int i = 0; i < net.N; i++) {
+            cplx<double> x = real_unit ? v[int64_t(i) * net.M + j] : cplx<double>{0, 0};
+            V[int64_t(i) * net.Mpad + j] = {T(x.re), T(x.im)};
+        }
+        if (vb) vb[j] = {0, 0};
+        return;
+    }
+    const int base = net.N + net.M;
+    // accumulate in fp64 first (duplicate connections sum)
+    for (int i = 0; i < net.N; i++) V[int64_t(i) * net.Mpad + j] = {T(0), T(0)};
+    if (real_unit) {
+        for (int i = 0; i < net.C; i++) {
+            const int vis = int(net.conn[int64_t(i) * net.M + j]);
+            const cplx<double> x = v[base + int64_t(i) * net.M + j];
+            cplx<T>& dst = V[int64_t(vis) * net.Mpad + j];
+            dst.re += T(x.re);
+            dst.im += T(x.im);
+        }
+    }
+    vb[j] = real_unit ? v[net.N + j] : cplx<double>{0, 0};
+    if (j < net.N) va[j] = v[j];
+    for (int i = net.Mpad + j; i < net.N; i += net.Mpad) va[i] = v[i];
+}
+
+// y (reference order) from Ya, Yb, Yw (+ shift * v)
+__global__ void pack_y_kernel(int layout, int N, int M, int Mpad, int C, const uint32_t* conn,
+                              const cplx<double>* Ya, const cplx<double>* Yb, const cplx<double>* Yw,
+                              const cplx<double>* v, double shift, int P, cplx<double>* y) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P) return;
+    cplx<double> r;
+    if (layout == LAYOUT_RBM) {
+        const int i = k / M, j = k % M;
+        r = Yw[int64_t(i) * Mpad + j];
+    } else if (k < N) {
+        r = Ya[k];
+    } else if (k < N + M) {
+        r = Yb[k - N];
+    } else {
+        const int q = k - N - M;
+        const int j = q % M;
+        r = Yw[int64_t(conn[q]) * Mpad + j];
+    }
+    if (v) {
+        r.re += shift * v[k].re;
+        r.im += shift * v[k].im;
+    }
+    y[k] = r;
+}
+
+// bf16x3 fragments of a [N][Mpad] complex fp32 matrix (layout of theta_mfma_kernel)
+__device__ inline uint16_t bf16_rn(float x) {
+    const uint32_t b = __float_as_uint(x);
+    return uint16_t((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+__global__ void pack_fragments_kernel(const cplx<float>* V, int N, int Mpad, int KT, int NT, uint16_t* out) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int ncol = 2 * Mpad;
+    if (e >= int64_t(KT) * 16 * ncol) return;
+    const int v = int(e / ncol), col = int(e % ncol);
+    float r = 0.f;
+    if (v < N) {
+        const cplx<float> x = V[int64_t(v) * Mpad + col / 2];
+        r = (col & 1) ? x.im : x.re;
+    }
+    const int kt = v / 16, kk = v % 16, nt = col / 16, cc = col % 16;
+    const int lane = (kk / 4) * 16 + cc, el = kk % 4;
+    for (int sp = 0; sp < 3; sp++) {
+        const uint16_t h = bf16_rn(r);
+        r -= __uint_as_float(uint32_t(h) << 16);
+        out[((((int64_t(sp) * KT + kt) * NT + nt) * 64 + lane) * 4) + el] = h;
+    }
+}
+
+constexpr int kDot = 1024;
+__global__ __launch_bounds__(kDot) void cdot_kernel(const cplx<double>* a, const cplx<double>* b, int64_t n,
+                                                    int conj_a, double* out) {
+    __shared__ double sr[kDot], si[kDot];
+    double re = 0, im = 0;
+    for (int64_t i = threadIdx.x; i < n; i += kDot) {
+        const double ar = a[i].re, ai = conj_a ? -a[i].im : a[i].im;
+        re += ar * b[i].re - ai * b[i].im;
+        im += ar * b[i].im + ai * b[i].re;
+    }
+    sr[threadIdx.x] = re;
+    si[threadIdx.x] = im;
+    __syncthreads();
+    for (int o = kDot / 2; o > 0; o >>= 1) {
+        if (int(threadIdx.x) < o) {
+            sr[threadIdx.x] += sr[threadIdx.x + o];
+            si[threadIdx.x] += si[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sr[0];
+        out[1] = si[0];
+    }
+}
+
+// u'_s = w_s * (u_s - c)   with c = (re, im) read from device memory
+__global__ void center_scale_kernel(cplx<double>* u, const double* w, const double* c, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u[i] = {w[i] * (u[i].re - c[0]), w[i] * (u[i].im - c[1])};
+}
+
+// x += alpha p ; r -= alpha Ap   (alpha real)
+__global__ void cg_step_kernel(cplx<double>* x, cplx<double>* r, const cplx<double>* p, const cplx<double>* Ap,
+                               double alpha, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i].re += alpha * p[i].re;
+    x[i].im += alpha * p[i].im;
+    r[i].re -= alpha * Ap[i].re;
+    r[i].im -= alpha * Ap[i].im;
+}
+// p = r + beta p
+__global__ void cg_dir_kernel(cplx<double>* p, const cplx<double>* r, double beta, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    p[i] = {r[i].re + beta * p[i].re, r[i].im + beta * p[i].im};
+}
+
+// S += alpha * conj(o) o^T
+__global__ void rank1_kernel(cplx<double>* S, const cplx<double>* o, int P, double alpha) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= int64_t(P) * P) return;
+    const int k = int(e / P), kp = int(e % P);
+    const cplx<double> a = o[k], b = o[kp];
+    S[e].re += alpha * (a.re * b.re + a.im * b.im);
+    S[e].im += alpha * (a.re * b.im - a.im * b.re);
+}
+
+__global__ void add_diag_kernel(cplx<double>* S, int P, double shift) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P) S[int64_t(i) * P + i].re += shift;
+}
+
+template <typename Tin, typename Tout>
+__global__ void convert_kernel(const cplx<Tin>* a, cplx<Tout>* b, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = {Tout(a[i].re), Tout(a[i].im)};
+}
+
+__global__ void fill_kernel(double* a, double v, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// coef_s = w_s * e_s (complex), or w_s when e == nullptr
+__global__ void weight_coef_kernel(const double* w, const cplx<double>* e, int64_t n, cplx<double>* coef) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    coef[i] = e ? cplx<double>{w[i] * e[i].re, w[i] * e[i].im} : cplx<double>{w[i], 0.0};
+}
+
+inline unsigned blocks_for(int64_t n, int b = 256) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+void launch_init_chains(uint64_t* confs, int num_chains, int words, int N, int move, int total_z,
+                        uint32_t call_index, uint64_t seed, hipStream_t stream) {
+    init_chains_kernel<<<blocks_for(num_chains, 64), 64, 0, stream>>>(confs, num_chains, words, N, move, total_z,
+                                                                      call_index, uint32_t(seed),
+                                                                      uint32_t(seed >> 32));
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_unpack_v(const DenseView<T>& net, const cplx<double>* v, cplx<double>* va, cplx<double>* vb,
+                     cplx<T>* V, hipStream_t stream) {
+    unpack_v_kernel<T><<<blocks_for(net.Mpad, 64), 64, 0, stream>>>(net, v, va, vb, V);
+    ANN_LAUNCH_CHECK();
+}
+template void launch_unpack_v<float>(const DenseView<float>&, const cplx<double>*, cplx<double>*, cplx<double>*,
+                                     cplx<float>*, hipStream_t);
+template void launch_unpack_v<double>(const DenseView<double>&, const cplx<double>*, cplx<double>*,
+                                      cplx<double>*, cplx<double>*, hipStream_t);
+
+void launch_pack_y(int layout, int N, int M, int Mpad, int C, const uint32_t* conn, const cplx<double>* Ya,
+                   const cplx<double>* Yb, const cplx<double>* Yw, const cplx<double>* v, double shift, int P,
+                   cplx<double>* y, hipStream_t stream) {
+    pack_y_kernel<<<blocks_for(P), 256, 0, stream>>>(layout, N, M, Mpad, C, conn, Ya, Yb, Yw, v, shift, P, y);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_pack_fragments(const cplx<float>* V, int N, int Mpad, int npad, uint16_t* out, hipStream_t stream) {
+    const int KT = npad / 16, NT = 2 * Mpad / 16;
+    const int64_t n = int64_t(npad) * 2 * Mpad;
+    pack_fragments_kernel<<<blocks_for(n), 256, 0, stream>>>(V, N, Mpad, KT, NT, out);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_cdot(const cplx<double>* a, const cplx<double>* b, int64_t n, int conj_a, double* out2,
+                 hipStream_t stream) {
+    cdot_kernel<<<1, kDot, 0, stream>>>(a, b, n, conj_a, out2);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_center_scale(cplx<double>* u, const double* w, const double* c2, int64_t n, hipStream_t stream) {
+    center_scale_kernel<<<blocks_for(n), 256, 0, stream>>>(u, w, c2, n);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_cg_step(cplx<double>* x, cplx<double>* r, const cplx<double>* p, const cplx<double>* Ap, double alpha,
+                    int64_t n, hipStream_t stream) {
+    cg_step_kernel<<<blocks_for(n), 256, 0, stream>>>(x, r, p, Ap, alpha, n);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_cg_dir(cplx<double>* p, const cplx<double>* r, double beta, int64_t n, hipStream_t stream) {
+    cg_dir_kernel<<<blocks_for(n), 256, 0, stream>>>(p, r, beta, n);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_rank1(cplx<double>* S, const cplx<double>* o, int P, double alpha, hipStream_t stream) {
+    rank1_kernel<<<blocks_for(int64_t(P) * P), 256, 0, stream>>>(S, o, P, alpha);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_add_diag(cplx<double>* S, int P, double shift, hipStream_t stream) {
+    add_diag_kernel<<<blocks_for(P), 256, 0, stream>>>(S, P, shift);
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename Tin, typename Tout>
+void launch_convert(const cplx<Tin>* a, cplx<Tout>* b, int64_t n, hipStream_t stream) {
+    if (n <= 0) return;
+    convert_kernel<Tin, Tout><<<blocks_for(n), 256, 0, stream>>>(a, b, n);
+    ANN_LAUNCH_CHECK();
+}
+template void launch_convert<float, double>(const cplx<float>*, cplx<double>*, int64_t, hipStream_t);
+template void launch_convert<double, double>(const cplx<double>*, cplx<double>*, int64_t, hipStream_t);
+template void launch_convert<double, float>(const cplx<double>*, cplx<float>*, int64_t, hipStream_t);
+
+void launch_fill(double* a, double v, int64_t n, hipStream_t stream) {
+    if (n <= 0) return;
+    fill_kernel<<<blocks_for(n), 256, 0, stream>>>(a, v, n);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_weight_coef(const double* w, const cplx<double>* e, int64_t n, cplx<double>* coef, hipStream_t stream) {
+    if (n <= 0) return;
+    weight_coef_kernel<<<blocks_for(n), 256, 0, stream>>>(w, e, n, coef);
+    ANN_LAUNCH_CHECK();
+}
+
+}  // namespace ann

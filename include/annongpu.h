@@ -1,0 +1,167 @@
+/*
+ * annongpu.h -- C ABI of the MI355X (gfx950) variational Monte Carlo library.
+ *
+ * Plain pointers and sizes only.  Complex arrays are interleaved (re, im) fp64
+ * pairs, i.e. numpy complex128 / C99 double complex / std::complex<double>.
+ * Spin configurations are little-endian uint64 words, bit i set <=> s_i = +1
+ * (reference: include/basis/Spins.h:104-118).  Every function returns 0 on
+ * success and a non-zero ANN_E_* code on failure; ann_last_error() then holds a
+ * thread-local message.  All calls are synchronous: results are on the host
+ * when the call returns.  Objects are not thread-safe; use one per thread.
+ *
+ * Each entry point names the reference interface it replaces.  The reference
+ * binds those through pybind11 in pyANNonGPU/main.cpp.template (cited as MAIN).
+ */
+#ifndef ANNONGPU_H
+#define ANNONGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ANN_OK 0
+#define ANN_E_INVALID 1      /* bad argument / capacity exceeded            */
+#define ANN_E_DEVICE 2       /* HIP runtime or kernel launch failure        */
+#define ANN_E_UNSUPPORTED 3  /* combination not implemented                 */
+#define ANN_E_NUMERIC 4      /* e.g. S + shift not positive definite        */
+
+typedef struct ann_psi_s* ann_psi;
+typedef struct ann_operator_s* ann_operator;
+typedef struct ann_ensemble_s* ann_ensemble;
+typedef struct ann_tdvp_s* ann_tdvp;
+typedef struct ann_proposal_s* ann_proposal;
+
+/* precision of the device path: FP64 everywhere, or FP32 sweeps / E_loc / O_k
+ * with the angle GEMM on MFMA (bf16 x3 split of W, fp32 accumulation).  fp64 is
+ * kept for every reduction, the SR solve and the returned results. */
+#define ANN_FP64 0
+#define ANN_FP32 1
+
+#define ANN_MOVE_FLIP 0       /* single spin flip  (policies/Update_Policy.hpp:19-33) */
+#define ANN_MOVE_EXCHANGE 1   /* swap of an anti-parallel pair, conserves S_z          */
+
+/* ---------------------------------------------------------------- runtime */
+const char* ann_last_error(void);
+/* setDevice (source/ANNonGPU.cu:7-9, MAIN:538) */
+int ann_set_device(int device);
+/* start_profiling / stop_profiling (source/ANNonGPU.cu:11-17, MAIN:539-540):
+ * switch the library's per-kernel hipEvent timers on and off. */
+int ann_profiler_start(void);
+int ann_profiler_stop(void);
+/* accumulated device time of one timed kernel class since ann_profiler_start */
+int ann_profiler_query(const char* name, double* milliseconds, int64_t* launches);
+int ann_profiler_reset(void);
+
+/* -------------------------------------------------------- wave functions */
+/* PsiRBM(W[N][M], final_weight, log_prefactor, gpu) -- PsiRBM.hpp:377-395, MAIN:120-154 */
+int ann_psi_create_rbm(ann_psi* out, int32_t N, int32_t M, const double* W, const double* final_weight,
+                       const double* log_prefactor);
+/* PsiDeep(num_sites, a[N], b_list, connections_list, W_list, final_weights,
+ * log_prefactor, gpu) -- PsiDeep.hpp:504-578, MAIN:71-114.  Layer l has
+ * sizes[l] units and lhs connectivity conns[l]; connections[l] and weights[l]
+ * are [conns[l]][sizes[l]] row-major, biases[l] is [sizes[l]]. */
+int ann_psi_create_deep(ann_psi* out, int32_t num_sites, int32_t N, const double* a, int32_t num_layers,
+                        const int32_t* sizes, const int32_t* conns, const uint32_t* const* connections,
+                        const double* const* weights, const double* const* biases, const double* final_weights,
+                        const double* log_prefactor);
+/* PsiCNN(extent[3], num_channels_list, connectivity_list[L][3], symmetry_classes[N],
+ * params, final_factor, log_prefactor, gpu) -- PsiCNN.hpp:308-340, MAIN:160-204 */
+int ann_psi_create_cnn(ann_psi* out, const int32_t* extent, int32_t num_layers, const int32_t* channels,
+                       const int32_t* connectivity, const uint32_t* symmetry_classes, int32_t num_params,
+                       const double* params, double final_factor, const double* log_prefactor);
+/* copy() (MAIN:82,127,171) */
+int ann_psi_copy(ann_psi src, ann_psi* out);
+int ann_psi_destroy(ann_psi psi);
+/* num_params / N / num_sites (MAIN:92-93,135-137,173-174) */
+int ann_psi_info(ann_psi psi, int32_t* num_params, int32_t* N, int32_t* num_sites);
+/* params property (MAIN:94-100,138-144,191-197); order of PsiDeep.cu:245-270 */
+int ann_psi_get_params(ann_psi psi, double* params);
+int ann_psi_set_params(ann_psi psi, const double* params);
+/* log_prefactor property (MAIN:84-88) */
+int ann_psi_get_log_prefactor(ann_psi psi, double* value2);
+int ann_psi_set_log_prefactor(ann_psi psi, const double* value2);
+/* additive: device precision (ANN_FP64 default) */
+int ann_psi_set_precision(ann_psi psi, int32_t precision);
+
+/* --------------------------------------------------------------- operator */
+/* Operator(PauliExpression, gpu) -- operator/Operator.cpp:18-39, MAIN:321-333.
+ * String n is (a[n][words], b[n][words]) in the PauliString encoding
+ * (basis/PauliString.hpp:37-56: X=(1,0), Y=(0,1), Z=(1,1)). */
+int ann_operator_create(ann_operator* out, int32_t num_strings, int32_t words, const uint64_t* a, const uint64_t* b,
+                        const double* coefficients);
+int ann_operator_destroy(ann_operator op);
+
+/* -------------------------------------------------------------- ensembles */
+/* MonteCarloSpins(num_samples, num_sweeps, num_thermalization_sweeps,
+ * num_markov_chains, gpu) -- ensembles/MonteCarlo.hpp:249-259, MAIN:358-366.
+ * Additive: move kind, S_z sector (2 S_z) for exchange moves, Philox seed. */
+int ann_monte_carlo_create(ann_ensemble* out, uint32_t num_samples, uint32_t num_sweeps,
+                           uint32_t num_thermalization_sweeps, uint32_t num_markov_chains, int32_t move,
+                           int32_t total_z, uint64_t seed);
+/* additive: keep chains between calls instead of re-initialising them
+ * (the reference re-initialises and re-thermalises on every call). */
+int ann_monte_carlo_set_persistent(ann_ensemble ens, int32_t persistent);
+/* ExactSummationSpins(num_sites, gpu) -- source/ensembles/ExactSummation.cu:16-25, MAIN:395-399 */
+int ann_exact_summation_create(ann_ensemble* out, uint32_t num_sites);
+int ann_ensemble_destroy(ann_ensemble ens);
+/* num_steps (MAIN:363,399) */
+int ann_ensemble_num_steps(ann_ensemble ens, uint64_t* num_steps);
+/* acceptance_rate (MAIN:364-366): raw counters of the last call */
+int ann_ensemble_acceptance(ann_ensemble ens, uint64_t* accepted, uint64_t* rejected);
+/* additive: configurations of the last sampling call, [num_steps][words] */
+int ann_ensemble_last_samples(ann_ensemble ens, uint64_t* confs, int64_t capacity, int64_t* count, int32_t* words);
+
+/* ------------------------------------------------------ network functions */
+/* log_psi_s(psi, spins) -- PsiVector.cu.template:101-133, MAIN:495-497 */
+int ann_log_psi_s(ann_psi psi, const uint64_t* conf, double* out2);
+/* psi_O_k(psi, spins) -- PsiOkVector.cu.template:40-74, MAIN:498-500 */
+int ann_psi_O_k(ann_psi psi, const uint64_t* conf, double* out);
+/* psi_O_k_vector(psi, exact) = sum_s O_k(s), unweighted -- PsiOkVector.cu.template:8-37 */
+int ann_psi_O_k_vector(ann_psi psi, ann_ensemble ens, double* out);
+/* log_psi(psi, ens) = sum_s w_s log psi(s) -- PsiVector.cu.template:73-99 */
+int ann_log_psi(ann_psi psi, ann_ensemble ens, double* out2);
+/* psi_vector / log_psi_vector -- PsiVector.cu.template:14-71 */
+int ann_psi_vector(ann_psi psi, ann_ensemble ens, double* out);
+int ann_log_psi_vector(ann_psi psi, ann_ensemble ens, double* out);
+/* psi.norm(exact) = sqrt(sum_s |psi(s)|^2) -- PsiNorm.cu.template:13-42 */
+int ann_psi_norm(ann_psi psi, ann_ensemble ens, double* out);
+/* apply_operator = psi(s) E_loc(s) -- ApplyOperator.cu.template:9-44 */
+int ann_apply_operator(ann_psi psi, ann_operator op, ann_ensemble ens, double* out);
+/* activation_function(x, layer) -- MAIN:534-536 */
+int ann_activation_function(const double* x2, int32_t layer, double* out2);
+
+/* ExpectationValue(gpu): __call__, fluctuation, gradient --
+ * ExpectationValue.cu.template:19-50,175-275, MAIN:420-433 */
+int ann_expectation_value(ann_operator op, ann_psi psi, ann_ensemble ens, double* out2);
+int ann_expectation_fluctuation(ann_operator op, ann_psi psi, ann_ensemble ens, double* sigma, double* value2);
+int ann_expectation_gradient(ann_operator op, ann_psi psi, ann_ensemble ens, double* gradient, double* value2);
+/* additive: per-sample local energies of the last sampling, [num_steps] */
+int ann_local_energies(ann_operator op, ann_psi psi, ann_ensemble ens, double* e_loc, int64_t capacity);
+
+/* ------------------------------------------------------------------- TDVP */
+/* TDVP(num_params, gpu) -- network_functions/TDVP.hpp:16-101, MAIN:465-489 */
+int ann_tdvp_create(ann_tdvp* out, uint32_t num_params);
+int ann_tdvp_destroy(ann_tdvp tdvp);
+/* eval (with_S = 1) and eval_F (with_S = 0) -- TDVP.cu.template:181-334 */
+int ann_tdvp_eval(ann_tdvp tdvp, ann_operator op, ann_psi psi, ann_ensemble ens, int32_t with_S);
+/* S_matrix [P][P], F_vector, O_k_vector, E_local, E2_local, var_H (MAIN:467-474) */
+int ann_tdvp_get_S(ann_tdvp tdvp, double* S);
+int ann_tdvp_get_F(ann_tdvp tdvp, double* F);
+int ann_tdvp_get_O(ann_tdvp tdvp, double* O);
+int ann_tdvp_get_energy(ann_tdvp tdvp, double* E2_complex, double* E2_local, double* var_H);
+/* S_dot_vector(vec, ensemble) -- TDVP.cu.template:336-443, MAIN:483-487 */
+int ann_tdvp_S_dot_vector(ann_tdvp tdvp, const double* v, double* out);
+/* additive (absent from the reference): solve (S + shift I) x = F.
+ * method 0 = fp64 Cholesky of the dense S (needs eval with_S = 1),
+ * method 1 = conjugate gradients on the matrix-free S (fp64 vectors and dots). */
+#define ANN_SOLVE_CHOLESKY 0
+#define ANN_SOLVE_CG 1
+int ann_tdvp_solve(ann_tdvp tdvp, double shift, int32_t method, double rtol, int32_t max_iter, double* x,
+                   int32_t* iterations, double* residual);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
