@@ -166,30 +166,81 @@ __device__ inline void flip_bit(Conf& c, int i) {
     c.w[3] ^= k == 3 ? m : 0;
 }
 
-// index of the rank-th set bit of mask (words of the configuration)
+// index of the rank-th set bit of mask (words of the configuration); defined below
+__device__ inline int select64(uint64_t x, uint32_t rank);
 __device__ inline int select_bit(const uint64_t* m, int words, uint32_t rank) {
     for (int w = 0; w < words; w++) {
         const uint32_t c = uint32_t(__popcll(m[w]));
-        if (rank < c) {
-            uint64_t x = m[w];
-            for (uint32_t r = 0; r < rank; r++) x &= x - 1;
-            return w * 64 + __ffsll((unsigned long long)x) - 1;
-        }
+        if (rank < c) return w * 64 + select64(m[w], rank);
         rank -= c;
     }
     return -1;
 }
 
 // ------------------------------------------------------ wave reductions
+// GFX9 DPP all-reduce: quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror and
+// row_mirror give 16-lane row sums, row_bcast:15 / row_bcast:31 fold the four
+// rows into lane 63, which is then read into a scalar register.  The result is
+// bit-identical in every lane (one summation order).
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline int dpp_i32(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline float dpp_f32(float v) {
+    return __int_as_float(dpp_i32<CTRL, ROW_MASK>(__float_as_int(v)));
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i32<CTRL, ROW_MASK>(int(b & 0xffffffff));
+    const int hi = dpp_i32<CTRL, ROW_MASK>(int(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 __device__ inline float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    v += dpp_f32<0xB1>(v);
+    v += dpp_f32<0x4E>(v);
+    v += dpp_f32<0x141>(v);
+    v += dpp_f32<0x140>(v);
+    v += dpp_f32<0x142, 0xa>(v);
+    v += dpp_f32<0x143, 0xc>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 __device__ inline double wave_sum(double v) {
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x140>(v);
+    v += dpp_f64<0x142, 0xa>(v);
+    v += dpp_f64<0x143, 0xc>(v);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(int(b & 0xffffffff), 63);
+    const int hi = __builtin_amdgcn_readlane(int(b >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// (x * n) >> 32: integer in [0, n) from a 32-bit random word (Lemire's
+// multiply-shift range reduction, without the rejection step)
+__host__ __device__ inline uint32_t range32(uint32_t x, uint32_t n) {
+    return uint32_t((uint64_t(x) * n) >> 32);
+}
+
+// position of the rank-th set bit of a 64-bit word (rank < popcount(x)),
+// by binary search on popcounts
+__device__ inline int select64(uint64_t x, uint32_t rank) {
+    int pos = 0;
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    for (int width = 32; width >= 1; width >>= 1) {
+        const uint64_t low = x & ((uint64_t(1) << width) - 1);
+        const uint32_t c = uint32_t(__popcll(low));
+        if (rank >= c) {
+            rank -= c;
+            x >>= width;
+            pos += width;
+        }
+    }
+    return pos;
 }
 
 __device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }

@@ -52,14 +52,15 @@ __device__ inline Conf load_conf(const uint64_t* confs, int64_t s, int words) {
 
 // Re(lc(z)) for the layer-0 polynomial, expanded in u = z^2 = a + i b:
 // Re(u/2 - u^2/12 + u^3/45) = a(1/2 - a/12 + a^2/45) + b^2 (1/12 - a/15)
+// with q = x y (b = 2q):  = a(1/2 - a/12 + a^2/45) + q^2 (1/3 - 4a/15)
 template <typename T>
 __device__ inline T logcosh0_re(cplx<T> z) {
     const T a = z.re * z.re - z.im * z.im;
-    const T h = z.re * z.im;            // b = 2h
-    const T b2 = T(4) * h * h;
+    const T q = z.re * z.im;
+    const T q2 = q * q;
     T p = a * T(1.0 / 45.0) - T(1.0 / 12.0);
     p = p * a + T(0.5);
-    return p * a + b2 * (T(1.0 / 12.0) - a * T(1.0 / 15.0));
+    return p * a + q2 * (T(1.0 / 3.0) - a * T(4.0 / 15.0));
 }
 
 // Sum_j Re(f_j lc(theta_j)) over this lane's units
@@ -88,7 +89,7 @@ __device__ inline cplx<T> lane_sum(const cplx<T> (&th)[U], const cplx<T> (&f)[U]
 
 template <typename T>
 __device__ inline cplx<T> wave_csum(cplx<T> v) {
-    return {uniform(wave_sum(v.re)), uniform(wave_sum(v.im))};
+    return {wave_sum(v.re), wave_sum(v.im)};
 }
 
 // ----------------------------------------------------------------- kernels
@@ -192,12 +193,39 @@ __global__ __launch_bounds__(kBlock) void net_log_psi_kernel(DenseView<T> net, c
 }
 
 // Metropolis sweeps (MonteCarlo.hpp:133-177) with single flips or S_z-conserving
-// exchanges.  The proposal for chain c at global proposal index t uses
-// Philox((c, t_lo, t_hi, 0), seed): x -> site i = x % N, y -> partner rank,
-// z -> acceptance uniform (z + 0.5) 2^-32.
-template <typename T, int U, int MOVE, bool REALF>
+// exchanges.  Proposal t of chain c uses Philox((c, t_lo, t_hi, 0), seed):
+// x -> site i = range32(x, N); y -> rank of the partner among the sites of
+// opposite spin (exchange); z -> acceptance uniform (z + 0.5) 2^-32.
+//
+// Scalar-unit budget: the scalar ALU is shared by the four SIMDs of a CU, so the
+// per-proposal scalar work is kept small: the Philox draws of 64 consecutive
+// proposals are computed at once on the vector ALU (lane l draws proposal
+// t0 + l) and read back with v_readlane; the partner is selected with a
+// lane-parallel prefix count (mbcnt) and a ballot.  The configuration words live
+// in scalar registers; the row of Wd for the next proposal's site is loaded while
+// the current one is evaluated.
+template <typename T, int U>
+__device__ inline void load_row(cplx<T> (&w)[U], const cplx<T>* Wd, int row, int Mpad, int lane) {
+    const cplx<T>* p = Wd + int64_t(row) * Mpad + lane;
+#pragma unroll
+    for (int u = 0; u < U; u++) w[u] = p[64 * u];
+}
+
+// position of the rank-th set bit of the wave-uniform mask o (rank < popcount(o))
+__device__ inline int select64_wave(uint64_t o, uint32_t rank, int lane) {
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(o >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(o), 0u));
+    const bool hit = ((o >> lane) & 1u) && below == rank;
+    const uint64_t m = __ballot(hit);
+    return __builtin_ctzll(m);
+}
+
+__device__ inline uint32_t readlane_u32(uint32_t v, int l) {
+    return uint32_t(__builtin_amdgcn_readlane(int(v), l));
+}
+
+template <typename T, int U, int MOVE, bool REALF, int WORDS>
 __global__ __launch_bounds__(kBlock) void mc_sweep_kernel(DenseView<T> net, uint64_t* chain_confs,
-                                                          cplx<T>* theta, int num_chains, int words,
+                                                          cplx<T>* theta, int num_chains,
                                                           int64_t num_proposals, uint64_t counter,
                                                           uint32_t key0, uint32_t key1,
                                                           uint32_t* accepted) {
@@ -205,71 +233,134 @@ __global__ __launch_bounds__(kBlock) void mc_sweep_kernel(DenseView<T> net, uint
     const int c = blockIdx.x * kWavesPerBlock + uniform_int(threadIdx.x >> 6);
     if (c >= num_chains) return;
     const int N = net.N;
-    Conf conf = load_conf(chain_confs, c, words);
+    uint64_t w[WORDS], msk[WORDS];
+#pragma unroll
+    for (int k = 0; k < WORDS; k++) {
+        w[k] = uniform_u64(chain_confs[int64_t(c) * WORDS + k]);
+        const int bits = N - 64 * k;
+        msk[k] = bits >= 64 ? ~uint64_t(0) : bits > 0 ? (uint64_t(1) << bits) - 1 : 0;
+    }
     cplx<T> th[U], f[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
         th[u] = theta[int64_t(c) * net.Mpad + lane + 64 * u];
         f[u] = net.f[lane + 64 * u];
     }
-    T cur = uniform(wave_sum(lane_re_sum<T, U, REALF>(th, f)));
-    uint64_t last_mask = N % 64 ? (uint64_t(1) << (N % 64)) - 1 : ~uint64_t(0);
+    T cur = wave_sum(lane_re_sum<T, U, REALF>(th, f));
     uint32_t acc = 0;
-    for (int64_t q = 0; q < num_proposals; q++) {
-        const uint64_t t = counter + uint64_t(q);
-        const u32x4 r = philox(uint32_t(c), uint32_t(t), uint32_t(t >> 32), 0u, key0, key1);
-        const int i = int(r.x % uint32_t(N));
-        const int si_bit = spin_bit(conf, i);
-        int k = -1;
+    // random words of proposals [base, base + 64), one proposal per lane
+    uint32_t rx, ry, rz;
+    auto draw_batch = [&](uint64_t base) {
+        const uint64_t t = base + uint64_t(lane);
+        const u32x4 rr = philox(uint32_t(c), uint32_t(t), uint32_t(t >> 32), 0u, key0, key1);
+        rx = rr.x;
+        ry = rr.y;
+        rz = rr.z;
+    };
+    draw_batch(counter);
+    int i = int(range32(readlane_u32(rx, 0), uint32_t(N)));
+    cplx<T> wa[U], wb[U];
+    load_row<T, U>(wa, net.Wd, i, net.Mpad, lane);
+    // one proposal: wi holds the row of site i, the row of the next site is loaded into wn
+    auto proposal = [&](int64_t q, cplx<T> (&wi)[U], cplx<T> (&wn)[U]) {
+        const int ql = int(q & 63);
+        if (ql == 0 && q > 0) draw_batch(counter + uint64_t(q));
+        const uint32_t r_y = readlane_u32(ry, ql);
+        const uint32_t r_z = readlane_u32(rz, ql);
+        uint64_t wsel = w[0];
+#pragma unroll
+        for (int k = 1; k < WORDS; k++) wsel = (i >> 6) == k ? w[k] : wsel;
+        const int si_bit = int((wsel >> (i & 63)) & 1u);
+        int kp = -1;
+        cplx<T> wk[U];
         if constexpr (MOVE == MOVE_EXCHANGE) {
-            uint64_t opp[kMaxWords];
+            uint64_t o[WORDS];
+            uint32_t cnt[WORDS];
             uint32_t n_opp = 0;
 #pragma unroll
-            for (int w = 0; w < kMaxWords; w++) {
-                uint64_t m = si_bit ? ~conf.w[w] : conf.w[w];
-                m = w < words ? m : 0;
-                m = w == words - 1 ? (m & last_mask) : m;
-                opp[w] = m;
-                n_opp += uint32_t(__popcll(m));
+            for (int k = 0; k < WORDS; k++) {
+                o[k] = (si_bit ? ~w[k] : w[k]) & msk[k];
+                cnt[k] = uint32_t(__popcll(o[k]));
+                n_opp += cnt[k];
             }
-            if (n_opp == 0) continue;
-            k = select_bit(opp, words, r.y % n_opp);
-        }
-        const T ci = si_bit ? T(-2) : T(2);   // theta' - theta = -2 s_i W_i
-        cplx<T> tn[U];
-        const cplx<T>* wi = net.Wd + int64_t(i) * net.Mpad + lane;
+            if (n_opp > 0) {
+                uint32_t rank = range32(r_y, n_opp);
+                uint64_t osel = o[0];
+                int word = 0;
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const cplx<T> w = wi[64 * u];
-            tn[u].re = th[u].re + ci * w.re;
-            tn[u].im = th[u].im + ci * w.im;
+                for (int k = 0; k < WORDS - 1; k++) {
+                    if (word == k && rank >= cnt[k]) {
+                        rank -= cnt[k];
+                        word = k + 1;
+                        osel = o[k + 1];
+                    }
+                }
+                kp = 64 * word + select64_wave(osel, rank, lane);
+                load_row<T, U>(wk, net.Wd, kp, net.Mpad, lane);
+            }
         }
-        if constexpr (MOVE == MOVE_EXCHANGE) {
-            const T ck = -ci;              // s_k = -s_i
-            const cplx<T>* wk = net.Wd + int64_t(k) * net.Mpad + lane;
+        // next proposal's site and the prefetch of its row (always a valid row)
+        const int qn = int((q + 1) & 63);
+        uint32_t xn;
+        if (qn != 0) {
+            xn = readlane_u32(rx, qn);
+        } else {
+            const uint64_t tn = counter + uint64_t(q + 1);
+            xn = philox(uint32_t(c), uint32_t(tn), uint32_t(tn >> 32), 0u, key0, key1).x;
+        }
+        const int i_next = int(range32(xn, uint32_t(N)));
+        load_row<T, U>(wn, net.Wd, i_next, net.Mpad, lane);
+        if (MOVE != MOVE_EXCHANGE || kp >= 0) {
+            // theta' = theta - 2 s_i W_i (+ 2 s_i W_k for the exchange partner, s_k = -s_i)
+            const T ci = si_bit ? T(-2) : T(2);
+            cplx<T> z[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const cplx<T> w = wk[64 * u];
-                tn[u].re += ck * w.re;
-                tn[u].im += ck * w.im;
+                if constexpr (MOVE == MOVE_EXCHANGE) {
+                    z[u].re = th[u].re + ci * (wi[u].re - wk[u].re);
+                    z[u].im = th[u].im + ci * (wi[u].im - wk[u].im);
+                } else {
+                    z[u].re = th[u].re + ci * wi[u].re;
+                    z[u].im = th[u].im + ci * wi[u].im;
+                }
             }
-        }
-        const T nxt = uniform(wave_sum(lane_re_sum<T, U, REALF>(tn, f)));
-        const T d2 = T(2) * (nxt - cur);
-        const T uu = (T(r.z) + T(0.5)) * T(2.3283064365386963e-10);
-        if (d2 > T(0) || uu <= dexp(d2)) {
+            T part = 0;
 #pragma unroll
-            for (int u = 0; u < U; u++) th[u] = tn[u];
-            cur = nxt;
-            flip_bit(conf, i);
-            if (k >= 0) flip_bit(conf, k);
-            acc++;
+            for (int u = 0; u < U; u++) {
+                if constexpr (REALF) {
+                    part += f[u].re * logcosh0_re(z[u]);
+                } else {
+                    const cplx<T> l = logcosh0(z[u]);
+                    part += f[u].re * l.re - f[u].im * l.im;
+                }
+            }
+            const T nxt = wave_sum(part);
+            const T d2 = T(2) * (nxt - cur);
+            const T uu = (T(r_z) + T(0.5)) * T(2.3283064365386963e-10);
+            // branch-free update of the angles (the decision is wave-uniform)
+            const bool accept = d2 > T(0) || uu <= dexp(d2);
+#pragma unroll
+            for (int u = 0; u < U; u++) th[u] = accept ? z[u] : th[u];
+            cur = accept ? nxt : cur;
+#pragma unroll
+            for (int k = 0; k < WORDS; k++) {
+                uint64_t flip = (i >> 6) == k ? uint64_t(1) << (i & 63) : 0;
+                if (kp >= 0 && (kp >> 6) == k) flip |= uint64_t(1) << (kp & 63);
+                w[k] ^= accept ? flip : 0;
+            }
+            acc += accept ? 1u : 0u;
         }
+        i = i_next;
+    };
+    for (int64_t q = 0; q < num_proposals; q += 2) {
+        proposal(q, wa, wb);
+        if (q + 1 < num_proposals) proposal(q + 1, wb, wa);
     }
 #pragma unroll
     for (int u = 0; u < U; u++) theta[int64_t(c) * net.Mpad + lane + 64 * u] = th[u];
     if (lane == 0) {
-        for (int w = 0; w < words; w++) chain_confs[int64_t(c) * words + w] = conf.w[w];
+#pragma unroll
+        for (int k = 0; k < WORDS; k++) chain_confs[int64_t(c) * WORDS + k] = w[k];
         accepted[c] += acc;
     }
 }
@@ -383,84 +474,120 @@ __global__ void materialize_ok_kernel(DenseView<T> net, const uint64_t* confs, i
     O[s * net.num_params + k] = o;
 }
 
-// Partial sums over a chunk of samples: P[chunk][v][j] = sum_s coef_s spin_sv X_sj
-// for v < N, row N holds sum_s coef_s X_sj.  Each wavefront owns 16 rows v.
-constexpr int kSwsRows = 16;
-constexpr int kSwsMaxChunks = 4096;
+// Y[v][c] = sum_s spin_sv X_sc on fp64 MFMA (v_mfma_f64_16x16x4_f64), the
+// transpose-GEMM behind <O>, F, the gradient and the O^H u half of S.v for a
+// dense net.  Rows v < N are spins (+-1, exact), row N is all ones (bias
+// parameters).  Columns: X_s,2j / 2j+1 = re / im of coef_s * g_sj (or
+// conj g_sj), columns 2 Mpad / 2 Mpad+1 = re / im of coef_s (visible
+// parameters).  One wavefront owns one 16-column tile for up to 8 row tiles and a
+// chunk of samples; lane l feeds sample 4t + l/16 of step t (A: row l%16, B:
+// column l%16, both at k = l/16).  Partial tiles per
+// chunk are summed by sum_partials_kernel in a fixed order (deterministic).
+typedef double double4_t __attribute__((ext_vector_type(4)));
+constexpr int kRowTilesPerWave = 8;
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void spin_weighted_kernel(DenseView<T> net, const uint64_t* confs,
-                                                               int words, const cplx<T>* g, int conj_g,
-                                                               const cplx<double>* coef, int64_t count,
-                                                               int64_t chunk, cplx<double>* partial) {
-    const int lane = threadIdx.x & 63;
-    const int wave = uniform_int(threadIdx.x >> 6);
-    const int j = blockIdx.x * 64 + lane;
-    const int v0 = (blockIdx.z * kWavesPerBlock + wave) * kSwsRows;
+__global__ __launch_bounds__(64) void spin_gemm_kernel(DenseView<T> net, const uint64_t* confs, int words,
+                                                       const cplx<T>* g, int conj_g, const cplx<double>* coef,
+                                                       int64_t count, int64_t chunk, int vpad, int ncols,
+                                                       double* partial) {
+    const int lane = threadIdx.x;
+    const int col = blockIdx.x * 16 + (lane & 15);
+    const int kk = lane >> 4;
+    const int vt0 = blockIdx.z * kRowTilesPerWave;
+    const int vtiles = vpad / 16;
     const int64_t s0 = int64_t(blockIdx.y) * chunk;
     const int64_t s1 = s0 + chunk < count ? s0 + chunk : count;
-    if (v0 > net.N) return;
-    double ar[kSwsRows], ai[kSwsRows];
+    const int two_m = 2 * net.Mpad;
+    const bool is_g = col < two_m;
+    const bool is_coef = col >= two_m && col < two_m + 2;
+    const int j = col >> 1;
+    const bool imag_part = col & 1;
+    double4_t acc[kRowTilesPerWave];
 #pragma unroll
-    for (int r = 0; r < kSwsRows; r++) ar[r] = ai[r] = 0.0;
-    for (int64_t s = s0; s < s1; s++) {
-        const Conf c = load_conf(confs, s, words);
-        const cplx<double> cf = coef[s];
-        const cplx<T> gv = g[s * net.Mpad + j];
-        const double gr = double(gv.re), gi = conj_g ? -double(gv.im) : double(gv.im);
-        const double xr = cf.re * gr - cf.im * gi;
-        const double xi = cf.re * gi + cf.im * gr;
+    for (int t = 0; t < kRowTilesPerWave; t++) acc[t] = {0.0, 0.0, 0.0, 0.0};
+    constexpr int kStage = 4;   // MFMA steps whose operands are loaded together
+    for (int64_t sb = s0; sb < s1; sb += 4 * kStage) {
+        double x[kStage];
+        uint64_t cw[kStage][kMaxWords];
 #pragma unroll
-        for (int r = 0; r < kSwsRows; r++) {
-            const int v = v0 + r;
-            const double sv = v < net.N ? (spin_bit(c, v) ? 1.0 : -1.0) : 1.0;
-            ar[r] += sv * xr;
-            ai[r] += sv * xi;
+        for (int st = 0; st < kStage; st++) {
+            const int64_t s = sb + 4 * st + kk;
+            x[st] = 0.0;
+#pragma unroll
+            for (int w = 0; w < kMaxWords; w++) cw[st][w] = 0;
+            if (s < s1) {
+                const cplx<double> cf = coef[s];
+                if (is_g) {
+                    const cplx<T> gv = g[s * net.Mpad + j];
+                    const double gr = double(gv.re), gi = conj_g ? -double(gv.im) : double(gv.im);
+                    if (conj_g == 2)   // coef_s |g_sj|^2 (diagonal of S)
+                        x[st] = imag_part ? 0.0 : cf.re * (gr * gr + gi * gi);
+                    else
+                        x[st] = imag_part ? cf.re * gi + cf.im * gr : cf.re * gr - cf.im * gi;
+                } else if (is_coef) {
+                    x[st] = imag_part ? cf.im : cf.re;
+                }
+#pragma unroll
+                for (int w = 0; w < kMaxWords; w++) cw[st][w] = w < words ? confs[s * words + w] : 0;
+            }
+        }
+#pragma unroll
+        for (int st = 0; st < kStage; st++) {
+#pragma unroll
+            for (int t = 0; t < kRowTilesPerWave; t++) {
+                if (vt0 + t < vtiles) {
+                    const int v = (vt0 + t) * 16 + (lane & 15);
+                    double a = 0.0;
+                    if (v < net.N) {
+                        const uint64_t word = v < 64 ? cw[st][0] : v < 128 ? cw[st][1] : v < 192 ? cw[st][2] : cw[st][3];
+                        a = ((word >> (v & 63)) & 1) ? 1.0 : -1.0;
+                    } else if (v == net.N) {
+                        a = 1.0;
+                    }
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, x[st], acc[t], 0, 0, 0);
+                }
+            }
         }
     }
-    const int rows = net.N + 1;
 #pragma unroll
-    for (int r = 0; r < kSwsRows; r++) {
-        const int v = v0 + r;
-        if (v < rows) partial[(int64_t(blockIdx.y) * rows + v) * net.Mpad + j] = {ar[r], ai[r]};
+    for (int t = 0; t < kRowTilesPerWave; t++) {
+        if (vt0 + t < vtiles) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                // f64 16x16x4 result layout: item r of lane l is row l/16 + 4 r, column l%16
+                const int v = (vt0 + t) * 16 + (lane >> 4) + 4 * r;
+                partial[(int64_t(blockIdx.y) * vpad + v) * ncols + col] = acc[t][r];
+            }
+        }
     }
 }
 
-__global__ void sum_chunks_kernel(const cplx<double>* partial, int chunks, int64_t n, cplx<double>* out) {
+__global__ void sum_partials_kernel(const double* partial, int chunks, int64_t n, double* out) {
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    double re = 0, im = 0;
-    for (int c = 0; c < chunks; c++) {
-        re += partial[c * n + e].re;
-        im += partial[c * n + e].im;
-    }
-    out[e] = {re, im};
+    double acc = 0.0;
+    for (int c = 0; c < chunks; c++) acc += partial[c * n + e];
+    out[e] = acc;
 }
 
-// Ya[v] = sum_s coef_s spin_sv
-__global__ __launch_bounds__(kBlock) void spin_coef_sums_kernel(const uint64_t* confs, int words, int N,
-                                                                const cplx<double>* coef, int64_t count,
-                                                                cplx<double>* Ya) {
-    const int v = blockIdx.x;
-    __shared__ double sr[kBlock], si[kBlock];
-    double re = 0, im = 0;
-    for (int64_t s = threadIdx.x; s < count; s += blockDim.x) {
-        const uint64_t wv = confs[s * words + (v >> 6)];
-        const double sv = ((wv >> (v & 63)) & 1) ? 1.0 : -1.0;
-        re += sv * coef[s].re;
-        im += sv * coef[s].im;
-    }
-    sr[threadIdx.x] = re;
-    si[threadIdx.x] = im;
-    __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
-        if (int(threadIdx.x) < o) {
-            sr[threadIdx.x] += sr[threadIdx.x + o];
-            si[threadIdx.x] += si[threadIdx.x + o];
+// Yw[v][j], Yb[j], Ya[v] from the GEMM result Y[vpad][ncols]
+__global__ void split_spin_gemm_kernel(const double* Y, int N, int Mpad, int ncols, cplx<double>* Yw,
+                                       cplx<double>* Yb, cplx<double>* Ya) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t nw = int64_t(N + 1) * Mpad;
+    if (e < nw) {
+        const int v = int(e / Mpad), j = int(e % Mpad);
+        const cplx<double> y = {Y[int64_t(v) * ncols + 2 * j], Y[int64_t(v) * ncols + 2 * j + 1]};
+        if (v < N) {
+            if (Yw) Yw[e] = y;
+        } else if (Yb) {
+            Yb[j] = y;
         }
-        __syncthreads();
+    } else if (e < nw + N && Ya) {
+        const int v = int(e - nw);
+        Ya[v] = {Y[int64_t(v) * ncols + 2 * Mpad], Y[int64_t(v) * ncols + 2 * Mpad + 1]};
     }
-    if (threadIdx.x == 0) Ya[v] = {sr[0], si[0]};
 }
 
 // u_s = sum_i s_i va_i + sum_j g_sj (vb_j + SV_sj)
@@ -597,6 +724,28 @@ void launch_net_log_psi(const DenseView<T>& net, const cplx<T>* theta, int64_t c
     ANN_LAUNCH_CHECK();
 }
 
+template <typename T, int U, int WORDS>
+void launch_sweep_words(const DenseView<T>& net, uint64_t* chain_confs, cplx<T>* theta, int num_chains, int move,
+                        int64_t num_proposals, uint64_t counter, uint32_t k0, uint32_t k1, uint32_t* accepted,
+                        hipStream_t stream) {
+    const unsigned grid = grid_waves(num_chains);
+    if (move == MOVE_EXCHANGE) {
+        if (net.real_f)
+            mc_sweep_kernel<T, U, MOVE_EXCHANGE, true, WORDS><<<grid, kBlock, 0, stream>>>(
+                net, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted);
+        else
+            mc_sweep_kernel<T, U, MOVE_EXCHANGE, false, WORDS><<<grid, kBlock, 0, stream>>>(
+                net, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted);
+    } else {
+        if (net.real_f)
+            mc_sweep_kernel<T, U, MOVE_FLIP, true, WORDS><<<grid, kBlock, 0, stream>>>(
+                net, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted);
+        else
+            mc_sweep_kernel<T, U, MOVE_FLIP, false, WORDS><<<grid, kBlock, 0, stream>>>(
+                net, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted);
+    }
+}
+
 template <typename T>
 void launch_mc_sweep(const DenseView<T>& net, uint64_t* chain_confs, cplx<T>* theta, int num_chains,
                      int words, int move, int64_t num_proposals, uint64_t counter, uint64_t seed,
@@ -605,21 +754,12 @@ void launch_mc_sweep(const DenseView<T>& net, uint64_t* chain_confs, cplx<T>* th
     const uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
     dispatch_u(net.Mpad / 64, [&](auto tag) {
         constexpr int U = std::decay_t<decltype(tag)>::value;
-        const unsigned grid = grid_waves(num_chains);
-        if (move == MOVE_EXCHANGE) {
-            if (net.real_f)
-                mc_sweep_kernel<T, U, MOVE_EXCHANGE, true><<<grid, kBlock, 0, stream>>>(
-                    net, chain_confs, theta, num_chains, words, num_proposals, counter, k0, k1, accepted);
-            else
-                mc_sweep_kernel<T, U, MOVE_EXCHANGE, false><<<grid, kBlock, 0, stream>>>(
-                    net, chain_confs, theta, num_chains, words, num_proposals, counter, k0, k1, accepted);
-        } else {
-            if (net.real_f)
-                mc_sweep_kernel<T, U, MOVE_FLIP, true><<<grid, kBlock, 0, stream>>>(
-                    net, chain_confs, theta, num_chains, words, num_proposals, counter, k0, k1, accepted);
-            else
-                mc_sweep_kernel<T, U, MOVE_FLIP, false><<<grid, kBlock, 0, stream>>>(
-                    net, chain_confs, theta, num_chains, words, num_proposals, counter, k0, k1, accepted);
+        switch (words) {
+            case 1: launch_sweep_words<T, U, 1>(net, chain_confs, theta, num_chains, move, num_proposals, counter, k0, k1, accepted, stream); break;
+            case 2: launch_sweep_words<T, U, 2>(net, chain_confs, theta, num_chains, move, num_proposals, counter, k0, k1, accepted, stream); break;
+            case 3: launch_sweep_words<T, U, 3>(net, chain_confs, theta, num_chains, move, num_proposals, counter, k0, k1, accepted, stream); break;
+            case 4: launch_sweep_words<T, U, 4>(net, chain_confs, theta, num_chains, move, num_proposals, counter, k0, k1, accepted, stream); break;
+            default: throw HipError("configurations of 1..4 words are supported");
         }
     });
     ANN_LAUNCH_CHECK();
@@ -661,35 +801,29 @@ void launch_spin_weighted_sums(const DenseView<T>& net, const uint64_t* confs, i
                                int conj_g, const cplx<double>* coef, int64_t count, cplx<double>* Yw,
                                cplx<double>* Yb, cplx<double>* Ya, hipStream_t stream) {
     if (count <= 0) return;
-    int64_t chunk = 256;
-    while ((count + chunk - 1) / chunk > kSwsMaxChunks) chunk *= 2;
-    const int chunks = int((count + chunk - 1) / chunk);
-    const int rows = net.N + 1;
-    const int64_t n = int64_t(rows) * net.Mpad;
-    cplx<double>* partial = nullptr;
-    ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(cplx<double>) * n * chunks, stream));
-    dim3 grid(unsigned(net.Mpad / 64), unsigned(chunks),
-              unsigned((rows + kSwsRows * kWavesPerBlock - 1) / (kSwsRows * kWavesPerBlock)));
-    spin_weighted_kernel<T><<<grid, kBlock, 0, stream>>>(net, confs, words, g, conj_g, coef, count, chunk,
-                                                         partial);
+    const int vpad = (net.N + 1 + 15) / 16 * 16;
+    const int ncols = 2 * net.Mpad + 16;
+    const int ctiles = ncols / 16;
+    const int zgroups = (vpad / 16 + kRowTilesPerWave - 1) / kRowTilesPerWave;
+    int64_t chunks = std::max<int64_t>(1, 4096 / (int64_t(ctiles) * zgroups));
+    chunks = std::min<int64_t>(chunks, (count + 63) / 64);
+    int64_t chunk = (count + chunks - 1) / chunks;
+    chunk = (chunk + 3) / 4 * 4;
+    chunks = (count + chunk - 1) / chunk;
+    const int64_t n = int64_t(vpad) * ncols;
+    double* partial = nullptr;
+    ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(double) * n * (chunks + 1), stream));
+    double* full = partial + n * chunks;
+    const dim3 grid{unsigned(ctiles), unsigned(chunks), unsigned(zgroups)};
+    spin_gemm_kernel<T><<<grid, 64, 0, stream>>>(net, confs, words, g, conj_g, coef, count, chunk, vpad, ncols,
+                                                 partial);
     ANN_LAUNCH_CHECK();
-    cplx<double>* full = nullptr;
-    ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&full), sizeof(cplx<double>) * n, stream));
-    sum_chunks_kernel<<<unsigned((n + 255) / 256), 256, 0, stream>>>(partial, chunks, n, full);
+    sum_partials_kernel<<<unsigned((n + 255) / 256), 256, 0, stream>>>(partial, int(chunks), n, full);
     ANN_LAUNCH_CHECK();
-    // rows 0..N-1 -> Yw, row N -> Yb
-    if (Yw)
-        ANN_HIP_CHECK(hipMemcpyAsync(Yw, full, sizeof(cplx<double>) * net.N * net.Mpad, hipMemcpyDeviceToDevice,
-                                     stream));
-    if (Yb)
-        ANN_HIP_CHECK(hipMemcpyAsync(Yb, full + int64_t(net.N) * net.Mpad, sizeof(cplx<double>) * net.Mpad,
-                                     hipMemcpyDeviceToDevice, stream));
-    if (Ya) {
-        spin_coef_sums_kernel<<<unsigned(net.N), kBlock, 0, stream>>>(confs, words, net.N, coef, count, Ya);
-        ANN_LAUNCH_CHECK();
-    }
+    const int64_t ns = int64_t(net.N + 1) * net.Mpad + net.N;
+    split_spin_gemm_kernel<<<unsigned((ns + 255) / 256), 256, 0, stream>>>(full, net.N, net.Mpad, ncols, Yw, Yb, Ya);
+    ANN_LAUNCH_CHECK();
     ANN_HIP_CHECK(hipFreeAsync(partial, stream));
-    ANN_HIP_CHECK(hipFreeAsync(full, stream));
 }
 
 template <typename T>

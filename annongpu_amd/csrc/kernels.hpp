@@ -85,8 +85,9 @@ void launch_materialize_ok(const DenseView<T>& net, const uint64_t* confs, int w
                            const cplx<T>* g, int64_t count, const cplx<double>* center,
                            cplx<T>* O, hipStream_t stream);
 
-// Y[v][j] += sum_s coef_s * spin_sv * X_sj  (and Yb[j] = sum_s coef_s X_sj,
-// Ya[v] = sum_s coef_s spin_sv), fp64 accumulation.  X_sj = g_sj or conj(g_sj).
+// Yw[v][j] = sum_s coef_s spin_sv X_sj, Yb[j] = sum_s coef_s X_sj,
+// Ya[v] = sum_s coef_s spin_sv, fp64 accumulation (fp64 MFMA).
+// X_sj = g_sj (conj_g = 0), conj(g_sj) (1) or |g_sj|^2 (2).
 template <typename T>
 void launch_spin_weighted_sums(const DenseView<T>& net, const uint64_t* confs, int words,
                                const cplx<T>* g, int conj_g, const cplx<double>* coef,

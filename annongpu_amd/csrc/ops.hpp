@@ -25,6 +25,10 @@ void launch_center_scale(cplx<double>* u, const double* w, const double* c2, int
 void launch_cg_step(cplx<double>* x, cplx<double>* r, const cplx<double>* p, const cplx<double>* Ap, double alpha,
                     int64_t n, hipStream_t stream);
 void launch_cg_dir(cplx<double>* p, const cplx<double>* r, double beta, int64_t n, hipStream_t stream);
+void launch_precond(cplx<double>* z, const cplx<double>* r, const double* diag, double shift, int64_t n,
+                    hipStream_t stream);
+void launch_sr_diag(int layout, int N, int M, int Mpad, double wsum, const cplx<double>* Dj,
+                    const cplx<double>* Obar, int P, double* diag, hipStream_t stream);
 void launch_rank1(cplx<double>* S, const cplx<double>* o, int P, double alpha, hipStream_t stream);
 void launch_add_diag(cplx<double>* S, int P, double shift, hipStream_t stream);
 template <typename Tin, typename Tout>

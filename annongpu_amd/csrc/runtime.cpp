@@ -21,6 +21,12 @@ hipStream_t Runtime::stream() {
     if (!stream_) {
         ANN_HIP_CHECK(hipSetDevice(device_));
         ANN_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        // keep stream-ordered allocations cached in the default pool
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device_) == hipSuccess) {
+            uint64_t threshold = ~uint64_t(0);
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+        }
     }
     return stream_;
 }
@@ -287,9 +293,9 @@ const DeviceOp& Operator::device(int words) {
     }
     std::vector<uint64_t> flip, zy;
     std::vector<int> begin;
-    std::vector<double> coef;
+    std::vector<double> scoef;   // per string: coefficient * (-i)^{#Y}, interleaved
     for (size_t g = 0; g < groups.size(); g++) {
-        begin.push_back(int(coef.size() / 2));
+        begin.push_back(int(scoef.size() / 2));
         flip.insert(flip.end(), masks[g].begin(), masks[g].end());
         for (int n : groups[g]) {
             int ny = 0;
@@ -300,17 +306,17 @@ const DeviceOp& Operator::device(int words) {
             if ((ny & 3) > 1) ph = -ph;
             if (ny & 1) ph *= cd(0.0, -1.0);
             const cd c = coef[n] * ph;
-            coef.push_back(c.real());
-            coef.push_back(c.imag());
+            scoef.push_back(c.real());
+            scoef.push_back(c.imag());
             for (int w = 0; w < words; w++) zy.push_back(b[size_t(n) * kMaxWords + w]);
         }
     }
-    begin.push_back(int(coef.size() / 2));
+    begin.push_back(int(scoef.size() / 2));
     d->num_groups = int(groups.size());
     d->group_flip.upload(flip.data(), flip.size());
     d->group_begin.upload(begin.data(), begin.size());
     d->string_zy.upload(zy.data(), zy.size());
-    d->string_coef.upload(coef.data(), coef.size());
+    d->string_coef.upload(scoef.data(), scoef.size());
     auto& ref = *d;
     compiled[words] = std::move(d);
     return ref;
@@ -473,20 +479,17 @@ struct Prec<float> {
 
 // weighted sums of the structured O, packed in parameter order: out = sum_s coef_s O_s (or conj O_s)
 template <typename T>
-void structured_sums(const DenseView<T>& net, const uint64_t* confs, int words, const cplx<T>* g, int conj_g,
-                     const cplx<double>* coef, int64_t count, cplx<double>* out, double shift,
+void structured_sums(Tdvp& t, const DenseView<T>& net, const uint64_t* confs, int words, const cplx<T>* g,
+                     int conj_g, const cplx<double>* coef, int64_t count, cplx<double>* out, double shift,
                      const cplx<double>* v) {
-    DevBuf<cplx<double>> Yw, Yb, Ya;
-    Yw.ensure(size_t(net.N) * net.Mpad);
-    Yb.ensure(net.Mpad);
-    Ya.ensure(net.N);
-    {
-        ScopedTimer timer("spin_sums");
-        launch_spin_weighted_sums<T>(net, confs, words, g, conj_g, coef, count, Yw.p, Yb.p, Ya.p, S());
-        launch_pack_y(net.layout, net.N, net.M, net.Mpad, net.C, net.conn, Ya.p, Yb.p, Yw.p, v, shift,
+    t.Yw.ensure(size_t(net.N) * net.Mpad);
+    t.Yb.ensure(net.Mpad);
+    t.Ya.ensure(net.N);
+    ScopedTimer timer("spin_sums");
+    launch_spin_weighted_sums<T>(net, confs, words, g, conj_g, coef, count, t.Yw.p, t.Yb.p, t.Ya.p, S());
+    if (out)
+        launch_pack_y(net.layout, net.N, net.M, net.Mpad, net.C, net.conn, t.Ya.p, t.Yb.p, t.Yw.p, v, shift,
                       net.num_params, out, S());
-    }
-    Runtime::get().sync();
 }
 
 template <typename T>
@@ -517,9 +520,14 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     t.Obar.ensure(P);
     t.Fd.ensure(P);
     launch_weight_coef(s.w.p, nullptr, n, coef.p, S());
-    structured_sums<T>(net, s.confs.p, s.words, g.p, 0, coef.p, n, t.Obar.p, 0.0, nullptr);
+    structured_sums<T>(t, net, s.confs.p, s.words, g.p, 0, coef.p, n, t.Obar.p, 0.0, nullptr);
+    // diagonal of S for the Jacobi preconditioner: sum_s w |g_sj|^2 - |<O_k>|^2
+    structured_sums<T>(t, net, s.confs.p, s.words, g.p, 2, coef.p, n, nullptr, 0.0, nullptr);
+    t.diag.ensure(P);
+    launch_sr_diag(net.layout, net.N, net.M, net.Mpad, t.wsum, t.Yb.p, t.Obar.p, P, t.diag.p, S());
     launch_weight_coef(s.w.p, s.e_loc.p, n, coef.p, S());
-    structured_sums<T>(net, s.confs.p, s.words, g.p, 1, coef.p, n, t.Fd.p, 0.0, nullptr);
+    structured_sums<T>(t, net, s.confs.p, s.words, g.p, 1, coef.p, n, t.Fd.p, 0.0, nullptr);
+    Runtime::get().sync();
     t.O.resize(P);
     t.F.resize(P);
     t.Obar.download(reinterpret_cast<cplx<double>*>(t.O.data()), P);
@@ -565,6 +573,21 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
 }
 
 template <typename T>
+struct Ws;
+template <>
+struct Ws<float> {
+    static DevBuf<cplx<float>>& V(Tdvp& t) { return t.V32; }
+    static DevBuf<cplx<float>>& SV(Tdvp& t) { return t.SV32; }
+    static DevBuf<cplx<float>>& zero(Tdvp& t) { return t.zero32; }
+};
+template <>
+struct Ws<double> {
+    static DevBuf<cplx<double>>& V(Tdvp& t) { return t.V64; }
+    static DevBuf<cplx<double>>& SV(Tdvp& t) { return t.SV64; }
+    static DevBuf<cplx<double>>& zero(Tdvp& t) { return t.zero64; }
+};
+
+template <typename T>
 void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y) {
     const int P = int(t.P);
     DenseView<T> net{};
@@ -575,40 +598,39 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y)
     net.layout = t.layout;
     net.num_params = P;
     net.conn = t.layout == LAYOUT_DEEP1 ? t.conn.p : nullptr;
-    DevBuf<cplx<double>> va, vb, u, coef, c;
-    DevBuf<cplx<T>> V, SV, zero;
-    va.ensure(t.N);
-    vb.ensure(t.Mpad);
+    auto& V = Ws<T>::V(t);
+    auto& SV = Ws<T>::SV(t);
+    auto& zero = Ws<T>::zero(t);
+    t.va.ensure(t.N);
+    t.vb.ensure(t.Mpad);
+    t.u.ensure(t.count);
+    t.c1.ensure(1);
     V.ensure(size_t(t.N) * t.Mpad);
     SV.ensure(size_t(t.count) * t.Mpad);
-    u.ensure(t.count);
-    c.ensure(1);
-    zero.ensure(t.Mpad);
-    ANN_HIP_CHECK(hipMemsetAsync(zero.p, 0, sizeof(cplx<T>) * t.Mpad, S()));
+    if (zero.n < size_t(t.Mpad)) {
+        zero.ensure(t.Mpad);
+        ANN_HIP_CHECK(hipMemsetAsync(zero.p, 0, sizeof(cplx<T>) * t.Mpad, S()));
+    }
     auto& g = Prec<T>::g(t);
     {
         ScopedTimer timer("sr_matvec");
-        launch_unpack_v<T>(net, v, va.p, vb.p, V.p, S());
+        launch_unpack_v<T>(net, v, t.va.p, t.vb.p, V.p, S());
         DenseView<T> vnet = net;
         vnet.Wd = V.p;
         vnet.b = zero.p;
         if constexpr (sizeof(T) == 4) {
-            DevBuf<uint16_t> frag;
-            frag.ensure(size_t(3) * (t.npad / 16) * (2 * t.Mpad / 16) * 64 * 4);
-            launch_pack_fragments(V.p, t.N, t.Mpad, t.npad, frag.p, S());
-            launch_theta_mfma(vnet, frag.p, t.npad, t.confs.p, t.words, t.count, SV.p, S());
-            launch_structured_Ov<T>(net, t.confs.p, t.words, g.p, SV.p, t.layout == LAYOUT_DEEP1 ? vb.p : nullptr,
-                                    t.layout == LAYOUT_DEEP1 ? va.p : nullptr, t.count, u.p, S());
-            Runtime::get().sync();
+            t.frag.ensure(size_t(3) * (t.npad / 16) * (2 * t.Mpad / 16) * 64 * 4);
+            launch_pack_fragments(V.p, t.N, t.Mpad, t.npad, t.frag.p, S());
+            launch_theta_mfma(vnet, t.frag.p, t.npad, t.confs.p, t.words, t.count, SV.p, S());
         } else {
             launch_theta_valu<T>(vnet, t.confs.p, t.words, t.count, SV.p, S());
-            launch_structured_Ov<T>(net, t.confs.p, t.words, g.p, SV.p, t.layout == LAYOUT_DEEP1 ? vb.p : nullptr,
-                                    t.layout == LAYOUT_DEEP1 ? va.p : nullptr, t.count, u.p, S());
         }
-        launch_cdot(t.Obar.p, v, P, 0, reinterpret_cast<double*>(c.p), S());
-        launch_center_scale(u.p, t.w.p, reinterpret_cast<const double*>(c.p), t.count, S());
+        launch_structured_Ov<T>(net, t.confs.p, t.words, g.p, SV.p, t.layout == LAYOUT_DEEP1 ? t.vb.p : nullptr,
+                                t.layout == LAYOUT_DEEP1 ? t.va.p : nullptr, t.count, t.u.p, S());
+        launch_cdot(t.Obar.p, v, P, 0, reinterpret_cast<double*>(t.c1.p), S());
+        launch_center_scale(t.u.p, t.w.p, reinterpret_cast<const double*>(t.c1.p), t.count, S());
     }
-    structured_sums<T>(net, t.confs.p, t.words, g.p, 1, u.p, t.count, y, shift, v);
+    structured_sums<T>(t, net, t.confs.p, t.words, g.p, 1, t.u.p, t.count, y, shift, v);
 }
 
 }  // namespace
@@ -669,17 +691,18 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         *iters = 1;
         return 0;
     }
-    // conjugate gradients, fp64 vectors and dot products on the device
+    // conjugate gradients with a Jacobi (diagonal) preconditioner; fp64 vectors and
+    // dot products on the device, one host round trip per scalar
     if (!t.has_samples) throw std::logic_error("CG solve needs TDVP.eval first");
-    DevBuf<cplx<double>> r, p, Ap;
+    DevBuf<cplx<double>> r, z, p, Ap;
     DevBuf<double> dots;
     r.ensure(P);
+    z.ensure(P);
     p.ensure(P);
     Ap.ensure(P);
     dots.ensure(2);
     ANN_HIP_CHECK(hipMemsetAsync(x.p, 0, sizeof(cplx<double>) * P, S()));
     ANN_HIP_CHECK(hipMemcpyAsync(r.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
-    ANN_HIP_CHECK(hipMemcpyAsync(p.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
     auto dot = [&](const cplx<double>* a, const cplx<double>* b) {
         launch_cdot(a, b, P, 1, dots.p, S());
         double h[2];
@@ -687,18 +710,30 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         Runtime::get().sync();
         return cd(h[0], h[1]);
     };
+    const bool jacobi = t.diag.p != nullptr && method == 1;
+    auto precondition = [&]() {
+        if (jacobi)
+            launch_precond(z.p, r.p, t.diag.p, shift, P, S());
+        else
+            ANN_HIP_CHECK(hipMemcpyAsync(z.p, r.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
+    };
+    precondition();
+    ANN_HIP_CHECK(hipMemcpyAsync(p.p, z.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
     double rr = dot(r.p, r.p).real();
+    double rz = dot(r.p, z.p).real();
     const double bnorm = std::sqrt(rr);
     int it = 0;
     while (it < max_iter && std::sqrt(rr) > rtol * bnorm) {
         tdvp_S_dot(t, p.p, shift, Ap.p);
         const double pAp = dot(p.p, Ap.p).real();
         if (!(pAp > 0.0)) break;
-        const double alpha = rr / pAp;
+        const double alpha = rz / pAp;
         launch_cg_step(x.p, r.p, p.p, Ap.p, alpha, P, S());
-        const double rr_new = dot(r.p, r.p).real();
-        launch_cg_dir(p.p, r.p, rr_new / rr, P, S());
-        rr = rr_new;
+        precondition();
+        rr = dot(r.p, r.p).real();
+        const double rz_new = dot(r.p, z.p).real();
+        launch_cg_dir(p.p, z.p, rz_new / rz, P, S());
+        rz = rz_new;
         it++;
     }
     x.download(reinterpret_cast<cplx<double>*>(x_out), P);

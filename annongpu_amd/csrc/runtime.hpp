@@ -205,6 +205,12 @@ struct Tdvp {
     DevBuf<double> w;
     DevBuf<cplx<float>> g32;
     DevBuf<cplx<double>> g64;
+    DevBuf<double> diag;   // diagonal of S (Jacobi preconditioner)
+    // S.v workspace (grow-only)
+    DevBuf<cplx<double>> va, vb, u, c1, Yw, Yb, Ya;
+    DevBuf<cplx<float>> V32, SV32, zero32;
+    DevBuf<cplx<double>> V64, SV64, zero64;
+    DevBuf<uint16_t> frag;
 };
 
 // core operations

@@ -30,7 +30,7 @@ __global__ void init_chains_kernel(uint64_t* confs, int num_chains, int words, i
         for (int k = 0; k < n_up; k++) {
             if ((k & 3) == 0) r = philox(uint32_t(c), uint32_t(k >> 2), call_index, 2u, k0, k1);
             const uint32_t x = (k & 3) == 0 ? r.x : (k & 3) == 1 ? r.y : (k & 3) == 2 ? r.z : r.w;
-            const int pos = select_bit(down, words, x % uint32_t(N - k));
+            const int pos = select_bit(down, words, range32(x, uint32_t(N - k)));
             down[pos >> 6] &= ~(uint64_t(1) << (pos & 63));
             w[pos >> 6] |= uint64_t(1) << (pos & 63);
         }
@@ -171,6 +171,35 @@ __global__ void cg_dir_kernel(cplx<double>* p, const cplx<double>* r, double bet
     p[i] = {r[i].re + beta * p[i].re, r[i].im + beta * p[i].im};
 }
 
+// Jacobi preconditioner: z = r / (diag + shift)
+__global__ void precond_kernel(cplx<double>* z, const cplx<double>* r, const double* diag, double shift, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double d = diag[i] + shift;
+    z[i] = {r[i].re / d, r[i].im / d};
+}
+
+// diag of S for a dense net: a part wsum - |<O_a>|^2, b and W parts D_j - |<O_k>|^2
+__global__ void sr_diag_kernel(int layout, int N, int M, int Mpad, double wsum, const cplx<double>* Dj,
+                               const cplx<double>* Obar, int P, double* diag) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P) return;
+    double d;
+    if (layout == LAYOUT_RBM) {
+        d = Dj[k % M].re;
+    } else if (k < N) {
+        d = wsum;
+    } else if (k < N + M) {
+        d = Dj[k - N].re;
+    } else {
+        d = Dj[(k - N - M) % M].re;
+    }
+    const cplx<double> o = Obar[k];
+    d -= o.re * o.re + o.im * o.im;
+    diag[k] = d > 0.0 ? d : 0.0;
+    (void)Mpad;
+}
+
 // S += alpha * conj(o) o^T
 __global__ void rank1_kernel(cplx<double>* S, const cplx<double>* o, int P, double alpha) {
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -260,6 +289,18 @@ void launch_cg_step(cplx<double>* x, cplx<double>* r, const cplx<double>* p, con
 
 void launch_cg_dir(cplx<double>* p, const cplx<double>* r, double beta, int64_t n, hipStream_t stream) {
     cg_dir_kernel<<<blocks_for(n), 256, 0, stream>>>(p, r, beta, n);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_precond(cplx<double>* z, const cplx<double>* r, const double* diag, double shift, int64_t n,
+                    hipStream_t stream) {
+    precond_kernel<<<blocks_for(n), 256, 0, stream>>>(z, r, diag, shift, n);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_sr_diag(int layout, int N, int M, int Mpad, double wsum, const cplx<double>* Dj,
+                    const cplx<double>* Obar, int P, double* diag, hipStream_t stream) {
+    sr_diag_kernel<<<blocks_for(P), 256, 0, stream>>>(layout, N, M, Mpad, wsum, Dj, Obar, P, diag);
     ANN_LAUNCH_CHECK();
 }
 

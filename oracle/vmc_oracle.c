@@ -168,6 +168,11 @@ static cd log_psi_from_theta(const orc_psi* psi, const cd* theta) {
         for (int j = 0; j < psi->M; j++) r += orc_logcosh(theta[j], 0) * psi->final_weight;
         return psi->log_prefactor + r;
     }
+    if (psi->num_layers == 1) {
+        cd r = 0.0;
+        for (int j = 0; j < psi->size[0]; j++) r += orc_logcosh(theta[j], 0) * psi->final_weights[j];
+        return psi->log_prefactor + r;
+    }
     return psi->log_prefactor + deep_forward(psi, theta, NULL);
 }
 
@@ -359,6 +364,36 @@ cd orc_local_energy(const orc_operator* op, const orc_psi* psi, const uint64_t* 
     return result;
 }
 
+/* E_loc with the reference's incremental angle update (Operator.hpp:60-75:
+ * update_input_units, log_psi_s, revert) for wave functions with a dense first
+ * layer; theta holds the angles of conf. */
+static cd local_energy_incremental(const orc_operator* op, const orc_psi* psi, const cd* Wd, int width,
+                                   const uint64_t* conf, const cd* theta, cd log_psi, cd* scratch) {
+    cd result = 0.0;
+    uint64_t prime[ORC_MAX_WORDS];
+    for (int n = 0; n < op->num_strings; n++) {
+        cd factor;
+        orc_pauli_apply(op->words, op->a + (size_t)n * op->words, op->b + (size_t)n * op->words,
+                        conf, prime, &factor);
+        factor *= op->coefficients[n];
+        int diag = 1;
+        for (int w = 0; w < op->words; w++) diag &= (prime[w] == conf[w]);
+        if (diag) { result += factor; continue; }
+        memcpy(scratch, theta, sizeof(cd) * width);
+        for (int w = 0; w < op->words; w++) {
+            uint64_t m = prime[w] ^ conf[w];
+            while (m) {
+                const int i = w * 64 + __builtin_ctzll(m);
+                m &= m - 1;
+                const double si = spin_at(conf, i);
+                for (int h = 0; h < width; h++) scratch[h] += -2.0 * si * Wd[(size_t)i * width + h];
+            }
+        }
+        result += factor * cexp(log_psi_from_theta(psi, scratch) - log_psi);
+    }
+    return result;
+}
+
 /* --------------------------------------------------------- enumeration */
 
 void orc_exact_log_psi_vector(const orc_psi* psi, int32_t num_sites, cd* out) {
@@ -383,13 +418,29 @@ void orc_tdvp_accumulate(const orc_operator* op, const orc_psi* psi, int64_t num
 #ifdef _OPENMP
     if (num_threads > 0) omp_set_num_threads(num_threads);
 #endif
-    #pragma omp parallel for schedule(dynamic, 16)
-    for (int64_t s = 0; s < num_samples; s++) {
-        const uint64_t* conf = confs + (size_t)s * words;
-        const cd lp = orc_log_psi_s(psi, conf);
-        El[s] = op ? orc_local_energy(op, psi, conf, lp) : 0.0;
-        orc_psi_O_k(psi, conf, O_all + (size_t)s * P);
+    int width = 0;
+    cd* Wd = psi->kind == ORC_CNN ? NULL : first_layer_dense(psi, &width);
+    #pragma omp parallel
+    {
+        cd* theta = Wd ? (cd*)malloc(sizeof(cd) * width) : NULL;
+        cd* scratch = Wd ? (cd*)malloc(sizeof(cd) * width) : NULL;
+        #pragma omp for schedule(dynamic, 16)
+        for (int64_t s = 0; s < num_samples; s++) {
+            const uint64_t* conf = confs + (size_t)s * words;
+            if (Wd) {
+                compute_theta(psi, conf, theta);
+                const cd lp = log_psi_from_theta(psi, theta);
+                El[s] = op ? local_energy_incremental(op, psi, Wd, width, conf, theta, lp, scratch) : 0.0;
+            } else {
+                const cd lp = orc_log_psi_s(psi, conf);
+                El[s] = op ? orc_local_energy(op, psi, conf, lp) : 0.0;
+            }
+            orc_psi_O_k(psi, conf, O_all + (size_t)s * P);
+        }
+        free(theta);
+        free(scratch);
     }
+    free(Wd);
     cd e = 0.0; double e2 = 0.0;
     for (int64_t s = 0; s < num_samples; s++) {
         e += weights[s] * El[s];
@@ -423,6 +474,9 @@ void orc_tdvp_accumulate(const orc_operator* op, const orc_psi* psi, int64_t num
 }
 
 /* --------------------------------------------------------- Monte Carlo */
+
+/* integer in [0, n) from a 32-bit random word: (x * n) >> 32 (Lemire) */
+static inline uint32_t range32(uint32_t x, uint32_t n) { return (uint32_t)(((uint64_t)x * n) >> 32); }
 
 static int select_set_bit(const uint64_t* mask, int words, uint32_t rank) {
     for (int w = 0; w < words; w++) {
@@ -459,7 +513,7 @@ static void init_configuration(const orc_psi* psi, const orc_mc_params* p, uint3
     uint32_t r[4] = {0, 0, 0, 0};
     for (int k = 0; k < n_up; k++) {
         if ((k & 3) == 0) draw(p->seed, chain, (uint32_t)(k >> 2), p->call_index, 2u, r);
-        const int pos = select_set_bit(down, words, r[k & 3] % (uint32_t)(N - k));
+        const int pos = select_set_bit(down, words, range32(r[k & 3], (uint32_t)(N - k)));
         down[pos >> 6] &= ~(((uint64_t)1) << (pos & 63));
         conf[pos >> 6] |= ((uint64_t)1) << (pos & 63);
     }
@@ -516,7 +570,7 @@ void orc_mc_run(const orc_psi* psi, const orc_mc_params* p, uint64_t* chain_conf
             uint32_t r[4];
             draw(p->seed, (uint32_t)c, (uint32_t)t, (uint32_t)(t >> 32), 0u, r);
             t++;
-            int i = (int)(r[0] % (uint32_t)N), j = -1;
+            int i = (int)range32(r[0], (uint32_t)N), j = -1;
             if (p->move == ORC_MOVE_EXCHANGE) {
                 uint64_t opp[ORC_MAX_WORDS];
                 const int up = spin_at(conf, i) > 0;
@@ -528,7 +582,7 @@ void orc_mc_run(const orc_psi* psi, const orc_mc_params* p, uint64_t* chain_conf
                     n_opp += __builtin_popcountll(m);
                 }
                 if (n_opp == 0) { rej_total++; continue; }
-                j = select_set_bit(opp, words, r[1] % (uint32_t)n_opp);
+                j = select_set_bit(opp, words, range32(r[1], (uint32_t)n_opp));
             }
             for (int w = 0; w < words; w++) next[w] = conf[w];
             next[i >> 6] ^= ((uint64_t)1) << (i & 63);

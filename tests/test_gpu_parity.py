@@ -211,3 +211,29 @@ def test_exchange_conserves_sz():
     ups = np.array([bin(int(x)).count("1") for x in s[:, 0]])
     assert np.all(ups == N // 2)
     assert 0.0 < ens.acceptance_rate <= 1.0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_structured_matvec_matches_dense_gram(precision):
+    """matrix-free S.v (two spin GEMMs) equals the dense Gram S = O^H W O applied to v,
+    on MC samples at a size the oracle cannot cover (P = 8384)."""
+    N, M = 64, 128
+    psi = A.seeded_rbm_deep(N, M, 9, sigma=0.05)
+    psi.precision = precision
+    ens = A.MonteCarloSpins(2048, 2, 5, 512, move="exchange", seed=5)
+    tdvp = A.TDVP(psi.num_params)
+    tdvp.eval(A.Operator(A.hamiltonians.heisenberg_chain(N)), psi, ens)
+    S = tdvp.S_matrix
+    assert np.allclose(S, S.conj().T, atol=1e-12 * np.abs(S).max())
+    rng = np.random.default_rng(1)
+    for _ in range(2):
+        v = cgauss(rng, psi.num_params, 1.0)
+        assert rel(tdvp.S_dot_vector(v), S @ v) < (1e-10 if precision == "fp64" else 2e-5)
+    # Jacobi preconditioner = diagonal of S: (S e_k)_k for a few k
+    for k in (0, 5, 64, 64 + 7, 64 + 128 + 300, psi.num_params - 1):
+        e = np.zeros(psi.num_params, dtype=complex)
+        e[k] = 1
+        assert tdvp.S_dot_vector(e)[k].real == pytest.approx(S[k, k].real, rel=1e-4, abs=1e-9)
+    x = tdvp.solve(1e-3, "cg", rtol=1e-5, max_iter=3000)
+    assert rel((S + 1e-3 * np.eye(psi.num_params)) @ x, tdvp.F_vector) < 2e-5
+    assert tdvp.last_solve["iterations"] < 3000
