@@ -3,7 +3,8 @@
 Drop-in for the reference's pyANNonGPU on the VMC path
 (``import annongpu_amd as pyANNonGPU``): wave functions PsiRBM / PsiDeep /
 PsiCNN, Pauli-sum Operators, MonteCarloSpins / ExactSummationSpins ensembles,
-ExpectationValue, TDVP (+ the SR solve) and the free functions of
+ExpectationValue, TDVP (+ the SR solve), a learned block proposal
+(BlockProposal) and the free functions of
 pyANNonGPU/main.cpp.template:494-540, all computed by hand-written gfx950 HIP
 kernels behind the C ABI in include/annongpu.h.
 """
@@ -12,7 +13,7 @@ from .api import (
     Spins, PsiRBM, PsiDeep, PsiCNN, Operator, MonteCarloSpins, ExactSummationSpins,
     ExpectationValue, TDVP, log_psi_s, psi_O_k, psi_O_k_vector, log_psi, psi_vector,
     log_psi_vector, apply_operator, local_energies, activation_function, setDevice,
-    start_profiling, stop_profiling, profiler_query, profiler_reset,
+    start_profiling, stop_profiling, profiler_query, profiler_reset, BlockProposal, rect_blocks, chain_blocks,
 )
 from .constructors import (
     new_RBM, new_deep_neural_network, new_convolutional_network, seeded_rbm, seeded_rbm_deep,
@@ -20,6 +21,7 @@ from .constructors import (
 )
 from .json_numpy import NumpyEncoder, NumpyDecoder
 from . import hamiltonians
+from .evolution import RK4, Euler
 
 __all__ = [
     "PauliExpression", "sigma_x", "sigma_y", "sigma_z", "identity", "Spins", "PsiRBM", "PsiDeep", "PsiCNN",
@@ -27,5 +29,6 @@ __all__ = [
     "psi_O_k_vector", "log_psi", "psi_vector", "log_psi_vector", "apply_operator", "local_energies",
     "activation_function", "setDevice", "start_profiling", "stop_profiling", "profiler_query", "profiler_reset",
     "new_RBM", "new_deep_neural_network", "new_convolutional_network", "seeded_rbm", "seeded_rbm_deep",
-    "seeded_cnn", "deep_connections", "NumpyEncoder", "NumpyDecoder", "hamiltonians",
+    "seeded_cnn", "deep_connections", "NumpyEncoder", "NumpyDecoder", "hamiltonians", "BlockProposal",
+    "rect_blocks", "chain_blocks", "RK4", "Euler",
 ]

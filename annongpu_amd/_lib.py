@@ -75,6 +75,15 @@ _SIGNATURES = {
     "ann_tdvp_get_energy": [P, P, P, P],
     "ann_tdvp_S_dot_vector": [P, P, P],
     "ann_tdvp_solve": [P, f64, i32, f64, i32, P, P, P],
+    "ann_proposal_create": [P, i32, i32, i32, i32, P, u64],
+    "ann_proposal_destroy": [P],
+    "ann_proposal_num_params": [P, P],
+    "ann_proposal_get_params": [P, P],
+    "ann_proposal_set_params": [P, P],
+    "ann_proposal_loss_grad": [P, P, i64, i32, P, P, P],
+    "ann_proposal_adam_step": [P, P, f64],
+    "ann_proposal_train": [P, P, i32, i32, f64, P],
+    "ann_monte_carlo_set_proposal": [P, P, u32],
 }
 
 for _name, _args in _SIGNATURES.items():

@@ -543,6 +543,110 @@ class MonteCarloSpins(_Ensemble):
         return a.value / tot if tot else float("nan")
 
 
+    def set_proposal(self, proposal, plain_sweeps=1):
+        """Attach a trained BlockProposal (None detaches).  Each sample interval then
+        holds num_sweeps block sweeps plus ``plain_sweeps`` plain sweeps."""
+        check(lib.ann_monte_carlo_set_proposal(self._h, proposal._h if proposal is not None else None,
+                                               int(plain_sweeps)))
+        self._proposal = proposal      # keeps the network alive while attached
+        self.plain_sweeps = int(plain_sweeps)
+
+    @property
+    def proposal(self):
+        return getattr(self, "_proposal", None)
+
+
+class BlockProposal:
+    """Learned S_z-preserving block proposal (additive; the reference has none).
+
+    ``blocks`` is a [num_blocks][block_size] array of site indices (block_size <= 8).
+    The network N -> hidden -> N (tanh) sees the configuration with the block's
+    spins masked and scores the block's sites; a move redraws the block among the
+    patterns with the same number of up spins.  Training (``fit``) minimises the
+    block pseudo-likelihood on samples the sampler itself produced.
+    """
+
+    def __init__(self, num_sites, blocks, hidden=256, seed=0):
+        blocks = np.ascontiguousarray(blocks, dtype=np.int32)
+        if blocks.ndim != 2:
+            raise ValueError("blocks must be [num_blocks][block_size]")
+        self.num_sites = int(num_sites)
+        self.hidden = int(hidden)
+        self.blocks = blocks.copy()
+        self._h = handle()
+        check(lib.ann_proposal_create(ref(self._h), self.num_sites, self.hidden, blocks.shape[0], blocks.shape[1],
+                                      ptr(blocks), int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib.ann_proposal_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def num_params(self):
+        n = ctypes.c_int64()
+        check(lib.ann_proposal_num_params(self._h, ref(n)))
+        return n.value
+
+    @property
+    def params(self):
+        out = np.zeros(self.num_params, dtype=np.float32)
+        check(lib.ann_proposal_get_params(self._h, ptr(out)))
+        return out
+
+    @params.setter
+    def params(self, value):
+        v = np.ascontiguousarray(value, dtype=np.float32).ravel()
+        if v.size != self.num_params:
+            raise ValueError("expected %d parameters" % self.num_params)
+        check(lib.ann_proposal_set_params(self._h, ptr(v)))
+
+    def loss_grad(self, confs, sites):
+        """loss and gradient on explicit rows: confs [count][words] (or [count]), sites [count][block_size]"""
+        c = np.ascontiguousarray(confs, dtype=np.uint64)
+        if c.ndim == 1:
+            c = c[:, None]
+        st = np.ascontiguousarray(sites, dtype=np.int32)
+        grad = np.zeros(self.num_params, dtype=np.float32)
+        loss = ctypes.c_double()
+        check(lib.ann_proposal_loss_grad(self._h, ptr(c), c.shape[0], c.shape[1], ptr(st), ref(loss), ptr(grad)))
+        return loss.value, grad
+
+    def adam_step(self, grad, lr):
+        g = np.ascontiguousarray(grad, dtype=np.float32)
+        check(lib.ann_proposal_adam_step(self._h, ptr(g), float(lr)))
+
+    def fit(self, ensemble, steps, batch=4096, lr=1e-3):
+        """Adam on random (sample, block) rows of the ensemble's last samples; returns the last batch loss"""
+        loss = ctypes.c_double()
+        check(lib.ann_proposal_train(self._h, ensemble._h, int(steps), int(batch), float(lr), ref(loss)))
+        return loss.value
+
+
+def rect_blocks(L, shape=(2, 4), both_orientations=True):
+    """all placements of a shape[0] x shape[1] rectangle on a periodic L[0] x L[1]
+    lattice (site r * L[1] + c), optionally also the transposed rectangle"""
+    Lx, Ly = (L, L) if isinstance(L, int) else L
+    shapes = [tuple(shape)]
+    if both_orientations and shape[0] != shape[1]:
+        shapes.append((shape[1], shape[0]))
+    blocks = []
+    for a, b in shapes:
+        for r in range(Lx):
+            for c in range(Ly):
+                blocks.append([((r + i) % Lx) * Ly + (c + j) % Ly for i in range(a) for j in range(b)])
+    return np.array(blocks, dtype=np.int32)
+
+
+def chain_blocks(N, size=8):
+    """all windows of ``size`` consecutive sites of a periodic chain"""
+    return np.array([[(i + j) % N for j in range(size)] for i in range(N)], dtype=np.int32)
+
+
 class ExactSummationSpins(_Ensemble):
     """ExactSummationSpins(num_sites, gpu) -- source/ensembles/ExactSummation.cu:16-25."""
 

@@ -7,6 +7,7 @@
 #include "runtime.hpp"
 
 #include <cmath>
+#include <random>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -22,6 +23,9 @@ struct ann_ensemble_s {
 };
 struct ann_tdvp_s {
     ann::Tdvp impl;
+};
+struct ann_proposal_s {
+    ann::Proposal impl;
 };
 
 namespace {
@@ -61,28 +65,19 @@ ann::Samples& single_sample(ann::Psi& psi, const uint64_t* conf, ann::Samples& s
     psi.sync_device();
     s.count = 1;
     s.words = psi.words;
-    s.Mpad = psi.Mpad;
+    s.Mpad = psi.dense() ? psi.Mpad : 0;
     s.precision = psi.precision;
     s.confs.upload(conf, psi.words);
     s.w.ensure(1);
     s.log_psi.ensure(1);
-    if (psi.precision == 0)
-        s.th64.ensure(psi.Mpad);
-    else
-        s.th32.ensure(psi.Mpad);
-    ann::compute_theta(psi, s.confs.p, 1, s, 0);
-    const ann::cplx<double> lp = {psi.log_prefactor.real(), psi.log_prefactor.imag()};
-    if (psi.precision == 0)
-        ann::launch_net_log_psi<double>(psi.view64(), s.th64.p, 1, lp, s.log_psi.p, nullptr,
-                                        ann::Runtime::get().stream());
-    else
-        ann::launch_net_log_psi<float>(psi.view32(), s.th32.p, 1, lp, s.log_psi.p, nullptr,
-                                       ann::Runtime::get().stream());
+    if (psi.dense()) {
+        if (psi.precision == 0)
+            s.th64.ensure(psi.Mpad);
+        else
+            s.th32.ensure(psi.Mpad);
+    }
+    ann::log_psi_of(psi, s.confs.p, 1, s, nullptr);
     return s;
-}
-
-void require_dense(const ann::Psi& psi) {
-    if (!psi.dense()) throw std::logic_error("this wave function kind is not available on the GPU path yet");
 }
 
 std::vector<cd> host_log_psi(ann::Samples& s) {
@@ -98,41 +93,10 @@ std::vector<double> host_weights(ann::Samples& s) {
     return w;
 }
 
-// sum_s coef_s O_s for the last samples of a dense net, in parameter order
-template <typename T>
-void ok_sums(ann::Psi& psi, ann::Samples& s, const ann::cplx<double>* coef, int conj_g, ann::cplx<double>* out) {
-    using namespace ann;
-    hipStream_t st = Runtime::get().stream();
-    DenseView<T> net;
-    if constexpr (sizeof(T) == 8)
-        net = psi.view64();
-    else
-        net = psi.view32();
-    DevBuf<cplx<T>> g;
-    g.ensure(size_t(s.count) * psi.Mpad);
-    const cplx<T>* th;
-    if constexpr (sizeof(T) == 8)
-        th = s.th64.p;
-    else
-        th = s.th32.p;
-    launch_g_factor<T>(net, th, s.count, g.p, st);
-    DevBuf<cplx<double>> Yw, Yb, Ya;
-    Yw.ensure(size_t(psi.N) * psi.Mpad);
-    Yb.ensure(psi.Mpad);
-    Ya.ensure(psi.N);
-    launch_spin_weighted_sums<T>(net, s.confs.p, s.words, g.p, conj_g, coef, s.count, Yw.p, Yb.p, Ya.p, st);
-    launch_pack_y(net.layout, net.N, net.M, net.Mpad, net.C, net.conn, Ya.p, Yb.p, Yw.p, nullptr, 0.0,
-                  psi.num_params, out, st);
-    Runtime::get().sync();
-}
-
 void structured_sum(ann::Psi& psi, ann::Samples& s, const ann::cplx<double>* coef, int conj_g, cd* host_out) {
     ann::DevBuf<ann::cplx<double>> out;
     out.ensure(psi.num_params);
-    if (psi.precision == 0)
-        ok_sums<double>(psi, s, coef, conj_g, out.p);
-    else
-        ok_sums<float>(psi, s, coef, conj_g, out.p);
+    ann::ok_weighted_sum(psi, s, coef, conj_g, out.p);
     out.download(reinterpret_cast<ann::cplx<double>*>(host_out), psi.num_params);
 }
 
@@ -474,7 +438,6 @@ int ann_log_psi_s(ann_psi psi, const uint64_t* conf, double* out2) {
     return guard([&] {
         need(psi, "psi");
         need(conf, "conf");
-        require_dense(psi->impl);
         ann::Samples s;
         single_sample(psi->impl, conf, s);
         auto lp = host_log_psi(s);
@@ -487,7 +450,6 @@ int ann_psi_O_k(ann_psi psi, const uint64_t* conf, double* out) {
         need(psi, "psi");
         need(conf, "conf");
         auto& q = psi->impl;
-        require_dense(q);
         ann::Samples s;
         single_sample(q, conf, s);
         ann::DevBuf<ann::cplx<double>> coef;
@@ -749,6 +711,123 @@ int ann_tdvp_solve(ann_tdvp tdvp, double shift, int32_t method, double rtol, int
         }
     });
     return g != ANN_OK ? g : rc;
+}
+
+// ---------------------------------------------------------- learned proposal
+
+int ann_proposal_create(ann_proposal* out, int32_t num_sites, int32_t hidden, int32_t num_blocks,
+                        int32_t block_size, const int32_t* block_sites, uint64_t seed) {
+    return guard([&] {
+        need(out, "out");
+        need(block_sites, "block_sites");
+        if (num_sites < 1 || num_sites > 64 * ann::kMaxWords) throw std::invalid_argument("1..256 sites");
+        if (hidden < 1 || hidden > 256) throw std::invalid_argument("hidden must be 1..256");
+        if (block_size < 1 || block_size > ann::kMaxBlock) throw std::invalid_argument("block_size must be 1..8");
+        if (num_blocks < 1) throw std::invalid_argument("num_blocks must be positive");
+        for (int64_t r = 0; r < int64_t(num_blocks) * block_size; r++)
+            if (block_sites[r] < 0 || block_sites[r] >= num_sites) throw std::invalid_argument("block site out of range");
+        for (int b = 0; b < num_blocks; b++)
+            for (int i = 0; i < block_size; i++)
+                for (int j = 0; j < i; j++)
+                    if (block_sites[b * block_size + i] == block_sites[b * block_size + j])
+                        throw std::invalid_argument("repeated site in a block");
+        auto p = new ann_proposal_s;
+        try {
+            auto& q = p->impl;
+            q.N = num_sites;
+            q.H = hidden;
+            q.bs = block_size;
+            q.num_blocks = num_blocks;
+            q.seed = seed;
+            q.blocks.upload(block_sites, size_t(num_blocks) * block_size);
+            // W ~ U(-1/sqrt(fan_in), 1/sqrt(fan_in)), biases 0
+            std::vector<float> h(q.num_params(), 0.0f);
+            std::mt19937_64 gen(seed);
+            std::uniform_real_distribution<double> u1(-1.0 / std::sqrt(double(num_sites)), 1.0 / std::sqrt(double(num_sites)));
+            std::uniform_real_distribution<double> u2(-1.0 / std::sqrt(double(hidden)), 1.0 / std::sqrt(double(hidden)));
+            const size_t nW1 = size_t(hidden) * num_sites;
+            for (size_t k = 0; k < nW1; k++) h[k] = float(u1(gen));
+            for (size_t k = 0; k < size_t(num_sites) * hidden; k++) h[nW1 + hidden + k] = float(u2(gen));
+            q.params.upload(h.data(), h.size());
+            std::vector<float> zero(h.size(), 0.0f);
+            q.m.upload(zero.data(), zero.size());
+            q.v.upload(zero.data(), zero.size());
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+int ann_proposal_destroy(ann_proposal p) {
+    return guard([&] { delete p; });
+}
+
+int ann_proposal_num_params(ann_proposal p, int64_t* n) {
+    return guard([&] {
+        need(p, "p");
+        need(n, "n");
+        *n = p->impl.num_params();
+    });
+}
+
+int ann_proposal_get_params(ann_proposal p, float* params) {
+    return guard([&] {
+        need(p, "p");
+        need(params, "params");
+        p->impl.params.download(params, p->impl.num_params());
+    });
+}
+
+int ann_proposal_set_params(ann_proposal p, const float* params) {
+    return guard([&] {
+        need(p, "p");
+        need(params, "params");
+        p->impl.params.upload(params, p->impl.num_params());
+        p->impl.packed = false;
+    });
+}
+
+int ann_proposal_loss_grad(ann_proposal p, const uint64_t* confs, int64_t count, int32_t words,
+                           const int32_t* sites, double* loss, float* grad) {
+    return guard([&] {
+        need(p, "p");
+        need(confs, "confs");
+        need(sites, "sites");
+        std::vector<float> g;
+        const double l = ann::proposal_loss_grad(p->impl, confs, count, words, sites, g);
+        if (loss) *loss = l;
+        if (grad) std::memcpy(grad, g.data(), sizeof(float) * g.size());
+    });
+}
+
+int ann_proposal_adam_step(ann_proposal p, const float* grad, double lr) {
+    return guard([&] {
+        need(p, "p");
+        need(grad, "grad");
+        ann::proposal_adam_step(p->impl, grad, lr);
+    });
+}
+
+int ann_proposal_train(ann_proposal p, ann_ensemble ens, int32_t steps, int32_t batch, double lr, double* loss) {
+    return guard([&] {
+        need(p, "p");
+        need(ens, "ens");
+        auto& s = ens->impl.samples;
+        if (s.count <= 0) throw std::invalid_argument("the ensemble holds no samples yet");
+        const double l = ann::proposal_train(p->impl, s.confs.p, s.count, s.words, steps, batch, lr);
+        if (loss) *loss = l;
+    });
+}
+
+int ann_monte_carlo_set_proposal(ann_ensemble ens, ann_proposal p, uint32_t plain_sweeps) {
+    return guard([&] {
+        need(ens, "ens");
+        if (ens->impl.kind != ann::ENS_MC) throw std::invalid_argument("proposals apply to Monte Carlo ensembles");
+        ens->impl.proposal = p ? &p->impl : nullptr;
+        ens->impl.plain_sweeps = plain_sweeps;
+    });
 }
 
 }  // extern "C"

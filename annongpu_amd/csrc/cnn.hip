@@ -1,0 +1,359 @@
+// cnn.hip -- gfx950 kernels for PsiCNN (periodic convolutions, dim 3).
+//
+// One 64-lane wavefront per configuration; lane l owns the sites x = l + 64 u.
+// The activations of every layer live in LDS (dynamic shared memory, one slice
+// per wavefront).  Reference behaviour restated here:
+//   forward      include/quantum_state/PsiCNN.hpp:97-175
+//   O_k          include/quantum_state/PsiCNN.hpp:183-266
+//   convolution  include/quantum_state/detail/Convolve.hpp:112-173
+//   Metropolis   include/ensembles/MonteCarlo.hpp:133-177 (the reference has no
+//                incremental update for PsiCNN, PsiCNN.hpp:177-181: every
+//                proposal re-runs the forward pass, as here)
+#include "cnn.hpp"
+
+#include <type_traits>
+
+namespace ann {
+
+namespace {
+
+// input of layer l at site y for channel ci: spins for layer 0, the previous
+// layer's activation otherwise (stored in A)
+template <typename T>
+__device__ inline cplx<T> layer_input(int l, const uint64_t* w, const cplx<T>* A, int N, int ci, int y) {
+    if (l == 0) {
+        const uint64_t word = y < 64 ? w[0] : y < 128 ? w[1] : y < 192 ? w[2] : w[3];
+        return {((word >> (y & 63)) & 1) ? T(1) : T(-1), T(0)};
+    }
+    return A[ci * N + y];
+}
+
+// forward pass; A (max_ch * N) receives the activations of each layer in turn
+// (A2 is the ping-pong partner), Z (optional) receives the pre-activations of
+// every layer at Z + zoff[l].  Returns the network part of log psi.
+template <typename T>
+__device__ cplx<T> cnn_forward(const CnnView<T>& net, const uint64_t* w, cplx<T>* A, cplx<T>* A2, cplx<T>* Z,
+                               int lane) {
+    const int N = net.N;
+    cplx<T> total = {0, 0};
+    cplx<T>* in = A;
+    cplx<T>* out = A2;
+    for (int l = 0; l < net.L; l++) {
+        const int prev = l > 0 ? net.ch[l - 1] : 1;
+        const int vol = net.vol[l];
+        const int C = net.ch[l];
+        const bool last = l == net.L - 1;
+        for (int cj = 0; cj < C; cj++) {
+            for (int x = lane; x < N; x += 64) {
+                cplx<T> z = {0, 0};
+                const int cls = int(net.sym[x]);
+                for (int ci = 0; ci < prev; ci++) {
+                    const cplx<T>* wk = net.params + net.link_off[l] + (int64_t(ci) * C + cj) * net.nsym * vol +
+                                        int64_t(cls) * vol;
+                    const int* nb = net.nbr + net.nbr_off[l] + int64_t(x) * vol;
+                    for (int k = 0; k < vol; k++) {
+                        const cplx<T> a = layer_input<T>(l, w, in, N, ci, nb[k]);
+                        const cplx<T> q = wk[k];
+                        z.re += q.re * a.re - q.im * a.im;
+                        z.im += q.re * a.im + q.im * a.re;
+                    }
+                }
+                if (Z) Z[net.zoff[l] + cj * N + x] = z;
+                const cplx<T> act_v = act(z, l);
+                if (last) {
+                    total = cadd(total, act_v);
+                } else {
+                    out[cj * N + x] = act_v;
+                }
+            }
+        }
+        __syncthreads();
+        cplx<T>* t = in;
+        in = out;
+        out = t;
+    }
+    const cplx<T> s = {wave_sum(total.re), wave_sum(total.im)};
+    return cscale(s, net.final_factor);
+}
+
+// s is uniform over the wavefront, so these become scalar loads
+__device__ inline void load_words(const uint64_t* confs, int64_t s, int words, uint64_t* w) {
+    for (int k = 0; k < kMaxWords; k++) w[k] = k < words ? confs[s * words + k] : 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void cnn_log_psi_kernel(CnnView<T> net, const uint64_t* confs, int words,
+                                                         int64_t count, cplx<double> lp, cplx<double>* log_psi,
+                                                         double* weight) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* A = reinterpret_cast<cplx<T>*>(smem);
+    cplx<T>* A2 = A + net.max_ch * net.N;
+    const int lane = threadIdx.x;
+    for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
+        uint64_t w[kMaxWords];
+        load_words(confs, s, words, w);
+        const cplx<T> L = cnn_forward(net, w, A, A2, static_cast<cplx<T>*>(nullptr), lane);
+        if (lane == 0) {
+            const cplx<double> r = {lp.re + double(L.re), lp.im + double(L.im)};
+            log_psi[s] = r;
+            if (weight) weight[s] = exp(2.0 * r.re);
+        }
+        __syncthreads();
+    }
+}
+
+// Metropolis chains, proposals as in the dense sampler (Philox stream per chain)
+template <typename T>
+__global__ __launch_bounds__(64) void cnn_mc_kernel(CnnView<T> net, uint64_t* chain_confs, int words, int num_chains,
+                                                    int move, int64_t num_proposals, uint64_t counter, uint32_t key0,
+                                                    uint32_t key1, uint32_t* accepted) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* A = reinterpret_cast<cplx<T>*>(smem);
+    cplx<T>* A2 = A + net.max_ch * net.N;
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x;
+    if (c >= num_chains) return;
+    const int N = net.N;
+    uint64_t w[kMaxWords];
+    load_words(chain_confs, c, words, w);
+    uint64_t msk[kMaxWords];
+    for (int k = 0; k < kMaxWords; k++) {
+        const int bits = N - 64 * k;
+        msk[k] = k >= words ? 0 : bits >= 64 ? ~uint64_t(0) : bits > 0 ? (uint64_t(1) << bits) - 1 : 0;
+    }
+    T cur = cnn_forward(net, w, A, A2, static_cast<cplx<T>*>(nullptr), lane).re;
+    uint32_t acc = 0;
+    for (int64_t q = 0; q < num_proposals; q++) {
+        const uint64_t t = counter + uint64_t(q);
+        const u32x4 r = philox(uint32_t(c), uint32_t(t), uint32_t(t >> 32), 0u, key0, key1);
+        const int i = int(range32(r.x, uint32_t(N)));
+        const int si_bit = int((w[i >> 6] >> (i & 63)) & 1u);
+        int kp = -1;
+        if (move == MOVE_EXCHANGE) {
+            uint64_t o[kMaxWords];
+            uint32_t n_opp = 0;
+            for (int k = 0; k < kMaxWords; k++) {
+                o[k] = (si_bit ? ~w[k] : w[k]) & msk[k];
+                n_opp += uint32_t(__popcll(o[k]));
+            }
+            if (n_opp == 0) continue;
+            kp = select_bit(o, words, range32(r.y, n_opp));
+        }
+        uint64_t wn[kMaxWords];
+        for (int k = 0; k < kMaxWords; k++) wn[k] = w[k];
+        wn[i >> 6] ^= uint64_t(1) << (i & 63);
+        if (kp >= 0) wn[kp >> 6] ^= uint64_t(1) << (kp & 63);
+        const T nxt = cnn_forward(net, wn, A, A2, static_cast<cplx<T>*>(nullptr), lane).re;
+        const T d2 = T(2) * (nxt - cur);
+        const T uu = (T(r.z) + T(0.5)) * T(2.3283064365386963e-10);
+        if (d2 > T(0) || uu <= dexp(d2)) {
+            for (int k = 0; k < kMaxWords; k++) w[k] = wn[k];
+            cur = nxt;
+            acc++;
+        }
+    }
+    if (lane == 0) {
+        for (int k = 0; k < words; k++) chain_confs[int64_t(c) * words + k] = w[k];
+        accepted[c] += acc;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void cnn_local_energy_kernel(CnnView<T> net, OpView op, const uint64_t* confs,
+                                                              int64_t count, cplx<double>* e_loc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* A = reinterpret_cast<cplx<T>*>(smem);
+    cplx<T>* A2 = A + net.max_ch * net.N;
+    const int lane = threadIdx.x;
+    const int words = op.words;
+    for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
+        uint64_t w[kMaxWords];
+        load_words(confs, s, words, w);
+        const cplx<T> L0 = cnn_forward(net, w, A, A2, static_cast<cplx<T>*>(nullptr), lane);
+        double e_re = 0.0, e_im = 0.0;
+        for (int g = 0; g < op.num_groups; g++) {
+            double c_re = 0.0, c_im = 0.0;
+            for (int n = op.group_begin[g]; n < op.group_begin[g + 1]; n++) {
+                int par = 0;
+                for (int k = 0; k < words; k++) par += __popcll(~w[k] & op.string_zy[int64_t(n) * words + k]);
+                const double sg = (par & 1) ? -1.0 : 1.0;
+                c_re += sg * op.string_coef[2 * n];
+                c_im += sg * op.string_coef[2 * n + 1];
+            }
+            if (c_re == 0.0 && c_im == 0.0) continue;
+            bool diag = true;
+            uint64_t wn[kMaxWords];
+            for (int k = 0; k < kMaxWords; k++) {
+                const uint64_t m = k < words ? op.group_flip[int64_t(g) * words + k] : 0;
+                diag = diag && m == 0;
+                wn[k] = w[k] ^ m;
+            }
+            if (diag) {
+                e_re += c_re;
+                e_im += c_im;
+                continue;
+            }
+            const cplx<T> L1 = cnn_forward(net, wn, A, A2, static_cast<cplx<T>*>(nullptr), lane);
+            const double d_re = double(L1.re) - double(L0.re), d_im = double(L1.im) - double(L0.im);
+            const double m = exp(d_re);
+            double sn, cs;
+            sincos(d_im, &sn, &cs);
+            e_re += c_re * m * cs - c_im * m * sn;
+            e_im += c_re * m * sn + c_im * m * cs;
+        }
+        if (lane == 0) e_loc[s] = {e_re, e_im};
+        __syncthreads();
+    }
+}
+
+// O_k of PsiCNN by back-propagation; row s of O is written in parameter order
+// (minus center if given).  LDS: Z (all pre-activations), A, A2, G (gradients).
+template <typename T>
+__global__ __launch_bounds__(64) void cnn_ok_kernel(CnnView<T> net, const uint64_t* confs, int words, int64_t count,
+                                                    const cplx<double>* center, cplx<T>* O) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = net.N;
+    cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
+    cplx<T>* A = Z + net.zoff[net.L];
+    cplx<T>* A2 = A + net.max_ch * N;
+    cplx<T>* G = A2 + net.max_ch * N;    // d logpsi / d output activation of the current layer
+    cplx<T>* D = G + net.max_ch * N;     // d logpsi / d pre-activation
+    const int lane = threadIdx.x;
+    for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
+        uint64_t w[kMaxWords];
+        load_words(confs, s, words, w);
+        cnn_forward(net, w, A, A2, Z, lane);
+        cplx<T>* row = O + s * net.num_params;
+        const int last = net.L - 1;
+        for (int e = lane; e < net.ch[last] * N; e += 64) G[e] = {net.final_factor, T(0)};
+        __syncthreads();
+        for (int l = last; l >= 0; l--) {
+            const int prev = l > 0 ? net.ch[l - 1] : 1;
+            const int C = net.ch[l];
+            const int vol = net.vol[l];
+            for (int e = lane; e < C * N; e += 64) D[e] = cmul(G[e], dact(Z[net.zoff[l] + e], l));
+            // activations of the previous layer (inputs of layer l)
+            if (l > 0)
+                for (int e = lane; e < prev * N; e += 64) A[e] = act(Z[net.zoff[l - 1] + e], l - 1);
+            __syncthreads();
+            for (int e = lane; e < prev * N; e += 64) G[e] = {0, 0};
+            __syncthreads();
+            for (int ci = 0; ci < prev; ci++) {
+                for (int cj = 0; cj < C; cj++) {
+                    const int64_t link = net.link_off[l] + (int64_t(ci) * C + cj) * net.nsym * vol;
+                    for (int k = 0; k < vol; k++) {
+                        for (int cls = 0; cls < net.nsym; cls++) {
+                            cplx<T> part = {0, 0};
+                            for (int x = lane; x < N; x += 64) {
+                                if (int(net.sym[x]) != cls) continue;
+                                const int y = net.nbr[net.nbr_off[l] + int64_t(x) * vol + k];
+                                part = cadd(part, cmul(D[cj * N + x], layer_input<T>(l, w, A, N, ci, y)));
+                            }
+                            const cplx<T> tot = {wave_sum(part.re), wave_sum(part.im)};
+                            if (lane == 0) {
+                                const int64_t pidx = link + int64_t(cls) * vol + k;
+                                cplx<T> v = tot;
+                                if (center) {
+                                    v.re -= T(center[pidx].re);
+                                    v.im -= T(center[pidx].im);
+                                }
+                                row[pidx] = v;
+                            }
+                        }
+                        if (l > 0) {
+                            // scatter into the previous layer: x -> y is a translation for fixed k
+                            for (int x = lane; x < N; x += 64) {
+                                const int y = net.nbr[net.nbr_off[l] + int64_t(x) * vol + k];
+                                const cplx<T> wq = net.params[link + int64_t(net.sym[x]) * vol + k];
+                                G[ci * N + y] = cadd(G[ci * N + y], cmul(wq, D[cj * N + x]));
+                            }
+                            __syncthreads();
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// kernels with more than 64 KiB of LDS per block must opt in (at most the 160 KiB of a gfx950 CU)
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+    if (bytes > 160 * 1024) throw std::invalid_argument("PsiCNN activations exceed the 160 KiB LDS of a CU");
+    if (bytes > 64 * 1024)
+        ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+}
+
+inline unsigned grid_for(int64_t count) {
+    const int64_t g = count < 65536 ? count : 65536;
+    return unsigned(g > 0 ? g : 1);
+}
+
+}  // namespace
+
+template <typename T>
+size_t cnn_forward_lds(const CnnView<T>& net) {
+    return sizeof(cplx<T>) * 2 * size_t(net.max_ch) * net.N;
+}
+template <typename T>
+size_t cnn_ok_lds(const CnnView<T>& net) {
+    return sizeof(cplx<T>) * (size_t(net.zoff[net.L]) + 4 * size_t(net.max_ch) * net.N);
+}
+
+template <typename T>
+void launch_cnn_log_psi(const CnnView<T>& net, const uint64_t* confs, int words, int64_t count,
+                        cplx<double> log_prefactor, cplx<double>* log_psi, double* weight, hipStream_t stream) {
+    if (count <= 0) return;
+    allow_lds(cnn_log_psi_kernel<T>, cnn_forward_lds(net));
+    cnn_log_psi_kernel<T><<<grid_for(count), 64, cnn_forward_lds(net), stream>>>(net, confs, words, count,
+                                                                                 log_prefactor, log_psi, weight);
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_cnn_mc(const CnnView<T>& net, uint64_t* chain_confs, int words, int num_chains, int move,
+                   int64_t num_proposals, uint64_t counter, uint64_t seed, uint32_t* accepted, hipStream_t stream) {
+    if (num_chains <= 0 || num_proposals <= 0) return;
+    allow_lds(cnn_mc_kernel<T>, cnn_forward_lds(net));
+    cnn_mc_kernel<T><<<unsigned(num_chains), 64, cnn_forward_lds(net), stream>>>(
+        net, chain_confs, words, num_chains, move, num_proposals, counter, uint32_t(seed), uint32_t(seed >> 32),
+        accepted);
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_cnn_local_energy(const CnnView<T>& net, const OpView& op, const uint64_t* confs, int64_t count,
+                             cplx<double>* e_loc, hipStream_t stream) {
+    if (count <= 0) return;
+    allow_lds(cnn_local_energy_kernel<T>, cnn_forward_lds(net));
+    cnn_local_energy_kernel<T><<<grid_for(count), 64, cnn_forward_lds(net), stream>>>(net, op, confs, count, e_loc);
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_cnn_ok(const CnnView<T>& net, const uint64_t* confs, int words, int64_t count, const cplx<double>* center,
+                   cplx<T>* O, hipStream_t stream) {
+    if (count <= 0) return;
+    allow_lds(cnn_ok_kernel<T>, cnn_ok_lds(net));
+    cnn_ok_kernel<T><<<grid_for(count), 64, cnn_ok_lds(net), stream>>>(net, confs, words, count, center, O);
+    ANN_LAUNCH_CHECK();
+}
+
+#define ANN_CNN_INSTANTIATE(T)                                                                                   \
+    template size_t cnn_forward_lds<T>(const CnnView<T>&);                                                      \
+    template size_t cnn_ok_lds<T>(const CnnView<T>&);                                                           \
+    template void launch_cnn_log_psi<T>(const CnnView<T>&, const uint64_t*, int, int64_t, cplx<double>,        \
+                                        cplx<double>*, double*, hipStream_t);                                   \
+    template void launch_cnn_mc<T>(const CnnView<T>&, uint64_t*, int, int, int, int64_t, uint64_t, uint64_t,    \
+                                   uint32_t*, hipStream_t);                                                     \
+    template void launch_cnn_local_energy<T>(const CnnView<T>&, const OpView&, const uint64_t*, int64_t,       \
+                                             cplx<double>*, hipStream_t);                                       \
+    template void launch_cnn_ok<T>(const CnnView<T>&, const uint64_t*, int, int64_t, const cplx<double>*,      \
+                                   cplx<T>*, hipStream_t);
+
+ANN_CNN_INSTANTIATE(float)
+ANN_CNN_INSTANTIATE(double)
+
+}  // namespace ann

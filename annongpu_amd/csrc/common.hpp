@@ -220,6 +220,26 @@ __device__ inline double wave_sum(double v) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+__device__ inline float wave_max(float v) {
+    v = fmaxf(v, dpp_f32<0xB1>(v));
+    v = fmaxf(v, dpp_f32<0x4E>(v));
+    v = fmaxf(v, dpp_f32<0x141>(v));
+    v = fmaxf(v, dpp_f32<0x140>(v));
+    v = fmaxf(v, __shfl_xor(v, 16));
+    v = fmaxf(v, __shfl_xor(v, 32));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// inclusive prefix sum over the 64 lanes (Hillis-Steele on ds_bpermute)
+__device__ inline float wave_incl_scan(float v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float y = __shfl_up(v, d);
+        v += lane >= d ? y : 0.0f;
+    }
+    return v;
+}
+
 // (x * n) >> 32: integer in [0, n) from a 32-bit random word (Lemire's
 // multiply-shift range reduction, without the rejection step)
 __host__ __device__ inline uint32_t range32(uint32_t x, uint32_t n) {

@@ -12,7 +12,7 @@
 //   Metropolis   include/ensembles/MonteCarlo.hpp:133-177
 //   local energy include/operator/Operator.hpp:36-121
 //   O_k          include/quantum_state/PsiDeep.hpp:345-445, PsiRBM.hpp:159-176
-#include "kernels.hpp"
+#include "wave_net.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -22,75 +22,7 @@ namespace ann {
 
 namespace {
 
-constexpr int kBlock = 256;            // 4 wavefronts
-constexpr int kWavesPerBlock = kBlock / kWave;
-
-template <typename T>
-__device__ inline T uniform(T v) {
-    if constexpr (sizeof(T) == 4) {
-        return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-    } else {
-        const long long b = __double_as_longlong(v);
-        const int lo = __builtin_amdgcn_readfirstlane(int(b & 0xffffffff));
-        const int hi = __builtin_amdgcn_readfirstlane(int(b >> 32));
-        return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-    }
-}
-
-__device__ inline uint64_t uniform_u64(uint64_t v) {
-    const int lo = __builtin_amdgcn_readfirstlane(int(v & 0xffffffffu));
-    const int hi = __builtin_amdgcn_readfirstlane(int(v >> 32));
-    return (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo);
-}
-
-__device__ inline Conf load_conf(const uint64_t* confs, int64_t s, int words) {
-    Conf c;
-#pragma unroll
-    for (int w = 0; w < kMaxWords; w++) c.w[w] = w < words ? uniform_u64(confs[s * words + w]) : 0;
-    return c;
-}
-
-// Re(lc(z)) for the layer-0 polynomial, expanded in u = z^2 = a + i b:
-// Re(u/2 - u^2/12 + u^3/45) = a(1/2 - a/12 + a^2/45) + b^2 (1/12 - a/15)
-// with q = x y (b = 2q):  = a(1/2 - a/12 + a^2/45) + q^2 (1/3 - 4a/15)
-template <typename T>
-__device__ inline T logcosh0_re(cplx<T> z) {
-    const T a = z.re * z.re - z.im * z.im;
-    const T q = z.re * z.im;
-    const T q2 = q * q;
-    T p = a * T(1.0 / 45.0) - T(1.0 / 12.0);
-    p = p * a + T(0.5);
-    return p * a + q2 * (T(1.0 / 3.0) - a * T(4.0 / 15.0));
-}
-
-// Sum_j Re(f_j lc(theta_j)) over this lane's units
-template <typename T, int U, bool REALF>
-__device__ inline T lane_re_sum(const cplx<T> (&th)[U], const cplx<T> (&f)[U]) {
-    T s = 0;
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        if constexpr (REALF) {
-            s += f[u].re * logcosh0_re(th[u]);
-        } else {
-            const cplx<T> l = logcosh0(th[u]);
-            s += f[u].re * l.re - f[u].im * l.im;
-        }
-    }
-    return s;
-}
-
-template <typename T, int U>
-__device__ inline cplx<T> lane_sum(const cplx<T> (&th)[U], const cplx<T> (&f)[U]) {
-    cplx<T> s = {0, 0};
-#pragma unroll
-    for (int u = 0; u < U; u++) s = cadd(s, cmul(f[u], logcosh0(th[u])));
-    return s;
-}
-
-template <typename T>
-__device__ inline cplx<T> wave_csum(cplx<T> v) {
-    return {wave_sum(v.re), wave_sum(v.im)};
-}
+using namespace wave;
 
 // ----------------------------------------------------------------- kernels
 
@@ -204,25 +136,6 @@ __global__ __launch_bounds__(kBlock) void net_log_psi_kernel(DenseView<T> net, c
 // lane-parallel prefix count (mbcnt) and a ballot.  The configuration words live
 // in scalar registers; the row of Wd for the next proposal's site is loaded while
 // the current one is evaluated.
-template <typename T, int U>
-__device__ inline void load_row(cplx<T> (&w)[U], const cplx<T>* Wd, int row, int Mpad, int lane) {
-    const cplx<T>* p = Wd + int64_t(row) * Mpad + lane;
-#pragma unroll
-    for (int u = 0; u < U; u++) w[u] = p[64 * u];
-}
-
-// position of the rank-th set bit of the wave-uniform mask o (rank < popcount(o))
-__device__ inline int select64_wave(uint64_t o, uint32_t rank, int lane) {
-    const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(o >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(o), 0u));
-    const bool hit = ((o >> lane) & 1u) && below == rank;
-    const uint64_t m = __ballot(hit);
-    return __builtin_ctzll(m);
-}
-
-__device__ inline uint32_t readlane_u32(uint32_t v, int l) {
-    return uint32_t(__builtin_amdgcn_readlane(int(v), l));
-}
-
 template <typename T, int U, int MOVE, bool REALF, int WORDS>
 __global__ __launch_bounds__(kBlock) void mc_sweep_kernel(DenseView<T> net, uint64_t* chain_confs,
                                                           cplx<T>* theta, int num_chains,
@@ -629,26 +542,6 @@ __global__ __launch_bounds__(kBlock) void structured_Ov_kernel(DenseView<T> net,
     if (lane == 0) u_out[s] = {re, im};
 }
 
-template <int U>
-struct UTag {
-    static constexpr int value = U;
-};
-
-// dispatch on the number of hidden units per lane
-template <typename F>
-void dispatch_u(int U, F&& fn) {
-    switch (U) {
-        case 1: fn(UTag<1>{}); break;
-        case 2: fn(UTag<2>{}); break;
-        case 4: fn(UTag<4>{}); break;
-        case 7: fn(UTag<7>{}); break;
-        case 8: fn(UTag<8>{}); break;
-        case 16: fn(UTag<16>{}); break;
-        default: throw HipError("unsupported hidden width (units per lane " + std::to_string(U) + ")");
-    }
-}
-
-inline unsigned grid_waves(int64_t n) { return unsigned((n + kWavesPerBlock - 1) / kWavesPerBlock); }
 
 uint16_t to_bf16(float x) {
     uint32_t b;

@@ -9,9 +9,10 @@ namespace ann {
 void launch_init_chains(uint64_t* confs, int num_chains, int words, int N, int move, int total_z,
                         uint32_t call_index, uint64_t seed, hipStream_t stream);
 
+// inv[vis][j] ([N][Mpad]): the unique i with conn(i, j) = vis, -1 if none, -2 if several
 template <typename T>
-void launch_unpack_v(const DenseView<T>& net, const cplx<double>* v, cplx<double>* va, cplx<double>* vb,
-                     cplx<T>* V, hipStream_t stream);
+void launch_unpack_v(const DenseView<T>& net, const int* inv, const cplx<double>* v, cplx<double>* va,
+                     cplx<double>* vb, cplx<T>* V, hipStream_t stream);
 
 void launch_pack_y(int layout, int N, int M, int Mpad, int C, const uint32_t* conn, const cplx<double>* Ya,
                    const cplx<double>* Yb, const cplx<double>* Yw, const cplx<double>* v, double shift, int P,
@@ -21,6 +22,14 @@ void launch_pack_fragments(const cplx<float>* V, int N, int Mpad, int npad, uint
 
 void launch_cdot(const cplx<double>* a, const cplx<double>* b, int64_t n, int conj_a, double* out2,
                  hipStream_t stream);
+// out4 = (<a,b>.re, <a,b>.im, <c,d>.re, <c,d>.im), conjugating a and c when conj_a;
+// scratch holds 256 doubles
+void launch_dot2(const cplx<double>* a, const cplx<double>* b, const cplx<double>* c, const cplx<double>* d,
+                 int64_t n, int conj_a, double* scratch, double* out4, hipStream_t stream);
+void launch_cg_update_dev(cplx<double>* x, cplx<double>* r, const cplx<double>* p, const cplx<double>* Ap,
+                          const double* rz, const double* pAp, int64_t n, hipStream_t stream);
+void launch_cg_dir_dev(cplx<double>* p, const cplx<double>* z, const double* rz_new, const double* rz_old, int64_t n,
+                       hipStream_t stream);
 void launch_center_scale(cplx<double>* u, const double* w, const double* c2, int64_t n, hipStream_t stream);
 void launch_cg_step(cplx<double>* x, cplx<double>* r, const cplx<double>* p, const cplx<double>* Ap, double alpha,
                     int64_t n, hipStream_t stream);

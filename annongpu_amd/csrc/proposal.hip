@@ -1,0 +1,496 @@
+// proposal.hip -- learned block proposal: fp32 training kernels (batch build,
+// LDS-tiled SGEMM forward/backward, loss, Adam) and the Metropolis-Hastings
+// block sweep on gfx950.
+//
+// The block sweep keeps, per chain (one wavefront), the RBM angles in VGPRs as
+// the single-spin sweep does, plus the first-layer pre-activations of the
+// proposal network for the whole configuration (4 hidden units per lane) in
+// exact 32-bit fixed point.  Masking a block subtracts its sites' columns; the
+// integer sums do not depend on the order of updates, so the logits are an exact
+// function of the spins outside the block and the proposal ratio of a move and
+// its reverse is exactly exp(e_old - e_new).
+#include "proposal.hpp"
+#include "wave_net.hpp"
+
+#include <cmath>
+#include <type_traits>
+
+namespace ann {
+
+namespace {
+
+using namespace wave;
+
+constexpr int kHPad = 256;   // hidden units of the inference form (4 per lane)
+
+__device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+__device__ inline uint16_t f32_to_bf16(float x) {
+    const uint32_t b = __float_as_uint(x);
+    return uint16_t((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
+template <int WORDS>
+__device__ inline uint32_t conf_bit(const uint64_t (&w)[WORDS], int i) {
+    uint64_t x = w[0];
+#pragma unroll
+    for (int k = 1; k < WORDS; k++) x = (i >> 6) == k ? w[k] : x;
+    return uint32_t((x >> (i & 63)) & 1u);
+}
+
+template <int WORDS>
+__device__ inline void conf_flip(uint64_t (&w)[WORDS], int i) {
+#pragma unroll
+    for (int k = 0; k < WORDS; k++) w[k] ^= (i >> 6) == k ? uint64_t(1) << (i & 63) : 0;
+}
+
+// ------------------------------------------------------------- block sweep
+template <typename T, int U, int WORDS>
+__global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, BlockNet bn, uint64_t* chain_confs,
+                                                             cplx<T>* theta, int num_chains, int64_t num_proposals,
+                                                             uint64_t counter, uint32_t key0, uint32_t key1,
+                                                             uint32_t* accepted) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * kWavesPerBlock + uniform_int(threadIdx.x >> 6);
+    if (c >= num_chains) return;
+    const int N = net.N;
+    const int bs = bn.bs;
+    uint64_t w[WORDS];
+#pragma unroll
+    for (int k = 0; k < WORDS; k++) w[k] = uniform_u64(chain_confs[int64_t(c) * WORDS + k]);
+    cplx<T> th[U], f[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        th[u] = theta[int64_t(c) * net.Mpad + lane + 64 * u];
+        f[u] = net.f[lane + 64 * u];
+    }
+    const bool realf = net.real_f != 0;
+    auto log_amp = [&](const cplx<T>(&z)[U]) {
+        return wave_sum(realf ? lane_re_sum<T, U, true>(z, f) : lane_re_sum<T, U, false>(z, f));
+    };
+    T cur = log_amp(th);
+    // first-layer pre-activations of the whole configuration
+    int32_t a1[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) a1[q] = bn.b1q[lane + 64 * q];
+    for (int i = 0; i < N; i++) {
+        const int32_t* col = bn.W1q + int64_t(i) * kHPad + lane;
+        const bool up = conf_bit(w, i);
+#pragma unroll
+        for (int q = 0; q < 4; q++) a1[q] += up ? col[64 * q] : -col[64 * q];
+    }
+    uint32_t acc = 0;
+    uint32_t rx = 0, ry = 0, rz = 0;
+    for (int64_t q = 0; q < num_proposals; q++) {
+        const int ql = int(q & 63);
+        if (ql == 0) {
+            const uint64_t t = counter + uint64_t(q) + uint64_t(lane);
+            const u32x4 rr = philox(uint32_t(c), uint32_t(t), uint32_t(t >> 32), 3u, key0, key1);
+            rx = rr.x;
+            ry = rr.y;
+            rz = rr.z;
+        }
+        const uint32_t r_x = readlane_u32(rx, ql), r_y = readlane_u32(ry, ql), r_z = readlane_u32(rz, ql);
+        const int b = int(range32(r_x, uint32_t(bn.num_blocks)));
+        const int* bsites = bn.blocks + int64_t(b) * bs;
+        int st[kMaxBlock];
+        uint32_t p_old = 0;
+#pragma unroll
+        for (int j = 0; j < kMaxBlock; j++) {
+            st[j] = j < bs ? bsites[j] : 0;
+            p_old |= j < bs ? conf_bit(w, st[j]) << j : 0u;
+        }
+        const int k = __popc(p_old);
+        // hidden units of the masked input
+        float hd[4];
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+            int32_t a = a1[qq];
+#pragma unroll
+            for (int j = 0; j < kMaxBlock; j++) {
+                if (j < bs) {
+                    const int32_t wv = bn.W1q[int64_t(st[j]) * kHPad + lane + 64 * qq];
+                    a -= (p_old >> j) & 1u ? wv : -wv;
+                }
+            }
+            hd[qq] = tanhf(float(a) * (1.0f / kFixScale));
+        }
+        // logits of the block's sites
+        float lg[kMaxBlock];
+#pragma unroll
+        for (int j = 0; j < kMaxBlock; j++) {
+            lg[j] = 0.f;
+            if (j < bs) {
+                const uint16_t* row = bn.W2h + int64_t(st[j]) * kHPad + lane;
+                float part = 0.f;
+#pragma unroll
+                for (int qq = 0; qq < 4; qq++) part += bf16_to_f32(row[64 * qq]) * hd[qq];
+                lg[j] = wave_sum(part) + bn.b2[st[j]];
+            }
+        }
+        // patterns 4 lane + r with k up spins; weights exp(e_p - e_max)
+        float e[4], wr[4];
+        bool ok[4];
+        float emax_l = -INFINITY;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint32_t p = uint32_t(4 * lane + r);
+            ok[r] = p < (1u << bs) && __popc(p) == k;
+            float s = 0.f;
+#pragma unroll
+            for (int j = 0; j < kMaxBlock; j++) s += (p >> j) & 1u ? lg[j] : 0.f;
+            e[r] = s;
+            if (ok[r]) emax_l = fmaxf(emax_l, s);
+        }
+        const float emax = wave_max(emax_l);
+        float lsum = 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            wr[r] = ok[r] ? __expf(e[r] - emax) : 0.f;
+            lsum += wr[r];
+        }
+        const float incl = wave_incl_scan(lsum, lane);
+        const float Z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(incl), 63));
+        const float target = (float(r_y) + 0.5f) * 2.3283064365386963e-10f * Z;
+        const uint64_t hit = __ballot(incl > target);
+        const uint64_t nz = __ballot(lsum > 0.f);
+        const int L = hit ? int(__builtin_ctzll(hit)) : nz ? 63 - int(__builtin_clzll(nz)) : 0;
+        float run = incl - lsum;
+        int pick = -1, last = -1;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (wr[r] > 0.f) {
+                run += wr[r];
+                last = 4 * lane + r;
+                if (pick < 0 && run > target) pick = 4 * lane + r;
+            }
+        }
+        if (pick < 0) pick = last;
+        if (pick < 0) pick = int(p_old);
+        const uint32_t p_new = uint32_t(__builtin_amdgcn_readlane(pick, L)) & ((1u << bs) - 1u);
+        const uint32_t flips = p_old ^ p_new;
+        if (flips == 0) {   // the current pattern was redrawn: accepted, nothing changes
+            acc++;
+            continue;
+        }
+        float e_old = 0.f, e_new = 0.f;
+#pragma unroll
+        for (int j = 0; j < kMaxBlock; j++) {
+            e_old += (p_old >> j) & 1u ? lg[j] : 0.f;
+            e_new += (p_new >> j) & 1u ? lg[j] : 0.f;
+        }
+        // psi ratio: theta' = theta - 2 s_j W_j over the flipped sites
+        cplx<T> z[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) z[u] = th[u];
+#pragma unroll
+        for (int j = 0; j < kMaxBlock; j++) {
+            if ((flips >> j) & 1u) {
+                const T cj = (p_old >> j) & 1u ? T(-2) : T(2);
+                cplx<T> row[U];
+                load_row<T, U>(row, net.Wd, st[j], net.Mpad, lane);
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    z[u].re += cj * row[u].re;
+                    z[u].im += cj * row[u].im;
+                }
+            }
+        }
+        const T nxt = log_amp(z);
+        const T d = T(2) * (nxt - cur) + T(e_old - e_new);
+        const T uu = (T(r_z) + T(0.5)) * T(2.3283064365386963e-10);
+        const bool accept = d > T(0) || uu <= dexp(d);
+        if (accept) {
+#pragma unroll
+            for (int u = 0; u < U; u++) th[u] = z[u];
+            cur = nxt;
+#pragma unroll
+            for (int j = 0; j < kMaxBlock; j++) {
+                if ((flips >> j) & 1u) {
+                    conf_flip(w, st[j]);
+                    const bool was_up = (p_old >> j) & 1u;
+                    const int32_t* col = bn.W1q + int64_t(st[j]) * kHPad + lane;
+#pragma unroll
+                    for (int qq = 0; qq < 4; qq++) a1[qq] += was_up ? -2 * col[64 * qq] : 2 * col[64 * qq];
+                }
+            }
+            acc++;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) theta[int64_t(c) * net.Mpad + lane + 64 * u] = th[u];
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < WORDS; k++) chain_confs[int64_t(c) * WORDS + k] = w[k];
+        accepted[c] += acc;
+    }
+}
+
+__global__ void block_net_pack_kernel(const float* params, int N, int H, int32_t* W1q, int32_t* b1q, uint16_t* W2h,
+                                      float* b2) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;   // e = i * 256 + h
+    if (e >= N * kHPad) return;
+    const int i = e / kHPad, h = e % kHPad;
+    const float* W1 = params;
+    const float* bb1 = W1 + int64_t(H) * N;
+    const float* W2 = bb1 + H;
+    const float* bb2 = W2 + int64_t(N) * H;
+    // bf16-rounded weights, clamped to |w| <= 4 so that 257 terms stay inside int32
+    auto fix = [](float x) {
+        const float r = bf16_to_f32(f32_to_bf16(fminf(fmaxf(x, -4.f), 4.f)));
+        return int32_t(rintf(r * kFixScale));
+    };
+    W1q[e] = h < H ? fix(W1[int64_t(h) * N + i]) : 0;
+    W2h[e] = h < H ? f32_to_bf16(W2[int64_t(i) * H + h]) : uint16_t(0);
+    if (i == 0) b1q[h] = h < H ? fix(bb1[h]) : 0;
+    if (h == 0) b2[i] = bb2[i];
+}
+
+// ------------------------------------------------------------------ training
+__global__ void draw_batch_kernel(int Bt, int64_t* sample, int* sites, int64_t count, const int* blocks,
+                                  int num_blocks, int bs, uint32_t step_lo, uint32_t step_hi, uint32_t key0,
+                                  uint32_t key1) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= Bt) return;
+    const u32x4 x = philox(uint32_t(r), step_lo, step_hi, 4u, key0, key1);
+    // 64-bit range reduction of the sample index for counts beyond 2^32
+    const uint64_t wide = (uint64_t(x.x) << 32) | x.z;
+    sample[r] = int64_t(__umul64hi(wide, uint64_t(count)));
+    const int b = int(range32(x.y, uint32_t(num_blocks)));
+    for (int j = 0; j < bs; j++) sites[r * bs + j] = blocks[b * bs + j];
+}
+
+__global__ void build_inputs_kernel(TrainBatch b, const uint64_t* confs, int words, int N, int bs) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= int64_t(b.Bt) * N) return;
+    const int r = int(e / N), i = int(e % N);
+    const uint64_t word = confs[b.sample[r] * words + (i >> 6)];
+    const bool up = (word >> (i & 63)) & 1u;
+    bool in_block = false;
+    for (int j = 0; j < bs; j++) in_block = in_block || b.sites[r * bs + j] == i;
+    b.X[e] = in_block ? 0.f : (up ? 1.f : -1.f);
+    b.Y[e] = up ? 1.f : 0.f;
+    b.Mask[e] = in_block ? 1.f : 0.f;
+}
+
+// C[m][n] = sum_k A(m, k) B(k, n) (+ bias[n], then tanh), fp32; 64x64 tiles, 4x4 per thread
+constexpr int kTK = 16;
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void sgemm_kernel(int M, int Nn, int K, const float* __restrict__ A, int lda,
+                                                    const float* __restrict__ B, int ldb,
+                                                    const float* __restrict__ bias, float* __restrict__ C, int ldc) {
+    __shared__ float As[kTK][64 + 4];
+    __shared__ float Bs[kTK][64 + 4];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.f;
+    for (int k0 = 0; k0 < K; k0 += kTK) {
+        for (int e = threadIdx.x; e < 64 * kTK; e += 256) {
+            int m, k;
+            if constexpr (TA) {
+                k = e / 64;
+                m = e % 64;
+            } else {
+                m = e / kTK;
+                k = e % kTK;
+            }
+            const int gm = m0 + m, gk = k0 + k;
+            float v = 0.f;
+            if (gm < M && gk < K) v = TA ? A[int64_t(gk) * lda + gm] : A[int64_t(gm) * lda + gk];
+            As[k][m] = v;
+            int n;
+            if constexpr (TB) {
+                n = e / kTK;
+                k = e % kTK;
+            } else {
+                k = e / 64;
+                n = e % 64;
+            }
+            const int gn = n0 + n, gk2 = k0 + k;
+            float u = 0.f;
+            if (gn < Nn && gk2 < K) u = TB ? B[int64_t(gn) * ldb + gk2] : B[int64_t(gk2) * ldb + gn];
+            Bs[k][n] = u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTK; k++) {
+            float a[4], bb[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = As[k][ty * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bb[j] = Bs[k][tx * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = fmaf(a[i], bb[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int m = m0 + ty * 4 + i;
+        if (m >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int n = n0 + tx * 4 + j;
+            if (n >= Nn) continue;
+            float v = acc[i][j];
+            if constexpr (EPI >= 1) v += bias[n];
+            if constexpr (EPI == 2) v = tanhf(v);
+            C[int64_t(m) * ldc + n] = v;
+        }
+    }
+}
+
+template <bool TA, bool TB, int EPI>
+void sgemm(int M, int Nn, int K, const float* A, int lda, const float* B, int ldb, const float* bias, float* C,
+           int ldc, hipStream_t stream) {
+    const dim3 grid(unsigned((Nn + 63) / 64), unsigned((M + 63) / 64));
+    sgemm_kernel<TA, TB, EPI><<<grid, 256, 0, stream>>>(M, Nn, K, A, lda, B, ldb, bias, C, ldc);
+    ANN_LAUNCH_CHECK();
+}
+
+// one wavefront per row: d loss / d logits in place, per-row loss in fp64
+__global__ __launch_bounds__(256) void loss_kernel(TrainBatch b, int N, double norm) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= b.Bt) return;
+    double loss = 0.0;
+    for (int o = lane; o < N; o += 64) {
+        const int64_t e = int64_t(r) * N + o;
+        float d = 0.f;
+        if (b.Mask[e] != 0.f) {
+            const float l = b.A2[e], y = b.Y[e];
+            const float sp = l > 0.f ? l + log1pf(__expf(-l)) : log1pf(__expf(l));
+            loss += double(sp - y * l);
+            d = (1.f / (1.f + __expf(-l)) - y) * float(1.0 / norm);
+        }
+        b.A2[e] = d;
+    }
+    loss = wave_sum(loss);
+    if (lane == 0) b.loss_part[r] = loss / norm;
+}
+
+__global__ void dtanh_kernel(float* dH, const float* Hd, int64_t n) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < n) dH[e] *= 1.f - Hd[e] * Hd[e];
+}
+
+// out[n] = sum_r D[r][n]: 256 threads = 64 columns x 4 row phases, LDS fold
+__global__ __launch_bounds__(256) void colsum_kernel(const float* D, int rows, int cols, float* out) {
+    __shared__ float part[4][64];
+    const int n = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int ph = threadIdx.x >> 6;
+    float s = 0.f;
+    if (n < cols)
+        for (int r = ph; r < rows; r += 4) s += D[int64_t(r) * cols + n];
+    part[ph][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (ph == 0 && n < cols) out[n] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                                      part[3][threadIdx.x];
+}
+
+__global__ void adam_kernel(float* p, float* m, float* v, const float* g, int64_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.f - b1) * gk;
+    const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+}
+
+inline unsigned blocks_of(int64_t n, int per = 256) { return unsigned((n + per - 1) / per); }
+
+}  // namespace
+
+void launch_block_net_pack(const float* params, int N, int H, int32_t* W1q, int32_t* b1q, uint16_t* W2h, float* b2,
+                           hipStream_t stream) {
+    if (H > kHPad) throw std::invalid_argument("the proposal network has at most 256 hidden units");
+    block_net_pack_kernel<<<blocks_of(int64_t(N) * kHPad), 256, 0, stream>>>(params, N, H, W1q, b1q, W2h, b2);
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_block_sweep(const DenseView<T>& net, const BlockNet& bn, uint64_t* chain_confs, cplx<T>* theta,
+                        int num_chains, int words, int64_t num_proposals, uint64_t counter, uint64_t seed,
+                        uint32_t* accepted, hipStream_t stream) {
+    if (num_chains <= 0 || num_proposals <= 0) return;
+    if (bn.bs < 1 || bn.bs > kMaxBlock) throw std::invalid_argument("block size must be 1..8");
+    if (bn.N != net.N) throw std::invalid_argument("proposal and wave function have different sizes");
+    const uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
+    dispatch_u(net.Mpad / 64, [&](auto tag) {
+        constexpr int U = std::decay_t<decltype(tag)>::value;
+        const unsigned grid = grid_waves(num_chains);
+        switch (words) {
+            case 1: block_sweep_kernel<T, U, 1><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted); break;
+            case 2: block_sweep_kernel<T, U, 2><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted); break;
+            case 3: block_sweep_kernel<T, U, 3><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted); break;
+            case 4: block_sweep_kernel<T, U, 4><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted); break;
+            default: throw HipError("configurations of 1..4 words are supported");
+        }
+    });
+    ANN_LAUNCH_CHECK();
+}
+
+template void launch_block_sweep<float>(const DenseView<float>&, const BlockNet&, uint64_t*, cplx<float>*, int, int,
+                                        int64_t, uint64_t, uint64_t, uint32_t*, hipStream_t);
+template void launch_block_sweep<double>(const DenseView<double>&, const BlockNet&, uint64_t*, cplx<double>*, int,
+                                         int, int64_t, uint64_t, uint64_t, uint32_t*, hipStream_t);
+
+void launch_draw_batch(const TrainBatch& b, int64_t count, const int* blocks, int num_blocks, int bs, uint64_t step,
+                       uint64_t seed, hipStream_t stream) {
+    draw_batch_kernel<<<blocks_of(b.Bt), 256, 0, stream>>>(b.Bt, b.sample, b.sites, count, blocks, num_blocks, bs,
+                                                           uint32_t(step), uint32_t(step >> 32), uint32_t(seed),
+                                                           uint32_t(seed >> 32));
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_build_inputs(const TrainBatch& b, const uint64_t* confs, int words, int N, int bs, hipStream_t stream) {
+    build_inputs_kernel<<<blocks_of(int64_t(b.Bt) * N), 256, 0, stream>>>(b, confs, words, N, bs);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_loss_grad(const TrainBatch& b, const float* params, int N, int H, double count_norm, float* grad,
+                      hipStream_t stream) {
+    const float* W1 = params;
+    const float* b1 = W1 + int64_t(H) * N;
+    const float* W2 = b1 + H;
+    const float* b2 = W2 + int64_t(N) * H;
+    float* gW1 = grad;
+    float* gb1 = gW1 + int64_t(H) * N;
+    float* gW2 = gb1 + H;
+    float* gb2 = gW2 + int64_t(N) * H;
+    const int Bt = b.Bt;
+    // forward: Hd = tanh(X W1^T + b1), A2 = Hd W2^T + b2
+    sgemm<false, true, 2>(Bt, H, N, b.X, N, W1, N, b1, b.Hd, H, stream);
+    sgemm<false, true, 1>(Bt, N, H, b.Hd, H, W2, H, b2, b.A2, N, stream);
+    loss_kernel<<<blocks_of(Bt, 4), 256, 0, stream>>>(b, N, count_norm);
+    ANN_LAUNCH_CHECK();
+    // backward
+    sgemm<true, false, 0>(N, H, Bt, b.A2, N, b.Hd, H, nullptr, gW2, H, stream);
+    colsum_kernel<<<blocks_of(N, 64), 256, 0, stream>>>(b.A2, Bt, N, gb2);
+    ANN_LAUNCH_CHECK();
+    sgemm<false, false, 0>(Bt, H, N, b.A2, N, W2, H, nullptr, b.dH, H, stream);
+    dtanh_kernel<<<blocks_of(int64_t(Bt) * H), 256, 0, stream>>>(b.dH, b.Hd, int64_t(Bt) * H);
+    ANN_LAUNCH_CHECK();
+    sgemm<true, false, 0>(H, N, Bt, b.dH, H, b.X, N, nullptr, gW1, N, stream);
+    colsum_kernel<<<blocks_of(H, 64), 256, 0, stream>>>(b.dH, Bt, H, gb1);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_adam(float* params, float* m, float* v, const float* grad, int64_t n, float lr, float beta1, float beta2,
+                 float eps, int64_t t, hipStream_t stream) {
+    const float c1 = float(1.0 - std::pow(double(beta1), double(t)));
+    const float c2 = float(1.0 - std::pow(double(beta2), double(t)));
+    adam_kernel<<<blocks_of(n), 256, 0, stream>>>(params, m, v, grad, n, lr, beta1, beta2, eps, c1, c2);
+    ANN_LAUNCH_CHECK();
+}
+
+}  // namespace ann

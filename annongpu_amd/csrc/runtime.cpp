@@ -87,6 +87,8 @@ void Runtime::collect() {
     pending_.clear();
 }
 
+hipStream_t lib_stream() { return Runtime::get().stream(); }
+
 static hipStream_t S() { return Runtime::get().stream(); }
 
 // --------------------------------------------------------------------- Psi
@@ -128,6 +130,29 @@ void Psi::finalize() {
 
 void Psi::sync_device() {
     if (!dirty) return;
+    if (kind == PSI_CNN) {
+        // parameters, neighbour tables (Convolve.hpp:122-148) and symmetry classes
+        cp64.upload(reinterpret_cast<const cplx<double>*>(params.data()), params.size());
+        std::vector<cplx<float>> p32(params.size());
+        for (size_t i = 0; i < params.size(); i++) p32[i] = {float(params[i].real()), float(params[i].imag())};
+        cp32.upload(p32.data(), p32.size());
+        const int e0 = extent[0], e1 = extent[1], e2 = extent[2], page = e1 * e2;
+        std::vector<int> nbr;
+        for (int l = 0; l < cnn_layers; l++) {
+            const int* cv = &connectivity[3 * l];
+            for (int x = 0; x < N; x++) {
+                const int pg = x / page, row = (x % page) / e2, col = x % e2;
+                for (int k = 0; k < cv[0]; k++)
+                    for (int i = 0; i < cv[1]; i++)
+                        for (int j = 0; j < cv[2]; j++)
+                            nbr.push_back(((pg + k) % e0) * page + ((row + i) % e1) * e2 + (col + j) % e2);
+            }
+        }
+        nbr_d.upload(nbr.data(), nbr.size());
+        sym_d.upload(sym.data(), sym.size());
+        dirty = false;
+        return;
+    }
     if (!dense()) {
         dirty = false;
         return;
@@ -181,6 +206,46 @@ DenseView<float> Psi::view32() {
     sync_device();
     return {N, M, Mpad, C, layout, num_params, real_f ? 1 : 0, Wd32.p, b32.p, f32.p,
             kind == PSI_DEEP ? conn_d.p : nullptr};
+}
+
+template <typename T>
+static CnnView<T> make_cnn_view(const Psi& q, const cplx<T>* p, const int* nbr, const uint32_t* sym) {
+    CnnView<T> v{};
+    v.N = q.N;
+    v.L = q.cnn_layers;
+    v.nsym = q.num_sym;
+    v.num_params = q.num_params;
+    v.max_ch = 1;
+    int link = 0, noff = 0, zoff = 0, prev = 1;
+    for (int l = 0; l < q.cnn_layers; l++) {
+        const int* cv = &q.connectivity[3 * l];
+        v.ch[l] = q.channels[l];
+        v.vol[l] = cv[0] * cv[1] * cv[2];
+        v.link_off[l] = link;
+        v.nbr_off[l] = noff;
+        v.zoff[l] = zoff;
+        link += prev * v.ch[l] * v.nsym * v.vol[l];
+        noff += q.N * v.vol[l];
+        zoff += v.ch[l] * q.N;
+        v.max_ch = std::max(v.max_ch, v.ch[l]);
+        prev = v.ch[l];
+    }
+    v.zoff[q.cnn_layers] = zoff;
+    v.final_factor = T(q.final_factor);
+    v.params = p;
+    v.nbr = nbr;
+    v.sym = sym;
+    return v;
+}
+
+CnnView<double> Psi::cnn64() {
+    sync_device();
+    return make_cnn_view<double>(*this, cp64.p, nbr_d.p, sym_d.p);
+}
+
+CnnView<float> Psi::cnn32() {
+    sync_device();
+    return make_cnn_view<float>(*this, cp32.p, nbr_d.p, sym_d.p);
 }
 
 std::vector<cd> Psi::get_params() const {
@@ -342,45 +407,33 @@ static void require_dense(const Psi& psi) {
 static void prepare_samples(Psi& psi, Samples& s, int64_t count) {
     s.count = count;
     s.words = psi.words;
-    s.Mpad = psi.Mpad;
+    s.Mpad = psi.dense() ? psi.Mpad : 0;
     s.precision = psi.precision;
     s.confs.ensure(size_t(count) * psi.words);
     s.w.ensure(count);
     s.log_psi.ensure(count);
+    if (!psi.dense()) return;
     if (psi.precision == 0)
         s.th64.ensure(size_t(count) * psi.Mpad);
     else
         s.th32.ensure(size_t(count) * psi.Mpad);
 }
 
-void sample(Psi& psi, Ensemble& ens) {
-    require_dense(psi);
-    psi.sync_device();
-    Samples& s = ens.samples;
-    const int words = psi.words;
-    if (ens.kind == ENS_EXACT) {
-        if (int(ens.num_sites) != psi.num_sites)
-            throw std::invalid_argument("ExactSummation num_sites does not match the wave function");
-        if (ens.num_sites > 26) throw std::invalid_argument("exact summation is limited to 26 sites");
-        const int64_t count = int64_t(1) << ens.num_sites;
-        prepare_samples(psi, s, count);
-        launch_enumerate_confs(s.confs.p, 0, count, words, S());
-        compute_theta(psi, s.confs.p, count, s, 0);
-        if (psi.precision == 0)
-            launch_net_log_psi<double>(psi.view64(), s.th64.p, count, {psi.log_prefactor.real(), psi.log_prefactor.imag()},
-                                       s.log_psi.p, s.w.p, S());
-        else
-            launch_net_log_psi<float>(psi.view32(), s.th32.p, count, {psi.log_prefactor.real(), psi.log_prefactor.imag()},
-                                      s.log_psi.p, s.w.p, S());
-        return;
-    }
+static int64_t exact_count(const Psi& psi, const Ensemble& ens) {
+    if (int(ens.num_sites) != psi.num_sites)
+        throw std::invalid_argument("ExactSummation num_sites does not match the wave function");
+    if (ens.num_sites > 26) throw std::invalid_argument("exact summation is limited to 26 sites");
+    return int64_t(1) << ens.num_sites;
+}
+
+// chain set-up shared by the samplers: (re)initialises the chains unless they
+// persist from the previous call; returns the thermalisation proposals per chain
+static int64_t start_chains(const Psi& psi, Ensemble& ens, int64_t* steps) {
     const int B = int(ens.num_chains);
     if (B <= 0) throw std::invalid_argument("num_markov_chains must be positive");
-    const int64_t steps = ens.num_samples / ens.num_chains;
-    if (steps <= 0) throw std::invalid_argument("num_samples must be at least num_markov_chains");
-    const int64_t count = steps * B;
-    prepare_samples(psi, s, count);
-    const int N = psi.N;
+    *steps = ens.num_samples / ens.num_chains;
+    if (*steps <= 0) throw std::invalid_argument("num_samples must be at least num_markov_chains");
+    const int words = psi.words, N = psi.N;
     const bool reinit = !ens.persistent || !ens.chains_valid || ens.chain_words != words || ens.chain_N != N;
     ens.chains.ensure(size_t(B) * words);
     ens.acc.ensure(B);
@@ -391,6 +444,101 @@ void sample(Psi& psi, Ensemble& ens) {
         launch_init_chains(ens.chains.p, B, words, N, ens.move, ens.total_z, ens.call_index, ens.seed, S());
         ens.call_index++;
     }
+    return reinit ? int64_t(ens.num_therm) * N : 0;
+}
+
+static void finish_chains(const Psi& psi, Ensemble& ens, int64_t therm, int64_t steps, int64_t per_step) {
+    const int B = int(ens.num_chains);
+    ens.chains_valid = true;
+    ens.chain_words = psi.words;
+    ens.chain_N = psi.N;
+    std::vector<uint32_t> acc(B);
+    ANN_HIP_CHECK(hipMemcpyAsync(acc.data(), ens.acc.p, sizeof(uint32_t) * B, hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    uint64_t total_acc = 0;
+    for (uint32_t x : acc) total_acc += x;
+    const uint64_t total = uint64_t(B) * uint64_t(therm + steps * per_step);
+    ens.accepted = total_acc;
+    ens.rejected = total - total_acc;
+}
+
+// log psi (and |psi|^2 weights when w is given) of configurations, any wave function
+void log_psi_of(Psi& psi, const uint64_t* confs, int64_t count, Samples& s, double* w) {
+    const cplx<double> lp = {psi.log_prefactor.real(), psi.log_prefactor.imag()};
+    if (psi.kind == PSI_CNN) {
+        ScopedTimer timer("log_psi");
+        if (psi.precision == 0)
+            launch_cnn_log_psi<double>(psi.cnn64(), confs, psi.words, count, lp, s.log_psi.p, w, S());
+        else
+            launch_cnn_log_psi<float>(psi.cnn32(), confs, psi.words, count, lp, s.log_psi.p, w, S());
+        return;
+    }
+    require_dense(psi);
+    compute_theta(psi, confs, count, s, 0);
+    if (psi.precision == 0)
+        launch_net_log_psi<double>(psi.view64(), s.th64.p, count, lp, s.log_psi.p, w, S());
+    else
+        launch_net_log_psi<float>(psi.view32(), s.th32.p, count, lp, s.log_psi.p, w, S());
+}
+
+// PsiCNN: Metropolis with a full forward pass per proposal (the reference keeps
+// no incremental state for PsiCNN, include/quantum_state/PsiCNN.hpp:177-181)
+static void sample_cnn(Psi& psi, Ensemble& ens) {
+    psi.sync_device();
+    Samples& s = ens.samples;
+    if (ens.kind == ENS_EXACT) {
+        const int64_t count = exact_count(psi, ens);
+        prepare_samples(psi, s, count);
+        launch_enumerate_confs(s.confs.p, 0, count, psi.words, S());
+        log_psi_of(psi, s.confs.p, count, s, s.w.p);
+        return;
+    }
+    if (ens.proposal) throw std::logic_error("the learned block proposal needs a PsiRBM or one-layer PsiDeep");
+    const int B = int(ens.num_chains);
+    int64_t steps = 0;
+    const int64_t therm = start_chains(psi, ens, &steps);
+    const int64_t count = steps * B;
+    prepare_samples(psi, s, count);
+    const int words = psi.words;
+    for (int64_t step = 0; step < steps; step++) {
+        const int64_t nprop = (step == 0 ? therm : 0) + int64_t(ens.num_sweeps) * psi.N;
+        {
+            ScopedTimer timer("mc_sweep");
+            if (psi.precision == 0)
+                launch_cnn_mc<double>(psi.cnn64(), ens.chains.p, words, B, ens.move, nprop, ens.counter, ens.seed,
+                                      ens.acc.p, S());
+            else
+                launch_cnn_mc<float>(psi.cnn32(), ens.chains.p, words, B, ens.move, nprop, ens.counter, ens.seed,
+                                     ens.acc.p, S());
+        }
+        ens.counter += uint64_t(nprop);
+        ANN_HIP_CHECK(hipMemcpyAsync(s.confs.p + step * B * words, ens.chains.p, sizeof(uint64_t) * B * words,
+                                     hipMemcpyDeviceToDevice, S()));
+    }
+    launch_fill(s.w.p, 1.0 / double(ens.num_samples), count, S());
+    log_psi_of(psi, s.confs.p, count, s, nullptr);
+    finish_chains(psi, ens, therm, steps, int64_t(ens.num_sweeps) * psi.N);
+}
+
+void sample(Psi& psi, Ensemble& ens) {
+    if (psi.kind == PSI_CNN) return sample_cnn(psi, ens);
+    require_dense(psi);
+    psi.sync_device();
+    Samples& s = ens.samples;
+    const int words = psi.words;
+    if (ens.kind == ENS_EXACT) {
+        const int64_t count = exact_count(psi, ens);
+        prepare_samples(psi, s, count);
+        launch_enumerate_confs(s.confs.p, 0, count, words, S());
+        log_psi_of(psi, s.confs.p, count, s, s.w.p);
+        return;
+    }
+    const int B = int(ens.num_chains);
+    int64_t steps = 0;
+    const int64_t therm = start_chains(psi, ens, &steps);
+    const int64_t count = steps * B;
+    prepare_samples(psi, s, count);
+    const int N = psi.N;
     // fresh angles of the chains
     Samples chain_view;
     if (psi.precision == 0) {
@@ -401,10 +549,20 @@ void sample(Psi& psi, Ensemble& ens) {
         chain_view.th32.p = ens.chain_th32.p;
     }
     compute_theta(psi, ens.chains.p, B, chain_view, 0);
-    const int64_t therm = reinit ? int64_t(ens.num_therm) * N : 0;
+    // with a learned proposal a sample interval is num_sweeps block sweeps
+    // (ceil(N / block size) block moves each) plus plain_sweeps plain sweeps
+    Proposal* prop = ens.proposal;
+    BlockNet bn{};
+    int64_t plain_per_step = int64_t(ens.num_sweeps) * N, block_per_step = 0;
+    if (prop) {
+        if (prop->N != N) throw std::invalid_argument("the proposal was built for another number of sites");
+        bn = prop->view();
+        plain_per_step = int64_t(ens.plain_sweeps) * N;
+        block_per_step = int64_t(ens.num_sweeps) * ((N + prop->bs - 1) / prop->bs);
+    }
     for (int64_t step = 0; step < steps; step++) {
-        const int64_t nprop = (step == 0 ? therm : 0) + int64_t(ens.num_sweeps) * N;
-        {
+        const int64_t nprop = (step == 0 ? therm : 0) + plain_per_step;
+        if (nprop > 0) {
             ScopedTimer timer("mc_sweep");
             if (psi.precision == 0)
                 launch_mc_sweep<double>(psi.view64(), ens.chains.p, ens.chain_th64.p, B, words, ens.move, nprop,
@@ -414,6 +572,16 @@ void sample(Psi& psi, Ensemble& ens) {
                                        ens.counter, ens.seed, ens.acc.p, S());
         }
         ens.counter += uint64_t(nprop);
+        if (block_per_step > 0) {
+            ScopedTimer timer("block_sweep");
+            if (psi.precision == 0)
+                launch_block_sweep<double>(psi.view64(), bn, ens.chains.p, ens.chain_th64.p, B, words, block_per_step,
+                                           ens.counter, ens.seed, ens.acc.p, S());
+            else
+                launch_block_sweep<float>(psi.view32(), bn, ens.chains.p, ens.chain_th32.p, B, words, block_per_step,
+                                          ens.counter, ens.seed, ens.acc.p, S());
+            ens.counter += uint64_t(block_per_step);
+        }
         ANN_HIP_CHECK(hipMemcpyAsync(s.confs.p + step * B * words, ens.chains.p, sizeof(uint64_t) * B * words,
                                      hipMemcpyDeviceToDevice, S()));
         compute_theta(psi, ens.chains.p, B, s, step * B);
@@ -427,24 +595,13 @@ void sample(Psi& psi, Ensemble& ens) {
     }
     chain_view.th64.p = nullptr;
     chain_view.th32.p = nullptr;
-    ens.chains_valid = true;
-    ens.chain_words = words;
-    ens.chain_N = N;
     launch_fill(s.w.p, 1.0 / double(ens.num_samples), count, S());
+    const cplx<double> lp = {psi.log_prefactor.real(), psi.log_prefactor.imag()};
     if (psi.precision == 0)
-        launch_net_log_psi<double>(psi.view64(), s.th64.p, count, {psi.log_prefactor.real(), psi.log_prefactor.imag()},
-                                   s.log_psi.p, nullptr, S());
+        launch_net_log_psi<double>(psi.view64(), s.th64.p, count, lp, s.log_psi.p, nullptr, S());
     else
-        launch_net_log_psi<float>(psi.view32(), s.th32.p, count, {psi.log_prefactor.real(), psi.log_prefactor.imag()},
-                                  s.log_psi.p, nullptr, S());
-    std::vector<uint32_t> acc(B);
-    ANN_HIP_CHECK(hipMemcpyAsync(acc.data(), ens.acc.p, sizeof(uint32_t) * B, hipMemcpyDeviceToHost, S()));
-    Runtime::get().sync();
-    uint64_t total_acc = 0;
-    for (uint32_t x : acc) total_acc += x;
-    const uint64_t total = uint64_t(B) * uint64_t(therm + steps * int64_t(ens.num_sweeps) * N);
-    ens.accepted = total_acc;
-    ens.rejected = total - total_acc;
+        launch_net_log_psi<float>(psi.view32(), s.th32.p, count, lp, s.log_psi.p, nullptr, S());
+    finish_chains(psi, ens, therm, steps, plain_per_step + block_per_step);
 }
 
 void local_energies(Psi& psi, Operator& op, Samples& s) {
@@ -452,6 +609,13 @@ void local_energies(Psi& psi, Operator& op, Samples& s) {
     const DeviceOp& d = op.device(psi.words);
     s.e_loc.ensure(s.count);
     ScopedTimer timer("local_energy");
+    if (psi.kind == PSI_CNN) {
+        if (psi.precision == 0)
+            launch_cnn_local_energy<double>(psi.cnn64(), d.view(), s.confs.p, s.count, s.e_loc.p, S());
+        else
+            launch_cnn_local_energy<float>(psi.cnn32(), d.view(), s.confs.p, s.count, s.e_loc.p, S());
+        return;
+    }
     if (psi.precision == 0)
         launch_local_energy<double>(psi.view64(), d.view(), s.confs.p, s.th64.p, s.count, s.e_loc.p, S());
     else
@@ -544,7 +708,16 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     t.layout = psi.layout;
     t.npad = psi.npad;
     t.count = n;
-    if (psi.kind == PSI_DEEP) t.conn.upload(psi.conn.data(), psi.conn.size());
+    if (psi.kind == PSI_DEEP) {
+        t.conn.upload(psi.conn.data(), psi.conn.size());
+        std::vector<int> inv(size_t(psi.N) * psi.Mpad, -1);
+        for (int i = 0; i < psi.C; i++)
+            for (int j = 0; j < psi.M; j++) {
+                int& slot = inv[size_t(psi.conn[size_t(i) * psi.M + j]) * psi.Mpad + j];
+                slot = slot == -1 ? i : -2;
+            }
+        t.inv.upload(inv.data(), inv.size());
+    }
     t.confs.ensure(size_t(n) * s.words);
     ANN_HIP_CHECK(hipMemcpyAsync(t.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice, S()));
     t.w.ensure(n);
@@ -570,6 +743,104 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
         t.has_S = true;
     }
     Runtime::get().sync();
+}
+
+template <typename T>
+DevBuf<cplx<T>>& materialised(Tdvp& t) {
+    if constexpr (sizeof(T) == 8)
+        return t.Og64;
+    else
+        return t.Og32;
+}
+
+// O_k of a PsiCNN for `count` configurations into O [count][P]
+template <typename T>
+void cnn_ok(Psi& psi, const uint64_t* confs, int64_t count, cplx<T>* O) {
+    ScopedTimer timer("o_k");
+    if constexpr (sizeof(T) == 8)
+        launch_cnn_ok<double>(psi.cnn64(), confs, psi.words, count, nullptr, O, S());
+    else
+        launch_cnn_ok<float>(psi.cnn32(), confs, psi.words, count, nullptr, O, S());
+}
+
+// TDVP statistics from a materialised O (TDVP.cu.template:77-126,181-302)
+template <typename T>
+void tdvp_eval_generic(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
+    sample(psi, ens);
+    Samples& s = ens.samples;
+    local_energies(psi, op, s);
+    const int64_t n = s.count;
+    const int P = psi.num_params;
+    DevBuf<double> sums;
+    sums.ensure(4);
+    launch_weighted_sums(s.e_loc.p, s.w.p, n, sums.p, S());
+    auto& O = materialised<T>(t);
+    O.ensure(size_t(n) * P);
+    cnn_ok<T>(psi, s.confs.p, n, O.p);
+    DevBuf<cplx<double>> coef, sq;
+    coef.ensure(n);
+    sq.ensure(P);
+    t.Obar.ensure(P);
+    t.Fd.ensure(P);
+    t.diag.ensure(P);
+    t.part.ensure(size_t(col_sum_partials(n)) * P);
+    {
+        ScopedTimer timer("spin_sums");
+        launch_weight_coef(s.w.p, nullptr, n, coef.p, S());
+        launch_ok_colsum<T>(O.p, n, P, coef.p, 0, 0.0, nullptr, t.part.p, t.Obar.p, S());
+        launch_ok_colsum<T>(O.p, n, P, coef.p, 2, 0.0, nullptr, t.part.p, sq.p, S());
+        launch_diag_from_sums(sq.p, t.Obar.p, P, t.diag.p, S());
+        launch_weight_coef(s.w.p, s.e_loc.p, n, coef.p, S());
+        launch_ok_colsum<T>(O.p, n, P, coef.p, 1, 0.0, nullptr, t.part.p, t.Fd.p, S());
+    }
+    double h[4];
+    ANN_HIP_CHECK(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    t.E = cd(h[0], h[1]);
+    t.E2 = h[2];
+    t.wsum = h[3];
+    t.O.resize(P);
+    t.F.resize(P);
+    t.Obar.download(reinterpret_cast<cplx<double>*>(t.O.data()), P);
+    t.Fd.download(reinterpret_cast<cplx<double>*>(t.F.data()), P);
+    for (int k = 0; k < P; k++) t.F[k] -= t.E * std::conj(t.O[k]);   // TDVP.cu.template:300
+    t.Fd.upload(reinterpret_cast<const cplx<double>*>(t.F.data()), P);
+    t.generic = true;
+    t.precision = psi.precision;
+    t.words = s.words;
+    t.N = psi.N;
+    t.count = n;
+    t.w.ensure(n);
+    ANN_HIP_CHECK(hipMemcpyAsync(t.w.p, s.w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, S()));
+    t.has_samples = true;
+    t.has_S = false;
+    if (with_S) {
+        t.S.ensure(size_t(P) * P);
+        {
+            ScopedTimer timer("gram");
+            launch_gram<T>(O.p, s.w.p, n, P, t.S.p, S());
+        }
+        // S = sum w conj(O) O - conj(<O>) <O>
+        launch_rank1(t.S.p, t.Obar.p, P, -1.0, S());
+        t.has_S = true;
+    }
+    Runtime::get().sync();
+}
+
+// y = S.v + shift v = sum_s conj(O_s) w_s (O_s.v - <O>.v) + shift v
+template <typename T>
+void tdvp_S_dot_generic(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y) {
+    const int P = int(t.P);
+    auto& O = materialised<T>(t);
+    t.u.ensure(t.count);
+    t.c1.ensure(2);
+    t.dot_scratch.ensure(256);
+    t.part.ensure(size_t(col_sum_partials(t.count)) * P);
+    ScopedTimer timer("sr_matvec");
+    launch_ok_rowdot<T>(O.p, t.count, P, v, t.u.p, S());
+    launch_dot2(t.Obar.p, v, nullptr, nullptr, P, 0, t.dot_scratch.p, reinterpret_cast<double*>(t.c1.p), S());
+    launch_center_scale(t.u.p, t.w.p, reinterpret_cast<const double*>(t.c1.p), t.count, S());
+    launch_ok_colsum<T>(O.p, t.count, P, t.u.p, 1, shift, v, t.part.p, y, S());
 }
 
 template <typename T>
@@ -604,7 +875,8 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y)
     t.va.ensure(t.N);
     t.vb.ensure(t.Mpad);
     t.u.ensure(t.count);
-    t.c1.ensure(1);
+    t.c1.ensure(2);
+    t.dot_scratch.ensure(256);
     V.ensure(size_t(t.N) * t.Mpad);
     SV.ensure(size_t(t.count) * t.Mpad);
     if (zero.n < size_t(t.Mpad)) {
@@ -614,7 +886,7 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y)
     auto& g = Prec<T>::g(t);
     {
         ScopedTimer timer("sr_matvec");
-        launch_unpack_v<T>(net, v, t.va.p, t.vb.p, V.p, S());
+        launch_unpack_v<T>(net, t.inv.p, v, t.va.p, t.vb.p, V.p, S());
         DenseView<T> vnet = net;
         vnet.Wd = V.p;
         vnet.b = zero.p;
@@ -627,7 +899,7 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y)
         }
         launch_structured_Ov<T>(net, t.confs.p, t.words, g.p, SV.p, t.layout == LAYOUT_DEEP1 ? t.vb.p : nullptr,
                                 t.layout == LAYOUT_DEEP1 ? t.va.p : nullptr, t.count, t.u.p, S());
-        launch_cdot(t.Obar.p, v, P, 0, reinterpret_cast<double*>(t.c1.p), S());
+        launch_dot2(t.Obar.p, v, nullptr, nullptr, P, 0, t.dot_scratch.p, reinterpret_cast<double*>(t.c1.p), S());
         launch_center_scale(t.u.p, t.w.p, reinterpret_cast<const double*>(t.c1.p), t.count, S());
     }
     structured_sums<T>(t, net, t.confs.p, t.words, g.p, 1, t.u.p, t.count, y, shift, v);
@@ -635,9 +907,62 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y)
 
 }  // namespace
 
-void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
+namespace {
+
+template <typename T>
+void ok_weighted_sum_t(Psi& psi, Samples& s, const cplx<double>* coef, int conj, cplx<double>* out) {
+    const int P = psi.num_params;
+    if (psi.kind == PSI_CNN) {
+        // materialise O in row chunks of at most 256 MiB, accumulating into out
+        const int64_t rows = std::max<int64_t>(1, (int64_t(256) << 20) / (int64_t(sizeof(cplx<T>)) * P));
+        DevBuf<cplx<T>> O;
+        DevBuf<cplx<double>> part;
+        O.ensure(size_t(std::min(rows, s.count)) * P);
+        part.ensure(size_t(col_sum_partials(std::min(rows, s.count))) * P);
+        ANN_HIP_CHECK(hipMemsetAsync(out, 0, sizeof(cplx<double>) * P, S()));
+        for (int64_t r0 = 0; r0 < s.count; r0 += rows) {
+            const int64_t m = std::min(rows, s.count - r0);
+            cnn_ok<T>(psi, s.confs.p + r0 * s.words, m, O.p);
+            launch_ok_colsum<T>(O.p, m, P, coef + r0, conj, 1.0, out, part.p, out, S());
+        }
+        Runtime::get().sync();
+        return;
+    }
     require_dense(psi);
+    DenseView<T> net = Prec<T>::view(psi);
+    DevBuf<cplx<T>> g;
+    g.ensure(size_t(s.count) * psi.Mpad);
+    launch_g_factor<T>(net, Prec<T>::th(s).p, s.count, g.p, S());
+    DevBuf<cplx<double>> Yw, Yb, Ya;
+    Yw.ensure(size_t(psi.N) * psi.Mpad);
+    Yb.ensure(psi.Mpad);
+    Ya.ensure(psi.N);
+    launch_spin_weighted_sums<T>(net, s.confs.p, s.words, g.p, conj, coef, s.count, Yw.p, Yb.p, Ya.p, S());
+    launch_pack_y(net.layout, net.N, net.M, net.Mpad, net.C, net.conn, Ya.p, Yb.p, Yw.p, nullptr, 0.0,
+                  psi.num_params, out, S());
+    Runtime::get().sync();
+}
+
+}  // namespace
+
+void ok_weighted_sum(Psi& psi, Samples& s, const cplx<double>* coef, int conj, cplx<double>* out) {
+    if (psi.precision == 0)
+        ok_weighted_sum_t<double>(psi, s, coef, conj, out);
+    else
+        ok_weighted_sum_t<float>(psi, s, coef, conj, out);
+}
+
+void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     if (uint32_t(psi.num_params) != t.P) throw std::invalid_argument("TDVP num_params does not match the wave function");
+    if (psi.kind == PSI_CNN) {
+        if (psi.precision == 0)
+            tdvp_eval_generic<double>(t, op, psi, ens, with_S);
+        else
+            tdvp_eval_generic<float>(t, op, psi, ens, with_S);
+        return;
+    }
+    require_dense(psi);
+    t.generic = false;
     if (psi.precision == 0)
         tdvp_eval_t<double>(t, op, psi, ens, with_S);
     else
@@ -646,6 +971,13 @@ void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
 
 void tdvp_S_dot(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y) {
     if (!t.has_samples) throw std::logic_error("TDVP.eval must run before S_dot_vector");
+    if (t.generic) {
+        if (t.precision == 0)
+            tdvp_S_dot_generic<double>(t, v, shift, y);
+        else
+            tdvp_S_dot_generic<float>(t, v, shift, y);
+        return;
+    }
     if (t.precision == 0)
         tdvp_S_dot_t<double>(t, v, shift, y);
     else
@@ -691,25 +1023,21 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         *iters = 1;
         return 0;
     }
-    // conjugate gradients with a Jacobi (diagonal) preconditioner; fp64 vectors and
-    // dot products on the device, one host round trip per scalar
+    // conjugate gradients with a Jacobi (diagonal) preconditioner.  Vectors, dot
+    // products and the CG scalars stay on the device (fp64); the host reads the
+    // residual norm every `check` iterations only.
     if (!t.has_samples) throw std::logic_error("CG solve needs TDVP.eval first");
     DevBuf<cplx<double>> r, z, p, Ap;
-    DevBuf<double> dots;
+    DevBuf<double> scal, scratch;
     r.ensure(P);
     z.ensure(P);
     p.ensure(P);
     Ap.ensure(P);
-    dots.ensure(2);
+    scratch.ensure(256);
+    // per iteration k: [k*8 + 0..3] = (r.z, r.r), [k*8 + 4..7] = (p.Ap)
+    scal.ensure(size_t(max_iter + 2) * 8);
     ANN_HIP_CHECK(hipMemsetAsync(x.p, 0, sizeof(cplx<double>) * P, S()));
     ANN_HIP_CHECK(hipMemcpyAsync(r.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
-    auto dot = [&](const cplx<double>* a, const cplx<double>* b) {
-        launch_cdot(a, b, P, 1, dots.p, S());
-        double h[2];
-        ANN_HIP_CHECK(hipMemcpyAsync(h, dots.p, sizeof(h), hipMemcpyDeviceToHost, S()));
-        Runtime::get().sync();
-        return cd(h[0], h[1]);
-    };
     const bool jacobi = t.diag.p != nullptr && method == 1;
     auto precondition = [&]() {
         if (jacobi)
@@ -719,27 +1047,119 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     };
     precondition();
     ANN_HIP_CHECK(hipMemcpyAsync(p.p, z.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
-    double rr = dot(r.p, r.p).real();
-    double rz = dot(r.p, z.p).real();
-    const double bnorm = std::sqrt(rr);
+    launch_dot2(r.p, z.p, r.p, r.p, P, 1, scratch.p, scal.p, S());
+    auto read_rr = [&](int k) {
+        double h = 0.0;
+        ANN_HIP_CHECK(hipMemcpyAsync(&h, scal.p + size_t(k) * 8 + 2, sizeof(double), hipMemcpyDeviceToHost, S()));
+        Runtime::get().sync();
+        return h;
+    };
+    const double bnorm = std::sqrt(read_rr(0));
+    const int check = 4;
     int it = 0;
+    double rr = bnorm * bnorm;
     while (it < max_iter && std::sqrt(rr) > rtol * bnorm) {
+        double* cur = scal.p + size_t(it) * 8;
+        double* nxt = scal.p + size_t(it + 1) * 8;
         tdvp_S_dot(t, p.p, shift, Ap.p);
-        const double pAp = dot(p.p, Ap.p).real();
-        if (!(pAp > 0.0)) break;
-        const double alpha = rz / pAp;
-        launch_cg_step(x.p, r.p, p.p, Ap.p, alpha, P, S());
+        launch_dot2(p.p, Ap.p, nullptr, nullptr, P, 1, scratch.p, cur + 4, S());
+        launch_cg_update_dev(x.p, r.p, p.p, Ap.p, cur + 0, cur + 4, P, S());
         precondition();
-        rr = dot(r.p, r.p).real();
-        const double rz_new = dot(r.p, z.p).real();
-        launch_cg_dir(p.p, z.p, rz_new / rz, P, S());
-        rz = rz_new;
+        launch_dot2(r.p, z.p, r.p, r.p, P, 1, scratch.p, nxt, S());
+        launch_cg_dir_dev(p.p, z.p, nxt + 0, cur + 0, P, S());
         it++;
+        if (it % check == 0 || it == max_iter) rr = read_rr(it);
     }
+    rr = read_rr(it);
     x.download(reinterpret_cast<cplx<double>*>(x_out), P);
     *iters = it;
     *res = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
     return std::sqrt(rr) <= rtol * bnorm ? 0 : 2;
+}
+
+// ------------------------------------------------------------ learned proposal
+
+TrainBatch Proposal::batch(int Bt) {
+    bsample.ensure(Bt);
+    bsites.ensure(size_t(Bt) * bs);
+    X.ensure(size_t(Bt) * N);
+    Y.ensure(size_t(Bt) * N);
+    Mask.ensure(size_t(Bt) * N);
+    Hd.ensure(size_t(Bt) * H);
+    A2.ensure(size_t(Bt) * N);
+    dH.ensure(size_t(Bt) * H);
+    loss_part.ensure(Bt);
+    return {Bt, bsample.p, bsites.p, X.p, Y.p, Mask.p, Hd.p, A2.p, dH.p, loss_part.p};
+}
+
+BlockNet Proposal::view() {
+    if (!packed) {
+        W1q.ensure(size_t(N) * 256);
+        b1q.ensure(256);
+        W2h.ensure(size_t(N) * 256);
+        b2.ensure(N);
+        launch_block_net_pack(params.p, N, H, W1q.p, b1q.p, W2h.p, b2.p, S());
+        packed = true;
+    }
+    return {N, H, bs, num_blocks, blocks.p, W1q.p, b1q.p, W2h.p, b2.p};
+}
+
+static double batch_loss(Proposal& p, int Bt) {
+    std::vector<double> h(Bt);
+    p.loss_part.download(h.data(), Bt);
+    double s = 0.0;
+    for (double x : h) s += x;
+    return s;
+}
+
+double proposal_train(Proposal& p, const uint64_t* confs, int64_t count, int words, int steps, int batch, double lr) {
+    if (count <= 0) throw std::invalid_argument("no configurations to train on");
+    if (batch <= 0 || steps < 0) throw std::invalid_argument("batch must be positive");
+    if (words != (p.N + 63) / 64) throw std::invalid_argument("configurations do not match the proposal's sites");
+    TrainBatch b = p.batch(batch);
+    p.grad.ensure(p.num_params());
+    for (int it = 0; it < steps; it++) {
+        ScopedTimer timer("proposal_train");
+        launch_draw_batch(b, count, p.blocks.p, p.num_blocks, p.bs, p.train_step++, p.seed, S());
+        launch_build_inputs(b, confs, words, p.N, p.bs, S());
+        launch_loss_grad(b, p.params.p, p.N, p.H, double(batch), p.grad.p, S());
+        launch_adam(p.params.p, p.m.p, p.v.p, p.grad.p, p.num_params(), float(lr), 0.9f, 0.999f, 1e-8f, ++p.adam_t,
+                    S());
+    }
+    p.packed = false;
+    Runtime::get().sync();
+    p.last_loss = steps > 0 ? batch_loss(p, batch) : 0.0;
+    return p.last_loss;
+}
+
+double proposal_loss_grad(Proposal& p, const uint64_t* confs, int64_t count, int words, const int* sites,
+                          std::vector<float>& grad) {
+    if (count <= 0 || count > (int64_t(1) << 24)) throw std::invalid_argument("1..2^24 rows");
+    if (words != (p.N + 63) / 64) throw std::invalid_argument("configurations do not match the proposal's sites");
+    for (int64_t r = 0; r < count * p.bs; r++)
+        if (sites[r] < 0 || sites[r] >= p.N) throw std::invalid_argument("block site out of range");
+    const int Bt = int(count);
+    TrainBatch b = p.batch(Bt);
+    DevBuf<uint64_t> dconfs;
+    dconfs.upload(confs, size_t(count) * words);
+    std::vector<int64_t> idx(Bt);
+    for (int r = 0; r < Bt; r++) idx[r] = r;
+    p.bsample.upload(idx.data(), Bt);
+    p.bsites.upload(sites, size_t(Bt) * p.bs);
+    p.grad.ensure(p.num_params());
+    launch_build_inputs(b, dconfs.p, words, p.N, p.bs, S());
+    launch_loss_grad(b, p.params.p, p.N, p.H, double(Bt), p.grad.p, S());
+    Runtime::get().sync();
+    grad.resize(p.num_params());
+    p.grad.download(grad.data(), grad.size());
+    return batch_loss(p, Bt);
+}
+
+void proposal_adam_step(Proposal& p, const float* grad, double lr) {
+    p.grad.upload(grad, p.num_params());
+    launch_adam(p.params.p, p.m.p, p.v.p, p.grad.p, p.num_params(), float(lr), 0.9f, 0.999f, 1e-8f, ++p.adam_t, S());
+    p.packed = false;
+    Runtime::get().sync();
 }
 
 }  // namespace ann

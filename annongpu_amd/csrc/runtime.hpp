@@ -2,7 +2,9 @@
 // per-kernel hipEvent timers, wave functions, operators, ensembles and TDVP state.
 #pragma once
 
+#include "cnn.hpp"
 #include "ops.hpp"
+#include "proposal.hpp"
 
 #include <complex>
 #include <map>
@@ -14,6 +16,9 @@
 namespace ann {
 
 using cd = std::complex<double>;
+
+// the library's HIP stream (non-blocking); every copy and launch goes through it
+hipStream_t lib_stream();
 
 template <typename T>
 struct DevBuf {
@@ -43,12 +48,19 @@ struct DevBuf {
         ANN_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (m > 0 ? m : 1)));
         n = m;
     }
+    // ordered after the work already queued on the library stream; returns when done
     void upload(const T* h, size_t m) {
         ensure(m);
-        if (m) ANN_HIP_CHECK(hipMemcpy(p, h, sizeof(T) * m, hipMemcpyHostToDevice));
+        if (m) {
+            ANN_HIP_CHECK(hipMemcpyAsync(p, h, sizeof(T) * m, hipMemcpyHostToDevice, lib_stream()));
+            ANN_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+        }
     }
     void download(T* h, size_t m) const {
-        if (m) ANN_HIP_CHECK(hipMemcpy(h, p, sizeof(T) * m, hipMemcpyDeviceToHost));
+        if (m) {
+            ANN_HIP_CHECK(hipMemcpyAsync(h, p, sizeof(T) * m, hipMemcpyDeviceToHost, lib_stream()));
+            ANN_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+        }
     }
 };
 
@@ -128,12 +140,19 @@ struct Psi {
     int npad = 0;
     DevBuf<cplx<double>> zero64;
     DevBuf<cplx<float>> zero32;
+    // PsiCNN device state
+    DevBuf<cplx<double>> cp64;
+    DevBuf<cplx<float>> cp32;
+    DevBuf<int> nbr_d;
+    DevBuf<uint32_t> sym_d;
 
     bool dense() const { return kind == PSI_RBM || (kind == PSI_DEEP && num_layers == 1); }
     void finalize();           // derived sizes after construction
     void sync_device();        // upload parameters when dirty
     DenseView<double> view64();
     DenseView<float> view32();
+    CnnView<double> cnn64();
+    CnnView<float> cnn32();
     std::vector<cd> get_params() const;
     void set_params(const cd* p);
     std::unique_ptr<Psi> clone() const;
@@ -170,6 +189,30 @@ struct Samples {
     DevBuf<cplx<double>> e_loc;
 };
 
+// learned block proposal: network parameters with Adam state, the inference
+// form read by the block sweep, and the training batch workspace
+struct Proposal {
+    int N = 0, H = 0, bs = 0, num_blocks = 0;
+    uint64_t seed = 0;
+    int64_t num_params() const { return int64_t(H) * N + H + int64_t(N) * H + N; }
+    DevBuf<int> blocks;
+    DevBuf<float> params, m, v, grad;
+    int64_t adam_t = 0;
+    uint64_t train_step = 0;
+    bool packed = false;
+    DevBuf<int32_t> W1q, b1q;
+    DevBuf<uint16_t> W2h;
+    DevBuf<float> b2;
+    // batch workspace (grow-only)
+    DevBuf<int64_t> bsample;
+    DevBuf<int> bsites;
+    DevBuf<float> X, Y, Mask, Hd, A2, dH;
+    DevBuf<double> loss_part;
+    TrainBatch batch(int Bt);
+    BlockNet view();
+    double last_loss = 0.0;
+};
+
 enum EnsembleKind : int { ENS_MC = 0, ENS_EXACT = 1 };
 
 struct Ensemble {
@@ -186,6 +229,9 @@ struct Ensemble {
     DevBuf<cplx<float>> chain_th32;
     DevBuf<cplx<double>> chain_th64;
     uint64_t accepted = 0, rejected = 0;
+    // learned block proposal (not owned); block sweeps per sample and plain sweeps per sample
+    Proposal* proposal = nullptr;
+    uint32_t plain_sweeps = 1;
     Samples samples;
     uint64_t num_steps() const { return kind == ENS_EXACT ? (uint64_t(1) << num_sites) : num_samples; }
 };
@@ -211,14 +257,32 @@ struct Tdvp {
     DevBuf<cplx<float>> V32, SV32, zero32;
     DevBuf<cplx<double>> V64, SV64, zero64;
     DevBuf<uint16_t> frag;
+    DevBuf<int> inv;            // inverse gather table [N][Mpad] (PsiDeep)
+    DevBuf<double> dot_scratch;
+    // materialised O [count][P] of wave functions without a structured O (PsiCNN)
+    bool generic = false;
+    DevBuf<cplx<float>> Og32;
+    DevBuf<cplx<double>> Og64;
+    DevBuf<cplx<double>> part;
 };
 
 // core operations
 void sample(Psi& psi, Ensemble& ens);
 void compute_theta(Psi& psi, const uint64_t* confs, int64_t count, Samples& out, int64_t offset);
+void log_psi_of(Psi& psi, const uint64_t* confs, int64_t count, Samples& s, double* w);
+// sum_s coef_s op(O_s) in parameter order (op: 0 identity, 1 conjugate), any wave function
+void ok_weighted_sum(Psi& psi, Samples& s, const cplx<double>* coef, int conj, cplx<double>* out);
 void local_energies(Psi& psi, Operator& op, Samples& s);
 void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S);
 void tdvp_S_dot(Tdvp& t, const cplx<double>* v_dev, double shift, cplx<double>* y_dev);
 int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x, int* iters, double* res);
+
+// proposal training: Adam steps on random (sample, block) rows of the given
+// device configurations; returns the loss of the last step
+double proposal_train(Proposal& p, const uint64_t* confs, int64_t count, int words, int steps, int batch, double lr);
+// loss and gradient on explicit rows (host configurations and block sites)
+double proposal_loss_grad(Proposal& p, const uint64_t* confs, int64_t count, int words, const int* sites,
+                          std::vector<float>& grad);
+void proposal_adam_step(Proposal& p, const float* grad, double lr);
 
 }  // namespace ann

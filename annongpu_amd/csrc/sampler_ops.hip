@@ -39,36 +39,49 @@ __global__ void init_chains_kernel(uint64_t* confs, int num_chains, int words, i
 }
 
 // v (reference parameter order) -> va [N], vb [Mpad], V [N][Mpad] (W part re-indexed
-// onto visible units: V[conn(i,j)][j] += v_W(i,j))
+// onto visible units: V[conn(i,j)][j] += v_W(i,j)); one thread per V element
 template <typename T>
-__global__ void unpack_v_kernel(DenseView<T> net, const cplx<double>* v, cplx<double>* va, cplx<double>* vb,
-                                cplx<T>* V) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= net.Mpad) return;
-    const bool real_unit = j < net.M;
-    if (net.layout == LAYOUT_RBM) {
-        for (int i = 0; i < net.N; i++) {
-            cplx<double> x = real_unit ? v[int64_t(i) * net.M + j] : cplx<double>{0, 0};
-            V[int64_t(i) * net.Mpad + j] = {T(x.re), T(x.im)};
+__global__ void unpack_v_kernel(DenseView<T> net, const int* inv, const cplx<double>* v, cplx<double>* va,
+                                cplx<double>* vb, cplx<T>* V) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t nV = int64_t(net.N) * net.Mpad;
+    if (e < nV) {
+        const int vis = int(e / net.Mpad), j = int(e % net.Mpad);
+        double re = 0.0, im = 0.0;
+        if (j < net.M) {
+            if (net.layout == LAYOUT_RBM) {
+                const cplx<double> x = v[int64_t(vis) * net.M + j];
+                re = x.re;
+                im = x.im;
+            } else {
+                // inv[vis][j]: the unique i with conn(i, j) = vis, -1 if none, -2 if several
+                const int i0 = inv[e];
+                if (i0 >= 0) {
+                    const cplx<double> x = v[net.N + net.M + int64_t(i0) * net.M + j];
+                    re = x.re;
+                    im = x.im;
+                } else if (i0 == -2) {
+                    for (int i = 0; i < net.C; i++) {
+                        if (int(net.conn[int64_t(i) * net.M + j]) == vis) {
+                            const cplx<double> x = v[net.N + net.M + int64_t(i) * net.M + j];
+                            re += x.re;
+                            im += x.im;
+                        }
+                    }
+                }
+            }
         }
-        if (vb) vb[j] = {0, 0};
+    V[e] = {T(re), T(im)};
         return;
     }
-    const int base = net.N + net.M;
-    // accumulate in fp64 first (duplicate connections sum)
-    for (int i = 0; i < net.N; i++) V[int64_t(i) * net.Mpad + j] = {T(0), T(0)};
-    if (real_unit) {
-        for (int i = 0; i < net.C; i++) {
-            const int vis = int(net.conn[int64_t(i) * net.M + j]);
-            const cplx<double> x = v[base + int64_t(i) * net.M + j];
-            cplx<T>& dst = V[int64_t(vis) * net.Mpad + j];
-            dst.re += T(x.re);
-            dst.im += T(x.im);
-        }
+    const int64_t e2 = e - nV;
+    if (e2 < net.Mpad) {
+        const int j = int(e2);
+        if (vb) vb[j] = (net.layout == LAYOUT_DEEP1 && j < net.M) ? v[net.N + j] : cplx<double>{0, 0};
+    } else if (e2 < int64_t(net.Mpad) + net.N) {
+        const int i = int(e2 - net.Mpad);
+        if (va) va[i] = net.layout == LAYOUT_DEEP1 ? v[i] : cplx<double>{0, 0};
     }
-    vb[j] = real_unit ? v[net.N + j] : cplx<double>{0, 0};
-    if (j < net.N) va[j] = v[j];
-    for (int i = net.Mpad + j; i < net.N; i += net.Mpad) va[i] = v[i];
 }
 
 // y (reference order) from Ya, Yb, Yw (+ shift * v)
@@ -145,6 +158,60 @@ __global__ __launch_bounds__(kDot) void cdot_kernel(const cplx<double>* a, const
         out[0] = sr[0];
         out[1] = si[0];
     }
+}
+
+// fused conjugated dot products: out[0..1] = <a, b>, out[2..3] = <c, d> (if c != nullptr)
+constexpr int kDotBlocks = 64;
+__global__ __launch_bounds__(256) void dot2_partial_kernel(const cplx<double>* a, const cplx<double>* b,
+                                                           const cplx<double>* c, const cplx<double>* d,
+                                                           int64_t n, int conj_a, double* partial) {
+    __shared__ double sh[4][256];
+    double acc[4] = {0, 0, 0, 0};
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const double ar = a[i].re, ai = conj_a ? -a[i].im : a[i].im;
+        acc[0] += ar * b[i].re - ai * b[i].im;
+        acc[1] += ar * b[i].im + ai * b[i].re;
+        if (c) {
+            const double cr = c[i].re, ci = conj_a ? -c[i].im : c[i].im;
+            acc[2] += cr * d[i].re - ci * d[i].im;
+            acc[3] += cr * d[i].im + ci * d[i].re;
+        }
+    }
+    for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (int(threadIdx.x) < o)
+            for (int k = 0; k < 4; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) partial[blockIdx.x * 4 + threadIdx.x] = sh[threadIdx.x][0];
+}
+__global__ void dot2_final_kernel(const double* partial, int blocks, double* out) {
+    const int k = threadIdx.x;
+    if (k >= 4) return;
+    double acc = 0.0;
+    for (int b = 0; b < blocks; b++) acc += partial[b * 4 + k];
+    out[k] = acc;
+}
+
+// CG: alpha = rz / pAp (0 on breakdown); x += alpha p; r -= alpha Ap
+__global__ void cg_update_dev_kernel(cplx<double>* x, cplx<double>* r, const cplx<double>* p,
+                                     const cplx<double>* Ap, const double* rz, const double* pAp, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double alpha = pAp[0] > 0.0 ? rz[0] / pAp[0] : 0.0;
+    x[i].re += alpha * p[i].re;
+    x[i].im += alpha * p[i].im;
+    r[i].re -= alpha * Ap[i].re;
+    r[i].im -= alpha * Ap[i].im;
+}
+// p = z + (rz_new / rz_old) p
+__global__ void cg_dir_dev_kernel(cplx<double>* p, const cplx<double>* z, const double* rz_new,
+                                  const double* rz_old, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double beta = rz_old[0] != 0.0 ? rz_new[0] / rz_old[0] : 0.0;
+    p[i] = {z[i].re + beta * p[i].re, z[i].im + beta * p[i].im};
 }
 
 // u'_s = w_s * (u_s - c)   with c = (re, im) read from device memory
@@ -246,14 +313,15 @@ void launch_init_chains(uint64_t* confs, int num_chains, int words, int N, int m
 }
 
 template <typename T>
-void launch_unpack_v(const DenseView<T>& net, const cplx<double>* v, cplx<double>* va, cplx<double>* vb,
-                     cplx<T>* V, hipStream_t stream) {
-    unpack_v_kernel<T><<<blocks_for(net.Mpad, 64), 64, 0, stream>>>(net, v, va, vb, V);
+void launch_unpack_v(const DenseView<T>& net, const int* inv, const cplx<double>* v, cplx<double>* va,
+                     cplx<double>* vb, cplx<T>* V, hipStream_t stream) {
+    const int64_t n = int64_t(net.N) * net.Mpad + net.Mpad + net.N;
+    unpack_v_kernel<T><<<blocks_for(n), 256, 0, stream>>>(net, inv, v, va, vb, V);
     ANN_LAUNCH_CHECK();
 }
-template void launch_unpack_v<float>(const DenseView<float>&, const cplx<double>*, cplx<double>*, cplx<double>*,
-                                     cplx<float>*, hipStream_t);
-template void launch_unpack_v<double>(const DenseView<double>&, const cplx<double>*, cplx<double>*,
+template void launch_unpack_v<float>(const DenseView<float>&, const int*, const cplx<double>*, cplx<double>*,
+                                     cplx<double>*, cplx<float>*, hipStream_t);
+template void launch_unpack_v<double>(const DenseView<double>&, const int*, const cplx<double>*, cplx<double>*,
                                       cplx<double>*, cplx<double>*, hipStream_t);
 
 void launch_pack_y(int layout, int N, int M, int Mpad, int C, const uint32_t* conn, const cplx<double>* Ya,
@@ -273,6 +341,26 @@ void launch_pack_fragments(const cplx<float>* V, int N, int Mpad, int npad, uint
 void launch_cdot(const cplx<double>* a, const cplx<double>* b, int64_t n, int conj_a, double* out2,
                  hipStream_t stream) {
     cdot_kernel<<<1, kDot, 0, stream>>>(a, b, n, conj_a, out2);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_dot2(const cplx<double>* a, const cplx<double>* b, const cplx<double>* c, const cplx<double>* d,
+                 int64_t n, int conj_a, double* scratch, double* out4, hipStream_t stream) {
+    dot2_partial_kernel<<<kDotBlocks, 256, 0, stream>>>(a, b, c, d, n, conj_a, scratch);
+    ANN_LAUNCH_CHECK();
+    dot2_final_kernel<<<1, 64, 0, stream>>>(scratch, kDotBlocks, out4);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_cg_update_dev(cplx<double>* x, cplx<double>* r, const cplx<double>* p, const cplx<double>* Ap,
+                          const double* rz, const double* pAp, int64_t n, hipStream_t stream) {
+    cg_update_dev_kernel<<<blocks_for(n), 256, 0, stream>>>(x, r, p, Ap, rz, pAp, n);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_cg_dir_dev(cplx<double>* p, const cplx<double>* z, const double* rz_new, const double* rz_old, int64_t n,
+                       hipStream_t stream) {
+    cg_dir_dev_kernel<<<blocks_for(n), 256, 0, stream>>>(p, z, rz_new, rz_old, n);
     ANN_LAUNCH_CHECK();
 }
 

@@ -161,6 +161,37 @@ int ann_tdvp_S_dot_vector(ann_tdvp tdvp, const double* v, double* out);
 int ann_tdvp_solve(ann_tdvp tdvp, double shift, int32_t method, double rtol, int32_t max_iter, double* x,
                    int32_t* iterations, double* residual);
 
+/* ------------------------------------------------------- learned proposal */
+/* Additive (absent from the reference): a two-layer tanh network
+ * x -> tanh(W1 x + b1) -> W2 h + b2 (N -> hidden -> N) that proposes
+ * S_z-preserving moves of whole blocks of block_size <= 8 sites.  block_sites is
+ * [num_blocks][block_size].  A move picks a block, masks its spins in the input
+ * and redraws them among the patterns with the same number of up spins with
+ * probability proportional to exp(sum of the logits of the up sites); the
+ * Metropolis-Hastings test uses |psi'/psi|^2 times the proposal ratio, so the
+ * sampled distribution stays |psi|^2.  Parameters are fp32, packed
+ * [W1 (hidden x N) | b1 | W2 (N x hidden) | b2]; hidden <= 256. */
+int ann_proposal_create(ann_proposal* out, int32_t num_sites, int32_t hidden, int32_t num_blocks,
+                        int32_t block_size, const int32_t* block_sites, uint64_t seed);
+int ann_proposal_destroy(ann_proposal p);
+int ann_proposal_num_params(ann_proposal p, int64_t* n);
+int ann_proposal_get_params(ann_proposal p, float* params);
+int ann_proposal_set_params(ann_proposal p, const float* params);
+/* loss (mean over rows of sum over the row's block of softplus(l) - y l) and its
+ * gradient for explicit rows: confs [count][words], sites [count][block_size] */
+int ann_proposal_loss_grad(ann_proposal p, const uint64_t* confs, int64_t count, int32_t words,
+                           const int32_t* sites, double* loss, float* grad);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the given gradient */
+int ann_proposal_adam_step(ann_proposal p, const float* grad, double lr);
+/* Adam training on random (sample, block) rows of the ensemble's last samples
+ * (device resident); loss = loss of the last batch */
+int ann_proposal_train(ann_proposal p, ann_ensemble ens, int32_t steps, int32_t batch, double lr, double* loss);
+/* attach (p != NULL) or detach a proposal.  With one attached, the interval
+ * between samples is num_sweeps block sweeps (ceil(N / block_size) block moves
+ * each) plus plain_sweeps single-flip/exchange sweeps.  The ensemble does not
+ * own p: keep it alive while attached. */
+int ann_monte_carlo_set_proposal(ann_ensemble ens, ann_proposal p, uint32_t plain_sweeps);
+
 #ifdef __cplusplus
 }
 #endif

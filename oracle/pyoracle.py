@@ -320,3 +320,54 @@ def cg_solve(S, shift, F, rtol=1e-10, max_iter=1000):
     rc = lib().orc_cg_solve(ctypes.c_int32(F.size), _ptr(S), ctypes.c_double(shift), _ptr(F), _ptr(x),
                             ctypes.c_double(rtol), ctypes.c_int32(max_iter), ctypes.byref(it))
     return x, it.value, rc
+
+
+# ------------------------------------------------------------ learned proposal
+# (oracle/proposal_oracle.c: no counterpart in the reference)
+
+def mlp_num_params(N, H):
+    return H * N + H + N * H + N
+
+
+def mlp_logits(N, H, params, conf, block):
+    """logits of the proposal network for conf with the block's sites masked"""
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    conf = np.ascontiguousarray(conf, dtype=np.uint64)
+    block = np.ascontiguousarray(block, dtype=np.int32)
+    out = np.zeros(N, dtype=np.float64)
+    lib().orc_mlp_logits(ctypes.c_int32(N), ctypes.c_int32(H), _ptr(params), _ptr(conf), _ptr(block),
+                         ctypes.c_int32(block.size), _ptr(out))
+    return out
+
+
+def mlp_loss_grad(N, H, params, confs, blocks):
+    """(loss, gradient) of the block pseudo-likelihood loss, mean over samples"""
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    confs = np.ascontiguousarray(confs, dtype=np.uint64)
+    if confs.ndim == 1:
+        confs = confs[:, None]
+    blocks = np.ascontiguousarray(blocks, dtype=np.int32)
+    grad = np.zeros(params.size, dtype=np.float64)
+    f = lib().orc_mlp_loss_grad
+    f.restype = ctypes.c_double
+    loss = f(ctypes.c_int32(N), ctypes.c_int32(H), _ptr(params), ctypes.c_int64(confs.shape[0]), _ptr(confs),
+             ctypes.c_int32(confs.shape[1]), _ptr(blocks), ctypes.c_int32(blocks.shape[1]), _ptr(grad))
+    return loss, grad
+
+
+def adam_step(params, m, v, grad, lr, t, beta1=0.9, beta2=0.999, eps=1e-8):
+    """one Adam step in place on float64 arrays"""
+    for a in (params, m, v):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    grad = np.ascontiguousarray(grad, dtype=np.float64)
+    lib().orc_adam_step(ctypes.c_int64(params.size), _ptr(params), _ptr(m), _ptr(v), _ptr(grad),
+                        ctypes.c_double(lr), ctypes.c_double(beta1), ctypes.c_double(beta2),
+                        ctypes.c_double(eps), ctypes.c_int64(t))
+
+
+def block_log_q(block_logits, p_new, k):
+    """log probability of block pattern p_new among the patterns with k up spins"""
+    bl = np.ascontiguousarray(block_logits, dtype=np.float64)
+    f = lib().orc_block_log_q
+    f.restype = ctypes.c_double
+    return f(_ptr(bl), ctypes.c_int32(bl.size), ctypes.c_uint32(p_new), ctypes.c_int32(k))

@@ -121,6 +121,16 @@ int orc_cholesky_solve(int32_t n, const double complex* S, double shift,
 int orc_cg_solve(int32_t n, const double complex* S, double shift, const double complex* F,
                  double complex* x, double rtol, int32_t max_iter, int32_t* iterations);
 
+/* learned block proposal (proposal_oracle.c; no counterpart in the reference) */
+void orc_mlp_logits(int32_t N, int32_t H, const double* params, const uint64_t* conf,
+                    const int32_t* block, int32_t bs, double* logits);
+double orc_mlp_loss_grad(int32_t N, int32_t H, const double* params, int64_t count,
+                         const uint64_t* confs, int32_t words, const int32_t* blocks, int32_t bs,
+                         double* grad);
+void orc_adam_step(int64_t n, double* params, double* m, double* v, const double* grad, double lr,
+                   double beta1, double beta2, double eps, int64_t t);
+double orc_block_log_q(const double* block_logits, int32_t bs, uint32_t p_new, int32_t k);
+
 #ifdef __cplusplus
 }
 #endif
