@@ -446,9 +446,12 @@ class Operator:
     """Operator(PauliExpression, gpu) -- source/operator/Operator.cpp:18-39."""
 
     def __init__(self, expr, gpu=True):
-        if not isinstance(expr, PauliExpression):
-            raise TypeError("Operator expects an annongpu_amd.PauliExpression")
-        terms = list(expr)
+        # a PauliExpression, or any iterable of ((a, b), coefficient) in the
+        # reference's (a, b) string encoding (include/basis/PauliString.hpp:37-56)
+        try:
+            terms = [((int(ab[0]), int(ab[1])), complex(c)) for ab, c in expr]
+        except (TypeError, ValueError) as e:
+            raise TypeError("Operator expects a PauliExpression or ((a, b), coefficient) pairs") from e
         n = len(terms)
         words = MAX_WORDS
         a = np.zeros((max(n, 1), words), dtype=np.uint64)
@@ -837,9 +840,16 @@ class TDVP:
         check(lib.ann_tdvp_S_dot_vector(self._h, ptr(v), ptr(out)))
         return out
 
-    def solve(self, shift=1e-3, method="cholesky", rtol=1e-6, max_iter=500, raise_on_failure=True):
+    def solve(self, shift=1e-3, method="cholesky", rtol=1e-6, max_iter=500, raise_on_failure=True, x0=None):
+        """(S + shift) x = F in fp64 by Cholesky of the dense S or by Jacobi-preconditioned
+        CG on the matrix-free S; ``x0`` starts CG from a previous solution"""
         m = {"cholesky": _lib.SOLVE_CHOLESKY, "cg": _lib.SOLVE_CG}[method]
         x = np.zeros(self.num_params, dtype=np.complex128)
+        if x0 is not None:
+            if method != "cg":
+                raise ValueError("a starting vector applies to CG only")
+            x[:] = np.asarray(x0, dtype=np.complex128).ravel()
+            m = _lib.SOLVE_CG_WARM
         it, res = ctypes.c_int32(), ctypes.c_double()
         rc = lib.ann_tdvp_solve(self._h, float(shift), m, float(rtol), int(max_iter), ptr(x), ref(it), ref(res))
         self.last_solve = dict(iterations=it.value, residual=res.value)

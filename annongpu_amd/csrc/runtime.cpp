@@ -984,7 +984,8 @@ void tdvp_S_dot(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y) {
         tdvp_S_dot_t<float>(t, v, shift, y);
 }
 
-int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x_out, int* iters, double* res) {
+int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x_out, int* iters, double* res,
+               const cd* x0) {
     const int P = int(t.P);
     DevBuf<cplx<double>> x;
     x.ensure(P);
@@ -1035,9 +1036,19 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     Ap.ensure(P);
     scratch.ensure(256);
     // per iteration k: [k*8 + 0..3] = (r.z, r.r), [k*8 + 4..7] = (p.Ap)
-    scal.ensure(size_t(max_iter + 2) * 8);
-    ANN_HIP_CHECK(hipMemsetAsync(x.p, 0, sizeof(cplx<double>) * P, S()));
-    ANN_HIP_CHECK(hipMemcpyAsync(r.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
+    scal.ensure(size_t(max_iter + 3) * 8);
+    double* fnorm2 = scal.p + size_t(max_iter + 2) * 8;
+    launch_dot2(t.Fd.p, t.Fd.p, nullptr, nullptr, P, 1, scratch.p, fnorm2, S());
+    if (x0) {
+        // warm start: r = F - (S + shift) x0
+        x.upload(reinterpret_cast<const cplx<double>*>(x0), P);
+        tdvp_S_dot(t, x.p, shift, Ap.p);
+        launch_cg_dir(Ap.p, t.Fd.p, -1.0, P, S());
+        ANN_HIP_CHECK(hipMemcpyAsync(r.p, Ap.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
+    } else {
+        ANN_HIP_CHECK(hipMemsetAsync(x.p, 0, sizeof(cplx<double>) * P, S()));
+        ANN_HIP_CHECK(hipMemcpyAsync(r.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
+    }
     const bool jacobi = t.diag.p != nullptr && method == 1;
     auto precondition = [&]() {
         if (jacobi)
@@ -1054,10 +1065,12 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         Runtime::get().sync();
         return h;
     };
-    const double bnorm = std::sqrt(read_rr(0));
+    double fn2 = 0.0;
+    ANN_HIP_CHECK(hipMemcpyAsync(&fn2, fnorm2, sizeof(double), hipMemcpyDeviceToHost, S()));
+    double rr = read_rr(0);
+    const double bnorm = std::sqrt(fn2);
     const int check = 4;
     int it = 0;
-    double rr = bnorm * bnorm;
     while (it < max_iter && std::sqrt(rr) > rtol * bnorm) {
         double* cur = scal.p + size_t(it) * 8;
         double* nxt = scal.p + size_t(it + 1) * 8;

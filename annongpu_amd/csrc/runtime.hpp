@@ -275,7 +275,9 @@ void ok_weighted_sum(Psi& psi, Samples& s, const cplx<double>* coef, int conj, c
 void local_energies(Psi& psi, Operator& op, Samples& s);
 void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S);
 void tdvp_S_dot(Tdvp& t, const cplx<double>* v_dev, double shift, cplx<double>* y_dev);
-int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x, int* iters, double* res);
+// method 0 Cholesky, 1 Jacobi-preconditioned CG from x0 (zero when x0 is null)
+int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x, int* iters, double* res,
+               const cd* x0 = nullptr);
 
 // proposal training: Adam steps on random (sample, block) rows of the given
 // device configurations; returns the loss of the last step

@@ -1,15 +1,26 @@
-"""ESS/s of one VMC step (sampling + E_loc + O_k + SR solve) on one MI355X.
+"""ESS/s of one VMC step (sampling + E_loc + O_k + SR solve) on one MI355X:
+learned block proposal (bf16 inference) vs the plain fp32 exchange sampler.
 
-Workload (default): 2D J1 Heisenberg 10x10 periodic (N=100, 600 Pauli strings),
-complex RBM with biases alpha=4 (PsiDeep, M=400, P=40,500, seeded complex
-Gaussian sigma=0.01, final weights 1), 16,384 persistent chains, 100 exchange
-sweeps per step in the S_z=0 sector, angles on MFMA (bf16x3 split, fp32
-accumulation), fp32 sweeps / E_loc / O_k, fp64 reductions and a matrix-free CG
-solve of (S + 1e-3 I) x = F with fp64 vectors, then the SR parameter update.
+Workload: 2D J1 Heisenberg 10x10 periodic (N=100, 600 Pauli strings), complex
+RBM with biases alpha=4 (PsiDeep, M=400, P=40,500, seeded complex Gaussian
+sigma=0.01, final weights 1), 16,384 persistent chains in the S_z=0 sector,
+angles on MFMA (bf16x3 split, fp32 accumulation), fp32 sweeps / E_loc / O_k,
+fp64 reductions, SR by Jacobi-preconditioned CG on the matrix-free S (fp64
+vectors, warm-started from the previous step), then the SR update.
 
-ESS per step = Ns * min(1, S_sw / (2 tau_int)), tau_int (in sweeps) of E_loc
-measured on 1024 chains recording E_loc every sweep (Sokal window c = 5).
-Prints one JSON line.
+Protocol (all on the GPU, untimed unless stated):
+  1. the RBM is pre-optimised by SR steps with the plain sampler, so that the
+     autocorrelation is that of a structured state, not of the near-uniform seed;
+  2. training data: num_chains x 64 samples of the plain sampler; the proposal
+     network (100 -> 256 -> 100, tanh; 2x4 / 4x2 plaquette blocks) is trained by
+     the library's own forward / backward / Adam kernels;
+  3. tau_int (Sokal window c = 5, batch means as a cross-check) of E_loc per sweep
+     on 1024 chains, for plain exchange sweeps and for learned block sweeps;
+  4. timed VMC steps of three paths on the same state:
+       plain-config  : 100 exchange sweeps per sample (the workload's definition)
+       plain-matched : ceil(2 tau_plain) exchange sweeps per sample
+       learned       : ceil(2 tau_learned) block sweeps per sample (headline)
+ESS per step = Ns * min(1, S_sw / (2 tau_int)).  Prints one JSON line.
 """
 import argparse
 import json
@@ -62,11 +73,11 @@ def build_workload(A, L=10, alpha=4, seed=3, precision="fp32"):
     return N, psi, H
 
 
-def cpu_baseline(A, L, alpha, seed, seconds):
+def cpu_baseline(A, L, alpha, seed, seconds, sweeps):
     """Oracle (fp64 C restatement of the reference host path, OpenMP over chains)
-    on a bounded sample of the same workload: chains x 100 exchange sweeps, then
-    E_loc and O_k per sample.  Excludes the SR solve (dense S of 40,500^2 does not
-    fit a CPU step), which makes the baseline optimistic."""
+    on a bounded sample of the same workload: chains x `sweeps` exchange sweeps,
+    then E_loc and O_k per sample.  Excludes the SR solve (dense S of 40,500^2
+    does not fit a CPU step), which makes the baseline optimistic."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
     N, psi, H = build_workload(A, L, alpha, seed, "fp64")
@@ -87,25 +98,69 @@ def cpu_baseline(A, L, alpha, seed, seconds):
     t0 = time.perf_counter()
     orc.mc_run(o, 4 * cores, 1, 10, 0, move=orc.MOVE_EXCHANGE, seed=1, num_threads=cores)
     per_sweep = (time.perf_counter() - t0) / 40          # `cores` chains in parallel
-    chains = int(seconds / (100 * per_sweep * 1.2) * cores)
+    chains = int(seconds / (sweeps * per_sweep * 1.2) * cores)
     chains = max(cores, min(4096, chains // cores * cores))
     t0 = time.perf_counter()
-    r = orc.mc_run(o, chains, 1, 100, 0, move=orc.MOVE_EXCHANGE, seed=2, num_threads=cores)
+    r = orc.mc_run(o, chains, 1, sweeps, 0, move=orc.MOVE_EXCHANGE, seed=2, num_threads=cores)
     w = np.full(chains, 1.0 / chains)
     orc.tdvp_accumulate(op, o, r["samples"], w, with_S=False, num_threads=cores)
     dt = time.perf_counter() - t0
     return dict(samples_per_s=chains / dt, seconds=dt, chains=chains, cores=cores)
 
 
+KERNELS = ("mc_sweep", "block_sweep", "theta", "local_energy", "o_k", "spin_sums", "sr_matvec")
+
+
+class Stepper:
+    """one VMC step: TDVP force on fresh samples, warm-started CG, SR update"""
+
+    def __init__(self, A, psi, op, ens, lr, cg_iters):
+        self.A, self.psi, self.op, self.ens = A, psi, op, ens
+        self.tdvp = A.TDVP(psi.num_params)
+        self.lr, self.cg_iters = lr, cg_iters
+        self.x = None
+        self.iters = []
+
+    def step(self):
+        self.tdvp.eval_F(self.op, self.psi, self.ens)
+        self.x = self.tdvp.solve(1e-3, "cg", rtol=1e-4, max_iter=self.cg_iters, raise_on_failure=False, x0=self.x)
+        self.iters.append(self.tdvp.last_solve["iterations"])
+        self.psi.params = self.psi.params - self.lr * self.x
+
+    def timed(self, steps, warmup):
+        A = self.A
+        for _ in range(warmup):
+            self.step()
+        A.profiler_reset()
+        A.start_profiling()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            self.step()
+        A.stop_profiling()    # synchronises the device
+        dt = time.perf_counter() - t0
+        kms = {k: A.profiler_query(k) for k in KERNELS}
+        return dt / steps * 1e3, kms
+
+
+def measure_tau(A, psi, op, proposal, chains=1024, T=400, seed=23):
+    ens = A.MonteCarloSpins(chains * T, 1, 30, chains, move="exchange", seed=seed)
+    if proposal is not None:
+        ens.set_proposal(proposal, plain_sweeps=0)
+    el = A.local_energies(op, psi, ens).real.reshape(T, chains)
+    return tau_int_sokal(el), tau_batch_means(el), ens.acceptance_rate, el
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--L", type=int, default=10)
     ap.add_argument("--alpha", type=int, default=4)
     ap.add_argument("--chains", type=int, default=16384)
     ap.add_argument("--sweeps", type=int, default=100)
+    ap.add_argument("--opt-steps", type=int, default=60)
+    ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--cg-iters", type=int, default=100)
     ap.add_argument("--lr", type=float, default=0.02)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -120,70 +175,123 @@ def main():
     N, psi, H = build_workload(A, args.L, args.alpha)
     op = A.Operator(H)
     B = args.chains
-    ens = A.MonteCarloSpins(B, args.sweeps, 20, B, move="exchange", seed=17, persistent=True)
-    tdvp = A.TDVP(psi.num_params)
-    shift = 1e-3
+    M = args.alpha * N
 
-    def step():
-        tdvp.eval_F(op, psi, ens)
-        x = tdvp.solve(shift, "cg", rtol=1e-4, max_iter=args.cg_iters, raise_on_failure=False)
-        psi.params = psi.params - args.lr * x
+    # 1. pre-optimisation with the plain sampler (10 sweeps per sample)
+    t0 = time.perf_counter()
+    opt_ens = A.MonteCarloSpins(B, 10, 30, B, move="exchange", seed=5, persistent=True)
+    opt = Stepper(A, psi, op, opt_ens, 0.05, args.cg_iters)
+    for _ in range(args.opt_steps):
+        opt.step()
+    E_opt = opt.tdvp.E_local.real / N
+    t_opt = time.perf_counter() - t0
+    print(f"pre-optimised: E/N = {E_opt:.4f} after {args.opt_steps} SR steps ({t_opt:.1f} s)", file=sys.stderr,
+          flush=True)
 
-    for _ in range(args.warmup):
-        step()
+    # 2. training data from the plain sampler and proposal training
+    t0 = time.perf_counter()
+    data = A.MonteCarloSpins(B * 64, 2, 30, B, move="exchange", seed=7)
+    A.log_psi(psi, data)
+    blocks = A.rect_blocks(args.L, (2, 4))
+    prop = A.BlockProposal(N, blocks, hidden=256, seed=11)
+    loss0, _ = prop.loss_grad(data.last_samples()[:4096], blocks[np.arange(4096) % len(blocks)])
     A.profiler_reset()
     A.start_profiling()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    A.stop_profiling()     # synchronises the device
-    dt = time.perf_counter() - t0
-    ms_step = dt / args.steps * 1e3
-    sweep_ms, sweep_n = A.profiler_query("mc_sweep")
-    kernel_ms = {k: A.profiler_query(k) for k in ("mc_sweep", "theta", "local_energy", "o_k", "spin_sums",
-                                                  "sr_matvec")}
+    loss = prop.fit(data, steps=args.train_steps, batch=4096, lr=2e-3)
+    A.stop_profiling()
+    train_ms = A.profiler_query("proposal_train")[0]
+    t_train = time.perf_counter() - t0
+    print(f"proposal trained: loss {loss0:.4f} -> {loss:.4f} ({t_train:.1f} s)", file=sys.stderr, flush=True)
 
-    # autocorrelation of E_loc, one sweep apart, on 1024 chains of the same psi
-    Bt, T = 1024, 400
-    ens_t = A.MonteCarloSpins(Bt * T, 1, 50, Bt, move="exchange", seed=23)
-    el = A.local_energies(op, psi, ens_t).real.reshape(T, Bt)
-    tau = tau_int_sokal(el)
-    tau_bm = tau_batch_means(el)
-    ess_frac = min(1.0, args.sweeps / (2.0 * tau))
-    ess_step = B * ess_frac
-    value = ess_step / (ms_step * 1e-3)
+    # 3. autocorrelation of E_loc, one (plain or block) sweep apart
+    tau_p, tau_p_bm, acc_p, el_p = measure_tau(A, psi, op, None)
+    tau_l, tau_l_bm, acc_l, el_l = measure_tau(A, psi, op, prop)
+    print(f"tau plain {tau_p:.2f} sweeps, tau learned {tau_l:.2f} block sweeps", file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel: exchange sweeps, 36 flop per hidden unit per proposal
-    M = args.alpha * N
-    flops_launch = B * args.sweeps * N * 36.0 * M
-    avg_ms = sweep_ms / max(sweep_n, 1)
-    achieved = flops_launch / (avg_ms * 1e-3) / 1e12
-    peak = 157.3
+    # 4. timed paths
+    paths = {}
+    s_match = max(1, math.ceil(2 * tau_p))
+    s_learn = max(1, math.ceil(2 * tau_l))
+    for name, sweeps, proposal, tau in (("plain-config", args.sweeps, None, tau_p),
+                                        ("plain-matched", s_match, None, tau_p),
+                                        ("learned", s_learn, prop, tau_l)):
+        ens = A.MonteCarloSpins(B, sweeps, 30, B, move="exchange", seed=17, persistent=True)
+        if proposal is not None:
+            ens.set_proposal(proposal, plain_sweeps=0)
+        st = Stepper(A, psi.copy(), op, ens, args.lr, args.cg_iters)
+        ms, kms = st.timed(args.steps, args.warmup)
+        ess = B * min(1.0, sweeps / (2 * tau))
+        paths[name] = dict(sweeps_per_sample=sweeps, ms_per_step=ms, ess_per_step=ess,
+                           ess_per_s=ess / (ms * 1e-3), tau_int=tau, acceptance=ens.acceptance_rate,
+                           E_per_site=st.tdvp.E_local.real / N, cg_iterations_mean=float(np.mean(st.iters)),
+                           kernel_ms_total={k: v[0] for k, v in kms.items()},
+                           kernel_launches={k: v[1] for k, v in kms.items()})
+        print(f"{name}: {ms:.2f} ms/step, ESS/s {paths[name]['ess_per_s']:.0f}", file=sys.stderr, flush=True)
+    head = paths["learned"]
+
+    # energies of the two samplers on the same state (samples of step 3)
+    def mean_se(el):
+        cm = el.mean(axis=0)
+        return float(cm.mean()), float(cm.std(ddof=1) / math.sqrt(el.shape[1]))
+    Ep, sep = mean_se(el_p)
+    El, sel = mean_se(el_l)
+
+    # roofline of the headline path's dominant kernel
+    kt = head["kernel_ms_total"]
+    dom = max(("block_sweep", "spin_sums", "sr_matvec", "local_energy"), key=lambda k: kt[k])
+    launches = head["kernel_launches"][dom]
+    avg_ms = kt[dom] / max(launches, 1)
+    bs = blocks.shape[1]
+    if dom == "block_sweep":
+        moves = s_learn * math.ceil(N / bs)
+        # per block move: second layer 2 bs H, angle update of <= bs rows 4 bs M, log-cosh sum 26 M
+        work = B * moves * (2.0 * bs * 256 + 4.0 * bs * M + 26.0 * M)
+        roof = {"bound": "mfma", "peak": 157.3, "engine": "VALU fp32 (vector fp32 peak 157.3 TFLOP/s)"}
+    elif dom == "spin_sums":
+        Mpad = 448
+        work = 2.0 * N * B * (2 * Mpad + 16)
+        roof = {"bound": "mfma", "peak": 78.6, "engine": "MFMA fp64"}
+    elif dom == "sr_matvec":
+        Mpad = 448
+        work = 3 * 2.0 * B * 112 * 2 * Mpad
+        roof = {"bound": "mfma", "peak": 2500.0, "engine": "MFMA bf16 (x3 split)"}
+    else:
+        work = B * 130 * 36.0 * M
+        roof = {"bound": "mfma", "peak": 157.3, "engine": "VALU fp32"}
+    achieved = work / (avg_ms * 1e-3) / 1e12
 
     cpu = None
     if not args.no_cpu:
-        c = cpu_baseline(A, args.L, args.alpha, 3, args.cpu_seconds)
+        c = cpu_baseline(A, args.L, args.alpha, 3, args.cpu_seconds, args.sweeps)
+        ess_frac = min(1.0, args.sweeps / (2 * tau_p))
         cpu = {"value": c["samples_per_s"] * ess_frac, "unit": "ESS/s", "cores": c["cores"], "kind": "port",
                "sample": f"{c['chains']} chains x {args.sweeps} exchange sweeps + E_loc + O_k in fp64 "
-                         f"({c['seconds']:.1f} s, OpenMP over chains); SR solve excluded; ESS fraction "
-                         f"of the GPU run applied"}
+                         f"({c['seconds']:.1f} s, OpenMP over chains) on the seeded RBM; SR solve excluded; "
+                         f"ESS fraction of the plain GPU path applied"}
 
     line = {
         "metric": "ESS/s with E_loc+O_k+SR, RBM N=100 alpha=4: learned-proposal bf16 vs plain fp32",
-        "value": value, "unit": "ESS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": head["ess_per_s"], "unit": "ESS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"Heisenberg {args.L}x{args.L} PBC, complex RBM alpha={args.alpha} "
-                               f"(P={psi.num_params}), {B} chains x {args.sweeps} exchange sweeps, "
-                               "plain sampler, fp32 + bf16x3 angle GEMM, CG SR (fp64 dots)",
-                   "sampler": "plain-exchange", "tau_int_sweeps": tau, "tau_batch_means": tau_bm,
-                   "ess_per_step": ess_step, "samples_per_step": B, "acceptance": ens.acceptance_rate,
-                   "E_per_site": tdvp.E_local.real / N,
-                   "kernel_ms_total": {k: v[0] for k, v in kernel_ms.items()},
-                   "cg_iterations": tdvp.last_solve["iterations"]},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None, "kernel": "mc_sweep",
-                     "avg_launch_ms": avg_ms},
+                               f"(P={psi.num_params}), {B} persistent chains, S_z=0; learned 2x4 block proposal "
+                               f"(MLP {N}-256-{N}, bf16 inference) vs plain exchange sweeps; fp32 sweeps/E_loc/O_k, "
+                               "bf16x3 angle GEMM, warm-started CG SR (fp64)",
+                   "learned_vs_plain_config": head["ess_per_s"] / paths["plain-config"]["ess_per_s"],
+                   "learned_vs_plain_matched": head["ess_per_s"] / paths["plain-matched"]["ess_per_s"],
+                   "paths": paths,
+                   "pre_optimisation": {"sr_steps": args.opt_steps, "E_per_site": E_opt, "seconds": t_opt},
+                   "training": {"samples": B * 64, "adam_steps": args.train_steps, "batch": 4096,
+                                "loss_before": loss0, "loss_after": loss, "device_ms": train_ms,
+                                "seconds": t_train},
+                   "tau": {"plain_sweeps": tau_p, "plain_batch_means": tau_p_bm,
+                           "learned_block_sweeps": tau_l, "learned_batch_means": tau_l_bm,
+                           "acceptance_plain": acc_p, "acceptance_learned": acc_l},
+                   "energy_check": {"E_plain": Ep, "se_plain": sep, "E_learned": El, "se_learned": sel,
+                                    "rel_diff": abs(El - Ep) / abs(Ep)}},
+        "roofline": dict(roof, achieved=achieved, unit="TFLOP/s", frac=achieved / roof["peak"], traffic=None,
+                         kernel=dom, avg_launch_ms=avg_ms),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -155,9 +155,12 @@ int ann_tdvp_get_energy(ann_tdvp tdvp, double* E2_complex, double* E2_local, dou
 int ann_tdvp_S_dot_vector(ann_tdvp tdvp, const double* v, double* out);
 /* additive (absent from the reference): solve (S + shift I) x = F.
  * method 0 = fp64 Cholesky of the dense S (needs eval with_S = 1),
- * method 1 = conjugate gradients on the matrix-free S (fp64 vectors and dots). */
+ * method 1 = conjugate gradients on the matrix-free S (fp64 vectors and dots,
+ * Jacobi preconditioner) started from zero, method 2 = the same started from
+ * the x passed in (warm start from the previous step's solution). */
 #define ANN_SOLVE_CHOLESKY 0
 #define ANN_SOLVE_CG 1
+#define ANN_SOLVE_CG_WARM 2
 int ann_tdvp_solve(ann_tdvp tdvp, double shift, int32_t method, double rtol, int32_t max_iter, double* x,
                    int32_t* iterations, double* residual);
 
