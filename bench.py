@@ -123,9 +123,12 @@ class Stepper:
 
     def step(self):
         self.tdvp.eval_F(self.op, self.psi, self.ens)
-        self.x = self.tdvp.solve(1e-3, "cg", rtol=1e-4, max_iter=self.cg_iters, raise_on_failure=False, x0=self.x)
+        x = self.tdvp.solve(1e-3, "cg", rtol=1e-4, max_iter=self.cg_iters, raise_on_failure=False, x0=self.x)
         self.iters.append(self.tdvp.last_solve["iterations"])
-        self.psi.params = self.psi.params - self.lr * self.x
+        if not np.all(np.isfinite(x)):
+            raise FloatingPointError("SR update is not finite")
+        self.x = x
+        self.psi.params = self.psi.params - self.lr * x
 
     def timed(self, steps, warmup):
         A = self.A
@@ -159,7 +162,8 @@ def main():
     ap.add_argument("--alpha", type=int, default=4)
     ap.add_argument("--chains", type=int, default=16384)
     ap.add_argument("--sweeps", type=int, default=100)
-    ap.add_argument("--opt-steps", type=int, default=60)
+    ap.add_argument("--opt-steps", type=int, default=40)
+    ap.add_argument("--opt-lr", type=float, default=0.02)
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--cg-iters", type=int, default=100)
     ap.add_argument("--lr", type=float, default=0.02)
@@ -180,7 +184,7 @@ def main():
     # 1. pre-optimisation with the plain sampler (10 sweeps per sample)
     t0 = time.perf_counter()
     opt_ens = A.MonteCarloSpins(B, 10, 30, B, move="exchange", seed=5, persistent=True)
-    opt = Stepper(A, psi, op, opt_ens, 0.05, args.cg_iters)
+    opt = Stepper(A, psi, op, opt_ens, args.opt_lr, args.cg_iters)
     for _ in range(args.opt_steps):
         opt.step()
     E_opt = opt.tdvp.E_local.real / N

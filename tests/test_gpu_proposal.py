@@ -89,22 +89,24 @@ def _chain_stats(values, chains):
     return cm.mean(), cm.std(ddof=1) / math.sqrt(chains)
 
 
-@pytest.mark.parametrize("precision,scale", [("fp64", 3.0), ("fp32", 3.0), ("fp32", 0.0)])
+@pytest.mark.parametrize("precision,scale", [("fp64", 3.0), ("fp32", 3.0), ("fp32", 0.0), ("fp32", None)])
 def test_block_sampler_preserves_psi_squared(precision, scale):
     """block moves alone (no plain sweeps after plain thermalisation) sample |psi|^2
-    restricted to S_z = 0; scale 0 is the uniform proposal (proposal ratio 1)"""
+    restricted to S_z = 0; scale 0 is the uniform proposal (proposal ratio 1) and
+    scale None the plain exchange sampler (the control)"""
     N = 12
-    psi = A.seeded_rbm_deep(N, 24, 21, sigma=0.25)
+    psi = A.seeded_rbm_deep(N, 24, 21, sigma=0.1)
     psi.precision = precision
     o = orc.deep(N, psi.a, psi.b, psi.connections, psi.W, psi.final_weights, 0.0)
     H = ham.heisenberg_chain(N)
     E_ex, c_ex = _sector_exact(o, N, H)
-    prop = A.BlockProposal(N, A.chain_blocks(N, 4), hidden=64, seed=5)
-    # a strongly non-uniform network: a wrong proposal ratio would bias the result
-    prop.params = scale * prop.params
     chains, steps = 2048, 48
     ens = A.MonteCarloSpins(chains * steps, 2, 10, chains, move="exchange", seed=8)
-    ens.set_proposal(prop, plain_sweeps=0)
+    if scale is not None:
+        prop = A.BlockProposal(N, A.chain_blocks(N, 4), hidden=64, seed=5)
+        # a strongly non-uniform network: a wrong proposal ratio would bias the result
+        prop.params = scale * prop.params
+        ens.set_proposal(prop, plain_sweeps=0)
     el = A.local_energies(A.Operator(H), psi, ens).real
     s = ens.last_samples()[:, 0].astype(np.int64)
     assert np.all([bin(int(x)).count("1") == N // 2 for x in s[:64]])
@@ -116,8 +118,9 @@ def test_block_sampler_preserves_psi_squared(precision, scale):
     assert 0.0 < ens.acceptance_rate < 1.0
 
 
-def _optimise_ring(N=16, M=32, steps=120, lr=0.05):
-    """imaginary-time SR on the Heisenberg ring with exact sums"""
+def _optimise_ring(N=12, M=24, steps=50, lr=0.02):
+    """imaginary-time SR on the Heisenberg ring with exact sums (the polynomial
+    activations diverge if the weights grow much further than 50 such steps take them)"""
     psi = A.seeded_rbm_deep(N, M, 4, sigma=0.05)
     psi.precision = "fp64"
     op = A.Operator(ham.heisenberg_chain(N))
@@ -148,16 +151,16 @@ def _tau_int(series):
 
 
 def test_trained_proposal_gains_over_untrained():
-    N = 16
+    N = 12
     psi, op, E_opt = _optimise_ring(N)
-    assert E_opt < -20.0      # ground state of the 16-site ring: -28.569 (Pauli units)
+    assert E_opt < -15.0      # ground state of the 12-site ring: -21.55 (Pauli units)
     chains = 1024
     # training data: plain exchange samples
     data = A.MonteCarloSpins(chains * 64, 2, 20, chains, move="exchange", seed=1)
     A.log_psi(psi, data)
-    blocks = A.chain_blocks(N, 8)
-    trained = A.BlockProposal(N, blocks, hidden=128, seed=2)
-    untrained = A.BlockProposal(N, blocks, hidden=128, seed=2)
+    blocks = A.chain_blocks(N, 4)
+    trained = A.BlockProposal(N, blocks, hidden=64, seed=2)
+    untrained = A.BlockProposal(N, blocks, hidden=64, seed=2)
     loss0, _ = trained.loss_grad(data.last_samples()[:4096], blocks[np.arange(4096) % N])
     loss1 = trained.fit(data, steps=400, batch=2048, lr=3e-3)
     assert loss1 < 0.8 * loss0
