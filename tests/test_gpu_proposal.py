@@ -150,27 +150,41 @@ def _tau_int(series):
     return tau
 
 
+def _staggered_sq(confs, N):
+    """squared staggered magnetisation of a ring, per sample"""
+    bits = (confs[..., None] >> np.arange(N, dtype=np.uint64)) & np.uint64(1)
+    s = 2.0 * bits.astype(np.float64) - 1.0
+    return (s * (-1.0) ** np.arange(N)).sum(axis=-1) ** 2
+
+
 def test_trained_proposal_gains_over_untrained():
+    """12-site Heisenberg ring after 50 exact SR steps; blocks of 6 consecutive sites.
+    Trained on plain-sampler configurations, the network's block moves are accepted
+    more often and decorrelate the staggered magnetisation and E_loc faster (per
+    block sweep) than the same network untrained; energies agree."""
     N = 12
     psi, op, E_opt = _optimise_ring(N)
     assert E_opt < -15.0      # ground state of the 12-site ring: -21.55 (Pauli units)
-    chains = 1024
+    chains, T = 1024, 200
     # training data: plain exchange samples
     data = A.MonteCarloSpins(chains * 64, 2, 20, chains, move="exchange", seed=1)
     A.log_psi(psi, data)
-    blocks = A.chain_blocks(N, 4)
+    blocks = A.chain_blocks(N, 6)
     trained = A.BlockProposal(N, blocks, hidden=64, seed=2)
     untrained = A.BlockProposal(N, blocks, hidden=64, seed=2)
-    loss0, _ = trained.loss_grad(data.last_samples()[:4096], blocks[np.arange(4096) % N])
+    rows = np.arange(4096)
+    loss0, _ = trained.loss_grad(data.last_samples()[:4096], blocks[rows % N])
     loss1 = trained.fit(data, steps=400, batch=2048, lr=3e-3)
-    assert loss1 < 0.8 * loss0
+    assert loss1 < 0.9 * loss0
     res = {}
     for name, prop in (("trained", trained), ("untrained", untrained)):
-        ens = A.MonteCarloSpins(chains * 200, 1, 5, chains, move="exchange", seed=9)
+        ens = A.MonteCarloSpins(chains * T, 1, 5, chains, move="exchange", seed=9)
         ens.set_proposal(prop, plain_sweeps=0)
-        el = A.local_energies(op, psi, ens).real.reshape(200, chains)
-        res[name] = (ens.acceptance_rate, _tau_int(el), el.mean())
-    (acc_t, tau_t, E_t), (acc_u, tau_u, E_u) = res["trained"], res["untrained"]
-    assert acc_t > 1.5 * acc_u, res
-    assert tau_t < 0.7 * tau_u, res
-    assert abs(E_t - E_opt) < 0.02 * abs(E_opt)
+        el = A.local_energies(op, psi, ens).real.reshape(T, chains)
+        stag = _staggered_sq(ens.last_samples()[:, 0].reshape(T, chains), N)
+        res[name] = dict(acc=ens.acceptance_rate, tau_E=_tau_int(el), tau_stag=_tau_int(stag), E=el.mean())
+    t, u = res["trained"], res["untrained"]
+    assert t["acc"] > 1.5 * u["acc"], res
+    assert t["tau_stag"] < 0.7 * u["tau_stag"], res
+    assert t["tau_E"] < u["tau_E"], res
+    assert abs(t["E"] - u["E"]) < 0.01 * abs(u["E"]), res
