@@ -103,6 +103,48 @@ void launch_structured_Ov(const DenseView<T>& net, const uint64_t* confs, int wo
                           hipStream_t stream);
 
 // weighted fp64 reductions
+// fp32 mode, bf16 matrix cores (sr_mfma.hip).  u = O.v of the structured O:
+// V [N][Mpad] (W part of v re-indexed onto visible units), vb, va; vfrag holds
+// 3 * npad/16 * 2Mpad/16 * 256 bf16, upart (2Mpad/64) * count complex.
+void launch_ov_mfma(const DenseView<float>& net, const cplx<float>* V, int npad, uint16_t* vfrag,
+                    const uint64_t* confs, int words, int64_t count, const cplx<float>* g, const cplx<double>* vb,
+                    const cplx<double>* va, cplx<double>* upart, cplx<double>* u, hipStream_t stream);
+// the spin GEMM of launch_spin_weighted_sums on bf16 matrix cores (fp32 accumulation
+// per sample chunk, chunks added in fp64)
+void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, int words, const cplx<float>* g,
+                           int mode, const cplx<double>* coef, int64_t count, cplx<double>* Yw, cplx<double>* Yb,
+                           cplx<double>* Ya, hipStream_t stream);
+
+// the spin GEMM's partition of the samples and its fp32 partial buffer ([chunks][n] floats)
+struct SpinPlan {
+    int vpad = 0, ncols = 0, cblocks = 0, zgroups = 0, chunks = 0;
+    int64_t chunk = 0, n = 0;
+};
+SpinPlan spin_plan(int N, int Mpad, int64_t count);
+void launch_spin_gemm_bf16(const DenseView<float>& net, const uint64_t* confs, int words, const cplx<float>* g,
+                           int mode, const cplx<double>* coef, int64_t count, const SpinPlan& p, float* partial,
+                           hipStream_t stream);
+// workspace of the fused S.v (fp32 dense nets); sizes: frag 3 * npad/16 * 2Mpad/16 * 256,
+// va N, vb Mpad, upart (2Mpad/64) * count, u count, spin_part chunks * n, opart 128,
+// ypart 2 * sr_y_parts(P)
+struct SrWorkspace {
+    uint16_t* frag;
+    cplx<double>* va;
+    cplx<double>* vb;
+    cplx<double>* upart;
+    cplx<double>* u;
+    float* spin_part;
+    double* opart;
+    double* ypart;
+};
+// y = (S + shift) v with S = sum_s w_s conj(O_s) O_s^T - conj(<O>) <O>^T for the
+// structured O; also leaves the block partials of v^H y in ws.ypart ([sr_y_parts(P)][2])
+int sr_y_parts(int P);
+void launch_sr_matvec_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const cplx<double>* obar,
+                           const double* w, const uint64_t* confs, int words, int64_t count, const cplx<float>* g,
+                           int npad, double shift, const SpinPlan& plan, SrWorkspace ws, cplx<double>* y,
+                           hipStream_t stream);
+
 void launch_weighted_sums(const cplx<double>* e_loc, const double* w, int64_t count,
                           double* out4 /* E.re, E.im, E2, sum w */, hipStream_t stream);
 

@@ -30,6 +30,17 @@ void launch_cg_update_dev(cplx<double>* x, cplx<double>* r, const cplx<double>* 
                           const double* rz, const double* pAp, int64_t n, hipStream_t stream);
 void launch_cg_dir_dev(cplx<double>* p, const cplx<double>* z, const double* rz_new, const double* rz_old, int64_t n,
                        hipStream_t stream);
+// preconditioned CG (cg.hip): dot products as kCgParts block partials [b][4] = (r^H z, -, r^H r, -);
+// a matvec's partials of p^H Ap are [b][2]
+constexpr int kCgParts = 64;
+void launch_pcg_init(const cplx<double>* r, cplx<double>* z, cplx<double>* p, const double* diag, double shift,
+                     int64_t n, double* part, hipStream_t stream);
+void launch_pcg_pap(const cplx<double>* p, const cplx<double>* Ap, int64_t n, double* part, hipStream_t stream);
+void launch_pcg_step(cplx<double>* x, cplx<double>* r, cplx<double>* z, const cplx<double>* p,
+                     const cplx<double>* Ap, const double* diag, double shift, int64_t n, const double* rzpart,
+                     const double* apart, int a_parts, double* newpart, hipStream_t stream);
+void launch_pcg_dir(cplx<double>* p, const cplx<double>* z, int64_t n, const double* newpart, const double* oldpart,
+                    hipStream_t stream);
 void launch_center_scale(cplx<double>* u, const double* w, const double* c2, int64_t n, hipStream_t stream);
 void launch_cg_step(cplx<double>* x, cplx<double>* r, const cplx<double>* p, const cplx<double>* Ap, double alpha,
                     int64_t n, hipStream_t stream);
@@ -44,5 +55,8 @@ template <typename Tin, typename Tout>
 void launch_convert(const cplx<Tin>* a, cplx<Tout>* b, int64_t n, hipStream_t stream);
 void launch_fill(double* a, double v, int64_t n, hipStream_t stream);
 void launch_weight_coef(const double* w, const cplx<double>* e, int64_t n, cplx<double>* coef, hipStream_t stream);
+// coef = w (e - E): F = sum_s coef_s conj(O_s) without the cancellation of E conj(<O>)
+void launch_weight_coef_centered(const double* w, const cplx<double>* e, cplx<double> E, int64_t n,
+                                 cplx<double>* coef, hipStream_t stream);
 
 }  // namespace ann

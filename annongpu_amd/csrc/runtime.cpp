@@ -650,7 +650,10 @@ void structured_sums(Tdvp& t, const DenseView<T>& net, const uint64_t* confs, in
     t.Yb.ensure(net.Mpad);
     t.Ya.ensure(net.N);
     ScopedTimer timer("spin_sums");
-    launch_spin_weighted_sums<T>(net, confs, words, g, conj_g, coef, count, t.Yw.p, t.Yb.p, t.Ya.p, S());
+    if constexpr (sizeof(T) == 4)
+        launch_spin_sums_bf16(net, confs, words, g, conj_g, coef, count, t.Yw.p, t.Yb.p, t.Ya.p, S());
+    else
+        launch_spin_weighted_sums<T>(net, confs, words, g, conj_g, coef, count, t.Yw.p, t.Yb.p, t.Ya.p, S());
     if (out)
         launch_pack_y(net.layout, net.N, net.M, net.Mpad, net.C, net.conn, t.Ya.p, t.Yb.p, t.Yw.p, v, shift,
                       net.num_params, out, S());
@@ -689,15 +692,14 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     structured_sums<T>(t, net, s.confs.p, s.words, g.p, 2, coef.p, n, nullptr, 0.0, nullptr);
     t.diag.ensure(P);
     launch_sr_diag(net.layout, net.N, net.M, net.Mpad, t.wsum, t.Yb.p, t.Obar.p, P, t.diag.p, S());
-    launch_weight_coef(s.w.p, s.e_loc.p, n, coef.p, S());
+    // F = sum w conj(O) E_loc - E conj(<O>) (TDVP.cu.template:300) = sum w (E_loc - E) conj(O)
+    launch_weight_coef_centered(s.w.p, s.e_loc.p, {t.E.real(), t.E.imag()}, n, coef.p, S());
     structured_sums<T>(t, net, s.confs.p, s.words, g.p, 1, coef.p, n, t.Fd.p, 0.0, nullptr);
     Runtime::get().sync();
     t.O.resize(P);
     t.F.resize(P);
     t.Obar.download(reinterpret_cast<cplx<double>*>(t.O.data()), P);
     t.Fd.download(reinterpret_cast<cplx<double>*>(t.F.data()), P);
-    for (int k = 0; k < P; k++) t.F[k] -= t.E * std::conj(t.O[k]);   // TDVP.cu.template:300
-    t.Fd.upload(reinterpret_cast<const cplx<double>*>(t.F.data()), P);
     // keep the structured O for S.v / CG
     t.precision = psi.precision;
     t.words = s.words;
@@ -790,8 +792,6 @@ void tdvp_eval_generic(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with
         launch_ok_colsum<T>(O.p, n, P, coef.p, 0, 0.0, nullptr, t.part.p, t.Obar.p, S());
         launch_ok_colsum<T>(O.p, n, P, coef.p, 2, 0.0, nullptr, t.part.p, sq.p, S());
         launch_diag_from_sums(sq.p, t.Obar.p, P, t.diag.p, S());
-        launch_weight_coef(s.w.p, s.e_loc.p, n, coef.p, S());
-        launch_ok_colsum<T>(O.p, n, P, coef.p, 1, 0.0, nullptr, t.part.p, t.Fd.p, S());
     }
     double h[4];
     ANN_HIP_CHECK(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, S()));
@@ -799,12 +799,16 @@ void tdvp_eval_generic(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with
     t.E = cd(h[0], h[1]);
     t.E2 = h[2];
     t.wsum = h[3];
+    {
+        // F = sum w (E_loc - E) conj(O)  (TDVP.cu.template:300 without the cancellation)
+        ScopedTimer timer("spin_sums");
+        launch_weight_coef_centered(s.w.p, s.e_loc.p, {t.E.real(), t.E.imag()}, n, coef.p, S());
+        launch_ok_colsum<T>(O.p, n, P, coef.p, 1, 0.0, nullptr, t.part.p, t.Fd.p, S());
+    }
     t.O.resize(P);
     t.F.resize(P);
     t.Obar.download(reinterpret_cast<cplx<double>*>(t.O.data()), P);
     t.Fd.download(reinterpret_cast<cplx<double>*>(t.F.data()), P);
-    for (int k = 0; k < P; k++) t.F[k] -= t.E * std::conj(t.O[k]);   // TDVP.cu.template:300
-    t.Fd.upload(reinterpret_cast<const cplx<double>*>(t.F.data()), P);
     t.generic = true;
     t.precision = psi.precision;
     t.words = s.words;
@@ -858,8 +862,11 @@ struct Ws<double> {
     static DevBuf<cplx<double>>& zero(Tdvp& t) { return t.zero64; }
 };
 
+// y = (S + shift) v for the structured O of a dense net.  The fp32 path runs the
+// fused bf16 matrix-core pipeline and also leaves the partials of v^H y (*ypart).
 template <typename T>
-void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y) {
+void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y, const double** ypart,
+                  int* yparts) {
     const int P = int(t.P);
     DenseView<T> net{};
     net.N = t.N;
@@ -869,40 +876,54 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y)
     net.layout = t.layout;
     net.num_params = P;
     net.conn = t.layout == LAYOUT_DEEP1 ? t.conn.p : nullptr;
-    auto& V = Ws<T>::V(t);
-    auto& SV = Ws<T>::SV(t);
-    auto& zero = Ws<T>::zero(t);
+    auto& g = Prec<T>::g(t);
     t.va.ensure(t.N);
     t.vb.ensure(t.Mpad);
     t.u.ensure(t.count);
-    t.c1.ensure(2);
-    t.dot_scratch.ensure(256);
-    V.ensure(size_t(t.N) * t.Mpad);
-    SV.ensure(size_t(t.count) * t.Mpad);
-    if (zero.n < size_t(t.Mpad)) {
-        zero.ensure(t.Mpad);
-        ANN_HIP_CHECK(hipMemsetAsync(zero.p, 0, sizeof(cplx<T>) * t.Mpad, S()));
-    }
-    auto& g = Prec<T>::g(t);
-    {
+    if constexpr (sizeof(T) == 4) {
+        const SpinPlan plan = spin_plan(t.N, t.Mpad, t.count);
+        t.frag.ensure(size_t(3) * (t.npad / 16) * (2 * t.Mpad / 16) * 64 * 4);
+        t.upart.ensure(size_t(2 * t.Mpad / 64) * t.count);
+        t.spin_part.ensure(size_t(plan.chunks) * plan.n);
+        t.opart.ensure(256);
+        t.ypart.ensure(512);
+        const SrWorkspace ws{t.frag.p, t.va.p, t.vb.p, t.upart.p, t.u.p, t.spin_part.p, t.opart.p, t.ypart.p};
         ScopedTimer timer("sr_matvec");
-        launch_unpack_v<T>(net, t.inv.p, v, t.va.p, t.vb.p, V.p, S());
-        DenseView<T> vnet = net;
-        vnet.Wd = V.p;
-        vnet.b = zero.p;
-        if constexpr (sizeof(T) == 4) {
-            t.frag.ensure(size_t(3) * (t.npad / 16) * (2 * t.Mpad / 16) * 64 * 4);
-            launch_pack_fragments(V.p, t.N, t.Mpad, t.npad, t.frag.p, S());
-            launch_theta_mfma(vnet, t.frag.p, t.npad, t.confs.p, t.words, t.count, SV.p, S());
-        } else {
-            launch_theta_valu<T>(vnet, t.confs.p, t.words, t.count, SV.p, S());
+        launch_sr_matvec_bf16(net, t.inv.p, v, t.Obar.p, t.w.p, t.confs.p, t.words, t.count, g.p, t.npad, shift,
+                              plan, ws, y, S());
+        if (ypart) *ypart = t.ypart.p;
+        if (yparts) *yparts = sr_y_parts(P);
+        return;
+    } else {
+        auto& V = Ws<T>::V(t);
+        auto& SV = Ws<T>::SV(t);
+        auto& zero = Ws<T>::zero(t);
+        t.c1.ensure(2);
+        t.dot_scratch.ensure(256);
+        V.ensure(size_t(t.N) * t.Mpad);
+        SV.ensure(size_t(t.count) * t.Mpad);
+        if (zero.n < size_t(t.Mpad)) {
+            zero.ensure(t.Mpad);
+            ANN_HIP_CHECK(hipMemsetAsync(zero.p, 0, sizeof(cplx<T>) * t.Mpad, S()));
         }
-        launch_structured_Ov<T>(net, t.confs.p, t.words, g.p, SV.p, t.layout == LAYOUT_DEEP1 ? t.vb.p : nullptr,
-                                t.layout == LAYOUT_DEEP1 ? t.va.p : nullptr, t.count, t.u.p, S());
-        launch_dot2(t.Obar.p, v, nullptr, nullptr, P, 0, t.dot_scratch.p, reinterpret_cast<double*>(t.c1.p), S());
-        launch_center_scale(t.u.p, t.w.p, reinterpret_cast<const double*>(t.c1.p), t.count, S());
+        {
+            ScopedTimer timer("sr_matvec");
+            launch_unpack_v<T>(net, t.inv.p, v, t.va.p, t.vb.p, V.p, S());
+            DenseView<T> vnet = net;
+            vnet.Wd = V.p;
+            vnet.b = zero.p;
+            const cplx<double>* vb = t.layout == LAYOUT_DEEP1 ? t.vb.p : nullptr;
+            const cplx<double>* va = t.layout == LAYOUT_DEEP1 ? t.va.p : nullptr;
+            launch_theta_valu<T>(vnet, t.confs.p, t.words, t.count, SV.p, S());
+            launch_structured_Ov<T>(net, t.confs.p, t.words, g.p, SV.p, vb, va, t.count, t.u.p, S());
+            launch_dot2(t.Obar.p, v, nullptr, nullptr, P, 0, t.dot_scratch.p, reinterpret_cast<double*>(t.c1.p),
+                        S());
+            launch_center_scale(t.u.p, t.w.p, reinterpret_cast<const double*>(t.c1.p), t.count, S());
+        }
+        structured_sums<T>(t, net, t.confs.p, t.words, g.p, 1, t.u.p, t.count, y, shift, v);
+        if (ypart) *ypart = nullptr;
+        if (yparts) *yparts = 0;
     }
-    structured_sums<T>(t, net, t.confs.p, t.words, g.p, 1, t.u.p, t.count, y, shift, v);
 }
 
 }  // namespace
@@ -937,7 +958,10 @@ void ok_weighted_sum_t(Psi& psi, Samples& s, const cplx<double>* coef, int conj,
     Yw.ensure(size_t(psi.N) * psi.Mpad);
     Yb.ensure(psi.Mpad);
     Ya.ensure(psi.N);
-    launch_spin_weighted_sums<T>(net, s.confs.p, s.words, g.p, conj, coef, s.count, Yw.p, Yb.p, Ya.p, S());
+    if constexpr (sizeof(T) == 4)
+        launch_spin_sums_bf16(net, s.confs.p, s.words, g.p, conj, coef, s.count, Yw.p, Yb.p, Ya.p, S());
+    else
+        launch_spin_weighted_sums<T>(net, s.confs.p, s.words, g.p, conj, coef, s.count, Yw.p, Yb.p, Ya.p, S());
     launch_pack_y(net.layout, net.N, net.M, net.Mpad, net.C, net.conn, Ya.p, Yb.p, Yw.p, nullptr, 0.0,
                   psi.num_params, out, S());
     Runtime::get().sync();
@@ -969,7 +993,9 @@ void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
         tdvp_eval_t<float>(t, op, psi, ens, with_S);
 }
 
-void tdvp_S_dot(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y) {
+void tdvp_S_dot(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y, const double** ypart, int* yparts) {
+    if (ypart) *ypart = nullptr;
+    if (yparts) *yparts = 0;
     if (!t.has_samples) throw std::logic_error("TDVP.eval must run before S_dot_vector");
     if (t.generic) {
         if (t.precision == 0)
@@ -979,9 +1005,9 @@ void tdvp_S_dot(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y) {
         return;
     }
     if (t.precision == 0)
-        tdvp_S_dot_t<double>(t, v, shift, y);
+        tdvp_S_dot_t<double>(t, v, shift, y, ypart, yparts);
     else
-        tdvp_S_dot_t<float>(t, v, shift, y);
+        tdvp_S_dot_t<float>(t, v, shift, y, ypart, yparts);
 }
 
 int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x_out, int* iters, double* res,
@@ -1025,20 +1051,23 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         return 0;
     }
     // conjugate gradients with a Jacobi (diagonal) preconditioner.  Vectors, dot
-    // products and the CG scalars stay on the device (fp64); the host reads the
-    // residual norm every `check` iterations only.
+    // products and the CG scalars stay on the device (fp64) as per-block partial
+    // sums; the host reads the residual norm every `check` iterations only.
     if (!t.has_samples) throw std::logic_error("CG solve needs TDVP.eval first");
     DevBuf<cplx<double>> r, z, p, Ap;
-    DevBuf<double> scal, scratch;
+    DevBuf<double> part, apart, scratch;
     r.ensure(P);
     z.ensure(P);
     p.ensure(P);
     Ap.ensure(P);
     scratch.ensure(256);
-    // per iteration k: [k*8 + 0..3] = (r.z, r.r), [k*8 + 4..7] = (p.Ap)
-    scal.ensure(size_t(max_iter + 3) * 8);
-    double* fnorm2 = scal.p + size_t(max_iter + 2) * 8;
-    launch_dot2(t.Fd.p, t.Fd.p, nullptr, nullptr, P, 1, scratch.p, fnorm2, S());
+    // partials of (r^H z, -, r^H r, -) per iteration, and of p^H Ap
+    const size_t stride = size_t(kCgParts) * 4;
+    part.ensure(size_t(max_iter + 1) * stride);
+    apart.ensure(size_t(kCgParts) * 2);
+    DevBuf<double> fnorm2;
+    fnorm2.ensure(4);
+    launch_dot2(t.Fd.p, t.Fd.p, nullptr, nullptr, P, 1, scratch.p, fnorm2.p, S());
     if (x0) {
         // warm start: r = F - (S + shift) x0
         x.upload(reinterpret_cast<const cplx<double>*>(x0), P);
@@ -1049,37 +1078,36 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         ANN_HIP_CHECK(hipMemsetAsync(x.p, 0, sizeof(cplx<double>) * P, S()));
         ANN_HIP_CHECK(hipMemcpyAsync(r.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
     }
-    const bool jacobi = t.diag.p != nullptr && method == 1;
-    auto precondition = [&]() {
-        if (jacobi)
-            launch_precond(z.p, r.p, t.diag.p, shift, P, S());
-        else
-            ANN_HIP_CHECK(hipMemcpyAsync(z.p, r.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
-    };
-    precondition();
-    ANN_HIP_CHECK(hipMemcpyAsync(p.p, z.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
-    launch_dot2(r.p, z.p, r.p, r.p, P, 1, scratch.p, scal.p, S());
+    const double* diag = t.diag.p != nullptr && method == 1 ? t.diag.p : nullptr;
+    launch_pcg_init(r.p, z.p, p.p, diag, shift, P, part.p, S());
+    std::vector<double> hp(stride);
     auto read_rr = [&](int k) {
-        double h = 0.0;
-        ANN_HIP_CHECK(hipMemcpyAsync(&h, scal.p + size_t(k) * 8 + 2, sizeof(double), hipMemcpyDeviceToHost, S()));
+        ANN_HIP_CHECK(hipMemcpyAsync(hp.data(), part.p + size_t(k) * stride, sizeof(double) * stride,
+                                     hipMemcpyDeviceToHost, S()));
         Runtime::get().sync();
+        double h = 0.0;
+        for (int b = 0; b < kCgParts; b++) h += hp[size_t(b) * 4 + 2];
         return h;
     };
     double fn2 = 0.0;
-    ANN_HIP_CHECK(hipMemcpyAsync(&fn2, fnorm2, sizeof(double), hipMemcpyDeviceToHost, S()));
+    ANN_HIP_CHECK(hipMemcpyAsync(&fn2, fnorm2.p, sizeof(double), hipMemcpyDeviceToHost, S()));
     double rr = read_rr(0);
     const double bnorm = std::sqrt(fn2);
     const int check = 4;
     int it = 0;
     while (it < max_iter && std::sqrt(rr) > rtol * bnorm) {
-        double* cur = scal.p + size_t(it) * 8;
-        double* nxt = scal.p + size_t(it + 1) * 8;
-        tdvp_S_dot(t, p.p, shift, Ap.p);
-        launch_dot2(p.p, Ap.p, nullptr, nullptr, P, 1, scratch.p, cur + 4, S());
-        launch_cg_update_dev(x.p, r.p, p.p, Ap.p, cur + 0, cur + 4, P, S());
-        precondition();
-        launch_dot2(r.p, z.p, r.p, r.p, P, 1, scratch.p, nxt, S());
-        launch_cg_dir_dev(p.p, z.p, nxt + 0, cur + 0, P, S());
+        const double* cur = part.p + size_t(it) * stride;
+        double* nxt = part.p + size_t(it + 1) * stride;
+        const double* ypart = nullptr;
+        int yparts = 0;
+        tdvp_S_dot(t, p.p, shift, Ap.p, &ypart, &yparts);
+        if (!ypart) {
+            launch_pcg_pap(p.p, Ap.p, P, apart.p, S());
+            ypart = apart.p;
+            yparts = kCgParts;
+        }
+        launch_pcg_step(x.p, r.p, z.p, p.p, Ap.p, diag, shift, P, cur, ypart, yparts, nxt, S());
+        launch_pcg_dir(p.p, z.p, P, nxt, cur, S());
         it++;
         if (it % check == 0 || it == max_iter) rr = read_rr(it);
     }

@@ -257,6 +257,9 @@ struct Tdvp {
     DevBuf<cplx<float>> V32, SV32, zero32;
     DevBuf<cplx<double>> V64, SV64, zero64;
     DevBuf<uint16_t> frag;
+    DevBuf<cplx<double>> upart;  // per column block partial O.v (fp32 mode)
+    DevBuf<float> spin_part;     // spin GEMM partials (fp32 mode)
+    DevBuf<double> opart, ypart; // dot-product partials of the fused matvec
     DevBuf<int> inv;            // inverse gather table [N][Mpad] (PsiDeep)
     DevBuf<double> dot_scratch;
     // materialised O [count][P] of wave functions without a structured O (PsiCNN)
@@ -274,7 +277,10 @@ void log_psi_of(Psi& psi, const uint64_t* confs, int64_t count, Samples& s, doub
 void ok_weighted_sum(Psi& psi, Samples& s, const cplx<double>* coef, int conj, cplx<double>* out);
 void local_energies(Psi& psi, Operator& op, Samples& s);
 void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S);
-void tdvp_S_dot(Tdvp& t, const cplx<double>* v_dev, double shift, cplx<double>* y_dev);
+// y = (S + shift) v on the device; when the matvec also produced the block partials
+// of v^H y, *ypart / *yparts describe them ([yparts][2]), else they are null / 0
+void tdvp_S_dot(Tdvp& t, const cplx<double>* v_dev, double shift, cplx<double>* y_dev,
+                const double** ypart = nullptr, int* yparts = nullptr);
 // method 0 Cholesky, 1 Jacobi-preconditioned CG from x0 (zero when x0 is null)
 int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x, int* iters, double* res,
                const cd* x0 = nullptr);

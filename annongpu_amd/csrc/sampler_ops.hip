@@ -294,10 +294,11 @@ __global__ void fill_kernel(double* a, double v, int64_t n) {
 }
 
 // coef_s = w_s * e_s (complex), or w_s when e == nullptr
-__global__ void weight_coef_kernel(const double* w, const cplx<double>* e, int64_t n, cplx<double>* coef) {
+__global__ void weight_coef_kernel(const double* w, const cplx<double>* e, cplx<double> shift, int64_t n,
+                                   cplx<double>* coef) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    coef[i] = e ? cplx<double>{w[i] * e[i].re, w[i] * e[i].im} : cplx<double>{w[i], 0.0};
+    coef[i] = e ? cplx<double>{w[i] * (e[i].re - shift.re), w[i] * (e[i].im - shift.im)} : cplx<double>{w[i], 0.0};
 }
 
 inline unsigned blocks_for(int64_t n, int b = 256) { return unsigned((n + b - 1) / b); }
@@ -420,7 +421,14 @@ void launch_fill(double* a, double v, int64_t n, hipStream_t stream) {
 
 void launch_weight_coef(const double* w, const cplx<double>* e, int64_t n, cplx<double>* coef, hipStream_t stream) {
     if (n <= 0) return;
-    weight_coef_kernel<<<blocks_for(n), 256, 0, stream>>>(w, e, n, coef);
+    weight_coef_kernel<<<blocks_for(n), 256, 0, stream>>>(w, e, cplx<double>{0.0, 0.0}, n, coef);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_weight_coef_centered(const double* w, const cplx<double>* e, cplx<double> E, int64_t n,
+                                 cplx<double>* coef, hipStream_t stream) {
+    if (n <= 0) return;
+    weight_coef_kernel<<<blocks_for(n), 256, 0, stream>>>(w, e, E, n, coef);
     ANN_LAUNCH_CHECK();
 }
 
