@@ -103,12 +103,6 @@ void launch_structured_Ov(const DenseView<T>& net, const uint64_t* confs, int wo
                           hipStream_t stream);
 
 // weighted fp64 reductions
-// fp32 mode, bf16 matrix cores (sr_mfma.hip).  u = O.v of the structured O:
-// V [N][Mpad] (W part of v re-indexed onto visible units), vb, va; vfrag holds
-// 3 * npad/16 * 2Mpad/16 * 256 bf16, upart (2Mpad/64) * count complex.
-void launch_ov_mfma(const DenseView<float>& net, const cplx<float>* V, int npad, uint16_t* vfrag,
-                    const uint64_t* confs, int words, int64_t count, const cplx<float>* g, const cplx<double>* vb,
-                    const cplx<double>* va, cplx<double>* upart, cplx<double>* u, hipStream_t stream);
 // the spin GEMM of launch_spin_weighted_sums on bf16 matrix cores (fp32 accumulation
 // per sample chunk, chunks added in fp64)
 void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, int words, const cplx<float>* g,

@@ -57,104 +57,75 @@ __device__ inline double row16_sum(double v) {
 }
 
 // ------------------------------------------------------------ u = O.v (part)
-// One wave: 16 samples x 4 column tiles of the split layout (tile 2p = real parts
-// of units 16p..16p+15, tile 2p+1 = imaginary parts).  upart[blockIdx.y][s].
-__global__ __launch_bounds__(64) void ov_mfma_kernel(DenseView<float> net, const short4_t* vfrag, int KT, int NT,
-                                                     const uint64_t* confs, int words, int64_t count,
-                                                     const cplx<float>* g, const cplx<double>* vb,
-                                                     cplx<double>* upart) {
-    const int lane = threadIdx.x;
-    const int64_t row0 = int64_t(blockIdx.x) * 16;
-    const int nt0 = blockIdx.y * 4;
-    const int64_t my_row = row0 + (lane & 15);
-    uint64_t w[kMaxWords];
+// A workgroup owns one column group (4 tiles of the split layout: tile 2p = real
+// parts of units 16p..16p+15, tile 2p+1 = imaginary parts) and keeps its bf16x3
+// fragments of V in LDS; each wave then walks 16-sample groups, computes their
+// angles of v on matrix cores and contracts them with g.  upart[col group][s].
+__global__ __launch_bounds__(256) void ov_mfma_kernel(DenseView<float> net, const short4_t* vfrag, int KT, int NT,
+                                                      const uint64_t* confs, int words, int64_t count,
+                                                      const cplx<float>* g, const cplx<double>* vb,
+                                                      cplx<double>* upart) {
+    extern __shared__ __align__(16) unsigned char ov_smem[];
+    short4_t* sv = reinterpret_cast<short4_t*>(ov_smem);   // [3][KT][4][64]
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int y = blockIdx.y;
+    const int nt0 = 4 * y;
+    const int nfrag = 3 * KT * 4 * 64;
+    for (int i = threadIdx.x; i < nfrag; i += 256) {
+        const int l = i & 63, t = (i >> 6) & 3, kt = (i >> 8) % KT, sp = (i >> 8) / KT;
+        sv[i] = vfrag[((int64_t(sp) * KT + kt) * NT + nt0 + t) * 64 + l];
+    }
+    __syncthreads();
+    cplx<double> bj[2];
+    int jq[2];
 #pragma unroll
-    for (int k = 0; k < kMaxWords; k++) w[k] = my_row < count && k < words ? confs[my_row * words + k] : 0;
-    float4_t acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) acc[t] = {0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < 2; q++) {
+        jq[q] = (2 * y + q) * 16 + (lane & 15);
+        bj[q] = vb && jq[q] < net.M ? vb[jq[q]] : cplx<double>{0.0, 0.0};
+    }
     const int kq = 4 * (lane >> 4);
-    for (int kt = 0; kt < KT; kt++) {
-        short4_t a;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int v = kt * 16 + kq + e;
-            a[e] = v < net.N ? spin_bf16(w, v, net.N) : short(0);   // V has no bias row
-        }
-#pragma unroll
-        for (int sp = 0; sp < 3; sp++) {
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const short4_t b = vfrag[((int64_t(sp) * KT + kt) * NT + nt0 + t) * 64 + lane];
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-            }
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int64_t s = row0 + 4 * (lane >> 4) + r;
-        double ur = 0.0, ui = 0.0;
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int j = (nt0 / 2 + q) * 16 + (lane & 15);
-            const bool ok = j < net.M && s < count;
-            const cplx<double> bj = vb && ok ? vb[j] : cplx<double>{0.0, 0.0};
-            const cplx<float> gv = ok ? g[s * net.Mpad + j] : cplx<float>{0.f, 0.f};
-            const double tr = double(acc[2 * q][r]) + bj.re, ti = double(acc[2 * q + 1][r]) + bj.im;
-            ur += double(gv.re) * tr - double(gv.im) * ti;
-            ui += double(gv.re) * ti + double(gv.im) * tr;
-        }
-        // row_mirror leaves the full row sum in every lane of the row
-        ur = row16_sum(ur);
-        ui = row16_sum(ui);
-        if ((lane & 15) == 15 && s < count) upart[int64_t(blockIdx.y) * count + s] = {ur, ui};
-    }
-}
-
-// u_s = sum of the column-block partials + sum_i s_i va_i
-__global__ void ov_finish_kernel(const cplx<double>* upart, int blocks, int64_t count, const uint64_t* confs,
-                                 int words, int N, const cplx<double>* va, cplx<double>* u) {
-    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= count) return;
-    double re = 0.0, im = 0.0;
-    for (int b = 0; b < blocks; b++) {
-        re += upart[int64_t(b) * count + s].re;
-        im += upart[int64_t(b) * count + s].im;
-    }
-    if (va) {
+    const int64_t groups = (count + 15) / 16;
+    for (int64_t grp = int64_t(blockIdx.x) * 4 + wave; grp < groups; grp += int64_t(gridDim.x) * 4) {
+        const int64_t row0 = grp * 16;
+        const int64_t my_row = row0 + (lane & 15);
         uint64_t w[kMaxWords];
 #pragma unroll
-        for (int k = 0; k < kMaxWords; k++) w[k] = k < words ? confs[s * words + k] : 0;
-        for (int i = 0; i < N; i++) {
-            const uint64_t word = i < 64 ? w[0] : i < 128 ? w[1] : i < 192 ? w[2] : w[3];
-            const double si = ((word >> (i & 63)) & 1) ? 1.0 : -1.0;
-            re += si * va[i].re;
-            im += si * va[i].im;
-        }
-    }
-    u[s] = {re, im};
-}
-
-// bf16x3 fragments of V [N][Mpad] in the split column layout of ov_mfma_kernel
-__global__ void pack_split_kernel(const cplx<float>* V, int N, int Mpad, int KT, int NT, uint16_t* out) {
-    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int ncol = 2 * Mpad;
-    if (e >= int64_t(KT) * 16 * ncol) return;
-    const int v = int(e / ncol), col = int(e % ncol);
-    const int nt = col / 16, cc = col % 16;
-    const int j = (nt / 2) * 16 + cc;
-    float r = 0.f;
-    if (v < N) {
-        const cplx<float> x = V[int64_t(v) * Mpad + j];
-        r = (nt & 1) ? x.im : x.re;
-    }
-    const int kt = v / 16, kk = v % 16;
-    const int lane = (kk / 4) * 16 + cc, el = kk % 4;
+        for (int k = 0; k < kMaxWords; k++) w[k] = my_row < count && k < words ? confs[my_row * words + k] : 0;
+        float4_t acc[4];
 #pragma unroll
-    for (int sp = 0; sp < 3; sp++) {
-        const short h = bf16_bits(r);
-        r -= bf16_value(h);
-        out[((((int64_t(sp) * KT + kt) * NT + nt) * 64 + lane) * 4) + el] = uint16_t(h);
+        for (int t = 0; t < 4; t++) acc[t] = {0.f, 0.f, 0.f, 0.f};
+        for (int kt = 0; kt < KT; kt++) {
+            short4_t a;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int v = kt * 16 + kq + e;
+                a[e] = v < net.N ? spin_bf16(w, v, net.N) : short(0);   // V has no bias row
+            }
+#pragma unroll
+            for (int sp = 0; sp < 3; sp++)
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, sv[((sp * KT + kt) * 4 + t) * 64 + lane],
+                                                                       acc[t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t s = row0 + kq + r;
+            double ur = 0.0, ui = 0.0;
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const bool ok = jq[q] < net.M && s < count;
+                const cplx<float> gv = ok ? g[s * net.Mpad + jq[q]] : cplx<float>{0.f, 0.f};
+                const double tr = double(acc[2 * q][r]) + bj[q].re, ti = double(acc[2 * q + 1][r]) + bj[q].im;
+                ur += double(gv.re) * tr - double(gv.im) * ti;
+                ui += double(gv.re) * ti + double(gv.im) * tr;
+            }
+            // row_mirror leaves the full 16-lane row sum in every lane of the row
+            ur = row16_sum(ur);
+            ui = row16_sum(ui);
+            if ((lane & 15) == 15 && s < count) upart[int64_t(y) * count + s] = {ur, ui};
+        }
     }
 }
 
@@ -162,128 +133,128 @@ __global__ void pack_split_kernel(const cplx<float>* V, int N, int Mpad, int KT,
 // X[s][c]: c < 2 Mpad from c_s g_sj (mode 0), c_s conj(g_sj) (mode 1) or
 // Re(c_s) |g_sj|^2 in the real column (mode 2); columns 2 Mpad, 2 Mpad + 1 hold c_s.
 constexpr int kSpinRowTiles = 7;     // rows 112 per wave (N + 1 <= 112 in one pass)
-constexpr int kSpinColTiles = 2;     // 32 columns per wave
 
-// raw operands of one 16-sample step of the spin GEMM (loaded one step ahead)
-template <int WORDS>
-struct SpinRaw {
-    uint64_t w[4][WORDS];
-    cplx<float> g[kSpinColTiles][4];
-    cplx<float> cf[4];
-};
-
-constexpr int kSpinWaves = 4;   // waves of a workgroup share a column block and split its sample chunk
+// A workgroup (4 waves) owns 64 real columns and a chunk of samples; per stage
+// of 32 samples its threads load the g rows coalesced (8 threads per sample, 4
+// complex each), form c_s g_sj, split them into bf16x3 B fragments in LDS
+// (double-buffered: the next stage is loaded while the current one multiplies)
+// and each wave runs 7 row tiles x 2 k-steps x 3 splits of MFMA on its column tile.
+constexpr int kSgStep = 32;
 
 template <int WORDS>
 __global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> net, const uint64_t* confs,
                                                              const cplx<float>* g, int mode,
                                                              const cplx<double>* coef, int64_t count, int64_t chunk,
                                                              int vpad, int ncols, float* partial) {
-    __shared__ float red[kSpinWaves - 1][kSpinRowTiles * kSpinColTiles * 4][64];
+    __shared__ short4_t sbf[2][3][2][4][64];
+    __shared__ uint64_t sw[2][kSgStep][WORDS];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int c0 = blockIdx.x * (16 * kSpinColTiles);
+    const int c0 = blockIdx.x * 64;
     const int vt0 = blockIdx.z * kSpinRowTiles;
     const int vtiles = vpad / 16;
     const int64_t s0 = int64_t(blockIdx.y) * chunk;
     const int64_t s1 = s0 + chunk < count ? s0 + chunk : count;
-    const int kq = 4 * (lane >> 4);
     const int two_m = 2 * net.Mpad;
-    float4_t acc[kSpinRowTiles][kSpinColTiles];
+    // loader role of this thread: sample sm of the stage, 4 complex columns from jj
+    const int sm = threadIdx.x >> 3;
+    const int jj = (threadIdx.x & 7) * 4;
+    const int jbase = c0 / 2 + jj;
+    cplx<float> rg[4];
+    cplx<double> rc;
+    uint64_t rw[WORDS];
+    auto load = [&](int64_t st) {
+        const int64_t s = st + sm;
+        const bool ok = s < s1;
+        rc = ok ? coef[s] : cplx<double>{0.0, 0.0};
+        if (ok && jbase < net.Mpad) {
+            const float4* src = reinterpret_cast<const float4*>(g + s * net.Mpad + jbase);
+            const float4 x0 = src[0], x1 = src[1];
+            rg[0] = {x0.x, x0.y};
+            rg[1] = {x0.z, x0.w};
+            rg[2] = {x1.x, x1.y};
+            rg[3] = {x1.z, x1.w};
+        } else {
 #pragma unroll
-    for (int t = 0; t < kSpinRowTiles; t++)
+            for (int i = 0; i < 4; i++) rg[i] = {0.f, 0.f};
+        }
+        if (threadIdx.x < kSgStep) {
+            const int64_t sw_s = st + threadIdx.x;
 #pragma unroll
-        for (int u = 0; u < kSpinColTiles; u++) acc[t][u] = {0.f, 0.f, 0.f, 0.f};
-    auto load = [&](int64_t sb, SpinRaw<WORDS>& r) {
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int64_t s = sb + kq + e;
-            const bool ok = s < s1;
-#pragma unroll
-            for (int k = 0; k < WORDS; k++) r.w[e][k] = ok ? confs[s * WORDS + k] : 0;
-            const cplx<double> c = ok ? coef[s] : cplx<double>{0.0, 0.0};
-            r.cf[e] = {float(c.re), float(c.im)};
-#pragma unroll
-            for (int u = 0; u < kSpinColTiles; u++) {
-                const int c = c0 + 16 * u + (lane & 15);
-                r.g[u][e] = ok && c < two_m ? g[s * net.Mpad + (c >> 1)] : cplx<float>{0.f, 0.f};
-            }
+            for (int k = 0; k < WORDS; k++) rw[k] = sw_s < s1 ? confs[sw_s * WORDS + k] : 0;
         }
     };
-    const int64_t stride = 16 * kSpinWaves;
-    SpinRaw<WORDS> cur;
-    load(s0 + 16 * wave, cur);
-    for (int64_t sb = s0 + 16 * wave; sb < s1; sb += stride) {
-        SpinRaw<WORDS> nxt;
-        load(sb + stride, nxt);
-        short4_t a[kSpinRowTiles];
-        short4_t b[3][kSpinColTiles];
+    auto store = [&](int buf) {
+        const float cr = float(rc.re), ci = float(rc.im);
+        const int ks = sm >> 4, kk = sm & 15;
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const bool ok = sb + kq + e < s1;
-            uint64_t w[kMaxWords];
+        for (int i = 0; i < 8; i++) {
+            const int col = 2 * jj + i;          // real column within the 64-column block
+            const int c = c0 + col;
+            float x;
+            if (c < two_m) {
+                const float gr = rg[i >> 1].re;
+                const float gi = mode == 1 ? -rg[i >> 1].im : rg[i >> 1].im;
+                x = mode == 2 ? ((i & 1) ? 0.f : cr * (gr * gr + gi * gi))
+                              : ((i & 1) ? cr * gi + ci * gr : cr * gr - ci * gi);
+            } else {
+                x = c == two_m ? cr : c == two_m + 1 ? ci : 0.f;
+            }
+            const int ct = col >> 4, cc = col & 15;
+            const int bl = (kk >> 2) * 16 + cc, e = kk & 3;
 #pragma unroll
-            for (int k = 0; k < kMaxWords; k++) w[k] = k < WORDS ? cur.w[e][k < WORDS ? k : 0] : 0;
+            for (int sp = 0; sp < 3; sp++) {
+                const short h = bf16_bits(x);
+                x -= bf16_value(h);
+                sbf[buf][sp][ks][ct][bl][e] = h;
+            }
+        }
+        if (threadIdx.x < kSgStep)
+#pragma unroll
+            for (int k = 0; k < WORDS; k++) sw[buf][threadIdx.x][k] = rw[k];
+    };
+    float4_t acc[kSpinRowTiles];
+#pragma unroll
+    for (int t = 0; t < kSpinRowTiles; t++) acc[t] = {0.f, 0.f, 0.f, 0.f};
+    const int kq = 4 * (lane >> 4);
+    int buf = 0;
+    load(s0);
+    store(0);
+    __syncthreads();
+    for (int64_t st = s0; st < s1; st += kSgStep) {
+        const bool more = st + kSgStep < s1;
+        if (more) load(st + kSgStep);
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            uint64_t w4[4][kMaxWords];
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+#pragma unroll
+                for (int k = 0; k < kMaxWords; k++) w4[e][k] = k < WORDS ? sw[buf][ks * 16 + kq + e][k < WORDS ? k : 0] : 0;
 #pragma unroll
             for (int t = 0; t < kSpinRowTiles; t++) {
+                if (vt0 + t >= vtiles) continue;
                 const int v = (vt0 + t) * 16 + (lane & 15);
-                a[t][e] = ok ? spin_bf16(w, v, net.N) : short(0);
-            }
-            const cplx<float> cf = cur.cf[e];
+                short4_t a;
 #pragma unroll
-            for (int u = 0; u < kSpinColTiles; u++) {
-                const int c = c0 + 16 * u + (lane & 15);
-                const float gr = cur.g[u][e].re;
-                const float gi = mode == 1 ? -cur.g[u][e].im : cur.g[u][e].im;
-                float x;
-                if (c < two_m)
-                    x = mode == 2 ? ((c & 1) ? 0.f : cf.re * (gr * gr + gi * gi))
-                                  : ((c & 1) ? cf.re * gi + cf.im * gr : cf.re * gr - cf.im * gi);
-                else
-                    x = c == two_m ? cf.re : c == two_m + 1 ? cf.im : 0.f;
+                for (int e = 0; e < 4; e++) a[e] = spin_bf16(w4[e], v, net.N);
 #pragma unroll
-                for (int sp = 0; sp < 3; sp++) {
-                    const short h = bf16_bits(x);
-                    x -= bf16_value(h);
-                    b[sp][u][e] = h;
-                }
+                for (int sp = 0; sp < 3; sp++)
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, sbf[buf][sp][ks][wave][lane], acc[t], 0, 0, 0);
             }
         }
-#pragma unroll
-        for (int t = 0; t < kSpinRowTiles; t++) {
-            if (vt0 + t >= vtiles) continue;
-#pragma unroll
-            for (int sp = 0; sp < 3; sp++)
-#pragma unroll
-                for (int u = 0; u < kSpinColTiles; u++)
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b[sp][u], acc[t][u], 0, 0, 0);
-        }
-        cur = nxt;
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
     }
-    // fold the waves' accumulators in a fixed order through LDS
-    if (wave > 0) {
-#pragma unroll
-        for (int t = 0; t < kSpinRowTiles; t++)
-#pragma unroll
-            for (int u = 0; u < kSpinColTiles; u++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) red[wave - 1][(t * kSpinColTiles + u) * 4 + r][lane] = acc[t][u][r];
-    }
-    __syncthreads();
-    if (wave != 0) return;
+    const int c = c0 + 16 * wave + (lane & 15);
 #pragma unroll
     for (int t = 0; t < kSpinRowTiles; t++) {
+        if (vt0 + t >= vtiles || c >= ncols) continue;
 #pragma unroll
-        for (int u = 0; u < kSpinColTiles; u++) {
-            const int c = c0 + 16 * u + (lane & 15);
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                float x = acc[t][u][r];
-#pragma unroll
-                for (int q = 0; q < kSpinWaves - 1; q++) x += red[q][(t * kSpinColTiles + u) * 4 + r][lane];
-                const int v = (vt0 + t) * 16 + 4 * (lane >> 4) + r;
-                if (vt0 + t < vtiles && c < ncols) partial[(int64_t(blockIdx.y) * vpad + v) * ncols + c] = x;
-            }
+        for (int r = 0; r < 4; r++) {
+            const int v = (vt0 + t) * 16 + kq + r;
+            partial[(int64_t(blockIdx.y) * vpad + v) * ncols + c] = acc[t][r];
         }
     }
 }
@@ -414,8 +385,12 @@ __global__ __launch_bounds__(256) void ov_finish_center_kernel(const cplx<double
                                                                const cplx<double>* va, const double* opart,
                                                                const double* w, cplx<double>* u) {
     __shared__ double sh[4];
+    __shared__ cplx<double> sva[4 * 64 * kMaxWords / 4];   // va of up to 256 sites
     const double c_re = parts_total(opart, kDotParts, 2, 0, sh);
     const double c_im = parts_total(opart, kDotParts, 2, 1, sh);
+    if (va)
+        for (int i = threadIdx.x; i < N; i += blockDim.x) sva[i] = va[i];
+    __syncthreads();
     const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (s >= count) return;
     double re = 0.0, im = 0.0;
@@ -430,8 +405,8 @@ __global__ __launch_bounds__(256) void ov_finish_center_kernel(const cplx<double
         for (int i = 0; i < N; i++) {
             const uint64_t word = i < 64 ? cw[0] : i < 128 ? cw[1] : i < 192 ? cw[2] : cw[3];
             const double si = ((word >> (i & 63)) & 1) ? 1.0 : -1.0;
-            re += si * va[i].re;
-            im += si * va[i].im;
+            re += si * sva[i].re;
+            im += si * sva[i].im;
         }
     }
     u[s] = {w[s] * (re - c_re), w[s] * (im - c_im)};
@@ -484,37 +459,20 @@ __global__ __launch_bounds__(256) void spin_y_kernel(const float* partial, int c
 
 }  // namespace
 
-void launch_ov_mfma(const DenseView<float>& net, const cplx<float>* V, int npad, uint16_t* vfrag,
-                    const uint64_t* confs, int words, int64_t count, const cplx<float>* g, const cplx<double>* vb,
-                    const cplx<double>* va, cplx<double>* upart, cplx<double>* u, hipStream_t stream) {
-    if (count <= 0) return;
-    const int KT = npad / 16, NT = 2 * net.Mpad / 16;
-    const int64_t n = int64_t(KT) * 16 * 2 * net.Mpad;
-    pack_split_kernel<<<unsigned((n + 255) / 256), 256, 0, stream>>>(V, net.N, net.Mpad, KT, NT, vfrag);
-    ANN_LAUNCH_CHECK();
-    const dim3 grid{unsigned((count + 15) / 16), unsigned(NT / 4)};
-    ov_mfma_kernel<<<grid, 64, 0, stream>>>(net, reinterpret_cast<const short4_t*>(vfrag), KT, NT, confs, words,
-                                            count, g, vb, upart);
-    ANN_LAUNCH_CHECK();
-    ov_finish_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(upart, NT / 4, count, confs, words, net.N,
-                                                                        va, u);
-    ANN_LAUNCH_CHECK();
-}
-
 int sr_y_parts(int P) { return std::min(256, std::max(1, (P + 255) / 256)); }
 
 SpinPlan spin_plan(int N, int Mpad, int64_t count) {
     SpinPlan p;
     p.vpad = (N + 1 + 15) / 16 * 16;
     p.ncols = 2 * Mpad + 16;
-    p.cblocks = (p.ncols + 16 * kSpinColTiles - 1) / (16 * kSpinColTiles);
+    p.cblocks = (p.ncols + 63) / 64;
     p.zgroups = (p.vpad / 16 + kSpinRowTiles - 1) / kSpinRowTiles;
-    // about 512 workgroups of 4 waves; each wave steps through 16 samples at a time
+    // about 512 workgroups of 4 waves, sample chunks of whole 32-sample stages
     int64_t chunks = std::max<int64_t>(1, 512 / (int64_t(p.cblocks) * p.zgroups));
     chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, (count + 255) / 256));
     int64_t chunk = (count + chunks - 1) / chunks;
-    chunk = (chunk + 63) / 64 * 64;
-    p.chunk = chunk > 0 ? chunk : 64;
+    chunk = (chunk + kSgStep - 1) / kSgStep * kSgStep;
+    p.chunk = chunk > 0 ? chunk : kSgStep;
     p.chunks = int((count + p.chunk - 1) / p.chunk);
     if (p.chunks < 1) p.chunks = 1;
     p.n = int64_t(p.vpad) * p.ncols;
@@ -550,9 +508,17 @@ void launch_sr_matvec_bf16(const DenseView<float>& net, const int* inv, const cp
         net, inv, v, KT, NT, pack_blocks, vab_blocks, ws.frag, ws.va, ws.vb, obar, ws.opart);
     ANN_LAUNCH_CHECK();
     const bool deep = net.layout == LAYOUT_DEEP1;
-    const dim3 grid{unsigned((count + 15) / 16), unsigned(NT / 4)};
-    ov_mfma_kernel<<<grid, 64, 0, stream>>>(net, reinterpret_cast<const short4_t*>(ws.frag), KT, NT, confs, words,
-                                            count, g, deep ? ws.vb : nullptr, ws.upart);
+    // about 1024 workgroups in all, each column group's V fragments in LDS
+    const int64_t groups = (count + 15) / 16;
+    const int per_col = int(std::max<int64_t>(1, std::min<int64_t>((groups + 3) / 4, 1024 / (NT / 4))));
+    const size_t lds = sizeof(short4_t) * 3 * KT * 4 * 64;
+    if (lds > 160 * 1024) throw std::invalid_argument("too many sites for the LDS-resident V fragments");
+    if (lds > 64 * 1024)
+        ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(ov_mfma_kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    const dim3 grid{unsigned(per_col), unsigned(NT / 4)};
+    ov_mfma_kernel<<<grid, 256, lds, stream>>>(net, reinterpret_cast<const short4_t*>(ws.frag), KT, NT, confs, words,
+                                               count, g, deep ? ws.vb : nullptr, ws.upart);
     ANN_LAUNCH_CHECK();
     ov_finish_center_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(
         ws.upart, NT / 4, count, confs, words, net.N, deep ? ws.va : nullptr, ws.opart, w, ws.u);
