@@ -12,6 +12,7 @@
 #include "proposal.hpp"
 #include "wave_net.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 
@@ -278,7 +279,12 @@ constexpr int kTK = 16;
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void sgemm_kernel(int M, int Nn, int K, const float* __restrict__ A, int lda,
                                                     const float* __restrict__ B, int ldb,
-                                                    const float* __restrict__ bias, float* __restrict__ C, int ldc) {
+                                                    const float* __restrict__ bias, float* __restrict__ C, int ldc,
+                                                    int ksplit) {
+    // split-K: block z covers k in [z ksplit, (z + 1) ksplit) and writes its own C slice
+    const int kbeg = blockIdx.z * ksplit;
+    const int kend = kbeg + ksplit < K ? kbeg + ksplit : K;
+    C += int64_t(blockIdx.z) * M * ldc;
     __shared__ float As[kTK][64 + 4];
     __shared__ float Bs[kTK][64 + 4];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int Nn, int K, const 
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0.f;
-    for (int k0 = 0; k0 < K; k0 += kTK) {
+    for (int k0 = kbeg; k0 < kend; k0 += kTK) {
         for (int e = threadIdx.x; e < 64 * kTK; e += 256) {
             int m, k;
             if constexpr (TA) {
@@ -300,7 +306,7 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int Nn, int K, const 
             }
             const int gm = m0 + m, gk = k0 + k;
             float v = 0.f;
-            if (gm < M && gk < K) v = TA ? A[int64_t(gk) * lda + gm] : A[int64_t(gm) * lda + gk];
+            if (gm < M && gk < kend) v = TA ? A[int64_t(gk) * lda + gm] : A[int64_t(gm) * lda + gk];
             As[k][m] = v;
             int n;
             if constexpr (TB) {
@@ -312,7 +318,7 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int Nn, int K, const 
             }
             const int gn = n0 + n, gk2 = k0 + k;
             float u = 0.f;
-            if (gn < Nn && gk2 < K) u = TB ? B[int64_t(gn) * ldb + gk2] : B[int64_t(gk2) * ldb + gn];
+            if (gn < Nn && gk2 < kend) u = TB ? B[int64_t(gn) * ldb + gk2] : B[int64_t(gk2) * ldb + gn];
             Bs[k][n] = u;
         }
         __syncthreads();
@@ -348,10 +354,20 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int Nn, int K, const 
 
 template <bool TA, bool TB, int EPI>
 void sgemm(int M, int Nn, int K, const float* A, int lda, const float* B, int ldb, const float* bias, float* C,
-           int ldc, hipStream_t stream) {
-    const dim3 grid(unsigned((Nn + 63) / 64), unsigned((M + 63) / 64));
-    sgemm_kernel<TA, TB, EPI><<<grid, 256, 0, stream>>>(M, Nn, K, A, lda, B, ldb, bias, C, ldc);
+           int ldc, hipStream_t stream, int splits = 1) {
+    const int ksplit = (K + splits - 1) / splits;
+    const dim3 grid(unsigned((Nn + 63) / 64), unsigned((M + 63) / 64), unsigned(splits));
+    sgemm_kernel<TA, TB, EPI><<<grid, 256, 0, stream>>>(M, Nn, K, A, lda, B, ldb, bias, C, ldc, ksplit);
     ANN_LAUNCH_CHECK();
+}
+
+// out[i] = sum_z part[z n + i] (fixed order)
+__global__ void reduce_splits_kernel(const float* part, int splits, int64_t n, float* out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int z = 0; z < splits; z++) s += part[z * n + i];
+    out[i] = s;
 }
 
 // one wavefront per row: d loss / d logits in place, per-row loss in fp64
@@ -380,18 +396,20 @@ __global__ void dtanh_kernel(float* dH, const float* Hd, int64_t n) {
     if (e < n) dH[e] *= 1.f - Hd[e] * Hd[e];
 }
 
-// out[n] = sum_r D[r][n]: 256 threads = 64 columns x 4 row phases, LDS fold
-__global__ __launch_bounds__(256) void colsum_kernel(const float* D, int rows, int cols, float* out) {
+// out[z][n] = sum of D[r][n] over the rows of split z: 64 columns x 4 row phases, LDS fold
+__global__ __launch_bounds__(256) void colsum_kernel(const float* D, int rows, int cols, int rsplit, float* out) {
     __shared__ float part[4][64];
     const int n = blockIdx.x * 64 + (threadIdx.x & 63);
     const int ph = threadIdx.x >> 6;
+    const int r0 = blockIdx.y * rsplit, r1 = r0 + rsplit < rows ? r0 + rsplit : rows;
     float s = 0.f;
     if (n < cols)
-        for (int r = ph; r < rows; r += 4) s += D[int64_t(r) * cols + n];
+        for (int r = r0 + ph; r < r1; r += 4) s += D[int64_t(r) * cols + n];
     part[ph][threadIdx.x & 63] = s;
     __syncthreads();
-    if (ph == 0 && n < cols) out[n] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
-                                      part[3][threadIdx.x];
+    if (ph == 0 && n < cols)
+        out[int64_t(blockIdx.y) * cols + n] =
+            part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
 }
 
 __global__ void adam_kernel(float* p, float* m, float* v, const float* g, int64_t n, float lr, float b1, float b2,
@@ -473,15 +491,27 @@ void launch_loss_grad(const TrainBatch& b, const float* params, int N, int H, do
     sgemm<false, true, 1>(Bt, N, H, b.Hd, H, W2, H, b2, b.A2, N, stream);
     loss_kernel<<<blocks_of(Bt, 4), 256, 0, stream>>>(b, N, count_norm);
     ANN_LAUNCH_CHECK();
-    // backward
-    sgemm<true, false, 0>(N, H, Bt, b.A2, N, b.Hd, H, nullptr, gW2, H, stream);
-    colsum_kernel<<<blocks_of(N, 64), 256, 0, stream>>>(b.A2, Bt, N, gb2);
+    // backward; the reductions over the batch run split over kGradSplits row ranges
+    // into b.gpart and are then added in a fixed order
+    const int splits = std::max(1, std::min(kGradSplits, Bt / 64));
+    const int rsplit = (Bt + splits - 1) / splits;
+    const int64_t nW = int64_t(N) * H;
+    float* pW = b.gpart;
+    float* pb = b.gpart + splits * nW;
+    sgemm<true, false, 0>(N, H, Bt, b.A2, N, b.Hd, H, nullptr, pW, H, stream, splits);
+    reduce_splits_kernel<<<blocks_of(nW), 256, 0, stream>>>(pW, splits, nW, gW2);
+    ANN_LAUNCH_CHECK();
+    colsum_kernel<<<dim3(unsigned(blocks_of(N, 64)), unsigned(splits)), 256, 0, stream>>>(b.A2, Bt, N, rsplit, pb);
+    reduce_splits_kernel<<<blocks_of(N), 256, 0, stream>>>(pb, splits, N, gb2);
     ANN_LAUNCH_CHECK();
     sgemm<false, false, 0>(Bt, H, N, b.A2, N, W2, H, nullptr, b.dH, H, stream);
     dtanh_kernel<<<blocks_of(int64_t(Bt) * H), 256, 0, stream>>>(b.dH, b.Hd, int64_t(Bt) * H);
     ANN_LAUNCH_CHECK();
-    sgemm<true, false, 0>(H, N, Bt, b.dH, H, b.X, N, nullptr, gW1, N, stream);
-    colsum_kernel<<<blocks_of(H, 64), 256, 0, stream>>>(b.dH, Bt, H, gb1);
+    sgemm<true, false, 0>(H, N, Bt, b.dH, H, b.X, N, nullptr, pW, N, stream, splits);
+    reduce_splits_kernel<<<blocks_of(nW), 256, 0, stream>>>(pW, splits, nW, gW1);
+    ANN_LAUNCH_CHECK();
+    colsum_kernel<<<dim3(unsigned(blocks_of(H, 64)), unsigned(splits)), 256, 0, stream>>>(b.dH, Bt, H, rsplit, pb);
+    reduce_splits_kernel<<<blocks_of(H), 256, 0, stream>>>(pb, splits, H, gb1);
     ANN_LAUNCH_CHECK();
 }
 

@@ -50,7 +50,9 @@ struct TrainBatch {
     float* A2;                // [Bt][N] logits, then d loss / d logits
     float* dH;                // [Bt][H] d loss / d hidden pre-activations
     double* loss_part;        // [Bt] per-row losses
+    float* gpart;             // [kGradSplits][N H + max(N, H)] split reductions of the gradient
 };
+constexpr int kGradSplits = 32;
 
 // rows of a random batch: sample = range32(x, count), block = range32(y, num_blocks)
 // from Philox((row, step, 0, 4), seed)

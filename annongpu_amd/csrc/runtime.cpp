@@ -1130,7 +1130,8 @@ TrainBatch Proposal::batch(int Bt) {
     A2.ensure(size_t(Bt) * N);
     dH.ensure(size_t(Bt) * H);
     loss_part.ensure(Bt);
-    return {Bt, bsample.p, bsites.p, X.p, Y.p, Mask.p, Hd.p, A2.p, dH.p, loss_part.p};
+    gpart.ensure(size_t(kGradSplits) * (size_t(N) * H + std::max(N, H)));
+    return {Bt, bsample.p, bsites.p, X.p, Y.p, Mask.p, Hd.p, A2.p, dH.p, loss_part.p, gpart.p};
 }
 
 BlockNet Proposal::view() {

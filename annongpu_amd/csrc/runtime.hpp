@@ -208,6 +208,7 @@ struct Proposal {
     DevBuf<int> bsites;
     DevBuf<float> X, Y, Mask, Hd, A2, dH;
     DevBuf<double> loss_part;
+    DevBuf<float> gpart;
     TrainBatch batch(int Bt);
     BlockNet view();
     double last_loss = 0.0;

@@ -72,9 +72,19 @@ __global__ __launch_bounds__(256) void ov_mfma_kernel(DenseView<float> net, cons
     const int y = blockIdx.y;
     const int nt0 = 4 * y;
     const int nfrag = 3 * KT * 4 * 64;
-    for (int i = threadIdx.x; i < nfrag; i += 256) {
-        const int l = i & 63, t = (i >> 6) & 3, kt = (i >> 8) % KT, sp = (i >> 8) / KT;
-        sv[i] = vfrag[((int64_t(sp) * KT + kt) * NT + nt0 + t) * 64 + l];
+    // the whole slice is loaded before any of it is stored (the loads overlap)
+    constexpr int kFill = 8;
+    for (int i0 = threadIdx.x; i0 < nfrag; i0 += 256 * kFill) {
+        short4_t tmp[kFill];
+#pragma unroll
+        for (int f = 0; f < kFill; f++) {
+            const int i = i0 + 256 * f;
+            const int l = i & 63, t = (i >> 6) & 3, kt = (i >> 8) % KT, sp = (i >> 8) / KT;
+            tmp[f] = i < nfrag ? vfrag[((int64_t(sp) * KT + kt) * NT + nt0 + t) * 64 + l] : short4_t{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int f = 0; f < kFill; f++)
+            if (i0 + 256 * f < nfrag) sv[i0 + 256 * f] = tmp[f];
     }
     __syncthreads();
     cplx<double> bj[2];
@@ -134,19 +144,21 @@ __global__ __launch_bounds__(256) void ov_mfma_kernel(DenseView<float> net, cons
 // Re(c_s) |g_sj|^2 in the real column (mode 2); columns 2 Mpad, 2 Mpad + 1 hold c_s.
 constexpr int kSpinRowTiles = 7;     // rows 112 per wave (N + 1 <= 112 in one pass)
 
-// A workgroup (4 waves) owns 64 real columns and a chunk of samples; per stage
-// of 32 samples its threads load the g rows coalesced (8 threads per sample, 4
-// complex each), form c_s g_sj, split them into bf16x3 B fragments in LDS
-// (double-buffered: the next stage is loaded while the current one multiplies)
-// and each wave runs 7 row tiles x 2 k-steps x 3 splits of MFMA on its column tile.
-constexpr int kSgStep = 32;
+// A workgroup (4 waves) owns 64 real columns and a chunk of samples.  Per stage
+// of 64 samples each thread loads 4 consecutive samples x 2 complex columns (one
+// 16-byte load per sample, 16 threads cover a 256-byte row segment), forms
+// c_s g_sj, splits it into bf16x3 and writes whole B-fragment quads (4 samples)
+// to LDS; the stages are double-buffered (the next stage's loads are in flight
+// while the current one multiplies) and each wave runs 7 row tiles x 4 k-steps
+// x 3 splits of MFMA on its own 16-column tile.
+constexpr int kSgStep = 64;
 
 template <int WORDS>
 __global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> net, const uint64_t* confs,
                                                              const cplx<float>* g, int mode,
                                                              const cplx<double>* coef, int64_t count, int64_t chunk,
                                                              int vpad, int ncols, float* partial) {
-    __shared__ short4_t sbf[2][3][2][4][64];
+    __shared__ short4_t sbf[2][3][4][4][64];        // [buf][split][k-step][col tile][lane]
     __shared__ uint64_t sw[2][kSgStep][WORDS];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -156,58 +168,58 @@ __global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> ne
     const int64_t s0 = int64_t(blockIdx.y) * chunk;
     const int64_t s1 = s0 + chunk < count ? s0 + chunk : count;
     const int two_m = 2 * net.Mpad;
-    // loader role of this thread: sample sm of the stage, 4 complex columns from jj
-    const int sm = threadIdx.x >> 3;
-    const int jj = (threadIdx.x & 7) * 4;
-    const int jbase = c0 / 2 + jj;
-    cplx<float> rg[4];
-    cplx<double> rc;
+    // loader role: samples 4 sg .. 4 sg + 3 of the stage, complex columns jb, jb + 1
+    const int sg = threadIdx.x >> 4;
+    const int jp = (threadIdx.x & 15) * 2;
+    const int jb = c0 / 2 + jp;
+    float4 rg[4];
+    cplx<float> rc[4];
     uint64_t rw[WORDS];
     auto load = [&](int64_t st) {
-        const int64_t s = st + sm;
-        const bool ok = s < s1;
-        rc = ok ? coef[s] : cplx<double>{0.0, 0.0};
-        if (ok && jbase < net.Mpad) {
-            const float4* src = reinterpret_cast<const float4*>(g + s * net.Mpad + jbase);
-            const float4 x0 = src[0], x1 = src[1];
-            rg[0] = {x0.x, x0.y};
-            rg[1] = {x0.z, x0.w};
-            rg[2] = {x1.x, x1.y};
-            rg[3] = {x1.z, x1.w};
-        } else {
 #pragma unroll
-            for (int i = 0; i < 4; i++) rg[i] = {0.f, 0.f};
+        for (int e = 0; e < 4; e++) {
+            const int64_t s = st + 4 * sg + e;
+            const bool ok = s < s1;
+            rg[e] = ok && jb < net.Mpad ? *reinterpret_cast<const float4*>(g + s * net.Mpad + jb)
+                                        : float4{0.f, 0.f, 0.f, 0.f};
+            const cplx<double> c = ok ? coef[s] : cplx<double>{0.0, 0.0};
+            rc[e] = {float(c.re), float(c.im)};
         }
         if (threadIdx.x < kSgStep) {
-            const int64_t sw_s = st + threadIdx.x;
+            const int64_t ws_ = st + threadIdx.x;
 #pragma unroll
-            for (int k = 0; k < WORDS; k++) rw[k] = sw_s < s1 ? confs[sw_s * WORDS + k] : 0;
+            for (int k = 0; k < WORDS; k++) rw[k] = ws_ < s1 ? confs[ws_ * WORDS + k] : 0;
         }
     };
     auto store = [&](int buf) {
-        const float cr = float(rc.re), ci = float(rc.im);
-        const int ks = sm >> 4, kk = sm & 15;
+        const int ks = sg >> 2, kl = (sg & 3) * 16;   // k-step of the quad, lane group of its 4 samples
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int col = 2 * jj + i;          // real column within the 64-column block
+        for (int i = 0; i < 4; i++) {                 // real columns 2 jp + i of the block
+            const int col = 2 * jp + i;
             const int c = c0 + col;
-            float x;
-            if (c < two_m) {
-                const float gr = rg[i >> 1].re;
-                const float gi = mode == 1 ? -rg[i >> 1].im : rg[i >> 1].im;
-                x = mode == 2 ? ((i & 1) ? 0.f : cr * (gr * gr + gi * gi))
-                              : ((i & 1) ? cr * gi + ci * gr : cr * gr - ci * gi);
-            } else {
-                x = c == two_m ? cr : c == two_m + 1 ? ci : 0.f;
+            short4_t h[3];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const float gr = (i >> 1) ? rg[e].z : rg[e].x;
+                const float gi0 = (i >> 1) ? rg[e].w : rg[e].y;
+                const float gi = mode == 1 ? -gi0 : gi0;
+                const float cr = rc[e].re, ci = rc[e].im;
+                float x;
+                if (c < two_m)
+                    x = mode == 2 ? ((i & 1) ? 0.f : cr * (gr * gr + gi * gi))
+                                  : ((i & 1) ? cr * gi + ci * gr : cr * gr - ci * gi);
+                else
+                    x = c == two_m ? cr : c == two_m + 1 ? ci : 0.f;
+#pragma unroll
+                for (int sp = 0; sp < 3; sp++) {
+                    const short b16 = bf16_bits(x);
+                    x -= bf16_value(b16);
+                    h[sp][e] = b16;
+                }
             }
             const int ct = col >> 4, cc = col & 15;
-            const int bl = (kk >> 2) * 16 + cc, e = kk & 3;
 #pragma unroll
-            for (int sp = 0; sp < 3; sp++) {
-                const short h = bf16_bits(x);
-                x -= bf16_value(h);
-                sbf[buf][sp][ks][ct][bl][e] = h;
-            }
+            for (int sp = 0; sp < 3; sp++) sbf[buf][sp][ks][ct][kl + cc] = h[sp];
         }
         if (threadIdx.x < kSgStep)
 #pragma unroll
@@ -224,8 +236,8 @@ __global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> ne
     for (int64_t st = s0; st < s1; st += kSgStep) {
         const bool more = st + kSgStep < s1;
         if (more) load(st + kSgStep);
-#pragma unroll
-        for (int ks = 0; ks < 2; ks++) {
+#pragma unroll 1
+        for (int ks = 0; ks < 4; ks++) {
             uint64_t w4[4][kMaxWords];
 #pragma unroll
             for (int e = 0; e < 4; e++)
@@ -394,6 +406,7 @@ __global__ __launch_bounds__(256) void ov_finish_center_kernel(const cplx<double
     const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (s >= count) return;
     double re = 0.0, im = 0.0;
+#pragma unroll 8
     for (int b = 0; b < blocks; b++) {
         re += upart[int64_t(b) * count + s].re;
         im += upart[int64_t(b) * count + s].im;
@@ -438,6 +451,7 @@ __global__ __launch_bounds__(256) void spin_y_kernel(const float* partial, int c
         }
         const int64_t off = int64_t(row) * ncols + col;
         double yr = 0.0, yi = 0.0;
+#pragma unroll 8
         for (int c = 0; c < chunks; c++) {
             yr += double(partial[c * n + off]);
             yi += double(partial[c * n + off + 1]);
