@@ -115,8 +115,13 @@ struct SpinPlan {
     int64_t chunk = 0, n = 0;
 };
 SpinPlan spin_plan(int N, int Mpad, int64_t count);
-void launch_spin_gemm_bf16(const DenseView<float>& net, const uint64_t* confs, int words, const cplx<float>* g,
-                           int mode, const cplx<double>* coef, int64_t count, const SpinPlan& p, float* partial,
+// bf16 spin fragments of the configurations (both GEMM orientations, see sr_mfma.hip);
+// ov: ceil(count/16) * npad/16 * 256 and sg: ceil(count/16) * vpad/16 * 256 uint16 (either may be null
+// when its dimension is 0)
+void launch_spin_frags(const uint64_t* confs, int words, int64_t count, int N, int npad, int vpad, uint16_t* ov,
+                       uint16_t* sg, hipStream_t stream);
+void launch_spin_gemm_bf16(const DenseView<float>& net, const uint16_t* sgfrag, const cplx<float>* g, int mode,
+                           const cplx<double>* coef, int64_t count, const SpinPlan& p, float* partial,
                            hipStream_t stream);
 // workspace of the fused S.v (fp32 dense nets); sizes: frag 3 * npad/16 * 2Mpad/16 * 256,
 // va N, vb Mpad, upart (2Mpad/64) * count, u count, spin_part chunks * n, opart 128,
@@ -128,6 +133,7 @@ struct SrWorkspace {
     cplx<double>* upart;
     cplx<double>* u;
     float* spin_part;
+    double* full;             // vpad * ncols
     double* opart;
     double* ypart;
 };
@@ -136,8 +142,8 @@ struct SrWorkspace {
 int sr_y_parts(int P);
 void launch_sr_matvec_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const cplx<double>* obar,
                            const double* w, const uint64_t* confs, int words, int64_t count, const cplx<float>* g,
-                           int npad, double shift, const SpinPlan& plan, SrWorkspace ws, cplx<double>* y,
-                           hipStream_t stream);
+                           const uint16_t* ovfrag, const uint16_t* sgfrag, int npad, double shift,
+                           const SpinPlan& plan, SrWorkspace ws, cplx<double>* y, hipStream_t stream);
 
 void launch_weighted_sums(const cplx<double>* e_loc, const double* w, int64_t count,
                           double* out4 /* E.re, E.im, E2, sum w */, hipStream_t stream);

@@ -725,6 +725,7 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     t.w.ensure(n);
     ANN_HIP_CHECK(hipMemcpyAsync(t.w.p, s.w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, S()));
     t.has_samples = true;
+    t.frags_valid = false;
     t.has_S = false;
     if (with_S) {
         DevBuf<cplx<T>> O;
@@ -887,10 +888,19 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y,
         t.spin_part.ensure(size_t(plan.chunks) * plan.n);
         t.opart.ensure(256);
         t.ypart.ensure(512);
-        const SrWorkspace ws{t.frag.p, t.va.p, t.vb.p, t.upart.p, t.u.p, t.spin_part.p, t.opart.p, t.ypart.p};
+        if (!t.frags_valid) {
+            // spin fragments of the samples, reused by every matvec until the next eval
+            t.ovfrag.ensure(size_t((t.count + 15) / 16) * (t.npad / 16) * 256);
+            t.sgfrag.ensure(size_t((t.count + 15) / 16) * (plan.vpad / 16) * 256);
+            launch_spin_frags(t.confs.p, t.words, t.count, t.N, t.npad, plan.vpad, t.ovfrag.p, t.sgfrag.p, S());
+            t.frags_valid = true;
+        }
+        t.spin_full.ensure(size_t(plan.n));
+        const SrWorkspace ws{t.frag.p,      t.va.p,        t.vb.p,    t.upart.p, t.u.p,
+                             t.spin_part.p, t.spin_full.p, t.opart.p, t.ypart.p};
         ScopedTimer timer("sr_matvec");
-        launch_sr_matvec_bf16(net, t.inv.p, v, t.Obar.p, t.w.p, t.confs.p, t.words, t.count, g.p, t.npad, shift,
-                              plan, ws, y, S());
+        launch_sr_matvec_bf16(net, t.inv.p, v, t.Obar.p, t.w.p, t.confs.p, t.words, t.count, g.p, t.ovfrag.p,
+                              t.sgfrag.p, t.npad, shift, plan, ws, y, S());
         if (ypart) *ypart = t.ypart.p;
         if (yparts) *yparts = sr_y_parts(P);
         return;

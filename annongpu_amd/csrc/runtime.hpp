@@ -260,7 +260,10 @@ struct Tdvp {
     DevBuf<uint16_t> frag;
     DevBuf<cplx<double>> upart;  // per column block partial O.v (fp32 mode)
     DevBuf<float> spin_part;     // spin GEMM partials (fp32 mode)
+    DevBuf<double> spin_full;    // their sum
     DevBuf<double> opart, ypart; // dot-product partials of the fused matvec
+    DevBuf<uint16_t> ovfrag, sgfrag;   // spin fragments of the samples (fp32 mode)
+    bool frags_valid = false;
     DevBuf<int> inv;            // inverse gather table [N][Mpad] (PsiDeep)
     DevBuf<double> dot_scratch;
     // materialised O [count][P] of wave functions without a structured O (PsiCNN)

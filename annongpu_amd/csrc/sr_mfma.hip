@@ -38,6 +38,18 @@ __device__ inline short bf16_bits(float x) {
 }
 __device__ inline float bf16_value(short h) { return __uint_as_float(uint32_t(uint16_t(h)) << 16); }
 
+// x = hi + mid + lo exactly (truncating split: each step keeps the top 8
+// significant bits, the remainder is exact in fp32 and has 8 bits fewer)
+__device__ inline void split3(float x, short& hi, short& mid, short& lo) {
+    const uint32_t b0 = __float_as_uint(x) & 0xFFFF0000u;
+    const float r1 = x - __uint_as_float(b0);
+    const uint32_t b1 = __float_as_uint(r1) & 0xFFFF0000u;
+    const float r2 = r1 - __uint_as_float(b1);
+    hi = short(b0 >> 16);
+    mid = short(b1 >> 16);
+    lo = short(__float_as_uint(r2) >> 16);
+}
+
 // +-1 / 0 of spin v of a configuration, as bf16 bits (v == N is the constant-1 row)
 __device__ inline short spin_bf16(const uint64_t* w, int v, int N) {
     if (v < N) {
@@ -56,13 +68,15 @@ __device__ inline double row16_sum(double v) {
     return v;
 }
 
+constexpr int kMaxKT = 16;   // site tiles of 16: configurations of up to 256 sites
+
 // ------------------------------------------------------------ u = O.v (part)
 // A workgroup owns one column group (4 tiles of the split layout: tile 2p = real
 // parts of units 16p..16p+15, tile 2p+1 = imaginary parts) and keeps its bf16x3
 // fragments of V in LDS; each wave then walks 16-sample groups, computes their
 // angles of v on matrix cores and contracts them with g.  upart[col group][s].
 __global__ __launch_bounds__(256) void ov_mfma_kernel(DenseView<float> net, const short4_t* vfrag, int KT, int NT,
-                                                      const uint64_t* confs, int words, int64_t count,
+                                                      const short4_t* afrag, int64_t count,
                                                       const cplx<float>* g, const cplx<double>* vb,
                                                       cplx<double>* upart) {
     extern __shared__ __align__(16) unsigned char ov_smem[];
@@ -87,55 +101,103 @@ __global__ __launch_bounds__(256) void ov_mfma_kernel(DenseView<float> net, cons
             if (i0 + 256 * f < nfrag) sv[i0 + 256 * f] = tmp[f];
     }
     __syncthreads();
-    cplx<double> bj[2];
+    cplx<float> bj[2];
     int jq[2];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
         jq[q] = (2 * y + q) * 16 + (lane & 15);
-        bj[q] = vb && jq[q] < net.M ? vb[jq[q]] : cplx<double>{0.0, 0.0};
+        const cplx<double> b = vb && jq[q] < net.M ? vb[jq[q]] : cplx<double>{0.0, 0.0};
+        bj[q] = {float(b.re), float(b.im)};
     }
     const int kq = 4 * (lane >> 4);
     const int64_t groups = (count + 15) / 16;
     for (int64_t grp = int64_t(blockIdx.x) * 4 + wave; grp < groups; grp += int64_t(gridDim.x) * 4) {
         const int64_t row0 = grp * 16;
-        const int64_t my_row = row0 + (lane & 15);
-        uint64_t w[kMaxWords];
+        // g of this lane's output elements, loaded ahead of the matrix-core work
+        cplx<float> gv[4][2];
 #pragma unroll
-        for (int k = 0; k < kMaxWords; k++) w[k] = my_row < count && k < words ? confs[my_row * words + k] : 0;
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int64_t s = row0 + kq + r;
+                gv[r][q] = jq[q] < net.M && s < count ? g[s * net.Mpad + jq[q]] : cplx<float>{0.f, 0.f};
+            }
         float4_t acc[4];
 #pragma unroll
         for (int t = 0; t < 4; t++) acc[t] = {0.f, 0.f, 0.f, 0.f};
-        for (int kt = 0; kt < KT; kt++) {
-            short4_t a;
+        // all spin fragments of the group are loaded before the first MFMA (KT <= 16)
+        const short4_t* ag = afrag + grp * KT * 64 + lane;
+        short4_t a[kMaxKT];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int v = kt * 16 + kq + e;
-                a[e] = v < net.N ? spin_bf16(w, v, net.N) : short(0);   // V has no bias row
-            }
+        for (int kt = 0; kt < kMaxKT; kt++) a[kt] = kt < KT ? ag[kt * 64] : short4_t{0, 0, 0, 0};
+#pragma unroll
+        for (int kt = 0; kt < kMaxKT; kt++) {
+            if (kt >= KT) break;
 #pragma unroll
             for (int sp = 0; sp < 3; sp++)
 #pragma unroll
                 for (int t = 0; t < 4; t++)
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, sv[((sp * KT + kt) * 4 + t) * 64 + lane],
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kt], sv[((sp * KT + kt) * 4 + t) * 64 + lane],
                                                                        acc[t], 0, 0, 0);
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int64_t s = row0 + kq + r;
-            double ur = 0.0, ui = 0.0;
+            float ur = 0.f, ui = 0.f;
 #pragma unroll
             for (int q = 0; q < 2; q++) {
-                const bool ok = jq[q] < net.M && s < count;
-                const cplx<float> gv = ok ? g[s * net.Mpad + jq[q]] : cplx<float>{0.f, 0.f};
-                const double tr = double(acc[2 * q][r]) + bj[q].re, ti = double(acc[2 * q + 1][r]) + bj[q].im;
-                ur += double(gv.re) * tr - double(gv.im) * ti;
-                ui += double(gv.re) * ti + double(gv.im) * tr;
+                const float tr = acc[2 * q][r] + bj[q].re, ti = acc[2 * q + 1][r] + bj[q].im;
+                ur += gv[r][q].re * tr - gv[r][q].im * ti;
+                ui += gv[r][q].re * ti + gv[r][q].im * tr;
             }
             // row_mirror leaves the full 16-lane row sum in every lane of the row
-            ur = row16_sum(ur);
-            ui = row16_sum(ui);
-            if ((lane & 15) == 15 && s < count) upart[int64_t(y) * count + s] = {ur, ui};
+            const double sr = row16_sum(double(ur)), si = row16_sum(double(ui));
+            if ((lane & 15) == 15 && s < count) upart[int64_t(y) * count + s] = {sr, si};
         }
+    }
+}
+
+// spin fragments of the sample configurations, computed once per TDVP
+// evaluation and reused by every matvec:
+//   ov : [group][kt][lane]   rows = samples 16 group + l%16, k = sites 16 kt + 4 (l/16) + e
+//   sg : [group][vt][lane]   rows = sites 16 vt + l%16 (site N = the constant-1 row),
+//                            k = samples 16 group + 4 (l/16) + e
+__global__ void spin_frag_kernel(const uint64_t* confs, int words, int64_t count, int N, int KT, int VT,
+                                 short4_t* ov, short4_t* sg) {
+    const int64_t e_all = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t groups = (count + 15) / 16;
+    const int per = (KT + VT) * 64;
+    if (e_all >= groups * per) return;
+    const int64_t grp = e_all / per;
+    const int rem = int(e_all % per);
+    const int l = rem & 63;
+    const int kq = 4 * (l >> 4);
+    if (rem < KT * 64) {
+        const int kt = rem >> 6;
+        const int64_t s = grp * 16 + (l & 15);
+        uint64_t w[kMaxWords];
+#pragma unroll
+        for (int k = 0; k < kMaxWords; k++) w[k] = s < count && k < words ? confs[s * words + k] : 0;
+        short4_t a;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int v = kt * 16 + kq + e;
+            a[e] = s < count && v < N ? spin_bf16(w, v, N) : short(0);
+        }
+        ov[(grp * KT + kt) * 64 + l] = a;
+    } else {
+        const int vt = (rem >> 6) - KT;
+        const int v = vt * 16 + (l & 15);
+        short4_t a;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int64_t s = grp * 16 + kq + e;
+            uint64_t w[kMaxWords];
+#pragma unroll
+            for (int k = 0; k < kMaxWords; k++) w[k] = s < count && k < words ? confs[s * words + k] : 0;
+            a[e] = s < count ? spin_bf16(w, v, N) : short(0);
+        }
+        sg[(grp * VT + vt) * 64 + l] = a;
     }
 }
 
@@ -153,13 +215,12 @@ constexpr int kSpinRowTiles = 7;     // rows 112 per wave (N + 1 <= 112 in one p
 // x 3 splits of MFMA on its own 16-column tile.
 constexpr int kSgStep = 64;
 
-template <int WORDS>
-__global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> net, const uint64_t* confs,
+__global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> net, const short4_t* afrag,
                                                              const cplx<float>* g, int mode,
                                                              const cplx<double>* coef, int64_t count, int64_t chunk,
                                                              int vpad, int ncols, float* partial) {
     __shared__ short4_t sbf[2][3][4][4][64];        // [buf][split][k-step][col tile][lane]
-    __shared__ uint64_t sw[2][kSgStep][WORDS];
+    __shared__ short4_t saf[2][4][kSpinRowTiles][64];  // [buf][k-step][row tile][lane] spin fragments
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int c0 = blockIdx.x * 64;
@@ -174,7 +235,9 @@ __global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> ne
     const int jb = c0 / 2 + jp;
     float4 rg[4];
     cplx<float> rc[4];
-    uint64_t rw[WORDS];
+    constexpr int kAfragPerThread = (4 * kSpinRowTiles * 64 + 255) / 256;
+    short4_t ra[kAfragPerThread];
+    const int vtl = vtiles - vt0 < kSpinRowTiles ? vtiles - vt0 : kSpinRowTiles;   // row tiles of this z group
     auto load = [&](int64_t st) {
 #pragma unroll
         for (int e = 0; e < 4; e++) {
@@ -185,10 +248,14 @@ __global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> ne
             const cplx<double> c = ok ? coef[s] : cplx<double>{0.0, 0.0};
             rc[e] = {float(c.re), float(c.im)};
         }
-        if (threadIdx.x < kSgStep) {
-            const int64_t ws_ = st + threadIdx.x;
+        // spin fragments of the stage: 4 k-steps x kSpinRowTiles tiles x 64 lanes, 4 per thread
 #pragma unroll
-            for (int k = 0; k < WORDS; k++) rw[k] = ws_ < s1 ? confs[ws_ * WORDS + k] : 0;
+        for (int q = 0; q < kAfragPerThread; q++) {
+            const int i = threadIdx.x + 256 * q;           // [ks][t][lane]
+            const int ks = i / (kSpinRowTiles * 64), t = (i / 64) % kSpinRowTiles, l = i & 63;
+            const int64_t grp = (st >> 4) + ks;
+            ra[q] = i < 4 * kSpinRowTiles * 64 && t < vtl && grp * 16 < s1
+                        ? afrag[(grp * (vtiles) + vt0 + t) * 64 + l] : short4_t{0, 0, 0, 0};
         }
     };
     auto store = [&](int buf) {
@@ -210,25 +277,28 @@ __global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> ne
                                   : ((i & 1) ? cr * gi + ci * gr : cr * gr - ci * gi);
                 else
                     x = c == two_m ? cr : c == two_m + 1 ? ci : 0.f;
-#pragma unroll
-                for (int sp = 0; sp < 3; sp++) {
-                    const short b16 = bf16_bits(x);
-                    x -= bf16_value(b16);
-                    h[sp][e] = b16;
-                }
+                short h0, h1, h2;
+                split3(x, h0, h1, h2);
+                h[0][e] = h0;
+                h[1][e] = h1;
+                h[2][e] = h2;
             }
             const int ct = col >> 4, cc = col & 15;
 #pragma unroll
             for (int sp = 0; sp < 3; sp++) sbf[buf][sp][ks][ct][kl + cc] = h[sp];
         }
-        if (threadIdx.x < kSgStep)
 #pragma unroll
-            for (int k = 0; k < WORDS; k++) sw[buf][threadIdx.x][k] = rw[k];
+        for (int q = 0; q < kAfragPerThread; q++) {
+            const int i = threadIdx.x + 256 * q;
+            if (i < 4 * kSpinRowTiles * 64)
+                saf[buf][i / (kSpinRowTiles * 64)][(i / 64) % kSpinRowTiles][i & 63] = ra[q];
+        }
     };
     float4_t acc[kSpinRowTiles];
 #pragma unroll
     for (int t = 0; t < kSpinRowTiles; t++) acc[t] = {0.f, 0.f, 0.f, 0.f};
     const int kq = 4 * (lane >> 4);
+    (void)net;
     int buf = 0;
     load(s0);
     store(0);
@@ -238,21 +308,16 @@ __global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> ne
         if (more) load(st + kSgStep);
 #pragma unroll 1
         for (int ks = 0; ks < 4; ks++) {
-            uint64_t w4[4][kMaxWords];
+            short4_t b[3];
 #pragma unroll
-            for (int e = 0; e < 4; e++)
-#pragma unroll
-                for (int k = 0; k < kMaxWords; k++) w4[e][k] = k < WORDS ? sw[buf][ks * 16 + kq + e][k < WORDS ? k : 0] : 0;
+            for (int sp = 0; sp < 3; sp++) b[sp] = sbf[buf][sp][ks][wave][lane];
 #pragma unroll
             for (int t = 0; t < kSpinRowTiles; t++) {
-                if (vt0 + t >= vtiles) continue;
-                const int v = (vt0 + t) * 16 + (lane & 15);
-                short4_t a;
-#pragma unroll
-                for (int e = 0; e < 4; e++) a[e] = spin_bf16(w4[e], v, net.N);
+                if (t >= vtl) continue;
+                const short4_t a = saf[buf][ks][t][lane];
 #pragma unroll
                 for (int sp = 0; sp < 3; sp++)
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, sbf[buf][sp][ks][wave][lane], acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[sp], acc[t], 0, 0, 0);
             }
         }
         if (more) store(buf ^ 1);
@@ -275,6 +340,7 @@ __global__ void sum_partials_f32_kernel(const float* partial, int chunks, int64_
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= n) return;
     double acc = 0.0;
+#pragma unroll 8
     for (int c = 0; c < chunks; c++) acc += double(partial[c * n + e]);
     out[e] = acc;
 }
@@ -427,7 +493,7 @@ __global__ __launch_bounds__(256) void ov_finish_center_kernel(const cplx<double
 
 // y_k (parameter order) = sum over chunks of the spin-GEMM partials + shift v_k,
 // plus the block partials of v^H y
-__global__ __launch_bounds__(256) void spin_y_kernel(const float* partial, int chunks, int64_t n, int ncols,
+__global__ __launch_bounds__(256) void spin_y_kernel(const double* full, int ncols,
                                                      int layout, int N, int M, int Mpad, const uint32_t* conn,
                                                      const cplx<double>* v, double shift, int P,
                                                      cplx<double>* y, double* ypart) {
@@ -450,12 +516,7 @@ __global__ __launch_bounds__(256) void spin_y_kernel(const float* partial, int c
             col = 2 * (q % M);
         }
         const int64_t off = int64_t(row) * ncols + col;
-        double yr = 0.0, yi = 0.0;
-#pragma unroll 8
-        for (int c = 0; c < chunks; c++) {
-            yr += double(partial[c * n + off]);
-            yi += double(partial[c * n + off + 1]);
-        }
+        double yr = full[off], yi = full[off + 1];
         const cplx<double> vk = v[k];
         yr += shift * vk.re;
         yi += shift * vk.im;
@@ -493,26 +554,32 @@ SpinPlan spin_plan(int N, int Mpad, int64_t count) {
     return p;
 }
 
-void launch_spin_gemm_bf16(const DenseView<float>& net, const uint64_t* confs, int words, const cplx<float>* g,
-                           int mode, const cplx<double>* coef, int64_t count, const SpinPlan& p, float* partial,
+void launch_spin_frags(const uint64_t* confs, int words, int64_t count, int N, int npad, int vpad, uint16_t* ov,
+                       uint16_t* sg, hipStream_t stream) {
+    if (count <= 0) return;
+    const int KT = npad / 16, VT = vpad / 16;
+    const int64_t n = (count + 15) / 16 * (KT + VT) * 64;
+    spin_frag_kernel<<<unsigned((n + 255) / 256), 256, 0, stream>>>(confs, words, count, N, KT, VT,
+                                                                    reinterpret_cast<short4_t*>(ov),
+                                                                    reinterpret_cast<short4_t*>(sg));
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_spin_gemm_bf16(const DenseView<float>& net, const uint16_t* sgfrag, const cplx<float>* g, int mode,
+                           const cplx<double>* coef, int64_t count, const SpinPlan& p, float* partial,
                            hipStream_t stream) {
     if (count <= 0) return;
     if (net.N + 1 > 16 * kSpinRowTiles * p.zgroups) throw std::invalid_argument("spin GEMM plan too small");
     const dim3 grid{unsigned(p.cblocks), unsigned(p.chunks), unsigned(p.zgroups)};
-    switch (words) {
-        case 1: spin_gemm_bf16_kernel<1><<<grid, 256, 0, stream>>>(net, confs, g, mode, coef, count, p.chunk, p.vpad, p.ncols, partial); break;
-        case 2: spin_gemm_bf16_kernel<2><<<grid, 256, 0, stream>>>(net, confs, g, mode, coef, count, p.chunk, p.vpad, p.ncols, partial); break;
-        case 3: spin_gemm_bf16_kernel<3><<<grid, 256, 0, stream>>>(net, confs, g, mode, coef, count, p.chunk, p.vpad, p.ncols, partial); break;
-        case 4: spin_gemm_bf16_kernel<4><<<grid, 256, 0, stream>>>(net, confs, g, mode, coef, count, p.chunk, p.vpad, p.ncols, partial); break;
-        default: throw std::invalid_argument("configurations of 1..4 words are supported");
-    }
+    spin_gemm_bf16_kernel<<<grid, 256, 0, stream>>>(net, reinterpret_cast<const short4_t*>(sgfrag), g, mode, coef,
+                                                    count, p.chunk, p.vpad, p.ncols, partial);
     ANN_LAUNCH_CHECK();
 }
 
 void launch_sr_matvec_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const cplx<double>* obar,
                            const double* w, const uint64_t* confs, int words, int64_t count, const cplx<float>* g,
-                           int npad, double shift, const SpinPlan& plan, SrWorkspace ws, cplx<double>* y,
-                           hipStream_t stream) {
+                           const uint16_t* ovfrag, const uint16_t* sgfrag, int npad, double shift,
+                           const SpinPlan& plan, SrWorkspace ws, cplx<double>* y, hipStream_t stream) {
     if (count <= 0) return;
     const int KT = npad / 16, NT = 2 * net.Mpad / 16;
     const int64_t npack = int64_t(KT) * 16 * 2 * net.Mpad;
@@ -531,16 +598,20 @@ void launch_sr_matvec_bf16(const DenseView<float>& net, const int* inv, const cp
         ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(ov_mfma_kernel),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     const dim3 grid{unsigned(per_col), unsigned(NT / 4)};
-    ov_mfma_kernel<<<grid, 256, lds, stream>>>(net, reinterpret_cast<const short4_t*>(ws.frag), KT, NT, confs, words,
-                                               count, g, deep ? ws.vb : nullptr, ws.upart);
+    ov_mfma_kernel<<<grid, 256, lds, stream>>>(net, reinterpret_cast<const short4_t*>(ws.frag), KT, NT,
+                                               reinterpret_cast<const short4_t*>(ovfrag), count, g,
+                                               deep ? ws.vb : nullptr, ws.upart);
     ANN_LAUNCH_CHECK();
     ov_finish_center_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(
         ws.upart, NT / 4, count, confs, words, net.N, deep ? ws.va : nullptr, ws.opart, w, ws.u);
     ANN_LAUNCH_CHECK();
-    launch_spin_gemm_bf16(net, confs, words, g, 1, ws.u, count, plan, ws.spin_part, stream);
+    launch_spin_gemm_bf16(net, sgfrag, g, 1, ws.u, count, plan, ws.spin_part, stream);
+    // chunk partials added in fp64 (coalesced), then gathered in parameter order
+    sum_partials_f32_kernel<<<unsigned((plan.n + 255) / 256), 256, 0, stream>>>(ws.spin_part, plan.chunks, plan.n,
+                                                                                ws.full);
+    ANN_LAUNCH_CHECK();
     spin_y_kernel<<<unsigned(sr_y_parts(net.num_params)), 256, 0, stream>>>(
-        ws.spin_part, plan.chunks, plan.n, plan.ncols, net.layout, net.N, net.M, net.Mpad, net.conn, v, shift,
-        net.num_params, y, ws.ypart);
+        ws.full, plan.ncols, net.layout, net.N, net.M, net.Mpad, net.conn, v, shift, net.num_params, y, ws.ypart);
     ANN_LAUNCH_CHECK();
 }
 
@@ -551,9 +622,13 @@ void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, i
     const SpinPlan p = spin_plan(net.N, net.Mpad, count);
     float* partial = nullptr;
     double* full = nullptr;
+    uint16_t* sg = nullptr;
+    const size_t nsg = size_t((count + 15) / 16) * (p.vpad / 16) * 64 * 4;
     ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(float) * p.n * p.chunks, stream));
     ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&full), sizeof(double) * p.n, stream));
-    launch_spin_gemm_bf16(net, confs, words, g, mode, coef, count, p, partial, stream);
+    ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&sg), sizeof(uint16_t) * nsg, stream));
+    launch_spin_frags(confs, words, count, net.N, 0, p.vpad, nullptr, sg, stream);
+    launch_spin_gemm_bf16(net, sg, g, mode, coef, count, p, partial, stream);
     sum_partials_f32_kernel<<<unsigned((p.n + 255) / 256), 256, 0, stream>>>(partial, p.chunks, p.n, full);
     ANN_LAUNCH_CHECK();
     const int64_t ns = int64_t(net.N + 1) * net.Mpad + net.N;
@@ -561,6 +636,7 @@ void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, i
     ANN_LAUNCH_CHECK();
     ANN_HIP_CHECK(hipFreeAsync(partial, stream));
     ANN_HIP_CHECK(hipFreeAsync(full, stream));
+    ANN_HIP_CHECK(hipFreeAsync(sg, stream));
 }
 
 }  // namespace ann
