@@ -22,6 +22,7 @@
 #include "wave_net.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace ann {
 
@@ -70,12 +71,41 @@ __device__ inline double row16_sum(double v) {
 
 constexpr int kMaxKT = 16;   // site tiles of 16: configurations of up to 256 sites
 
+template <int K>
+struct KtTag {
+    static constexpr int value = K;
+};
+
+template <typename F>
+void dispatch_kt(int kt, F&& fn) {
+    switch (kt) {
+        case 1: fn(KtTag<1>{}); break;
+        case 2: fn(KtTag<2>{}); break;
+        case 3: fn(KtTag<3>{}); break;
+        case 4: fn(KtTag<4>{}); break;
+        case 5: fn(KtTag<5>{}); break;
+        case 6: fn(KtTag<6>{}); break;
+        case 7: fn(KtTag<7>{}); break;
+        case 8: fn(KtTag<8>{}); break;
+        case 9: fn(KtTag<9>{}); break;
+        case 10: fn(KtTag<10>{}); break;
+        case 11: fn(KtTag<11>{}); break;
+        case 12: fn(KtTag<12>{}); break;
+        case 13: fn(KtTag<13>{}); break;
+        case 14: fn(KtTag<14>{}); break;
+        case 15: fn(KtTag<15>{}); break;
+        case 16: fn(KtTag<16>{}); break;
+        default: throw std::invalid_argument("configurations of 1..256 sites are supported");
+    }
+}
+
 // ------------------------------------------------------------ u = O.v (part)
 // A workgroup owns one column group (4 tiles of the split layout: tile 2p = real
 // parts of units 16p..16p+15, tile 2p+1 = imaginary parts) and keeps its bf16x3
 // fragments of V in LDS; each wave then walks 16-sample groups, computes their
 // angles of v on matrix cores and contracts them with g.  upart[col group][s].
-__global__ __launch_bounds__(256) void ov_mfma_kernel(DenseView<float> net, const short4_t* vfrag, int KT, int NT,
+template <int KT>
+__global__ __launch_bounds__(256) void ov_mfma_kernel(DenseView<float> net, const short4_t* vfrag, int NT,
                                                       const short4_t* afrag, int64_t count,
                                                       const cplx<float>* g, const cplx<double>* vb,
                                                       cplx<double>* upart) {
@@ -125,14 +155,13 @@ __global__ __launch_bounds__(256) void ov_mfma_kernel(DenseView<float> net, cons
         float4_t acc[4];
 #pragma unroll
         for (int t = 0; t < 4; t++) acc[t] = {0.f, 0.f, 0.f, 0.f};
-        // all spin fragments of the group are loaded before the first MFMA (KT <= 16)
+        // all spin fragments of the group are loaded before the first MFMA
         const short4_t* ag = afrag + grp * KT * 64 + lane;
-        short4_t a[kMaxKT];
+        short4_t a[KT];
 #pragma unroll
-        for (int kt = 0; kt < kMaxKT; kt++) a[kt] = kt < KT ? ag[kt * 64] : short4_t{0, 0, 0, 0};
+        for (int kt = 0; kt < KT; kt++) a[kt] = ag[kt * 64];
 #pragma unroll
-        for (int kt = 0; kt < kMaxKT; kt++) {
-            if (kt >= KT) break;
+        for (int kt = 0; kt < KT; kt++) {
 #pragma unroll
             for (int sp = 0; sp < 3; sp++)
 #pragma unroll
@@ -594,13 +623,16 @@ void launch_sr_matvec_bf16(const DenseView<float>& net, const int* inv, const cp
     const int per_col = int(std::max<int64_t>(1, std::min<int64_t>((groups + 3) / 4, 1024 / (NT / 4))));
     const size_t lds = sizeof(short4_t) * 3 * KT * 4 * 64;
     if (lds > 160 * 1024) throw std::invalid_argument("too many sites for the LDS-resident V fragments");
-    if (lds > 64 * 1024)
-        ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(ov_mfma_kernel),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     const dim3 grid{unsigned(per_col), unsigned(NT / 4)};
-    ov_mfma_kernel<<<grid, 256, lds, stream>>>(net, reinterpret_cast<const short4_t*>(ws.frag), KT, NT,
-                                               reinterpret_cast<const short4_t*>(ovfrag), count, g,
-                                               deep ? ws.vb : nullptr, ws.upart);
+    dispatch_kt(KT, [&](auto tag) {
+        constexpr int kt = std::decay_t<decltype(tag)>::value;
+        if (lds > 64 * 1024)
+            ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(ov_mfma_kernel<kt>),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        ov_mfma_kernel<kt><<<grid, 256, lds, stream>>>(net, reinterpret_cast<const short4_t*>(ws.frag), NT,
+                                                       reinterpret_cast<const short4_t*>(ovfrag), count, g,
+                                                       deep ? ws.vb : nullptr, ws.upart);
+    });
     ANN_LAUNCH_CHECK();
     ov_finish_center_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(
         ws.upart, NT / 4, count, confs, words, net.N, deep ? ws.va : nullptr, ws.opart, w, ws.u);
