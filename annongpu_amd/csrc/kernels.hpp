@@ -138,12 +138,16 @@ struct SrWorkspace {
     double* ypart;
 };
 // y = (S + shift) v with S = sum_s w_s conj(O_s) O_s^T - conj(<O>) <O>^T for the
-// structured O; also leaves the block partials of v^H y in ws.ypart ([sr_y_parts(P)][2])
+// structured O, in three launches: launch_sr_ov_bf16 (u_s = w_s (O_s.v - <O>.v) into
+// ws.u), launch_spin_gemm_bf16 (mode 1, coefficients ws.u, partials into
+// ws.spin_part), launch_sr_y_bf16 (y, and the partials of v^H y in ws.ypart,
+// [sr_y_parts(P)][2])
 int sr_y_parts(int P);
-void launch_sr_matvec_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const cplx<double>* obar,
-                           const double* w, const uint64_t* confs, int words, int64_t count, const cplx<float>* g,
-                           const uint16_t* ovfrag, const uint16_t* sgfrag, int npad, double shift,
-                           const SpinPlan& plan, SrWorkspace ws, cplx<double>* y, hipStream_t stream);
+void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const cplx<double>* obar,
+                       const double* w, const uint64_t* confs, int words, int64_t count, const cplx<float>* g,
+                       const uint16_t* ovfrag, int npad, SrWorkspace ws, hipStream_t stream);
+void launch_sr_y_bf16(const DenseView<float>& net, const cplx<double>* v, double shift, const SpinPlan& plan,
+                      SrWorkspace ws, cplx<double>* y, hipStream_t stream);
 
 void launch_weighted_sums(const cplx<double>* e_loc, const double* w, int64_t count,
                           double* out4 /* E.re, E.im, E2, sum w */, hipStream_t stream);

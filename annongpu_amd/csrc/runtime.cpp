@@ -898,9 +898,19 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y,
         t.spin_full.ensure(size_t(plan.n));
         const SrWorkspace ws{t.frag.p,      t.va.p,        t.vb.p,    t.upart.p, t.u.p,
                              t.spin_part.p, t.spin_full.p, t.opart.p, t.ypart.p};
-        ScopedTimer timer("sr_matvec");
-        launch_sr_matvec_bf16(net, t.inv.p, v, t.Obar.p, t.w.p, t.confs.p, t.words, t.count, g.p, t.ovfrag.p,
-                              t.sgfrag.p, t.npad, shift, plan, ws, y, S());
+        {
+            ScopedTimer timer("sr_ov");
+            launch_sr_ov_bf16(net, t.inv.p, v, t.Obar.p, t.w.p, t.confs.p, t.words, t.count, g.p, t.ovfrag.p,
+                              t.npad, ws, S());
+        }
+        {
+            ScopedTimer timer("spin_gemm");
+            launch_spin_gemm_bf16(net, t.sgfrag.p, g.p, 1, ws.u, t.count, plan, ws.spin_part, S());
+        }
+        {
+            ScopedTimer timer("sr_y");
+            launch_sr_y_bf16(net, v, shift, plan, ws, y, S());
+        }
         if (ypart) *ypart = t.ypart.p;
         if (yparts) *yparts = sr_y_parts(P);
         return;

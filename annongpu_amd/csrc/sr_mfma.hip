@@ -605,10 +605,9 @@ void launch_spin_gemm_bf16(const DenseView<float>& net, const uint16_t* sgfrag, 
     ANN_LAUNCH_CHECK();
 }
 
-void launch_sr_matvec_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const cplx<double>* obar,
-                           const double* w, const uint64_t* confs, int words, int64_t count, const cplx<float>* g,
-                           const uint16_t* ovfrag, const uint16_t* sgfrag, int npad, double shift,
-                           const SpinPlan& plan, SrWorkspace ws, cplx<double>* y, hipStream_t stream) {
+void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const cplx<double>* obar,
+                       const double* w, const uint64_t* confs, int words, int64_t count, const cplx<float>* g,
+                       const uint16_t* ovfrag, int npad, SrWorkspace ws, hipStream_t stream) {
     if (count <= 0) return;
     const int KT = npad / 16, NT = 2 * net.Mpad / 16;
     const int64_t npack = int64_t(KT) * 16 * 2 * net.Mpad;
@@ -637,7 +636,10 @@ void launch_sr_matvec_bf16(const DenseView<float>& net, const int* inv, const cp
     ov_finish_center_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(
         ws.upart, NT / 4, count, confs, words, net.N, deep ? ws.va : nullptr, ws.opart, w, ws.u);
     ANN_LAUNCH_CHECK();
-    launch_spin_gemm_bf16(net, sgfrag, g, 1, ws.u, count, plan, ws.spin_part, stream);
+}
+
+void launch_sr_y_bf16(const DenseView<float>& net, const cplx<double>* v, double shift, const SpinPlan& plan,
+                      SrWorkspace ws, cplx<double>* y, hipStream_t stream) {
     // chunk partials added in fp64 (coalesced), then gathered in parameter order
     sum_partials_f32_kernel<<<unsigned((plan.n + 255) / 256), 256, 0, stream>>>(ws.spin_part, plan.chunks, plan.n,
                                                                                 ws.full);
