@@ -70,6 +70,19 @@ __device__ inline double row16_sum(double v) {
 }
 
 constexpr int kMaxKT = 16;   // site tiles of 16: configurations of up to 256 sites
+constexpr int kXcds = 8;     // workgroups are dispatched round-robin over the 8 XCDs
+
+// XCD-aware tile order: the workgroups of one "row" (a sample chunk or slice,
+// shared data: its spin fragments and coefficients) all land on the same XCD,
+// so that data is fetched into one XCD's L2 only.  Linear workgroup id L runs on
+// XCD L % 8; rows xcd, xcd + 8, ... are given to that XCD, cols fastest.
+__device__ inline bool xcd_tile(int L, int rows, int cols, int& row, int& col) {
+    const int xcd = L % kXcds, slot = L / kXcds;
+    row = xcd + kXcds * (slot / cols);
+    col = slot % cols;
+    return row < rows;
+}
+inline int xcd_grid(int rows, int cols) { return kXcds * ((rows + kXcds - 1) / kXcds) * cols; }
 
 template <int K>
 struct KtTag {
@@ -106,14 +119,15 @@ void dispatch_kt(int kt, F&& fn) {
 // angles of v on matrix cores and contracts them with g.  upart[col group][s].
 template <int KT>
 __global__ __launch_bounds__(256) void ov_mfma_kernel(DenseView<float> net, const short4_t* vfrag, int NT,
-                                                      const short4_t* afrag, int64_t count,
+                                                      int slices, const short4_t* afrag, int64_t count,
                                                       const cplx<float>* g, const cplx<double>* vb,
                                                       cplx<double>* upart) {
     extern __shared__ __align__(16) unsigned char ov_smem[];
     short4_t* sv = reinterpret_cast<short4_t*>(ov_smem);   // [3][KT][4][64]
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int y = blockIdx.y;
+    int slice, y;
+    if (!xcd_tile(blockIdx.x, slices, NT / 4, slice, y)) return;
     const int nt0 = 4 * y;
     const int nfrag = 3 * KT * 4 * 64;
     // the whole slice is loaded before any of it is stored (the loads overlap)
@@ -141,7 +155,7 @@ __global__ __launch_bounds__(256) void ov_mfma_kernel(DenseView<float> net, cons
     }
     const int kq = 4 * (lane >> 4);
     const int64_t groups = (count + 15) / 16;
-    for (int64_t grp = int64_t(blockIdx.x) * 4 + wave; grp < groups; grp += int64_t(gridDim.x) * 4) {
+    for (int64_t grp = int64_t(slice) * 4 + wave; grp < groups; grp += int64_t(slices) * 4) {
         const int64_t row0 = grp * 16;
         // g of this lane's output elements, loaded ahead of the matrix-core work
         cplx<float> gv[4][2];
@@ -247,15 +261,18 @@ constexpr int kSgStep = 64;
 __global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> net, const short4_t* afrag,
                                                              const cplx<float>* g, int mode,
                                                              const cplx<double>* coef, int64_t count, int64_t chunk,
-                                                             int vpad, int ncols, float* partial) {
+                                                             int chunks, int cblocks, int vpad, int ncols,
+                                                             float* partial) {
     __shared__ short4_t sbf[2][3][4][4][64];        // [buf][split][k-step][col tile][lane]
     __shared__ short4_t saf[2][4][kSpinRowTiles][64];  // [buf][k-step][row tile][lane] spin fragments
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int c0 = blockIdx.x * 64;
-    const int vt0 = blockIdx.z * kSpinRowTiles;
+    int chunk_id, cb;
+    if (!xcd_tile(blockIdx.x, chunks, cblocks, chunk_id, cb)) return;
+    const int c0 = cb * 64;
+    const int vt0 = blockIdx.y * kSpinRowTiles;
     const int vtiles = vpad / 16;
-    const int64_t s0 = int64_t(blockIdx.y) * chunk;
+    const int64_t s0 = int64_t(chunk_id) * chunk;
     const int64_t s1 = s0 + chunk < count ? s0 + chunk : count;
     const int two_m = 2 * net.Mpad;
     // loader role: samples 4 sg .. 4 sg + 3 of the stage, complex columns jb, jb + 1
@@ -360,7 +377,7 @@ __global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> ne
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int v = (vt0 + t) * 16 + kq + r;
-            partial[(int64_t(blockIdx.y) * vpad + v) * ncols + c] = acc[t][r];
+            partial[(int64_t(chunk_id) * vpad + v) * ncols + c] = acc[t][r];
         }
     }
 }
@@ -486,38 +503,49 @@ __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, 
     }
 }
 
-// u_s = w_s (sum_b upart[b][s] + sum_i s_i va_i - <O>.v)
+// u_s = w_s (sum_b upart[b][s] + sum_i s_i va_i - <O>.v); four lanes per sample
+// (a quad), each taking every fourth partial and site, summed by quad DPP
+__device__ inline double quad_sum(double v) {
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    return v;
+}
+
 __global__ __launch_bounds__(256) void ov_finish_center_kernel(const cplx<double>* upart, int blocks, int64_t count,
                                                                const uint64_t* confs, int words, int N,
                                                                const cplx<double>* va, const double* opart,
                                                                const double* w, cplx<double>* u) {
     __shared__ double sh[4];
-    __shared__ cplx<double> sva[4 * 64 * kMaxWords / 4];   // va of up to 256 sites
+    __shared__ cplx<double> sva[64 * kMaxWords];   // va of up to 256 sites
     const double c_re = parts_total(opart, kDotParts, 2, 0, sh);
     const double c_im = parts_total(opart, kDotParts, 2, 1, sh);
     if (va)
         for (int i = threadIdx.x; i < N; i += blockDim.x) sva[i] = va[i];
     __syncthreads();
-    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= count) return;
+    const int64_t s = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 2;
+    const int q = threadIdx.x & 3;
+    const bool ok = s < count;
     double re = 0.0, im = 0.0;
-#pragma unroll 8
-    for (int b = 0; b < blocks; b++) {
-        re += upart[int64_t(b) * count + s].re;
-        im += upart[int64_t(b) * count + s].im;
-    }
-    if (va) {
-        uint64_t cw[kMaxWords];
+    if (ok) {
+        for (int b = q; b < blocks; b += 4) {
+            re += upart[int64_t(b) * count + s].re;
+            im += upart[int64_t(b) * count + s].im;
+        }
+        if (va) {
+            uint64_t cw[kMaxWords];
 #pragma unroll
-        for (int k = 0; k < kMaxWords; k++) cw[k] = k < words ? confs[s * words + k] : 0;
-        for (int i = 0; i < N; i++) {
-            const uint64_t word = i < 64 ? cw[0] : i < 128 ? cw[1] : i < 192 ? cw[2] : cw[3];
-            const double si = ((word >> (i & 63)) & 1) ? 1.0 : -1.0;
-            re += si * sva[i].re;
-            im += si * sva[i].im;
+            for (int k = 0; k < kMaxWords; k++) cw[k] = k < words ? confs[s * words + k] : 0;
+            for (int i = q; i < N; i += 4) {
+                const uint64_t word = i < 64 ? cw[0] : i < 128 ? cw[1] : i < 192 ? cw[2] : cw[3];
+                const double si = ((word >> (i & 63)) & 1) ? 1.0 : -1.0;
+                re += si * sva[i].re;
+                im += si * sva[i].im;
+            }
         }
     }
-    u[s] = {w[s] * (re - c_re), w[s] * (im - c_im)};
+    re = quad_sum(re);
+    im = quad_sum(im);
+    if (ok && q == 0) u[s] = {w[s] * (re - c_re), w[s] * (im - c_im)};
 }
 
 // y_k (parameter order) = sum over chunks of the spin-GEMM partials + shift v_k,
@@ -599,9 +627,9 @@ void launch_spin_gemm_bf16(const DenseView<float>& net, const uint16_t* sgfrag, 
                            hipStream_t stream) {
     if (count <= 0) return;
     if (net.N + 1 > 16 * kSpinRowTiles * p.zgroups) throw std::invalid_argument("spin GEMM plan too small");
-    const dim3 grid{unsigned(p.cblocks), unsigned(p.chunks), unsigned(p.zgroups)};
+    const dim3 grid{unsigned(xcd_grid(p.chunks, p.cblocks)), unsigned(p.zgroups)};
     spin_gemm_bf16_kernel<<<grid, 256, 0, stream>>>(net, reinterpret_cast<const short4_t*>(sgfrag), g, mode, coef,
-                                                    count, p.chunk, p.vpad, p.ncols, partial);
+                                                    count, p.chunk, p.chunks, p.cblocks, p.vpad, p.ncols, partial);
     ANN_LAUNCH_CHECK();
 }
 
@@ -619,21 +647,22 @@ void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<d
     const bool deep = net.layout == LAYOUT_DEEP1;
     // about 1024 workgroups in all, each column group's V fragments in LDS
     const int64_t groups = (count + 15) / 16;
-    const int per_col = int(std::max<int64_t>(1, std::min<int64_t>((groups + 3) / 4, 1024 / (NT / 4))));
+    int slices = int(std::max<int64_t>(1, std::min<int64_t>((groups + 3) / 4, 1024 / (NT / 4))));
+    slices = (slices + kXcds - 1) / kXcds * kXcds;
     const size_t lds = sizeof(short4_t) * 3 * KT * 4 * 64;
     if (lds > 160 * 1024) throw std::invalid_argument("too many sites for the LDS-resident V fragments");
-    const dim3 grid{unsigned(per_col), unsigned(NT / 4)};
+    const dim3 grid{unsigned(xcd_grid(slices, NT / 4))};
     dispatch_kt(KT, [&](auto tag) {
         constexpr int kt = std::decay_t<decltype(tag)>::value;
         if (lds > 64 * 1024)
             ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(ov_mfma_kernel<kt>),
                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
         ov_mfma_kernel<kt><<<grid, 256, lds, stream>>>(net, reinterpret_cast<const short4_t*>(ws.frag), NT,
-                                                       reinterpret_cast<const short4_t*>(ovfrag), count, g,
+                                                       slices, reinterpret_cast<const short4_t*>(ovfrag), count, g,
                                                        deep ? ws.vb : nullptr, ws.upart);
     });
     ANN_LAUNCH_CHECK();
-    ov_finish_center_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(
+    ov_finish_center_kernel<<<unsigned((4 * count + 255) / 256), 256, 0, stream>>>(
         ws.upart, NT / 4, count, confs, words, net.N, deep ? ws.va : nullptr, ws.opart, w, ws.u);
     ANN_LAUNCH_CHECK();
 }
