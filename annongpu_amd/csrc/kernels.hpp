@@ -109,43 +109,53 @@ void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, i
                            int mode, const cplx<double>* coef, int64_t count, cplx<double>* Yw, cplx<double>* Yb,
                            cplx<double>* Ya, hipStream_t stream);
 
-// the spin GEMM's partition of the samples and its fp32 partial buffer ([chunks][n] floats)
+// the spin GEMM's partition of the samples: npairs column pairs (16 hidden units
+// each, the last one the coefficient columns) x chunks of `chunk` samples x zgroups
+// row groups; fp32 partials [chunks][n] floats, n = vpad * ncols
 struct SpinPlan {
-    int vpad = 0, ncols = 0, cblocks = 0, zgroups = 0, chunks = 0;
-    int64_t chunk = 0, n = 0;
+    int vpad = 0, ncols = 0, npairs = 0, zgroups = 0, chunks = 0;
+    int64_t chunk = 0, n = 0, bit_blocks = 0;
 };
 SpinPlan spin_plan(int N, int Mpad, int64_t count);
-// bf16 spin fragments of the configurations (both GEMM orientations, see sr_mfma.hip);
-// ov: ceil(count/16) * npad/16 * 256 and sg: ceil(count/16) * vpad/16 * 256 uint16 (either may be null
-// when its dimension is 0)
-void launch_spin_frags(const uint64_t* confs, int words, int64_t count, int N, int npad, int vpad, uint16_t* ov,
-                       uint16_t* sg, hipStream_t stream);
-void launch_spin_gemm_bf16(const DenseView<float>& net, const uint16_t* sgfrag, const cplx<float>* g, int mode,
-                           const cplx<double>* coef, int64_t count, const SpinPlan& p, float* partial,
-                           hipStream_t stream);
-// workspace of the fused S.v (fp32 dense nets); sizes: frag 3 * npad/16 * 2Mpad/16 * 256,
-// va N, vb Mpad, upart (2Mpad/64) * count, u count, spin_part chunks * n, opart 128,
+// transposed spin bits of the configurations: bits[(count+63)/64][vpad], bit b of
+// word [blk][v] = spin v of sample 64 blk + b (row N all ones)
+void launch_spin_bits(const uint64_t* confs, int words, int64_t count, int N, int vpad, uint64_t* bits,
+                      hipStream_t stream);
+// rows of g and of the fp32 coefficients the spin GEMM and O.v kernels read: count
+// rounded up to whole 256-sample rounds, the rows beyond count zero
+int64_t sr_count_pad(int64_t count);
+// Y = S^T X partials per sample chunk (X from g and coef as selected by mode, see
+// sr_mfma.hip); with_coef = 0 leaves the coefficient columns zero; g and coef
+// have sr_count_pad(count) rows
+void launch_spin_gemm32(const DenseView<float>& net, const uint64_t* bits, const cplx<float>* g, int mode,
+                        const cplx<float>* coef, int with_coef, int64_t count, const SpinPlan& p, float* partial,
+                        hipStream_t stream);
+
+// workspace of the fused fp32 matvec: frag sr_frag_elems(N, Mpad), va N, vb Mpad,
+// upart sr_ov_pairs(M) * count, u sr_count_pad(count), spin_part chunks * n, opart 128,
 // ypart 2 * sr_y_parts(P)
 struct SrWorkspace {
     uint16_t* frag;
     cplx<double>* va;
     cplx<double>* vb;
-    cplx<double>* upart;
-    cplx<double>* u;
+    cplx<float>* upart;
+    cplx<float>* u;
     float* spin_part;
-    double* full;             // vpad * ncols
     double* opart;
     double* ypart;
 };
 // y = (S + shift) v with S = sum_s w_s conj(O_s) O_s^T - conj(<O>) <O>^T for the
 // structured O, in three launches: launch_sr_ov_bf16 (u_s = w_s (O_s.v - <O>.v) into
-// ws.u), launch_spin_gemm_bf16 (mode 1, coefficients ws.u, partials into
+// ws.u), launch_spin_gemm32 (mode 1, coefficients ws.u, partials into
 // ws.spin_part), launch_sr_y_bf16 (y, and the partials of v^H y in ws.ypart,
 // [sr_y_parts(P)][2])
 int sr_y_parts(int P);
+int sr_kt32(int N);
+int sr_ov_pairs(int M);
+size_t sr_frag_elems(int N, int Mpad);
 void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const cplx<double>* obar,
                        const double* w, const uint64_t* confs, int words, int64_t count, const cplx<float>* g,
-                       const uint16_t* ovfrag, int npad, SrWorkspace ws, hipStream_t stream);
+                       SrWorkspace ws, hipStream_t stream);
 void launch_sr_y_bf16(const DenseView<float>& net, const cplx<double>* v, double shift, const SpinPlan& plan,
                       SrWorkspace ws, cplx<double>* y, hipStream_t stream);
 

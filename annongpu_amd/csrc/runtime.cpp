@@ -641,6 +641,17 @@ struct Prec<float> {
     static DenseView<float> view(Psi& p) { return p.view32(); }
 };
 
+// g = f tanh(theta) of `count` samples into g, whose rows are padded to
+// sr_count_pad(count) with zeros (the matrix-core kernels read whole 256-sample rounds)
+template <typename T>
+void g_factor_padded(const DenseView<T>& net, const cplx<T>* theta, int64_t count, DevBuf<cplx<T>>& g) {
+    const int64_t pad = sr_count_pad(count);
+    g.ensure(size_t(pad) * net.Mpad);
+    launch_g_factor<T>(net, theta, count, g.p, S());
+    if (pad > count)
+        ANN_HIP_CHECK(hipMemsetAsync(g.p + size_t(count) * net.Mpad, 0, sizeof(cplx<T>) * (pad - count) * net.Mpad, S()));
+}
+
 // weighted sums of the structured O, packed in parameter order: out = sum_s coef_s O_s (or conj O_s)
 template <typename T>
 void structured_sums(Tdvp& t, const DenseView<T>& net, const uint64_t* confs, int words, const cplx<T>* g,
@@ -677,10 +688,9 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     t.wsum = h[3];
     auto net = Prec<T>::view(psi);
     auto& g = Prec<T>::g(t);
-    g.ensure(size_t(n) * psi.Mpad);
     {
         ScopedTimer timer("o_k");
-        launch_g_factor<T>(net, Prec<T>::th(s).p, n, g.p, S());
+        g_factor_padded<T>(net, Prec<T>::th(s).p, n, g);
     }
     DevBuf<cplx<double>> coef;
     coef.ensure(n);
@@ -725,7 +735,7 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     t.w.ensure(n);
     ANN_HIP_CHECK(hipMemcpyAsync(t.w.p, s.w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, S()));
     t.has_samples = true;
-    t.frags_valid = false;
+    t.bits_valid = false;
     t.has_S = false;
     if (with_S) {
         DevBuf<cplx<T>> O;
@@ -883,29 +893,27 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y,
     t.u.ensure(t.count);
     if constexpr (sizeof(T) == 4) {
         const SpinPlan plan = spin_plan(t.N, t.Mpad, t.count);
-        t.frag.ensure(size_t(3) * (t.npad / 16) * (2 * t.Mpad / 16) * 64 * 4);
-        t.upart.ensure(size_t(2 * t.Mpad / 64) * t.count);
+        t.frag.ensure(sr_frag_elems(t.N, t.Mpad));
+        t.upart.ensure(size_t(sr_ov_pairs(t.M)) * t.count);
+        t.u32.ensure(size_t(sr_count_pad(t.count)));
         t.spin_part.ensure(size_t(plan.chunks) * plan.n);
         t.opart.ensure(256);
         t.ypart.ensure(512);
-        if (!t.frags_valid) {
-            // spin fragments of the samples, reused by every matvec until the next eval
-            t.ovfrag.ensure(size_t((t.count + 15) / 16) * (t.npad / 16) * 256);
-            t.sgfrag.ensure(size_t((t.count + 15) / 16) * (plan.vpad / 16) * 256);
-            launch_spin_frags(t.confs.p, t.words, t.count, t.N, t.npad, plan.vpad, t.ovfrag.p, t.sgfrag.p, S());
-            t.frags_valid = true;
+        if (!t.bits_valid) {
+            // transposed spins of the samples, reused by every matvec until the next eval
+            t.spin_bits.ensure(size_t(plan.bit_blocks) * plan.vpad);
+            launch_spin_bits(t.confs.p, t.words, t.count, t.N, plan.vpad, t.spin_bits.p, S());
+            t.bits_valid = true;
         }
-        t.spin_full.ensure(size_t(plan.n));
-        const SrWorkspace ws{t.frag.p,      t.va.p,        t.vb.p,    t.upart.p, t.u.p,
-                             t.spin_part.p, t.spin_full.p, t.opart.p, t.ypart.p};
+        const SrWorkspace ws{t.frag.p, t.va.p, t.vb.p, t.upart.p, t.u32.p, t.spin_part.p, t.opart.p, t.ypart.p};
         {
             ScopedTimer timer("sr_ov");
-            launch_sr_ov_bf16(net, t.inv.p, v, t.Obar.p, t.w.p, t.confs.p, t.words, t.count, g.p, t.ovfrag.p,
-                              t.npad, ws, S());
+            launch_sr_ov_bf16(net, t.inv.p, v, t.Obar.p, t.w.p, t.confs.p, t.words, t.count, g.p, ws, S());
         }
         {
             ScopedTimer timer("spin_gemm");
-            launch_spin_gemm_bf16(net, t.sgfrag.p, g.p, 1, ws.u, t.count, plan, ws.spin_part, S());
+            launch_spin_gemm32(net, t.spin_bits.p, g.p, 1, ws.u, t.layout == LAYOUT_DEEP1 ? 1 : 0, t.count, plan,
+                               ws.spin_part, S());
         }
         {
             ScopedTimer timer("sr_y");
@@ -972,8 +980,7 @@ void ok_weighted_sum_t(Psi& psi, Samples& s, const cplx<double>* coef, int conj,
     require_dense(psi);
     DenseView<T> net = Prec<T>::view(psi);
     DevBuf<cplx<T>> g;
-    g.ensure(size_t(s.count) * psi.Mpad);
-    launch_g_factor<T>(net, Prec<T>::th(s).p, s.count, g.p, S());
+    g_factor_padded<T>(net, Prec<T>::th(s).p, s.count, g);
     DevBuf<cplx<double>> Yw, Yb, Ya;
     Yw.ensure(size_t(psi.N) * psi.Mpad);
     Yb.ensure(psi.Mpad);

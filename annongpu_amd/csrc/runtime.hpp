@@ -257,13 +257,13 @@ struct Tdvp {
     DevBuf<cplx<double>> va, vb, u, c1, Yw, Yb, Ya;
     DevBuf<cplx<float>> V32, SV32, zero32;
     DevBuf<cplx<double>> V64, SV64, zero64;
-    DevBuf<uint16_t> frag;
-    DevBuf<cplx<double>> upart;  // per column block partial O.v (fp32 mode)
+    DevBuf<uint16_t> frag;       // bf16x3 fragments of v (fp32 mode)
+    DevBuf<cplx<float>> upart;   // per column pair partial O.v (fp32 mode)
+    DevBuf<cplx<float>> u32;     // w (O.v - <O>.v) per sample (fp32 mode)
     DevBuf<float> spin_part;     // spin GEMM partials (fp32 mode)
-    DevBuf<double> spin_full;    // their sum
     DevBuf<double> opart, ypart; // dot-product partials of the fused matvec
-    DevBuf<uint16_t> ovfrag, sgfrag;   // spin fragments of the samples (fp32 mode)
-    bool frags_valid = false;
+    DevBuf<uint64_t> spin_bits;  // transposed spin bits of the samples (fp32 mode)
+    bool bits_valid = false;
     DevBuf<int> inv;            // inverse gather table [N][Mpad] (PsiDeep)
     DevBuf<double> dot_scratch;
     // materialised O [count][P] of wave functions without a structured O (PsiCNN)

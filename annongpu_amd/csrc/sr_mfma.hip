@@ -11,13 +11,22 @@
 // Both GEMMs have the spins as one operand (+-1: exact in bf16) and fp32 data as
 // the other, which is split into three bf16 terms (hi + mid + lo carry 24
 // significant bits), so the products are exact and the sums are fp32 matrix-core
-// accumulations.  v_mfma_f32_16x16x16_bf16: A lane l = row l%16, k = 4(l/16)+e;
-// B lane l = k = 4(l/16)+e, column l%16; D lane l = row 4(l/16)+r, column l%16.
+// accumulations.  v_mfma_f32_16x16x32_bf16: A lane l = row l%16, k = 8(l/16)+e;
+// B lane l = k = 8(l/16)+e, column l%16; D lane l = row 4(l/16)+r, column l%16
+// (any k order shared by A and B gives the same product).
 //
-// ov_mfma_kernel fuses the first GEMM with the contraction against g, so the
-// [count][Mpad] angles of v are never written to memory; spin_gemm_bf16_kernel
-// streams g once per call and writes fp32 partial sums per sample chunk, which a
-// second kernel adds in fp64 in a fixed order.
+// Columns are handled in "pairs": pair p = hidden units 16p..16p+15, as two
+// 16-column tiles (real parts, imaginary parts).  One wavefront owns one pair,
+// keeps what it multiplies in registers and streams its column of g, so neither
+// kernel touches LDS in its main loop:
+//   ov32_kernel        V fragments of the pair in VGPRs, spins built from the
+//                      configuration words, angles of v contracted with g on the
+//                      fly; per-pair partial sums of O.v.
+//   spin_gemm32_kernel spins built from a transposed bit array (bit b of
+//                      bits[blk][v] = spin v of sample 64 blk + b), c o g split on
+//                      the fly; the 4 waves of a workgroup share a pair, reduce
+//                      through LDS once at the end and write one fp32 partial per
+//                      sample chunk, added in fp64 in a fixed order by the consumer.
 #include "kernels.hpp"
 #include "wave_net.hpp"
 
@@ -30,8 +39,14 @@ namespace {
 
 using namespace wave;
 
-typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t dw4_t __attribute__((ext_vector_type(4)));   // 8 bf16
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+__device__ inline float4_t mfma32(dw4_t a, dw4_t b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                   c, 0, 0, 0);
+}
 
 __device__ inline short bf16_bits(float x) {
     const uint32_t b = __float_as_uint(x);
@@ -40,55 +55,47 @@ __device__ inline short bf16_bits(float x) {
 __device__ inline float bf16_value(short h) { return __uint_as_float(uint32_t(uint16_t(h)) << 16); }
 
 // x = hi + mid + lo exactly (truncating split: each step keeps the top 8
-// significant bits, the remainder is exact in fp32 and has 8 bits fewer)
-__device__ inline void split3(float x, short& hi, short& mid, short& lo) {
-    const uint32_t b0 = __float_as_uint(x) & 0xFFFF0000u;
-    const float r1 = x - __uint_as_float(b0);
-    const uint32_t b1 = __float_as_uint(r1) & 0xFFFF0000u;
-    const float r2 = r1 - __uint_as_float(b1);
-    hi = short(b0 >> 16);
-    mid = short(b1 >> 16);
-    lo = short(__float_as_uint(r2) >> 16);
+// significant bits, the remainder is exact in fp32 and has 8 bits fewer); the
+// three terms are returned as fp32 bit patterns whose low halves are zero
+__device__ inline void split3_bits(float x, uint32_t& hi, uint32_t& mid, uint32_t& lo) {
+    hi = __float_as_uint(x) & 0xFFFF0000u;
+    const float r1 = x - __uint_as_float(hi);
+    mid = __float_as_uint(r1) & 0xFFFF0000u;
+    const float r2 = r1 - __uint_as_float(mid);
+    lo = __float_as_uint(r2) & 0xFFFF0000u;
 }
+// two truncated fp32 patterns -> one dword of two bf16 (element 2k low, 2k+1 high)
+__device__ inline uint32_t pack_bf16x2(uint32_t even, uint32_t odd) { return (even >> 16) | odd; }
 
-// +-1 / 0 of spin v of a configuration, as bf16 bits (v == N is the constant-1 row)
-__device__ inline short spin_bf16(const uint64_t* w, int v, int N) {
-    if (v < N) {
-        const uint64_t word = v < 64 ? w[0] : v < 128 ? w[1] : v < 192 ? w[2] : w[3];
-        return ((word >> (v & 63)) & 1) ? short(0x3F80) : short(0xBF80);
+// fragment of 8 spins from 8 bits (bit e = element e, 1 = up = +1):
+// dword k = bf16(element 2k) | bf16(element 2k+1) << 16, +1 = 0x3F80, -1 = 0xBF80
+__device__ inline dw4_t spin_frag8(uint32_t x) {
+    dw4_t a;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t t = x << (15 - 2 * k);   // bit 2k -> 15, bit 2k+1 -> 16 (x < 256)
+        a[k] = ((t | (t << 15)) & 0x80008000u) ^ 0xBF80BF80u;
     }
-    return v == N ? short(0x3F80) : short(0);
+    return a;
 }
 
-// 16-lane row sum (DPP within each row of 16 lanes)
-__device__ inline double row16_sum(double v) {
-    v += dpp_f64<0xB1>(v);
-    v += dpp_f64<0x4E>(v);
-    v += dpp_f64<0x141>(v);
-    v += dpp_f64<0x140>(v);
+__device__ inline float row16_sum_f(float v) {
+    v += dpp_f32<0xB1>(v);
+    v += dpp_f32<0x4E>(v);
+    v += dpp_f32<0x141>(v);
+    v += dpp_f32<0x140>(v);   // row_mirror: every lane of the 16-lane row holds the sum
     return v;
 }
 
-constexpr int kMaxKT = 16;   // site tiles of 16: configurations of up to 256 sites
-constexpr int kXcds = 8;     // workgroups are dispatched round-robin over the 8 XCDs
-
-// XCD-aware tile order: the workgroups of one "row" (a sample chunk or slice,
-// shared data: its spin fragments and coefficients) all land on the same XCD,
-// so that data is fetched into one XCD's L2 only.  Linear workgroup id L runs on
-// XCD L % 8; rows xcd, xcd + 8, ... are given to that XCD, cols fastest.
-__device__ inline bool xcd_tile(int L, int rows, int cols, int& row, int& col) {
-    const int xcd = L % kXcds, slot = L / kXcds;
-    row = xcd + kXcds * (slot / cols);
-    col = slot % cols;
-    return row < rows;
-}
-inline int xcd_grid(int rows, int cols) { return kXcds * ((rows + kXcds - 1) / kXcds) * cols; }
+constexpr int kMaxVpad = 64 * kMaxWords + 16;            // spin rows incl. the bias row
+constexpr int kBitWords = (kMaxVpad + 63) / 64;
 
 template <int K>
 struct KtTag {
     static constexpr int value = K;
 };
 
+// site tiles of 32: configurations of up to 256 sites
 template <typename F>
 void dispatch_kt(int kt, F&& fn) {
     switch (kt) {
@@ -100,284 +107,266 @@ void dispatch_kt(int kt, F&& fn) {
         case 6: fn(KtTag<6>{}); break;
         case 7: fn(KtTag<7>{}); break;
         case 8: fn(KtTag<8>{}); break;
-        case 9: fn(KtTag<9>{}); break;
-        case 10: fn(KtTag<10>{}); break;
-        case 11: fn(KtTag<11>{}); break;
-        case 12: fn(KtTag<12>{}); break;
-        case 13: fn(KtTag<13>{}); break;
-        case 14: fn(KtTag<14>{}); break;
-        case 15: fn(KtTag<15>{}); break;
-        case 16: fn(KtTag<16>{}); break;
         default: throw std::invalid_argument("configurations of 1..256 sites are supported");
     }
 }
 
+// spin fragments of all 256 bytes in LDS: fragment of byte x = lut[x] (one
+// ds_read_b128 instead of a dozen VALU operations per fragment)
+__device__ inline void fill_spin_lut(dw4_t* lut) {
+    for (int x = threadIdx.x; x < 256; x += blockDim.x) lut[x] = spin_frag8(uint32_t(x));
+}
+
 // ------------------------------------------------------------ u = O.v (part)
-// A workgroup owns one column group (4 tiles of the split layout: tile 2p = real
-// parts of units 16p..16p+15, tile 2p+1 = imaginary parts) and keeps its bf16x3
-// fragments of V in LDS; each wave then walks 16-sample groups, computes their
-// angles of v on matrix cores and contracts them with g.  upart[col group][s].
+// Wave (pair p, slice): the V fragments of the pair (vfrag[((sp KT + kt) NT + 2p + h)
+// 64 + lane], 8 bf16 each: sites 32 kt + 8 (lane/16) + e, unit 16p + lane%16,
+// h = 0 real / 1 imaginary parts) stay in registers; the wave walks the 16-sample
+// groups slice, slice + slices, ...; the next group's configuration words and g
+// are loaded while the current group multiplies (configuration indices clamped
+// to the last sample, g padded to sr_count_pad(count) rows: rows beyond count are
+// computed and never stored).  upart[p][s] = sum over the pair's units of
+// g_sj (theta_v(s)_j + vb_j).
 template <int KT>
-__global__ __launch_bounds__(256) void ov_mfma_kernel(DenseView<float> net, const short4_t* vfrag, int NT,
-                                                      int slices, const short4_t* afrag, int64_t count,
-                                                      const cplx<float>* g, const cplx<double>* vb,
-                                                      cplx<double>* upart) {
-    extern __shared__ __align__(16) unsigned char ov_smem[];
-    short4_t* sv = reinterpret_cast<short4_t*>(ov_smem);   // [3][KT][4][64]
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    int slice, y;
-    if (!xcd_tile(blockIdx.x, slices, NT / 4, slice, y)) return;
-    const int nt0 = 4 * y;
-    const int nfrag = 3 * KT * 4 * 64;
-    // the whole slice is loaded before any of it is stored (the loads overlap)
-    constexpr int kFill = 8;
-    for (int i0 = threadIdx.x; i0 < nfrag; i0 += 256 * kFill) {
-        short4_t tmp[kFill];
-#pragma unroll
-        for (int f = 0; f < kFill; f++) {
-            const int i = i0 + 256 * f;
-            const int l = i & 63, t = (i >> 6) & 3, kt = (i >> 8) % KT, sp = (i >> 8) / KT;
-            tmp[f] = i < nfrag ? vfrag[((int64_t(sp) * KT + kt) * NT + nt0 + t) * 64 + l] : short4_t{0, 0, 0, 0};
-        }
-#pragma unroll
-        for (int f = 0; f < kFill; f++)
-            if (i0 + 256 * f < nfrag) sv[i0 + 256 * f] = tmp[f];
-    }
+__global__ __launch_bounds__(256) void ov32_kernel(const dw4_t* vfrag, int NT, int npairs, int slices,
+                                                   const uint64_t* confs, int64_t count, int Mpad, int M,
+                                                   const float2* g, const cplx<double>* vb, float2* upart) {
+    constexpr int WORDS = (KT + 1) / 2;
+    __shared__ dw4_t lut[256];
+    fill_spin_lut(lut);
     __syncthreads();
-    cplx<float> bj[2];
-    int jq[2];
+    const int lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * kWavesPerBlock + uniform_int(threadIdx.x >> 6);
+    if (wid >= npairs * slices) return;   // whole waves leave after the only barrier
+    const int p = wid % npairs, slice = wid / npairs;
+    const int q = lane >> 4, col = lane & 15;
+    const int j = 16 * p + col;
+    dw4_t bv[3][KT][2];
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-        jq[q] = (2 * y + q) * 16 + (lane & 15);
-        const cplx<double> b = vb && jq[q] < net.M ? vb[jq[q]] : cplx<double>{0.0, 0.0};
-        bj[q] = {float(b.re), float(b.im)};
+    for (int sp = 0; sp < 3; sp++)
+#pragma unroll
+        for (int kt = 0; kt < KT; kt++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) bv[sp][kt][h] = vfrag[((sp * KT + kt) * NT + 2 * p + h) * 64 + lane];
+    float bre = 0.f, bim = 0.f;
+    if (vb && j < M) {
+        bre = float(vb[j].re);
+        bim = float(vb[j].im);
     }
-    const int kq = 4 * (lane >> 4);
     const int64_t groups = (count + 15) / 16;
-    for (int64_t grp = int64_t(slice) * 4 + wave; grp < groups; grp += int64_t(slices) * 4) {
-        const int64_t row0 = grp * 16;
-        // g of this lane's output elements, loaded ahead of the matrix-core work
-        cplx<float> gv[4][2];
+    const int64_t last = count - 1;
+    uint64_t wn[WORDS];
+    float2 gn[4];
+    auto load = [&](int64_t grp) {
+        const int64_t sa = grp * 16 + col < last ? grp * 16 + col : last;   // sample of this lane's A row
 #pragma unroll
-        for (int r = 0; r < 4; r++)
+        for (int k = 0; k < WORDS; k++) wn[k] = confs[sa * WORDS + k];
+        const float2* gp = g + (grp * 16 + 4 * q) * Mpad + j;
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const int64_t s = row0 + kq + r;
-                gv[r][q] = jq[q] < net.M && s < count ? g[s * net.Mpad + jq[q]] : cplx<float>{0.f, 0.f};
-            }
-        float4_t acc[4];
+        for (int r = 0; r < 4; r++) gn[r] = gp[int64_t(r) * Mpad];
+    };
+    int64_t grp = slice;
+    if (grp < groups) load(grp);
+    for (; grp < groups; grp += slices) {
+        uint64_t w[WORDS];
+        float2 gv[4];
 #pragma unroll
-        for (int t = 0; t < 4; t++) acc[t] = {0.f, 0.f, 0.f, 0.f};
-        // all spin fragments of the group are loaded before the first MFMA
-        const short4_t* ag = afrag + grp * KT * 64 + lane;
-        short4_t a[KT];
+        for (int k = 0; k < WORDS; k++) w[k] = wn[k];
 #pragma unroll
-        for (int kt = 0; kt < KT; kt++) a[kt] = ag[kt * 64];
+        for (int r = 0; r < 4; r++) gv[r] = gn[r];
+        if (grp + slices < groups) load(grp + slices);
+        float4_t are = {0.f, 0.f, 0.f, 0.f}, aim = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < KT; kt++) {
+            const int o = 32 * kt + 8 * q;
+            const dw4_t a = lut[uint32_t(w[kt >> 1] >> (o & 63)) & 0xFFu];
 #pragma unroll
-            for (int sp = 0; sp < 3; sp++)
-#pragma unroll
-                for (int t = 0; t < 4; t++)
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kt], sv[((sp * KT + kt) * 4 + t) * 64 + lane],
-                                                                       acc[t], 0, 0, 0);
+            for (int sp = 0; sp < 3; sp++) {
+                are = mfma32(a, bv[sp][kt][0], are);
+                aim = mfma32(a, bv[sp][kt][1], aim);
+            }
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int64_t s = row0 + kq + r;
-            float ur = 0.f, ui = 0.f;
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const float tr = acc[2 * q][r] + bj[q].re, ti = acc[2 * q + 1][r] + bj[q].im;
-                ur += gv[r][q].re * tr - gv[r][q].im * ti;
-                ui += gv[r][q].re * ti + gv[r][q].im * tr;
-            }
-            // row_mirror leaves the full 16-lane row sum in every lane of the row
-            const double sr = row16_sum(double(ur)), si = row16_sum(double(ui));
-            if ((lane & 15) == 15 && s < count) upart[int64_t(y) * count + s] = {sr, si};
+            const float tr = are[r] + bre, ti = aim[r] + bim;
+            const float ur = row16_sum_f(gv[r].x * tr - gv[r].y * ti);
+            const float ui = row16_sum_f(gv[r].x * ti + gv[r].y * tr);
+            const int64_t s = grp * 16 + 4 * q + r;
+            if (col == 15 && s < count) upart[int64_t(p) * count + s] = float2{ur, ui};
         }
     }
 }
 
-// spin fragments of the sample configurations, computed once per TDVP
-// evaluation and reused by every matvec:
-//   ov : [group][kt][lane]   rows = samples 16 group + l%16, k = sites 16 kt + 4 (l/16) + e
-//   sg : [group][vt][lane]   rows = sites 16 vt + l%16 (site N = the constant-1 row),
-//                            k = samples 16 group + 4 (l/16) + e
-__global__ void spin_frag_kernel(const uint64_t* confs, int words, int64_t count, int N, int KT, int VT,
-                                 short4_t* ov, short4_t* sg) {
-    const int64_t e_all = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t groups = (count + 15) / 16;
-    const int per = (KT + VT) * 64;
-    if (e_all >= groups * per) return;
-    const int64_t grp = e_all / per;
-    const int rem = int(e_all % per);
-    const int l = rem & 63;
-    const int kq = 4 * (l >> 4);
-    if (rem < KT * 64) {
-        const int kt = rem >> 6;
-        const int64_t s = grp * 16 + (l & 15);
-        uint64_t w[kMaxWords];
+// transposed spin bits: bit b of bits[blk][v] = spin v of sample 64 blk + b is up
+// (v < N); row N is all ones (the bias row), rows above it zero; samples beyond
+// count have zero bits.  One wave per block of 64 samples, one ballot per row.
+__global__ __launch_bounds__(256) void spin_bits_kernel(const uint64_t* confs, int words, int64_t count, int N,
+                                                        int vpad, uint64_t* bits) {
+    const int lane = threadIdx.x & 63;
+    const int64_t blk = int64_t(blockIdx.x) * kWavesPerBlock + uniform_int(threadIdx.x >> 6);
+    if (blk >= (count + 63) / 64) return;
+    const int64_t s = blk * 64 + lane;
+    const bool ok = s < count;
+    uint64_t w[kMaxWords];
 #pragma unroll
-        for (int k = 0; k < kMaxWords; k++) w[k] = s < count && k < words ? confs[s * words + k] : 0;
-        short4_t a;
+    for (int k = 0; k < kMaxWords; k++) w[k] = ok && k < words ? confs[s * words + k] : 0;
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int v = kt * 16 + kq + e;
-            a[e] = s < count && v < N ? spin_bf16(w, v, N) : short(0);
+    for (int k = 0; k < kBitWords; k++) {
+        const uint64_t word = k < kMaxWords ? w[k < kMaxWords ? k : 0] : 0;
+        uint64_t keep = 0;
+        for (int vv = 0; vv < 64; vv++) {
+            const int v = 64 * k + vv;
+            if (v >= vpad) break;
+            const bool up = v < N ? ((word >> vv) & 1) != 0 : v == N;
+            const uint64_t m = __ballot(ok && up);
+            if (vv == lane) keep = m;
         }
-        ov[(grp * KT + kt) * 64 + l] = a;
-    } else {
-        const int vt = (rem >> 6) - KT;
-        const int v = vt * 16 + (l & 15);
-        short4_t a;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int64_t s = grp * 16 + kq + e;
-            uint64_t w[kMaxWords];
-#pragma unroll
-            for (int k = 0; k < kMaxWords; k++) w[k] = s < count && k < words ? confs[s * words + k] : 0;
-            a[e] = s < count ? spin_bf16(w, v, N) : short(0);
-        }
-        sg[(grp * VT + vt) * 64 + l] = a;
+        if (64 * k + lane < vpad) bits[blk * vpad + 64 * k + lane] = keep;
     }
 }
 
 // --------------------------------------------------------------- Y = S^T X
-// X[s][c]: c < 2 Mpad from c_s g_sj (mode 0), c_s conj(g_sj) (mode 1) or
-// Re(c_s) |g_sj|^2 in the real column (mode 2); columns 2 Mpad, 2 Mpad + 1 hold c_s.
-constexpr int kSpinRowTiles = 7;     // rows 112 per wave (N + 1 <= 112 in one pass)
+// X[s][c]: c < 2 Mpad from c_s g_sj (MODE 0), c_s conj(g_sj) (MODE 1) or
+// Re(c_s) |g_sj|^2 in the real column (MODE 2); columns 2 Mpad, 2 Mpad + 1 hold c_s.
+// Output rows v < vpad (spins; row N = constant 1), columns in the interleaved
+// [v][2 j + re/im] layout.
+constexpr int kSpinRowTiles = 7;                // rows 112 per wave (N + 1 <= 112 in one pass)
+constexpr int kSgStep = 32;                     // samples per matrix-core k step
+constexpr int kSgAlign = 2 * kWavesPerBlock * kSgStep;   // chunk granularity (whole 64-sample bit blocks)
 
-// A workgroup (4 waves) owns 64 real columns and a chunk of samples.  Per stage
-// of 64 samples each thread loads 4 consecutive samples x 2 complex columns (one
-// 16-byte load per sample, 16 threads cover a 256-byte row segment), forms
-// c_s g_sj, splits it into bf16x3 and writes whole B-fragment quads (4 samples)
-// to LDS; the stages are double-buffered (the next stage's loads are in flight
-// while the current one multiplies) and each wave runs 7 row tiles x 4 k-steps
-// x 3 splits of MFMA on its own 16-column tile.
-constexpr int kSgStep = 64;
-
-__global__ __launch_bounds__(256) void spin_gemm_bf16_kernel(DenseView<float> net, const short4_t* afrag,
-                                                             const cplx<float>* g, int mode,
-                                                             const cplx<double>* coef, int64_t count, int64_t chunk,
-                                                             int chunks, int cblocks, int vpad, int ncols,
-                                                             float* partial) {
-    __shared__ short4_t sbf[2][3][4][4][64];        // [buf][split][k-step][col tile][lane]
-    __shared__ short4_t saf[2][4][kSpinRowTiles][64];  // [buf][k-step][row tile][lane] spin fragments
+// Workgroup (pair p of npairs = Mpad/16 + 1, sample chunk) for the VTL row tiles
+// starting at vt0: the 4 waves take the chunk's 32-sample steps round robin; per
+// step a lane loads g of its 8 samples (k = 8(l/16)+e) at unit 16p + l%16 and
+// their coefficients (g and coef have sr_count_pad(count) rows, those beyond
+// count zero, so no load needs a guard), forms the two B fragments (real /
+// imaginary column tile) as
+// bf16x3, and takes the spin A fragment of each row tile from one byte of the
+// transposed bits.  The next step's loads are issued before the current step
+// multiplies.  Pair Mpad/16 holds the coefficient columns (only when with_coef);
+// pairs beyond the hidden units write zeros.
+template <int MODE, int VTL>
+__global__ __launch_bounds__(256, 2) void spin_gemm32_kernel(const uint64_t* bits, int vpad, int vt0,
+                                                             const float2* g, int Mpad, int M, const float2* coef,
+                                                             int64_t count, int64_t chunk, int npairs,
+                                                             int with_coef, int ncols, float* partial) {
+    __shared__ float4_t red[kWavesPerBlock][VTL][2][64];
+    __shared__ dw4_t lut[256];
+    fill_spin_lut(lut);
+    __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    int chunk_id, cb;
-    if (!xcd_tile(blockIdx.x, chunks, cblocks, chunk_id, cb)) return;
-    const int c0 = cb * 64;
-    const int vt0 = blockIdx.y * kSpinRowTiles;
-    const int vtiles = vpad / 16;
+    const int wave = uniform_int(threadIdx.x >> 6);
+    const int q = lane >> 4, col = lane & 15;
+    const int p = blockIdx.x % npairs;
+    const int chunk_id = blockIdx.x / npairs;
+    const bool coef_pair = 16 * p >= Mpad;
+    const bool active = coef_pair ? with_coef != 0 : 16 * p < M;
     const int64_t s0 = int64_t(chunk_id) * chunk;
     const int64_t s1 = s0 + chunk < count ? s0 + chunk : count;
-    const int two_m = 2 * net.Mpad;
-    // loader role: samples 4 sg .. 4 sg + 3 of the stage, complex columns jb, jb + 1
-    const int sg = threadIdx.x >> 4;
-    const int jp = (threadIdx.x & 15) * 2;
-    const int jb = c0 / 2 + jp;
-    float4 rg[4];
-    cplx<float> rc[4];
-    constexpr int kAfragPerThread = (4 * kSpinRowTiles * 64 + 255) / 256;
-    short4_t ra[kAfragPerThread];
-    const int vtl = vtiles - vt0 < kSpinRowTiles ? vtiles - vt0 : kSpinRowTiles;   // row tiles of this z group
-    auto load = [&](int64_t st) {
+    const int j = 16 * p + col;
+    float4_t acc[VTL][2];
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int64_t s = st + 4 * sg + e;
-            const bool ok = s < s1;
-            rg[e] = ok && jb < net.Mpad ? *reinterpret_cast<const float4*>(g + s * net.Mpad + jb)
-                                        : float4{0.f, 0.f, 0.f, 0.f};
-            const cplx<double> c = ok ? coef[s] : cplx<double>{0.0, 0.0};
-            rc[e] = {float(c.re), float(c.im)};
-        }
-        // spin fragments of the stage: 4 k-steps x kSpinRowTiles tiles x 64 lanes, 4 per thread
+    for (int t = 0; t < VTL; t++) acc[t][0] = acc[t][1] = float4_t{0.f, 0.f, 0.f, 0.f};
+    const int nsteps = active && s1 > s0 ? int((s1 - s0 + kSgStep - 1) / kSgStep) : 0;
+    const uint64_t* brow = bits + (vt0 * 16 + col);
+    auto run = [&](auto coef_tag) {
+        constexpr bool COEF = decltype(coef_tag)::value;
+        float2 gn[8], cn[8];
+        uint64_t bn[VTL];
+        auto load = [&](int step) {
+            const int64_t st = s0 + int64_t(kSgStep) * step;
+            const float2* gp = g + (st + 8 * q) * Mpad + j;
+            const float2* cp = coef + st + 8 * q;
 #pragma unroll
-        for (int q = 0; q < kAfragPerThread; q++) {
-            const int i = threadIdx.x + 256 * q;           // [ks][t][lane]
-            const int ks = i / (kSpinRowTiles * 64), t = (i / 64) % kSpinRowTiles, l = i & 63;
-            const int64_t grp = (st >> 4) + ks;
-            ra[q] = i < 4 * kSpinRowTiles * 64 && t < vtl && grp * 16 < s1
-                        ? afrag[(grp * (vtiles) + vt0 + t) * 64 + l] : short4_t{0, 0, 0, 0};
-        }
-    };
-    auto store = [&](int buf) {
-        const int ks = sg >> 2, kl = (sg & 3) * 16;   // k-step of the quad, lane group of its 4 samples
-#pragma unroll
-        for (int i = 0; i < 4; i++) {                 // real columns 2 jp + i of the block
-            const int col = 2 * jp + i;
-            const int c = c0 + col;
-            short4_t h[3];
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const float gr = (i >> 1) ? rg[e].z : rg[e].x;
-                const float gi0 = (i >> 1) ? rg[e].w : rg[e].y;
-                const float gi = mode == 1 ? -gi0 : gi0;
-                const float cr = rc[e].re, ci = rc[e].im;
-                float x;
-                if (c < two_m)
-                    x = mode == 2 ? ((i & 1) ? 0.f : cr * (gr * gr + gi * gi))
-                                  : ((i & 1) ? cr * gi + ci * gr : cr * gr - ci * gi);
-                else
-                    x = c == two_m ? cr : c == two_m + 1 ? ci : 0.f;
-                short h0, h1, h2;
-                split3(x, h0, h1, h2);
-                h[0][e] = h0;
-                h[1][e] = h1;
-                h[2][e] = h2;
+            for (int e = 0; e < 8; e++) {
+                if constexpr (!COEF) gn[e] = gp[int64_t(e) * Mpad];
+                cn[e] = cp[e];
             }
-            const int ct = col >> 4, cc = col & 15;
+            const uint64_t* bb = brow + (st >> 6) * vpad;
 #pragma unroll
-            for (int sp = 0; sp < 3; sp++) sbf[buf][sp][ks][ct][kl + cc] = h[sp];
-        }
+            for (int t = 0; t < VTL; t++) bn[t] = bb[t * 16];
+        };
+        int step = wave;
+        if (step < nsteps) load(step);
+        for (; step < nsteps; step += kWavesPerBlock) {
+            float2 gv[8], cv[8];
+            uint64_t bw[VTL];
 #pragma unroll
-        for (int q = 0; q < kAfragPerThread; q++) {
-            const int i = threadIdx.x + 256 * q;
-            if (i < 4 * kSpinRowTiles * 64)
-                saf[buf][i / (kSpinRowTiles * 64)][(i / 64) % kSpinRowTiles][i & 63] = ra[q];
-        }
-    };
-    float4_t acc[kSpinRowTiles];
+            for (int e = 0; e < 8; e++) {
+                if constexpr (!COEF) gv[e] = gn[e];
+                cv[e] = cn[e];
+            }
 #pragma unroll
-    for (int t = 0; t < kSpinRowTiles; t++) acc[t] = {0.f, 0.f, 0.f, 0.f};
-    const int kq = 4 * (lane >> 4);
-    (void)net;
-    int buf = 0;
-    load(s0);
-    store(0);
-    __syncthreads();
-    for (int64_t st = s0; st < s1; st += kSgStep) {
-        const bool more = st + kSgStep < s1;
-        if (more) load(st + kSgStep);
-#pragma unroll 1
-        for (int ks = 0; ks < 4; ks++) {
-            short4_t b[3];
+            for (int t = 0; t < VTL; t++) bw[t] = bn[t];
+            const int shift = int(((s0 + int64_t(kSgStep) * step) >> 5) & 1) * 32 + 8 * q;
+            if (step + kWavesPerBlock < nsteps) load(step + kWavesPerBlock);
+            // B fragments [split], real and imaginary column tiles
+            dw4_t bre[3], bim[3];
 #pragma unroll
-            for (int sp = 0; sp < 3; sp++) b[sp] = sbf[buf][sp][ks][wave][lane];
+            for (int k = 0; k < 4; k++) {
+                uint32_t r[2][3], i[2][3];
 #pragma unroll
-            for (int t = 0; t < kSpinRowTiles; t++) {
-                if (t >= vtl) continue;
-                const short4_t a = saf[buf][ks][t][lane];
+                for (int h = 0; h < 2; h++) {
+                    const int e = 2 * k + h;
+                    float xr, xi;
+                    if constexpr (COEF) {
+                        xr = col == 0 ? cv[e].x : 0.f;
+                        xi = col == 0 ? cv[e].y : 0.f;
+                    } else {
+                        const float gr = gv[e].x, gi = MODE == 1 ? -gv[e].y : gv[e].y;
+                        const float cr = cv[e].x, ci = cv[e].y;
+                        if constexpr (MODE == 2) {
+                            xr = cr * (gr * gr + gi * gi);
+                            xi = 0.f;
+                        } else {
+                            xr = cr * gr - ci * gi;
+                            xi = cr * gi + ci * gr;
+                        }
+                    }
+                    split3_bits(xr, r[h][0], r[h][1], r[h][2]);
+                    split3_bits(xi, i[h][0], i[h][1], i[h][2]);
+                }
 #pragma unroll
-                for (int sp = 0; sp < 3; sp++)
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[sp], acc[t], 0, 0, 0);
+                for (int sp = 0; sp < 3; sp++) {
+                    bre[sp][k] = pack_bf16x2(r[0][sp], r[1][sp]);
+                    bim[sp][k] = pack_bf16x2(i[0][sp], i[1][sp]);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < VTL; t++) {
+                const dw4_t a = lut[uint32_t(bw[t] >> shift) & 0xFFu];
+#pragma unroll
+                for (int sp = 0; sp < 3; sp++) {
+                    acc[t][0] = mfma32(a, bre[sp], acc[t][0]);
+                    acc[t][1] = mfma32(a, bim[sp], acc[t][1]);
+                }
             }
         }
-        if (more) store(buf ^ 1);
-        __syncthreads();
-        buf ^= 1;
+    };
+    if (coef_pair)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
+    // the 4 waves' sums of each tile, added in a fixed order; wave w finishes row tiles w, w + 4
+#pragma unroll
+    for (int t = 0; t < VTL; t++) {
+        red[wave][t][0][lane] = acc[t][0];
+        red[wave][t][1][lane] = acc[t][1];
     }
-    const int c = c0 + 16 * wave + (lane & 15);
+    __syncthreads();
+    const int c = 2 * j;
+    if (c >= ncols) return;
 #pragma unroll
-    for (int t = 0; t < kSpinRowTiles; t++) {
-        if (vt0 + t >= vtiles || c >= ncols) continue;
+    for (int t = 0; t < VTL; t++) {
+        if ((t & (kWavesPerBlock - 1)) != wave) continue;
+        float4_t sr = red[0][t][0][lane], si = red[0][t][1][lane];
+#pragma unroll
+        for (int w = 1; w < kWavesPerBlock; w++) {
+            sr += red[w][t][0][lane];
+            si += red[w][t][1][lane];
+        }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int v = (vt0 + t) * 16 + kq + r;
-            partial[(int64_t(chunk_id) * vpad + v) * ncols + c] = acc[t][r];
+            const int v = (vt0 + t) * 16 + 4 * q + r;
+            *reinterpret_cast<float2*>(partial + (int64_t(chunk_id) * vpad + v) * ncols + c) = float2{sr[r], si[r]};
         }
     }
 }
@@ -409,6 +398,15 @@ __global__ void split_y_kernel(const double* Y, int N, int Mpad, int ncols, cplx
     }
 }
 
+// fp32 coefficients, zero for count <= s < pad
+__global__ void coef_f32_kernel(const cplx<double>* coef, int64_t count, int64_t pad, float2* out) {
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s < count)
+        out[s] = float2{float(coef[s].re), float(coef[s].im)};
+    else if (s < pad)
+        out[s] = float2{0.f, 0.f};
+}
+
 // 256-thread block sum, the same fixed order in every block
 __device__ inline double block_sum256(double v, double* sh) {
     v = wave_sum(v);
@@ -427,9 +425,11 @@ __device__ inline double parts_total(const double* partial, int nparts, int stri
 
 constexpr int kDotParts = 64;   // blocks of the fused dot products
 
-// v (parameter order) -> bf16x3 fragments of V (split layout), va, vb, and the
-// partials of <O>.v.  Blocks [0, pack) pack, then [pack, pack + vab) write va /
-// vb, then kDotParts blocks reduce <O>.v.
+// v (parameter order) -> bf16x3 fragments of V (pair layout, 32-site k tiles:
+// frag[(((sp KT + kt) NT + nt) 64 + lane) 8 + el], site 32 kt + 8 (lane/16) + el,
+// unit 16 (nt/2) + lane%16, nt even = real parts), va, vb, and the partials of
+// <O>.v.  Blocks [0, pack) pack, then [pack, pack + vab) write va / vb, then
+// kDotParts blocks reduce <O>.v.
 __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, const int* inv,
                                                           const cplx<double>* v, int KT, int NT, int pack_blocks,
                                                           int vab_blocks, uint16_t* frag, cplx<double>* va,
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, 
     if (b < pack_blocks) {
         const int64_t e = int64_t(b) * 256 + threadIdx.x;
         const int ncol = 2 * net.Mpad;
-        if (e >= int64_t(KT) * 16 * ncol) return;
+        if (e >= int64_t(KT) * 32 * ncol) return;
         const int vis = int(e / ncol), col = int(e % ncol);
         const int nt = col / 16, cc = col % 16;
         const int j = (nt / 2) * 16 + cc;
@@ -466,13 +466,13 @@ __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, 
             }
         }
         float r = float((nt & 1) ? im : re);
-        const int kt = vis / 16, kk = vis % 16;
-        const int lane = (kk / 4) * 16 + cc, el = kk % 4;
+        const int kt = vis / 32, kk = vis % 32;
+        const int lane = (kk / 8) * 16 + cc, el = kk % 8;
 #pragma unroll
         for (int sp = 0; sp < 3; sp++) {
             const short h = bf16_bits(r);
             r -= bf16_value(h);
-            frag[((((int64_t(sp) * KT + kt) * NT + nt) * 64 + lane) * 4) + el] = uint16_t(h);
+            frag[((((int64_t(sp) * KT + kt) * NT + nt) * 64 + lane) * 8) + el] = uint16_t(h);
         }
         return;
     }
@@ -503,18 +503,20 @@ __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, 
     }
 }
 
-// u_s = w_s (sum_b upart[b][s] + sum_i s_i va_i - <O>.v); four lanes per sample
-// (a quad), each taking every fourth partial and site, summed by quad DPP
+// u_s = w_s (sum_p upart[p][s] + sum_i s_i va_i - <O>.v), as fp32 for the spin GEMM
+// (zero for count <= s < pad); four lanes per sample (a quad), each taking every
+// fourth partial and site, summed by quad DPP
 __device__ inline double quad_sum(double v) {
     v += dpp_f64<0xB1>(v);
     v += dpp_f64<0x4E>(v);
     return v;
 }
 
-__global__ __launch_bounds__(256) void ov_finish_center_kernel(const cplx<double>* upart, int blocks, int64_t count,
+__global__ __launch_bounds__(256) void ov_finish_center_kernel(const float2* upart, int npairs, int64_t count,
+                                                               int64_t pad,
                                                                const uint64_t* confs, int words, int N,
                                                                const cplx<double>* va, const double* opart,
-                                                               const double* w, cplx<double>* u) {
+                                                               const double* w, float2* u) {
     __shared__ double sh[4];
     __shared__ cplx<double> sva[64 * kMaxWords];   // va of up to 256 sites
     const double c_re = parts_total(opart, kDotParts, 2, 0, sh);
@@ -527,9 +529,10 @@ __global__ __launch_bounds__(256) void ov_finish_center_kernel(const cplx<double
     const bool ok = s < count;
     double re = 0.0, im = 0.0;
     if (ok) {
-        for (int b = q; b < blocks; b += 4) {
-            re += upart[int64_t(b) * count + s].re;
-            im += upart[int64_t(b) * count + s].im;
+        for (int b = q; b < npairs; b += 4) {
+            const float2 x = upart[int64_t(b) * count + s];
+            re += double(x.x);
+            im += double(x.y);
         }
         if (va) {
             uint64_t cw[kMaxWords];
@@ -545,12 +548,13 @@ __global__ __launch_bounds__(256) void ov_finish_center_kernel(const cplx<double
     }
     re = quad_sum(re);
     im = quad_sum(im);
-    if (ok && q == 0) u[s] = {w[s] * (re - c_re), w[s] * (im - c_im)};
+    if (ok && q == 0) u[s] = float2{float(w[s] * (re - c_re)), float(w[s] * (im - c_im))};
+    if (!ok && q == 0 && s < pad) u[s] = float2{0.f, 0.f};
 }
 
-// y_k (parameter order) = sum over chunks of the spin-GEMM partials + shift v_k,
-// plus the block partials of v^H y
-__global__ __launch_bounds__(256) void spin_y_kernel(const double* full, int ncols,
+// y_k (parameter order) = sum over sample chunks of the spin-GEMM partials (fp64,
+// chunk order) + shift v_k, plus the block partials of v^H y
+__global__ __launch_bounds__(256) void spin_y_kernel(const float* partial, int chunks, int64_t n, int ncols,
                                                      int layout, int N, int M, int Mpad, const uint32_t* conn,
                                                      const cplx<double>* v, double shift, int P,
                                                      cplx<double>* y, double* ypart) {
@@ -573,7 +577,12 @@ __global__ __launch_bounds__(256) void spin_y_kernel(const double* full, int nco
             col = 2 * (q % M);
         }
         const int64_t off = int64_t(row) * ncols + col;
-        double yr = full[off], yi = full[off + 1];
+        double yr = 0.0, yi = 0.0;
+        for (int c = 0; c < chunks; c++) {
+            const float2 x = *reinterpret_cast<const float2*>(partial + c * n + off);
+            yr += double(x.x);
+            yi += double(x.y);
+        }
         const cplx<double> vk = v[k];
         yr += shift * vk.re;
         yi += shift * vk.im;
@@ -593,88 +602,127 @@ __global__ __launch_bounds__(256) void spin_y_kernel(const double* full, int nco
 
 int sr_y_parts(int P) { return std::min(256, std::max(1, (P + 255) / 256)); }
 
+int sr_kt32(int N) { return (N + 31) / 32; }
+
+int64_t sr_count_pad(int64_t count) { return (count + kSgAlign - 1) / kSgAlign * kSgAlign; }
+
 SpinPlan spin_plan(int N, int Mpad, int64_t count) {
     SpinPlan p;
     p.vpad = (N + 1 + 15) / 16 * 16;
     p.ncols = 2 * Mpad + 16;
-    p.cblocks = (p.ncols + 63) / 64;
+    p.npairs = Mpad / 16 + 1;
     p.zgroups = (p.vpad / 16 + kSpinRowTiles - 1) / kSpinRowTiles;
-    // about 512 workgroups of 4 waves, sample chunks of whole 32-sample stages
-    int64_t chunks = std::max<int64_t>(1, 512 / (int64_t(p.cblocks) * p.zgroups));
-    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, (count + 255) / 256));
+    // about 512 workgroups (two per CU), chunks of whole 256-sample rounds
+    int64_t chunks = std::max<int64_t>(1, 512 / (int64_t(p.npairs) * p.zgroups));
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, (count + kSgAlign - 1) / kSgAlign));
     int64_t chunk = (count + chunks - 1) / chunks;
-    chunk = (chunk + kSgStep - 1) / kSgStep * kSgStep;
-    p.chunk = chunk > 0 ? chunk : kSgStep;
+    chunk = (chunk + kSgAlign - 1) / kSgAlign * kSgAlign;
+    p.chunk = chunk > 0 ? chunk : kSgAlign;
     p.chunks = int((count + p.chunk - 1) / p.chunk);
     if (p.chunks < 1) p.chunks = 1;
     p.n = int64_t(p.vpad) * p.ncols;
+    p.bit_blocks = (count + 63) / 64;
     return p;
 }
 
-void launch_spin_frags(const uint64_t* confs, int words, int64_t count, int N, int npad, int vpad, uint16_t* ov,
-                       uint16_t* sg, hipStream_t stream) {
+void launch_spin_bits(const uint64_t* confs, int words, int64_t count, int N, int vpad, uint64_t* bits,
+                      hipStream_t stream) {
     if (count <= 0) return;
-    const int KT = npad / 16, VT = vpad / 16;
-    const int64_t n = (count + 15) / 16 * (KT + VT) * 64;
-    spin_frag_kernel<<<unsigned((n + 255) / 256), 256, 0, stream>>>(confs, words, count, N, KT, VT,
-                                                                    reinterpret_cast<short4_t*>(ov),
-                                                                    reinterpret_cast<short4_t*>(sg));
+    if (vpad > kMaxVpad || words > kMaxWords) throw std::invalid_argument("too many sites for the spin bit array");
+    spin_bits_kernel<<<grid_waves((count + 63) / 64), kBlock, 0, stream>>>(confs, words, count, N, vpad, bits);
     ANN_LAUNCH_CHECK();
 }
 
-void launch_spin_gemm_bf16(const DenseView<float>& net, const uint16_t* sgfrag, const cplx<float>* g, int mode,
-                           const cplx<double>* coef, int64_t count, const SpinPlan& p, float* partial,
-                           hipStream_t stream) {
+template <int MODE>
+void spin_gemm32_rows(int vtl, dim3 grid, hipStream_t stream, const uint64_t* bits, int vpad, int vt0,
+                      const float2* g, int Mpad, int M, const float2* coef, int64_t count, int64_t chunk, int npairs,
+                      int with_coef, int ncols, float* partial) {
+    auto run = [&](auto kern) {
+        kern<<<grid, 256, 0, stream>>>(bits, vpad, vt0, g, Mpad, M, coef, count, chunk, npairs, with_coef, ncols,
+                                       partial);
+    };
+    switch (vtl) {
+        case 1: run(spin_gemm32_kernel<MODE, 1>); break;
+        case 2: run(spin_gemm32_kernel<MODE, 2>); break;
+        case 3: run(spin_gemm32_kernel<MODE, 3>); break;
+        case 4: run(spin_gemm32_kernel<MODE, 4>); break;
+        case 5: run(spin_gemm32_kernel<MODE, 5>); break;
+        case 6: run(spin_gemm32_kernel<MODE, 6>); break;
+        case 7: run(spin_gemm32_kernel<MODE, 7>); break;
+        default: throw std::invalid_argument("spin GEMM row group of 1..7 tiles");
+    }
+}
+
+void launch_spin_gemm32(const DenseView<float>& net, const uint64_t* bits, const cplx<float>* g, int mode,
+                        const cplx<float>* coef, int with_coef, int64_t count, const SpinPlan& p, float* partial,
+                        hipStream_t stream) {
     if (count <= 0) return;
-    if (net.N + 1 > 16 * kSpinRowTiles * p.zgroups) throw std::invalid_argument("spin GEMM plan too small");
-    const dim3 grid{unsigned(xcd_grid(p.chunks, p.cblocks)), unsigned(p.zgroups)};
-    spin_gemm_bf16_kernel<<<grid, 256, 0, stream>>>(net, reinterpret_cast<const short4_t*>(sgfrag), g, mode, coef,
-                                                    count, p.chunk, p.chunks, p.cblocks, p.vpad, p.ncols, partial);
-    ANN_LAUNCH_CHECK();
+    if (net.N + 1 > 16 * kSpinRowTiles * p.zgroups || p.vpad > kMaxVpad)
+        throw std::invalid_argument("spin GEMM plan too small");
+    if (p.chunk % kSgAlign != 0) throw std::invalid_argument("spin GEMM chunks must be whole 256-sample rounds");
+    if (int64_t(p.chunks) * p.chunk < count) throw std::invalid_argument("spin GEMM chunks do not cover the samples");
+    const dim3 grid{unsigned(p.npairs * p.chunks)};
+    const float2* g2 = reinterpret_cast<const float2*>(g);
+    const float2* c2 = reinterpret_cast<const float2*>(coef);
+    const int vtiles = p.vpad / 16;
+    for (int z = 0; z < p.zgroups; z++) {
+        const int vt0 = z * kSpinRowTiles;
+        const int vtl = std::min(kSpinRowTiles, vtiles - vt0);
+        if (mode == 0)
+            spin_gemm32_rows<0>(vtl, grid, stream, bits, p.vpad, vt0, g2, net.Mpad, net.M, c2, count, p.chunk,
+                                p.npairs, with_coef, p.ncols, partial);
+        else if (mode == 1)
+            spin_gemm32_rows<1>(vtl, grid, stream, bits, p.vpad, vt0, g2, net.Mpad, net.M, c2, count, p.chunk,
+                                p.npairs, with_coef, p.ncols, partial);
+        else
+            spin_gemm32_rows<2>(vtl, grid, stream, bits, p.vpad, vt0, g2, net.Mpad, net.M, c2, count, p.chunk,
+                                p.npairs, with_coef, p.ncols, partial);
+        ANN_LAUNCH_CHECK();
+    }
 }
 
 void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const cplx<double>* obar,
                        const double* w, const uint64_t* confs, int words, int64_t count, const cplx<float>* g,
-                       const uint16_t* ovfrag, int npad, SrWorkspace ws, hipStream_t stream) {
+                       SrWorkspace ws, hipStream_t stream) {
     if (count <= 0) return;
-    const int KT = npad / 16, NT = 2 * net.Mpad / 16;
-    const int64_t npack = int64_t(KT) * 16 * 2 * net.Mpad;
+    const int KT = sr_kt32(net.N), NT = 2 * net.Mpad / 16;
+    if (KT > 8 || words != (KT + 1) / 2) throw std::invalid_argument("configurations of 1..256 sites are supported");
+    const int64_t npack = int64_t(KT) * 32 * 2 * net.Mpad;
     const int pack_blocks = int((npack + 255) / 256);
     const int vab_blocks = (net.Mpad + net.N + 255) / 256;
     unpack_pack_kernel<<<unsigned(pack_blocks + vab_blocks + kDotParts), 256, 0, stream>>>(
         net, inv, v, KT, NT, pack_blocks, vab_blocks, ws.frag, ws.va, ws.vb, obar, ws.opart);
     ANN_LAUNCH_CHECK();
     const bool deep = net.layout == LAYOUT_DEEP1;
-    // about 1024 workgroups in all, each column group's V fragments in LDS
+    const int npairs = sr_ov_pairs(net.M);
     const int64_t groups = (count + 15) / 16;
-    int slices = int(std::max<int64_t>(1, std::min<int64_t>((groups + 3) / 4, 1024 / (NT / 4))));
-    slices = (slices + kXcds - 1) / kXcds * kXcds;
-    const size_t lds = sizeof(short4_t) * 3 * KT * 4 * 64;
-    if (lds > 160 * 1024) throw std::invalid_argument("too many sites for the LDS-resident V fragments");
-    const dim3 grid{unsigned(xcd_grid(slices, NT / 4))};
+    // about 2048 waves (two per SIMD)
+    const int slices = int(std::max<int64_t>(1, std::min<int64_t>(groups, (2048 + npairs - 1) / npairs)));
+    const int64_t waves = int64_t(npairs) * slices;
+    float2* upart = reinterpret_cast<float2*>(ws.upart);
     dispatch_kt(KT, [&](auto tag) {
         constexpr int kt = std::decay_t<decltype(tag)>::value;
-        if (lds > 64 * 1024)
-            ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(ov_mfma_kernel<kt>),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-        ov_mfma_kernel<kt><<<grid, 256, lds, stream>>>(net, reinterpret_cast<const short4_t*>(ws.frag), NT,
-                                                       slices, reinterpret_cast<const short4_t*>(ovfrag), count, g,
-                                                       deep ? ws.vb : nullptr, ws.upart);
+        ov32_kernel<kt><<<unsigned((waves + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, stream>>>(
+            reinterpret_cast<const dw4_t*>(ws.frag), NT, npairs, slices, confs, count, net.Mpad, net.M,
+            reinterpret_cast<const float2*>(g), deep ? ws.vb : nullptr, upart);
     });
     ANN_LAUNCH_CHECK();
-    ov_finish_center_kernel<<<unsigned((4 * count + 255) / 256), 256, 0, stream>>>(
-        ws.upart, NT / 4, count, confs, words, net.N, deep ? ws.va : nullptr, ws.opart, w, ws.u);
+    const int64_t pad = sr_count_pad(count);
+    ov_finish_center_kernel<<<unsigned((4 * pad + 255) / 256), 256, 0, stream>>>(
+        upart, npairs, count, pad, confs, words, net.N, deep ? ws.va : nullptr, ws.opart, w,
+        reinterpret_cast<float2*>(ws.u));
     ANN_LAUNCH_CHECK();
 }
 
+int sr_ov_pairs(int M) { return (M + 15) / 16; }
+
+size_t sr_frag_elems(int N, int Mpad) { return size_t(3) * sr_kt32(N) * (2 * Mpad / 16) * 64 * 8; }
+
 void launch_sr_y_bf16(const DenseView<float>& net, const cplx<double>* v, double shift, const SpinPlan& plan,
                       SrWorkspace ws, cplx<double>* y, hipStream_t stream) {
-    // chunk partials added in fp64 (coalesced), then gathered in parameter order
-    sum_partials_f32_kernel<<<unsigned((plan.n + 255) / 256), 256, 0, stream>>>(ws.spin_part, plan.chunks, plan.n,
-                                                                                ws.full);
-    ANN_LAUNCH_CHECK();
     spin_y_kernel<<<unsigned(sr_y_parts(net.num_params)), 256, 0, stream>>>(
-        ws.full, plan.ncols, net.layout, net.N, net.M, net.Mpad, net.conn, v, shift, net.num_params, y, ws.ypart);
+        ws.spin_part, plan.chunks, plan.n, plan.ncols, net.layout, net.N, net.M, net.Mpad, net.conn, v, shift,
+        net.num_params, y, ws.ypart);
     ANN_LAUNCH_CHECK();
 }
 
@@ -685,13 +733,18 @@ void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, i
     const SpinPlan p = spin_plan(net.N, net.Mpad, count);
     float* partial = nullptr;
     double* full = nullptr;
-    uint16_t* sg = nullptr;
-    const size_t nsg = size_t((count + 15) / 16) * (p.vpad / 16) * 64 * 4;
+    uint64_t* bits = nullptr;
+    cplx<float>* c32 = nullptr;
     ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(float) * p.n * p.chunks, stream));
     ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&full), sizeof(double) * p.n, stream));
-    ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&sg), sizeof(uint16_t) * nsg, stream));
-    launch_spin_frags(confs, words, count, net.N, 0, p.vpad, nullptr, sg, stream);
-    launch_spin_gemm_bf16(net, sg, g, mode, coef, count, p, partial, stream);
+    ANN_HIP_CHECK(
+        hipMallocAsync(reinterpret_cast<void**>(&bits), sizeof(uint64_t) * p.bit_blocks * p.vpad, stream));
+    const int64_t pad = sr_count_pad(count);
+    ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&c32), sizeof(cplx<float>) * pad, stream));
+    launch_spin_bits(confs, words, count, net.N, p.vpad, bits, stream);
+    coef_f32_kernel<<<unsigned((pad + 255) / 256), 256, 0, stream>>>(coef, count, pad, reinterpret_cast<float2*>(c32));
+    ANN_LAUNCH_CHECK();
+    launch_spin_gemm32(net, bits, g, mode, c32, 1, count, p, partial, stream);
     sum_partials_f32_kernel<<<unsigned((p.n + 255) / 256), 256, 0, stream>>>(partial, p.chunks, p.n, full);
     ANN_LAUNCH_CHECK();
     const int64_t ns = int64_t(net.N + 1) * net.Mpad + net.N;
@@ -699,7 +752,8 @@ void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, i
     ANN_LAUNCH_CHECK();
     ANN_HIP_CHECK(hipFreeAsync(partial, stream));
     ANN_HIP_CHECK(hipFreeAsync(full, stream));
-    ANN_HIP_CHECK(hipFreeAsync(sg, stream));
+    ANN_HIP_CHECK(hipFreeAsync(bits, stream));
+    ANN_HIP_CHECK(hipFreeAsync(c32, stream));
 }
 
 }  // namespace ann
