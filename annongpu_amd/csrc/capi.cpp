@@ -124,15 +124,22 @@ int ann_profiler_query(const char* name, double* ms, int64_t* launches) {
         need(name, "name");
         ann::Runtime::get().sync();
         auto& t = ann::Runtime::get().totals;
+        auto& u = ann::Runtime::get().untimed;
         auto it = t.find(name);
-        if (ms) *ms = it == t.end() ? 0.0 : it->second.first;
-        if (launches) *launches = it == t.end() ? 0 : it->second.second;
+        auto iu = u.find(name);
+        const double timed_ms = it == t.end() ? 0.0 : it->second.first;
+        const int64_t timed = it == t.end() ? 0 : it->second.second;
+        const int64_t all = timed + (iu == u.end() ? 0 : iu->second);
+        // classes timed on a sample of their launches are scaled to all launches
+        if (ms) *ms = timed > 0 ? timed_ms * double(all) / double(timed) : 0.0;
+        if (launches) *launches = all;
     });
 }
 int ann_profiler_reset(void) {
     return guard([&] {
         ann::Runtime::get().sync();
         ann::Runtime::get().totals.clear();
+        ann::Runtime::get().untimed.clear();
     });
 }
 

@@ -63,13 +63,15 @@ __global__ __launch_bounds__(256) void cg_pap_kernel(const cplx<double>* p, cons
     }
 }
 
-// alpha = rz / pAp; x += alpha p; r -= alpha Ap; z = M^-1 r; new partials (r^H z, r^H r)
+// alpha = rz / pAp; x += alpha p; r -= alpha Ap; z = M^-1 r; new partials (r^H z, r^H r).
+// Nothing changes once done[0] is set (the solve has converged).
 __global__ __launch_bounds__(256) void cg_step_kernel(cplx<double>* x, cplx<double>* r, cplx<double>* z,
                                                       const cplx<double>* p, const cplx<double>* Ap,
                                                       const double* diag, double shift, int64_t n,
                                                       const double* rzpart, int rz_parts, const double* apart,
-                                                      int a_parts, double* newpart) {
+                                                      int a_parts, double* newpart, const int* done) {
     __shared__ double sh[4];
+    if (done[0]) return;
     const double rz = total(rzpart, rz_parts, 4, 0, sh);
     const double pap = total(apart, a_parts, 2, 0, sh);
     const double alpha = pap > 0.0 ? rz / pap : 0.0;
@@ -94,10 +96,12 @@ __global__ __launch_bounds__(256) void cg_step_kernel(cplx<double>* x, cplx<doub
     }
 }
 
-// p = z + (rz_new / rz_old) p
+// p = z + (rz_new / rz_old) p (nothing once done[0] is set)
 __global__ __launch_bounds__(256) void cg_dir_kernel(cplx<double>* p, const cplx<double>* z, int64_t n,
-                                                     const double* newpart, const double* oldpart, int parts) {
+                                                     const double* newpart, const double* oldpart, int parts,
+                                                     const int* done) {
     __shared__ double sh[4];
+    if (done[0]) return;
     const double rn = total(newpart, parts, 4, 0, sh);
     const double ro = total(oldpart, parts, 4, 0, sh);
     const double beta = ro != 0.0 ? rn / ro : 0.0;
@@ -107,7 +111,25 @@ __global__ __launch_bounds__(256) void cg_dir_kernel(cplx<double>* p, const cplx
     }
 }
 
+// convergence test of iteration `iter` on the device: r^H r from its partials;
+// the first iteration with r^H r <= tol2 sets done = {1, iter}
+__global__ __launch_bounds__(256) void cg_check_kernel(const double* part, int parts, double tol2, int iter,
+                                                       int* done) {
+    __shared__ double sh[4];
+    if (done[0]) return;
+    const double rr = total(part, parts, 4, 2, sh);
+    if (threadIdx.x == 0 && rr <= tol2) {
+        done[0] = 1;
+        done[1] = iter;
+    }
+}
+
 }  // namespace
+
+void launch_pcg_check(const double* part, double tol2, int iter, int* done, hipStream_t stream) {
+    cg_check_kernel<<<1, 256, 0, stream>>>(part, kCgParts, tol2, iter, done);
+    ANN_LAUNCH_CHECK();
+}
 
 void launch_pcg_init(const cplx<double>* r, cplx<double>* z, cplx<double>* p, const double* diag, double shift,
                     int64_t n, double* part, hipStream_t stream) {
@@ -122,16 +144,16 @@ void launch_pcg_pap(const cplx<double>* p, const cplx<double>* Ap, int64_t n, do
 
 void launch_pcg_step(cplx<double>* x, cplx<double>* r, cplx<double>* z, const cplx<double>* p,
                     const cplx<double>* Ap, const double* diag, double shift, int64_t n, const double* rzpart,
-                    const double* apart, int a_parts, double* newpart, hipStream_t stream) {
+                    const double* apart, int a_parts, double* newpart, const int* done, hipStream_t stream) {
     if (a_parts > 256) throw std::invalid_argument("too many partial sums");
     cg_step_kernel<<<kCgParts, 256, 0, stream>>>(x, r, z, p, Ap, diag, shift, n, rzpart, kCgParts, apart, a_parts,
-                                                 newpart);
+                                                 newpart, done);
     ANN_LAUNCH_CHECK();
 }
 
 void launch_pcg_dir(cplx<double>* p, const cplx<double>* z, int64_t n, const double* newpart, const double* oldpart,
-                   hipStream_t stream) {
-    cg_dir_kernel<<<kCgParts, 256, 0, stream>>>(p, z, n, newpart, oldpart, kCgParts);
+                   const int* done, hipStream_t stream) {
+    cg_dir_kernel<<<kCgParts, 256, 0, stream>>>(p, z, n, newpart, oldpart, kCgParts, done);
     ANN_LAUNCH_CHECK();
 }
 

@@ -36,11 +36,14 @@ constexpr int kCgParts = 64;
 void launch_pcg_init(const cplx<double>* r, cplx<double>* z, cplx<double>* p, const double* diag, double shift,
                      int64_t n, double* part, hipStream_t stream);
 void launch_pcg_pap(const cplx<double>* p, const cplx<double>* Ap, int64_t n, double* part, hipStream_t stream);
+// done = {converged, iteration}: step and dir leave x, r, z, p untouched once it is set,
+// launch_pcg_check sets it when r^H r of the iteration's partials is <= tol2
 void launch_pcg_step(cplx<double>* x, cplx<double>* r, cplx<double>* z, const cplx<double>* p,
                      const cplx<double>* Ap, const double* diag, double shift, int64_t n, const double* rzpart,
-                     const double* apart, int a_parts, double* newpart, hipStream_t stream);
+                     const double* apart, int a_parts, double* newpart, const int* done, hipStream_t stream);
 void launch_pcg_dir(cplx<double>* p, const cplx<double>* z, int64_t n, const double* newpart, const double* oldpart,
-                    hipStream_t stream);
+                    const int* done, hipStream_t stream);
+void launch_pcg_check(const double* part, double tol2, int iter, int* done, hipStream_t stream);
 void launch_center_scale(cplx<double>* u, const double* w, const double* c2, int64_t n, hipStream_t stream);
 void launch_cg_step(cplx<double>* x, cplx<double>* r, const cplx<double>* p, const cplx<double>* Ap, double alpha,
                     int64_t n, hipStream_t stream);

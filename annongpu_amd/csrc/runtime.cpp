@@ -906,17 +906,20 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y,
             t.bits_valid = true;
         }
         const SrWorkspace ws{t.frag.p, t.va.p, t.vb.p, t.upart.p, t.u32.p, t.spin_part.p, t.opart.p, t.ypart.p};
+        // timing events between the launches stall the queue for microseconds, so only
+        // every 16th product is timed (averages per launch stay exact)
+        const bool timed = t.matvecs++ % 16 == 0;
         {
-            ScopedTimer timer("sr_ov");
+            ScopedTimer timer("sr_ov", timed);
             launch_sr_ov_bf16(net, t.inv.p, v, t.Obar.p, t.w.p, t.confs.p, t.words, t.count, g.p, ws, S());
         }
         {
-            ScopedTimer timer("spin_gemm");
+            ScopedTimer timer("spin_gemm", timed);
             launch_spin_gemm32(net, t.spin_bits.p, g.p, 1, ws.u, t.layout == LAYOUT_DEEP1 ? 1 : 0, t.count, plan,
                                ws.spin_part, S());
         }
         {
-            ScopedTimer timer("sr_y");
+            ScopedTimer timer("sr_y", timed);
             launch_sr_y_bf16(net, v, shift, plan, ws, y, S());
         }
         if (ypart) *ypart = t.ypart.p;
@@ -1107,22 +1110,41 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     }
     const double* diag = t.diag.p != nullptr && method == 1 ? t.diag.p : nullptr;
     launch_pcg_init(r.p, z.p, p.p, diag, shift, P, part.p, S());
-    std::vector<double> hp(stride);
-    auto read_rr = [&](int k) {
-        ANN_HIP_CHECK(hipMemcpyAsync(hp.data(), part.p + size_t(k) * stride, sizeof(double) * stride,
-                                     hipMemcpyDeviceToHost, S()));
-        Runtime::get().sync();
-        double h = 0.0;
-        for (int b = 0; b < kCgParts; b++) h += hp[size_t(b) * 4 + 2];
-        return h;
+    // ||F|| and the initial residual, read once
+    t.rr_host.ensure(stride + 1);
+    ANN_HIP_CHECK(hipMemcpyAsync(t.rr_host.p, part.p, sizeof(double) * stride, hipMemcpyDeviceToHost, S()));
+    ANN_HIP_CHECK(hipMemcpyAsync(t.rr_host.p + stride, fnorm2.p, sizeof(double), hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    auto rr_of = [&](const double* h) {
+        double v = 0.0;
+        for (int b = 0; b < kCgParts; b++) v += h[size_t(b) * 4 + 2];
+        return v;
     };
-    double fn2 = 0.0;
-    ANN_HIP_CHECK(hipMemcpyAsync(&fn2, fnorm2.p, sizeof(double), hipMemcpyDeviceToHost, S()));
-    double rr = read_rr(0);
-    const double bnorm = std::sqrt(fn2);
+    double rr = rr_of(t.rr_host.p);
+    const double bnorm = std::sqrt(t.rr_host.p[stride]);
+    const double tol2 = (rtol * bnorm) * (rtol * bnorm);
+    // Convergence is decided on the device every `check` iterations (a flag that
+    // freezes x, r, z, p from then on); the host reads the flag behind an event one
+    // check late, so the stream never drains, and the result is the iterate of the
+    // first converged check.
+    DevBuf<int> done;
+    done.ensure(2);
+    ANN_HIP_CHECK(hipMemsetAsync(done.p, 0, sizeof(int) * 2, S()));
+    t.flag_host.ensure(4);
+    hipEvent_t ev[2];
+    ANN_HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    ANN_HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    struct EventPair {
+        hipEvent_t* e;
+        ~EventPair() {
+            (void)hipEventDestroy(e[0]);
+            (void)hipEventDestroy(e[1]);
+        }
+    } ev_guard{ev};
     const int check = 4;
-    int it = 0;
-    while (it < max_iter && std::sqrt(rr) > rtol * bnorm) {
+    int it = 0, pending = -1;   // slot of the last issued, not yet read flag copy
+    bool stop = rr <= tol2;
+    while (!stop && it < max_iter) {
         const double* cur = part.p + size_t(it) * stride;
         double* nxt = part.p + size_t(it + 1) * stride;
         const double* ypart = nullptr;
@@ -1133,16 +1155,34 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
             ypart = apart.p;
             yparts = kCgParts;
         }
-        launch_pcg_step(x.p, r.p, z.p, p.p, Ap.p, diag, shift, P, cur, ypart, yparts, nxt, S());
-        launch_pcg_dir(p.p, z.p, P, nxt, cur, S());
+        launch_pcg_step(x.p, r.p, z.p, p.p, Ap.p, diag, shift, P, cur, ypart, yparts, nxt, done.p, S());
+        launch_pcg_dir(p.p, z.p, P, nxt, cur, done.p, S());
         it++;
-        if (it % check == 0 || it == max_iter) rr = read_rr(it);
+        if (it % check == 0 || it == max_iter) {
+            launch_pcg_check(part.p + size_t(it) * stride, tol2, it, done.p, S());
+            const int slot = (it / check) & 1;
+            ANN_HIP_CHECK(hipMemcpyAsync(t.flag_host.p + 2 * slot, done.p, sizeof(int) * 2, hipMemcpyDeviceToHost,
+                                         S()));
+            ANN_HIP_CHECK(hipEventRecord(ev[slot], S()));
+            if (pending >= 0) {
+                ANN_HIP_CHECK(hipEventSynchronize(ev[pending]));
+                stop = t.flag_host.p[2 * pending] != 0;
+            }
+            pending = slot;
+        }
     }
-    rr = read_rr(it);
+    Runtime::get().sync();
+    int hflag[2] = {0, 0};
+    ANN_HIP_CHECK(hipMemcpy(hflag, done.p, sizeof(hflag), hipMemcpyDeviceToHost));
+    if (hflag[0]) it = hflag[1];
+    ANN_HIP_CHECK(hipMemcpyAsync(t.rr_host.p, part.p + size_t(it) * stride, sizeof(double) * stride,
+                                 hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    rr = rr_of(t.rr_host.p);
     x.download(reinterpret_cast<cplx<double>*>(x_out), P);
     *iters = it;
     *res = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
-    return std::sqrt(rr) <= rtol * bnorm ? 0 : 2;
+    return hflag[0] != 0 || rr <= tol2 ? 0 : 2;
 }
 
 // ------------------------------------------------------------ learned proposal

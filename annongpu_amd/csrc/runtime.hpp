@@ -70,12 +70,14 @@ public:
     hipStream_t stream();
     void set_device(int device);
     void sync();
-    // per-kernel-class device timers (hipEvents on the library stream)
+    // per-kernel-class device timers (hipEvents on the library stream); a class
+    // may be timed on a sample of its launches (`untimed` counts the others)
     bool profiling = false;
     void begin(const char* name);
     void end();
     void collect();
-    std::map<std::string, std::pair<double, int64_t>> totals;
+    std::map<std::string, std::pair<double, int64_t>> totals;   // timed ms, timed launches
+    std::map<std::string, int64_t> untimed;
 
 private:
     hipStream_t stream_ = nullptr;
@@ -91,11 +93,17 @@ private:
     hipEvent_t event();
 };
 
+// device time of the enclosed launches under `name` (nothing when profiling is
+// off); timed = false only counts the launch (sampled timing of launches in tight
+// loops, where the events themselves would stall the queue)
 struct ScopedTimer {
-    explicit ScopedTimer(const char* name) {
-        if (Runtime::get().profiling) {
+    explicit ScopedTimer(const char* name, bool timed = true) {
+        if (!Runtime::get().profiling) return;
+        if (timed) {
             Runtime::get().begin(name);
             on = true;
+        } else {
+            Runtime::get().untimed[name]++;
         }
     }
     ~ScopedTimer() {
@@ -237,6 +245,27 @@ struct Ensemble {
     uint64_t num_steps() const { return kind == ENS_EXACT ? (uint64_t(1) << num_sites) : num_samples; }
 };
 
+// page-locked host buffer (hipHostMalloc), for asynchronous device-to-host reads
+template <typename T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    void ensure(size_t m) {
+        if (m <= n && p) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        ANN_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (m > 0 ? m : 1), hipHostMallocDefault));
+        n = m;
+    }
+};
+
 struct Tdvp {
     uint32_t P = 0;
     cd E = 0.0;
@@ -264,6 +293,9 @@ struct Tdvp {
     DevBuf<double> opart, ypart; // dot-product partials of the fused matvec
     DevBuf<uint64_t> spin_bits;  // transposed spin bits of the samples (fp32 mode)
     bool bits_valid = false;
+    int64_t matvecs = 0;         // S.v products so far (every 16th is timed when profiling)
+    PinnedBuf<double> rr_host;   // CG residual partials and ||F||^2 (page-locked)
+    PinnedBuf<int> flag_host;    // two slots of the CG convergence flag, read behind events
     DevBuf<int> inv;            // inverse gather table [N][Mpad] (PsiDeep)
     DevBuf<double> dot_scratch;
     // materialised O [count][P] of wave functions without a structured O (PsiCNN)

@@ -50,7 +50,9 @@ int ann_set_device(int device);
  * switch the library's per-kernel hipEvent timers on and off. */
 int ann_profiler_start(void);
 int ann_profiler_stop(void);
-/* accumulated device time of one timed kernel class since ann_profiler_start */
+/* accumulated device time of one timed kernel class since ann_profiler_start, and
+ * its number of launches; classes inside the CG loop are timed on every 16th launch
+ * and the time is scaled to all launches */
 int ann_profiler_query(const char* name, double* milliseconds, int64_t* launches);
 int ann_profiler_reset(void);
 
