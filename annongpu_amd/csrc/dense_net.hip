@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void mc_sweep_kernel(DenseView<T> net, uint
 
 // E_loc(s) = sum_groups coef_g(s) * psi(s')/psi(s)  (Operator.hpp:36-121)
 template <typename T, int U>
-__global__ __launch_bounds__(kBlock) void local_energy_kernel(DenseView<T> net, OpView op,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void local_energy_kernel(DenseView<T> net, OpView op,
                                                               const uint64_t* confs,
                                                               const cplx<T>* theta, int64_t count,
                                                               cplx<double>* e_loc) {
@@ -297,6 +297,23 @@ __global__ __launch_bounds__(kBlock) void local_energy_kernel(DenseView<T> net, 
     }
     const cplx<T> L0 = wave_csum(lane_sum<T, U>(th, f));
     double e_re = 0.0, e_im = 0.0;
+    // off-diagonal terms wait in lane `pend` (log-ratio and coefficient) until 64 are
+    // collected; then every lane evaluates its own exp / sincos once
+    // (kept in T: the ratio is only as accurate as the T-precision angles anyway)
+    T q_dre = 0, q_dim = 0, q_cre = 0, q_cim = 0;
+    double a_re = 0.0, a_im = 0.0;
+    int pend = 0;
+    auto flush = [&]() {
+        if (lane < pend) {
+            const double m = exp(double(q_dre));
+            double sn, cs;
+            sincos(double(q_dim), &sn, &cs);
+            const double r_re = m * cs, r_im = m * sn;
+            a_re += q_cre * r_re - q_cim * r_im;
+            a_im += q_cre * r_im + q_cim * r_re;
+        }
+        pend = 0;
+    };
     for (int g = 0; g < op.num_groups; g++) {
         double c_re = 0.0, c_im = 0.0;
         const int b0 = op.group_begin[g], b1 = op.group_begin[g + 1];
@@ -334,16 +351,18 @@ __global__ __launch_bounds__(kBlock) void local_energy_kernel(DenseView<T> net, 
             }
         }
         const cplx<T> L1 = wave_csum(lane_sum<T, U>(tn, f));
-        const double d_re = double(L1.re) - double(L0.re);
-        const double d_im = double(L1.im) - double(L0.im);
-        const double m = exp(d_re);
-        double sn, cs;
-        sincos(d_im, &sn, &cs);
-        const double r_re = m * cs, r_im = m * sn;
-        e_re += c_re * r_re - c_im * r_im;
-        e_im += c_re * r_im + c_im * r_re;
+        if (lane == pend) {
+            q_dre = T(double(L1.re) - double(L0.re));
+            q_dim = T(double(L1.im) - double(L0.im));
+            q_cre = T(c_re);
+            q_cim = T(c_im);
+        }
+        if (++pend == 64) flush();
     }
-    if (lane == 0) e_loc[s] = {e_re, e_im};
+    flush();
+    a_re = wave_sum(a_re);
+    a_im = wave_sum(a_im);
+    if (lane == 0) e_loc[s] = {e_re + a_re, e_im + a_im};
 }
 
 template <typename T>

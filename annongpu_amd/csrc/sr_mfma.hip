@@ -529,6 +529,7 @@ __global__ __launch_bounds__(256) void ov_finish_center_kernel(const float2* upa
     const bool ok = s < count;
     double re = 0.0, im = 0.0;
     if (ok) {
+#pragma unroll 8
         for (int b = q; b < npairs; b += 4) {
             const float2 x = upart[int64_t(b) * count + s];
             re += double(x.x);
@@ -578,6 +579,7 @@ __global__ __launch_bounds__(256) void spin_y_kernel(const float* partial, int c
         }
         const int64_t off = int64_t(row) * ncols + col;
         double yr = 0.0, yi = 0.0;
+#pragma unroll 8
         for (int c = 0; c < chunks; c++) {
             const float2 x = *reinterpret_cast<const float2*>(partial + c * n + off);
             yr += double(x.x);
