@@ -110,13 +110,14 @@ void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, i
                            cplx<double>* Ya, hipStream_t stream);
 
 // the spin GEMM's partition of the samples: npairs column pairs (16 hidden units
-// each, the last one the coefficient columns) x chunks of `chunk` samples x zgroups
-// row groups; fp32 partials [chunks][n] floats, n = vpad * ncols
+// each, plus the coefficient columns when with_coef) x chunks of `chunk` samples x
+// zgroups row groups; fp32 partials [chunks][n] floats, n = vpad * ncols (columns of
+// the hidden-unit padding are not written)
 struct SpinPlan {
     int vpad = 0, ncols = 0, npairs = 0, zgroups = 0, chunks = 0;
     int64_t chunk = 0, n = 0, bit_blocks = 0;
 };
-SpinPlan spin_plan(int N, int Mpad, int64_t count);
+SpinPlan spin_plan(int N, int M, int Mpad, int64_t count, bool with_coef);
 // transposed spin bits of the configurations: bits[(count+63)/64][vpad], bit b of
 // word [blk][v] = spin v of sample 64 blk + b (row N all ones)
 void launch_spin_bits(const uint64_t* confs, int words, int64_t count, int N, int vpad, uint64_t* bits,
@@ -149,13 +150,25 @@ struct SrWorkspace {
 // ws.u), launch_spin_gemm32 (mode 1, coefficients ws.u, partials into
 // ws.spin_part), launch_sr_y_bf16 (y, and the partials of v^H y in ws.ypart,
 // [sr_y_parts(P)][2])
+// the CG direction update p_new = z + (rz_new / rz_old) p_old, fused into the first
+// kernel of the matvec (z null: plain matvec of v).  rz partials as in cg.hip
+// ([parts][4], element 0); nothing changes once done[0] is set (p_new = p_old).
+struct CgDir {
+    const cplx<double>* z = nullptr;
+    const cplx<double>* p_old = nullptr;
+    const double* newpart = nullptr;
+    const double* oldpart = nullptr;
+    int parts = 0;
+    const int* done = nullptr;
+    cplx<double>* p_new = nullptr;
+};
 int sr_y_parts(int P);
 int sr_kt32(int N);
 int sr_ov_pairs(int M);
 size_t sr_frag_elems(int N, int Mpad);
-void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const cplx<double>* obar,
-                       const double* w, const uint64_t* confs, int words, int64_t count, const cplx<float>* g,
-                       SrWorkspace ws, hipStream_t stream);
+void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const CgDir& dir,
+                       const cplx<double>* obar, const double* w, const uint64_t* confs, int words, int64_t count,
+                       const cplx<float>* g, SrWorkspace ws, hipStream_t stream);
 void launch_sr_y_bf16(const DenseView<float>& net, const cplx<double>* v, double shift, const SpinPlan& plan,
                       SrWorkspace ws, cplx<double>* y, hipStream_t stream);
 

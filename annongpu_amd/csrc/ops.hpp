@@ -32,7 +32,7 @@ void launch_cg_dir_dev(cplx<double>* p, const cplx<double>* z, const double* rz_
                        hipStream_t stream);
 // preconditioned CG (cg.hip): dot products as kCgParts block partials [b][4] = (r^H z, -, r^H r, -);
 // a matvec's partials of p^H Ap are [b][2]
-constexpr int kCgParts = 64;
+constexpr int kCgParts = 256;
 void launch_pcg_init(const cplx<double>* r, cplx<double>* z, cplx<double>* p, const double* diag, double shift,
                      int64_t n, double* part, hipStream_t stream);
 void launch_pcg_pap(const cplx<double>* p, const cplx<double>* Ap, int64_t n, double* part, hipStream_t stream);

@@ -877,7 +877,7 @@ struct Ws<double> {
 // fused bf16 matrix-core pipeline and also leaves the partials of v^H y (*ypart).
 template <typename T>
 void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y, const double** ypart,
-                  int* yparts) {
+                  int* yparts, const CgDir* dir = nullptr) {
     const int P = int(t.P);
     DenseView<T> net{};
     net.N = t.N;
@@ -892,7 +892,8 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y,
     t.vb.ensure(t.Mpad);
     t.u.ensure(t.count);
     if constexpr (sizeof(T) == 4) {
-        const SpinPlan plan = spin_plan(t.N, t.Mpad, t.count);
+        const bool with_coef = t.layout == LAYOUT_DEEP1;
+        const SpinPlan plan = spin_plan(t.N, t.M, t.Mpad, t.count, with_coef);
         t.frag.ensure(sr_frag_elems(t.N, t.Mpad));
         t.upart.ensure(size_t(sr_ov_pairs(t.M)) * t.count);
         t.u32.ensure(size_t(sr_count_pad(t.count)));
@@ -911,12 +912,13 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y,
         const bool timed = t.matvecs++ % 16 == 0;
         {
             ScopedTimer timer("sr_ov", timed);
-            launch_sr_ov_bf16(net, t.inv.p, v, t.Obar.p, t.w.p, t.confs.p, t.words, t.count, g.p, ws, S());
+            launch_sr_ov_bf16(net, t.inv.p, v, dir ? *dir : CgDir{}, t.Obar.p, t.w.p, t.confs.p, t.words, t.count,
+                              g.p, ws, S());
         }
         {
             ScopedTimer timer("spin_gemm", timed);
-            launch_spin_gemm32(net, t.spin_bits.p, g.p, 1, ws.u, t.layout == LAYOUT_DEEP1 ? 1 : 0, t.count, plan,
-                               ws.spin_part, S());
+            launch_spin_gemm32(net, t.spin_bits.p, g.p, 1, ws.u, with_coef ? 1 : 0, t.count, plan, ws.spin_part,
+                               S());
         }
         {
             ScopedTimer timer("sr_y", timed);
@@ -1040,6 +1042,20 @@ void tdvp_S_dot(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y, c
         tdvp_S_dot_t<float>(t, v, shift, y, ypart, yparts);
 }
 
+const cplx<double>* tdvp_S_dot_dir(Tdvp& t, const CgDir& dir, double shift, cplx<double>* y, const double** ypart,
+                                   int* yparts) {
+    if (!t.has_samples) throw std::logic_error("TDVP.eval must run before S_dot_vector");
+    if (!t.generic && t.precision != 0) {
+        // the fp32 structured matvec forms the direction inside its first kernel
+        tdvp_S_dot_t<float>(t, dir.p_new, shift, y, ypart, yparts, &dir);
+        return dir.p_new;
+    }
+    cplx<double>* p = const_cast<cplx<double>*>(dir.p_old);
+    launch_pcg_dir(p, dir.z, int64_t(t.P), dir.newpart, dir.oldpart, dir.done, S());
+    tdvp_S_dot(t, p, shift, y, ypart, yparts);
+    return p;
+}
+
 int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x_out, int* iters, double* res,
                const cd* x0) {
     const int P = int(t.P);
@@ -1084,11 +1100,12 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     // products and the CG scalars stay on the device (fp64) as per-block partial
     // sums; the host reads the residual norm every `check` iterations only.
     if (!t.has_samples) throw std::logic_error("CG solve needs TDVP.eval first");
-    DevBuf<cplx<double>> r, z, p, Ap;
+    DevBuf<cplx<double>> r, z, p, p2, Ap;
     DevBuf<double> part, apart, scratch;
     r.ensure(P);
     z.ensure(P);
     p.ensure(P);
+    p2.ensure(P);
     Ap.ensure(P);
     scratch.ensure(256);
     // partials of (r^H z, -, r^H r, -) per iteration, and of p^H Ap
@@ -1144,19 +1161,38 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     const int check = 4;
     int it = 0, pending = -1;   // slot of the last issued, not yet read flag copy
     bool stop = rr <= tol2;
+    // the direction update of iteration k happens in the first kernel of its matvec
+    // (p_k = z + beta p_{k-1} into the other of two buffers)
+    const cplx<double>* pk = p.p;
+    cplx<double>* spare = p2.p;
     while (!stop && it < max_iter) {
         const double* cur = part.p + size_t(it) * stride;
         double* nxt = part.p + size_t(it + 1) * stride;
         const double* ypart = nullptr;
         int yparts = 0;
-        tdvp_S_dot(t, p.p, shift, Ap.p, &ypart, &yparts);
+        if (it == 0) {
+            tdvp_S_dot(t, pk, shift, Ap.p, &ypart, &yparts);
+        } else {
+            CgDir dir;
+            dir.z = z.p;
+            dir.p_old = pk;
+            dir.newpart = cur;
+            dir.oldpart = cur - stride;
+            dir.parts = kCgParts;
+            dir.done = done.p;
+            dir.p_new = spare;
+            const cplx<double>* pn = tdvp_S_dot_dir(t, dir, shift, Ap.p, &ypart, &yparts);
+            if (pn == spare) {
+                spare = const_cast<cplx<double>*>(pk);
+                pk = pn;
+            }
+        }
         if (!ypart) {
-            launch_pcg_pap(p.p, Ap.p, P, apart.p, S());
+            launch_pcg_pap(pk, Ap.p, P, apart.p, S());
             ypart = apart.p;
             yparts = kCgParts;
         }
-        launch_pcg_step(x.p, r.p, z.p, p.p, Ap.p, diag, shift, P, cur, ypart, yparts, nxt, done.p, S());
-        launch_pcg_dir(p.p, z.p, P, nxt, cur, done.p, S());
+        launch_pcg_step(x.p, r.p, z.p, pk, Ap.p, diag, shift, P, cur, ypart, yparts, nxt, done.p, S());
         it++;
         if (it % check == 0 || it == max_iter) {
             launch_pcg_check(part.p + size_t(it) * stride, tol2, it, done.p, S());

@@ -231,7 +231,8 @@ __global__ __launch_bounds__(256) void spin_bits_kernel(const uint64_t* confs, i
 // [v][2 j + re/im] layout.
 constexpr int kSpinRowTiles = 7;                // rows 112 per wave (N + 1 <= 112 in one pass)
 constexpr int kSgStep = 32;                     // samples per matrix-core k step
-constexpr int kSgAlign = 2 * kWavesPerBlock * kSgStep;   // chunk granularity (whole 64-sample bit blocks)
+constexpr int kSgAlign = kWavesPerBlock * kSgStep;   // chunk granularity: one step per wave
+constexpr int kSgPad = 256;                          // sample padding of g and the coefficients
 
 // Workgroup (pair p of npairs = Mpad/16 + 1, sample chunk) for the VTL row tiles
 // starting at vt0: the 4 waves take the chunk's 32-sample steps round robin; per
@@ -241,13 +242,16 @@ constexpr int kSgAlign = 2 * kWavesPerBlock * kSgStep;   // chunk granularity (w
 // imaginary column tile) as
 // bf16x3, and takes the spin A fragment of each row tile from one byte of the
 // transposed bits.  The next step's loads are issued before the current step
-// multiplies.  Pair Mpad/16 holds the coefficient columns (only when with_coef);
-// pairs beyond the hidden units write zeros.
+// multiplies.  Pair Mpad/16 holds the coefficient columns (launched only when
+// with_coef); pairs beyond the hidden units are not launched, their partial
+// columns are left as they are.
 template <int MODE, int VTL>
 __global__ __launch_bounds__(256, 2) void spin_gemm32_kernel(const uint64_t* bits, int vpad, int vt0,
                                                              const float2* g, int Mpad, int M, const float2* coef,
                                                              int64_t count, int64_t chunk, int npairs,
                                                              int with_coef, int ncols, float* partial) {
+    // launched pairs: the unit pairs 0 .. ceil(M/16)-1, then the coefficient pair
+    // Mpad/16 when with_coef (npairs counts the launched ones)
     __shared__ float4_t red[kWavesPerBlock][VTL][2][64];
     __shared__ dw4_t lut[256];
     fill_spin_lut(lut);
@@ -255,10 +259,12 @@ __global__ __launch_bounds__(256, 2) void spin_gemm32_kernel(const uint64_t* bit
     const int lane = threadIdx.x & 63;
     const int wave = uniform_int(threadIdx.x >> 6);
     const int q = lane >> 4, col = lane & 15;
-    const int p = blockIdx.x % npairs;
+    const int unit_pairs = (M + 15) / 16;
+    const int pl = blockIdx.x % npairs;
+    const int p = pl < unit_pairs ? pl : Mpad / 16;
     const int chunk_id = blockIdx.x / npairs;
-    const bool coef_pair = 16 * p >= Mpad;
-    const bool active = coef_pair ? with_coef != 0 : 16 * p < M;
+    const bool coef_pair = pl >= unit_pairs;
+    const bool active = !coef_pair || with_coef != 0;
     const int64_t s0 = int64_t(chunk_id) * chunk;
     const int64_t s1 = s0 + chunk < count ? s0 + chunk : count;
     const int j = 16 * p + col;
@@ -429,14 +435,31 @@ constexpr int kDotParts = 64;   // blocks of the fused dot products
 // frag[(((sp KT + kt) NT + nt) 64 + lane) 8 + el], site 32 kt + 8 (lane/16) + el,
 // unit 16 (nt/2) + lane%16, nt even = real parts), va, vb, and the partials of
 // <O>.v.  Blocks [0, pack) pack, then [pack, pack + vab) write va / vb, then
-// kDotParts blocks reduce <O>.v.
+// kDotParts blocks reduce <O>.v.  With dir.z set, v is the new CG direction
+// z + beta p_old, formed on the fly by every block (beta from the rz partials)
+// and written to dir.p_new by the <O>.v blocks.
 __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, const int* inv,
-                                                          const cplx<double>* v, int KT, int NT, int pack_blocks,
-                                                          int vab_blocks, uint16_t* frag, cplx<double>* va,
-                                                          cplx<double>* vb, const cplx<double>* obar,
-                                                          double* opart) {
+                                                          const cplx<double>* v, CgDir dir, int KT, int NT,
+                                                          int pack_blocks, int vab_blocks, uint16_t* frag,
+                                                          cplx<double>* va, cplx<double>* vb,
+                                                          const cplx<double>* obar, double* opart) {
     __shared__ double sh[4];
     const int b = blockIdx.x;
+    double beta = 0.0;
+    bool frozen = false;
+    if (dir.z) {
+        const double rn = parts_total(dir.newpart, dir.parts, 4, 0, sh);
+        const double ro = parts_total(dir.oldpart, dir.parts, 4, 0, sh);
+        beta = ro != 0.0 ? rn / ro : 0.0;
+        frozen = dir.done[0] != 0;
+    }
+    auto vk = [&](int64_t k) -> cplx<double> {
+        if (!dir.z) return v[k];
+        const cplx<double> po = dir.p_old[k];
+        if (frozen) return po;
+        const cplx<double> zz = dir.z[k];
+        return {zz.re + beta * po.re, zz.im + beta * po.im};
+    };
     if (b < pack_blocks) {
         const int64_t e = int64_t(b) * 256 + threadIdx.x;
         const int ncol = 2 * net.Mpad;
@@ -447,19 +470,22 @@ __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, 
         double re = 0.0, im = 0.0;
         if (vis < net.N && j < net.M) {
             if (net.layout == LAYOUT_RBM) {
-                re = v[int64_t(vis) * net.M + j].re;
-                im = v[int64_t(vis) * net.M + j].im;
+                const cplx<double> x = vk(int64_t(vis) * net.M + j);
+                re = x.re;
+                im = x.im;
             } else {
                 // inv[vis][j]: the unique i with conn(i, j) = vis, -1 if none, -2 if several
                 const int i0 = inv[int64_t(vis) * net.Mpad + j];
                 if (i0 >= 0) {
-                    re = v[net.N + net.M + int64_t(i0) * net.M + j].re;
-                    im = v[net.N + net.M + int64_t(i0) * net.M + j].im;
+                    const cplx<double> x = vk(net.N + net.M + int64_t(i0) * net.M + j);
+                    re = x.re;
+                    im = x.im;
                 } else if (i0 == -2) {
                     for (int i = 0; i < net.C; i++) {
                         if (int(net.conn[int64_t(i) * net.M + j]) == vis) {
-                            re += v[net.N + net.M + int64_t(i) * net.M + j].re;
-                            im += v[net.N + net.M + int64_t(i) * net.M + j].im;
+                            const cplx<double> x = vk(net.N + net.M + int64_t(i) * net.M + j);
+                            re += x.re;
+                            im += x.im;
                         }
                     }
                 }
@@ -480,18 +506,20 @@ __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, 
         const int e2 = (b - pack_blocks) * 256 + threadIdx.x;
         const bool deep = net.layout == LAYOUT_DEEP1;
         if (e2 < net.Mpad) {
-            vb[e2] = deep && e2 < net.M ? v[net.N + e2] : cplx<double>{0.0, 0.0};
+            vb[e2] = deep && e2 < net.M ? vk(net.N + e2) : cplx<double>{0.0, 0.0};
         } else if (e2 < net.Mpad + net.N) {
             const int i = e2 - net.Mpad;
-            va[i] = deep ? v[i] : cplx<double>{0.0, 0.0};
+            va[i] = deep ? vk(i) : cplx<double>{0.0, 0.0};
         }
         return;
     }
-    // <O>.v (no conjugation), grid-stride over the parameters
+    // <O>.v (no conjugation), grid-stride over the parameters; these blocks also
+    // store the new direction
     const int db = b - pack_blocks - vab_blocks;
     double re = 0.0, im = 0.0;
     for (int64_t k = int64_t(db) * 256 + threadIdx.x; k < net.num_params; k += int64_t(kDotParts) * 256) {
-        const cplx<double> o = obar[k], x = v[k];
+        const cplx<double> o = obar[k], x = vk(k);
+        if (dir.z) dir.p_new[k] = x;
         re += o.re * x.re - o.im * x.im;
         im += o.re * x.im + o.im * x.re;
     }
@@ -606,15 +634,15 @@ int sr_y_parts(int P) { return std::min(256, std::max(1, (P + 255) / 256)); }
 
 int sr_kt32(int N) { return (N + 31) / 32; }
 
-int64_t sr_count_pad(int64_t count) { return (count + kSgAlign - 1) / kSgAlign * kSgAlign; }
+int64_t sr_count_pad(int64_t count) { return (count + kSgPad - 1) / kSgPad * kSgPad; }
 
-SpinPlan spin_plan(int N, int Mpad, int64_t count) {
+SpinPlan spin_plan(int N, int M, int Mpad, int64_t count, bool with_coef) {
     SpinPlan p;
     p.vpad = (N + 1 + 15) / 16 * 16;
     p.ncols = 2 * Mpad + 16;
-    p.npairs = Mpad / 16 + 1;
+    p.npairs = (M + 15) / 16 + (with_coef ? 1 : 0);
     p.zgroups = (p.vpad / 16 + kSpinRowTiles - 1) / kSpinRowTiles;
-    // about 512 workgroups (two per CU), chunks of whole 256-sample rounds
+    // about 512 workgroups (two per CU), chunks of whole 128-sample rounds
     int64_t chunks = std::max<int64_t>(1, 512 / (int64_t(p.npairs) * p.zgroups));
     chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, (count + kSgAlign - 1) / kSgAlign));
     int64_t chunk = (count + chunks - 1) / chunks;
@@ -661,7 +689,8 @@ void launch_spin_gemm32(const DenseView<float>& net, const uint64_t* bits, const
     if (count <= 0) return;
     if (net.N + 1 > 16 * kSpinRowTiles * p.zgroups || p.vpad > kMaxVpad)
         throw std::invalid_argument("spin GEMM plan too small");
-    if (p.chunk % kSgAlign != 0) throw std::invalid_argument("spin GEMM chunks must be whole 256-sample rounds");
+    if (p.chunk % kSgAlign != 0) throw std::invalid_argument("spin GEMM chunks must be whole 128-sample rounds");
+    if (p.npairs != (net.M + 15) / 16 + (with_coef ? 1 : 0)) throw std::invalid_argument("spin GEMM plan mismatch");
     if (int64_t(p.chunks) * p.chunk < count) throw std::invalid_argument("spin GEMM chunks do not cover the samples");
     const dim3 grid{unsigned(p.npairs * p.chunks)};
     const float2* g2 = reinterpret_cast<const float2*>(g);
@@ -683,17 +712,20 @@ void launch_spin_gemm32(const DenseView<float>& net, const uint64_t* bits, const
     }
 }
 
-void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const cplx<double>* obar,
-                       const double* w, const uint64_t* confs, int words, int64_t count, const cplx<float>* g,
-                       SrWorkspace ws, hipStream_t stream) {
+void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const CgDir& dir,
+                       const cplx<double>* obar, const double* w, const uint64_t* confs, int words, int64_t count,
+                       const cplx<float>* g, SrWorkspace ws, hipStream_t stream) {
     if (count <= 0) return;
     const int KT = sr_kt32(net.N), NT = 2 * net.Mpad / 16;
     if (KT > 8 || words != (KT + 1) / 2) throw std::invalid_argument("configurations of 1..256 sites are supported");
+    if (dir.z && (!dir.p_old || !dir.p_new || !dir.newpart || !dir.oldpart || !dir.done || dir.parts < 1 ||
+                  dir.parts > 256))
+        throw std::invalid_argument("incomplete CG direction update");
     const int64_t npack = int64_t(KT) * 32 * 2 * net.Mpad;
     const int pack_blocks = int((npack + 255) / 256);
     const int vab_blocks = (net.Mpad + net.N + 255) / 256;
     unpack_pack_kernel<<<unsigned(pack_blocks + vab_blocks + kDotParts), 256, 0, stream>>>(
-        net, inv, v, KT, NT, pack_blocks, vab_blocks, ws.frag, ws.va, ws.vb, obar, ws.opart);
+        net, inv, v, dir, KT, NT, pack_blocks, vab_blocks, ws.frag, ws.va, ws.vb, obar, ws.opart);
     ANN_LAUNCH_CHECK();
     const bool deep = net.layout == LAYOUT_DEEP1;
     const int npairs = sr_ov_pairs(net.M);
@@ -732,7 +764,7 @@ void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, i
                            int mode, const cplx<double>* coef, int64_t count, cplx<double>* Yw, cplx<double>* Yb,
                            cplx<double>* Ya, hipStream_t stream) {
     if (count <= 0) return;
-    const SpinPlan p = spin_plan(net.N, net.Mpad, count);
+    const SpinPlan p = spin_plan(net.N, net.M, net.Mpad, count, true);
     float* partial = nullptr;
     double* full = nullptr;
     uint64_t* bits = nullptr;
@@ -746,6 +778,8 @@ void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, i
     launch_spin_bits(confs, words, count, net.N, p.vpad, bits, stream);
     coef_f32_kernel<<<unsigned((pad + 255) / 256), 256, 0, stream>>>(coef, count, pad, reinterpret_cast<float2*>(c32));
     ANN_LAUNCH_CHECK();
+    // columns of the hidden-unit padding are not computed: zero
+    ANN_HIP_CHECK(hipMemsetAsync(partial, 0, sizeof(float) * p.n * p.chunks, stream));
     launch_spin_gemm32(net, bits, g, mode, c32, 1, count, p, partial, stream);
     sum_partials_f32_kernel<<<unsigned((p.n + 255) / 256), 256, 0, stream>>>(partial, p.chunks, p.n, full);
     ANN_LAUNCH_CHECK();
