@@ -146,8 +146,8 @@ struct SrWorkspace {
     double* ypart;
 };
 // y = (S + shift) v with S = sum_s w_s conj(O_s) O_s^T - conj(<O>) <O>^T for the
-// structured O, in three launches: launch_sr_ov_bf16 (u_s = w_s (O_s.v - <O>.v) into
-// ws.u), launch_spin_gemm32 (mode 1, coefficients ws.u, partials into
+// structured O: launch_sr_pack, launch_sr_ov32, launch_sr_finish (u_s = w_s (O_s.v -
+// <O>.v) into ws.u), launch_spin_gemm32 (mode 1, coefficients ws.u, partials into
 // ws.spin_part), launch_sr_y_bf16 (y, and the partials of v^H y in ws.ypart,
 // [sr_y_parts(P)][2])
 // the CG direction update p_new = z + (rz_new / rz_old) p_old, fused into the first
@@ -166,9 +166,14 @@ int sr_y_parts(int P);
 int sr_kt32(int N);
 int sr_ov_pairs(int M);
 size_t sr_frag_elems(int N, int Mpad);
-void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const CgDir& dir,
-                       const cplx<double>* obar, const double* w, const uint64_t* confs, int words, int64_t count,
-                       const cplx<float>* g, SrWorkspace ws, hipStream_t stream);
+// the first half in three launches: V fragments (and the fused direction update),
+// O.v per column pair, u_s = w_s (O_s.v - <O>.v)
+void launch_sr_pack(const DenseView<float>& net, const int* inv, const cplx<double>* v, const CgDir& dir,
+                    const cplx<double>* obar, SrWorkspace ws, hipStream_t stream);
+void launch_sr_ov32(const DenseView<float>& net, const uint64_t* confs, int words, int64_t count,
+                    const cplx<float>* g, SrWorkspace ws, hipStream_t stream);
+void launch_sr_finish(const DenseView<float>& net, const double* w, const uint64_t* confs, int words, int64_t count,
+                      SrWorkspace ws, hipStream_t stream);
 void launch_sr_y_bf16(const DenseView<float>& net, const cplx<double>* v, double shift, const SpinPlan& plan,
                       SrWorkspace ws, cplx<double>* y, hipStream_t stream);
 

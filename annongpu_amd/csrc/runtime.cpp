@@ -911,9 +911,16 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y,
         // every 16th product is timed (averages per launch stay exact)
         const bool timed = t.matvecs++ % 16 == 0;
         {
+            ScopedTimer timer("sr_pack", timed);
+            launch_sr_pack(net, t.inv.p, v, dir ? *dir : CgDir{}, t.Obar.p, ws, S());
+        }
+        {
             ScopedTimer timer("sr_ov", timed);
-            launch_sr_ov_bf16(net, t.inv.p, v, dir ? *dir : CgDir{}, t.Obar.p, t.w.p, t.confs.p, t.words, t.count,
-                              g.p, ws, S());
+            launch_sr_ov32(net, t.confs.p, t.words, t.count, g.p, ws, S());
+        }
+        {
+            ScopedTimer timer("sr_finish", timed);
+            launch_sr_finish(net, t.w.p, t.confs.p, t.words, t.count, ws, S());
         }
         {
             ScopedTimer timer("spin_gemm", timed);

@@ -712,12 +712,10 @@ void launch_spin_gemm32(const DenseView<float>& net, const uint64_t* bits, const
     }
 }
 
-void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<double>* v, const CgDir& dir,
-                       const cplx<double>* obar, const double* w, const uint64_t* confs, int words, int64_t count,
-                       const cplx<float>* g, SrWorkspace ws, hipStream_t stream) {
-    if (count <= 0) return;
+void launch_sr_pack(const DenseView<float>& net, const int* inv, const cplx<double>* v, const CgDir& dir,
+                    const cplx<double>* obar, SrWorkspace ws, hipStream_t stream) {
     const int KT = sr_kt32(net.N), NT = 2 * net.Mpad / 16;
-    if (KT > 8 || words != (KT + 1) / 2) throw std::invalid_argument("configurations of 1..256 sites are supported");
+    if (KT > 8) throw std::invalid_argument("configurations of 1..256 sites are supported");
     if (dir.z && (!dir.p_old || !dir.p_new || !dir.newpart || !dir.oldpart || !dir.done || dir.parts < 1 ||
                   dir.parts > 256))
         throw std::invalid_argument("incomplete CG direction update");
@@ -727,24 +725,36 @@ void launch_sr_ov_bf16(const DenseView<float>& net, const int* inv, const cplx<d
     unpack_pack_kernel<<<unsigned(pack_blocks + vab_blocks + kDotParts), 256, 0, stream>>>(
         net, inv, v, dir, KT, NT, pack_blocks, vab_blocks, ws.frag, ws.va, ws.vb, obar, ws.opart);
     ANN_LAUNCH_CHECK();
+}
+
+void launch_sr_ov32(const DenseView<float>& net, const uint64_t* confs, int words, int64_t count,
+                    const cplx<float>* g, SrWorkspace ws, hipStream_t stream) {
+    if (count <= 0) return;
+    const int KT = sr_kt32(net.N), NT = 2 * net.Mpad / 16;
+    if (KT > 8 || words != (KT + 1) / 2) throw std::invalid_argument("configurations of 1..256 sites are supported");
     const bool deep = net.layout == LAYOUT_DEEP1;
     const int npairs = sr_ov_pairs(net.M);
     const int64_t groups = (count + 15) / 16;
     // about 2048 waves (two per SIMD)
     const int slices = int(std::max<int64_t>(1, std::min<int64_t>(groups, (2048 + npairs - 1) / npairs)));
     const int64_t waves = int64_t(npairs) * slices;
-    float2* upart = reinterpret_cast<float2*>(ws.upart);
     dispatch_kt(KT, [&](auto tag) {
         constexpr int kt = std::decay_t<decltype(tag)>::value;
         ov32_kernel<kt><<<unsigned((waves + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, stream>>>(
             reinterpret_cast<const dw4_t*>(ws.frag), NT, npairs, slices, confs, count, net.Mpad, net.M,
-            reinterpret_cast<const float2*>(g), deep ? ws.vb : nullptr, upart);
+            reinterpret_cast<const float2*>(g), deep ? ws.vb : nullptr, reinterpret_cast<float2*>(ws.upart));
     });
     ANN_LAUNCH_CHECK();
+}
+
+void launch_sr_finish(const DenseView<float>& net, const double* w, const uint64_t* confs, int words, int64_t count,
+                      SrWorkspace ws, hipStream_t stream) {
+    if (count <= 0) return;
+    const bool deep = net.layout == LAYOUT_DEEP1;
     const int64_t pad = sr_count_pad(count);
     ov_finish_center_kernel<<<unsigned((4 * pad + 255) / 256), 256, 0, stream>>>(
-        upart, npairs, count, pad, confs, words, net.N, deep ? ws.va : nullptr, ws.opart, w,
-        reinterpret_cast<float2*>(ws.u));
+        reinterpret_cast<const float2*>(ws.upart), sr_ov_pairs(net.M), count, pad, confs, words, net.N,
+        deep ? ws.va : nullptr, ws.opart, w, reinterpret_cast<float2*>(ws.u));
     ANN_LAUNCH_CHECK();
 }
 

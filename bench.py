@@ -114,7 +114,8 @@ def cpu_baseline(A, L, alpha, seed, seconds, sweeps):
     return dict(samples_per_s=chains / dt, seconds=dt, chains=chains, cores=cores)
 
 
-KERNELS = ("mc_sweep", "block_sweep", "theta", "local_energy", "o_k", "spin_sums", "sr_ov", "spin_gemm", "sr_y")
+KERNELS = ("mc_sweep", "block_sweep", "theta", "local_energy", "o_k", "spin_sums", "sr_pack", "sr_ov", "sr_finish",
+           "spin_gemm", "sr_y")
 
 
 class Stepper:
@@ -258,16 +259,22 @@ def main():
 
     # roofline of the headline path's dominant single-kernel launch class (each timer
     # below brackets exactly one kernel launch with hipEvents on the library stream;
-    # spin_gemm is timed on every 16th launch, so that the events do not stall the CG loop)
+    # sr_ov and spin_gemm are timed on every 16th launch, so that the events do not
+    # stall the CG loop)
     def roofline(path, name):
         kt = paths[path]["kernel_ms_total"]
         launches = paths[path]["kernel_launches"][name]
         avg_ms = kt[name] / max(launches, 1)
-        if name == "spin_gemm":
-            # streams g (8 B per hidden unit), the fp32 coefficient and the transposed spin bits
-            # (8 B per 64 samples and row, N + 1 rows padded to 16) of every sample once
-            vpad = (N + 1 + 15) // 16 * 16
-            work = B * (8.0 * M + 8.0 + 8.0 * vpad / 64)
+        if name in ("spin_gemm", "sr_ov"):
+            if name == "spin_gemm":
+                # streams g (8 B per hidden unit), the fp32 coefficient and the transposed spin bits
+                # (8 B per 64 samples and row, N + 1 rows padded to 16) of every sample once
+                vpad = (N + 1 + 15) // 16 * 16
+                work = B * (8.0 * M + 8.0 + 8.0 * vpad / 64)
+            else:
+                # streams g once, reads each configuration (8 B per word) and writes one fp32 complex
+                # partial per column pair of 16 hidden units
+                work = B * (8.0 * M + 8.0 * math.ceil(N / 64) + 8.0 * math.ceil(M / 16))
             r = {"bound": "hbm", "peak": 8000.0, "unit": "GB/s", "engine": "HBM3E"}
             achieved = work / (avg_ms * 1e-3) / 1e9
         else:
@@ -283,10 +290,25 @@ def main():
             r = {"bound": "mfma", "peak": 157.3, "unit": "TFLOP/s",
                  "engine": "fp32 VALU (MI355X vector fp32 peak 157.3 TFLOP/s = fp32 matrix peak)"}
             achieved = work / (avg_ms * 1e-3) / 1e12
-        return dict(r, achieved=achieved, frac=achieved / r["peak"], traffic=None, kernel=name,
-                    avg_launch_ms=avg_ms, launches=launches)
+        traffic = measured_traffic(name) if r["bound"] == "hbm" else None
+        return dict(r, achieved=achieved, frac=achieved / r["peak"], traffic=traffic, kernel=name,
+                    avg_launch_ms=avg_ms, launches=launches, algorithmic_bytes=work)
 
-    candidates = ("block_sweep", "spin_gemm", "local_energy", "mc_sweep")
+    def measured_traffic(name):
+        """HBM bytes per launch of the kernel behind timer `name` on this workload, from the
+        committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when not measured)"""
+        if (args.L, args.alpha, B) != (10, 4, 16384):
+            return None
+        path = os.path.join(ROOT, "profiles", "sr_matvec_heisenberg10x10_hbm_traffic.md")
+        if not os.path.exists(path):
+            return None
+        for line in open(path):
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cols) >= 9 and cols[1] == name:
+                return float(cols[7]) * 1e6
+        return None
+
+    candidates = ("block_sweep", "sr_ov", "spin_gemm", "local_energy", "mc_sweep")
     dom = max(candidates, key=lambda k: head["kernel_ms_total"][k])
     roof = roofline("learned", dom)
     roof_plain = roofline("plain-config", "mc_sweep")
