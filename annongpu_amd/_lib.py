@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libannongpu.so")
 ANN_OK, ANN_E_INVALID, ANN_E_DEVICE, ANN_E_UNSUPPORTED, ANN_E_NUMERIC = 0, 1, 2, 3, 4
 FP64, FP32 = 0, 1
 MOVE_FLIP, MOVE_EXCHANGE = 0, 1
-SOLVE_CHOLESKY, SOLVE_CG, SOLVE_CG_WARM = 0, 1, 2
+SOLVE_CHOLESKY, SOLVE_CG, SOLVE_CG_WARM, SOLVE_CG_KFAC, SOLVE_CG_KFAC_WARM = 0, 1, 2, 3, 4
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(

@@ -841,15 +841,17 @@ class TDVP:
         return out
 
     def solve(self, shift=1e-3, method="cholesky", rtol=1e-6, max_iter=500, raise_on_failure=True, x0=None):
-        """(S + shift) x = F in fp64 by Cholesky of the dense S or by Jacobi-preconditioned
-        CG on the matrix-free S; ``x0`` starts CG from a previous solution"""
-        m = {"cholesky": _lib.SOLVE_CHOLESKY, "cg": _lib.SOLVE_CG}[method]
+        """(S + shift) x = F in fp64 by Cholesky of the dense S ("cholesky"), or by CG on
+        the matrix-free S with a Jacobi ("cg") or Kronecker-factored ("cg_kfac": spin x
+        hidden-unit covariance factors, dense nets of up to 128 sites) preconditioner;
+        ``x0`` starts CG from a previous solution"""
+        m = {"cholesky": _lib.SOLVE_CHOLESKY, "cg": _lib.SOLVE_CG, "cg_kfac": _lib.SOLVE_CG_KFAC}[method]
         x = np.zeros(self.num_params, dtype=np.complex128)
         if x0 is not None:
-            if method != "cg":
+            if method == "cholesky":
                 raise ValueError("a starting vector applies to CG only")
             x[:] = np.asarray(x0, dtype=np.complex128).ravel()
-            m = _lib.SOLVE_CG_WARM
+            m = _lib.SOLVE_CG_WARM if method == "cg" else _lib.SOLVE_CG_KFAC_WARM
         it, res = ctypes.c_int32(), ctypes.c_double()
         rc = lib.ann_tdvp_solve(self._h, float(shift), m, float(rtol), int(max_iter), ptr(x), ref(it), ref(res))
         self.last_solve = dict(iterations=it.value, residual=res.value)

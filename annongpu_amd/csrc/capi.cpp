@@ -706,11 +706,13 @@ int ann_tdvp_solve(ann_tdvp tdvp, double shift, int32_t method, double rtol, int
         need(x, "x");
         int it = 0;
         double res = 0.0;
-        if (method < ANN_SOLVE_CHOLESKY || method > ANN_SOLVE_CG_WARM) throw std::invalid_argument("unknown method");
+        if (method < ANN_SOLVE_CHOLESKY || method > ANN_SOLVE_CG_KFAC_WARM) throw std::invalid_argument("unknown method");
         std::vector<cd> x0;
-        if (method == ANN_SOLVE_CG_WARM) x0.assign(C(x), C(x) + tdvp->impl.P);
-        const int r = ann::tdvp_solve(tdvp->impl, shift, method == ANN_SOLVE_CG_WARM ? 1 : method, rtol, max_iter,
-                                      C(x), &it, &res, x0.empty() ? nullptr : x0.data());
+        const bool warm = method == ANN_SOLVE_CG_WARM || method == ANN_SOLVE_CG_KFAC_WARM;
+        if (warm) x0.assign(C(x), C(x) + tdvp->impl.P);
+        const int inner = method == ANN_SOLVE_CG_WARM ? 1 : method == ANN_SOLVE_CG_KFAC_WARM ? 3 : method;
+        const int r = ann::tdvp_solve(tdvp->impl, shift, inner, rtol, max_iter, C(x), &it, &res,
+                                      x0.empty() ? nullptr : x0.data());
         if (iterations) *iterations = it;
         if (residual) *residual = res;
         if (r == 1) {

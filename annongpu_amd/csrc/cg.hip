@@ -28,23 +28,33 @@ __device__ inline cplx<double> precond(const cplx<double> r, const double* diag,
     return {r.re / d, r.im / d};
 }
 
-// z = M^-1 r, p = z, part[b] = (r^H z, r^H r) partials
+// r^H z of one iteration's partials: slot 0 (Jacobi part) + slot 1 (K-FAC part)
+__device__ inline double total_rz(const double* part, int nparts, double* sh) {
+    return total(part, nparts, 4, 0, sh) + total(part, nparts, 4, 1, sh);
+}
+
+// z = M^-1 r and p = z for the parameters k < jac_end (the rest are preconditioned
+// by the K-FAC kernels afterwards), part[b] = (r^H z over k < jac_end, 0, r^H r)
 __global__ __launch_bounds__(256) void cg_init_kernel(const cplx<double>* r, cplx<double>* z, cplx<double>* p,
-                                                      const double* diag, double shift, int64_t n, double* part) {
+                                                      const double* diag, double shift, int64_t n, int64_t jac_end,
+                                                      double* part) {
     __shared__ double sh[4];
     double rz = 0.0, rr = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
         const cplx<double> ri = r[i];
-        const cplx<double> zi = precond(ri, diag, shift, i);
-        z[i] = zi;
-        p[i] = zi;
-        rz += ri.re * zi.re + ri.im * zi.im;
+        if (i < jac_end) {
+            const cplx<double> zi = precond(ri, diag, shift, i);
+            z[i] = zi;
+            p[i] = zi;
+            rz += ri.re * zi.re + ri.im * zi.im;
+        }
         rr += ri.re * ri.re + ri.im * ri.im;
     }
     rz = block_sum(rz, sh);
     rr = block_sum(rr, sh);
     if (threadIdx.x == 0) {
         part[blockIdx.x * 4 + 0] = rz;
+        part[blockIdx.x * 4 + 1] = 0.0;
         part[blockIdx.x * 4 + 2] = rr;
     }
 }
@@ -63,16 +73,18 @@ __global__ __launch_bounds__(256) void cg_pap_kernel(const cplx<double>* p, cons
     }
 }
 
-// alpha = rz / pAp; x += alpha p; r -= alpha Ap; z = M^-1 r; new partials (r^H z, r^H r).
-// Nothing changes once done[0] is set (the solve has converged).
+// alpha = rz / pAp; x += alpha p; r -= alpha Ap; z = M^-1 r for k < jac_end; new
+// partials (r^H z over k < jac_end, 0, r^H r).  Nothing changes once done[0] is set
+// (the solve has converged).
 __global__ __launch_bounds__(256) void cg_step_kernel(cplx<double>* x, cplx<double>* r, cplx<double>* z,
                                                       const cplx<double>* p, const cplx<double>* Ap,
                                                       const double* diag, double shift, int64_t n,
-                                                      const double* rzpart, int rz_parts, const double* apart,
-                                                      int a_parts, double* newpart, const int* done) {
+                                                      int64_t jac_end, const double* rzpart, int rz_parts,
+                                                      const double* apart, int a_parts, double* newpart,
+                                                      const int* done) {
     __shared__ double sh[4];
     if (done[0]) return;
-    const double rz = total(rzpart, rz_parts, 4, 0, sh);
+    const double rz = total_rz(rzpart, rz_parts, sh);
     const double pap = total(apart, a_parts, 2, 0, sh);
     const double alpha = pap > 0.0 ? rz / pap : 0.0;
     double rz2 = 0.0, rr = 0.0;
@@ -83,15 +95,18 @@ __global__ __launch_bounds__(256) void cg_step_kernel(cplx<double>* x, cplx<doub
         ri.re -= alpha * Ap[i].re;
         ri.im -= alpha * Ap[i].im;
         r[i] = ri;
-        const cplx<double> zi = precond(ri, diag, shift, i);
-        z[i] = zi;
-        rz2 += ri.re * zi.re + ri.im * zi.im;
+        if (i < jac_end) {
+            const cplx<double> zi = precond(ri, diag, shift, i);
+            z[i] = zi;
+            rz2 += ri.re * zi.re + ri.im * zi.im;
+        }
         rr += ri.re * ri.re + ri.im * ri.im;
     }
     rz2 = block_sum(rz2, sh);
     rr = block_sum(rr, sh);
     if (threadIdx.x == 0) {
         newpart[blockIdx.x * 4 + 0] = rz2;
+        newpart[blockIdx.x * 4 + 1] = 0.0;
         newpart[blockIdx.x * 4 + 2] = rr;
     }
 }
@@ -102,8 +117,8 @@ __global__ __launch_bounds__(256) void cg_dir_kernel(cplx<double>* p, const cplx
                                                      const int* done) {
     __shared__ double sh[4];
     if (done[0]) return;
-    const double rn = total(newpart, parts, 4, 0, sh);
-    const double ro = total(oldpart, parts, 4, 0, sh);
+    const double rn = total_rz(newpart, parts, sh);
+    const double ro = total_rz(oldpart, parts, sh);
     const double beta = ro != 0.0 ? rn / ro : 0.0;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
         p[i].re = z[i].re + beta * p[i].re;
@@ -132,8 +147,8 @@ void launch_pcg_check(const double* part, double tol2, int iter, int* done, hipS
 }
 
 void launch_pcg_init(const cplx<double>* r, cplx<double>* z, cplx<double>* p, const double* diag, double shift,
-                    int64_t n, double* part, hipStream_t stream) {
-    cg_init_kernel<<<kCgParts, 256, 0, stream>>>(r, z, p, diag, shift, n, part);
+                    int64_t n, int64_t jac_end, double* part, hipStream_t stream) {
+    cg_init_kernel<<<kCgParts, 256, 0, stream>>>(r, z, p, diag, shift, n, jac_end, part);
     ANN_LAUNCH_CHECK();
 }
 
@@ -143,11 +158,12 @@ void launch_pcg_pap(const cplx<double>* p, const cplx<double>* Ap, int64_t n, do
 }
 
 void launch_pcg_step(cplx<double>* x, cplx<double>* r, cplx<double>* z, const cplx<double>* p,
-                    const cplx<double>* Ap, const double* diag, double shift, int64_t n, const double* rzpart,
-                    const double* apart, int a_parts, double* newpart, const int* done, hipStream_t stream) {
+                    const cplx<double>* Ap, const double* diag, double shift, int64_t n, int64_t jac_end,
+                    const double* rzpart, const double* apart, int a_parts, double* newpart, const int* done,
+                    hipStream_t stream) {
     if (a_parts > 256) throw std::invalid_argument("too many partial sums");
-    cg_step_kernel<<<kCgParts, 256, 0, stream>>>(x, r, z, p, Ap, diag, shift, n, rzpart, kCgParts, apart, a_parts,
-                                                 newpart, done);
+    cg_step_kernel<<<kCgParts, 256, 0, stream>>>(x, r, z, p, Ap, diag, shift, n, jac_end, rzpart, kCgParts, apart,
+                                                 a_parts, newpart, done);
     ANN_LAUNCH_CHECK();
 }
 

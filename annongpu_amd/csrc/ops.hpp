@@ -34,13 +34,49 @@ void launch_cg_dir_dev(cplx<double>* p, const cplx<double>* z, const double* rz_
 // a matvec's partials of p^H Ap are [b][2]
 constexpr int kCgParts = 256;
 void launch_pcg_init(const cplx<double>* r, cplx<double>* z, cplx<double>* p, const double* diag, double shift,
-                     int64_t n, double* part, hipStream_t stream);
+                     int64_t n, int64_t jac_end, double* part, hipStream_t stream);
+
+// K-FAC preconditioner of the W parameters of a dense net (kfac.hip): factors from the
+// first kKfacSamples samples (moments per group of 32 samples, then a fixed-order
+// reduction), inverted once per solve; apply writes z (and p when p_init is set) at
+// every W parameter and slot 1 of the partials (kCgParts blocks)
+constexpr int64_t kKfacSamples = 2048;
+constexpr int kKfacMaxSites = 128;
+constexpr int kKfacMaxUnits = 2048;
+struct KfacWork {
+    double* spart;        // kfac_spart_elems: spin moments per sample group
+    cplx<double>* gpart;  // kfac_gpart_elems: g moments per sample group and unit block
+    double* C;            // [N][N]
+    double* Ci;           // [N][N] (C + gc)^-1
+    cplx<double>* Dblk;   // kfac_dblock_elems: D blocks [64][64], inverted in place
+    double* gam;          // [2] gc, gd
+};
+struct KfacView {
+    int N, M;
+    const int* idx;              // [N][M] parameter index of (site, unit), -1 where none
+    const double* Ci;
+    const cplx<double>* Dblk;
+};
+int kfac_dblocks(int M);
+size_t kfac_dblock_elems(int M);
+size_t kfac_spart_elems(int N, int64_t count);
+size_t kfac_gpart_elems(int M, int64_t count);
+void launch_kfac_setup(const uint64_t* confs, int words, const cplx<float>* g32, const cplx<double>* g64, int Mpad,
+                       const double* w, int64_t count, int N, int M, double shift, KfacWork& kw,
+                       hipStream_t stream);
+// idx of KfacView: RBM v M + j, PsiDeep N + M + inv[v][j] M + j
+void launch_kfac_index(int layout, int N, int M, int Mpad, const int* inv, int* idx, hipStream_t stream);
+void launch_kfac_apply(const KfacView& kv, const cplx<double>* r, cplx<double>* z, cplx<double>* p_init,
+                       double* part, cplx<double>* T, hipStream_t stream);
 void launch_pcg_pap(const cplx<double>* p, const cplx<double>* Ap, int64_t n, double* part, hipStream_t stream);
 // done = {converged, iteration}: step and dir leave x, r, z, p untouched once it is set,
-// launch_pcg_check sets it when r^H r of the iteration's partials is <= tol2
+// launch_pcg_check sets it when r^H r of the iteration's partials is <= tol2.  The
+// Jacobi preconditioner covers the parameters k < jac_end (all of them without K-FAC);
+// r^H z is the sum of slots 0 (Jacobi) and 1 (K-FAC) of the partials.
 void launch_pcg_step(cplx<double>* x, cplx<double>* r, cplx<double>* z, const cplx<double>* p,
-                     const cplx<double>* Ap, const double* diag, double shift, int64_t n, const double* rzpart,
-                     const double* apart, int a_parts, double* newpart, const int* done, hipStream_t stream);
+                     const cplx<double>* Ap, const double* diag, double shift, int64_t n, int64_t jac_end,
+                     const double* rzpart, const double* apart, int a_parts, double* newpart, const int* done,
+                     hipStream_t stream);
 void launch_pcg_dir(cplx<double>* p, const cplx<double>* z, int64_t n, const double* newpart, const double* oldpart,
                     const int* done, hipStream_t stream);
 void launch_pcg_check(const double* part, double tol2, int iter, int* done, hipStream_t stream);

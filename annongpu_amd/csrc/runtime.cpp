@@ -729,6 +729,7 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
                 slot = slot == -1 ? i : -2;
             }
         t.inv.upload(inv.data(), inv.size());
+        t.inv_unique = std::find(inv.begin(), inv.end(), -2) == inv.end();
     }
     t.confs.ensure(size_t(n) * s.words);
     ANN_HIP_CHECK(hipMemcpyAsync(t.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice, S()));
@@ -1132,8 +1133,38 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         ANN_HIP_CHECK(hipMemsetAsync(x.p, 0, sizeof(cplx<double>) * P, S()));
         ANN_HIP_CHECK(hipMemcpyAsync(r.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
     }
-    const double* diag = t.diag.p != nullptr && method == 1 ? t.diag.p : nullptr;
-    launch_pcg_init(r.p, z.p, p.p, diag, shift, P, part.p, S());
+    // method 1: Jacobi preconditioner; method 3: K-FAC on the W parameters of a dense
+    // net (the biases keep Jacobi), factors from this evaluation's samples
+    const bool kfac = method == 3;
+    const double* diag = t.diag.p != nullptr && (method == 1 || method == 3) ? t.diag.p : nullptr;
+    int64_t jac_end = P;
+    KfacView kv{};
+    if (kfac) {
+        if (t.generic) throw std::invalid_argument("the K-FAC preconditioner needs a PsiRBM or one-layer PsiDeep");
+        if (t.layout == LAYOUT_DEEP1 && !t.inv_unique)
+            throw std::invalid_argument("the K-FAC preconditioner needs distinct connection sites per hidden unit");
+        t.kf_spart.ensure(kfac_spart_elems(t.N, t.count));
+        t.kf_gpart.ensure(kfac_gpart_elems(t.M, t.count));
+        t.kf_C.ensure(size_t(t.N) * t.N);
+        t.kf_Ci.ensure(size_t(t.N) * t.N);
+        t.kf_D.ensure(kfac_dblock_elems(t.M));
+        t.kf_gam.ensure(2);
+        t.kf_T.ensure(size_t(t.N) * t.M);
+        t.kf_idx.ensure(size_t(t.N) * t.M);
+        KfacWork kw{t.kf_spart.p, t.kf_gpart.p, t.kf_C.p, t.kf_Ci.p, t.kf_D.p, t.kf_gam.p};
+        {
+            ScopedTimer timer("kfac_setup");
+            launch_kfac_index(t.layout, t.N, t.M, t.Mpad, t.layout == LAYOUT_DEEP1 ? t.inv.p : nullptr, t.kf_idx.p,
+                              S());
+            launch_kfac_setup(t.confs.p, t.words, t.precision != 0 ? t.g32.p : nullptr,
+                              t.precision != 0 ? nullptr : t.g64.p, t.Mpad, t.w.p, t.count, t.N, t.M, shift, kw,
+                              S());
+        }
+        kv = {t.N, t.M, t.kf_idx.p, t.kf_Ci.p, t.kf_D.p};
+        jac_end = t.layout == LAYOUT_DEEP1 ? int64_t(t.N) + t.M : 0;
+    }
+    launch_pcg_init(r.p, z.p, p.p, diag, shift, P, jac_end, part.p, S());
+    if (kfac) launch_kfac_apply(kv, r.p, z.p, p.p, part.p, t.kf_T.p, S());
     // ||F|| and the initial residual, read once
     t.rr_host.ensure(stride + 1);
     ANN_HIP_CHECK(hipMemcpyAsync(t.rr_host.p, part.p, sizeof(double) * stride, hipMemcpyDeviceToHost, S()));
@@ -1199,7 +1230,11 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
             ypart = apart.p;
             yparts = kCgParts;
         }
-        launch_pcg_step(x.p, r.p, z.p, pk, Ap.p, diag, shift, P, cur, ypart, yparts, nxt, done.p, S());
+        launch_pcg_step(x.p, r.p, z.p, pk, Ap.p, diag, shift, P, jac_end, cur, ypart, yparts, nxt, done.p, S());
+        if (kfac) {
+            ScopedTimer timer("kfac_apply", (it & 15) == 0);
+            launch_kfac_apply(kv, r.p, z.p, nullptr, nxt, t.kf_T.p, S());
+        }
         it++;
         if (it % check == 0 || it == max_iter) {
             launch_pcg_check(part.p + size_t(it) * stride, tol2, it, done.p, S());

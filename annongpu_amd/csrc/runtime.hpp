@@ -296,6 +296,11 @@ struct Tdvp {
     int64_t matvecs = 0;         // S.v products so far (every 16th is timed when profiling)
     PinnedBuf<double> rr_host;   // CG residual partials and ||F||^2 (page-locked)
     PinnedBuf<int> flag_host;    // two slots of the CG convergence flag, read behind events
+    bool inv_unique = true;      // PsiDeep: every (site, unit) carries at most one connection
+    // K-FAC preconditioner (kfac.hip)
+    DevBuf<double> kf_spart, kf_C, kf_Ci, kf_gam;
+    DevBuf<cplx<double>> kf_gpart, kf_D, kf_T;
+    DevBuf<int> kf_idx;
     DevBuf<int> inv;            // inverse gather table [N][Mpad] (PsiDeep)
     DevBuf<double> dot_scratch;
     // materialised O [count][P] of wave functions without a structured O (PsiCNN)
@@ -317,7 +322,8 @@ void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S);
 // of v^H y, *ypart / *yparts describe them ([yparts][2]), else they are null / 0
 void tdvp_S_dot(Tdvp& t, const cplx<double>* v_dev, double shift, cplx<double>* y_dev,
                 const double** ypart = nullptr, int* yparts = nullptr);
-// method 0 Cholesky, 1 Jacobi-preconditioned CG from x0 (zero when x0 is null)
+// method 0 Cholesky, 1 Jacobi-preconditioned CG, 3 K-FAC-preconditioned CG (dense
+// nets), the CG methods from x0 (zero when x0 is null)
 int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x, int* iters, double* res,
                const cd* x0 = nullptr);
 

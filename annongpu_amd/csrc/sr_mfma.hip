@@ -448,8 +448,9 @@ __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, 
     double beta = 0.0;
     bool frozen = false;
     if (dir.z) {
-        const double rn = parts_total(dir.newpart, dir.parts, 4, 0, sh);
-        const double ro = parts_total(dir.oldpart, dir.parts, 4, 0, sh);
+        // r^H z: slot 0 (Jacobi part) + slot 1 (K-FAC part) of the CG partials
+        const double rn = parts_total(dir.newpart, dir.parts, 4, 0, sh) + parts_total(dir.newpart, dir.parts, 4, 1, sh);
+        const double ro = parts_total(dir.oldpart, dir.parts, 4, 0, sh) + parts_total(dir.oldpart, dir.parts, 4, 1, sh);
         beta = ro != 0.0 ? rn / ro : 0.0;
         frozen = dir.done[0] != 0;
     }

@@ -163,6 +163,13 @@ int ann_tdvp_S_dot_vector(ann_tdvp tdvp, const double* v, double* out);
 #define ANN_SOLVE_CHOLESKY 0
 #define ANN_SOLVE_CG 1
 #define ANN_SOLVE_CG_WARM 2
+/* CG preconditioned by Kronecker factors of S (spin covariance x covariance of the
+ * hidden-unit factors g, blocks of 64 units) on the W parameters and Jacobi on the
+ * biases; PsiRBM and one-layer PsiDeep of up to 128 sites.  No counterpart in the
+ * reference, whose solve is a dense numpy call: an alternative solver for the same
+ * (S + shift) x = F.  The _WARM variant starts from the x passed in. */
+#define ANN_SOLVE_CG_KFAC 3
+#define ANN_SOLVE_CG_KFAC_WARM 4
 int ann_tdvp_solve(ann_tdvp tdvp, double shift, int32_t method, double rtol, int32_t max_iter, double* x,
                    int32_t* iterations, double* residual);
 
