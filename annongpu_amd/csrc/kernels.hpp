@@ -31,6 +31,26 @@ struct DenseView {
     const uint32_t* conn;  // [C*M] PsiDeep connections (nullptr for PsiRBM)
 };
 
+// The W parameter of (site v, hidden unit j) without a table: PsiRBM v M + j; a
+// PsiDeep whose connection i of unit j is site (j shift + i) mod N (the circulant
+// one-dimensional layout, shift >= 0) N + M + i M + j with i = (v - j shift) mod N
+// when i < C, else none (-1).  shift < 0: a PsiDeep with other connections (the
+// kernels then read tables).
+struct WParamMap {
+    int rbm, N, M, C, shift;
+    __host__ __device__ bool analytic() const { return rbm || shift >= 0; }
+    __host__ __device__ int param(int v, int j) const { return param_at(v, j, rbm ? 0 : (j * shift) % N); }
+    // the same with jm = (j shift) mod N given (no division per site)
+    __host__ __device__ int param_at(int v, int j, int jm) const {
+        if (rbm) return v * M + j;
+        int i = v - jm;
+        if (i < 0) i += N;
+        return i < C ? N + M + i * M + j : -1;
+    }
+    // site of connection i of unit j (circulant PsiDeep)
+    __host__ __device__ int site(int i, int j) const { return (j * shift + i) % N; }
+};
+
 // Pauli-sum operator compiled for the device: strings grouped by flip mask so
 // that X and Y strings that reach the same s' share one wave-function ratio.
 struct OpView {
@@ -168,14 +188,14 @@ int sr_ov_pairs(int M);
 size_t sr_frag_elems(int N, int Mpad);
 // the first half in three launches: V fragments (and the fused direction update),
 // O.v per column pair, u_s = w_s (O_s.v - <O>.v)
-void launch_sr_pack(const DenseView<float>& net, const int* inv, const cplx<double>* v, const CgDir& dir,
-                    const cplx<double>* obar, SrWorkspace ws, hipStream_t stream);
+void launch_sr_pack(const DenseView<float>& net, const WParamMap& wm, const int* inv, const cplx<double>* v,
+                    const CgDir& dir, const cplx<double>* obar, SrWorkspace ws, hipStream_t stream);
 void launch_sr_ov32(const DenseView<float>& net, const uint64_t* confs, int words, int64_t count,
                     const cplx<float>* g, SrWorkspace ws, hipStream_t stream);
 void launch_sr_finish(const DenseView<float>& net, const double* w, const uint64_t* confs, int words, int64_t count,
                       SrWorkspace ws, hipStream_t stream);
-void launch_sr_y_bf16(const DenseView<float>& net, const cplx<double>* v, double shift, const SpinPlan& plan,
-                      SrWorkspace ws, cplx<double>* y, hipStream_t stream);
+void launch_sr_y_bf16(const DenseView<float>& net, const WParamMap& wm, const cplx<double>* v, double shift,
+                      const SpinPlan& plan, SrWorkspace ws, cplx<double>* y, hipStream_t stream);
 
 void launch_weighted_sums(const cplx<double>* e_loc, const double* w, int64_t count,
                           double* out4 /* E.re, E.im, E2, sum w */, hipStream_t stream);

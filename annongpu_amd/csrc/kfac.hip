@@ -220,36 +220,37 @@ __device__ void gj_invert(E* A, int n) {   // A: n x n in LDS, row-major; blockD
     __syncthreads();
 }
 
-// Ci = (C + gc I)^-1 (one block, C in LDS)
+// Ci = (C + gc I)^-1 (block 0) and Di_b = (D_b + gd I)^-1 (block 1 + b, stored
+// transposed in place: Dblk[b][k][j] = Di_b[j][k]), each matrix in the block's
+// dynamic LDS; the two inversions run side by side in one launch.  Entries beyond a
+// D block's n stay as the reduction left them (zero).
 constexpr int kInvThreads = 1024;
 
-__global__ __launch_bounds__(kInvThreads) void kfac_invert_c_kernel(const double* C, int N, const double* gam,
-                                                                    double* Ci) {
-    extern __shared__ double lds_c[];
+__global__ __launch_bounds__(kInvThreads) void kfac_invert_kernel(const double* C, int N, const double* gam,
+                                                                  double* Ci, cplx<double>* Dblk, int M) {
+    extern __shared__ double lds_inv[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-    for (int r = wave; r < N; r += waves)
-        for (int c = lane; c < N; c += 64) lds_c[r * N + c] = r == c ? gj_addr(C[r * N + c], gam[0]) : C[r * N + c];
-    gj_invert<double>(lds_c, N);
-    for (int r = wave; r < N; r += waves)
-        for (int c = lane; c < N; c += 64) Ci[r * N + c] = lds_c[r * N + c];
-}
-
-// Di block b = (D_b + gd I)^-1 (one block per diagonal block of D), stored transposed
-// in place: Dblk[b][k][j] = Di_b[j][k]
-__global__ __launch_bounds__(kInvThreads) void kfac_invert_d_kernel(cplx<double>* Dblk, int M, const double* gam) {
-    extern __shared__ cplx<double> lds_d[];
-    const int b = blockIdx.x;
-    const int n = std::min(kDBlock, M - b * kDBlock);
+    if (blockIdx.x == 0) {
+        double* A = lds_inv;
+        for (int r = wave; r < N; r += waves)
+            for (int c = lane; c < N; c += 64) A[r * N + c] = r == c ? gj_addr(C[r * N + c], gam[0]) : C[r * N + c];
+        gj_invert<double>(A, N);
+        for (int r = wave; r < N; r += waves)
+            for (int c = lane; c < N; c += 64) Ci[r * N + c] = A[r * N + c];
+        return;
+    }
+    cplx<double>* A = reinterpret_cast<cplx<double>*>(lds_inv);
+    const int b = blockIdx.x - 1;
+    const int n = min(kDBlock, M - b * kDBlock);
     cplx<double>* src = Dblk + int64_t(b) * kDBlock * kDBlock;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
     for (int r = wave; r < n; r += waves)
         for (int c = lane; c < n; c += 64) {
             const cplx<double> x = src[r * kDBlock + c];
-            lds_d[r * n + c] = r == c ? gj_addr(x, gam[1]) : x;
+            A[r * n + c] = r == c ? gj_addr(x, gam[1]) : x;
         }
-    gj_invert<cplx<double>>(lds_d, n);
+    gj_invert<cplx<double>>(A, n);
     for (int r = wave; r < n; r += waves)
-        for (int c = lane; c < n; c += 64) src[c * kDBlock + r] = lds_d[r * n + c];
+        for (int c = lane; c < n; c += 64) src[c * kDBlock + r] = A[r * n + c];
 }
 
 // parameter index of W at (site v, unit j): RBM v M + j; PsiDeep N + M + i M + j with
@@ -266,81 +267,246 @@ __global__ void kfac_index_kernel(int layout, int N, int M, int Mpad, const int*
     }
 }
 
-__device__ inline double readlane_f64(double x, int l) {
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readlane(int(b & 0xffffffff), l);
-    const int hi = __builtin_amdgcn_readlane(int(b >> 32), l);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
+constexpr int kStepRows = 4;                      // sites per task of the fused step
+constexpr int kStepWaves = 8;                     // the site sum of a task split over 8 waves
+constexpr int kStepThreads = 64 * kStepWaves;
+constexpr int kStepSiteIters = (kKfacMaxSites + kStepWaves - 1) / kStepWaves;
+constexpr int kStepK = kDBlock / kStepWaves;      // k of Di per wave
 
-// T[v][j] = sum_{k in block(j)} X[v][k] Di[j][k], X[v][k] = r at (v, k): one wave per
-// (site v, unit block b); lane l gathers X[v][64 b + l] and owns j = 64 b + l; the
-// X values reach every lane by readlane, Di (stored transposed) is read coalesced
-__global__ __launch_bounds__(256) void kfac_apply_d_kernel(int N, int M, const int* idx, const cplx<double>* DiT,
-                                                           const cplx<double>* r, cplx<double>* T) {
-    const int lane = threadIdx.x & 63;
-    const int nblk = (M + kDBlock - 1) / kDBlock;
-    const int task = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (task >= N * nblk) return;   // whole waves; no barrier in this kernel
-    const int v = task / nblk, b = task % nblk;
-    const int n = min(kDBlock, M - b * kDBlock);
-    const int j = b * kDBlock + lane;
-    const int q = lane < n ? idx[int64_t(v) * M + j] : -1;
-    const cplx<double> x = q >= 0 ? r[q] : cplx<double>{0.0, 0.0};
-    const cplx<double>* di = DiT + int64_t(b) * kDBlock * kDBlock + lane;
-    // all 64 k: beyond n both X (lanes >= n) and the stored block are zero
-    double re = 0.0, im = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < kDBlock; k++) {
-        const double xr = readlane_f64(x.re, k), xi = readlane_f64(x.im, k);
-        const cplx<double> d = di[k * kDBlock];
-        re += xr * d.re - xi * d.im;
-        im += xr * d.im + xi * d.re;
-    }
-    if (lane < n) T[int64_t(v) * M + j] = {re, im};
-}
-
-__device__ inline double block_sum_kf(double v, double* sh) {
+__device__ inline double block_sum_step(double v, double* sh) {
     v = wave_sum(v);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    return sh[0] + sh[1] + sh[2] + sh[3];
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kStepWaves; w++) t += sh[w];
+    return t;
 }
 
-// Z[v][j] = sum_u Ci[v][u] T[u][j]: one wave per (site v, 64 units); the row of Ci is
-// wave-uniform, T is read coalesced.  z (and p when p_init is set) at every W
-// parameter; part[b][1] = Re sum conj(r) z of the block's waves.  Launched with
-// kCgParts blocks of 4 waves (grid-stride over the tasks).
-__global__ __launch_bounds__(256) void kfac_apply_c_kernel(int N, int M, const int* idx, const double* Ci,
-                                                           const cplx<double>* T, const cplx<double>* r,
-                                                           cplx<double>* z, cplx<double>* p_init, double* part) {
-    __shared__ double sh[4];
-    const int lane = threadIdx.x & 63;
-    const int jblk = (M + 63) / 64;
-    double rz = 0.0;
-    for (int task = blockIdx.x * 4 + (threadIdx.x >> 6); task < N * jblk; task += gridDim.x * 4) {
-        const int v = task / jblk, j = (task % jblk) * 64 + lane;
-        const double* ci = Ci + int64_t(v) * N;
-        const cplx<double>* tc = T + min(j, M - 1);
-        double re = 0.0, im = 0.0;
-#pragma unroll 8
-        for (int u = 0; u < N; u++) {
-            const double c = ci[u];
-            const cplx<double> t = tc[int64_t(u) * M];
-            re += c * t.re;
-            im += c * t.im;
+// One CG iteration after the matvec, with the K-FAC preconditioner (kCgParts blocks
+// of 8 waves):
+//   alpha = r^H z / p^H Ap (from the partials), x' = x + alpha p, r' = r - alpha Ap,
+//   z = (diag + shift)^-1 r' for the parameters k < jac_end (Jacobi),
+//   Z = Ci R' Di on the W parameters, R'[v][j] = r' at parameter (v, j),
+//   newpart[b] = (r'^H z over k < jac_end, r'^H z over the W parameters, r'^H r', 0).
+// Without Ap (the initial preconditioning) r' = r, x is not touched and p_init = z.
+// Z is formed as (Ci R') Di so that a task (sites v0 .. v0+3, unit block b of 64)
+// needs only its own rows of Ci R' (Y: the site sum split over the 8 waves) and the
+// block's Di (stored transposed, DiT[k][j]: wave w takes k = 8w .. 8w+7); the wave
+// partials are added in a fixed order through LDS.  A task issues all its loads
+// (alpha's partials, the done flag, its rows of Ci into LDS, R' at its 64 units, the
+// wave's Di rows, the owned parameters) before the first use of any.  Every task
+// reads r at all sites of its unit block, so r' goes to r_out, a buffer other than
+// r_in; each parameter is written by exactly one thread (its task, or the Jacobi
+// loop of the last blocks, which have no task at these sizes).  Nothing is written
+// once done[0] is set.
+template <bool STEP, bool ANALYTIC>
+__global__ __launch_bounds__(kStepThreads) void kfac_step_kernel(KfacStep a, int N, int M, const int* idx,
+                                                                 WParamMap wm, const double* Ci,
+                                                                 const cplx<double>* DiT) {
+    // loads are unconditional (indices clamped, values selected afterwards), so that
+    // none of them waits for another
+    __shared__ double sh[kStepWaves];
+    __shared__ cplx<double> red[kStepWaves][kStepRows][64];
+    __shared__ cplx<double> ys[kStepRows][64];
+    __shared__ double cis[kStepRows][kKfacMaxSites];
+    const int t = threadIdx.x, lane = t & 63, wave = uniform_int(t >> 6);
+    const cplx<double> zero{0.0, 0.0};
+    // alpha from its partials; the loads are issued with a task's other loads (inside
+    // each loop body, so they are not hoisted ahead of them), the sums taken at first use
+    struct Partials {
+        double2 rz[4];
+        double ap[4];
+    };
+    auto load_partials = [&](Partials& pt) {
+        if constexpr (STEP) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int b = lane + 64 * i;
+                pt.rz[i] = *reinterpret_cast<const double2*>(a.rzpart + b * 4);
+                pt.ap[i] = a.apart[min(b, a.a_parts - 1) * 2];
+            }
         }
-        const int q = j < M ? idx[int64_t(v) * M + j] : -1;
-        if (q >= 0) {
-            z[q] = {re, im};
-            if (p_init) p_init[q] = {re, im};
-            const cplx<double> rk = r[q];
-            rz += rk.re * re + rk.im * im;
+    };
+    auto get_alpha = [&](const Partials& pt) {
+        if constexpr (!STEP) return 0.0;
+        double x = 0.0, y = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            x += pt.rz[i].x + pt.rz[i].y;
+            y += lane + 64 * i < a.a_parts ? pt.ap[i] : 0.0;
+        }
+        x = wave_sum(x);
+        y = wave_sum(y);
+        return y > 0.0 ? x / y : 0.0;
+    };
+    auto rnew = [&](cplx<double> r, cplx<double> ap, double alpha) -> cplx<double> {
+        if constexpr (STEP) return {r.re - alpha * ap.re, r.im - alpha * ap.im};
+        return r;
+    };
+    double rz0 = 0.0, rz1 = 0.0, rr = 0.0;
+    // the owner's update of parameter q: r' = rq, z = zq, p and x as loaded
+    auto own = [&](int q, cplx<double> rq, cplx<double> zq, cplx<double> pq, cplx<double> xq, double alpha,
+                   double& rz) {
+        if constexpr (STEP) {
+            a.x[q] = {xq.re + alpha * pq.re, xq.im + alpha * pq.im};
+            a.r_out[q] = rq;
+        }
+        a.z[q] = zq;
+        if (a.p_init) a.p_init[q] = zq;
+        rz += rq.re * zq.re + rq.im * zq.im;
+        rr += rq.re * rq.re + rq.im * rq.im;
+    };
+    bool frozen = false;
+    const int nblk = (M + kDBlock - 1) / kDBlock;
+    const int tasks = (N + kStepRows - 1) / kStepRows * nblk;
+    for (int task = blockIdx.x; task < tasks; task += gridDim.x) {
+        const int v0 = task / nblk * kStepRows, b = task % nblk;
+        const int j = b * kDBlock + lane;
+        const bool jok = j < M;
+        const int jc = jok ? j : M - 1;
+        // parameter indices (-1: none) and their clamped form for the loads
+        const int vo = v0 + (t >> 6);
+        const bool owner = t < kStepRows * 64 && vo < N && jok;
+        const int voc = vo < N ? vo : N - 1;
+        const int jm = ANALYTIC && !wm.rbm ? (jc * wm.shift) % N : 0;
+        int qo = ANALYTIC ? wm.param_at(voc, jc, jm) : idx[voc * M + jc];
+        qo = owner ? qo : -1;
+        int qs[kStepSiteIters];
+#pragma unroll
+        for (int i = 0; i < kStepSiteIters; i++) {
+            const int u = wave + kStepWaves * i;
+            const int uc = u < N ? u : N - 1;
+            const int q = ANALYTIC ? wm.param_at(uc, jc, jm) : idx[uc * M + jc];
+            qs[i] = u < N && jok ? q : -1;
+        }
+        // Ci rows of the task
+        const int cr = t / kKfacMaxSites, cu = t % kKfacMaxSites;
+        const bool cok = cr < kStepRows && cu < N && v0 + cr < N;
+        const double cload = Ci[cok ? (v0 + cr) * N + cu : 0];
+        const int qoc = qo >= 0 ? qo : 0;
+        cplx<double> ro = a.r_in[qoc], ao = zero, po = zero, xo = zero;
+        if constexpr (STEP) {
+            ao = a.Ap[qoc];
+            po = a.p[qoc];
+            xo = a.x[qoc];
+        }
+        Partials pt;
+        load_partials(pt);
+        // R' at this wave's sites in two batches of 8 (the registers of 16 would spill)
+        constexpr int kBatch = kStepSiteIters / 2;
+        cplx<double> rv[kBatch], av[kBatch];
+        auto load_sites = [&](int i0) {
+#pragma unroll
+            for (int i = 0; i < kBatch; i++) {
+                const int qc = qs[i0 + i] >= 0 ? qs[i0 + i] : 0;
+                rv[i] = a.r_in[qc];
+                av[i] = zero;
+                if constexpr (STEP) av[i] = a.Ap[qc];
+            }
+        };
+        // Y[r][k] = sum_u Ci[v0 + r][u] R'[u][64 b + k] over this wave's sites u = wave + 8 i
+        cplx<double> acc[kStepRows];
+#pragma unroll
+        for (int r = 0; r < kStepRows; r++) acc[r] = zero;
+        auto add_sites = [&](int i0, double alpha) {
+#pragma unroll
+            for (int i = 0; i < kBatch; i++) {
+                const int u = wave + kStepWaves * (i0 + i);   // < 128; cis and x are zero for u >= N
+                cplx<double> x = rnew(rv[i], av[i], alpha);
+                if (qs[i0 + i] < 0) x = zero;
+#pragma unroll
+                for (int r = 0; r < kStepRows; r++) {
+                    const double c = cis[r][u];
+                    acc[r].re += c * x.re;
+                    acc[r].im += c * x.im;
+                }
+            }
+        };
+        load_sites(0);
+        frozen = a.done && a.done[0] != 0;
+        if (cr < kStepRows) cis[cr][cu] = cok ? cload : 0.0;
+        const double alpha = get_alpha(pt);
+        __syncthreads();
+        add_sites(0, alpha);
+        load_sites(kBatch);
+        cplx<double> d[kStepK];
+        const cplx<double>* di = DiT + (b * kDBlock + wave * kStepK) * kDBlock + lane;
+#pragma unroll
+        for (int i = 0; i < kStepK; i++) d[i] = di[i * kDBlock];
+        add_sites(kBatch, alpha);
+#pragma unroll
+        for (int r = 0; r < kStepRows; r++) red[wave][r][lane] = acc[r];
+        __syncthreads();
+        if (t < kStepRows * 64) {
+            const int r = t >> 6;
+            cplx<double> y = red[0][r][lane];
+#pragma unroll
+            for (int w = 1; w < kStepWaves; w++) {
+                y.re += red[w][r][lane].re;
+                y.im += red[w][r][lane].im;
+            }
+            ys[r][lane] = y;
+        }
+        __syncthreads();
+        // Z[r][j] = sum_k Y[r][k] Di[j][k], this wave's k = 8 wave + i
+#pragma unroll
+        for (int r = 0; r < kStepRows; r++) {
+            double zr = 0.0, zi = 0.0;
+#pragma unroll
+            for (int i = 0; i < kStepK; i++) {
+                const cplx<double> y = ys[r][wave * kStepK + i];
+                zr += y.re * d[i].re - y.im * d[i].im;
+                zi += y.re * d[i].im + y.im * d[i].re;
+            }
+            red[wave][r][lane] = {zr, zi};
+        }
+        __syncthreads();
+        if (qo >= 0 && !frozen) {
+            const int r = t >> 6;
+            cplx<double> zq = red[0][r][lane];
+#pragma unroll
+            for (int w = 1; w < kStepWaves; w++) {
+                zq.re += red[w][r][lane].re;
+                zq.im += red[w][r][lane].im;
+            }
+            own(qo, rnew(ro, ao, alpha), zq, po, xo, alpha, rz1);
+        }
+        __syncthreads();
+    }
+    // Jacobi part, from the last block down (no task at these sizes); alpha is taken
+    // before the loop, where every lane of the wave is active for its reduction
+    const int jb = int(gridDim.x) - 1 - int(blockIdx.x);
+    if (int64_t(jb) * kStepThreads < a.jac_end) {
+        Partials pt;
+        load_partials(pt);
+        const double alpha = get_alpha(pt);
+        const bool fz = a.done && a.done[0] != 0;
+        for (int k = jb * kStepThreads + t; k < a.jac_end; k += int(gridDim.x) * kStepThreads) {
+            const cplx<double> r0 = a.r_in[k];
+            cplx<double> ap = zero, pq = zero, xq = zero;
+            if constexpr (STEP) {
+                ap = a.Ap[k];
+                pq = a.p[k];
+                xq = a.x[k];
+            }
+            const double d = a.diag ? a.diag[k] + a.shift : 1.0;
+            const cplx<double> rq = rnew(r0, ap, alpha);
+            if (!fz) own(k, rq, {rq.re / d, rq.im / d}, pq, xq, alpha, rz0);
         }
     }
-    rz = block_sum_kf(rz, sh);
-    if (threadIdx.x == 0) part[blockIdx.x * 4 + 1] = rz;
+    if (a.done) frozen = a.done[0] != 0;
+    if (frozen) return;   // uniform: no barrier is skipped by part of the block
+    rz0 = block_sum_step(rz0, sh);
+    rz1 = block_sum_step(rz1, sh);
+    rr = block_sum_step(rr, sh);
+    if (t == 0) {
+        a.newpart[blockIdx.x * 4 + 0] = rz0;
+        a.newpart[blockIdx.x * 4 + 1] = rz1;
+        a.newpart[blockIdx.x * 4 + 2] = rr;
+        a.newpart[blockIdx.x * 4 + 3] = 0.0;
+    }
 }
 
 }  // namespace
@@ -365,15 +531,10 @@ void launch_kfac_setup(const uint64_t* confs, int words, const cplx<float>* g32,
     kfac_damping_kernel<<<1, 256, 0, stream>>>(kw.C, N, kw.Dblk, M, shift, kw.gam);
     ANN_LAUNCH_CHECK();
     // dynamic LDS beyond the default limit is opted in (up to 128 KiB here, of 160 KiB per CU)
-    const size_t bytes_c = sizeof(double) * N * N;
-    ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kfac_invert_c_kernel),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes_c)));
-    kfac_invert_c_kernel<<<1, kInvThreads, bytes_c, stream>>>(kw.C, N, kw.gam, kw.Ci);
-    ANN_LAUNCH_CHECK();
-    const size_t bytes_d = sizeof(cplx<double>) * kDBlock * kDBlock;
-    ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kfac_invert_d_kernel),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes_d)));
-    kfac_invert_d_kernel<<<unsigned(nblk), kInvThreads, bytes_d, stream>>>(kw.Dblk, M, kw.gam);
+    const size_t bytes = std::max(sizeof(double) * N * N, sizeof(cplx<double>) * kDBlock * kDBlock);
+    ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kfac_invert_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    kfac_invert_kernel<<<unsigned(1 + nblk), kInvThreads, bytes, stream>>>(kw.C, N, kw.gam, kw.Ci, kw.Dblk, M);
     ANN_LAUNCH_CHECK();
 }
 
@@ -383,12 +544,19 @@ void launch_kfac_index(int layout, int N, int M, int Mpad, const int* inv, int* 
     ANN_LAUNCH_CHECK();
 }
 
-void launch_kfac_apply(const KfacView& kv, const cplx<double>* r, cplx<double>* z, cplx<double>* p_init,
-                       double* part, cplx<double>* T, hipStream_t stream) {
-    const int64_t dtasks = int64_t(kv.N) * kfac_dblocks(kv.M);
-    kfac_apply_d_kernel<<<unsigned((dtasks + 3) / 4), 256, 0, stream>>>(kv.N, kv.M, kv.idx, kv.Dblk, r, T);
-    ANN_LAUNCH_CHECK();
-    kfac_apply_c_kernel<<<kCgParts, 256, 0, stream>>>(kv.N, kv.M, kv.idx, kv.Ci, T, r, z, p_init, part);
+void launch_kfac_step(const KfacView& kv, const KfacStep& a, hipStream_t stream) {
+    if (a.a_parts > 256) throw std::invalid_argument("too many partial sums");
+    if (a.Ap && (!a.x || !a.p || !a.r_out || a.r_out == a.r_in || !a.apart || !a.rzpart))
+        throw std::invalid_argument("incomplete K-FAC CG step");
+    if (int64_t(kv.N) * kv.M + a.jac_end > INT32_MAX) throw std::invalid_argument("too many parameters");
+    const bool affine = kv.wm.analytic();
+    auto run = [&](auto kern) {
+        kern<<<kCgParts, kStepThreads, 0, stream>>>(a, kv.N, kv.M, kv.idx, kv.wm, kv.Ci, kv.Dblk);
+    };
+    if (a.Ap)
+        affine ? run(kfac_step_kernel<true, true>) : run(kfac_step_kernel<true, false>);
+    else
+        affine ? run(kfac_step_kernel<false, true>) : run(kfac_step_kernel<false, false>);
     ANN_LAUNCH_CHECK();
 }
 

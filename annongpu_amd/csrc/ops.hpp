@@ -38,8 +38,8 @@ void launch_pcg_init(const cplx<double>* r, cplx<double>* z, cplx<double>* p, co
 
 // K-FAC preconditioner of the W parameters of a dense net (kfac.hip): factors from the
 // first kKfacSamples samples (moments per group of 32 samples, then a fixed-order
-// reduction), inverted once per solve; apply writes z (and p when p_init is set) at
-// every W parameter and slot 1 of the partials (kCgParts blocks)
+// reduction), inverted once per solve; the step kernel takes the CG update and the
+// preconditioning of every parameter (Jacobi below jac_end, K-FAC on the W part)
 constexpr int64_t kKfacSamples = 2048;
 constexpr int kKfacMaxSites = 128;
 constexpr int kKfacMaxUnits = 2048;
@@ -54,8 +54,28 @@ struct KfacWork {
 struct KfacView {
     int N, M;
     const int* idx;              // [N][M] parameter index of (site, unit), -1 where none
+    WParamMap wm;                // the same without the table when wm.analytic()
     const double* Ci;
     const cplx<double>* Dblk;
+};
+// one CG iteration after the matvec with the K-FAC preconditioner, or (Ap null) the
+// preconditioning of the initial residual; the partials are [kCgParts][4]
+struct KfacStep {
+    cplx<double>* x;             // x += alpha p (step)
+    const cplx<double>* r_in;    // r of the previous iteration
+    cplx<double>* r_out;         // r - alpha Ap (step; a buffer other than r_in)
+    cplx<double>* z;
+    cplx<double>* p_init;        // initial preconditioning: p = z (else null)
+    const cplx<double>* p;
+    const cplx<double>* Ap;
+    const double* diag;          // Jacobi diagonal of the parameters k < jac_end (null: identity)
+    double shift;
+    int64_t jac_end;
+    const double* rzpart;        // partials of the previous iteration
+    const double* apart;         // p^H Ap partials [a_parts][2]
+    int a_parts;
+    double* newpart;
+    const int* done;
 };
 int kfac_dblocks(int M);
 size_t kfac_dblock_elems(int M);
@@ -66,8 +86,7 @@ void launch_kfac_setup(const uint64_t* confs, int words, const cplx<float>* g32,
                        hipStream_t stream);
 // idx of KfacView: RBM v M + j, PsiDeep N + M + inv[v][j] M + j
 void launch_kfac_index(int layout, int N, int M, int Mpad, const int* inv, int* idx, hipStream_t stream);
-void launch_kfac_apply(const KfacView& kv, const cplx<double>* r, cplx<double>* z, cplx<double>* p_init,
-                       double* part, cplx<double>* T, hipStream_t stream);
+void launch_kfac_step(const KfacView& kv, const KfacStep& a, hipStream_t stream);
 void launch_pcg_pap(const cplx<double>* p, const cplx<double>* Ap, int64_t n, double* part, hipStream_t stream);
 // done = {converged, iteration}: step and dir leave x, r, z, p untouched once it is set,
 // launch_pcg_check sets it when r^H r of the iteration's partials is <= tol2.  The

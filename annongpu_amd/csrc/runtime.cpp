@@ -720,6 +720,7 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     t.layout = psi.layout;
     t.npad = psi.npad;
     t.count = n;
+    t.conn_shift = -1;
     if (psi.kind == PSI_DEEP) {
         t.conn.upload(psi.conn.data(), psi.conn.size());
         std::vector<int> inv(size_t(psi.N) * psi.Mpad, -1);
@@ -730,6 +731,16 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
             }
         t.inv.upload(inv.data(), inv.size());
         t.inv_unique = std::find(inv.begin(), inv.end(), -2) == inv.end();
+        // circulant layout: connection i of unit j is site (j d + i) mod N
+        const int d = psi.M > 1 ? int(psi.conn[1]) : 0;
+        bool circ = psi.C <= psi.N;
+        for (int i = 0; i < psi.C && circ; i++)
+            for (int j = 0; j < psi.M; j++)
+                if (int(psi.conn[size_t(i) * psi.M + j]) != int((int64_t(j) * d + i) % psi.N)) {
+                    circ = false;
+                    break;
+                }
+        t.conn_shift = circ ? d : -1;
     }
     t.confs.ensure(size_t(n) * s.words);
     ANN_HIP_CHECK(hipMemcpyAsync(t.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice, S()));
@@ -913,7 +924,7 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y,
         const bool timed = t.matvecs++ % 16 == 0;
         {
             ScopedTimer timer("sr_pack", timed);
-            launch_sr_pack(net, t.inv.p, v, dir ? *dir : CgDir{}, t.Obar.p, ws, S());
+            launch_sr_pack(net, t.wmap(), t.inv.p, v, dir ? *dir : CgDir{}, t.Obar.p, ws, S());
         }
         {
             ScopedTimer timer("sr_ov", timed);
@@ -930,7 +941,7 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y,
         }
         {
             ScopedTimer timer("sr_y", timed);
-            launch_sr_y_bf16(net, v, shift, plan, ws, y, S());
+            launch_sr_y_bf16(net, t.wmap(), v, shift, plan, ws, y, S());
         }
         if (ypart) *ypart = t.ypart.p;
         if (yparts) *yparts = sr_y_parts(P);
@@ -1149,7 +1160,6 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         t.kf_Ci.ensure(size_t(t.N) * t.N);
         t.kf_D.ensure(kfac_dblock_elems(t.M));
         t.kf_gam.ensure(2);
-        t.kf_T.ensure(size_t(t.N) * t.M);
         t.kf_idx.ensure(size_t(t.N) * t.M);
         KfacWork kw{t.kf_spart.p, t.kf_gpart.p, t.kf_C.p, t.kf_Ci.p, t.kf_D.p, t.kf_gam.p};
         {
@@ -1160,11 +1170,27 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
                               t.precision != 0 ? nullptr : t.g64.p, t.Mpad, t.w.p, t.count, t.N, t.M, shift, kw,
                               S());
         }
-        kv = {t.N, t.M, t.kf_idx.p, t.kf_Ci.p, t.kf_D.p};
         jac_end = t.layout == LAYOUT_DEEP1 ? int64_t(t.N) + t.M : 0;
+        kv = {t.N, t.M, t.kf_idx.p, t.wmap(), t.kf_Ci.p, t.kf_D.p};
     }
-    launch_pcg_init(r.p, z.p, p.p, diag, shift, P, jac_end, part.p, S());
-    if (kfac) launch_kfac_apply(kv, r.p, z.p, p.p, part.p, t.kf_T.p, S());
+    // r is double-buffered under K-FAC (every workgroup of its step reads r at many
+    // parameters while their owners write r - alpha Ap)
+    DevBuf<cplx<double>> r2;
+    cplx<double>* r_cur = r.p;
+    if (kfac) {
+        r2.ensure(P);
+        KfacStep a{};
+        a.r_in = r.p;
+        a.z = z.p;
+        a.p_init = p.p;
+        a.diag = diag;
+        a.shift = shift;
+        a.jac_end = jac_end;
+        a.newpart = part.p;
+        launch_kfac_step(kv, a, S());
+    } else {
+        launch_pcg_init(r.p, z.p, p.p, diag, shift, P, jac_end, part.p, S());
+    }
     // ||F|| and the initial residual, read once
     t.rr_host.ensure(stride + 1);
     ANN_HIP_CHECK(hipMemcpyAsync(t.rr_host.p, part.p, sizeof(double) * stride, hipMemcpyDeviceToHost, S()));
@@ -1230,10 +1256,27 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
             ypart = apart.p;
             yparts = kCgParts;
         }
-        launch_pcg_step(x.p, r.p, z.p, pk, Ap.p, diag, shift, P, jac_end, cur, ypart, yparts, nxt, done.p, S());
         if (kfac) {
-            ScopedTimer timer("kfac_apply", (it & 15) == 0);
-            launch_kfac_apply(kv, r.p, z.p, nullptr, nxt, t.kf_T.p, S());
+            ScopedTimer timer("kfac_step", (it & 15) == 0);
+            KfacStep a{};
+            a.x = x.p;
+            a.r_in = r_cur;
+            a.r_out = r_cur == r.p ? r2.p : r.p;
+            a.z = z.p;
+            a.p = pk;
+            a.Ap = Ap.p;
+            a.diag = diag;
+            a.shift = shift;
+            a.jac_end = jac_end;
+            a.rzpart = cur;
+            a.apart = ypart;
+            a.a_parts = yparts;
+            a.newpart = nxt;
+            a.done = done.p;
+            launch_kfac_step(kv, a, S());
+            r_cur = a.r_out;
+        } else {
+            launch_pcg_step(x.p, r.p, z.p, pk, Ap.p, diag, shift, P, jac_end, cur, ypart, yparts, nxt, done.p, S());
         }
         it++;
         if (it % check == 0 || it == max_iter) {

@@ -297,9 +297,11 @@ struct Tdvp {
     PinnedBuf<double> rr_host;   // CG residual partials and ||F||^2 (page-locked)
     PinnedBuf<int> flag_host;    // two slots of the CG convergence flag, read behind events
     bool inv_unique = true;      // PsiDeep: every (site, unit) carries at most one connection
+    int conn_shift = -1;         // PsiDeep: connection i of unit j is site (j shift + i) mod N (else -1)
     // K-FAC preconditioner (kfac.hip)
     DevBuf<double> kf_spart, kf_C, kf_Ci, kf_gam;
-    DevBuf<cplx<double>> kf_gpart, kf_D, kf_T;
+    DevBuf<cplx<double>> kf_gpart, kf_D;
+    WParamMap wmap() const { return {layout == LAYOUT_RBM ? 1 : 0, N, M, C, conn_shift}; }
     DevBuf<int> kf_idx;
     DevBuf<int> inv;            // inverse gather table [N][Mpad] (PsiDeep)
     DevBuf<double> dot_scratch;

@@ -431,67 +431,111 @@ __device__ inline double parts_total(const double* partial, int nparts, int stri
 
 constexpr int kDotParts = 64;   // blocks of the fused dot products
 
+// beta = r^H z (this iteration) / r^H z (previous) from the CG partials (slots 0 + 1
+// of [parts][4], parts <= 256): loads issued by load(), the sum taken by every wave
+// alone at value() (same fixed order in every wave, no barrier; all lanes active)
+struct CgBeta {
+    double2 pn[4], po[4];
+    __device__ void load(const CgDir& dir) {
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int b = min(lane + 64 * i, dir.parts - 1);
+            pn[i] = *reinterpret_cast<const double2*>(dir.newpart + b * 4);
+            po[i] = *reinterpret_cast<const double2*>(dir.oldpart + b * 4);
+        }
+    }
+    __device__ double value(const CgDir& dir) const {
+        const int lane = threadIdx.x & 63;
+        double rn = 0.0, ro = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const bool ok = lane + 64 * i < dir.parts;
+            rn += ok ? pn[i].x + pn[i].y : 0.0;
+            ro += ok ? po[i].x + po[i].y : 0.0;
+        }
+        rn = wave_sum(rn);
+        ro = wave_sum(ro);
+        return ro != 0.0 ? rn / ro : 0.0;
+    }
+};
+
 // v (parameter order) -> bf16x3 fragments of V (pair layout, 32-site k tiles:
 // frag[(((sp KT + kt) NT + nt) 64 + lane) 8 + el], site 32 kt + 8 (lane/16) + el,
 // unit 16 (nt/2) + lane%16, nt even = real parts), va, vb, and the partials of
 // <O>.v.  Blocks [0, pack) pack, then [pack, pack + vab) write va / vb, then
 // kDotParts blocks reduce <O>.v.  With dir.z set, v is the new CG direction
 // z + beta p_old, formed on the fly by every block (beta from the rz partials)
-// and written to dir.p_new by the <O>.v blocks.
-__global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, const int* inv,
+// and written to dir.p_new by the <O>.v blocks; it stays p_old once dir.done[0] is
+// set.  The PsiDeep parameter of (site, unit) comes from wm when it is analytic,
+// else from the table inv.
+// Every role issues its loads (z and p_old, or v, at clamped indices) before it
+// waits for the beta partials, so a block takes one memory round trip.
+__global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, WParamMap wm, const int* inv,
                                                           const cplx<double>* v, CgDir dir, int KT, int NT,
                                                           int pack_blocks, int vab_blocks, uint16_t* frag,
                                                           cplx<double>* va, cplx<double>* vb,
                                                           const cplx<double>* obar, double* opart) {
     __shared__ double sh[4];
     const int b = blockIdx.x;
-    double beta = 0.0;
-    bool frozen = false;
-    if (dir.z) {
-        // r^H z: slot 0 (Jacobi part) + slot 1 (K-FAC part) of the CG partials
-        const double rn = parts_total(dir.newpart, dir.parts, 4, 0, sh) + parts_total(dir.newpart, dir.parts, 4, 1, sh);
-        const double ro = parts_total(dir.oldpart, dir.parts, 4, 0, sh) + parts_total(dir.oldpart, dir.parts, 4, 1, sh);
-        beta = ro != 0.0 ? rn / ro : 0.0;
-        frozen = dir.done[0] != 0;
+    const bool cg = dir.z != nullptr;
+    CgBeta cb;
+    int frozen = 0;
+    if (cg) {
+        cb.load(dir);
+        frozen = dir.done[0];
     }
-    auto vk = [&](int64_t k) -> cplx<double> {
-        if (!dir.z) return v[k];
-        const cplx<double> po = dir.p_old[k];
-        if (frozen) return po;
-        const cplx<double> zz = dir.z[k];
-        return {zz.re + beta * po.re, zz.im + beta * po.im};
+    // the value at parameter k (clamped to 0 for loads; the caller discards k < 0)
+    struct Pending {
+        cplx<double> a, b;
     };
+    auto fetch = [&](int64_t k) -> Pending {
+        const int64_t kc = k >= 0 ? k : 0;
+        if (!cg) return {v[kc], {0.0, 0.0}};
+        return {dir.z[kc], dir.p_old[kc]};
+    };
+    auto value = [&](const Pending& x, double beta) -> cplx<double> {
+        if (!cg) return x.a;
+        if (frozen) return x.b;
+        return {x.a.re + beta * x.b.re, x.a.im + beta * x.b.im};
+    };
+    auto get_beta = [&]() { return cg ? cb.value(dir) : 0.0; };
     if (b < pack_blocks) {
         const int64_t e = int64_t(b) * 256 + threadIdx.x;
         const int ncol = 2 * net.Mpad;
-        if (e >= int64_t(KT) * 32 * ncol) return;
-        const int vis = int(e / ncol), col = int(e % ncol);
+        const bool in = e < int64_t(KT) * 32 * ncol;
+        const int vis = in ? int(e / ncol) : 0, col = in ? int(e % ncol) : 0;
         const int nt = col / 16, cc = col % 16;
         const int j = (nt / 2) * 16 + cc;
-        double re = 0.0, im = 0.0;
-        if (vis < net.N && j < net.M) {
+        // parameter of (site vis, unit j): -1 none, -2 several connections (summed below)
+        int64_t k = -1;
+        if (in && vis < net.N && j < net.M) {
             if (net.layout == LAYOUT_RBM) {
-                const cplx<double> x = vk(int64_t(vis) * net.M + j);
-                re = x.re;
-                im = x.im;
+                k = int64_t(vis) * net.M + j;
+            } else if (wm.shift >= 0) {
+                k = wm.param(vis, j);
             } else {
-                // inv[vis][j]: the unique i with conn(i, j) = vis, -1 if none, -2 if several
                 const int i0 = inv[int64_t(vis) * net.Mpad + j];
-                if (i0 >= 0) {
-                    const cplx<double> x = vk(net.N + net.M + int64_t(i0) * net.M + j);
-                    re = x.re;
-                    im = x.im;
-                } else if (i0 == -2) {
-                    for (int i = 0; i < net.C; i++) {
-                        if (int(net.conn[int64_t(i) * net.M + j]) == vis) {
-                            const cplx<double> x = vk(net.N + net.M + int64_t(i) * net.M + j);
-                            re += x.re;
-                            im += x.im;
-                        }
-                    }
+                k = i0 >= 0 ? net.N + net.M + int64_t(i0) * net.M + j : (i0 == -2 ? -2 : -1);
+            }
+        }
+        const Pending pd = fetch(k);
+        const double beta = get_beta();
+        double re = 0.0, im = 0.0;
+        if (k >= 0) {
+            const cplx<double> x = value(pd, beta);
+            re = x.re;
+            im = x.im;
+        } else if (k == -2) {
+            for (int i = 0; i < net.C; i++) {
+                if (int(net.conn[int64_t(i) * net.M + j]) == vis) {
+                    const cplx<double> x = value(fetch(net.N + net.M + int64_t(i) * net.M + j), beta);
+                    re += x.re;
+                    im += x.im;
                 }
             }
         }
+        if (!in) return;
         float r = float((nt & 1) ? im : re);
         const int kt = vis / 32, kk = vis % 32;
         const int lane = (kk / 8) * 16 + cc, el = kk % 8;
@@ -506,23 +550,43 @@ __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, 
     if (b < pack_blocks + vab_blocks) {
         const int e2 = (b - pack_blocks) * 256 + threadIdx.x;
         const bool deep = net.layout == LAYOUT_DEEP1;
-        if (e2 < net.Mpad) {
-            vb[e2] = deep && e2 < net.M ? vk(net.N + e2) : cplx<double>{0.0, 0.0};
-        } else if (e2 < net.Mpad + net.N) {
-            const int i = e2 - net.Mpad;
-            va[i] = deep ? vk(i) : cplx<double>{0.0, 0.0};
-        }
+        const int64_t k = !deep ? -1 : e2 < net.M ? net.N + e2 : (e2 >= net.Mpad && e2 < net.Mpad + net.N) ? e2 - net.Mpad : -1;
+        const Pending pd = fetch(k);
+        const double beta = get_beta();
+        const cplx<double> x = k >= 0 ? value(pd, beta) : cplx<double>{0.0, 0.0};
+        if (e2 < net.Mpad)
+            vb[e2] = x;
+        else if (e2 < net.Mpad + net.N)
+            va[e2 - net.Mpad] = x;
         return;
     }
-    // <O>.v (no conjugation), grid-stride over the parameters; these blocks also
-    // store the new direction
+    // <O>.v (no conjugation) in rounds of 64 blocks x 256 parameters, four rounds per
+    // batch of loads; these blocks also store the new direction
     const int db = b - pack_blocks - vab_blocks;
+    const int64_t stride = int64_t(kDotParts) * 256;
+    const int64_t rounds = (net.num_params + stride - 1) / stride;
     double re = 0.0, im = 0.0;
-    for (int64_t k = int64_t(db) * 256 + threadIdx.x; k < net.num_params; k += int64_t(kDotParts) * 256) {
-        const cplx<double> o = obar[k], x = vk(k);
-        if (dir.z) dir.p_new[k] = x;
-        re += o.re * x.re - o.im * x.im;
-        im += o.re * x.im + o.im * x.re;
+    double beta = 0.0;
+    for (int64_t r0 = 0; r0 < rounds; r0 += 4) {
+        Pending pd[4];
+        cplx<double> o[4];
+        int64_t ks[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int64_t k = (r0 + i) * stride + int64_t(db) * 256 + threadIdx.x;
+            ks[i] = k < net.num_params ? k : -1;
+            pd[i] = fetch(ks[i]);
+            o[i] = obar[ks[i] >= 0 ? ks[i] : 0];
+        }
+        if (r0 == 0) beta = get_beta();
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (ks[i] < 0) continue;
+            const cplx<double> x = value(pd[i], beta);
+            if (cg) dir.p_new[ks[i]] = x;
+            re += o[i].re * x.re - o[i].im * x.im;
+            im += o[i].re * x.im + o[i].im * x.re;
+        }
     }
     re = block_sum256(re, sh);
     im = block_sum256(im, sh);
@@ -541,39 +605,51 @@ __device__ inline double quad_sum(double v) {
     return v;
 }
 
+// (all loads at clamped indices first: the <O>.v partials, va, the pair partials and
+// the configuration; <O>.v is then summed by every wave alone, kDotParts = 64 lanes)
+static_assert(kDotParts == 64, "one <O>.v partial per lane");
+constexpr int kFinishBatch = 8;   // pair partials per lane and load batch
 __global__ __launch_bounds__(256) void ov_finish_center_kernel(const float2* upart, int npairs, int64_t count,
                                                                int64_t pad,
                                                                const uint64_t* confs, int words, int N,
                                                                const cplx<double>* va, const double* opart,
                                                                const double* w, float2* u) {
-    __shared__ double sh[4];
     __shared__ cplx<double> sva[64 * kMaxWords];   // va of up to 256 sites
-    const double c_re = parts_total(opart, kDotParts, 2, 0, sh);
-    const double c_im = parts_total(opart, kDotParts, 2, 1, sh);
-    if (va)
-        for (int i = threadIdx.x; i < N; i += blockDim.x) sva[i] = va[i];
-    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const double2 oc = *reinterpret_cast<const double2*>(opart + lane * 2);
+    const cplx<double> vai = va && int(threadIdx.x) < N ? va[threadIdx.x] : cplx<double>{0.0, 0.0};
     const int64_t s = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 2;
     const int q = threadIdx.x & 3;
     const bool ok = s < count;
-    double re = 0.0, im = 0.0;
-    if (ok) {
-#pragma unroll 8
-        for (int b = q; b < npairs; b += 4) {
-            const float2 x = upart[int64_t(b) * count + s];
-            re += double(x.x);
-            im += double(x.y);
-        }
-        if (va) {
-            uint64_t cw[kMaxWords];
+    const int64_t sc = ok ? s : count - 1;
+    uint64_t cw[kMaxWords];
 #pragma unroll
-            for (int k = 0; k < kMaxWords; k++) cw[k] = k < words ? confs[s * words + k] : 0;
-            for (int i = q; i < N; i += 4) {
-                const uint64_t word = i < 64 ? cw[0] : i < 128 ? cw[1] : i < 192 ? cw[2] : cw[3];
-                const double si = ((word >> (i & 63)) & 1) ? 1.0 : -1.0;
-                re += si * sva[i].re;
-                im += si * sva[i].im;
+    for (int k = 0; k < kMaxWords; k++) cw[k] = confs[sc * words + (k < words ? k : words - 1)];
+    double re = 0.0, im = 0.0;
+    for (int b0 = q; b0 < npairs; b0 += 4 * kFinishBatch) {
+        float2 x[kFinishBatch];
+#pragma unroll
+        for (int i = 0; i < kFinishBatch; i++) {
+            const int b = b0 + 4 * i;
+            x[i] = upart[int64_t(b < npairs ? b : npairs - 1) * count + sc];
+        }
+#pragma unroll
+        for (int i = 0; i < kFinishBatch; i++) {
+            if (b0 + 4 * i < npairs) {
+                re += double(x[i].x);
+                im += double(x[i].y);
             }
+        }
+    }
+    const double c_re = wave_sum(oc.x), c_im = wave_sum(oc.y);
+    if (va && int(threadIdx.x) < N) sva[threadIdx.x] = vai;
+    __syncthreads();
+    if (ok && va) {
+        for (int i = q; i < N; i += 4) {
+            const uint64_t word = i < 64 ? cw[0] : i < 128 ? cw[1] : i < 192 ? cw[2] : cw[3];
+            const double si = ((word >> (i & 63)) & 1) ? 1.0 : -1.0;
+            re += si * sva[i].re;
+            im += si * sva[i].im;
         }
     }
     re = quad_sum(re);
@@ -583,9 +659,14 @@ __global__ __launch_bounds__(256) void ov_finish_center_kernel(const float2* upa
 }
 
 // y_k (parameter order) = sum over sample chunks of the spin-GEMM partials (fp64,
-// chunk order) + shift v_k, plus the block partials of v^H y
+// chunk order) + shift v_k, plus the block partials of v^H y.  The chunk partials of
+// a parameter are loaded in batches of kYBatch at clamped indices (one memory round
+// trip for up to kYBatch chunks); the site of a PsiDeep connection comes from wm
+// when it is analytic, else from conn.
+constexpr int kYBatch = 32;
 __global__ __launch_bounds__(256) void spin_y_kernel(const float* partial, int chunks, int64_t n, int ncols,
-                                                     int layout, int N, int M, int Mpad, const uint32_t* conn,
+                                                     int layout, int N, int M, int Mpad, WParamMap wm,
+                                                     const uint32_t* conn,
                                                      const cplx<double>* v, double shift, int P,
                                                      cplx<double>* y, double* ypart) {
     __shared__ double sh[4];
@@ -603,18 +684,27 @@ __global__ __launch_bounds__(256) void spin_y_kernel(const float* partial, int c
             col = 2 * (k - N);
         } else {
             const int q = k - N - M;
-            row = int(conn[q]);
+            row = wm.shift >= 0 ? wm.site(q / M, q % M) : int(conn[q]);
             col = 2 * (q % M);
         }
         const int64_t off = int64_t(row) * ncols + col;
-        double yr = 0.0, yi = 0.0;
-#pragma unroll 8
-        for (int c = 0; c < chunks; c++) {
-            const float2 x = *reinterpret_cast<const float2*>(partial + c * n + off);
-            yr += double(x.x);
-            yi += double(x.y);
-        }
         const cplx<double> vk = v[k];
+        double yr = 0.0, yi = 0.0;
+        for (int c0 = 0; c0 < chunks; c0 += kYBatch) {
+            float2 x[kYBatch];
+#pragma unroll
+            for (int i = 0; i < kYBatch; i++) {
+                const int c = c0 + i < chunks ? c0 + i : chunks - 1;
+                x[i] = *reinterpret_cast<const float2*>(partial + c * n + off);
+            }
+#pragma unroll
+            for (int i = 0; i < kYBatch; i++) {
+                if (c0 + i < chunks) {
+                    yr += double(x[i].x);
+                    yi += double(x[i].y);
+                }
+            }
+        }
         yr += shift * vk.re;
         yi += shift * vk.im;
         y[k] = {yr, yi};
@@ -713,8 +803,8 @@ void launch_spin_gemm32(const DenseView<float>& net, const uint64_t* bits, const
     }
 }
 
-void launch_sr_pack(const DenseView<float>& net, const int* inv, const cplx<double>* v, const CgDir& dir,
-                    const cplx<double>* obar, SrWorkspace ws, hipStream_t stream) {
+void launch_sr_pack(const DenseView<float>& net, const WParamMap& wm, const int* inv, const cplx<double>* v,
+                    const CgDir& dir, const cplx<double>* obar, SrWorkspace ws, hipStream_t stream) {
     const int KT = sr_kt32(net.N), NT = 2 * net.Mpad / 16;
     if (KT > 8) throw std::invalid_argument("configurations of 1..256 sites are supported");
     if (dir.z && (!dir.p_old || !dir.p_new || !dir.newpart || !dir.oldpart || !dir.done || dir.parts < 1 ||
@@ -724,7 +814,7 @@ void launch_sr_pack(const DenseView<float>& net, const int* inv, const cplx<doub
     const int pack_blocks = int((npack + 255) / 256);
     const int vab_blocks = (net.Mpad + net.N + 255) / 256;
     unpack_pack_kernel<<<unsigned(pack_blocks + vab_blocks + kDotParts), 256, 0, stream>>>(
-        net, inv, v, dir, KT, NT, pack_blocks, vab_blocks, ws.frag, ws.va, ws.vb, obar, ws.opart);
+        net, wm, inv, v, dir, KT, NT, pack_blocks, vab_blocks, ws.frag, ws.va, ws.vb, obar, ws.opart);
     ANN_LAUNCH_CHECK();
 }
 
@@ -763,10 +853,10 @@ int sr_ov_pairs(int M) { return (M + 15) / 16; }
 
 size_t sr_frag_elems(int N, int Mpad) { return size_t(3) * sr_kt32(N) * (2 * Mpad / 16) * 64 * 8; }
 
-void launch_sr_y_bf16(const DenseView<float>& net, const cplx<double>* v, double shift, const SpinPlan& plan,
-                      SrWorkspace ws, cplx<double>* y, hipStream_t stream) {
+void launch_sr_y_bf16(const DenseView<float>& net, const WParamMap& wm, const cplx<double>* v, double shift,
+                      const SpinPlan& plan, SrWorkspace ws, cplx<double>* y, hipStream_t stream) {
     spin_y_kernel<<<unsigned(sr_y_parts(net.num_params)), 256, 0, stream>>>(
-        ws.spin_part, plan.chunks, plan.n, plan.ncols, net.layout, net.N, net.M, net.Mpad, net.conn, v, shift,
+        ws.spin_part, plan.chunks, plan.n, plan.ncols, net.layout, net.N, net.M, net.Mpad, wm, net.conn, v, shift,
         net.num_params, y, ws.ypart);
     ANN_LAUNCH_CHECK();
 }

@@ -119,7 +119,7 @@ def cpu_baseline(A, L, alpha, seed, seconds, sweeps):
 
 
 KERNELS = ("mc_sweep", "block_sweep", "theta", "local_energy", "o_k", "spin_sums", "sr_pack", "sr_ov", "sr_finish",
-           "spin_gemm", "sr_y", "kfac_setup", "kfac_apply")
+           "spin_gemm", "sr_y", "kfac_setup", "kfac_step")
 
 
 class Stepper:

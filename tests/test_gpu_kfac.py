@@ -106,6 +106,23 @@ def test_kfac_cg_converges_faster_than_jacobi(precision):
         assert np.linalg.norm(r) <= 1.05e-2 * np.linalg.norm(F)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_kfac_cg_general_connections(precision):
+    """PsiDeep with shuffled (non-circulant, distinct) connections: the K-FAC step reads
+    the index table; the solution equals the dense solve with the library's S"""
+    from test_gpu_parity import shuffled_deep
+    psi = shuffled_deep(24, 48, 10, 21, repeated=False)
+    psi.precision = precision
+    ens = A.MonteCarloSpins(2048, 2, 10, 512, seed=4)
+    tdvp = A.TDVP(psi.num_params)
+    tdvp.eval(A.Operator(ham.heisenberg_chain(24)), psi, ens)
+    S, F = tdvp.S_matrix, tdvp.F_vector
+    shift = 1e-2
+    x_ref = np.linalg.solve(S + shift * np.eye(psi.num_params), F)
+    x = tdvp.solve(shift, "cg_kfac", rtol=1e-10 if precision == "fp64" else 1e-6, max_iter=3000)
+    assert rel(x, x_ref) < (1e-7 if precision == "fp64" else 1e-3)
+
+
 def test_kfac_rejects_generic_nets():
     psi = A.seeded_cnn([8], [(2, [3]), (1, [3])], 1, sigma=0.1)
     ens = A.ExactSummationSpins(8)

@@ -244,3 +244,34 @@ def test_structured_matvec_matches_dense_gram(precision):
     x = tdvp.solve(1e-3, "cg", rtol=1e-5, max_iter=3000)
     assert rel((S + 1e-3 * np.eye(psi.num_params)) @ x, tdvp.F_vector) < 2e-5
     assert tdvp.last_solve["iterations"] < 3000
+
+
+def shuffled_deep(N, M, C, seed, repeated):
+    """one-layer PsiDeep whose connections are not the circulant 1d layout: per unit a
+    random choice of C sites, distinct (repeated=False) or with repeats (several
+    connections of a unit on one site, repeated=True)"""
+    rng = np.random.default_rng(seed)
+    conn = np.stack([rng.choice(N, C, replace=repeated) for _ in range(M)], axis=1).astype(np.uint32)
+    a, b = cgauss(rng, N, 0.05), cgauss(rng, M, 0.05)
+    W = cgauss(rng, (C, M), 0.05)
+    return A.PsiDeep(N, a, [b], [conn], [W], np.ones(M), 0.0)
+
+
+@pytest.mark.parametrize("repeated", [False, True])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_structured_matvec_general_connections(precision, repeated):
+    """S.v through the connection tables (a PsiDeep whose connections are not the
+    circulant layout) equals the dense Gram matrix applied to v"""
+    N, M, C = 32, 64, 12
+    psi = shuffled_deep(N, M, C, 17, repeated)
+    psi.precision = precision
+    ens = A.MonteCarloSpins(1024, 2, 5, 256, move="exchange", seed=8)
+    tdvp = A.TDVP(psi.num_params)
+    tdvp.eval(A.Operator(A.hamiltonians.heisenberg_chain(N)), psi, ens)
+    S = tdvp.S_matrix
+    rng = np.random.default_rng(2)
+    for _ in range(2):
+        v = cgauss(rng, psi.num_params, 1.0)
+        assert rel(tdvp.S_dot_vector(v), S @ v) < (1e-10 if precision == "fp64" else 2e-5)
+    x = tdvp.solve(1e-3, "cg", rtol=1e-6, max_iter=3000)
+    assert rel((S + 1e-3 * np.eye(psi.num_params)) @ x, tdvp.F_vector) < 1e-5
