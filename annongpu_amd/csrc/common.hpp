@@ -264,5 +264,11 @@ __device__ inline int select64(uint64_t x, uint32_t rank) {
 }
 
 __device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// a load from a wave-uniform address of read-only data through the constant address
+// space (the backend emits a scalar load)
+template <typename T>
+__device__ inline T ld_uniform(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)(p);
+}
 
 }  // namespace ann

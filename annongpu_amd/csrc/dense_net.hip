@@ -314,19 +314,27 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void local_energy_k
         }
         pend = 0;
     };
-    for (int g = 0; g < op.num_groups; g++) {
-        double c_re = 0.0, c_im = 0.0;
-        const int b0 = op.group_begin[g], b1 = op.group_begin[g + 1];
-        for (int n = b0; n < b1; n++) {
+    // coefficient of group g at this configuration (the operator tables are read as
+    // wave-uniform constants: scalar loads, which do not wait behind the row loads)
+    auto group_coef = [&](int g, double& c_re, double& c_im) {   // (g, n: wave-uniform)
+        c_re = 0.0;
+        c_im = 0.0;
+        g = uniform_int(g);
+        const int b0 = ld_uniform(op.group_begin + g), b1 = ld_uniform(op.group_begin + g + 1);
+        for (int n = uniform_int(b0); n < b1; n++) {
             int par = 0;
-            for (int w = 0; w < words; w++) par += __popcll(~conf.w[w] & op.string_zy[int64_t(n) * words + w]);
+            for (int w = 0; w < words; w++) par += __popcll(~conf.w[w] & ld_uniform(op.string_zy + n * words + w));
             const double sg = (par & 1) ? -1.0 : 1.0;
-            c_re += sg * op.string_coef[2 * n];
-            c_im += sg * op.string_coef[2 * n + 1];
+            c_re += sg * ld_uniform(op.string_coef + 2 * n);
+            c_im += sg * ld_uniform(op.string_coef + 2 * n + 1);
         }
+    };
+    for (int g = 0; g < op.num_groups; g++) {
+        double c_re, c_im;
+        group_coef(g, c_re, c_im);
         if (c_re == 0.0 && c_im == 0.0) continue;
         bool diag = true;
-        for (int w = 0; w < words; w++) diag = diag && op.group_flip[int64_t(g) * words + w] == 0;
+        for (int w = 0; w < words; w++) diag = diag && ld_uniform(op.group_flip + g * words + w) == 0;
         if (diag) {
             e_re += c_re;
             e_im += c_im;
@@ -336,7 +344,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void local_energy_k
 #pragma unroll
         for (int u = 0; u < U; u++) tn[u] = th[u];
         for (int w = 0; w < words; w++) {
-            uint64_t m = op.group_flip[int64_t(g) * words + w];
+            uint64_t m = ld_uniform(op.group_flip + g * words + w);
             while (m) {
                 const int i = w * 64 + __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;

@@ -45,6 +45,42 @@ __device__ inline void conf_flip(uint64_t (&w)[WORDS], int i) {
     for (int k = 0; k < WORDS; k++) w[k] ^= (i >> 6) == k ? uint64_t(1) << (i & 63) : 0;
 }
 
+// tanh(x) = 1 - 2 / (1 + e^{2x}) from the hardware exp and reciprocal (about 1e-6
+// relative): the proposal only has to be a deterministic function of the spins
+// outside the block, the Metropolis-Hastings ratio uses the same logits both ways
+__device__ inline float tanh_hw(float x) {
+    const float e = __expf(2.f * x);
+    return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + e);
+}
+
+// the 8 logit partials of every lane summed over the wave: lane l ends with the
+// total of logit (l >> 3) & 7.  Lanes l and l + 32 exchange halves (permlane32
+// swap), then rows 2r and 2r + 1 (permlane16 swap), then lanes l and l ^ 8 (DPP
+// row_ror:8); the last three levels are DPP sums inside groups of 8 lanes.
+__device__ inline float logits8_reduce(const float (&part)[kMaxBlock]) {
+    static_assert(kMaxBlock == 8, "eight logits per block");
+    const int lane = threadIdx.x & 63;
+    float s4[4], s2[2];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(part[i]), __float_as_uint(part[4 + i]), false,
+                                                        false);
+        s4[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);   // logit 4 (l >> 5) + i
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(s4[i]), __float_as_uint(s4[2 + i]), false,
+                                                        false);
+        s2[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);   // logit 4 (l >> 5) + 2 ((l >> 4) & 1) + i
+    }
+    const bool b3 = (lane >> 3) & 1;
+    float s = (b3 ? s2[1] : s2[0]) + dpp_f32<0x128>(b3 ? s2[0] : s2[1]);
+    s += dpp_f32<0x141>(s);   // row_half_mirror: lanes i and 7 - i of 8
+    s += dpp_f32<0xB1>(s);    // quad_perm [1, 0, 3, 2]
+    s += dpp_f32<0x4E>(s);    // quad_perm [2, 3, 0, 1]
+    return s;
+}
+
 // ------------------------------------------------------------- block sweep
 template <typename T, int U, int WORDS>
 __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, BlockNet bn, uint64_t* chain_confs,
@@ -114,32 +150,39 @@ __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, B
                     a -= (p_old >> j) & 1u ? wv : -wv;
                 }
             }
-            hd[qq] = tanhf(float(a) * (1.0f / kFixScale));
+            hd[qq] = tanh_hw(float(a) * (1.0f / kFixScale));
         }
-        // logits of the block's sites
-        float lg[kMaxBlock];
+        // logits of the block's sites: lane partials, one transposed wave reduction
+        float part[kMaxBlock];
 #pragma unroll
         for (int j = 0; j < kMaxBlock; j++) {
-            lg[j] = 0.f;
+            part[j] = 0.f;
             if (j < bs) {
                 const uint16_t* row = bn.W2h + int64_t(st[j]) * kHPad + lane;
-                float part = 0.f;
 #pragma unroll
-                for (int qq = 0; qq < 4; qq++) part += bf16_to_f32(row[64 * qq]) * hd[qq];
-                lg[j] = wave_sum(part) + bn.b2[st[j]];
+                for (int qq = 0; qq < 4; qq++) part[j] += bf16_to_f32(row[64 * qq]) * hd[qq];
             }
         }
-        // patterns 4 lane + r with k up spins; weights exp(e_p - e_max)
+        const float lsum8 = logits8_reduce(part);
+        float lg[kMaxBlock];
+#pragma unroll
+        for (int j = 0; j < kMaxBlock; j++)
+            lg[j] = j < bs ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lsum8), 8 * j)) + bn.b2[st[j]]
+                           : 0.f;
+        // patterns 4 lane + r with k up spins; weights exp(e_p - e_max).  Bits 2..7 of
+        // the pattern are the lane's bits 0..5: their energy is shared by the 4 patterns.
         float e[4], wr[4];
         bool ok[4];
         float emax_l = -INFINITY;
+        float e_hi = 0.f;
+#pragma unroll
+        for (int j = 2; j < kMaxBlock; j++) e_hi += (lane >> (j - 2)) & 1 ? lg[j] : 0.f;
+        const int pc_hi = __popc(uint32_t(lane));
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const uint32_t p = uint32_t(4 * lane + r);
-            ok[r] = p < (1u << bs) && __popc(p) == k;
-            float s = 0.f;
-#pragma unroll
-            for (int j = 0; j < kMaxBlock; j++) s += (p >> j) & 1u ? lg[j] : 0.f;
+            ok[r] = p < (1u << bs) && pc_hi + __popc(uint32_t(r)) == k;
+            const float s = e_hi + ((r & 1) ? lg[0] : 0.f) + ((r & 2) ? lg[1] : 0.f);
             e[r] = s;
             if (ok[r]) emax_l = fmaxf(emax_l, s);
         }
