@@ -58,93 +58,171 @@ __global__ __launch_bounds__(256) void kfac_moments_spin_kernel(const uint64_t* 
     }
 }
 
-// raw g moments of one group of 32 samples and one block of 64 units:
-// part[gi][b] = (sum w conj(g_j) g_k [64][64], sum w g_j [64])
-__global__ __launch_bounds__(256) void kfac_moments_g_kernel(const cplx<float>* g32, const cplx<double>* g64,
-                                                             int Mpad, const double* w, int64_t nsub, int M,
-                                                             int nblk, cplx<double>* part) {
-    __shared__ cplx<double> sg[kGroup][kDBlock];
-    __shared__ double sw[kGroup];
-    const int gi = blockIdx.x, b = blockIdx.y;
-    const int64_t s0 = int64_t(gi) * kGroup;
-    for (int e = threadIdx.x; e < kGroup * kDBlock; e += blockDim.x) {
-        const int s = e / kDBlock, jj = e % kDBlock;
-        const int j = b * kDBlock + jj;
-        cplx<double> x = {0.0, 0.0};
-        if (s0 + s < nsub && j < M) {
-            const int64_t o = (s0 + s) * Mpad + j;
-            x = g32 ? cplx<double>{double(g32[o].re), double(g32[o].im)} : g64[o];
-        }
-        sg[s][jj] = x;
+// spin covariance from the transposed spin bits when every sample has the same weight
+// (Monte Carlo): over the first nsub samples, sum_s s_v s_u = nsub - 2 #(s_v != s_u)
+// and sum_s s_v = 2 #up_v - nsub, counted 64 samples at a time by popcount.  Block v,
+// lane u; bits[blk][vpad] (bit b = spin of sample 64 blk + b up, zero beyond count).
+__global__ __launch_bounds__(kKfacMaxSites) void kfac_c_popcount_kernel(const uint64_t* bits, int vpad,
+                                                                      int64_t nsub, int N, double* C) {
+    const int v = blockIdx.x, u = threadIdx.x;
+    const int uc = u < N ? u : N - 1;
+    const int64_t blocks = (nsub + 63) / 64;
+    const uint64_t tail = (nsub & 63) ? (uint64_t(1) << (nsub & 63)) - 1 : ~uint64_t(0);
+    int64_t diff = 0, up_u = 0, up_v = 0;
+#pragma unroll 16
+    for (int64_t blk = 0; blk < blocks; blk++) {
+        const uint64_t m = blk == blocks - 1 ? tail : ~uint64_t(0);
+        const uint64_t bv = bits[blk * vpad + v] & m, bu = bits[blk * vpad + uc] & m;
+        diff += __popcll(bv ^ bu);
+        up_u += __popcll(bu);
+        up_v += __popcll(bv);
     }
-    for (int s = threadIdx.x; s < kGroup; s += blockDim.x) sw[s] = s0 + s < nsub ? w[s0 + s] : 0.0;
-    __syncthreads();
-    cplx<double>* out = part + (int64_t(gi) * nblk + b) * (kDBlock * kDBlock + kDBlock);
-    for (int e = threadIdx.x; e < kDBlock * kDBlock + kDBlock; e += blockDim.x) {
-        double re = 0.0, im = 0.0;
-        if (e < kDBlock * kDBlock) {
-            const int jj = e / kDBlock, kk = e % kDBlock;
-            for (int s = 0; s < kGroup; s++) {
-                const cplx<double> x = sg[s][jj], y = sg[s][kk];
-                re += sw[s] * (x.re * y.re + x.im * y.im);   // conj(x) y
-                im += sw[s] * (x.re * y.im - x.im * y.re);
-            }
-        } else {
-            const int jj = e - kDBlock * kDBlock;
-            for (int s = 0; s < kGroup; s++) {
-                re += sw[s] * sg[s][jj].re;
-                im += sw[s] * sg[s][jj].im;
-            }
-        }
-        out[e] = {re, im};
+    if (u < N) {
+        const double n = double(nsub);
+        const double mv = (2.0 * double(up_v) - n) / n, mu = (2.0 * double(up_u) - n) / n;
+        C[int64_t(v) * N + u] = (n - 2.0 * double(diff)) / n - mv * mu;
     }
 }
 
-// sums of the group moments (group order) and centring:
-// C = sum w s s^T / W - m m^T, D block b = sum w conj(g) g^T / W - conj(mg) mg^T
-__global__ void kfac_reduce_kernel(const double* spart, const cplx<double>* gpart, int groups, int N, int M,
-                                   int nblk, double* C, cplx<double>* Dblk) {
+// raw g moments of one group of kGroupG samples and one block of 64 units, staged
+// through LDS kChunkG samples at a time:
+// part[gi][b] = (sum w conj(g_j) g_k [64][64], sum w g_j [64], sum w)
+constexpr int kGroupG = 128;
+constexpr int kChunkG = 32;
+constexpr int kGOut = kDBlock * kDBlock + kDBlock + 1;
+__global__ __launch_bounds__(256) void kfac_moments_g_kernel(const cplx<float>* g32, const cplx<double>* g64,
+                                                             int Mpad, const double* w, int64_t nsub, int M,
+                                                             int nblk, cplx<double>* part) {
+    __shared__ cplx<double> sg[kChunkG][kDBlock];
+    __shared__ double sw[kChunkG];
+    constexpr int kPer = kDBlock * kDBlock / 256 + 1;   // 16 products + 1 (mean or weight)
+    static_assert(kDBlock == 64, "256 threads cover a 64 x 64 block in 16 rounds");
+    const int gi = blockIdx.x, b = blockIdx.y;
+    double ar[kPer], ai[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; i++) ar[i] = ai[i] = 0.0;
+    for (int c0 = 0; c0 < kGroupG; c0 += kChunkG) {
+        const int64_t s0 = int64_t(gi) * kGroupG + c0;
+        if (s0 >= nsub) break;
+        __syncthreads();   // the previous chunk is consumed
+        for (int e = threadIdx.x; e < kChunkG * kDBlock; e += blockDim.x) {
+            const int s = e / kDBlock, jj = e % kDBlock;
+            const int j = b * kDBlock + jj;
+            cplx<double> x = {0.0, 0.0};
+            if (s0 + s < nsub && j < M) {
+                const int64_t o = (s0 + s) * Mpad + j;
+                x = g32 ? cplx<double>{double(g32[o].re), double(g32[o].im)} : g64[o];
+            }
+            sg[s][jj] = x;
+        }
+        for (int s = threadIdx.x; s < kChunkG; s += blockDim.x) sw[s] = s0 + s < nsub ? w[s0 + s] : 0.0;
+        __syncthreads();
+        // samples outer (not unrolled), the thread's outputs inner: output i of thread t
+        // is (jj, kk) = (t / 64 + 4 i, t % 64) for i < 16, then the mean of unit t
+        // (t < 64) and the weight sum (t = 64)
+        const int t = threadIdx.x;
+#pragma unroll 1
+        for (int s = 0; s < kChunkG; s++) {
+            const double ws = sw[s];
+            const cplx<double> y = sg[s][t & 63];
+#pragma unroll
+            for (int i = 0; i < kPer - 1; i++) {
+                const cplx<double> x = sg[s][(t >> 6) + 4 * i];
+                ar[i] += ws * (x.re * y.re + x.im * y.im);   // conj(x) y
+                ai[i] += ws * (x.re * y.im - x.im * y.re);
+            }
+            if (t < kDBlock) {
+                ar[kPer - 1] += ws * y.re;
+                ai[kPer - 1] += ws * y.im;
+            } else if (t == kDBlock) {
+                ar[kPer - 1] += ws;
+            }
+        }
+    }
+    cplx<double>* out = part + (int64_t(gi) * nblk + b) * kGOut;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < kPer - 1; i++) out[((t >> 6) + 4 * i) * kDBlock + (t & 63)] = {ar[i], ai[i]};
+    if (t <= kDBlock) out[kDBlock * kDBlock + t] = {ar[kPer - 1], ai[kPer - 1]};
+}
+
+// sums of the group moments and centring, C = sum w s s^T / W - m m^T (when spart is
+// given; else C is already set) by one thread per entry, D block b = sum w conj(g) g^T
+// / W - conj(mg) mg^T by a quad of lanes per entry (lane q of the quad adds the groups
+// q, q + 4, ..., the quad sum is taken in a fixed order)
+__device__ inline double quad_sum_kf(double v) {
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    return v;
+}
+
+__global__ void kfac_reduce_kernel(const double* spart, int sgroups, const cplx<double>* gpart, int ggroups, int N,
+                                   int M, int nblk, double* C, cplx<double>* Dblk) {
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t ns = int64_t(N) * N + N + 1, ng = int64_t(kDBlock) * kDBlock + kDBlock;
-    double W = 0.0;
+    const int64_t nc = spart ? (int64_t(N) * N + 3) / 4 * 4 : 0;   // C entries, rounded to whole quads
+    if (e < nc) {
+        if (e >= int64_t(N) * N) return;
+        const int64_t ns = int64_t(N) * N + N + 1;
+        double W = 0.0;
 #pragma unroll 8
-    for (int gi = 0; gi < groups; gi++) W += spart[gi * ns + ns - 1];
-    const double iw = W > 0.0 ? 1.0 / W : 0.0;
-    if (e < int64_t(N) * N) {
+        for (int gi = 0; gi < sgroups; gi++) W += spart[gi * ns + ns - 1];
+        const double iw = W > 0.0 ? 1.0 / W : 0.0;
         const int v = int(e / N), u = int(e % N);
         double s2 = 0.0, mv = 0.0, mu = 0.0;
 #pragma unroll 8
-        for (int gi = 0; gi < groups; gi++) {
+        for (int gi = 0; gi < sgroups; gi++) {
             s2 += spart[gi * ns + e];
             mv += spart[gi * ns + int64_t(N) * N + v];
             mu += spart[gi * ns + int64_t(N) * N + u];
         }
         C[e] = s2 * iw - (mv * iw) * (mu * iw);
-    } else if (e < int64_t(N) * N + int64_t(nblk) * kDBlock * kDBlock) {
-        const int64_t f = e - int64_t(N) * N;
-        const int b = int(f / (kDBlock * kDBlock));
-        const int jj = int(f % (kDBlock * kDBlock)) / kDBlock, kk = int(f % kDBlock);
-        cplx<double> out = {0.0, 0.0};
-        if (b * kDBlock + jj < M && b * kDBlock + kk < M) {
-            double re = 0.0, im = 0.0, ajr = 0.0, aji = 0.0, akr = 0.0, aki = 0.0;
+        return;
+    }
+    // D entries: 4 lanes each, whole quads (the block size is a multiple of 4)
+    const int64_t qd = (e - nc) >> 2;
+    const int q = int((e - nc) & 3);
+    const int64_t nd = int64_t(nblk) * kDBlock * kDBlock;
+    if (qd >= nd) return;   // (a whole quad leaves together)
+    const int b = int(qd / (kDBlock * kDBlock));
+    const int jj = int(qd % (kDBlock * kDBlock)) / kDBlock, kk = int(qd % kDBlock);
+    const bool ok = b * kDBlock + jj < M && b * kDBlock + kk < M;
+    double re = 0.0, im = 0.0, ajr = 0.0, aji = 0.0, akr = 0.0, aki = 0.0, W = 0.0;
 #pragma unroll 8
-            for (int gi = 0; gi < groups; gi++) {
-                const cplx<double>* q = gpart + (int64_t(gi) * nblk + b) * ng;
-                re += q[jj * kDBlock + kk].re;
-                im += q[jj * kDBlock + kk].im;
-                ajr += q[kDBlock * kDBlock + jj].re;
-                aji += q[kDBlock * kDBlock + jj].im;
-                akr += q[kDBlock * kDBlock + kk].re;
-                aki += q[kDBlock * kDBlock + kk].im;
-            }
+    for (int gi = q; gi < ggroups; gi += 4) {
+        const cplx<double>* g = gpart + (int64_t(gi) * nblk + b) * kGOut;
+        const cplx<double> x = g[jj * kDBlock + kk], mj = g[kDBlock * kDBlock + jj], mk = g[kDBlock * kDBlock + kk];
+        re += x.re;
+        im += x.im;
+        ajr += mj.re;
+        aji += mj.im;
+        akr += mk.re;
+        aki += mk.im;
+        W += g[kGOut - 1].re;
+    }
+    re = quad_sum_kf(re);
+    im = quad_sum_kf(im);
+    ajr = quad_sum_kf(ajr);
+    aji = quad_sum_kf(aji);
+    akr = quad_sum_kf(akr);
+    aki = quad_sum_kf(aki);
+    W = quad_sum_kf(W);
+    if (q == 0) {
+        cplx<double> out = {0.0, 0.0};
+        if (ok) {
+            const double iw = W > 0.0 ? 1.0 / W : 0.0;
             ajr *= iw; aji *= iw; akr *= iw; aki *= iw;
             out = {re * iw - (ajr * akr + aji * aki), im * iw - (ajr * aki - aji * akr)};
         }
-        Dblk[f] = out;
+        Dblk[qd] = out;
     }
 }
 
-// damping split from the traces: gam[0] = gc, gam[1] = gd (one block)
+// damping split from the traces: gam[0] = gc, gam[1] = gd (one block).  The product
+// gc gd = kKfacDamping^2 shift is below the shift of the system: on an equilibrated
+// 16,384-sample Heisenberg 10x10 problem (M = 400, shift 1e-3) the iterations to
+// 1e-2 were 128 / 107 / 78 / 73 / 62 / 63 for multipliers 10 / 3 / 1 / 0.5 / 0.3 / 0.1
+// with factors from all samples (96 with 2,048 samples and multiplier 1)
+constexpr double kKfacDamping = 0.3;
 __global__ void kfac_damping_kernel(const double* C, int N, const cplx<double>* Dblk, int M, double shift,
                                     double* gam) {
     __shared__ double sh[2][256];
@@ -167,8 +245,8 @@ __global__ void kfac_damping_kernel(const double* C, int N, const cplx<double>* 
     if (threadIdx.x == 0) {
         const double ac = sh[0][0] / N, ad = sh[1][0] / M;
         const double pi = ac > 0.0 && ad > 0.0 ? sqrt(ac / ad) : 1.0;
-        gam[0] = sqrt(shift) * pi;
-        gam[1] = sqrt(shift) / pi;
+        gam[0] = kKfacDamping * sqrt(shift) * pi;
+        gam[1] = kKfacDamping * sqrt(shift) / pi;
     }
 }
 
@@ -511,22 +589,33 @@ __global__ __launch_bounds__(kStepThreads) void kfac_step_kernel(KfacStep a, int
 
 }  // namespace
 
-void launch_kfac_setup(const uint64_t* confs, int words, const cplx<float>* g32, const cplx<double>* g64, int Mpad,
-                       const double* w, int64_t count, int N, int M, double shift, KfacWork& kw,
-                       hipStream_t stream) {
+void launch_kfac_setup(const uint64_t* confs, int words, const uint64_t* bits, int vpad, bool w_uniform,
+                       const cplx<float>* g32, const cplx<double>* g64, int Mpad, const double* w, int64_t count, int N,
+                       int M, double shift, KfacWork& kw, hipStream_t stream) {
     if (N < 1 || N > kKfacMaxSites) throw std::invalid_argument("the K-FAC preconditioner supports 1..128 sites");
     if (M < 1 || M > kKfacMaxUnits) throw std::invalid_argument("the K-FAC preconditioner supports 1..2048 hidden units");
     const int64_t nsub = std::min<int64_t>(count, kKfacSamples);
-    const int groups = int((nsub + kGroup - 1) / kGroup);
     const int nblk = kfac_dblocks(M);
-    kfac_moments_spin_kernel<<<unsigned(groups), 256, 0, stream>>>(confs, words, w, nsub, N, kw.spart);
+    int sgroups = 0;
+    if (w_uniform) {
+        if (!bits || vpad < N) throw std::invalid_argument("the K-FAC spin moments need the transposed spin bits");
+        kfac_c_popcount_kernel<<<unsigned(N), kKfacMaxSites, 0, stream>>>(bits, vpad, nsub, N, kw.C);
+    } else {
+        sgroups = int((nsub + kGroup - 1) / kGroup);
+        kfac_moments_spin_kernel<<<unsigned(sgroups), 256, 0, stream>>>(confs, words, w, nsub, N, kw.spart);
+    }
     ANN_LAUNCH_CHECK();
-    kfac_moments_g_kernel<<<dim3(unsigned(groups), unsigned(nblk)), 256, 0, stream>>>(g32, g64, Mpad, w, nsub, M, nblk,
-                                                                                      kw.gpart);
+    const int ggroups = int((nsub + kGroupG - 1) / kGroupG);
+    kfac_moments_g_kernel<<<dim3(unsigned(ggroups), unsigned(nblk)), 256, 0, stream>>>(g32, g64, Mpad, w, nsub, M,
+                                                                                       nblk, kw.gpart);
     ANN_LAUNCH_CHECK();
-    const int64_t nout = int64_t(N) * N + int64_t(nblk) * kDBlock * kDBlock;
-    kfac_reduce_kernel<<<unsigned((nout + 255) / 256), 256, 0, stream>>>(kw.spart, kw.gpart, groups, N, M, nblk,
-                                                                         kw.C, kw.Dblk);
+    // C entries (general weights only) then 4 lanes per D entry; N * N is rounded up to
+    // a multiple of 4 so that every quad of D lanes is whole
+    const int64_t nc = w_uniform ? 0 : (int64_t(N) * N + 3) / 4 * 4;
+    const int64_t nout = nc + 4 * int64_t(nblk) * kDBlock * kDBlock;
+    kfac_reduce_kernel<<<unsigned((nout + 255) / 256), 256, 0, stream>>>(w_uniform ? nullptr : kw.spart, sgroups,
+                                                                         kw.gpart, ggroups, N, M, nblk, kw.C,
+                                                                         kw.Dblk);
     ANN_LAUNCH_CHECK();
     kfac_damping_kernel<<<1, 256, 0, stream>>>(kw.C, N, kw.Dblk, M, shift, kw.gam);
     ANN_LAUNCH_CHECK();
@@ -562,10 +651,13 @@ void launch_kfac_step(const KfacView& kv, const KfacStep& a, hipStream_t stream)
 
 int kfac_dblocks(int M) { return (M + kDBlock - 1) / kDBlock; }
 size_t kfac_dblock_elems(int M) { return size_t(kfac_dblocks(M)) * kDBlock * kDBlock; }
-int kfac_groups(int64_t count) { return int((std::min<int64_t>(count, kKfacSamples) + kGroup - 1) / kGroup); }
-size_t kfac_spart_elems(int N, int64_t count) { return size_t(kfac_groups(count)) * (size_t(N) * N + N + 1); }
+size_t kfac_spart_elems(int N, int64_t count) {
+    const int64_t groups = (std::min<int64_t>(count, kKfacSamples) + kGroup - 1) / kGroup;
+    return size_t(groups) * (size_t(N) * N + N + 1);
+}
 size_t kfac_gpart_elems(int M, int64_t count) {
-    return size_t(kfac_groups(count)) * kfac_dblocks(M) * (kDBlock * kDBlock + kDBlock);
+    const int64_t groups = (std::min<int64_t>(count, kKfacSamples) + kGroupG - 1) / kGroupG;
+    return size_t(groups) * kfac_dblocks(M) * kGOut;
 }
 
 }  // namespace ann

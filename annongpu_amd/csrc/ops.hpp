@@ -37,10 +37,11 @@ void launch_pcg_init(const cplx<double>* r, cplx<double>* z, cplx<double>* p, co
                      int64_t n, int64_t jac_end, double* part, hipStream_t stream);
 
 // K-FAC preconditioner of the W parameters of a dense net (kfac.hip): factors from the
-// first kKfacSamples samples (moments per group of 32 samples, then a fixed-order
-// reduction), inverted once per solve; the step kernel takes the CG update and the
+// first kKfacSamples samples (spin covariance by popcount for equally weighted
+// samples, else moments per group of 32 samples; g moments per group of 128 samples;
+// fixed-order reductions), inverted once per solve; the step kernel takes the CG update and the
 // preconditioning of every parameter (Jacobi below jac_end, K-FAC on the W part)
-constexpr int64_t kKfacSamples = 2048;
+constexpr int64_t kKfacSamples = 16384;
 constexpr int kKfacMaxSites = 128;
 constexpr int kKfacMaxUnits = 2048;
 struct KfacWork {
@@ -81,9 +82,11 @@ int kfac_dblocks(int M);
 size_t kfac_dblock_elems(int M);
 size_t kfac_spart_elems(int N, int64_t count);
 size_t kfac_gpart_elems(int M, int64_t count);
-void launch_kfac_setup(const uint64_t* confs, int words, const cplx<float>* g32, const cplx<double>* g64, int Mpad,
-                       const double* w, int64_t count, int N, int M, double shift, KfacWork& kw,
-                       hipStream_t stream);
+// bits: the transposed spin bits (launch_spin_bits, vpad rows), read when w_uniform
+// (every sample weighs the same: the spin covariance is counted by popcount)
+void launch_kfac_setup(const uint64_t* confs, int words, const uint64_t* bits, int vpad, bool w_uniform,
+                       const cplx<float>* g32, const cplx<double>* g64, int Mpad, const double* w, int64_t count, int N,
+                       int M, double shift, KfacWork& kw, hipStream_t stream);
 // idx of KfacView: RBM v M + j, PsiDeep N + M + inv[v][j] M + j
 void launch_kfac_index(int layout, int N, int M, int Mpad, const int* inv, int* idx, hipStream_t stream);
 void launch_kfac_step(const KfacView& kv, const KfacStep& a, hipStream_t stream);

@@ -406,6 +406,7 @@ static void require_dense(const Psi& psi) {
 
 static void prepare_samples(Psi& psi, Samples& s, int64_t count) {
     s.count = count;
+    s.w_uniform = false;
     s.words = psi.words;
     s.Mpad = psi.dense() ? psi.Mpad : 0;
     s.precision = psi.precision;
@@ -516,6 +517,7 @@ static void sample_cnn(Psi& psi, Ensemble& ens) {
                                      hipMemcpyDeviceToDevice, S()));
     }
     launch_fill(s.w.p, 1.0 / double(ens.num_samples), count, S());
+    s.w_uniform = true;
     log_psi_of(psi, s.confs.p, count, s, nullptr);
     finish_chains(psi, ens, therm, steps, int64_t(ens.num_sweeps) * psi.N);
 }
@@ -596,6 +598,7 @@ void sample(Psi& psi, Ensemble& ens) {
     chain_view.th64.p = nullptr;
     chain_view.th32.p = nullptr;
     launch_fill(s.w.p, 1.0 / double(ens.num_samples), count, S());
+    s.w_uniform = true;
     const cplx<double> lp = {psi.log_prefactor.real(), psi.log_prefactor.imag()};
     if (psi.precision == 0)
         launch_net_log_psi<double>(psi.view64(), s.th64.p, count, lp, s.log_psi.p, nullptr, S());
@@ -746,6 +749,7 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     ANN_HIP_CHECK(hipMemcpyAsync(t.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice, S()));
     t.w.ensure(n);
     ANN_HIP_CHECK(hipMemcpyAsync(t.w.p, s.w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, S()));
+    t.w_uniform = s.w_uniform;
     t.has_samples = true;
     t.bits_valid = false;
     t.has_S = false;
@@ -839,6 +843,7 @@ void tdvp_eval_generic(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with
     t.count = n;
     t.w.ensure(n);
     ANN_HIP_CHECK(hipMemcpyAsync(t.w.p, s.w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, S()));
+    t.w_uniform = s.w_uniform;
     t.has_samples = true;
     t.has_S = false;
     if (with_S) {
@@ -1164,11 +1169,18 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         KfacWork kw{t.kf_spart.p, t.kf_gpart.p, t.kf_C.p, t.kf_Ci.p, t.kf_D.p, t.kf_gam.p};
         {
             ScopedTimer timer("kfac_setup");
+            // the transposed spin bits (shared with the fp32 S.v) give the spin covariance
+            const SpinPlan plan = spin_plan(t.N, t.M, t.Mpad, t.count, t.layout == LAYOUT_DEEP1);
+            if (!t.bits_valid) {
+                t.spin_bits.ensure(size_t(plan.bit_blocks) * plan.vpad);
+                launch_spin_bits(t.confs.p, t.words, t.count, t.N, plan.vpad, t.spin_bits.p, S());
+                t.bits_valid = true;
+            }
             launch_kfac_index(t.layout, t.N, t.M, t.Mpad, t.layout == LAYOUT_DEEP1 ? t.inv.p : nullptr, t.kf_idx.p,
                               S());
-            launch_kfac_setup(t.confs.p, t.words, t.precision != 0 ? t.g32.p : nullptr,
-                              t.precision != 0 ? nullptr : t.g64.p, t.Mpad, t.w.p, t.count, t.N, t.M, shift, kw,
-                              S());
+            launch_kfac_setup(t.confs.p, t.words, t.spin_bits.p, plan.vpad, t.w_uniform,
+                              t.precision != 0 ? t.g32.p : nullptr, t.precision != 0 ? nullptr : t.g64.p, t.Mpad,
+                              t.w.p, t.count, t.N, t.M, shift, kw, S());
         }
         jac_end = t.layout == LAYOUT_DEEP1 ? int64_t(t.N) + t.M : 0;
         kv = {t.N, t.M, t.kf_idx.p, t.wmap(), t.kf_Ci.p, t.kf_D.p};

@@ -191,6 +191,7 @@ struct Samples {
     int words = 0, Mpad = 0, precision = 0;
     DevBuf<uint64_t> confs;
     DevBuf<double> w;
+    bool w_uniform = false;      // every sample weighs 1 / count (Monte Carlo)
     DevBuf<cplx<double>> log_psi;
     DevBuf<cplx<float>> th32;
     DevBuf<cplx<double>> th64;
@@ -291,7 +292,8 @@ struct Tdvp {
     DevBuf<cplx<float>> u32;     // w (O.v - <O>.v) per sample (fp32 mode)
     DevBuf<float> spin_part;     // spin GEMM partials (fp32 mode)
     DevBuf<double> opart, ypart; // dot-product partials of the fused matvec
-    DevBuf<uint64_t> spin_bits;  // transposed spin bits of the samples (fp32 mode)
+    DevBuf<uint64_t> spin_bits;  // transposed spin bits of the samples (S.v in fp32 mode, K-FAC)
+    bool w_uniform = false;      // the samples' weights are all equal (Monte Carlo)
     bool bits_valid = false;
     int64_t matvecs = 0;         // S.v products so far (every 16th is timed when profiling)
     PinnedBuf<double> rr_host;   // CG residual partials and ||F||^2 (page-locked)

@@ -25,7 +25,11 @@ Protocol (all on the GPU, untimed unless stated):
      network (100 -> 256 -> 100, tanh; 2x4 / 4x2 plaquette blocks) is trained by
      the library's own forward / backward / Adam kernels;
   3. tau_int (Sokal window c = 5, batch means as a cross-check) of E_loc per sweep
-     on 1024 chains, for plain exchange sweeps and for learned block sweeps;
+     on 4096 chains x 400 sweeps after 300 plain thermalisation sweeps (the energy of
+     the pre-optimised state relaxes over ~100 sweeps from a random start), for plain
+     exchange sweeps and for learned block sweeps (the sweep count ceil(2 tau) is a
+     step function of the estimate, so it is taken on enough chains not to flip with
+     the noise);
   4. timed VMC steps of three paths on the same state:
        plain-config  : 100 exchange sweeps per sample (the workload's definition)
        plain-matched : ceil(2 tau_plain) exchange sweeps per sample
@@ -166,10 +170,19 @@ class Stepper:
         return dt / steps * 1e3, kms
 
 
-def measure_tau(A, psi, op, proposal, chains=1024, T=400, seed=23):
-    ens = A.MonteCarloSpins(chains * T, 1, 30, chains, move="exchange", seed=seed)
+THERM_SWEEPS = 300   # plain sweeps before a chain's first sample (also for learned paths)
+
+
+def measure_tau(A, psi, op, proposal, chains=4096, T=400, seed=23):
+    # 300 plain thermalisation sweeps: from a random S_z = 0 start the energy of this
+    # state relaxes over ~100 sweeps (a series started after 30 sweeps is a transient).
+    # The first pass over the persistent chains thermalises and is discarded; the series
+    # and the acceptance rate (which counts every proposal of a pass, thermalisation
+    # included, as the reference's does) come from the second pass.
+    ens = A.MonteCarloSpins(chains * T, 1, THERM_SWEEPS, chains, move="exchange", seed=seed, persistent=True)
     if proposal is not None:
         ens.set_proposal(proposal, plain_sweeps=0)
+    A.log_psi(psi, ens)
     el = A.local_energies(op, psi, ens).real.reshape(T, chains)
     return tau_int_sokal(el), tau_batch_means(el), ens.acceptance_rate, el
 
@@ -249,7 +262,7 @@ def main():
     for name, sweeps, proposal, tau in (("plain-config", args.sweeps, None, tau_p),
                                         ("plain-matched", s_match, None, tau_p),
                                         ("learned", s_learn, prop, tau_l)):
-        ens = A.MonteCarloSpins(B, sweeps, 30, B, move="exchange", seed=17, persistent=True)
+        ens = A.MonteCarloSpins(B, sweeps, THERM_SWEEPS, B, move="exchange", seed=17, persistent=True)
         if proposal is not None:
             ens.set_proposal(proposal, plain_sweeps=0)
         st = Stepper(A, psi.copy(), op, ens, args.lr, args.cg_iters, solver=args.solver, rtol=args.rtol,
