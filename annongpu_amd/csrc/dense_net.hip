@@ -295,7 +295,8 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void local_energy_k
         th[u] = theta[s * net.Mpad + lane + 64 * u];
         f[u] = net.f[lane + 64 * u];
     }
-    const cplx<T> L0 = wave_csum(lane_sum<T, U>(th, f));
+    const bool realf = net.real_f != 0;
+    const cplx<T> L0 = wave_csum(realf ? lane_sum_realf<T, U>(th, f) : lane_sum<T, U>(th, f));
     double e_re = 0.0, e_im = 0.0;
     // off-diagonal terms wait in lane `pend` (log-ratio and coefficient) until 64 are
     // collected; then every lane evaluates its own exp / sincos once
@@ -358,7 +359,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void local_energy_k
                 }
             }
         }
-        const cplx<T> L1 = wave_csum(lane_sum<T, U>(tn, f));
+        const cplx<T> L1 = wave_csum(realf ? lane_sum_realf<T, U>(tn, f) : lane_sum<T, U>(tn, f));
         if (lane == pend) {
             q_dre = T(double(L1.re) - double(L0.re));
             q_dim = T(double(L1.im) - double(L0.im));

@@ -75,6 +75,19 @@ __device__ inline cplx<T> lane_sum(const cplx<T> (&th)[U], const cplx<T> (&f)[U]
     return s;
 }
 
+// the same for real final weights (f[u].im = 0)
+template <typename T, int U>
+__device__ inline cplx<T> lane_sum_realf(const cplx<T> (&th)[U], const cplx<T> (&f)[U]) {
+    cplx<T> s = {0, 0};
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const cplx<T> l = logcosh0(th[u]);
+        s.re += f[u].re * l.re;
+        s.im += f[u].re * l.im;
+    }
+    return s;
+}
+
 template <typename T>
 __device__ inline cplx<T> wave_csum(cplx<T> v) {
     return {wave_sum(v.re), wave_sum(v.im)};
