@@ -125,9 +125,18 @@ void launch_structured_Ov(const DenseView<T>& net, const uint64_t* confs, int wo
 // weighted fp64 reductions
 // the spin GEMM of launch_spin_weighted_sums on bf16 matrix cores (fp32 accumulation
 // per sample chunk, chunks added in fp64)
+// ws (optional): the caller's transposed spin bits of the samples (launch_spin_bits with
+// the plan's vpad) and workspace of spin_sums_work sizes; without it the call builds the
+// bits and allocates its workspace on the stream
+struct SpinSumsWork {
+    const uint64_t* bits;
+    float* partial;          // plan.chunks * plan.n
+    double* full;            // plan.n
+    cplx<float>* c32;        // sr_count_pad(count)
+};
 void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, int words, const cplx<float>* g,
                            int mode, const cplx<double>* coef, int64_t count, cplx<double>* Yw, cplx<double>* Yb,
-                           cplx<double>* Ya, hipStream_t stream);
+                           cplx<double>* Ya, hipStream_t stream, const SpinSumsWork* ws = nullptr);
 
 // the spin GEMM's partition of the samples: npairs column pairs (16 hidden units
 // each, plus the coefficient columns when with_coef) x chunks of `chunk` samples x

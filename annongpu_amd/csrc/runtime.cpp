@@ -664,9 +664,21 @@ void structured_sums(Tdvp& t, const DenseView<T>& net, const uint64_t* confs, in
     t.Yb.ensure(net.Mpad);
     t.Ya.ensure(net.N);
     ScopedTimer timer("spin_sums");
-    if constexpr (sizeof(T) == 4)
-        launch_spin_sums_bf16(net, confs, words, g, conj_g, coef, count, t.Yw.p, t.Yb.p, t.Ya.p, S());
-    else
+    if constexpr (sizeof(T) == 4) {
+        // the transposed spin bits of this evaluation's samples are built once and
+        // shared with the S.v products and the K-FAC set-up; the workspace persists
+        const SpinPlan plan = spin_plan(net.N, net.M, net.Mpad, count, true);
+        if (!t.bits_valid) {
+            t.spin_bits.ensure(size_t(plan.bit_blocks) * plan.vpad);
+            launch_spin_bits(confs, words, count, net.N, plan.vpad, t.spin_bits.p, S());
+            t.bits_valid = true;
+        }
+        t.ss_partial.ensure(size_t(plan.chunks) * plan.n);
+        t.ss_full.ensure(size_t(plan.n));
+        t.ss_c32.ensure(size_t(sr_count_pad(count)));
+        const SpinSumsWork ws{t.spin_bits.p, t.ss_partial.p, t.ss_full.p, t.ss_c32.p};
+        launch_spin_sums_bf16(net, confs, words, g, conj_g, coef, count, t.Yw.p, t.Yb.p, t.Ya.p, S(), &ws);
+    } else
         launch_spin_weighted_sums<T>(net, confs, words, g, conj_g, coef, count, t.Yw.p, t.Yb.p, t.Ya.p, S());
     if (out)
         launch_pack_y(net.layout, net.N, net.M, net.Mpad, net.C, net.conn, t.Ya.p, t.Yb.p, t.Yw.p, v, shift,
@@ -676,6 +688,7 @@ void structured_sums(Tdvp& t, const DenseView<T>& net, const uint64_t* confs, in
 template <typename T>
 void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     sample(psi, ens);
+    t.bits_valid = false;   // new samples: their transposed spin bits are built on first use
     Samples& s = ens.samples;
     local_energies(psi, op, s);
     const int64_t n = s.count;
@@ -751,7 +764,6 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     ANN_HIP_CHECK(hipMemcpyAsync(t.w.p, s.w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, S()));
     t.w_uniform = s.w_uniform;
     t.has_samples = true;
-    t.bits_valid = false;
     t.has_S = false;
     if (with_S) {
         DevBuf<cplx<T>> O;

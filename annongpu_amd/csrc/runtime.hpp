@@ -291,6 +291,9 @@ struct Tdvp {
     DevBuf<cplx<float>> upart;   // per column pair partial O.v (fp32 mode)
     DevBuf<cplx<float>> u32;     // w (O.v - <O>.v) per sample (fp32 mode)
     DevBuf<float> spin_part;     // spin GEMM partials (fp32 mode)
+    DevBuf<float> ss_partial;    // workspace of the TDVP statistics' spin GEMM (fp32 mode)
+    DevBuf<double> ss_full;
+    DevBuf<cplx<float>> ss_c32;
     DevBuf<double> opart, ypart; // dot-product partials of the fused matvec
     DevBuf<uint64_t> spin_bits;  // transposed spin bits of the samples (S.v in fp32 mode, K-FAC)
     bool w_uniform = false;      // the samples' weights are all equal (Monte Carlo)

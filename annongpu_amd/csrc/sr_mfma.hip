@@ -863,20 +863,29 @@ void launch_sr_y_bf16(const DenseView<float>& net, const WParamMap& wm, const cp
 
 void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, int words, const cplx<float>* g,
                            int mode, const cplx<double>* coef, int64_t count, cplx<double>* Yw, cplx<double>* Yb,
-                           cplx<double>* Ya, hipStream_t stream) {
+                           cplx<double>* Ya, hipStream_t stream, const SpinSumsWork* ws) {
     if (count <= 0) return;
     const SpinPlan p = spin_plan(net.N, net.M, net.Mpad, count, true);
+    const int64_t pad = sr_count_pad(count);
     float* partial = nullptr;
     double* full = nullptr;
-    uint64_t* bits = nullptr;
+    uint64_t* own_bits = nullptr;
     cplx<float>* c32 = nullptr;
-    ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(float) * p.n * p.chunks, stream));
-    ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&full), sizeof(double) * p.n, stream));
-    ANN_HIP_CHECK(
-        hipMallocAsync(reinterpret_cast<void**>(&bits), sizeof(uint64_t) * p.bit_blocks * p.vpad, stream));
-    const int64_t pad = sr_count_pad(count);
-    ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&c32), sizeof(cplx<float>) * pad, stream));
-    launch_spin_bits(confs, words, count, net.N, p.vpad, bits, stream);
+    const uint64_t* bits = nullptr;
+    if (ws) {
+        partial = ws->partial;
+        full = ws->full;
+        c32 = ws->c32;
+        bits = ws->bits;
+    } else {
+        ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(float) * p.n * p.chunks, stream));
+        ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&full), sizeof(double) * p.n, stream));
+        ANN_HIP_CHECK(
+            hipMallocAsync(reinterpret_cast<void**>(&own_bits), sizeof(uint64_t) * p.bit_blocks * p.vpad, stream));
+        ANN_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&c32), sizeof(cplx<float>) * pad, stream));
+        launch_spin_bits(confs, words, count, net.N, p.vpad, own_bits, stream);
+        bits = own_bits;
+    }
     coef_f32_kernel<<<unsigned((pad + 255) / 256), 256, 0, stream>>>(coef, count, pad, reinterpret_cast<float2*>(c32));
     ANN_LAUNCH_CHECK();
     // columns of the hidden-unit padding are not computed: zero
@@ -887,10 +896,12 @@ void launch_spin_sums_bf16(const DenseView<float>& net, const uint64_t* confs, i
     const int64_t ns = int64_t(net.N + 1) * net.Mpad + net.N;
     split_y_kernel<<<unsigned((ns + 255) / 256), 256, 0, stream>>>(full, net.N, net.Mpad, p.ncols, Yw, Yb, Ya);
     ANN_LAUNCH_CHECK();
-    ANN_HIP_CHECK(hipFreeAsync(partial, stream));
-    ANN_HIP_CHECK(hipFreeAsync(full, stream));
-    ANN_HIP_CHECK(hipFreeAsync(bits, stream));
-    ANN_HIP_CHECK(hipFreeAsync(c32, stream));
+    if (!ws) {
+        ANN_HIP_CHECK(hipFreeAsync(partial, stream));
+        ANN_HIP_CHECK(hipFreeAsync(full, stream));
+        ANN_HIP_CHECK(hipFreeAsync(own_bits, stream));
+        ANN_HIP_CHECK(hipFreeAsync(c32, stream));
+    }
 }
 
 }  // namespace ann
