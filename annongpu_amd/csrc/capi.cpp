@@ -421,6 +421,7 @@ int ann_ensemble_num_steps(ann_ensemble ens, uint64_t* n) {
 int ann_ensemble_acceptance(ann_ensemble ens, uint64_t* accepted, uint64_t* rejected) {
     return guard([&] {
         need(ens, "ens");
+        ens->impl.settle_acceptance();
         if (accepted) *accepted = ens->impl.accepted;
         if (rejected) *rejected = ens->impl.rejected;
     });

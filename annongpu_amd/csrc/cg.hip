@@ -128,21 +128,49 @@ __global__ __launch_bounds__(256) void cg_dir_kernel(cplx<double>* p, const cplx
 
 // convergence test of iteration `iter` on the device: r^H r from its partials;
 // the first iteration with r^H r <= tol2 sets done = {1, iter}
-__global__ __launch_bounds__(256) void cg_check_kernel(const double* part, int parts, double tol2, int iter,
-                                                       int* done) {
+// tolerance (rtol ||F||)^2 with ||F||^2 = fnorm2[0]
+__device__ inline double cg_tol2(const double* fnorm2, double rtol) {
+    const double tol = rtol * sqrt(fnorm2[0]);
+    return tol * tol;
+}
+
+__global__ __launch_bounds__(256) void cg_check_kernel(const double* part, int parts, const double* fnorm2,
+                                                       double rtol, int iter, int* done) {
     __shared__ double sh[4];
     if (done[0]) return;
     const double rr = total(part, parts, 4, 2, sh);
-    if (threadIdx.x == 0 && rr <= tol2) {
+    if (threadIdx.x == 0 && rr <= cg_tol2(fnorm2, rtol)) {
         done[0] = 1;
         done[1] = iter;
     }
 }
 
+// out = (r^H r, iteration, ||F||^2, converged) of the solve's result: the iterate of
+// the first converged check, else iteration `it`
+__global__ __launch_bounds__(256) void cg_final_kernel(const double* part, int parts, int64_t stride, int it,
+                                                       const int* done, const double* fnorm2, double* out) {
+    __shared__ double sh[4];
+    const int conv = done[0];
+    const int iter = conv ? done[1] : it;
+    const double rr = total(part + int64_t(iter) * stride, parts, 4, 2, sh);
+    if (threadIdx.x == 0) {
+        out[0] = rr;
+        out[1] = double(iter);
+        out[2] = fnorm2[0];
+        out[3] = double(conv);
+    }
+}
+
 }  // namespace
 
-void launch_pcg_check(const double* part, double tol2, int iter, int* done, hipStream_t stream) {
-    cg_check_kernel<<<1, 256, 0, stream>>>(part, kCgParts, tol2, iter, done);
+void launch_pcg_check(const double* part, const double* fnorm2, double rtol, int iter, int* done, hipStream_t stream) {
+    cg_check_kernel<<<1, 256, 0, stream>>>(part, kCgParts, fnorm2, rtol, iter, done);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_pcg_final(const double* part, int it, const int* done, const double* fnorm2, double* out,
+                      hipStream_t stream) {
+    cg_final_kernel<<<1, 256, 0, stream>>>(part, kCgParts, int64_t(kCgParts) * 4, it, done, fnorm2, out);
     ANN_LAUNCH_CHECK();
 }
 

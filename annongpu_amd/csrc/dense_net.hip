@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <stdexcept>
 #include <type_traits>
 
 namespace ann {
@@ -570,6 +571,54 @@ __global__ __launch_bounds__(kBlock) void structured_Ov_kernel(DenseView<T> net,
     if (lane == 0) u_out[s] = {re, im};
 }
 
+// bf16 of x rounded to nearest even (the host's to_bf16 bit for bit)
+__device__ inline uint16_t bf16_rne(float x) {
+    const uint32_t b = __float_as_uint(x);
+    return uint16_t((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
+// Device copy of a dense net rebuilt from its parameter vector: Wd (fp64, fp32), the
+// bias (PsiDeep) and the three-term bf16 theta fragments of pack_theta_fragments.
+// Thread (v, j) owns site v, hidden unit j < M; the padding (j >= M, rows >= N)
+// keeps the zeros of the full upload.  PsiDeep sums the connections of (v, j) in
+// ascending i from zero, as the host packing does.
+__global__ void net_repack_kernel(int layout, int N, int M, int Mpad, int C, const cplx<double>* p,
+                                  const uint32_t* conn, int KT, int NT, cplx<double>* Wd64, cplx<float>* Wd32,
+                                  cplx<double>* b64, cplx<float>* b32, uint16_t* frag) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (j >= M) return;
+    double wr = 0.0, wi = 0.0;
+    if (layout == LAYOUT_RBM) {
+        wr = p[size_t(v) * M + j].re;
+        wi = p[size_t(v) * M + j].im;
+    } else {
+        const cplx<double>* W = p + N + M;
+        for (int i = 0; i < C; i++)
+            if (conn[size_t(i) * M + j] == uint32_t(v)) {
+                wr += W[size_t(i) * M + j].re;
+                wi += W[size_t(i) * M + j].im;
+            }
+        if (v == 0) {
+            b64[j] = p[N + j];
+            b32[j] = {float(p[N + j].re), float(p[N + j].im)};
+        }
+    }
+    Wd64[size_t(v) * Mpad + j] = {wr, wi};
+    Wd32[size_t(v) * Mpad + j] = {float(wr), float(wi)};
+    const int kt = v / 16, kk = v % 16, lane_base = (kk / 4) * 16, e = kk % 4;
+#pragma unroll
+    for (int comp = 0; comp < 2; comp++) {
+        const int col = 2 * j + comp, nt = col / 16, cc = col % 16;
+        double r = comp ? wi : wr;
+#pragma unroll
+        for (int sp = 0; sp < 3; sp++) {
+            const uint16_t h = bf16_rne(float(r));
+            r -= double(__uint_as_float(uint32_t(h) << 16));
+            frag[((((size_t(sp) * KT + kt) * NT + nt) * 64 + lane_base + cc) * 4) + e] = h;
+        }
+    }
+}
 
 uint16_t to_bf16(float x) {
     uint32_t b;
@@ -620,6 +669,18 @@ void pack_theta_fragments(const double* Wd, int N, int Mpad, int nsplit, std::ve
             }
         }
     }
+}
+
+void launch_net_repack(int layout, int N, int M, int Mpad, int C, const cplx<double>* params, const uint32_t* conn,
+                       int npad, cplx<double>* Wd64, cplx<float>* Wd32, cplx<double>* b64, cplx<float>* b32,
+                       uint16_t* wfrag, hipStream_t stream) {
+    if (N <= 0 || M <= 0) return;
+    if (npad < N || M > Mpad) throw std::invalid_argument("net_repack: inconsistent padding");
+    if (layout != LAYOUT_RBM && conn == nullptr) throw std::invalid_argument("net_repack: PsiDeep needs connections");
+    dim3 grid(unsigned((M + 63) / 64), unsigned(N));
+    net_repack_kernel<<<grid, 64, 0, stream>>>(layout, N, M, Mpad, C, params, conn, npad / 16, 2 * Mpad / 16, Wd64,
+                                               Wd32, b64, b32, wfrag);
+    ANN_LAUNCH_CHECK();
 }
 
 void launch_theta_mfma(const DenseView<float>& net, const uint16_t* wfrag, int npad, const uint64_t* confs,

@@ -92,7 +92,8 @@ void launch_kfac_index(int layout, int N, int M, int Mpad, const int* inv, int* 
 void launch_kfac_step(const KfacView& kv, const KfacStep& a, hipStream_t stream);
 void launch_pcg_pap(const cplx<double>* p, const cplx<double>* Ap, int64_t n, double* part, hipStream_t stream);
 // done = {converged, iteration}: step and dir leave x, r, z, p untouched once it is set,
-// launch_pcg_check sets it when r^H r of the iteration's partials is <= tol2.  The
+// launch_pcg_check sets it when r^H r of the iteration's partials is <= (rtol ||F||)^2,
+// ||F||^2 = fnorm2[0] on the device.  The
 // Jacobi preconditioner covers the parameters k < jac_end (all of them without K-FAC);
 // r^H z is the sum of slots 0 (Jacobi) and 1 (K-FAC) of the partials.
 void launch_pcg_step(cplx<double>* x, cplx<double>* r, cplx<double>* z, const cplx<double>* p,
@@ -101,14 +102,18 @@ void launch_pcg_step(cplx<double>* x, cplx<double>* r, cplx<double>* z, const cp
                      hipStream_t stream);
 void launch_pcg_dir(cplx<double>* p, const cplx<double>* z, int64_t n, const double* newpart, const double* oldpart,
                     const int* done, hipStream_t stream);
-void launch_pcg_check(const double* part, double tol2, int iter, int* done, hipStream_t stream);
+void launch_pcg_check(const double* part, const double* fnorm2, double rtol, int iter, int* done, hipStream_t stream);
+// out[4] = (r^H r, iteration, ||F||^2, converged) of the solve's result (part: the
+// partials of every iteration, [iteration][kCgParts][4]; it: iterations run)
+void launch_pcg_final(const double* part, int it, const int* done, const double* fnorm2, double* out,
+                      hipStream_t stream);
 void launch_center_scale(cplx<double>* u, const double* w, const double* c2, int64_t n, hipStream_t stream);
 void launch_cg_step(cplx<double>* x, cplx<double>* r, const cplx<double>* p, const cplx<double>* Ap, double alpha,
                     int64_t n, hipStream_t stream);
 void launch_cg_dir(cplx<double>* p, const cplx<double>* r, double beta, int64_t n, hipStream_t stream);
 void launch_precond(cplx<double>* z, const cplx<double>* r, const double* diag, double shift, int64_t n,
                     hipStream_t stream);
-void launch_sr_diag(int layout, int N, int M, int Mpad, double wsum, const cplx<double>* Dj,
+void launch_sr_diag(int layout, int N, int M, int Mpad, const double* wsum, const cplx<double>* Dj,
                     const cplx<double>* Obar, int P, double* diag, hipStream_t stream);
 void launch_rank1(cplx<double>* S, const cplx<double>* o, int P, double alpha, hipStream_t stream);
 void launch_add_diag(cplx<double>* S, int P, double shift, hipStream_t stream);
@@ -119,5 +124,8 @@ void launch_weight_coef(const double* w, const cplx<double>* e, int64_t n, cplx<
 // coef = w (e - E): F = sum_s coef_s conj(O_s) without the cancellation of E conj(<O>)
 void launch_weight_coef_centered(const double* w, const cplx<double>* e, cplx<double> E, int64_t n,
                                  cplx<double>* coef, hipStream_t stream);
+// the same with E = (sums[0], sums[1]) on the device (launch_weighted_sums' output)
+void launch_weight_coef_centered_dev(const double* w, const cplx<double>* e, const double* sums, int64_t n,
+                                     cplx<double>* coef, hipStream_t stream);
 
 }  // namespace ann

@@ -129,7 +129,19 @@ void Psi::finalize() {
 }
 
 void Psi::sync_device() {
+    if (!dirty && params_dirty) {
+        // parameters only: one asynchronous copy and a repack on the device
+        params_d.ensure(size_t(num_params));
+        ANN_HIP_CHECK(hipMemcpyAsync(params_d.p, stage.p, sizeof(cplx<double>) * num_params, hipMemcpyHostToDevice,
+                                     S()));
+        ANN_HIP_CHECK(hipEventRecord(stage_ev.get(), S()));
+        launch_net_repack(layout, N, M, Mpad, C, params_d.p, kind == PSI_DEEP ? conn_d.p : nullptr, npad, Wd64.p,
+                          Wd32.p, b64.p, b32.p, wfrag.p, S());
+        params_dirty = false;
+        return;
+    }
     if (!dirty) return;
+    params_dirty = false;
     if (kind == PSI_CNN) {
         // parameters, neighbour tables (Convolve.hpp:122-148) and symmetry classes
         cp64.upload(reinterpret_cast<const cplx<double>*>(params.data()), params.size());
@@ -285,7 +297,15 @@ void Psi::set_params(const cd* p) {
     } else {
         params.assign(p, p + num_params);
     }
-    dirty = true;
+    if (dense() && !dirty) {
+        // the previous staged copy must have left the page-locked buffer
+        if (stage_ev.e) ANN_HIP_CHECK(hipEventSynchronize(stage_ev.e));
+        stage.ensure(size_t(num_params));
+        std::copy(p, p + num_params, reinterpret_cast<cd*>(stage.p));
+        params_dirty = true;
+    } else {
+        dirty = true;
+    }
 }
 
 std::unique_ptr<Psi> Psi::clone() const {
@@ -453,14 +473,21 @@ static void finish_chains(const Psi& psi, Ensemble& ens, int64_t therm, int64_t 
     ens.chains_valid = true;
     ens.chain_words = psi.words;
     ens.chain_N = psi.N;
-    std::vector<uint32_t> acc(B);
-    ANN_HIP_CHECK(hipMemcpyAsync(acc.data(), ens.acc.p, sizeof(uint32_t) * B, hipMemcpyDeviceToHost, S()));
+    // no synchronisation here: the counters are read when the acceptance is asked for
+    ens.acc_host.ensure(size_t(B));
+    ANN_HIP_CHECK(hipMemcpyAsync(ens.acc_host.p, ens.acc.p, sizeof(uint32_t) * B, hipMemcpyDeviceToHost, S()));
+    ens.acc_total = uint64_t(B) * uint64_t(therm + steps * per_step);
+    ens.acc_pending = true;
+}
+
+void Ensemble::settle_acceptance() {
+    if (!acc_pending) return;
     Runtime::get().sync();
     uint64_t total_acc = 0;
-    for (uint32_t x : acc) total_acc += x;
-    const uint64_t total = uint64_t(B) * uint64_t(therm + steps * per_step);
-    ens.accepted = total_acc;
-    ens.rejected = total - total_acc;
+    for (size_t c = 0; c < size_t(num_chains); c++) total_acc += acc_host.p[c];
+    accepted = total_acc;
+    rejected = acc_total - total_acc;
+    acc_pending = false;
 }
 
 // log psi (and |psi|^2 weights when w is given) of configurations, any wave function
@@ -693,22 +720,17 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     local_energies(psi, op, s);
     const int64_t n = s.count;
     const int P = psi.num_params;
-    DevBuf<double> sums;
-    sums.ensure(4);
-    launch_weighted_sums(s.e_loc.p, s.w.p, n, sums.p, S());
-    double h[4];
-    ANN_HIP_CHECK(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, S()));
-    Runtime::get().sync();
-    t.E = cd(h[0], h[1]);
-    t.E2 = h[2];
-    t.wsum = h[3];
+    // E, E2 and sum w stay on the device until the end of the evaluation (no
+    // synchronisation between the local energies and the statistics)
+    t.sums.ensure(4);
+    launch_weighted_sums(s.e_loc.p, s.w.p, n, t.sums.p, S());
     auto net = Prec<T>::view(psi);
     auto& g = Prec<T>::g(t);
     {
         ScopedTimer timer("o_k");
         g_factor_padded<T>(net, Prec<T>::th(s).p, n, g);
     }
-    DevBuf<cplx<double>> coef;
+    DevBuf<cplx<double>>& coef = t.coef;
     coef.ensure(n);
     t.Obar.ensure(P);
     t.Fd.ensure(P);
@@ -717,11 +739,16 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     // diagonal of S for the Jacobi preconditioner: sum_s w |g_sj|^2 - |<O_k>|^2
     structured_sums<T>(t, net, s.confs.p, s.words, g.p, 2, coef.p, n, nullptr, 0.0, nullptr);
     t.diag.ensure(P);
-    launch_sr_diag(net.layout, net.N, net.M, net.Mpad, t.wsum, t.Yb.p, t.Obar.p, P, t.diag.p, S());
+    launch_sr_diag(net.layout, net.N, net.M, net.Mpad, t.sums.p + 3, t.Yb.p, t.Obar.p, P, t.diag.p, S());
     // F = sum w conj(O) E_loc - E conj(<O>) (TDVP.cu.template:300) = sum w (E_loc - E) conj(O)
-    launch_weight_coef_centered(s.w.p, s.e_loc.p, {t.E.real(), t.E.imag()}, n, coef.p, S());
+    launch_weight_coef_centered_dev(s.w.p, s.e_loc.p, t.sums.p, n, coef.p, S());
     structured_sums<T>(t, net, s.confs.p, s.words, g.p, 1, coef.p, n, t.Fd.p, 0.0, nullptr);
+    t.sums_host.ensure(4);
+    ANN_HIP_CHECK(hipMemcpyAsync(t.sums_host.p, t.sums.p, sizeof(double) * 4, hipMemcpyDeviceToHost, S()));
     Runtime::get().sync();
+    t.E = cd(t.sums_host.p[0], t.sums_host.p[1]);
+    t.E2 = t.sums_host.p[2];
+    t.wsum = t.sums_host.p[3];
     t.O.resize(P);
     t.F.resize(P);
     t.Obar.download(reinterpret_cast<cplx<double>*>(t.O.data()), P);
@@ -1095,9 +1122,9 @@ const cplx<double>* tdvp_S_dot_dir(Tdvp& t, const CgDir& dir, double shift, cplx
 int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x_out, int* iters, double* res,
                const cd* x0) {
     const int P = int(t.P);
-    DevBuf<cplx<double>> x;
-    x.ensure(P);
     if (method == 0) {
+        DevBuf<cplx<double>> x;
+        x.ensure(P);
         if (!t.has_S) throw std::logic_error("Cholesky solve needs TDVP.eval with the S matrix");
         DevBuf<cplx<double>> A;
         A.ensure(size_t(P) * P);
@@ -1136,21 +1163,31 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     // products and the CG scalars stay on the device (fp64) as per-block partial
     // sums; the host reads the residual norm every `check` iterations only.
     if (!t.has_samples) throw std::logic_error("CG solve needs TDVP.eval first");
-    DevBuf<cplx<double>> r, z, p, p2, Ap;
-    DevBuf<double> part, apart, scratch;
+    if (max_iter < 0) throw std::invalid_argument("max_iter must not be negative");
+    // the solve's vectors live in the Tdvp (no allocation, hence no device-wide
+    // synchronisation, per solve); the host synchronises once, at the end
+    DevBuf<cplx<double>>& x = t.cg_x;
+    DevBuf<cplx<double>>& r = t.cg_r;
+    DevBuf<cplx<double>>& z = t.cg_z;
+    DevBuf<cplx<double>>& p = t.cg_p;
+    DevBuf<cplx<double>>& p2 = t.cg_p2;
+    DevBuf<cplx<double>>& Ap = t.cg_Ap;
+    DevBuf<double>& part = t.cg_part;
+    DevBuf<double>& apart = t.cg_apart;
+    DevBuf<double>& fnorm2 = t.cg_fnorm2;
+    x.ensure(P);
     r.ensure(P);
     z.ensure(P);
     p.ensure(P);
     p2.ensure(P);
     Ap.ensure(P);
-    scratch.ensure(256);
+    t.cg_scratch.ensure(256);
     // partials of (r^H z, -, r^H r, -) per iteration, and of p^H Ap
     const size_t stride = size_t(kCgParts) * 4;
     part.ensure(size_t(max_iter + 1) * stride);
     apart.ensure(size_t(kCgParts) * 2);
-    DevBuf<double> fnorm2;
     fnorm2.ensure(4);
-    launch_dot2(t.Fd.p, t.Fd.p, nullptr, nullptr, P, 1, scratch.p, fnorm2.p, S());
+    launch_dot2(t.Fd.p, t.Fd.p, nullptr, nullptr, P, 1, t.cg_scratch.p, fnorm2.p, S());
     if (x0) {
         // warm start: r = F - (S + shift) x0
         x.upload(reinterpret_cast<const cplx<double>*>(x0), P);
@@ -1199,7 +1236,7 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     }
     // r is double-buffered under K-FAC (every workgroup of its step reads r at many
     // parameters while their owners write r - alpha Ap)
-    DevBuf<cplx<double>> r2;
+    DevBuf<cplx<double>>& r2 = t.cg_r2;
     cplx<double>* r_cur = r.p;
     if (kfac) {
         r2.ensure(P);
@@ -1215,40 +1252,19 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     } else {
         launch_pcg_init(r.p, z.p, p.p, diag, shift, P, jac_end, part.p, S());
     }
-    // ||F|| and the initial residual, read once
-    t.rr_host.ensure(stride + 1);
-    ANN_HIP_CHECK(hipMemcpyAsync(t.rr_host.p, part.p, sizeof(double) * stride, hipMemcpyDeviceToHost, S()));
-    ANN_HIP_CHECK(hipMemcpyAsync(t.rr_host.p + stride, fnorm2.p, sizeof(double), hipMemcpyDeviceToHost, S()));
-    Runtime::get().sync();
-    auto rr_of = [&](const double* h) {
-        double v = 0.0;
-        for (int b = 0; b < kCgParts; b++) v += h[size_t(b) * 4 + 2];
-        return v;
-    };
-    double rr = rr_of(t.rr_host.p);
-    const double bnorm = std::sqrt(t.rr_host.p[stride]);
-    const double tol2 = (rtol * bnorm) * (rtol * bnorm);
-    // Convergence is decided on the device every `check` iterations (a flag that
-    // freezes x, r, z, p from then on); the host reads the flag behind an event one
-    // check late, so the stream never drains, and the result is the iterate of the
-    // first converged check.
-    DevBuf<int> done;
+    // Convergence is decided on the device, against (rtol ||F||)^2, at the start and
+    // every `check` iterations (a flag that freezes x, r, z, p from then on); the host
+    // reads the flag behind an event one check late, so the stream never drains, and
+    // the result is the iterate of the first converged check.
+    DevBuf<int>& done = t.cg_done;
     done.ensure(2);
     ANN_HIP_CHECK(hipMemsetAsync(done.p, 0, sizeof(int) * 2, S()));
+    launch_pcg_check(part.p, fnorm2.p, rtol, 0, done.p, S());
     t.flag_host.ensure(4);
-    hipEvent_t ev[2];
-    ANN_HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    ANN_HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    struct EventPair {
-        hipEvent_t* e;
-        ~EventPair() {
-            (void)hipEventDestroy(e[0]);
-            (void)hipEventDestroy(e[1]);
-        }
-    } ev_guard{ev};
+    hipEvent_t ev[2] = {t.cg_ev[0].get(), t.cg_ev[1].get()};
     const int check = 4;
     int it = 0, pending = -1;   // slot of the last issued, not yet read flag copy
-    bool stop = rr <= tol2;
+    bool stop = false;
     // the direction update of iteration k happens in the first kernel of its matvec
     // (p_k = z + beta p_{k-1} into the other of two buffers)
     const cplx<double>* pk = p.p;
@@ -1304,7 +1320,7 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         }
         it++;
         if (it % check == 0 || it == max_iter) {
-            launch_pcg_check(part.p + size_t(it) * stride, tol2, it, done.p, S());
+            launch_pcg_check(part.p + size_t(it) * stride, fnorm2.p, rtol, it, done.p, S());
             const int slot = (it / check) & 1;
             ANN_HIP_CHECK(hipMemcpyAsync(t.flag_host.p + 2 * slot, done.p, sizeof(int) * 2, hipMemcpyDeviceToHost,
                                          S()));
@@ -1316,18 +1332,18 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
             pending = slot;
         }
     }
-    Runtime::get().sync();
-    int hflag[2] = {0, 0};
-    ANN_HIP_CHECK(hipMemcpy(hflag, done.p, sizeof(hflag), hipMemcpyDeviceToHost));
-    if (hflag[0]) it = hflag[1];
-    ANN_HIP_CHECK(hipMemcpyAsync(t.rr_host.p, part.p + size_t(it) * stride, sizeof(double) * stride,
-                                 hipMemcpyDeviceToHost, S()));
-    Runtime::get().sync();
-    rr = rr_of(t.rr_host.p);
+    // residual, iteration and ||F||^2 of the result, read with x (one synchronisation)
+    t.rr_host.ensure(4);
+    launch_pcg_final(part.p, it, done.p, fnorm2.p, t.cg_scratch.p, S());
+    ANN_HIP_CHECK(hipMemcpyAsync(t.rr_host.p, t.cg_scratch.p, sizeof(double) * 4, hipMemcpyDeviceToHost, S()));
     x.download(reinterpret_cast<cplx<double>*>(x_out), P);
-    *iters = it;
+    const double rr = t.rr_host.p[0];
+    const double bnorm = std::sqrt(t.rr_host.p[2]);
+    const double tol2 = (rtol * bnorm) * (rtol * bnorm);
+    const bool converged = t.rr_host.p[3] != 0.0;
+    *iters = int(t.rr_host.p[1]);
     *res = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
-    return hflag[0] != 0 || rr <= tol2 ? 0 : 2;
+    return converged || rr <= tol2 ? 0 : 2;
 }
 
 // ------------------------------------------------------------ learned proposal

@@ -112,6 +112,62 @@ struct ScopedTimer {
     bool on = false;
 };
 
+// page-locked host buffer (hipHostMalloc), for asynchronous device-to-host reads
+template <typename T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(PinnedBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    PinnedBuf& operator=(PinnedBuf&& o) noexcept {
+        if (this != &o) {
+            if (p) (void)hipHostFree(p);
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    void ensure(size_t m) {
+        if (m <= n && p) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        ANN_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (m > 0 ? m : 1), hipHostMallocDefault));
+        n = m;
+    }
+};
+
+// a HIP event created on first use (no timing) and destroyed with its owner
+struct OwnedEvent {
+    hipEvent_t e = nullptr;
+    OwnedEvent() = default;
+    OwnedEvent(const OwnedEvent&) = delete;
+    OwnedEvent& operator=(const OwnedEvent&) = delete;
+    OwnedEvent(OwnedEvent&& o) noexcept : e(o.e) { o.e = nullptr; }
+    OwnedEvent& operator=(OwnedEvent&& o) noexcept {
+        if (this != &o) {
+            if (e) (void)hipEventDestroy(e);
+            e = o.e;
+            o.e = nullptr;
+        }
+        return *this;
+    }
+    ~OwnedEvent() {
+        if (e) (void)hipEventDestroy(e);
+    }
+    hipEvent_t get() {
+        if (!e) ANN_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return e;
+    }
+};
+
 enum PsiKind : int { PSI_RBM = 0, PSI_DEEP = 1, PSI_CNN = 2 };
 
 struct Psi {
@@ -138,8 +194,14 @@ struct Psi {
     std::vector<cd> params;  // PsiCNN parameters
     double final_factor = 1.0;
 
-    // device state
+    // device state: dirty asks for the full upload; params_dirty (after the first upload,
+    // dense nets) only for the parameters, staged in page-locked memory and repacked
+    // on the device
     bool dirty = true;
+    bool params_dirty = false;
+    PinnedBuf<cplx<double>> stage;
+    DevBuf<cplx<double>> params_d;
+    OwnedEvent stage_ev;   // the staged copy has left the host buffer
     bool real_f = true;
     DevBuf<cplx<double>> Wd64, b64, f64;
     DevBuf<cplx<float>> Wd32, b32, f32;
@@ -239,6 +301,12 @@ struct Ensemble {
     DevBuf<cplx<float>> chain_th32;
     DevBuf<cplx<double>> chain_th64;
     uint64_t accepted = 0, rejected = 0;
+    // per-chain accepted proposals of the last sampling call, copied to the host
+    // asynchronously and counted when the acceptance is asked for
+    PinnedBuf<uint32_t> acc_host;
+    uint64_t acc_total = 0;
+    bool acc_pending = false;
+    void settle_acceptance();
     // learned block proposal (not owned); block sweeps per sample and plain sweeps per sample
     Proposal* proposal = nullptr;
     uint32_t plain_sweeps = 1;
@@ -246,26 +314,6 @@ struct Ensemble {
     uint64_t num_steps() const { return kind == ENS_EXACT ? (uint64_t(1) << num_sites) : num_samples; }
 };
 
-// page-locked host buffer (hipHostMalloc), for asynchronous device-to-host reads
-template <typename T>
-struct PinnedBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    PinnedBuf() = default;
-    PinnedBuf(const PinnedBuf&) = delete;
-    PinnedBuf& operator=(const PinnedBuf&) = delete;
-    ~PinnedBuf() {
-        if (p) (void)hipHostFree(p);
-    }
-    void ensure(size_t m) {
-        if (m <= n && p) return;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        n = 0;
-        ANN_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (m > 0 ? m : 1), hipHostMallocDefault));
-        n = m;
-    }
-};
 
 struct Tdvp {
     uint32_t P = 0;
@@ -310,11 +358,19 @@ struct Tdvp {
     DevBuf<int> kf_idx;
     DevBuf<int> inv;            // inverse gather table [N][Mpad] (PsiDeep)
     DevBuf<double> dot_scratch;
+    DevBuf<double> sums;           // E.re, E.im, E2, sum w of the last evaluation
+    PinnedBuf<double> sums_host;
+    DevBuf<cplx<double>> coef;     // per-sample weights of the statistics
     // materialised O [count][P] of wave functions without a structured O (PsiCNN)
     bool generic = false;
     DevBuf<cplx<float>> Og32;
     DevBuf<cplx<double>> Og64;
     DevBuf<cplx<double>> part;
+    // CG state of tdvp_solve (grow-only) and its two flag events
+    DevBuf<cplx<double>> cg_x, cg_r, cg_r2, cg_z, cg_p, cg_p2, cg_Ap;
+    DevBuf<double> cg_part, cg_apart, cg_scratch, cg_fnorm2;
+    DevBuf<int> cg_done;
+    OwnedEvent cg_ev[2];
 };
 
 // core operations

@@ -247,10 +247,12 @@ __global__ void precond_kernel(cplx<double>* z, const cplx<double>* r, const dou
 }
 
 // diag of S for a dense net: a part wsum - |<O_a>|^2, b and W parts D_j - |<O_k>|^2
-__global__ void sr_diag_kernel(int layout, int N, int M, int Mpad, double wsum, const cplx<double>* Dj,
+// (wsum read from the device)
+__global__ void sr_diag_kernel(int layout, int N, int M, int Mpad, const double* wsum_p, const cplx<double>* Dj,
                                const cplx<double>* Obar, int P, double* diag) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P) return;
+    const double wsum = *wsum_p;
     double d;
     if (layout == LAYOUT_RBM) {
         d = Dj[k % M].re;
@@ -291,6 +293,15 @@ __global__ void convert_kernel(const cplx<Tin>* a, cplx<Tout>* b, int64_t n) {
 __global__ void fill_kernel(double* a, double v, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
+}
+
+// coef_s = w_s (e_s - E) with E = (sums[0], sums[1]) read from the device
+__global__ void weight_coef_centered_dev_kernel(const double* w, const cplx<double>* e, const double* sums, int64_t n,
+                                                cplx<double>* coef) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double er = sums[0], ei = sums[1];
+    coef[i] = {w[i] * (e[i].re - er), w[i] * (e[i].im - ei)};
 }
 
 // coef_s = w_s * e_s (complex), or w_s when e == nullptr
@@ -387,7 +398,7 @@ void launch_precond(cplx<double>* z, const cplx<double>* r, const double* diag, 
     ANN_LAUNCH_CHECK();
 }
 
-void launch_sr_diag(int layout, int N, int M, int Mpad, double wsum, const cplx<double>* Dj,
+void launch_sr_diag(int layout, int N, int M, int Mpad, const double* wsum, const cplx<double>* Dj,
                     const cplx<double>* Obar, int P, double* diag, hipStream_t stream) {
     sr_diag_kernel<<<blocks_for(P), 256, 0, stream>>>(layout, N, M, Mpad, wsum, Dj, Obar, P, diag);
     ANN_LAUNCH_CHECK();
@@ -422,6 +433,13 @@ void launch_fill(double* a, double v, int64_t n, hipStream_t stream) {
 void launch_weight_coef(const double* w, const cplx<double>* e, int64_t n, cplx<double>* coef, hipStream_t stream) {
     if (n <= 0) return;
     weight_coef_kernel<<<blocks_for(n), 256, 0, stream>>>(w, e, cplx<double>{0.0, 0.0}, n, coef);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_weight_coef_centered_dev(const double* w, const cplx<double>* e, const double* sums, int64_t n,
+                                     cplx<double>* coef, hipStream_t stream) {
+    if (n <= 0) return;
+    weight_coef_centered_dev_kernel<<<blocks_for(n), 256, 0, stream>>>(w, e, sums, n, coef);
     ANN_LAUNCH_CHECK();
 }
 

@@ -275,3 +275,41 @@ def test_structured_matvec_general_connections(precision, repeated):
         assert rel(tdvp.S_dot_vector(v), S @ v) < (1e-10 if precision == "fp64" else 2e-5)
     x = tdvp.solve(1e-3, "cg", rtol=1e-6, max_iter=3000)
     assert rel((S + 1e-3 * np.eye(psi.num_params)) @ x, tdvp.F_vector) < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["rbm", "deep", "deep_repeated"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_params_update_matches_fresh_upload(precision, kind):
+    """New parameters of a net already on the device (staged and repacked there by a
+    kernel) give bit for bit the log psi, O_k, energy and F of a net first uploaded
+    with them (host packing)"""
+    N, M, C = 20, 40, 8
+
+    def make():
+        if kind == "rbm":
+            return A.PsiRBM(cgauss(np.random.default_rng(1), (N, M), 0.1), 1.0 - 0.3j, 0.1)
+        return shuffled_deep(N, M, C, 1, kind == "deep_repeated")
+
+    a = make()
+    a.precision = precision
+    idx = [int(i) for i in np.random.default_rng(3).integers(0, 2 ** N, 4)]
+    A.log_psi_s(a, idx[0])                      # first, full upload
+    rng = np.random.default_rng(4)
+    for _ in range(2):
+        newp = a.params + cgauss(rng, a.num_params, 0.02)
+        a.params = newp                          # staged update
+        b = make()
+        b.params = newp                          # never uploaded: full upload
+        b.precision = precision
+        for i in idx:
+            assert A.log_psi_s(a, i) == A.log_psi_s(b, i)
+            assert np.array_equal(A.psi_O_k(a, i), A.psi_O_k(b, i))
+        op = A.Operator(A.hamiltonians.heisenberg_chain(N))
+        res = []
+        for psi in (a, b):
+            ens = A.MonteCarloSpins(512, 2, 10, 128, move="exchange", seed=9)
+            tdvp = A.TDVP(psi.num_params)
+            tdvp.eval_F(op, psi, ens)
+            res.append((tdvp.E_local, tdvp.F_vector))
+        assert res[0][0] == res[1][0]
+        assert np.array_equal(res[0][1], res[1][1])
