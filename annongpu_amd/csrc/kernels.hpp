@@ -36,13 +36,17 @@ struct DenseView {
 // one-dimensional layout, shift >= 0) N + M + i M + j with i = (v - j shift) mod N
 // when i < C, else none (-1).  shift < 0: a PsiDeep with other connections (the
 // kernels then read tables).
+// sm = 1 (a circulant PsiDeep with C = N, where (site, unit) -> parameter is a
+// bijection): vectors in site-major order instead, W of (v, j) at N + M + v M + j
+// (the order the CG solve keeps its vectors in; to_site_major converts).
 struct WParamMap {
-    int rbm, N, M, C, shift;
+    int rbm, N, M, C, shift, sm = 0;
     __host__ __device__ bool analytic() const { return rbm || shift >= 0; }
-    __host__ __device__ int param(int v, int j) const { return param_at(v, j, rbm ? 0 : (j * shift) % N); }
+    __host__ __device__ int param(int v, int j) const { return param_at(v, j, rbm || sm ? 0 : (j * shift) % N); }
     // the same with jm = (j shift) mod N given (no division per site)
     __host__ __device__ int param_at(int v, int j, int jm) const {
         if (rbm) return v * M + j;
+        if (sm) return N + M + v * M + j;
         int i = v - jm;
         if (i < 0) i += N;
         return i < C ? N + M + i * M + j : -1;
@@ -169,7 +173,7 @@ void launch_spin_gemm32(const DenseView<float>& net, const uint64_t* bits, const
 
 // workspace of the fused fp32 matvec: frag sr_frag_elems(N, Mpad), va N, vb Mpad,
 // upart sr_ov_pairs(M) * count, u sr_count_pad(count), spin_part chunks * n, opart 128,
-// ypart 2 * sr_y_parts(P)
+// ypart 2 * sr_y_parts(net, wm) (at most 512)
 struct SrWorkspace {
     uint16_t* frag;
     cplx<double>* va;
@@ -184,7 +188,7 @@ struct SrWorkspace {
 // structured O: launch_sr_pack, launch_sr_ov32, launch_sr_finish (u_s = w_s (O_s.v -
 // <O>.v) into ws.u), launch_spin_gemm32 (mode 1, coefficients ws.u, partials into
 // ws.spin_part), launch_sr_y_bf16 (y, and the partials of v^H y in ws.ypart,
-// [sr_y_parts(P)][2])
+// [sr_y_parts(net, wm)][2])
 // the CG direction update p_new = z + (rz_new / rz_old) p_old, fused into the first
 // kernel of the matvec (z null: plain matvec of v).  rz partials as in cg.hip
 // ([parts][4], element 0); nothing changes once done[0] is set (p_new = p_old).
@@ -197,7 +201,7 @@ struct CgDir {
     const int* done = nullptr;
     cplx<double>* p_new = nullptr;
 };
-int sr_y_parts(int P);
+int sr_y_parts(const DenseView<float>& net, const WParamMap& wm);
 int sr_kt32(int N);
 int sr_ov_pairs(int M);
 size_t sr_frag_elems(int N, int Mpad);
@@ -211,6 +215,12 @@ void launch_sr_finish(const DenseView<float>& net, const double* w, const uint64
                       SrWorkspace ws, hipStream_t stream);
 void launch_sr_y_bf16(const DenseView<float>& net, const WParamMap& wm, const cplx<double>* v, double shift,
                       const SpinPlan& plan, SrWorkspace ws, cplx<double>* y, hipStream_t stream);
+
+// site-major order of a circulant PsiDeep's parameter vector (WParamMap sm):
+// forward out[N + M + v M + j] = in[N + M + i M + j] with v = (j shift + i) mod N
+// (C = N), the a and b parts copied; backward the inverse
+void launch_site_major(const cplx<double>* in, int N, int M, int shift, bool forward, cplx<double>* out,
+                       hipStream_t stream);
 
 void launch_weighted_sums(const cplx<double>* e_loc, const double* w, int64_t count,
                           double* out4 /* E.re, E.im, E2, sum w */, hipStream_t stream);

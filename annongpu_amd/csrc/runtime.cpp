@@ -963,12 +963,14 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y,
             t.bits_valid = true;
         }
         const SrWorkspace ws{t.frag.p, t.va.p, t.vb.p, t.upart.p, t.u32.p, t.spin_part.p, t.opart.p, t.ypart.p};
+        const WParamMap wm = t.order();
+        const cplx<double>* obar = t.solve_sm ? t.obar_sm.p : t.Obar.p;
         // timing events between the launches stall the queue for microseconds, so only
         // every 16th product is timed (averages per launch stay exact)
         const bool timed = t.matvecs++ % 16 == 0;
         {
             ScopedTimer timer("sr_pack", timed);
-            launch_sr_pack(net, t.wmap(), t.inv.p, v, dir ? *dir : CgDir{}, t.Obar.p, ws, S());
+            launch_sr_pack(net, wm, t.inv.p, v, dir ? *dir : CgDir{}, obar, ws, S());
         }
         {
             ScopedTimer timer("sr_ov", timed);
@@ -985,10 +987,10 @@ void tdvp_S_dot_t(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y,
         }
         {
             ScopedTimer timer("sr_y", timed);
-            launch_sr_y_bf16(net, t.wmap(), v, shift, plan, ws, y, S());
+            launch_sr_y_bf16(net, wm, v, shift, plan, ws, y, S());
         }
         if (ypart) *ypart = t.ypart.p;
-        if (yparts) *yparts = sr_y_parts(P);
+        if (yparts) *yparts = sr_y_parts(net, wm);
         return;
     } else {
         auto& V = Ws<T>::V(t);
@@ -1188,19 +1190,41 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     apart.ensure(size_t(kCgParts) * 2);
     fnorm2.ensure(4);
     launch_dot2(t.Fd.p, t.Fd.p, nullptr, nullptr, P, 1, t.cg_scratch.p, fnorm2.p, S());
-    if (x0) {
-        // warm start: r = F - (S + shift) x0
-        x.upload(reinterpret_cast<const cplx<double>*>(x0), P);
-        tdvp_S_dot(t, x.p, shift, Ap.p);
-        launch_cg_dir(Ap.p, t.Fd.p, -1.0, P, S());
-        ANN_HIP_CHECK(hipMemcpyAsync(r.p, Ap.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
-    } else {
-        ANN_HIP_CHECK(hipMemsetAsync(x.p, 0, sizeof(cplx<double>) * P, S()));
-        ANN_HIP_CHECK(hipMemcpyAsync(r.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
-    }
     // method 1: Jacobi preconditioner; method 3: K-FAC on the W parameters of a dense
     // net (the biases keep Jacobi), factors from this evaluation's samples
     const bool kfac = method == 3;
+    // K-FAC with the fp32 S.v of a circulant PsiDeep with C = N keeps the solve's
+    // vectors in site-major order (the S.v and K-FAC kernels then read and write whole
+    // rows of 64 units); F, x0 and <O> are converted on entry and x on exit
+    const bool sm = kfac && t.precision != 0 && !t.generic && t.layout == LAYOUT_DEEP1 && t.conn_shift >= 0 &&
+                    t.C == t.N && int64_t(t.N) + t.M + int64_t(t.N) * t.M == P;
+    struct OrderGuard {
+        Tdvp& t;
+        ~OrderGuard() { t.solve_sm = false; }
+    } order_guard{t};
+    t.solve_sm = sm;
+    const cplx<double>* F = t.Fd.p;
+    if (sm) {
+        t.obar_sm.ensure(P);
+        launch_site_major(t.Obar.p, t.N, t.M, t.conn_shift, true, t.obar_sm.p, S());
+        launch_site_major(t.Fd.p, t.N, t.M, t.conn_shift, true, z.p, S());
+        F = z.p;
+    }
+    if (x0) {
+        // warm start: r = F - (S + shift) x0
+        if (sm) {
+            Ap.upload(reinterpret_cast<const cplx<double>*>(x0), P);
+            launch_site_major(Ap.p, t.N, t.M, t.conn_shift, true, x.p, S());
+        } else {
+            x.upload(reinterpret_cast<const cplx<double>*>(x0), P);
+        }
+        tdvp_S_dot(t, x.p, shift, Ap.p);
+        launch_cg_dir(Ap.p, F, -1.0, P, S());
+        ANN_HIP_CHECK(hipMemcpyAsync(r.p, Ap.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
+    } else {
+        ANN_HIP_CHECK(hipMemsetAsync(x.p, 0, sizeof(cplx<double>) * P, S()));
+        ANN_HIP_CHECK(hipMemcpyAsync(r.p, F, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
+    }
     const double* diag = t.diag.p != nullptr && (method == 1 || method == 3) ? t.diag.p : nullptr;
     int64_t jac_end = P;
     KfacView kv{};
@@ -1232,7 +1256,7 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
                               t.w.p, t.count, t.N, t.M, shift, kw, S());
         }
         jac_end = t.layout == LAYOUT_DEEP1 ? int64_t(t.N) + t.M : 0;
-        kv = {t.N, t.M, t.kf_idx.p, t.wmap(), t.kf_Ci.p, t.kf_D.p};
+        kv = {t.N, t.M, t.kf_idx.p, t.order(), t.kf_Ci.p, t.kf_D.p};
     }
     // r is double-buffered under K-FAC (every workgroup of its step reads r at many
     // parameters while their owners write r - alpha Ap)
@@ -1336,7 +1360,12 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     t.rr_host.ensure(4);
     launch_pcg_final(part.p, it, done.p, fnorm2.p, t.cg_scratch.p, S());
     ANN_HIP_CHECK(hipMemcpyAsync(t.rr_host.p, t.cg_scratch.p, sizeof(double) * 4, hipMemcpyDeviceToHost, S()));
-    x.download(reinterpret_cast<cplx<double>*>(x_out), P);
+    if (sm) {
+        launch_site_major(x.p, t.N, t.M, t.conn_shift, false, z.p, S());
+        z.download(reinterpret_cast<cplx<double>*>(x_out), P);
+    } else {
+        x.download(reinterpret_cast<cplx<double>*>(x_out), P);
+    }
     const double rr = t.rr_host.p[0];
     const double bnorm = std::sqrt(t.rr_host.p[2]);
     const double tol2 = (rtol * bnorm) * (rtol * bnorm);

@@ -355,6 +355,15 @@ struct Tdvp {
     DevBuf<double> kf_spart, kf_C, kf_Ci, kf_gam;
     DevBuf<cplx<double>> kf_gpart, kf_D;
     WParamMap wmap() const { return {layout == LAYOUT_RBM ? 1 : 0, N, M, C, conn_shift}; }
+    // while a CG solve runs with its vectors in site-major order (circulant PsiDeep,
+    // C = N, fp32 S.v), the S.v products read and write that order and <O> in it
+    bool solve_sm = false;
+    DevBuf<cplx<double>> obar_sm;
+    WParamMap order() const {
+        WParamMap m = wmap();
+        m.sm = solve_sm ? 1 : 0;
+        return m;
+    }
     DevBuf<int> kf_idx;
     DevBuf<int> inv;            // inverse gather table [N][Mpad] (PsiDeep)
     DevBuf<double> dot_scratch;

@@ -663,28 +663,58 @@ __global__ __launch_bounds__(256) void ov_finish_center_kernel(const float2* upa
 // a parameter are loaded in batches of kYBatch at clamped indices (one memory round
 // trip for up to kYBatch chunks); the site of a PsiDeep connection comes from wm
 // when it is analytic, else from conn.
+// Threads walk slots: the parameters in order, except the W block of a circulant
+// PsiDeep, which is walked in tiles of 16 connections x 16 units (connections padded
+// to a multiple of 16, padding slots idle): connection i of unit j sits on row
+// (j shift + i) mod N, so a tile reads runs of up to 16 units on 16 + 15 shift rows
+// instead of a different row in every lane.
 constexpr int kYBatch = 32;
+constexpr int kYTile = 16;
+__host__ __device__ inline bool spin_y_tiled(int layout, const WParamMap& wm) {
+    return layout == LAYOUT_DEEP1 && wm.shift >= 0 && !wm.sm;
+}
+__host__ __device__ inline int64_t spin_y_slots(int layout, int N, int M, int C, int P, const WParamMap& wm) {
+    if (!spin_y_tiled(layout, wm)) return P;
+    const int64_t cp = (C + kYTile - 1) / kYTile * kYTile, mp = (M + kYTile - 1) / kYTile * kYTile;
+    return int64_t(N) + M + cp * mp;
+}
 __global__ __launch_bounds__(256) void spin_y_kernel(const float* partial, int chunks, int64_t n, int ncols,
-                                                     int layout, int N, int M, int Mpad, WParamMap wm,
+                                                     int layout, int N, int M, int Mpad, int C, WParamMap wm,
                                                      const uint32_t* conn,
-                                                     const cplx<double>* v, double shift, int P,
+                                                     const cplx<double>* v, double shift, int P, int64_t slots,
                                                      cplx<double>* y, double* ypart) {
     __shared__ double sh[4];
     double dr = 0.0, di = 0.0;
-    for (int k = blockIdx.x * 256 + threadIdx.x; k < P; k += gridDim.x * 256) {
+    const bool tiled = spin_y_tiled(layout, wm);
+    const int jtiles = (M + kYTile - 1) / kYTile;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < slots; e += int64_t(gridDim.x) * 256) {
         int row, col;
+        int64_t k = e;
         if (layout == LAYOUT_RBM) {
-            row = k / M;
-            col = 2 * (k % M);
+            row = int(k / M);
+            col = 2 * int(k % M);
         } else if (k < N) {
-            row = k;
+            row = int(k);
             col = 2 * Mpad;
         } else if (k < N + M) {
             row = N;
-            col = 2 * (k - N);
+            col = 2 * int(k - N);
+        } else if (wm.sm) {
+            const int q = int(k) - N - M;
+            row = q / M;
+            col = 2 * (q % M);
+        } else if (tiled) {
+            const int64_t q = e - N - M;
+            const int64_t tile = q / (kYTile * kYTile);
+            const int w = int(q % (kYTile * kYTile));
+            const int i = int(tile / jtiles) * kYTile + w / kYTile, j = int(tile % jtiles) * kYTile + w % kYTile;
+            if (i >= C || j >= M) continue;
+            k = int64_t(N) + M + int64_t(i) * M + j;
+            row = wm.site(i, j);
+            col = 2 * j;
         } else {
-            const int q = k - N - M;
-            row = wm.shift >= 0 ? wm.site(q / M, q % M) : int(conn[q]);
+            const int q = int(k) - N - M;
+            row = int(conn[q]);
             col = 2 * (q % M);
         }
         const int64_t off = int64_t(row) * ncols + col;
@@ -721,7 +751,29 @@ __global__ __launch_bounds__(256) void spin_y_kernel(const float* partial, int c
 
 }  // namespace
 
-int sr_y_parts(int P) { return std::min(256, std::max(1, (P + 255) / 256)); }
+// site-major order (one thread per site-major slot; the reads of the backward
+// direction are contiguous, the writes of the forward one)
+__global__ void site_major_kernel(const cplx<double>* in, int N, int M, int shift, int forward, cplx<double>* out) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t P = int64_t(N) + M + int64_t(N) * M;
+    if (e >= P) return;
+    int64_t k = e;
+    if (e >= N + M) {
+        const int q = int(e - N - M), v = q / M, j = q % M;
+        int i = v - int((int64_t(j) * shift) % N);
+        if (i < 0) i += N;
+        k = int64_t(N) + M + int64_t(i) * M + j;
+    }
+    if (forward)
+        out[e] = in[k];
+    else
+        out[k] = in[e];
+}
+
+int sr_y_parts(const DenseView<float>& net, const WParamMap& wm) {
+    const int64_t slots = spin_y_slots(net.layout, net.N, net.M, net.C, net.num_params, wm);
+    return int(std::min<int64_t>(256, std::max<int64_t>(1, (slots + 255) / 256)));
+}
 
 int sr_kt32(int N) { return (N + 31) / 32; }
 
@@ -853,11 +905,19 @@ int sr_ov_pairs(int M) { return (M + 15) / 16; }
 
 size_t sr_frag_elems(int N, int Mpad) { return size_t(3) * sr_kt32(N) * (2 * Mpad / 16) * 64 * 8; }
 
+void launch_site_major(const cplx<double>* in, int N, int M, int shift, bool forward, cplx<double>* out,
+                       hipStream_t stream) {
+    if (shift < 0) throw std::invalid_argument("site-major order needs a circulant PsiDeep");
+    const int64_t P = int64_t(N) + M + int64_t(N) * M;
+    site_major_kernel<<<unsigned((P + 255) / 256), 256, 0, stream>>>(in, N, M, shift, forward ? 1 : 0, out);
+    ANN_LAUNCH_CHECK();
+}
+
 void launch_sr_y_bf16(const DenseView<float>& net, const WParamMap& wm, const cplx<double>* v, double shift,
                       const SpinPlan& plan, SrWorkspace ws, cplx<double>* y, hipStream_t stream) {
-    spin_y_kernel<<<unsigned(sr_y_parts(net.num_params)), 256, 0, stream>>>(
-        ws.spin_part, plan.chunks, plan.n, plan.ncols, net.layout, net.N, net.M, net.Mpad, wm, net.conn, v, shift,
-        net.num_params, y, ws.ypart);
+    spin_y_kernel<<<unsigned(sr_y_parts(net, wm)), 256, 0, stream>>>(
+        ws.spin_part, plan.chunks, plan.n, plan.ncols, net.layout, net.N, net.M, net.Mpad, net.C, wm, net.conn, v,
+        shift, net.num_params, spin_y_slots(net.layout, net.N, net.M, net.C, net.num_params, wm), y, ws.ypart);
     ANN_LAUNCH_CHECK();
 }
 

@@ -219,13 +219,19 @@ def main():
     B = args.chains
     M = args.alpha * N
 
-    # 1. pre-optimisation with the plain sampler (10 sweeps per sample)
+    # 1. pre-optimisation with the plain sampler (10 sweeps per sample), in fp64: the
+    # state it reaches sets the measured workload (tau, sweeps per sample, CG
+    # iterations), and a Monte Carlo trajectory flips accept decisions on perturbations
+    # of the ratios at the fp32 rounding level, so an fp32 optimisation would land on
+    # another state whenever an fp32 kernel changes its summation order
     t0 = time.perf_counter()
+    psi.precision = "fp64"
     opt_ens = A.MonteCarloSpins(B, 10, 30, B, move="exchange", seed=5, persistent=True)
     # (the stability of this schedule was established with the Jacobi solve below)
     opt = Stepper(A, psi, op, opt_ens, args.opt_lr, args.cg_iters, shift=1e-2, solver="cg", rtol=1e-4)
     for _ in range(args.opt_steps):
         opt.step()
+    psi.precision = "fp32"
     E_opt = opt.tdvp.E_local.real / N
     var_opt = opt.tdvp.var_H / N
     if not (var_opt < 50.0 and E_opt > -2.75):

@@ -22,12 +22,15 @@ def rel(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
 
 
+@pytest.mark.parametrize("shape", [(8, 16), (12, 6)])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_kfac_cg_matches_dense_solve_deep(precision):
+def test_kfac_cg_matches_dense_solve_deep(precision, shape):
     """PsiDeep (biases by Jacobi, W by the Kronecker factors) on ExactSummation: the
-    solution equals numpy's solve of the oracle's S + shift."""
-    N = 8
-    spec = random_deep(N, [16], [N], seed=6, sigma=0.1)
+    solution equals numpy's solve of the oracle's S + shift.  Connection shifts 1
+    (8 sites, 16 units) and 2 (12 sites, 6 units); with C = N the fp32 solve runs in
+    site-major order."""
+    N, M = shape
+    spec = random_deep(N, [M], [N], seed=6, sigma=0.1)
     o = spec.oracle()
     psi = A.PsiDeep(*spec.args())
     psi.precision = precision
