@@ -581,8 +581,10 @@ __device__ inline uint16_t bf16_rne(float x) {
 // bias (PsiDeep) and the three-term bf16 theta fragments of pack_theta_fragments.
 // Thread (v, j) owns site v, hidden unit j < M; the padding (j >= M, rows >= N)
 // keeps the zeros of the full upload.  PsiDeep sums the connections of (v, j) in
-// ascending i from zero, as the host packing does.
-__global__ void net_repack_kernel(int layout, int N, int M, int Mpad, int C, const cplx<double>* p,
+// ascending i from zero, as the host packing does; with shift >= 0 (connection i of
+// unit j is site (j shift + i) mod N, C <= N) the one connection i = (v - j shift) mod N
+// is found directly.
+__global__ void net_repack_kernel(int layout, int N, int M, int Mpad, int C, int shift, const cplx<double>* p,
                                   const uint32_t* conn, int KT, int NT, cplx<double>* Wd64, cplx<float>* Wd32,
                                   cplx<double>* b64, cplx<float>* b32, uint16_t* frag) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -592,6 +594,14 @@ __global__ void net_repack_kernel(int layout, int N, int M, int Mpad, int C, con
     if (layout == LAYOUT_RBM) {
         wr = p[size_t(v) * M + j].re;
         wi = p[size_t(v) * M + j].im;
+    } else if (shift >= 0) {
+        const cplx<double>* W = p + N + M;
+        int i = v - int((int64_t(j) * shift) % N);
+        if (i < 0) i += N;
+        if (i < C) {
+            wr += W[size_t(i) * M + j].re;
+            wi += W[size_t(i) * M + j].im;
+        }
     } else {
         const cplx<double>* W = p + N + M;
         for (int i = 0; i < C; i++)
@@ -599,10 +609,10 @@ __global__ void net_repack_kernel(int layout, int N, int M, int Mpad, int C, con
                 wr += W[size_t(i) * M + j].re;
                 wi += W[size_t(i) * M + j].im;
             }
-        if (v == 0) {
-            b64[j] = p[N + j];
-            b32[j] = {float(p[N + j].re), float(p[N + j].im)};
-        }
+    }
+    if (layout != LAYOUT_RBM && v == 0) {
+        b64[j] = p[N + j];
+        b32[j] = {float(p[N + j].re), float(p[N + j].im)};
     }
     Wd64[size_t(v) * Mpad + j] = {wr, wi};
     Wd32[size_t(v) * Mpad + j] = {float(wr), float(wi)};
@@ -671,15 +681,17 @@ void pack_theta_fragments(const double* Wd, int N, int Mpad, int nsplit, std::ve
     }
 }
 
-void launch_net_repack(int layout, int N, int M, int Mpad, int C, const cplx<double>* params, const uint32_t* conn,
-                       int npad, cplx<double>* Wd64, cplx<float>* Wd32, cplx<double>* b64, cplx<float>* b32,
-                       uint16_t* wfrag, hipStream_t stream) {
+void launch_net_repack(int layout, int N, int M, int Mpad, int C, int shift, const cplx<double>* params,
+                       const uint32_t* conn, int npad, cplx<double>* Wd64, cplx<float>* Wd32, cplx<double>* b64,
+                       cplx<float>* b32, uint16_t* wfrag, hipStream_t stream) {
     if (N <= 0 || M <= 0) return;
     if (npad < N || M > Mpad) throw std::invalid_argument("net_repack: inconsistent padding");
-    if (layout != LAYOUT_RBM && conn == nullptr) throw std::invalid_argument("net_repack: PsiDeep needs connections");
+    if (layout != LAYOUT_RBM && shift < 0 && conn == nullptr)
+        throw std::invalid_argument("net_repack: PsiDeep needs connections");
+    if (shift >= 0 && C > N) throw std::invalid_argument("net_repack: a circulant layout has C <= N");
     dim3 grid(unsigned((M + 63) / 64), unsigned(N));
-    net_repack_kernel<<<grid, 64, 0, stream>>>(layout, N, M, Mpad, C, params, conn, npad / 16, 2 * Mpad / 16, Wd64,
-                                               Wd32, b64, b32, wfrag);
+    net_repack_kernel<<<grid, 64, 0, stream>>>(layout, N, M, Mpad, C, shift, params, conn, npad / 16, 2 * Mpad / 16,
+                                               Wd64, Wd32, b64, b32, wfrag);
     ANN_LAUNCH_CHECK();
 }
 

@@ -86,10 +86,11 @@ void pack_theta_fragments(const double* Wd_interleaved, int N, int Mpad, int nsp
                           std::vector<uint16_t>& out, int& npad);
 // Wd64/Wd32, the PsiDeep bias and the 3-term fragments (npad rows) rebuilt on the device
 // from the parameter vector (PsiRBM W[v][j]; one-layer PsiDeep a, b, W[i][j] with
-// conn[i][j]); buffers laid out and padded as by the host packing, padding untouched
-void launch_net_repack(int layout, int N, int M, int Mpad, int C, const cplx<double>* params, const uint32_t* conn,
-                       int npad, cplx<double>* Wd64, cplx<float>* Wd32, cplx<double>* b64, cplx<float>* b32,
-                       uint16_t* wfrag, hipStream_t stream);
+// conn[i][j], or with conn[i][j] = (j shift + i) mod N when shift >= 0); buffers laid
+// out and padded as by the host packing, padding untouched
+void launch_net_repack(int layout, int N, int M, int Mpad, int C, int shift, const cplx<double>* params,
+                       const uint32_t* conn, int npad, cplx<double>* Wd64, cplx<float>* Wd32, cplx<double>* b64,
+                       cplx<float>* b32, uint16_t* wfrag, hipStream_t stream);
 
 template <typename T>
 void launch_net_log_psi(const DenseView<T>& net, const cplx<T>* theta, int64_t count,

@@ -250,9 +250,8 @@ __global__ void kfac_damping_kernel(const double* C, int N, const cplx<double>* 
     }
 }
 
-// in-place Gauss-Jordan inverse of the Hermitian positive definite A + gamma I
-// (n <= 128 real / 64 complex, whole matrix in LDS; no pivoting is needed for a
-// positive definite matrix)
+// Gauss-Jordan inverse of the Hermitian positive definite A + gamma I (n <= 128
+// real / 64 complex; no pivoting is needed for a positive definite matrix)
 template <typename E>
 __device__ inline E gj_mul(E a, E b);
 template <>
@@ -269,66 +268,105 @@ __device__ inline cplx<double> gj_sub(cplx<double> a, cplx<double> b) { return {
 __device__ inline double gj_addr(double a, double g) { return a + g; }
 __device__ inline cplx<double> gj_addr(cplx<double> a, double g) { return {a.re + g, a.im}; }
 
-template <typename E>
-__device__ void gj_invert(E* A, int n) {   // A: n x n in LDS, row-major; blockDim a multiple of 64
-    __shared__ E piv_row[128];
-    __shared__ E piv_col[128];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+// The matrix lives in registers: thread (wave w, lane l) of 16 waves holds rows
+// w + 16 i (i < RI) and columns l + 64 h (h < CH).  Per pivot k the pivot row and
+// column go through LDS (double-buffered, so one barrier per pivot); with p = 1/A_kk
+// and the pivot row scaled, R_c = A_kc p (R_k = p), row k becomes R and every other
+// row r becomes A_rc - A_rk R_c (column k: -A_rk p).
+// load(r, c) gives the entries, store(r, c, x) takes the inverse (after every load).
+template <typename E, int RI, int CH, typename Load, typename Store>
+__device__ void gj_invert_regs(int n, Load load, Store store) {
+    static_assert(CH == 1 || CH == 2, "columns lane and lane + 64");
+    __shared__ E prow[2][64 * CH];
+    __shared__ E pcol[2][16 * RI];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    E a[RI][CH];
+#pragma unroll
+    for (int i = 0; i < RI; i++)
+#pragma unroll
+        for (int h = 0; h < CH; h++) {
+            const int r = wave + 16 * i, c = lane + 64 * h;
+            a[i][h] = r < n && c < n ? load(r, c) : E{};
+        }
+    // entries outside n x n are never stored: they take part in the updates without
+    // guards (no branch per entry) and never feed an entry that is stored
     for (int k = 0; k < n; k++) {
-        __syncthreads();
-        const E p = gj_inv(A[k * n + k]);
-        // row k scaled, column k kept for the update
-        for (int c = threadIdx.x; c < n; c += blockDim.x) {
-            piv_row[c] = c == k ? p : gj_mul(A[k * n + c], p);
-            piv_col[c] = A[c * n + k];
+        const int b = k & 1, ik = k >> 4, hk = k >> 6;
+        // the owner wave of row k writes its register slot ik (wave-uniform branches,
+        // static register indices)
+        if (wave == (k & 15)) {
+#pragma unroll
+            for (int i = 0; i < RI; i++)
+                if (i == ik) {
+#pragma unroll
+                    for (int h = 0; h < CH; h++) prow[b][lane + 64 * h] = a[i][h];
+                }
+        }
+        if (lane == (k & 63)) {   // the owner lane of column k (slot hk)
+#pragma unroll
+            for (int i = 0; i < RI; i++) {
+                E x = a[i][0];
+                if constexpr (CH > 1) x = hk == 1 ? a[i][1] : x;
+                pcol[b][wave + 16 * i] = x;
+            }
         }
         __syncthreads();
-        // waves over rows, lanes over columns (no index division)
-        for (int r = wave; r < n; r += waves) {
-            const E f = piv_col[r];
-            for (int c = lane; c < n; c += 64) {
-                E& a = A[r * n + c];
-                if (r == k)
-                    a = piv_row[c];
-                else
-                    a = c == k ? gj_sub(E{}, gj_mul(f, p)) : gj_sub(a, gj_mul(f, piv_row[c]));
+        const E p = gj_inv(prow[b][k]);
+        E pr[CH];
+#pragma unroll
+        for (int h = 0; h < CH; h++) pr[h] = lane + 64 * h == k ? p : gj_mul(prow[b][lane + 64 * h], p);
+#pragma unroll
+        for (int i = 0; i < RI; i++) {
+            const E f = pcol[b][wave + 16 * i];
+            const bool rk = wave + 16 * i == k;
+#pragma unroll
+            for (int h = 0; h < CH; h++) {
+                const E fp = gj_mul(f, pr[h]);
+                const E upd = gj_sub(lane + 64 * h == k ? E{} : a[i][h], fp);
+                a[i][h] = rk ? pr[h] : upd;
             }
         }
     }
-    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < RI; i++)
+#pragma unroll
+        for (int h = 0; h < CH; h++) {
+            const int r = wave + 16 * i, c = lane + 64 * h;
+            if (r < n && c < n) store(r, c, a[i][h]);
+        }
 }
 
 // Ci = (C + gc I)^-1 (block 0) and Di_b = (D_b + gd I)^-1 (block 1 + b, stored
-// transposed in place: Dblk[b][k][j] = Di_b[j][k]), each matrix in the block's
-// dynamic LDS; the two inversions run side by side in one launch.  Entries beyond a
-// D block's n stay as the reduction left them (zero).
+// transposed in place: Dblk[b][k][j] = Di_b[j][k]), each in the registers of its
+// block; the two inversions run side by side in one launch.  Entries beyond a D
+// block's n stay as the reduction left them (zero).
 constexpr int kInvThreads = 1024;
+static_assert(kKfacMaxSites <= 128 && kDBlock <= 64, "register tiles of the inversion");
 
 __global__ __launch_bounds__(kInvThreads) void kfac_invert_kernel(const double* C, int N, const double* gam,
                                                                   double* Ci, cplx<double>* Dblk, int M) {
-    extern __shared__ double lds_inv[];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
     if (blockIdx.x == 0) {
-        double* A = lds_inv;
-        for (int r = wave; r < N; r += waves)
-            for (int c = lane; c < N; c += 64) A[r * N + c] = r == c ? gj_addr(C[r * N + c], gam[0]) : C[r * N + c];
-        gj_invert<double>(A, N);
-        for (int r = wave; r < N; r += waves)
-            for (int c = lane; c < N; c += 64) Ci[r * N + c] = A[r * N + c];
+        const double gc = gam[0];
+        gj_invert_regs<double, 8, 2>(
+            N,
+            [&](int r, int c) {
+                const double x = C[r * N + c];
+                return r == c ? gj_addr(x, gc) : x;
+            },
+            [&](int r, int c, double x) { Ci[r * N + c] = x; });
         return;
     }
-    cplx<double>* A = reinterpret_cast<cplx<double>*>(lds_inv);
     const int b = blockIdx.x - 1;
     const int n = min(kDBlock, M - b * kDBlock);
     cplx<double>* src = Dblk + int64_t(b) * kDBlock * kDBlock;
-    for (int r = wave; r < n; r += waves)
-        for (int c = lane; c < n; c += 64) {
+    const double gd = gam[1];
+    gj_invert_regs<cplx<double>, 4, 1>(
+        n,
+        [&](int r, int c) {
             const cplx<double> x = src[r * kDBlock + c];
-            A[r * n + c] = r == c ? gj_addr(x, gam[1]) : x;
-        }
-    gj_invert<cplx<double>>(A, n);
-    for (int r = wave; r < n; r += waves)
-        for (int c = lane; c < n; c += 64) src[c * kDBlock + r] = A[r * n + c];
+            return r == c ? gj_addr(x, gd) : x;
+        },
+        [&](int r, int c, cplx<double> x) { src[c * kDBlock + r] = x; });
 }
 
 // parameter index of W at (site v, unit j): RBM v M + j; PsiDeep N + M + i M + j with
@@ -619,11 +657,7 @@ void launch_kfac_setup(const uint64_t* confs, int words, const uint64_t* bits, i
     ANN_LAUNCH_CHECK();
     kfac_damping_kernel<<<1, 256, 0, stream>>>(kw.C, N, kw.Dblk, M, shift, kw.gam);
     ANN_LAUNCH_CHECK();
-    // dynamic LDS beyond the default limit is opted in (up to 128 KiB here, of 160 KiB per CU)
-    const size_t bytes = std::max(sizeof(double) * N * N, sizeof(cplx<double>) * kDBlock * kDBlock);
-    ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kfac_invert_kernel),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
-    kfac_invert_kernel<<<unsigned(1 + nblk), kInvThreads, bytes, stream>>>(kw.C, N, kw.gam, kw.Ci, kw.Dblk, M);
+    kfac_invert_kernel<<<unsigned(1 + nblk), kInvThreads, 0, stream>>>(kw.C, N, kw.gam, kw.Ci, kw.Dblk, M);
     ANN_LAUNCH_CHECK();
 }
 

@@ -135,8 +135,8 @@ void Psi::sync_device() {
         ANN_HIP_CHECK(hipMemcpyAsync(params_d.p, stage.p, sizeof(cplx<double>) * num_params, hipMemcpyHostToDevice,
                                      S()));
         ANN_HIP_CHECK(hipEventRecord(stage_ev.get(), S()));
-        launch_net_repack(layout, N, M, Mpad, C, params_d.p, kind == PSI_DEEP ? conn_d.p : nullptr, npad, Wd64.p,
-                          Wd32.p, b64.p, b32.p, wfrag.p, S());
+        launch_net_repack(layout, N, M, Mpad, C, conn_shift, params_d.p, kind == PSI_DEEP ? conn_d.p : nullptr, npad,
+                          Wd64.p, Wd32.p, b64.p, b32.p, wfrag.p, S());
         params_dirty = false;
         return;
     }
@@ -201,7 +201,20 @@ void Psi::sync_device() {
     std::vector<cd> z(Mpad, 0.0);
     up64(zero64, z);
     up32(zero32, z);
-    if (kind == PSI_DEEP) conn_d.upload(conn.data(), conn.size());
+    conn_shift = -1;
+    if (kind == PSI_DEEP) {
+        conn_d.upload(conn.data(), conn.size());
+        // circulant layout: connection i of unit j is site (j d + i) mod N
+        const int d = M > 1 ? int(conn[1]) : 0;
+        bool circ = C <= N;
+        for (int i = 0; i < C && circ; i++)
+            for (int j = 0; j < M; j++)
+                if (int(conn[size_t(i) * M + j]) != int((int64_t(j) * d + i) % N)) {
+                    circ = false;
+                    break;
+                }
+        conn_shift = circ ? d : -1;
+    }
     std::vector<uint16_t> frag;
     pack_theta_fragments(reinterpret_cast<const double*>(Wd.data()), N, Mpad, 3, frag, npad);
     wfrag.upload(frag.data(), frag.size());

@@ -202,6 +202,7 @@ struct Psi {
     PinnedBuf<cplx<double>> stage;
     DevBuf<cplx<double>> params_d;
     OwnedEvent stage_ev;   // the staged copy has left the host buffer
+    int conn_shift = -1;   // PsiDeep: connection i of unit j is site (j shift + i) mod N (else -1)
     bool real_f = true;
     DevBuf<cplx<double>> Wd64, b64, f64;
     DevBuf<cplx<float>> Wd32, b32, f32;

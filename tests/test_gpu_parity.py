@@ -277,7 +277,7 @@ def test_structured_matvec_general_connections(precision, repeated):
     assert rel((S + 1e-3 * np.eye(psi.num_params)) @ x, tdvp.F_vector) < 1e-5
 
 
-@pytest.mark.parametrize("kind", ["rbm", "deep", "deep_repeated"])
+@pytest.mark.parametrize("kind", ["rbm", "deep", "deep_repeated", "deep_circulant"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_params_update_matches_fresh_upload(precision, kind):
     """New parameters of a net already on the device (staged and repacked there by a
@@ -288,6 +288,8 @@ def test_params_update_matches_fresh_upload(precision, kind):
     def make():
         if kind == "rbm":
             return A.PsiRBM(cgauss(np.random.default_rng(1), (N, M), 0.1), 1.0 - 0.3j, 0.1)
+        if kind == "deep_circulant":
+            return A.seeded_rbm_deep(N, M, 1, sigma=0.1)
         return shuffled_deep(N, M, C, 1, kind == "deep_repeated")
 
     a = make()
