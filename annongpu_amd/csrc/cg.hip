@@ -134,14 +134,25 @@ __device__ inline double cg_tol2(const double* fnorm2, double rtol) {
     return tol * tol;
 }
 
+// (mirror: a host-visible copy of done, written every check)
 __global__ __launch_bounds__(256) void cg_check_kernel(const double* part, int parts, const double* fnorm2,
-                                                       double rtol, int iter, int* done) {
+                                                       double rtol, int iter, int* done, int* mirror) {
     __shared__ double sh[4];
-    if (done[0]) return;
-    const double rr = total(part, parts, 4, 2, sh);
-    if (threadIdx.x == 0 && rr <= cg_tol2(fnorm2, rtol)) {
-        done[0] = 1;
-        done[1] = iter;
+    int d0 = done[0], d1 = done[1];
+    if (!d0) {
+        const double rr = total(part, parts, 4, 2, sh);
+        if (rr <= cg_tol2(fnorm2, rtol)) {
+            d0 = 1;
+            d1 = iter;
+            if (threadIdx.x == 0) {
+                done[0] = 1;
+                done[1] = iter;
+            }
+        }
+    }
+    if (mirror && threadIdx.x == 0) {
+        mirror[0] = d0;
+        mirror[1] = d1;
     }
 }
 
@@ -163,8 +174,9 @@ __global__ __launch_bounds__(256) void cg_final_kernel(const double* part, int p
 
 }  // namespace
 
-void launch_pcg_check(const double* part, const double* fnorm2, double rtol, int iter, int* done, hipStream_t stream) {
-    cg_check_kernel<<<1, 256, 0, stream>>>(part, kCgParts, fnorm2, rtol, iter, done);
+void launch_pcg_check(const double* part, const double* fnorm2, double rtol, int iter, int* done, hipStream_t stream,
+                      int* mirror) {
+    cg_check_kernel<<<1, 256, 0, stream>>>(part, kCgParts, fnorm2, rtol, iter, done, mirror);
     ANN_LAUNCH_CHECK();
 }
 

@@ -102,7 +102,9 @@ void launch_pcg_step(cplx<double>* x, cplx<double>* r, cplx<double>* z, const cp
                      hipStream_t stream);
 void launch_pcg_dir(cplx<double>* p, const cplx<double>* z, int64_t n, const double* newpart, const double* oldpart,
                     const int* done, hipStream_t stream);
-void launch_pcg_check(const double* part, const double* fnorm2, double rtol, int iter, int* done, hipStream_t stream);
+// mirror (optional): device address of host-visible memory that receives done
+void launch_pcg_check(const double* part, const double* fnorm2, double rtol, int iter, int* done, hipStream_t stream,
+                      int* mirror = nullptr);
 // out[4] = (r^H r, iteration, ||F||^2, converged) of the solve's result (part: the
 // partials of every iteration, [iteration][kCgParts][4]; it: iterations run)
 void launch_pcg_final(const double* part, int it, const int* done, const double* fnorm2, double* out,

@@ -1297,7 +1297,10 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     done.ensure(2);
     ANN_HIP_CHECK(hipMemsetAsync(done.p, 0, sizeof(int) * 2, S()));
     launch_pcg_check(part.p, fnorm2.p, rtol, 0, done.p, S());
-    t.flag_host.ensure(4);
+    if (!t.flag_host.p) {
+        t.flag_host.ensure(4);
+        ANN_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&t.flag_dev), t.flag_host.p, 0));
+    }
     hipEvent_t ev[2] = {t.cg_ev[0].get(), t.cg_ev[1].get()};
     const int check = 4;
     int it = 0, pending = -1;   // slot of the last issued, not yet read flag copy
@@ -1357,10 +1360,9 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         }
         it++;
         if (it % check == 0 || it == max_iter) {
-            launch_pcg_check(part.p + size_t(it) * stride, fnorm2.p, rtol, it, done.p, S());
             const int slot = (it / check) & 1;
-            ANN_HIP_CHECK(hipMemcpyAsync(t.flag_host.p + 2 * slot, done.p, sizeof(int) * 2, hipMemcpyDeviceToHost,
-                                         S()));
+            // the check writes the flag straight into the page-locked slot (no copy)
+            launch_pcg_check(part.p + size_t(it) * stride, fnorm2.p, rtol, it, done.p, S(), t.flag_dev + 2 * slot);
             ANN_HIP_CHECK(hipEventRecord(ev[slot], S()));
             if (pending >= 0) {
                 ANN_HIP_CHECK(hipEventSynchronize(ev[pending]));
