@@ -627,9 +627,9 @@ __global__ __launch_bounds__(kStepThreads) void kfac_step_kernel(KfacStep a, int
 
 }  // namespace
 
-void launch_kfac_setup(const uint64_t* confs, int words, const uint64_t* bits, int vpad, bool w_uniform,
-                       const cplx<float>* g32, const cplx<double>* g64, int Mpad, const double* w, int64_t count, int N,
-                       int M, double shift, KfacWork& kw, hipStream_t stream) {
+void launch_kfac_moments(const uint64_t* confs, int words, const uint64_t* bits, int vpad, bool w_uniform,
+                         const cplx<float>* g32, const cplx<double>* g64, int Mpad, const double* w, int64_t count,
+                         int N, int M, KfacWork& kw, hipStream_t stream) {
     if (N < 1 || N > kKfacMaxSites) throw std::invalid_argument("the K-FAC preconditioner supports 1..128 sites");
     if (M < 1 || M > kKfacMaxUnits) throw std::invalid_argument("the K-FAC preconditioner supports 1..2048 hidden units");
     const int64_t nsub = std::min<int64_t>(count, kKfacSamples);
@@ -655,10 +655,22 @@ void launch_kfac_setup(const uint64_t* confs, int words, const uint64_t* bits, i
                                                                          kw.gpart, ggroups, N, M, nblk, kw.C,
                                                                          kw.Dblk);
     ANN_LAUNCH_CHECK();
+}
+
+void launch_kfac_finish(int N, int M, double shift, KfacWork& kw, hipStream_t stream) {
+    if (N < 1 || N > kKfacMaxSites || M < 1 || M > kKfacMaxUnits) throw std::invalid_argument("K-FAC factor sizes");
+    const int nblk = kfac_dblocks(M);
     kfac_damping_kernel<<<1, 256, 0, stream>>>(kw.C, N, kw.Dblk, M, shift, kw.gam);
     ANN_LAUNCH_CHECK();
     kfac_invert_kernel<<<unsigned(1 + nblk), kInvThreads, 0, stream>>>(kw.C, N, kw.gam, kw.Ci, kw.Dblk, M);
     ANN_LAUNCH_CHECK();
+}
+
+void launch_kfac_setup(const uint64_t* confs, int words, const uint64_t* bits, int vpad, bool w_uniform,
+                       const cplx<float>* g32, const cplx<double>* g64, int Mpad, const double* w, int64_t count, int N,
+                       int M, double shift, KfacWork& kw, hipStream_t stream) {
+    launch_kfac_moments(confs, words, bits, vpad, w_uniform, g32, g64, Mpad, w, count, N, M, kw, stream);
+    launch_kfac_finish(N, M, shift, kw, stream);
 }
 
 void launch_kfac_index(int layout, int N, int M, int Mpad, const int* inv, int* idx, hipStream_t stream) {

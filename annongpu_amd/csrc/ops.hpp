@@ -87,6 +87,12 @@ size_t kfac_gpart_elems(int M, int64_t count);
 void launch_kfac_setup(const uint64_t* confs, int words, const uint64_t* bits, int vpad, bool w_uniform,
                        const cplx<float>* g32, const cplx<double>* g64, int Mpad, const double* w, int64_t count, int N,
                        int M, double shift, KfacWork& kw, hipStream_t stream);
+// the set-up in two parts: the factors C and D from the samples (kw.C, kw.Dblk), then
+// the damping for `shift` and the inverses (Ci, Dblk inverted in place)
+void launch_kfac_moments(const uint64_t* confs, int words, const uint64_t* bits, int vpad, bool w_uniform,
+                         const cplx<float>* g32, const cplx<double>* g64, int Mpad, const double* w, int64_t count,
+                         int N, int M, KfacWork& kw, hipStream_t stream);
+void launch_kfac_finish(int N, int M, double shift, KfacWork& kw, hipStream_t stream);
 // idx of KfacView: RBM v M + j, PsiDeep N + M + inv[v][j] M + j
 void launch_kfac_index(int layout, int N, int M, int Mpad, const int* inv, int* idx, hipStream_t stream);
 void launch_kfac_step(const KfacView& kv, const KfacStep& a, hipStream_t stream);

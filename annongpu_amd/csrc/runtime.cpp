@@ -17,7 +17,14 @@ Runtime& Runtime::get() {
     return rt;
 }
 
+hipStream_t Runtime::side() {
+    stream();
+    if (!side_) ANN_HIP_CHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    return side_;
+}
+
 hipStream_t Runtime::stream() {
+    if (override_) return override_;
     if (!stream_) {
         ANN_HIP_CHECK(hipSetDevice(device_));
         ANN_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -39,6 +46,11 @@ void Runtime::set_device(int device) {
         pool_.clear();
         ANN_HIP_CHECK(hipStreamDestroy(stream_));
         stream_ = nullptr;
+        if (side_) {
+            ANN_HIP_CHECK(hipStreamSynchronize(side_));
+            ANN_HIP_CHECK(hipStreamDestroy(side_));
+            side_ = nullptr;
+        }
     }
     device_ = device;
     ANN_HIP_CHECK(hipSetDevice(device_));
@@ -725,48 +737,36 @@ void structured_sums(Tdvp& t, const DenseView<T>& net, const uint64_t* confs, in
                       net.num_params, out, S());
 }
 
+// K-FAC factor buffers of the Tdvp's current sizes
+KfacWork kfac_work(Tdvp& t) {
+    t.kf_spart.ensure(kfac_spart_elems(t.N, t.count));
+    t.kf_gpart.ensure(kfac_gpart_elems(t.M, t.count));
+    t.kf_C.ensure(size_t(t.N) * t.N);
+    t.kf_Ci.ensure(size_t(t.N) * t.N);
+    t.kf_D.ensure(kfac_dblock_elems(t.M));
+    t.kf_gam.ensure(2);
+    return {t.kf_spart.p, t.kf_gpart.p, t.kf_C.p, t.kf_Ci.p, t.kf_D.p, t.kf_gam.p};
+}
+
+// The statistics that do not need E_loc (<O>, the diagonal sums of S) and, when the
+// last solve of this Tdvp used K-FAC, the factor moments of the new samples run on the
+// side stream while the local energies run on the library stream.
 template <typename T>
 void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     sample(psi, ens);
     t.bits_valid = false;   // new samples: their transposed spin bits are built on first use
+    t.kf_ready = false;
     Samples& s = ens.samples;
-    local_energies(psi, op, s);
     const int64_t n = s.count;
     const int P = psi.num_params;
-    // E, E2 and sum w stay on the device until the end of the evaluation (no
-    // synchronisation between the local energies and the statistics)
-    t.sums.ensure(4);
-    launch_weighted_sums(s.e_loc.p, s.w.p, n, t.sums.p, S());
     auto net = Prec<T>::view(psi);
     auto& g = Prec<T>::g(t);
     {
         ScopedTimer timer("o_k");
         g_factor_padded<T>(net, Prec<T>::th(s).p, n, g);
     }
-    DevBuf<cplx<double>>& coef = t.coef;
-    coef.ensure(n);
-    t.Obar.ensure(P);
-    t.Fd.ensure(P);
-    launch_weight_coef(s.w.p, nullptr, n, coef.p, S());
-    structured_sums<T>(t, net, s.confs.p, s.words, g.p, 0, coef.p, n, t.Obar.p, 0.0, nullptr);
-    // diagonal of S for the Jacobi preconditioner: sum_s w |g_sj|^2 - |<O_k>|^2
-    structured_sums<T>(t, net, s.confs.p, s.words, g.p, 2, coef.p, n, nullptr, 0.0, nullptr);
-    t.diag.ensure(P);
-    launch_sr_diag(net.layout, net.N, net.M, net.Mpad, t.sums.p + 3, t.Yb.p, t.Obar.p, P, t.diag.p, S());
-    // F = sum w conj(O) E_loc - E conj(<O>) (TDVP.cu.template:300) = sum w (E_loc - E) conj(O)
-    launch_weight_coef_centered_dev(s.w.p, s.e_loc.p, t.sums.p, n, coef.p, S());
-    structured_sums<T>(t, net, s.confs.p, s.words, g.p, 1, coef.p, n, t.Fd.p, 0.0, nullptr);
-    t.sums_host.ensure(4);
-    ANN_HIP_CHECK(hipMemcpyAsync(t.sums_host.p, t.sums.p, sizeof(double) * 4, hipMemcpyDeviceToHost, S()));
-    Runtime::get().sync();
-    t.E = cd(t.sums_host.p[0], t.sums_host.p[1]);
-    t.E2 = t.sums_host.p[2];
-    t.wsum = t.sums_host.p[3];
-    t.O.resize(P);
-    t.F.resize(P);
-    t.Obar.download(reinterpret_cast<cplx<double>*>(t.O.data()), P);
-    t.Fd.download(reinterpret_cast<cplx<double>*>(t.F.data()), P);
-    // keep the structured O for S.v / CG
+    // the evaluation's description of the structured O (for S.v, CG and K-FAC); the
+    // connection tables are uploaded when the connections change
     t.precision = psi.precision;
     t.words = s.words;
     t.N = psi.N;
@@ -776,8 +776,13 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     t.layout = psi.layout;
     t.npad = psi.npad;
     t.count = n;
-    t.conn_shift = -1;
-    if (psi.kind == PSI_DEEP) {
+    if (psi.kind != PSI_DEEP) {
+        t.conn_shift = -1;
+        t.conn_host.clear();
+    } else if (t.conn_host != psi.conn || t.conn_mpad != psi.Mpad || t.conn_n != psi.N) {
+        t.conn_host = psi.conn;
+        t.conn_mpad = psi.Mpad;
+        t.conn_n = psi.N;
         t.conn.upload(psi.conn.data(), psi.conn.size());
         std::vector<int> inv(size_t(psi.N) * psi.Mpad, -1);
         for (int i = 0; i < psi.C; i++)
@@ -798,6 +803,60 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
                 }
         t.conn_shift = circ ? d : -1;
     }
+    DevBuf<cplx<double>>& coef = t.coef;
+    coef.ensure(n);
+    t.Obar.ensure(P);
+    t.Fd.ensure(P);
+    t.diag.ensure(P);
+    t.sums.ensure(4);
+    launch_weight_coef(s.w.p, nullptr, n, coef.p, S());
+    const bool pre_kfac = t.kfac_hint && psi.N <= kKfacMaxSites && psi.M <= kKfacMaxUnits &&
+                          (psi.layout == LAYOUT_RBM || t.inv_unique);
+    KfacWork kw{};
+    if (pre_kfac) kw = kfac_work(t);
+    if (sizeof(T) == 4 || pre_kfac) {
+        // the transposed spin bits (fp32 statistics and S.v, the K-FAC spin covariance)
+        // are shared by both streams' kernels: built before the fork
+        const SpinPlan plan = spin_plan(net.N, net.M, net.Mpad, n, true);
+        t.spin_bits.ensure(size_t(plan.bit_blocks) * plan.vpad);
+        launch_spin_bits(s.confs.p, s.words, n, net.N, plan.vpad, t.spin_bits.p, S());
+        t.bits_valid = true;
+    }
+    ANN_HIP_CHECK(hipEventRecord(t.fork_ev.get(), S()));
+    {
+        SideStream side;
+        ANN_HIP_CHECK(hipStreamWaitEvent(S(), t.fork_ev.get(), 0));
+        structured_sums<T>(t, net, s.confs.p, s.words, g.p, 0, coef.p, n, t.Obar.p, 0.0, nullptr);
+        // diagonal of S for the Jacobi preconditioner: sum_s w |g_sj|^2 - |<O_k>|^2
+        structured_sums<T>(t, net, s.confs.p, s.words, g.p, 2, coef.p, n, nullptr, 0.0, nullptr);
+        if (pre_kfac) {
+            ScopedTimer timer("kfac_setup");
+            const SpinPlan plan = spin_plan(t.N, t.M, t.Mpad, n, t.layout == LAYOUT_DEEP1);
+            launch_kfac_moments(s.confs.p, s.words, t.spin_bits.p, plan.vpad, s.w_uniform,
+                                t.precision != 0 ? t.g32.p : nullptr, t.precision != 0 ? nullptr : t.g64.p, t.Mpad,
+                                s.w.p, n, t.N, t.M, kw, S());
+            t.kf_ready = true;
+        }
+        ANN_HIP_CHECK(hipEventRecord(t.join_ev.get(), S()));
+    }
+    local_energies(psi, op, s);
+    // E, E2 and sum w stay on the device until the end of the evaluation
+    launch_weighted_sums(s.e_loc.p, s.w.p, n, t.sums.p, S());
+    ANN_HIP_CHECK(hipStreamWaitEvent(S(), t.join_ev.get(), 0));
+    launch_sr_diag(net.layout, net.N, net.M, net.Mpad, t.sums.p + 3, t.Yb.p, t.Obar.p, P, t.diag.p, S());
+    // F = sum w conj(O) E_loc - E conj(<O>) (TDVP.cu.template:300) = sum w (E_loc - E) conj(O)
+    launch_weight_coef_centered_dev(s.w.p, s.e_loc.p, t.sums.p, n, coef.p, S());
+    structured_sums<T>(t, net, s.confs.p, s.words, g.p, 1, coef.p, n, t.Fd.p, 0.0, nullptr);
+    t.sums_host.ensure(4);
+    ANN_HIP_CHECK(hipMemcpyAsync(t.sums_host.p, t.sums.p, sizeof(double) * 4, hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    t.E = cd(t.sums_host.p[0], t.sums_host.p[1]);
+    t.E2 = t.sums_host.p[2];
+    t.wsum = t.sums_host.p[3];
+    t.O.resize(P);
+    t.F.resize(P);
+    t.Obar.download(reinterpret_cast<cplx<double>*>(t.O.data()), P);
+    t.Fd.download(reinterpret_cast<cplx<double>*>(t.F.data()), P);
     t.confs.ensure(size_t(n) * s.words);
     ANN_HIP_CHECK(hipMemcpyAsync(t.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice, S()));
     t.w.ensure(n);
@@ -1206,6 +1265,7 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     // method 1: Jacobi preconditioner; method 3: K-FAC on the W parameters of a dense
     // net (the biases keep Jacobi), factors from this evaluation's samples
     const bool kfac = method == 3;
+    t.kfac_hint = kfac;
     // K-FAC with the fp32 S.v of a circulant PsiDeep with C = N keeps the solve's
     // vectors in site-major order (the S.v and K-FAC kernels then read and write whole
     // rows of 64 units); F, x0 and <O> are converted on entry and x on exit
@@ -1245,28 +1305,29 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         if (t.generic) throw std::invalid_argument("the K-FAC preconditioner needs a PsiRBM or one-layer PsiDeep");
         if (t.layout == LAYOUT_DEEP1 && !t.inv_unique)
             throw std::invalid_argument("the K-FAC preconditioner needs distinct connection sites per hidden unit");
-        t.kf_spart.ensure(kfac_spart_elems(t.N, t.count));
-        t.kf_gpart.ensure(kfac_gpart_elems(t.M, t.count));
-        t.kf_C.ensure(size_t(t.N) * t.N);
-        t.kf_Ci.ensure(size_t(t.N) * t.N);
-        t.kf_D.ensure(kfac_dblock_elems(t.M));
-        t.kf_gam.ensure(2);
         t.kf_idx.ensure(size_t(t.N) * t.M);
-        KfacWork kw{t.kf_spart.p, t.kf_gpart.p, t.kf_C.p, t.kf_Ci.p, t.kf_D.p, t.kf_gam.p};
+        KfacWork kw = kfac_work(t);
         {
             ScopedTimer timer("kfac_setup");
-            // the transposed spin bits (shared with the fp32 S.v) give the spin covariance
-            const SpinPlan plan = spin_plan(t.N, t.M, t.Mpad, t.count, t.layout == LAYOUT_DEEP1);
-            if (!t.bits_valid) {
-                t.spin_bits.ensure(size_t(plan.bit_blocks) * plan.vpad);
-                launch_spin_bits(t.confs.p, t.words, t.count, t.N, plan.vpad, t.spin_bits.p, S());
-                t.bits_valid = true;
+            if (!t.kf_ready) {
+                // factor moments of this evaluation's samples (else computed during the
+                // evaluation, when the previous solve of this Tdvp also used K-FAC); the
+                // transposed spin bits (shared with the fp32 S.v) give the spin covariance
+                const SpinPlan plan = spin_plan(t.N, t.M, t.Mpad, t.count, t.layout == LAYOUT_DEEP1);
+                if (!t.bits_valid) {
+                    t.spin_bits.ensure(size_t(plan.bit_blocks) * plan.vpad);
+                    launch_spin_bits(t.confs.p, t.words, t.count, t.N, plan.vpad, t.spin_bits.p, S());
+                    t.bits_valid = true;
+                }
+                launch_kfac_moments(t.confs.p, t.words, t.spin_bits.p, plan.vpad, t.w_uniform,
+                                    t.precision != 0 ? t.g32.p : nullptr, t.precision != 0 ? nullptr : t.g64.p,
+                                    t.Mpad, t.w.p, t.count, t.N, t.M, kw, S());
             }
+            // the moments are used once: the inversion overwrites D
+            t.kf_ready = false;
             launch_kfac_index(t.layout, t.N, t.M, t.Mpad, t.layout == LAYOUT_DEEP1 ? t.inv.p : nullptr, t.kf_idx.p,
                               S());
-            launch_kfac_setup(t.confs.p, t.words, t.spin_bits.p, plan.vpad, t.w_uniform,
-                              t.precision != 0 ? t.g32.p : nullptr, t.precision != 0 ? nullptr : t.g64.p, t.Mpad,
-                              t.w.p, t.count, t.N, t.M, shift, kw, S());
+            launch_kfac_finish(t.N, t.M, shift, kw, S());
         }
         jac_end = t.layout == LAYOUT_DEEP1 ? int64_t(t.N) + t.M : 0;
         kv = {t.N, t.M, t.kf_idx.p, t.order(), t.kf_Ci.p, t.kf_D.p};

@@ -68,6 +68,10 @@ class Runtime {
 public:
     static Runtime& get();
     hipStream_t stream();
+    // a second stream (default priority) for work that overlaps the library stream;
+    // while a SideStream guard lives, stream() returns it
+    hipStream_t side();
+    hipStream_t override_ = nullptr;
     void set_device(int device);
     void sync();
     // per-kernel-class device timers (hipEvents on the library stream); a class
@@ -81,6 +85,7 @@ public:
 
 private:
     hipStream_t stream_ = nullptr;
+    hipStream_t side_ = nullptr;
     int device_ = 0;
     struct Pending {
         std::string name;
@@ -91,6 +96,14 @@ private:
     std::string open_name_;
     hipEvent_t open_a_ = nullptr;
     hipEvent_t event();
+};
+
+// launches inside the guard go to the side stream (no synchronisation may happen there)
+struct SideStream {
+    SideStream() { Runtime::get().override_ = Runtime::get().side(); }
+    ~SideStream() { Runtime::get().override_ = nullptr; }
+    SideStream(const SideStream&) = delete;
+    SideStream& operator=(const SideStream&) = delete;
 };
 
 // device time of the enclosed launches under `name` (nothing when profiling is
@@ -355,6 +368,11 @@ struct Tdvp {
     int conn_shift = -1;         // PsiDeep: connection i of unit j is site (j shift + i) mod N (else -1)
     // K-FAC preconditioner (kfac.hip)
     DevBuf<double> kf_spart, kf_C, kf_Ci, kf_gam;
+    bool kfac_hint = false;      // the last solve used K-FAC: the next evaluation computes the factor moments
+    bool kf_ready = false;       // kf_C / kf_D hold the moments of the current samples
+    OwnedEvent fork_ev, join_ev; // the evaluation's side-stream fork and join
+    std::vector<uint32_t> conn_host;   // connections of the uploaded tables (PsiDeep)
+    int conn_mpad = 0, conn_n = 0;
     DevBuf<cplx<double>> kf_gpart, kf_D;
     WParamMap wmap() const { return {layout == LAYOUT_RBM ? 1 : 0, N, M, C, conn_shift}; }
     // while a CG solve runs with its vectors in site-major order (circulant PsiDeep,

@@ -133,3 +133,30 @@ def test_kfac_rejects_generic_nets():
     tdvp.eval_F(A.Operator(ham.transverse_ising_chain(8, 1.0, 0.5)), psi, ens)
     with pytest.raises(ValueError):
         tdvp.solve(1e-3, "cg_kfac", rtol=1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_kfac_moments_precomputed_during_eval(precision):
+    """After a K-FAC solve the next evaluation of the same TDVP computes the factor
+    moments on a second stream while the local energies run; its statistics and solve
+    equal bit for bit those of a TDVP that computes the moments in the solve."""
+    N, M = 16, 32
+    psi = A.seeded_rbm_deep(N, M, 3, sigma=0.05)
+    psi.precision = precision
+    op = A.Operator(ham.heisenberg_chain(N))
+    t1 = A.TDVP(psi.num_params)
+    t1.eval_F(op, psi, A.MonteCarloSpins(2048, 2, 10, 512, move="exchange", seed=4))
+    t1.solve(1e-3, "cg_kfac", rtol=1e-6, max_iter=500)
+    t2 = A.TDVP(psi.num_params)
+    t1.eval_F(op, psi, A.MonteCarloSpins(2048, 2, 10, 512, move="exchange", seed=9))
+    t2.eval_F(op, psi, A.MonteCarloSpins(2048, 2, 10, 512, move="exchange", seed=9))
+    assert t1.E_local == t2.E_local
+    assert np.array_equal(t1.F_vector, t2.F_vector)
+    x1 = t1.solve(1e-3, "cg_kfac", rtol=1e-6, max_iter=500)
+    x2 = t2.solve(1e-3, "cg_kfac", rtol=1e-6, max_iter=500)
+    assert t1.last_solve["iterations"] == t2.last_solve["iterations"]
+    assert np.array_equal(x1, x2)
+    # a second solve of the same evaluation recomputes the moments (the first one's
+    # inversion overwrote them)
+    x3 = t1.solve(1e-3, "cg_kfac", rtol=1e-6, max_iter=500)
+    assert np.array_equal(x1, x3)
