@@ -665,6 +665,7 @@ int ann_tdvp_get_S(ann_tdvp tdvp, double* S) {
 int ann_tdvp_get_F(ann_tdvp tdvp, double* F) {
     return guard([&] {
         need(tdvp, "tdvp");
+        tdvp->impl.fetch_host_vectors();
         std::memcpy(F, tdvp->impl.F.data(), sizeof(cd) * tdvp->impl.F.size());
     });
 }
@@ -672,6 +673,7 @@ int ann_tdvp_get_F(ann_tdvp tdvp, double* F) {
 int ann_tdvp_get_O(ann_tdvp tdvp, double* O) {
     return guard([&] {
         need(tdvp, "tdvp");
+        tdvp->impl.fetch_host_vectors();
         std::memcpy(O, tdvp->impl.O.data(), sizeof(cd) * tdvp->impl.O.size());
     });
 }
@@ -680,6 +682,7 @@ int ann_tdvp_get_energy(ann_tdvp tdvp, double* E2c, double* E2_local, double* va
     return guard([&] {
         need(tdvp, "tdvp");
         auto& t = tdvp->impl;
+        t.settle();
         if (E2c) *C(E2c) = t.E;
         if (E2_local) *E2_local = t.E2;
         if (var_H) *var_H = t.E2 - std::norm(t.E);   // TDVP.hpp:69-71

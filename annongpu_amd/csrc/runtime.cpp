@@ -505,6 +505,25 @@ static void finish_chains(const Psi& psi, Ensemble& ens, int64_t therm, int64_t 
     ens.acc_pending = true;
 }
 
+void Tdvp::settle() {
+    if (!sums_pending) return;
+    Runtime::get().sync();
+    E = cd(sums_host.p[0], sums_host.p[1]);
+    E2 = sums_host.p[2];
+    wsum = sums_host.p[3];
+    sums_pending = false;
+}
+
+void Tdvp::fetch_host_vectors() {
+    settle();
+    if (!of_pending) return;
+    O.resize(P);
+    F.resize(P);
+    Obar.download(reinterpret_cast<cplx<double>*>(O.data()), P);
+    Fd.download(reinterpret_cast<cplx<double>*>(F.data()), P);
+    of_pending = false;
+}
+
 void Ensemble::settle_acceptance() {
     if (!acc_pending) return;
     Runtime::get().sync();
@@ -847,16 +866,13 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     // F = sum w conj(O) E_loc - E conj(<O>) (TDVP.cu.template:300) = sum w (E_loc - E) conj(O)
     launch_weight_coef_centered_dev(s.w.p, s.e_loc.p, t.sums.p, n, coef.p, S());
     structured_sums<T>(t, net, s.confs.p, s.words, g.p, 1, coef.p, n, t.Fd.p, 0.0, nullptr);
+    // E, E2 and sum w go to page-locked memory; the host reads them, and <O> and F,
+    // when they are asked for (the evaluation returns without synchronising, so a
+    // solve that follows is queued behind it at once)
     t.sums_host.ensure(4);
     ANN_HIP_CHECK(hipMemcpyAsync(t.sums_host.p, t.sums.p, sizeof(double) * 4, hipMemcpyDeviceToHost, S()));
-    Runtime::get().sync();
-    t.E = cd(t.sums_host.p[0], t.sums_host.p[1]);
-    t.E2 = t.sums_host.p[2];
-    t.wsum = t.sums_host.p[3];
-    t.O.resize(P);
-    t.F.resize(P);
-    t.Obar.download(reinterpret_cast<cplx<double>*>(t.O.data()), P);
-    t.Fd.download(reinterpret_cast<cplx<double>*>(t.F.data()), P);
+    t.sums_pending = true;
+    t.of_pending = true;
     t.confs.ensure(size_t(n) * s.words);
     ANN_HIP_CHECK(hipMemcpyAsync(t.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice, S()));
     t.w.ensure(n);
@@ -878,11 +894,11 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
         }
         // reference S = sum w conj(O) O - conj(<O>) <O>; with centred O this needs
         // (1 - sum w) conj(<O>) <O> added back (zero for normalised weights)
+        t.settle();
         if (std::abs(1.0 - t.wsum) > 0.0) launch_rank1(t.S.p, t.Obar.p, P, 1.0 - t.wsum, S());
         Runtime::get().sync();
         t.has_S = true;
     }
-    Runtime::get().sync();
 }
 
 template <typename T>
@@ -937,6 +953,8 @@ void tdvp_eval_generic(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with
     t.E = cd(h[0], h[1]);
     t.E2 = h[2];
     t.wsum = h[3];
+    t.sums_pending = false;
+    t.of_pending = false;
     {
         // F = sum w (E_loc - E) conj(O)  (TDVP.cu.template:300 without the cancellation)
         ScopedTimer timer("spin_sums");
@@ -1219,6 +1237,7 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         if (hinfo != 0) return 1;
         x.download(reinterpret_cast<cplx<double>*>(x_out), P);
         // residual || (S + shift) x - F || / || F ||
+        t.fetch_host_vectors();
         DevBuf<cplx<double>> y;
         y.ensure(P);
         launch_dense_matvec(t.S.p, x.p, P, shift, y.p, S());

@@ -371,6 +371,11 @@ struct Tdvp {
     bool kfac_hint = false;      // the last solve used K-FAC: the next evaluation computes the factor moments
     bool kf_ready = false;       // kf_C / kf_D hold the moments of the current samples
     OwnedEvent fork_ev, join_ev; // the evaluation's side-stream fork and join
+    // E, E2, wsum (sums_host) and O, F (from Obar, Fd) of the last evaluation are read
+    // to the host on first use
+    bool sums_pending = false, of_pending = false;
+    void settle();
+    void fetch_host_vectors();
     std::vector<uint32_t> conn_host;   // connections of the uploaded tables (PsiDeep)
     int conn_mpad = 0, conn_n = 0;
     DevBuf<cplx<double>> kf_gpart, kf_D;

@@ -139,6 +139,9 @@ class Stepper:
         self.x = None
         self.iters = []
         self.residuals = []
+        # the host copy of the parameters is kept here (psi is only written by this
+        # stepper while it runs), so a step does not read them back from the library
+        self.params = psi.params.copy()
 
     def step(self):
         self.tdvp.eval_F(self.op, self.psi, self.ens)
@@ -153,7 +156,8 @@ class Stepper:
         m = np.abs(d).max()
         if m > self.clip:
             d *= self.clip / m
-        self.psi.params = self.psi.params - d
+        self.params -= d
+        self.psi.params = self.params
 
     def timed(self, steps, warmup):
         A = self.A
