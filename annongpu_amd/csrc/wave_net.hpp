@@ -88,6 +88,33 @@ __device__ inline cplx<T> lane_sum_realf(const cplx<T> (&th)[U], const cplx<T> (
     return s;
 }
 
+// eight lane partials summed over the wave in one transposed reduction: lane l ends
+// with the total of part[(l >> 3) & 7].  Lanes l and l + 32 exchange halves
+// (permlane32 swap), then rows 2r and 2r + 1 (permlane16 swap), then lanes l and l ^ 8
+// (DPP row_ror:8); the last three levels are DPP sums inside groups of 8 lanes.
+__device__ inline float wave_sum8(const float (&part)[8]) {
+    const int lane = threadIdx.x & 63;
+    float s4[4], s2[2];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(part[i]), __float_as_uint(part[4 + i]), false,
+                                                        false);
+        s4[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);   // part 4 (l >> 5) + i
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(s4[i]), __float_as_uint(s4[2 + i]), false,
+                                                        false);
+        s2[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);   // part 4 (l >> 5) + 2 ((l >> 4) & 1) + i
+    }
+    const bool b3 = (lane >> 3) & 1;
+    float s = (b3 ? s2[1] : s2[0]) + dpp_f32<0x128>(b3 ? s2[0] : s2[1]);
+    s += dpp_f32<0x141>(s);   // row_half_mirror: lanes i and 7 - i of 8
+    s += dpp_f32<0xB1>(s);    // quad_perm [1, 0, 3, 2]
+    s += dpp_f32<0x4E>(s);    // quad_perm [2, 3, 0, 1]
+    return s;
+}
+
 template <typename T>
 __device__ inline cplx<T> wave_csum(cplx<T> v) {
     return {wave_sum(v.re), wave_sum(v.im)};
