@@ -775,6 +775,7 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     sample(psi, ens);
     t.bits_valid = false;   // new samples: their transposed spin bits are built on first use
     t.kf_ready = false;
+    t.kf_inverted = false;
     Samples& s = ens.samples;
     const int64_t n = s.count;
     const int P = psi.num_params;
@@ -832,7 +833,10 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     const bool pre_kfac = t.kfac_hint && psi.N <= kKfacMaxSites && psi.M <= kKfacMaxUnits &&
                           (psi.layout == LAYOUT_RBM || t.inv_unique);
     KfacWork kw{};
-    if (pre_kfac) kw = kfac_work(t);
+    if (pre_kfac) {
+        kw = kfac_work(t);
+        t.kf_idx.ensure(size_t(t.N) * t.M);
+    }
     if (sizeof(T) == 4 || pre_kfac) {
         // the transposed spin bits (fp32 statistics and S.v, the K-FAC spin covariance)
         // are shared by both streams' kernels: built before the fork
@@ -854,7 +858,13 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
             launch_kfac_moments(s.confs.p, s.words, t.spin_bits.p, plan.vpad, s.w_uniform,
                                 t.precision != 0 ? t.g32.p : nullptr, t.precision != 0 ? nullptr : t.g64.p, t.Mpad,
                                 s.w.p, n, t.N, t.M, kw, S());
-            t.kf_ready = true;
+            // and the inverses for the shift of that solve (used when the next solve
+            // asks for the same shift)
+            launch_kfac_index(t.layout, t.N, t.M, t.Mpad, t.layout == LAYOUT_DEEP1 ? t.inv.p : nullptr, t.kf_idx.p,
+                              S());
+            launch_kfac_finish(t.N, t.M, t.kfac_shift, kw, S());
+            t.kf_inverted = true;
+            t.kf_inv_shift = t.kfac_shift;
         }
         ANN_HIP_CHECK(hipEventRecord(t.join_ev.get(), S()));
     }
@@ -1285,6 +1295,7 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     // net (the biases keep Jacobi), factors from this evaluation's samples
     const bool kfac = method == 3;
     t.kfac_hint = kfac;
+    if (kfac) t.kfac_shift = shift;
     // K-FAC with the fp32 S.v of a circulant PsiDeep with C = N keeps the solve's
     // vectors in site-major order (the S.v and K-FAC kernels then read and write whole
     // rows of 64 units); F, x0 and <O> are converted on entry and x on exit
@@ -1328,25 +1339,30 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         KfacWork kw = kfac_work(t);
         {
             ScopedTimer timer("kfac_setup");
-            if (!t.kf_ready) {
-                // factor moments of this evaluation's samples (else computed during the
-                // evaluation, when the previous solve of this Tdvp also used K-FAC); the
-                // transposed spin bits (shared with the fp32 S.v) give the spin covariance
-                const SpinPlan plan = spin_plan(t.N, t.M, t.Mpad, t.count, t.layout == LAYOUT_DEEP1);
-                if (!t.bits_valid) {
-                    t.spin_bits.ensure(size_t(plan.bit_blocks) * plan.vpad);
-                    launch_spin_bits(t.confs.p, t.words, t.count, t.N, plan.vpad, t.spin_bits.p, S());
-                    t.bits_valid = true;
+            // The factors of this evaluation's samples: computed and inverted during
+            // the evaluation when the previous solve of this Tdvp also used K-FAC (and
+            // reused when its shift is asked for again), else here
+            if (!(t.kf_inverted && t.kf_inv_shift == shift)) {
+                if (!t.kf_ready) {
+                    // the transposed spin bits (shared with the fp32 S.v) give the spin covariance
+                    const SpinPlan plan = spin_plan(t.N, t.M, t.Mpad, t.count, t.layout == LAYOUT_DEEP1);
+                    if (!t.bits_valid) {
+                        t.spin_bits.ensure(size_t(plan.bit_blocks) * plan.vpad);
+                        launch_spin_bits(t.confs.p, t.words, t.count, t.N, plan.vpad, t.spin_bits.p, S());
+                        t.bits_valid = true;
+                    }
+                    launch_kfac_moments(t.confs.p, t.words, t.spin_bits.p, plan.vpad, t.w_uniform,
+                                        t.precision != 0 ? t.g32.p : nullptr, t.precision != 0 ? nullptr : t.g64.p,
+                                        t.Mpad, t.w.p, t.count, t.N, t.M, kw, S());
                 }
-                launch_kfac_moments(t.confs.p, t.words, t.spin_bits.p, plan.vpad, t.w_uniform,
-                                    t.precision != 0 ? t.g32.p : nullptr, t.precision != 0 ? nullptr : t.g64.p,
-                                    t.Mpad, t.w.p, t.count, t.N, t.M, kw, S());
+                // the moments are used once: the inversion overwrites D
+                t.kf_ready = false;
+                launch_kfac_index(t.layout, t.N, t.M, t.Mpad, t.layout == LAYOUT_DEEP1 ? t.inv.p : nullptr,
+                                  t.kf_idx.p, S());
+                launch_kfac_finish(t.N, t.M, shift, kw, S());
+                t.kf_inverted = true;
+                t.kf_inv_shift = shift;
             }
-            // the moments are used once: the inversion overwrites D
-            t.kf_ready = false;
-            launch_kfac_index(t.layout, t.N, t.M, t.Mpad, t.layout == LAYOUT_DEEP1 ? t.inv.p : nullptr, t.kf_idx.p,
-                              S());
-            launch_kfac_finish(t.N, t.M, shift, kw, S());
         }
         jac_end = t.layout == LAYOUT_DEEP1 ? int64_t(t.N) + t.M : 0;
         kv = {t.N, t.M, t.kf_idx.p, t.order(), t.kf_Ci.p, t.kf_D.p};

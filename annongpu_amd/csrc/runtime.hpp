@@ -370,6 +370,9 @@ struct Tdvp {
     DevBuf<double> kf_spart, kf_C, kf_Ci, kf_gam;
     bool kfac_hint = false;      // the last solve used K-FAC: the next evaluation computes the factor moments
     bool kf_ready = false;       // kf_C / kf_D hold the moments of the current samples
+    double kfac_shift = 0.0;     // shift of the last K-FAC solve
+    bool kf_inverted = false;    // kf_Ci / kf_D hold the inverses for the current samples and kf_inv_shift
+    double kf_inv_shift = 0.0;
     OwnedEvent fork_ev, join_ev; // the evaluation's side-stream fork and join
     // E, E2, wsum (sums_host) and O, F (from Obar, Fd) of the last evaluation are read
     // to the host on first use

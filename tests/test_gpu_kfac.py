@@ -137,9 +137,10 @@ def test_kfac_rejects_generic_nets():
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_kfac_moments_precomputed_during_eval(precision):
-    """After a K-FAC solve the next evaluation of the same TDVP computes the factor
-    moments on a second stream while the local energies run; its statistics and solve
-    equal bit for bit those of a TDVP that computes the moments in the solve."""
+    """After a K-FAC solve the next evaluation of the same TDVP computes the factors and
+    their inverses for that solve's shift on a second stream while the local energies
+    run; its statistics and solves equal bit for bit those of a TDVP that computes the
+    factors in the solve, for the same shift and for another one."""
     N, M = 16, 32
     psi = A.seeded_rbm_deep(N, M, 3, sigma=0.05)
     psi.precision = precision
@@ -156,7 +157,10 @@ def test_kfac_moments_precomputed_during_eval(precision):
     x2 = t2.solve(1e-3, "cg_kfac", rtol=1e-6, max_iter=500)
     assert t1.last_solve["iterations"] == t2.last_solve["iterations"]
     assert np.array_equal(x1, x2)
-    # a second solve of the same evaluation recomputes the moments (the first one's
-    # inversion overwrote them)
+    # a second solve with the same shift reuses the inverses; another shift recomputes
+    # the factors from the samples
     x3 = t1.solve(1e-3, "cg_kfac", rtol=1e-6, max_iter=500)
     assert np.array_equal(x1, x3)
+    x4 = t1.solve(3e-3, "cg_kfac", rtol=1e-6, max_iter=500)
+    x5 = t2.solve(3e-3, "cg_kfac", rtol=1e-6, max_iter=500)
+    assert np.array_equal(x4, x5)
