@@ -658,7 +658,9 @@ void sample(Psi& psi, Ensemble& ens) {
         ANN_HIP_CHECK(hipMemcpyAsync(s.confs.p + step * B * words, ens.chains.p, sizeof(uint64_t) * B * words,
                                      hipMemcpyDeviceToDevice, S()));
         compute_theta(psi, ens.chains.p, B, s, step * B);
-        // restart the next sweep from exact angles
+        // restart the next sweep from exact angles (after the last step the next call
+        // recomputes the chains' angles anyway)
+        if (step + 1 == steps) break;
         if (psi.precision == 0)
             ANN_HIP_CHECK(hipMemcpyAsync(ens.chain_th64.p, s.th64.p + step * B * psi.Mpad,
                                          sizeof(cplx<double>) * B * psi.Mpad, hipMemcpyDeviceToDevice, S()));
