@@ -6,8 +6,10 @@
  * Spin configurations are little-endian uint64 words, bit i set <=> s_i = +1
  * (reference: include/basis/Spins.h:104-118).  Every function returns 0 on
  * success and a non-zero ANN_E_* code on failure; ann_last_error() then holds a
- * thread-local message.  All calls are synchronous: results are on the host
- * when the call returns.  Objects are not thread-safe; use one per thread.
+ * thread-local message.  Results are on the host when the call that returns
+ * them returns; ann_tdvp_eval and the sampling calls may return before their
+ * device work ends (a later call that reads a result waits for it, and reports
+ * a device error of that work).  Objects are not thread-safe; use one per thread.
  *
  * Each entry point names the reference interface it replaces.  The reference
  * binds those through pybind11 in pyANNonGPU/main.cpp.template (cited as MAIN).
