@@ -1473,12 +1473,16 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
     t.rr_host.ensure(4);
     launch_pcg_final(part.p, it, done.p, fnorm2.p, t.cg_scratch.p, S());
     ANN_HIP_CHECK(hipMemcpyAsync(t.rr_host.p, t.cg_scratch.p, sizeof(double) * 4, hipMemcpyDeviceToHost, S()));
+    // x through page-locked memory (one fast copy, then a host copy)
+    const cplx<double>* xr = x.p;
     if (sm) {
         launch_site_major(x.p, t.N, t.M, t.conn_shift, false, z.p, S());
-        z.download(reinterpret_cast<cplx<double>*>(x_out), P);
-    } else {
-        x.download(reinterpret_cast<cplx<double>*>(x_out), P);
+        xr = z.p;
     }
+    t.x_host.ensure(size_t(P));
+    ANN_HIP_CHECK(hipMemcpyAsync(t.x_host.p, xr, sizeof(cplx<double>) * P, hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    std::memcpy(x_out, t.x_host.p, sizeof(cplx<double>) * P);
     const double rr = t.rr_host.p[0];
     const double bnorm = std::sqrt(t.rr_host.p[2]);
     const double tol2 = (rtol * bnorm) * (rtol * bnorm);

@@ -362,6 +362,7 @@ struct Tdvp {
     bool bits_valid = false;
     int64_t matvecs = 0;         // S.v products so far (every 16th is timed when profiling)
     PinnedBuf<double> rr_host;   // CG residual partials and ||F||^2 (page-locked)
+    PinnedBuf<cplx<double>> x_host;   // the solution on its way to the caller
     PinnedBuf<int> flag_host;    // two slots of the CG convergence flag, read behind events
     int* flag_dev = nullptr;     // device address of flag_host (written by the check kernel)
     bool inv_unique = true;      // PsiDeep: every (site, unit) carries at most one connection
