@@ -423,12 +423,6 @@ __device__ inline double block_sum256(double v, double* sh) {
     return sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-// sum over blocks of partial[b * stride + idx], b < nparts <= 256 (all threads get it)
-__device__ inline double parts_total(const double* partial, int nparts, int stride, int idx, double* sh) {
-    const double x = int(threadIdx.x) < nparts ? partial[int64_t(threadIdx.x) * stride + idx] : 0.0;
-    return block_sum256(x, sh);
-}
-
 constexpr int kDotParts = 64;   // blocks of the fused dot products
 
 // beta = r^H z (this iteration) / r^H z (previous) from the CG partials (slots 0 + 1
