@@ -58,18 +58,32 @@ __global__ __launch_bounds__(kBlock) void theta_valu_kernel(DenseView<T> net, co
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(64) void theta_mfma_kernel(DenseView<float> net, const short4_t* wfrag,
-                                                         int KT, int NT, int nsplit,
-                                                         const uint64_t* confs, int words,
-                                                         int64_t count, float* theta) {
-    const int lane = threadIdx.x;
-    const int64_t row0 = int64_t(blockIdx.x) * 16;
+// Workgroup (column group of 4 tiles = 64 interleaved columns, 256 samples): the
+// group's B fragments (nsplit x KT x 4 tiles, 6 KB per k tile) are staged in LDS once
+// and shared by the 16 waves, each of which takes one tile of 16 samples (the
+// fragments were read from L2 by every 16-sample wave before: 43 KB per wave at
+// N = 100).  Per accumulator the matrix-core steps run in the order k tile, then split.
+constexpr int kThetaWaves = 16;
+__global__ __launch_bounds__(64 * kThetaWaves) void theta_mfma_kernel(DenseView<float> net, const short4_t* wfrag,
+                                                                       int KT, int NT, int nsplit,
+                                                                       const uint64_t* confs, int words,
+                                                                       int64_t count, float* theta) {
+    extern __shared__ short4_t sb[];   // [nsplit * KT][4][64]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nt0 = blockIdx.y * 4;
+    const int nb = nsplit * KT * 4 * 64;
+    for (int e = threadIdx.x; e < nb; e += 64 * kThetaWaves)
+        sb[e] = wfrag[(int64_t(e >> 8) * NT + nt0 + ((e >> 6) & 3)) * 64 + (e & 63)];
+    __syncthreads();
+    const int64_t row0 = (int64_t(blockIdx.x) * kThetaWaves + wave) * 16;
+    if (row0 >= count) return;   // no barrier follows
     const int64_t my_row = row0 + (lane & 15);
-    Conf c = {{0, 0, 0, 0}};
-    if (my_row < count) {
-        for (int w = 0; w < words; w++) c.w[w] = confs[my_row * words + w];
-    }
+    // the configuration words as scalars (as an array they end up in scratch memory)
+    const bool rok = my_row < count;
+    const uint64_t w0 = rok ? confs[my_row * words] : 0;
+    const uint64_t w1 = rok && words > 1 ? confs[my_row * words + 1] : 0;
+    const uint64_t w2 = rok && words > 2 ? confs[my_row * words + 2] : 0;
+    const uint64_t w3 = rok && words > 3 ? confs[my_row * words + 3] : 0;
     float4_t acc[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) acc[t] = {0.f, 0.f, 0.f, 0.f};
@@ -79,13 +93,15 @@ __global__ __launch_bounds__(64) void theta_mfma_kernel(DenseView<float> net, co
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int v = kt * 16 + kq + e;
+            const int k = v >> 6;
+            const uint64_t x = k == 0 ? w0 : k == 1 ? w1 : k == 2 ? w2 : w3;
             const short one = v < net.N ? short(0x3F80) : short(0);
-            a[e] = (v < net.N && spin_bit(c, v)) ? one : short(v < net.N ? 0xBF80 : 0);
+            a[e] = (v < net.N && ((x >> (v & 63)) & 1u)) ? one : short(v < net.N ? 0xBF80 : 0);
         }
         for (int sp = 0; sp < nsplit; sp++) {
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const short4_t b = wfrag[((int64_t(sp) * KT + kt) * NT + nt0 + t) * 64 + lane];
+                const short4_t b = sb[((sp * KT + kt) * 4 + t) * 64 + lane];
                 acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
             }
         }
@@ -699,10 +715,17 @@ void launch_theta_mfma(const DenseView<float>& net, const uint16_t* wfrag, int n
                        int words, int64_t count, cplx<float>* theta, hipStream_t stream) {
     if (count <= 0) return;
     const int KT = npad / 16, NT = 2 * net.Mpad / 16;
-    dim3 grid(unsigned((count + 15) / 16), unsigned(NT / 4));
-    // nsplit is recovered from the fragment buffer size by the caller convention: 3 terms
-    theta_mfma_kernel<<<grid, 64, 0, stream>>>(net, reinterpret_cast<const short4_t*>(wfrag), KT, NT, 3,
-                                               confs, words, count, reinterpret_cast<float*>(theta));
+    if (NT % 4 != 0) throw std::invalid_argument("theta_mfma: column tiles in groups of 4");
+    const int64_t per_block = int64_t(kThetaWaves) * 16;
+    dim3 grid(unsigned((count + per_block - 1) / per_block), unsigned(NT / 4));
+    // the fragments are 3 bf16 terms (pack_theta_fragments with nsplit 3); LDS for them
+    // beyond the default 64 KiB (N > 160) is opted in, up to 96 KiB at 256 sites
+    const size_t lds = size_t(3) * KT * 4 * 64 * sizeof(short4_t);
+    if (lds > 64 * 1024)
+        ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(theta_mfma_kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    theta_mfma_kernel<<<grid, 64 * kThetaWaves, lds, stream>>>(net, reinterpret_cast<const short4_t*>(wfrag), KT, NT, 3, confs,
+                                                  words, count, reinterpret_cast<float*>(theta));
     ANN_LAUNCH_CHECK();
 }
 
