@@ -196,7 +196,7 @@ class _Psi:
 
 
 class PsiRBM(_Psi):
-    """PsiRBM(W, final_weight, log_prefactor, gpu) -- include/quantum_state/PsiRBM.hpp:377-395.
+    """PsiRBM(W, final_weight, log_prefactor, gpu) -- include/quantum_state/PsiRBM.hpp:225-243.
 
     log psi(s) = log_prefactor + final_weight * sum_j lc(theta_j), theta_j = sum_i W_ij s_i.
     """
@@ -546,6 +546,16 @@ class MonteCarloSpins(_Ensemble):
         return a.value / tot if tot else float("nan")
 
 
+    @property
+    def block_stats(self):
+        """block moves of the last sampling call with a learned proposal: proposals made,
+        redraws of the current pattern (null moves; acceptance_rate counts them as not
+        accepted, since they do not move the chain) and flipped sites whose rows of W
+        entered a psi ratio"""
+        p, n, r = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib.ann_ensemble_block_stats(self._h, ref(p), ref(n), ref(r)))
+        return dict(proposals=p.value, null_moves=n.value, rows_evaluated=r.value)
+
     def set_proposal(self, proposal, plain_sweeps=1):
         """Attach a trained BlockProposal (None detaches).  Each sample interval then
         holds num_sweeps block sweeps plus ``plain_sweeps`` plain sweeps."""
@@ -656,6 +666,34 @@ class ExactSummationSpins(_Ensemble):
     def __init__(self, num_sites, gpu=True):
         self._h = handle()
         check(lib.ann_exact_summation_create(ref(self._h), int(num_sites)))
+        self.num_sites = int(num_sites)
+
+
+class SampleList(_Ensemble):
+    """Additive (no counterpart in the reference): an ensemble of given configurations.
+
+    ``configurations`` are integers, ``Spins`` or rows of uint64 words (bit i set
+    <=> s_i = +1); ``weights`` default to 1 / count.  Every network function treats
+    them as one sampling call's samples, so two wave functions, two precisions or a
+    run and a re-run see exactly the same samples.
+    """
+
+    def __init__(self, num_sites, configurations, weights=None, gpu=True):
+        words = _words_for(num_sites)
+        confs = np.asarray(configurations)
+        if confs.dtype == np.uint64 and confs.ndim == 2:
+            arr = np.zeros((confs.shape[0], words), dtype=np.uint64)
+            k = min(words, confs.shape[1])
+            arr[:, :k] = confs[:, :k]
+        else:
+            arr = np.stack([_conf_words(c)[:words] for c in configurations]) if len(configurations) else \
+                np.zeros((0, words), dtype=np.uint64)
+        arr = np.ascontiguousarray(arr, dtype=np.uint64)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64).ravel()
+        if w is not None and w.size != arr.shape[0]:
+            raise ValueError("one weight per configuration")
+        self._h = handle()
+        check(lib.ann_sample_list_create(ref(self._h), int(num_sites), ptr(arr), arr.shape[0], words, ptr(w)))
         self.num_sites = int(num_sites)
 
 
@@ -833,6 +871,46 @@ class TDVP:
     @property
     def var_H(self):
         return self._energy()[2]
+
+    # threshold gates the reference's evaluation against a reference wave function
+    # (eval_with_psi_ref, PsiClassical); the evaluations here do not read it
+    threshold = -1e6
+
+    def _samples(self):
+        cnt, words, tw = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_double()
+        check(lib.ann_tdvp_get_samples(self._h, None, None, None, 0, ref(cnt), ref(words), ref(tw)))
+        return cnt.value, words.value, tw.value
+
+    @property
+    def total_weight(self):
+        """sum of the sample weights of the last evaluation (MAIN:471)"""
+        return self._samples()[2]
+
+    @property
+    def E_local_samples(self):
+        """local energy of every sample of the last evaluation (MAIN:473)"""
+        n, _, _ = self._samples()
+        out = np.zeros(n, dtype=np.complex128)
+        check(lib.ann_tdvp_get_samples(self._h, None, None, ptr(out), n, None, None, None))
+        return out
+
+    @property
+    def samples(self):
+        """(configurations [count][words] uint64, weights [count]) of the last evaluation"""
+        n, words, _ = self._samples()
+        confs = np.zeros((n, words), dtype=np.uint64)
+        w = np.zeros(n)
+        check(lib.ann_tdvp_get_samples(self._h, ptr(confs), ptr(w), None, n, None, None, None))
+        return confs, w
+
+    @property
+    def O_k_samples(self):
+        """uncentred O_k of every sample of the last evaluation, [count][num_params] (MAIN:472;
+        the reference returns it flattened)"""
+        n, _, _ = self._samples()
+        out = np.zeros((n, self.num_params), dtype=np.complex128)
+        check(lib.ann_tdvp_get_O_samples(self._h, ptr(out), out.size))
+        return out
 
     def S_dot_vector(self, vec, ensemble=None):
         v = as_c128(vec).ravel()

@@ -407,6 +407,41 @@ int ann_exact_summation_create(ann_ensemble* out, uint32_t num_sites) {
     });
 }
 
+int ann_sample_list_create(ann_ensemble* out, uint32_t num_sites, const uint64_t* confs, int64_t count,
+                           int32_t words, const double* weights) {
+    return guard([&] {
+        need(out, "out");
+        need(confs, "confs");
+        if (num_sites == 0 || num_sites > 64 * ann::kMaxWords) throw std::invalid_argument("num_sites must be 1..256");
+        if (count <= 0 || count > int64_t(UINT32_MAX)) throw std::invalid_argument("count must be positive");
+        if (words < int32_t((num_sites + 63) / 64) || words > ann::kMaxWords)
+            throw std::invalid_argument("words does not fit num_sites");
+        std::vector<uint64_t> list(size_t(count) * ann::kMaxWords, 0);
+        for (int64_t i = 0; i < count; i++)
+            for (int w = 0; w < words; w++) {
+                const uint64_t x = confs[size_t(i) * words + w];
+                const int lo = 64 * w;
+                const uint64_t allowed = int(num_sites) >= lo + 64 ? ~uint64_t(0)
+                                         : int(num_sites) <= lo ? 0 : (uint64_t(1) << (num_sites - lo)) - 1;
+                if (x & ~allowed) throw std::invalid_argument("a configuration has bits beyond num_sites");
+                list[size_t(i) * ann::kMaxWords + w] = x;
+            }
+        std::vector<double> wt(size_t(count), 1.0 / double(count));
+        if (weights)
+            for (int64_t i = 0; i < count; i++) {
+                if (!std::isfinite(weights[i]) || weights[i] < 0.0) throw std::invalid_argument("weights must be >= 0");
+                wt[size_t(i)] = weights[i];
+            }
+        auto p = new ann_ensemble_s;
+        p->impl.kind = ann::ENS_LIST;
+        p->impl.num_sites = num_sites;
+        p->impl.num_samples = uint32_t(count);
+        p->impl.list_confs = std::move(list);
+        p->impl.list_w = std::move(wt);
+        *out = p;
+    });
+}
+
 int ann_ensemble_destroy(ann_ensemble ens) {
     return guard([&] { delete ens; });
 }
@@ -424,6 +459,16 @@ int ann_ensemble_acceptance(ann_ensemble ens, uint64_t* accepted, uint64_t* reje
         ens->impl.settle_acceptance();
         if (accepted) *accepted = ens->impl.accepted;
         if (rejected) *rejected = ens->impl.rejected;
+    });
+}
+
+int ann_ensemble_block_stats(ann_ensemble ens, uint64_t* proposals, uint64_t* null_moves, uint64_t* rows_evaluated) {
+    return guard([&] {
+        need(ens, "ens");
+        ens->impl.settle_acceptance();
+        if (proposals) *proposals = ens->impl.block_proposals;
+        if (null_moves) *null_moves = ens->impl.null_moves;
+        if (rows_evaluated) *rows_evaluated = ens->impl.rows_evaluated;
     });
 }
 
@@ -689,6 +734,35 @@ int ann_tdvp_get_energy(ann_tdvp tdvp, double* E2c, double* E2_local, double* va
     });
 }
 
+int ann_tdvp_get_samples(ann_tdvp tdvp, uint64_t* confs, double* weights, double* e_local, int64_t capacity,
+                         int64_t* count, int32_t* words, double* total_weight) {
+    return guard([&] {
+        need(tdvp, "tdvp");
+        auto& t = tdvp->impl;
+        if (!t.has_samples) throw std::logic_error("TDVP has not been evaluated");
+        t.settle();
+        const int64_t n = t.count;
+        if (count) *count = n;
+        if (words) *words = t.words;
+        if (total_weight) *total_weight = t.wsum;
+        if ((confs || weights || e_local) && capacity < n) throw std::invalid_argument("buffer too small");
+        if (confs) t.confs.download(confs, size_t(n) * t.words);
+        if (weights) t.w.download(weights, size_t(n));
+        if (e_local) t.e_loc.download(reinterpret_cast<ann::cplx<double>*>(e_local), size_t(n));
+    });
+}
+
+int ann_tdvp_get_O_samples(ann_tdvp tdvp, double* out, int64_t capacity) {
+    return guard([&] {
+        need(tdvp, "tdvp");
+        need(out, "out");
+        auto& t = tdvp->impl;
+        if (!t.has_samples) throw std::logic_error("TDVP has not been evaluated");
+        if (capacity < t.count * int64_t(t.P)) throw std::invalid_argument("buffer too small");
+        ann::tdvp_O_samples(t, C(out));
+    });
+}
+
 int ann_tdvp_S_dot_vector(ann_tdvp tdvp, const double* v, double* out) {
     return guard([&] {
         need(tdvp, "tdvp");
@@ -706,11 +780,18 @@ int ann_tdvp_solve(ann_tdvp tdvp, double shift, int32_t method, double rtol, int
                    int32_t* iterations, double* residual) {
     int rc = ANN_OK;
     const int g = guard([&] {
+        // arguments are checked before the handle, so that they fail the same way with or without a GPU
+        if (method < ANN_SOLVE_CHOLESKY || method > ANN_SOLVE_CG_KFAC_WARM) throw std::invalid_argument("unknown method");
+        if (!std::isfinite(shift)) throw std::invalid_argument("shift must be finite");
+        // the K-FAC damping is 0.3 sqrt(shift) times the factor scale: a shift <= 0 leaves the
+        // spin covariance undamped, and under S_z-conserving moves it has the exact null vector
+        // (1, ..., 1), so its inverse does not exist
+        if ((method == ANN_SOLVE_CG_KFAC || method == ANN_SOLVE_CG_KFAC_WARM) && !(shift > 0.0))
+            throw std::invalid_argument("the K-FAC preconditioned solve needs shift > 0");
         need(tdvp, "tdvp");
         need(x, "x");
         int it = 0;
         double res = 0.0;
-        if (method < ANN_SOLVE_CHOLESKY || method > ANN_SOLVE_CG_KFAC_WARM) throw std::invalid_argument("unknown method");
         std::vector<cd> x0;
         const bool warm = method == ANN_SOLVE_CG_WARM || method == ANN_SOLVE_CG_KFAC_WARM;
         if (warm) x0.assign(C(x), C(x) + tdvp->impl.P);

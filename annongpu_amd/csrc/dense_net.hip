@@ -465,12 +465,12 @@ __global__ __launch_bounds__(64) void spin_gemm_kernel(DenseView<T> net, const u
 #pragma unroll
     for (int t = 0; t < kRowTilesPerWave; t++) acc[t] = {0.0, 0.0, 0.0, 0.0};
     constexpr int kStage = 4;   // MFMA steps whose operands are loaded together
-    for (int64_t sb = s0; sb < s1; sb += 4 * kStage) {
+    for (int64_t sq = s0; sq < s1; sq += 4 * kStage) {
         double x[kStage];
         uint64_t cw[kStage][kMaxWords];
 #pragma unroll
         for (int st = 0; st < kStage; st++) {
-            const int64_t s = sb + 4 * st + kk;
+            const int64_t s = sq + 4 * st + kk;
             x[st] = 0.0;
 #pragma unroll
             for (int w = 0; w < kMaxWords; w++) cw[st][w] = 0;

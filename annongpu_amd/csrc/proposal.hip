@@ -65,7 +65,7 @@ template <typename T, int U, int WORDS>
 __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, BlockNet bn, uint64_t* chain_confs,
                                                              cplx<T>* theta, int num_chains, int64_t num_proposals,
                                                              uint64_t counter, uint32_t key0, uint32_t key1,
-                                                             uint32_t* accepted) {
+                                                             uint32_t* accepted, uint32_t* bstat) {
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * kWavesPerBlock + uniform_int(threadIdx.x >> 6);
     if (c >= num_chains) return;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, B
 #pragma unroll
         for (int q = 0; q < 4; q++) a1[q] += up ? col[64 * q] : -col[64 * q];
     }
-    uint32_t acc = 0;
+    uint32_t acc = 0, nul = 0, rows = 0;
     uint32_t rx = 0, ry = 0, rz = 0;
     for (int64_t q = 0; q < num_proposals; q++) {
         const int ql = int(q & 63);
@@ -192,10 +192,11 @@ __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, B
         if (pick < 0) pick = int(p_old);
         const uint32_t p_new = uint32_t(__builtin_amdgcn_readlane(pick, L)) & ((1u << bs) - 1u);
         const uint32_t flips = p_old ^ p_new;
-        if (flips == 0) {   // the current pattern was redrawn: accepted, nothing changes
-            acc++;
+        if (flips == 0) {   // the current pattern was redrawn: nothing to evaluate, nothing changes
+            nul++;
             continue;
         }
+        rows += uint32_t(__popc(flips));
         float e_old = 0.f, e_new = 0.f;
 #pragma unroll
         for (int j = 0; j < kMaxBlock; j++) {
@@ -246,6 +247,8 @@ __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, B
 #pragma unroll
         for (int k = 0; k < WORDS; k++) chain_confs[int64_t(c) * WORDS + k] = w[k];
         accepted[c] += acc;
+        bstat[2 * c] += nul;
+        bstat[2 * c + 1] += rows;
     }
 }
 
@@ -460,7 +463,7 @@ void launch_block_net_pack(const float* params, int N, int H, int32_t* W1q, int3
 template <typename T>
 void launch_block_sweep(const DenseView<T>& net, const BlockNet& bn, uint64_t* chain_confs, cplx<T>* theta,
                         int num_chains, int words, int64_t num_proposals, uint64_t counter, uint64_t seed,
-                        uint32_t* accepted, hipStream_t stream) {
+                        uint32_t* accepted, uint32_t* bstat, hipStream_t stream) {
     if (num_chains <= 0 || num_proposals <= 0) return;
     if (bn.bs < 1 || bn.bs > kMaxBlock) throw std::invalid_argument("block size must be 1..8");
     if (bn.N != net.N) throw std::invalid_argument("proposal and wave function have different sizes");
@@ -469,10 +472,10 @@ void launch_block_sweep(const DenseView<T>& net, const BlockNet& bn, uint64_t* c
         constexpr int U = std::decay_t<decltype(tag)>::value;
         const unsigned grid = grid_waves(num_chains);
         switch (words) {
-            case 1: block_sweep_kernel<T, U, 1><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted); break;
-            case 2: block_sweep_kernel<T, U, 2><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted); break;
-            case 3: block_sweep_kernel<T, U, 3><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted); break;
-            case 4: block_sweep_kernel<T, U, 4><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted); break;
+            case 1: block_sweep_kernel<T, U, 1><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
+            case 2: block_sweep_kernel<T, U, 2><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
+            case 3: block_sweep_kernel<T, U, 3><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
+            case 4: block_sweep_kernel<T, U, 4><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
             default: throw HipError("configurations of 1..4 words are supported");
         }
     });
@@ -480,9 +483,9 @@ void launch_block_sweep(const DenseView<T>& net, const BlockNet& bn, uint64_t* c
 }
 
 template void launch_block_sweep<float>(const DenseView<float>&, const BlockNet&, uint64_t*, cplx<float>*, int, int,
-                                        int64_t, uint64_t, uint64_t, uint32_t*, hipStream_t);
+                                        int64_t, uint64_t, uint64_t, uint32_t*, uint32_t*, hipStream_t);
 template void launch_block_sweep<double>(const DenseView<double>&, const BlockNet&, uint64_t*, cplx<double>*, int,
-                                         int, int64_t, uint64_t, uint64_t, uint32_t*, hipStream_t);
+                                         int, int64_t, uint64_t, uint64_t, uint32_t*, uint32_t*, hipStream_t);
 
 void launch_draw_batch(const TrainBatch& b, int64_t count, const int* blocks, int num_blocks, int bs, uint64_t step,
                        uint64_t seed, hipStream_t stream) {

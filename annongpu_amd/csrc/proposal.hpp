@@ -32,11 +32,14 @@ struct BlockNet {
 void launch_block_net_pack(const float* params, int N, int H, int32_t* W1q, int32_t* b1q, uint16_t* W2h, float* b2,
                            hipStream_t stream);
 
-// num_proposals block moves per chain on Philox stream 3 (counter = counter + q)
+// num_proposals block moves per chain on Philox stream 3 (counter = counter + q);
+// accepted[c] counts the moves that changed the configuration, bstat[c][0] the
+// redraws of the current pattern (nothing to evaluate), bstat[c][1] the flipped
+// sites whose rows of W entered a psi ratio
 template <typename T>
 void launch_block_sweep(const DenseView<T>& net, const BlockNet& bn, uint64_t* chain_confs, cplx<T>* theta,
                         int num_chains, int words, int64_t num_proposals, uint64_t counter, uint64_t seed,
-                        uint32_t* accepted, hipStream_t stream);
+                        uint32_t* accepted, uint32_t* bstat, hipStream_t stream);
 
 // ------------------------------------------------------------------ training
 struct TrainBatch {

@@ -525,12 +525,21 @@ void Tdvp::fetch_host_vectors() {
 }
 
 void Ensemble::settle_acceptance() {
+    if (bstat_pending) {
+        Runtime::get().sync();
+        null_moves = rows_evaluated = 0;
+        for (size_t c = 0; c < size_t(num_chains); c++) {
+            null_moves += bstat_host.p[2 * c];
+            rows_evaluated += bstat_host.p[2 * c + 1];
+        }
+        bstat_pending = false;
+    }
     if (!acc_pending) return;
     Runtime::get().sync();
     uint64_t total_acc = 0;
     for (size_t c = 0; c < size_t(num_chains); c++) total_acc += acc_host.p[c];
     accepted = total_acc;
-    rejected = acc_total - total_acc;
+    rejected = acc_total - total_acc;   // redraws of the current block pattern did not move the chain
     acc_pending = false;
 }
 
@@ -553,10 +562,28 @@ void log_psi_of(Psi& psi, const uint64_t* confs, int64_t count, Samples& s, doub
         launch_net_log_psi<float>(psi.view32(), s.th32.p, count, lp, s.log_psi.p, w, S());
 }
 
+// caller-given configurations (ENS_LIST): uploaded as they are, with their weights
+static void sample_list(Psi& psi, Ensemble& ens) {
+    if (int(ens.num_sites) != psi.num_sites)
+        throw std::invalid_argument("the sample list was built for another number of sites");
+    Samples& s = ens.samples;
+    const int64_t count = ens.num_samples;
+    const int words = psi.words;
+    prepare_samples(psi, s, count);
+    std::vector<uint64_t> packed(size_t(count) * words);
+    for (int64_t i = 0; i < count; i++)
+        for (int w = 0; w < words; w++) packed[size_t(i) * words + w] = ens.list_confs[size_t(i) * kMaxWords + w];
+    s.confs.upload(packed.data(), packed.size());
+    s.w.upload(ens.list_w.data(), size_t(count));
+    s.w_uniform = std::all_of(ens.list_w.begin(), ens.list_w.end(), [&](double x) { return x == ens.list_w[0]; });
+    log_psi_of(psi, s.confs.p, count, s, nullptr);
+}
+
 // PsiCNN: Metropolis with a full forward pass per proposal (the reference keeps
 // no incremental state for PsiCNN, include/quantum_state/PsiCNN.hpp:177-181)
 static void sample_cnn(Psi& psi, Ensemble& ens) {
     psi.sync_device();
+    if (ens.kind == ENS_LIST) return sample_list(psi, ens);
     Samples& s = ens.samples;
     if (ens.kind == ENS_EXACT) {
         const int64_t count = exact_count(psi, ens);
@@ -597,6 +624,7 @@ void sample(Psi& psi, Ensemble& ens) {
     if (psi.kind == PSI_CNN) return sample_cnn(psi, ens);
     require_dense(psi);
     psi.sync_device();
+    if (ens.kind == ENS_LIST) return sample_list(psi, ens);
     Samples& s = ens.samples;
     const int words = psi.words;
     if (ens.kind == ENS_EXACT) {
@@ -632,6 +660,8 @@ void sample(Psi& psi, Ensemble& ens) {
         bn = prop->view();
         plain_per_step = int64_t(ens.plain_sweeps) * N;
         block_per_step = int64_t(ens.num_sweeps) * ((N + prop->bs - 1) / prop->bs);
+        ens.bstat.ensure(size_t(2) * B);
+        ANN_HIP_CHECK(hipMemsetAsync(ens.bstat.p, 0, sizeof(uint32_t) * 2 * B, S()));
     }
     for (int64_t step = 0; step < steps; step++) {
         const int64_t nprop = (step == 0 ? therm : 0) + plain_per_step;
@@ -649,10 +679,10 @@ void sample(Psi& psi, Ensemble& ens) {
             ScopedTimer timer("block_sweep");
             if (psi.precision == 0)
                 launch_block_sweep<double>(psi.view64(), bn, ens.chains.p, ens.chain_th64.p, B, words, block_per_step,
-                                           ens.counter, ens.seed, ens.acc.p, S());
+                                           ens.counter, ens.seed, ens.acc.p, ens.bstat.p, S());
             else
                 launch_block_sweep<float>(psi.view32(), bn, ens.chains.p, ens.chain_th32.p, B, words, block_per_step,
-                                          ens.counter, ens.seed, ens.acc.p, S());
+                                          ens.counter, ens.seed, ens.acc.p, ens.bstat.p, S());
             ens.counter += uint64_t(block_per_step);
         }
         ANN_HIP_CHECK(hipMemcpyAsync(s.confs.p + step * B * words, ens.chains.p, sizeof(uint64_t) * B * words,
@@ -678,6 +708,16 @@ void sample(Psi& psi, Ensemble& ens) {
     else
         launch_net_log_psi<float>(psi.view32(), s.th32.p, count, lp, s.log_psi.p, nullptr, S());
     finish_chains(psi, ens, therm, steps, plain_per_step + block_per_step);
+    ens.block_proposals = uint64_t(B) * uint64_t(steps * block_per_step);
+    if (block_per_step > 0) {
+        ens.bstat_host.ensure(size_t(2) * B);
+        ANN_HIP_CHECK(hipMemcpyAsync(ens.bstat_host.p, ens.bstat.p, sizeof(uint32_t) * 2 * B, hipMemcpyDeviceToHost,
+                                     S()));
+        ens.bstat_pending = true;
+    } else {
+        ens.bstat_pending = false;
+        ens.null_moves = ens.rows_evaluated = 0;
+    }
 }
 
 void local_energies(Psi& psi, Operator& op, Samples& s) {
@@ -889,6 +929,9 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     ANN_HIP_CHECK(hipMemcpyAsync(t.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice, S()));
     t.w.ensure(n);
     ANN_HIP_CHECK(hipMemcpyAsync(t.w.p, s.w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, S()));
+    t.e_loc.ensure(n);
+    ANN_HIP_CHECK(hipMemcpyAsync(t.e_loc.p, s.e_loc.p, sizeof(cplx<double>) * n, hipMemcpyDeviceToDevice, S()));
+    t.generic = false;
     t.w_uniform = s.w_uniform;
     t.has_samples = true;
     t.has_S = false;
@@ -984,6 +1027,10 @@ void tdvp_eval_generic(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with
     t.count = n;
     t.w.ensure(n);
     ANN_HIP_CHECK(hipMemcpyAsync(t.w.p, s.w.p, sizeof(double) * n, hipMemcpyDeviceToDevice, S()));
+    t.confs.ensure(size_t(n) * s.words);
+    ANN_HIP_CHECK(hipMemcpyAsync(t.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice, S()));
+    t.e_loc.ensure(n);
+    ANN_HIP_CHECK(hipMemcpyAsync(t.e_loc.p, s.e_loc.p, sizeof(cplx<double>) * n, hipMemcpyDeviceToDevice, S()));
     t.w_uniform = s.w_uniform;
     t.has_samples = true;
     t.has_S = false;
@@ -1190,6 +1237,44 @@ void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
         tdvp_eval_t<double>(t, op, psi, ens, with_S);
     else
         tdvp_eval_t<float>(t, op, psi, ens, with_S);
+}
+
+namespace {
+template <typename T>
+void o_samples_t(Tdvp& t, cd* out) {
+    const int64_t n = t.count;
+    const int P = int(t.P);
+    std::vector<cplx<T>> host;
+    if (t.generic) {
+        auto& O = materialised<T>(t);
+        host.resize(size_t(n) * P);
+        O.download(host.data(), host.size());
+        for (size_t i = 0; i < host.size(); i++) out[i] = cd(host[i].re, host[i].im);
+        return;
+    }
+    // the structured O (g and the spins) of the evaluation, materialised in slices of samples
+    const DenseView<T> net{t.N, t.M, t.Mpad, t.C, t.layout, P, 1, nullptr, nullptr, nullptr,
+                           t.layout == LAYOUT_DEEP1 ? t.conn.p : nullptr};
+    const cplx<T>* g = Prec<T>::g(t).p;
+    const int64_t slice = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t(1) << 27) / std::max(P, 1)));
+    DevBuf<cplx<T>> O;
+    O.ensure(size_t(slice) * P);
+    host.resize(size_t(slice) * P);
+    for (int64_t s0 = 0; s0 < n; s0 += slice) {
+        const int64_t m = std::min(slice, n - s0);
+        launch_materialize_ok<T>(net, t.confs.p + s0 * t.words, t.words, g + s0 * t.Mpad, m, nullptr, O.p, S());
+        O.download(host.data(), size_t(m) * P);
+        for (size_t i = 0; i < size_t(m) * P; i++) out[size_t(s0) * P + i] = cd(host[i].re, host[i].im);
+    }
+}
+}  // namespace
+
+void tdvp_O_samples(Tdvp& t, cd* out) {
+    if (!t.has_samples) throw std::logic_error("TDVP has not been evaluated");
+    if (t.precision == 0)
+        o_samples_t<double>(t, out);
+    else
+        o_samples_t<float>(t, out);
 }
 
 void tdvp_S_dot(Tdvp& t, const cplx<double>* v, double shift, cplx<double>* y, const double** ypart, int* yparts) {

@@ -299,7 +299,7 @@ struct Proposal {
     double last_loss = 0.0;
 };
 
-enum EnsembleKind : int { ENS_MC = 0, ENS_EXACT = 1 };
+enum EnsembleKind : int { ENS_MC = 0, ENS_EXACT = 1, ENS_LIST = 2 };
 
 struct Ensemble {
     int kind = ENS_MC;
@@ -324,7 +324,16 @@ struct Ensemble {
     // learned block proposal (not owned); block sweeps per sample and plain sweeps per sample
     Proposal* proposal = nullptr;
     uint32_t plain_sweeps = 1;
+    // block-move counters of the last call: per chain [null redraws, rows evaluated],
+    // summed on the host when asked for, and the block proposals made
+    DevBuf<uint32_t> bstat;
+    PinnedBuf<uint32_t> bstat_host;
+    uint64_t block_proposals = 0, null_moves = 0, rows_evaluated = 0;
+    bool bstat_pending = false;
     Samples samples;
+    // caller-given configurations [num_samples][kMaxWords] and weights (ENS_LIST)
+    std::vector<uint64_t> list_confs;
+    std::vector<double> list_w;
     uint64_t num_steps() const { return kind == ENS_EXACT ? (uint64_t(1) << num_sites) : num_samples; }
 };
 
@@ -342,6 +351,7 @@ struct Tdvp {
     DevBuf<uint32_t> conn;
     DevBuf<uint64_t> confs;
     DevBuf<double> w;
+    DevBuf<cplx<double>> e_loc;   // per-sample local energies of the last evaluation
     DevBuf<cplx<float>> g32;
     DevBuf<cplx<double>> g64;
     DevBuf<double> diag;   // diagonal of S (Jacobi preconditioner)
@@ -427,6 +437,8 @@ void tdvp_S_dot(Tdvp& t, const cplx<double>* v_dev, double shift, cplx<double>* 
 // nets), the CG methods from x0 (zero when x0 is null)
 int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd* x, int* iters, double* res,
                const cd* x0 = nullptr);
+// O_k of every sample of the last evaluation, uncentred, [count][P] in parameter order (host)
+void tdvp_O_samples(Tdvp& t, cd* out);
 
 // proposal training: Adam steps on random (sample, block) rows of the given
 // device configurations; returns the loss of the last step

@@ -59,7 +59,7 @@ int ann_profiler_query(const char* name, double* milliseconds, int64_t* launches
 int ann_profiler_reset(void);
 
 /* -------------------------------------------------------- wave functions */
-/* PsiRBM(W[N][M], final_weight, log_prefactor, gpu) -- PsiRBM.hpp:377-395, MAIN:120-154 */
+/* PsiRBM(W[N][M], final_weight, log_prefactor, gpu) -- PsiRBM.hpp:225-243, MAIN:120-154 */
 int ann_psi_create_rbm(ann_psi* out, int32_t N, int32_t M, const double* W, const double* final_weight,
                        const double* log_prefactor);
 /* PsiDeep(num_sites, a[N], b_list, connections_list, W_list, final_weights,
@@ -114,8 +114,19 @@ int ann_ensemble_destroy(ann_ensemble ens);
 int ann_ensemble_num_steps(ann_ensemble ens, uint64_t* num_steps);
 /* acceptance_rate (MAIN:364-366): raw counters of the last call */
 int ann_ensemble_acceptance(ann_ensemble ens, uint64_t* accepted, uint64_t* rejected);
+/* additive: block moves of the last sampling call with a learned proposal: proposals
+ * made, redraws of the current pattern (ann_ensemble_acceptance counts them with the
+ * rejected proposals: they do not move the chain) and flipped sites whose rows
+ * entered a psi ratio */
+int ann_ensemble_block_stats(ann_ensemble ens, uint64_t* proposals, uint64_t* null_moves, uint64_t* rows_evaluated);
 /* additive: configurations of the last sampling call, [num_steps][words] */
 int ann_ensemble_last_samples(ann_ensemble ens, uint64_t* confs, int64_t capacity, int64_t* count, int32_t* words);
+/* additive (no counterpart in the reference): an ensemble of caller-given configurations
+ * confs[count][words] (bit i of a configuration set <=> s_i = +1, words <= 4) with weights
+ * (null: 1 / count each), evaluated as they are; it lets two precisions or two builds
+ * see the same samples, the role the reference's ExactSummation plays at N <= 20 */
+int ann_sample_list_create(ann_ensemble* out, uint32_t num_sites, const uint64_t* confs, int64_t count,
+                           int32_t words, const double* weights);
 
 /* ------------------------------------------------------ network functions */
 /* log_psi_s(psi, spins) -- PsiVector.cu.template:101-133, MAIN:495-497 */
@@ -155,6 +166,14 @@ int ann_tdvp_get_S(ann_tdvp tdvp, double* S);
 int ann_tdvp_get_F(ann_tdvp tdvp, double* F);
 int ann_tdvp_get_O(ann_tdvp tdvp, double* O);
 int ann_tdvp_get_energy(ann_tdvp tdvp, double* E2_complex, double* E2_local, double* var_H);
+/* E_local_samples, total_weight (MAIN:470-474; TDVP.hpp:29-34) and the samples of the last
+ * evaluation: confs[count][words], weights[count], e_local[count] (interleaved complex);
+ * any output may be null, capacity is the number of samples the buffers hold */
+int ann_tdvp_get_samples(ann_tdvp tdvp, uint64_t* confs, double* weights, double* e_local, int64_t capacity,
+                         int64_t* count, int32_t* words, double* total_weight);
+/* O_k_samples (MAIN:472; TDVP.cu.template:69,121: uncentred O_k of every sample),
+ * out[count][P] interleaved complex; materialised on demand from the evaluation's state */
+int ann_tdvp_get_O_samples(ann_tdvp tdvp, double* out, int64_t capacity);
 /* S_dot_vector(vec, ensemble) -- TDVP.cu.template:336-443, MAIN:483-487 */
 int ann_tdvp_S_dot_vector(ann_tdvp tdvp, const double* v, double* out);
 /* additive (absent from the reference): solve (S + shift I) x = F.

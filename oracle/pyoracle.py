@@ -125,7 +125,7 @@ class OraclePsi:
 
 
 def rbm(W, final_weight, log_prefactor=0.0):
-    """PsiRBM(W, final_weight, log_prefactor) -- include/quantum_state/PsiRBM.hpp:377-395"""
+    """PsiRBM(W, final_weight, log_prefactor) -- include/quantum_state/PsiRBM.hpp:225-243"""
     W = np.ascontiguousarray(W, dtype=np.complex128)
     o = OraclePsi()
     N, M = W.shape

@@ -70,3 +70,22 @@ def test_activation_function_matches_oracle():
     for z in (0.1 + 0.2j, -0.4 + 0.05j):
         for layer in (0, 1):
             assert A.activation_function(z, layer) == pytest.approx(orc.activation(z, layer), rel=1e-14)
+
+
+def test_kfac_solve_rejects_non_positive_shift():
+    """The K-FAC damping scales with sqrt(shift); shift <= 0 would invert the singular
+    spin covariance of S_z-conserving samples, so the boundary refuses it up front."""
+    from annongpu_amd import _lib
+    x = np.zeros(4, dtype=np.complex128)
+    it, res = ctypes.c_int32(), ctypes.c_double()
+    for method in (_lib.SOLVE_CG_KFAC, _lib.SOLVE_CG_KFAC_WARM):
+        for shift in (0.0, -1e-3, float("nan")):
+            rc = _lib.lib.ann_tdvp_solve(None, shift, method, 1e-6, 10, x.ctypes.data,
+                                         ctypes.byref(it), ctypes.byref(res))
+            assert rc == _lib.ANN_E_INVALID
+            assert b"shift" in _lib.lib.ann_last_error()
+    rc = _lib.lib.ann_tdvp_solve(None, float("inf"), _lib.SOLVE_CG, 1e-6, 10, x.ctypes.data, None, None)
+    assert rc == _lib.ANN_E_INVALID and b"shift" in _lib.lib.ann_last_error()
+    # a valid shift with no handle fails on the handle, not on the shift
+    rc = _lib.lib.ann_tdvp_solve(None, 1e-3, _lib.SOLVE_CG_KFAC, 1e-6, 10, x.ctypes.data, None, None)
+    assert rc == _lib.ANN_E_INVALID and b"tdvp" in _lib.lib.ann_last_error()

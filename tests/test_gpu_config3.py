@@ -1,0 +1,160 @@
+"""The headline workload at its full per-sample size on the HIP path against the
+fp64 oracle, on the same configurations: 2D J1 Heisenberg 10x10 PBC (200 bonds,
+600 Pauli strings), one-layer PsiDeep with M = 400 hidden units (P = 40,500),
+exchange samples in the S_z = 0 sector.
+
+The samples come from the fp32 device sampler (512 chains, one sample each) and
+are handed to both precisions and to the oracle through a SampleList, so every
+difference below is arithmetic, not sampling.
+
+Tolerances (device against oracle):
+  fp32 mode: per-sample E_loc 1e-5 relative (to max(1, |E_loc|)); E 1e-6 relative;
+             <O>, F and O_k samples 1e-4 relative to the vector norm; S.v 2e-5
+             relative to the norm of O_c^H diag(w) O_c v built from the oracle's O_k.
+  fp64 mode: 1e-10 on all of these.
+The shifted solve (K-FAC-preconditioned CG, shift 1e-3) must reach the contract's
+relative residual 1e-6, checked through the device S.v, and agree with the fp64
+Woodbury solution of the oracle's (S + shift) x = F.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from helpers import cgauss, oracle_operator
+import annongpu_amd as A
+
+pytestmark = pytest.mark.gpu
+
+L, ALPHA = 10, 4
+N, M = L * L, ALPHA * L * L
+SHIFT = 1e-3
+CHAINS = 512
+TOL = {"fp64": dict(el=1e-10, E=1e-10, vec=1e-10, sv=1e-10),
+       "fp32": dict(el=1e-5, E=1e-6, vec=1e-4, sv=2e-5)}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+def woodbury_solve(Oc, w, shift, F):
+    """x = (Oc^H diag(w) Oc + shift I)^-1 F for a rank-count S, in fp64:
+    with U = diag(sqrt w) Oc, x = (F - U^H (shift I + U U^H)^-1 U F) / shift"""
+    U = np.sqrt(w)[:, None] * Oc
+    G = U @ U.conj().T
+    G[np.diag_indices_from(G)] += shift
+    return (F - U.conj().T @ np.linalg.solve(G, U @ F)) / shift
+
+
+@pytest.fixture(scope="module", params=[0.01, 0.05], ids=["sigma0.01", "sigma0.05"])
+def workload(request):
+    sigma = request.param
+    psi = A.seeded_rbm_deep(N, M, seed=3, sigma=sigma, final_weight=1.0)
+    H = A.hamiltonians.heisenberg_square(L)
+    op = A.Operator(H)
+    psi.precision = "fp32"
+    mc = A.MonteCarloSpins(CHAINS, 2, 20, CHAINS, move="exchange", seed=31)
+    tdvp_mc = A.TDVP(psi.num_params)
+    tdvp_mc.eval_F(op, psi, mc)
+    confs, w = tdvp_mc.samples
+    assert confs.shape == (CHAINS, 2)
+    assert np.all([bin(int(c[0])).count("1") + bin(int(c[1])).count("1") == N // 2 for c in confs])
+    o = orc.deep(N, psi.a, psi.b, psi.connections, psi.W, psi.final_weights, 0.0)
+    ref = orc.tdvp_accumulate(oracle_operator(H, 2), o, confs, w, with_S=False)
+    Oc = ref["O_samples"] - ref["O"][None, :]
+    return dict(sigma=sigma, psi=psi, op=op, confs=confs, w=w, ref=ref, Oc=Oc, mc_El=tdvp_mc.E_local_samples,
+                mc_E=tdvp_mc.E_local, mc_F=tdvp_mc.F_vector)
+
+
+def evaluate(wl, precision):
+    psi = wl["psi"].copy()
+    psi.precision = precision
+    ens = A.SampleList(N, wl["confs"])
+    tdvp = A.TDVP(psi.num_params)
+    tdvp.eval_F(wl["op"], psi, ens)
+    return tdvp
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_statistics_match_oracle(workload, precision):
+    wl, ref = workload, workload["ref"]
+    tol = TOL[precision]
+    tdvp = evaluate(wl, precision)
+    assert tdvp.total_weight == pytest.approx(1.0, rel=1e-14)
+    c, w = tdvp.samples
+    assert np.array_equal(c, wl["confs"]) and np.array_equal(w, wl["w"])
+    El = tdvp.E_local_samples
+    el_err = np.abs(El - ref["E_loc"]) / np.maximum(1.0, np.abs(ref["E_loc"]))
+    assert el_err.max() < tol["el"], el_err.max()
+    assert abs(tdvp.E_local - ref["E"]) < tol["E"] * abs(ref["E"])
+    assert rel(tdvp.O_k_vector, ref["O"]) < tol["vec"]
+    assert rel(tdvp.F_vector, ref["F"]) < tol["vec"]
+    assert rel(tdvp.O_k_samples, ref["O_samples"]) < tol["vec"]
+    var_ref = ref["E2"] - abs(ref["E"]) ** 2
+    assert tdvp.var_H == pytest.approx(var_ref, rel=100 * tol["E"] * abs(ref["E"]) ** 2 / var_ref)
+    if precision == "fp32":
+        # the sampler's own evaluation of the same samples is the same computation
+        assert np.array_equal(wl["mc_El"], El)
+        assert wl["mc_E"] == tdvp.E_local
+        assert np.array_equal(wl["mc_F"], tdvp.F_vector)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_S_dot_vector_matches_oracle(workload, precision):
+    wl = workload
+    tdvp = evaluate(wl, precision)
+    Oc, w = wl["Oc"], wl["w"]
+    rng = np.random.default_rng(5)
+    for _ in range(2):
+        v = cgauss(rng, tdvp.num_params, 1.0)
+        y_ref = Oc.conj().T @ (w * (Oc @ v))
+        assert rel(tdvp.S_dot_vector(v), y_ref) < TOL[precision]["sv"]
+
+
+@pytest.mark.parametrize("method", ["cg_kfac", "cg"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_shifted_solve_contract_tolerance(workload, precision, method):
+    """(S + 1e-3) x = F to relative residual 1e-6 (the contract's CG tolerance)"""
+    wl, ref = workload, workload["ref"]
+    tdvp = evaluate(wl, precision)
+    x = tdvp.solve(SHIFT, method, rtol=1e-6, max_iter=2000)
+    assert tdvp.last_solve["residual"] <= 1e-6
+    F = tdvp.F_vector
+    r = tdvp.S_dot_vector(x) + SHIFT * x - F
+    assert np.linalg.norm(r) / np.linalg.norm(F) < 2e-6
+    x_ref = woodbury_solve(wl["Oc"], wl["w"], SHIFT, ref["F"])
+    # the solve stops at residual 1e-6; the rank-512 S has eigenvalues down to the
+    # shift, so the solution error is bounded by the residual times (norm S / shift)
+    assert rel(x, x_ref) < (1e-4 if precision == "fp64" else 2e-3)
+
+
+def test_fp32_error_against_fp64_on_the_same_samples(workload):
+    """per-quantity error of the fp32 device mode against the fp64 device mode on the
+    same 512 samples (both against the oracle too); printed for the record"""
+    wl, ref = workload, workload["ref"]
+    t32, t64 = evaluate(wl, "fp32"), evaluate(wl, "fp64")
+    v = cgauss(np.random.default_rng(9), t32.num_params, 1.0)
+    x32 = t32.solve(SHIFT, "cg_kfac", rtol=1e-6, max_iter=2000)
+    x64 = t64.solve(SHIFT, "cg_kfac", rtol=1e-6, max_iter=2000)
+    El32, El64 = t32.E_local_samples, t64.E_local_samples
+    rows = {
+        "E_loc per sample (max rel)": float((np.abs(El32 - El64) / np.abs(El64)).max()),
+        "E (rel)": abs(t32.E_local - t64.E_local) / abs(t64.E_local),
+        "var_H (rel)": abs(t32.var_H - t64.var_H) / t64.var_H,
+        "<O> (rel norm)": rel(t32.O_k_vector, t64.O_k_vector),
+        "F (rel norm)": rel(t32.F_vector, t64.F_vector),
+        "S.v (rel norm)": rel(t32.S_dot_vector(v), t64.S_dot_vector(v)),
+        "x = (S+1e-3)^-1 F, K-FAC CG 1e-6 (rel norm)": rel(x32, x64),
+    }
+    oracle = {"E (rel)": abs(t64.E_local - ref["E"]) / abs(ref["E"]),
+              "F (rel norm)": rel(t64.F_vector, ref["F"])}
+    print(f"\nconfig-3 shapes, sigma {wl['sigma']}, {CHAINS} samples: fp32 vs fp64 device")
+    for k, val in rows.items():
+        print(f"  {k:45s} {val:.3e}")
+    for k, val in oracle.items():
+        print(f"  fp64 device vs oracle {k:23s} {val:.3e}")
+    assert rows["E (rel)"] < 1e-6
+    assert rows["F (rel norm)"] < 1e-4
+    assert all(math.isfinite(val) for val in rows.values())

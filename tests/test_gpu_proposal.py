@@ -118,6 +118,54 @@ def test_block_sampler_preserves_psi_squared(precision, scale):
     assert 0.0 < ens.acceptance_rate < 1.0
 
 
+@pytest.mark.parametrize("precision,scale", [("fp32", 1.0), ("fp32", 3.0), ("fp64", 3.0), ("fp32", "trained")])
+def test_plaquette_block_sampler_preserves_psi_squared(precision, scale):
+    """the benchmark's block shape: 2x4 and 4x2 plaquettes (8 sites, so every lane of the
+    pattern draw holds patterns and the shared high-bit energies span bits 2..7) with the
+    benchmark's 256 hidden units, on a 4x4 Heisenberg torus.  E and <s_0 s_1> within 4
+    standard errors of exact sums over the 12,870 states of the S_z = 0 sector, for the
+    network as initialised, scaled to be strongly non-uniform, and trained on plain
+    samples; redraws of the current pattern are counted apart from accepted moves"""
+    L, N = 4, 16
+    psi = A.seeded_rbm_deep(N, 32, 21, sigma=0.15)
+    psi.precision = precision
+    o = orc.deep(N, psi.a, psi.b, psi.connections, psi.W, psi.final_weights, 0.0)
+    H = ham.heisenberg_square(L)
+    E_ex, c_ex = _sector_exact(o, N, H)
+    blocks = A.rect_blocks(L, (2, 4))
+    assert blocks.shape == (32, 8) and all(len(set(b)) == 8 for b in blocks.tolist())
+    prop = A.BlockProposal(N, blocks, hidden=256, seed=5)
+    if scale == "trained":
+        data = A.MonteCarloSpins(1024 * 32, 2, 20, 1024, move="exchange", seed=2)
+        A.log_psi(psi, data)
+        prop.fit(data, steps=300, batch=2048, lr=3e-3)
+    else:
+        prop.params = scale * prop.params
+    chains, steps = 2048, 48
+    ens = A.MonteCarloSpins(chains * steps, 2, 10, chains, move="exchange", seed=8)
+    ens.set_proposal(prop, plain_sweeps=0)
+    el = A.local_energies(A.Operator(H), psi, ens).real
+    s = ens.last_samples()[:, 0].astype(np.int64)
+    assert np.all([bin(int(x)).count("1") == N // 2 for x in s[:256]])
+    E, se = _chain_stats(el, chains)
+    assert abs(E - E_ex) < 4 * se + 1e-9, (E, E_ex, se)
+    s0s1 = np.where((s & 1) == (s >> 1 & 1), 1.0, -1.0)
+    c, sec = _chain_stats(s0s1, chains)
+    assert abs(c - c_ex) < 4 * sec, (c, c_ex, sec)
+    st = ens.block_stats
+    assert st["proposals"] == chains * steps * 2 * 2     # 2 block sweeps of ceil(16 / 8) moves per sample
+    assert 0 <= st["null_moves"] < st["proposals"]
+    moved = ens.acceptance_rate * st["proposals"]
+    assert 0.0 < ens.acceptance_rate < 1.0
+    # every evaluated proposal flips at least 2 sites (S_z is conserved) and at most 8
+    evaluated = st["proposals"] - st["null_moves"]
+    assert 2 * evaluated <= st["rows_evaluated"] <= 8 * evaluated
+    assert moved <= evaluated
+    print(f"\n{precision} scale {scale}: E {E:.4f} +- {se:.4f} (exact {E_ex:.4f}), acceptance "
+          f"{ens.acceptance_rate:.3f}, null redraws {st['null_moves'] / st['proposals']:.3f}, "
+          f"rows per evaluated move {st['rows_evaluated'] / max(evaluated, 1):.2f}")
+
+
 def _optimise_ring(N=12, M=24, steps=50, lr=0.02):
     """imaginary-time SR on the Heisenberg ring with exact sums (the polynomial
     activations diverge if the weights grow much further than 50 such steps take them)"""
