@@ -11,7 +11,7 @@ kernels behind the C ABI in include/annongpu.h.
 from .pauli import PauliExpression, sigma_x, sigma_y, sigma_z, identity
 from .api import (
     Spins, PsiRBM, PsiDeep, PsiCNN, Operator, MonteCarloSpins, ExactSummationSpins, SampleList,
-    ExpectationValue, TDVP, log_psi_s, psi_O_k, psi_O_k_vector, log_psi, psi_vector,
+    ExpectationValue, TDVP, HilbertSpaceDistance, log_psi_s, psi_O_k, psi_O_k_vector, log_psi, psi_vector,
     log_psi_vector, apply_operator, local_energies, activation_function, setDevice,
     start_profiling, stop_profiling, profiler_query, profiler_reset, BlockProposal, rect_blocks, chain_blocks,
 )
@@ -25,7 +25,7 @@ from .evolution import RK4, Euler
 
 __all__ = [
     "PauliExpression", "sigma_x", "sigma_y", "sigma_z", "identity", "Spins", "PsiRBM", "PsiDeep", "PsiCNN",
-    "Operator", "MonteCarloSpins", "ExactSummationSpins", "SampleList", "ExpectationValue", "TDVP", "log_psi_s", "psi_O_k",
+    "Operator", "MonteCarloSpins", "ExactSummationSpins", "SampleList", "ExpectationValue", "TDVP", "HilbertSpaceDistance", "log_psi_s", "psi_O_k",
     "psi_O_k_vector", "log_psi", "psi_vector", "log_psi_vector", "apply_operator", "local_energies",
     "activation_function", "setDevice", "start_profiling", "stop_profiling", "profiler_query", "profiler_reset",
     "new_RBM", "new_deep_neural_network", "new_convolutional_network", "seeded_rbm", "seeded_rbm_deep",

@@ -810,6 +810,29 @@ class ExpectationValue:
         return g, complex(out[0], out[1])
 
 
+class HilbertSpaceDistance:
+    """HilbertSpaceDistance(num_params, gpu) -- network_functions/HilbertSpaceDistance.hpp,
+    bound at MAIN:436-441.  distance(psi', U psi) on psi's samples and its gradient with
+    respect to psi''s parameters."""
+
+    def __init__(self, num_params, gpu=True):
+        self.num_params = int(num_params)
+
+    def __call__(self, psi, psi_prime, operator_, is_unitary, spin_ensemble):
+        d = ctypes.c_double()
+        check(lib.ann_hilbert_space_distance(psi._h, psi_prime._h, operator_._h, 1 if is_unitary else 0,
+                                             spin_ensemble._h, 1.0, None, ref(d)))
+        return d.value
+
+    def gradient(self, psi, psi_prime, operator_, is_unitary, spin_ensemble, nu):
+        """(gradient, distance); the gradient of the distance itself for nu = 1"""
+        g = np.zeros(psi_prime.num_params, dtype=np.complex128)
+        d = ctypes.c_double()
+        check(lib.ann_hilbert_space_distance(psi._h, psi_prime._h, operator_._h, 1 if is_unitary else 0,
+                                             spin_ensemble._h, float(nu), ptr(g), ref(d)))
+        return g, d.value
+
+
 class TDVP:
     """TDVP(num_params, gpu) -- network_functions/TDVP.hpp:16-101.
 

@@ -674,6 +674,19 @@ int ann_local_energies(ann_operator op, ann_psi psi, ann_ensemble ens, double* e
     });
 }
 
+int ann_hilbert_space_distance(ann_psi psi, ann_psi psi_prime, ann_operator op, int32_t is_unitary, ann_ensemble ens,
+                               double nu, double* gradient, double* distance) {
+    return guard([&] {
+        need(psi, "psi");
+        need(psi_prime, "psi_prime");
+        need(op, "op");
+        need(ens, "ens");
+        need(distance, "distance");
+        *distance = ann::hilbert_space_distance(psi->impl, psi_prime->impl, op->impl, is_unitary != 0, ens->impl, nu,
+                                                gradient ? C(gradient) : nullptr);
+    });
+}
+
 // ---------------------------------------------------------------------- TDVP
 
 int ann_tdvp_create(ann_tdvp* out, uint32_t num_params) {

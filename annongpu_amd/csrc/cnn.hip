@@ -1,14 +1,17 @@
-// cnn.hip -- gfx950 kernels for PsiCNN (periodic convolutions, dim 3).
+// cnn.hip -- gfx950 kernels for the wave functions without a structured O:
+// PsiCNN (periodic convolutions, dim 3) and PsiDeep with several hidden layers.
 //
-// One 64-lane wavefront per configuration; lane l owns the sites x = l + 64 u.
-// The activations of every layer live in LDS (dynamic shared memory, one slice
-// per wavefront).  Reference behaviour restated here:
-//   forward      include/quantum_state/PsiCNN.hpp:97-175
-//   O_k          include/quantum_state/PsiCNN.hpp:183-266
-//   convolution  include/quantum_state/detail/Convolve.hpp:112-173
-//   Metropolis   include/ensembles/MonteCarlo.hpp:133-177 (the reference has no
-//                incremental update for PsiCNN, PsiCNN.hpp:177-181: every
-//                proposal re-runs the forward pass, as here)
+// One 64-lane wavefront per configuration; lane l owns the sites (units)
+// x = l + 64 u.  The activations of every layer live in LDS (dynamic shared memory,
+// one slice per wavefront).  Reference behaviour restated here:
+//   PsiCNN forward  include/quantum_state/PsiCNN.hpp:97-175
+//   PsiCNN O_k      include/quantum_state/PsiCNN.hpp:183-266
+//   convolution     include/quantum_state/detail/Convolve.hpp:112-173
+//   PsiDeep forward include/quantum_state/PsiDeep.hpp:171-266
+//   PsiDeep O_k     include/quantum_state/PsiDeep.hpp:345-445
+//   Metropolis      include/ensembles/MonteCarlo.hpp:133-177 (the reference keeps no
+//                   incremental state for these networks, PsiCNN.hpp:177-181: every
+//                   proposal re-runs the forward pass, as here)
 #include "cnn.hpp"
 
 #include <type_traits>
@@ -76,23 +79,66 @@ __device__ cplx<T> cnn_forward(const CnnView<T>& net, const uint64_t* w, cplx<T>
     return cscale(s, net.final_factor);
 }
 
+template <typename T>
+__device__ inline cplx<T> view_forward(const CnnView<T>& net, const uint64_t* w, unsigned char* smem, int lane) {
+    cplx<T>* A = reinterpret_cast<cplx<T>*>(smem);
+    return cnn_forward(net, w, A, A + net.max_ch * net.N, static_cast<cplx<T>*>(nullptr), lane);
+}
+
+// PsiDeep forward: Z[zoff[l] + j] = z_l[j], Act[zoff[l] + j] = act_l(z_l[j])
+template <typename T>
+__device__ cplx<T> mlp_forward(const MlpView<T>& net, const uint64_t* w, cplx<T>* Z, cplx<T>* Act, int lane) {
+    for (int l = 0; l < net.L; l++) {
+        const int M = net.size[l], C = net.conn[l];
+        for (int j = lane; j < M; j += 64) {
+            cplx<T> z = net.b[l][j];
+            for (int i = 0; i < C; i++) {
+                const int src = int(net.cn[l][i * M + j]);
+                const cplx<T> q = net.W[l][i * M + j];
+                if (l == 0) {
+                    const uint64_t word = src < 64 ? w[0] : src < 128 ? w[1] : src < 192 ? w[2] : w[3];
+                    const T sv = ((word >> (src & 63)) & 1) ? T(1) : T(-1);
+                    z.re += q.re * sv;
+                    z.im += q.im * sv;
+                } else {
+                    const cplx<T> a = Act[net.zoff[l - 1] + src];
+                    z.re += q.re * a.re - q.im * a.im;
+                    z.im += q.re * a.im + q.im * a.re;
+                }
+            }
+            Z[net.zoff[l] + j] = z;
+            Act[net.zoff[l] + j] = act(z, l);
+        }
+        __syncthreads();
+    }
+    const int last = net.L - 1;
+    cplx<T> total = {0, 0};
+    for (int j = lane; j < net.size[last]; j += 64) total = cadd(total, cmul(net.f[j], Act[net.zoff[last] + j]));
+    return {wave_sum(total.re), wave_sum(total.im)};
+}
+
+template <typename T>
+__device__ inline cplx<T> view_forward(const MlpView<T>& net, const uint64_t* w, unsigned char* smem, int lane) {
+    cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
+    return mlp_forward(net, w, Z, Z + net.zoff[net.L], lane);
+}
+
 // s is uniform over the wavefront, so these become scalar loads
 __device__ inline void load_words(const uint64_t* confs, int64_t s, int words, uint64_t* w) {
     for (int k = 0; k < kMaxWords; k++) w[k] = k < words ? confs[s * words + k] : 0;
 }
 
-template <typename T>
-__global__ __launch_bounds__(64) void cnn_log_psi_kernel(CnnView<T> net, const uint64_t* confs, int words,
-                                                         int64_t count, cplx<double> lp, cplx<double>* log_psi,
-                                                         double* weight) {
+template <typename V>
+__global__ __launch_bounds__(64) void generic_log_psi_kernel(V net, const uint64_t* confs, int words,
+                                                             int64_t count, cplx<double> lp, cplx<double>* log_psi,
+                                                             double* weight) {
+    using T = typename V::real_t;
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<T>* A = reinterpret_cast<cplx<T>*>(smem);
-    cplx<T>* A2 = A + net.max_ch * net.N;
     const int lane = threadIdx.x;
     for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
         uint64_t w[kMaxWords];
         load_words(confs, s, words, w);
-        const cplx<T> L = cnn_forward(net, w, A, A2, static_cast<cplx<T>*>(nullptr), lane);
+        const cplx<T> L = view_forward(net, w, smem, lane);
         if (lane == 0) {
             const cplx<double> r = {lp.re + double(L.re), lp.im + double(L.im)};
             log_psi[s] = r;
@@ -103,13 +149,12 @@ __global__ __launch_bounds__(64) void cnn_log_psi_kernel(CnnView<T> net, const u
 }
 
 // Metropolis chains, proposals as in the dense sampler (Philox stream per chain)
-template <typename T>
-__global__ __launch_bounds__(64) void cnn_mc_kernel(CnnView<T> net, uint64_t* chain_confs, int words, int num_chains,
-                                                    int move, int64_t num_proposals, uint64_t counter, uint32_t key0,
-                                                    uint32_t key1, uint32_t* accepted) {
+template <typename V>
+__global__ __launch_bounds__(64) void generic_mc_kernel(V net, uint64_t* chain_confs, int words, int num_chains,
+                                                        int move, int64_t num_proposals, uint64_t counter,
+                                                        uint32_t key0, uint32_t key1, uint32_t* accepted) {
+    using T = typename V::real_t;
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<T>* A = reinterpret_cast<cplx<T>*>(smem);
-    cplx<T>* A2 = A + net.max_ch * net.N;
     const int lane = threadIdx.x;
     const int c = blockIdx.x;
     if (c >= num_chains) return;
@@ -121,7 +166,7 @@ __global__ __launch_bounds__(64) void cnn_mc_kernel(CnnView<T> net, uint64_t* ch
         const int bits = N - 64 * k;
         msk[k] = k >= words ? 0 : bits >= 64 ? ~uint64_t(0) : bits > 0 ? (uint64_t(1) << bits) - 1 : 0;
     }
-    T cur = cnn_forward(net, w, A, A2, static_cast<cplx<T>*>(nullptr), lane).re;
+    T cur = view_forward(net, w, smem, lane).re;
     uint32_t acc = 0;
     for (int64_t q = 0; q < num_proposals; q++) {
         const uint64_t t = counter + uint64_t(q);
@@ -143,7 +188,7 @@ __global__ __launch_bounds__(64) void cnn_mc_kernel(CnnView<T> net, uint64_t* ch
         for (int k = 0; k < kMaxWords; k++) wn[k] = w[k];
         wn[i >> 6] ^= uint64_t(1) << (i & 63);
         if (kp >= 0) wn[kp >> 6] ^= uint64_t(1) << (kp & 63);
-        const T nxt = cnn_forward(net, wn, A, A2, static_cast<cplx<T>*>(nullptr), lane).re;
+        const T nxt = view_forward(net, wn, smem, lane).re;
         const T d2 = T(2) * (nxt - cur);
         const T uu = (T(r.z) + T(0.5)) * T(2.3283064365386963e-10);
         if (d2 > T(0) || uu <= dexp(d2)) {
@@ -158,18 +203,17 @@ __global__ __launch_bounds__(64) void cnn_mc_kernel(CnnView<T> net, uint64_t* ch
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(64) void cnn_local_energy_kernel(CnnView<T> net, OpView op, const uint64_t* confs,
-                                                              int64_t count, cplx<double>* e_loc) {
+template <typename V>
+__global__ __launch_bounds__(64) void generic_local_energy_kernel(V net, OpView op, const uint64_t* confs,
+                                                                  int64_t count, cplx<double>* e_loc) {
+    using T = typename V::real_t;
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<T>* A = reinterpret_cast<cplx<T>*>(smem);
-    cplx<T>* A2 = A + net.max_ch * net.N;
     const int lane = threadIdx.x;
     const int words = op.words;
     for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
         uint64_t w[kMaxWords];
         load_words(confs, s, words, w);
-        const cplx<T> L0 = cnn_forward(net, w, A, A2, static_cast<cplx<T>*>(nullptr), lane);
+        const cplx<T> L0 = view_forward(net, w, smem, lane);
         double e_re = 0.0, e_im = 0.0;
         for (int g = 0; g < op.num_groups; g++) {
             double c_re = 0.0, c_im = 0.0;
@@ -193,7 +237,7 @@ __global__ __launch_bounds__(64) void cnn_local_energy_kernel(CnnView<T> net, Op
                 e_im += c_im;
                 continue;
             }
-            const cplx<T> L1 = cnn_forward(net, wn, A, A2, static_cast<cplx<T>*>(nullptr), lane);
+            const cplx<T> L1 = view_forward(net, wn, smem, lane);
             const double d_re = double(L1.re) - double(L0.re), d_im = double(L1.im) - double(L0.im);
             const double m = exp(d_re);
             double sn, cs;
@@ -277,10 +321,69 @@ __global__ __launch_bounds__(64) void cnn_ok_kernel(CnnView<T> net, const uint64
     }
 }
 
+// O_k of a multi-layer PsiDeep by back-propagation (PsiDeep.hpp:345-445); row s of O is
+// written in parameter order (minus center if given).  LDS: Z, Act (forward), D (the
+// derivative of log psi by the pre-activations of every layer).
+template <typename T>
+__global__ __launch_bounds__(64) void mlp_ok_kernel(MlpView<T> net, const uint64_t* confs, int words, int64_t count,
+                                                    const cplx<double>* center, cplx<T>* O) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
+    cplx<T>* Act = Z + net.zoff[net.L];
+    cplx<T>* D = Act + net.zoff[net.L];
+    const int lane = threadIdx.x;
+    auto put = [&](cplx<T>* row, int k, cplx<T> v) {
+        if (center) {
+            v.re -= T(center[k].re);
+            v.im -= T(center[k].im);
+        }
+        row[k] = v;
+    };
+    for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
+        uint64_t w[kMaxWords];
+        load_words(confs, s, words, w);
+        mlp_forward(net, w, Z, Act, lane);
+        cplx<T>* row = O + s * net.num_params;
+        auto spin = [&](int i) {
+            const uint64_t word = i < 64 ? w[0] : i < 128 ? w[1] : i < 192 ? w[2] : w[3];
+            return ((word >> (i & 63)) & 1) ? T(1) : T(-1);
+        };
+        for (int i = lane; i < net.N; i += 64) put(row, i, {spin(i), T(0)});   // a: O = s_i
+        const int last = net.L - 1;
+        for (int j = lane; j < net.size[last]; j += 64)
+            D[net.zoff[last] + j] = cmul(net.f[j], dact(Z[net.zoff[last] + j], last));
+        __syncthreads();
+        for (int l = last; l >= 0; l--) {
+            const int M = net.size[l], C = net.conn[l];
+            for (int j = lane; j < M; j += 64) {
+                const cplx<T> dj = D[net.zoff[l] + j];
+                put(row, net.poff[l] + j, dj);
+                for (int i = 0; i < C; i++) {
+                    const int src = int(net.cn[l][i * M + j]);
+                    const cplx<T> in = l == 0 ? cplx<T>{spin(src), T(0)} : Act[net.zoff[l - 1] + src];
+                    put(row, net.poff[l] + M + i * M + j, cmul(dj, in));
+                }
+            }
+            if (l > 0) {
+                // D of layer l - 1: the weights reading each unit, gathered in a fixed order
+                for (int i = lane; i < net.size[l - 1]; i += 64) {
+                    cplx<T> acc = {0, 0};
+                    for (int e = net.rb[l][i]; e < net.rb[l][i + 1]; e++) {
+                        const int k = net.ri[l][e];
+                        acc = cadd(acc, cmul(net.W[l][k], D[net.zoff[l] + k % M]));
+                    }
+                    D[net.zoff[l - 1] + i] = cmul(acc, dact(Z[net.zoff[l - 1] + i], l - 1));
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // kernels with more than 64 KiB of LDS per block must opt in (at most the 160 KiB of a gfx950 CU)
 template <typename K>
 void allow_lds(K kernel, size_t bytes) {
-    if (bytes > 160 * 1024) throw std::invalid_argument("PsiCNN activations exceed the 160 KiB LDS of a CU");
+    if (bytes > 160 * 1024) throw std::invalid_argument("the network state of one configuration exceeds the 160 KiB LDS of a CU");
     if (bytes > 64 * 1024)
         ANN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
@@ -294,66 +397,81 @@ inline unsigned grid_for(int64_t count) {
 }  // namespace
 
 template <typename T>
-size_t cnn_forward_lds(const CnnView<T>& net) {
+size_t forward_lds(const CnnView<T>& net) {
     return sizeof(cplx<T>) * 2 * size_t(net.max_ch) * net.N;
 }
 template <typename T>
-size_t cnn_ok_lds(const CnnView<T>& net) {
+size_t forward_lds(const MlpView<T>& net) {
+    return sizeof(cplx<T>) * 2 * size_t(net.zoff[net.L]);
+}
+template <typename T>
+size_t ok_lds(const CnnView<T>& net) {
     return sizeof(cplx<T>) * (size_t(net.zoff[net.L]) + 4 * size_t(net.max_ch) * net.N);
 }
-
 template <typename T>
-void launch_cnn_log_psi(const CnnView<T>& net, const uint64_t* confs, int words, int64_t count,
-                        cplx<double> log_prefactor, cplx<double>* log_psi, double* weight, hipStream_t stream) {
+size_t ok_lds(const MlpView<T>& net) {
+    return sizeof(cplx<T>) * 3 * size_t(net.zoff[net.L]);
+}
+
+template <typename V>
+void launch_generic_log_psi(const V& net, const uint64_t* confs, int words, int64_t count,
+                            cplx<double> log_prefactor, cplx<double>* log_psi, double* weight, hipStream_t stream) {
     if (count <= 0) return;
-    allow_lds(cnn_log_psi_kernel<T>, cnn_forward_lds(net));
-    cnn_log_psi_kernel<T><<<grid_for(count), 64, cnn_forward_lds(net), stream>>>(net, confs, words, count,
+    allow_lds(generic_log_psi_kernel<V>, forward_lds(net));
+    generic_log_psi_kernel<V><<<grid_for(count), 64, forward_lds(net), stream>>>(net, confs, words, count,
                                                                                  log_prefactor, log_psi, weight);
     ANN_LAUNCH_CHECK();
 }
 
-template <typename T>
-void launch_cnn_mc(const CnnView<T>& net, uint64_t* chain_confs, int words, int num_chains, int move,
-                   int64_t num_proposals, uint64_t counter, uint64_t seed, uint32_t* accepted, hipStream_t stream) {
+template <typename V>
+void launch_generic_mc(const V& net, uint64_t* chain_confs, int words, int num_chains, int move,
+                       int64_t num_proposals, uint64_t counter, uint64_t seed, uint32_t* accepted,
+                       hipStream_t stream) {
     if (num_chains <= 0 || num_proposals <= 0) return;
-    allow_lds(cnn_mc_kernel<T>, cnn_forward_lds(net));
-    cnn_mc_kernel<T><<<unsigned(num_chains), 64, cnn_forward_lds(net), stream>>>(
+    allow_lds(generic_mc_kernel<V>, forward_lds(net));
+    generic_mc_kernel<V><<<unsigned(num_chains), 64, forward_lds(net), stream>>>(
         net, chain_confs, words, num_chains, move, num_proposals, counter, uint32_t(seed), uint32_t(seed >> 32),
         accepted);
     ANN_LAUNCH_CHECK();
 }
 
-template <typename T>
-void launch_cnn_local_energy(const CnnView<T>& net, const OpView& op, const uint64_t* confs, int64_t count,
-                             cplx<double>* e_loc, hipStream_t stream) {
+template <typename V>
+void launch_generic_local_energy(const V& net, const OpView& op, const uint64_t* confs, int64_t count,
+                                 cplx<double>* e_loc, hipStream_t stream) {
     if (count <= 0) return;
-    allow_lds(cnn_local_energy_kernel<T>, cnn_forward_lds(net));
-    cnn_local_energy_kernel<T><<<grid_for(count), 64, cnn_forward_lds(net), stream>>>(net, op, confs, count, e_loc);
+    allow_lds(generic_local_energy_kernel<V>, forward_lds(net));
+    generic_local_energy_kernel<V><<<grid_for(count), 64, forward_lds(net), stream>>>(net, op, confs, count, e_loc);
     ANN_LAUNCH_CHECK();
 }
 
-template <typename T>
-void launch_cnn_ok(const CnnView<T>& net, const uint64_t* confs, int words, int64_t count, const cplx<double>* center,
-                   cplx<T>* O, hipStream_t stream) {
+template <typename V>
+void launch_generic_ok(const V& net, const uint64_t* confs, int words, int64_t count, const cplx<double>* center,
+                       cplx<typename V::real_t>* O, hipStream_t stream) {
     if (count <= 0) return;
-    allow_lds(cnn_ok_kernel<T>, cnn_ok_lds(net));
-    cnn_ok_kernel<T><<<grid_for(count), 64, cnn_ok_lds(net), stream>>>(net, confs, words, count, center, O);
+    using T = typename V::real_t;
+    if constexpr (std::is_same<V, CnnView<T>>::value) {
+        allow_lds(cnn_ok_kernel<T>, ok_lds(net));
+        cnn_ok_kernel<T><<<grid_for(count), 64, ok_lds(net), stream>>>(net, confs, words, count, center, O);
+    } else {
+        allow_lds(mlp_ok_kernel<T>, ok_lds(net));
+        mlp_ok_kernel<T><<<grid_for(count), 64, ok_lds(net), stream>>>(net, confs, words, count, center, O);
+    }
     ANN_LAUNCH_CHECK();
 }
 
-#define ANN_CNN_INSTANTIATE(T)                                                                                   \
-    template size_t cnn_forward_lds<T>(const CnnView<T>&);                                                      \
-    template size_t cnn_ok_lds<T>(const CnnView<T>&);                                                           \
-    template void launch_cnn_log_psi<T>(const CnnView<T>&, const uint64_t*, int, int64_t, cplx<double>,        \
-                                        cplx<double>*, double*, hipStream_t);                                   \
-    template void launch_cnn_mc<T>(const CnnView<T>&, uint64_t*, int, int, int, int64_t, uint64_t, uint64_t,    \
-                                   uint32_t*, hipStream_t);                                                     \
-    template void launch_cnn_local_energy<T>(const CnnView<T>&, const OpView&, const uint64_t*, int64_t,       \
-                                             cplx<double>*, hipStream_t);                                       \
-    template void launch_cnn_ok<T>(const CnnView<T>&, const uint64_t*, int, int64_t, const cplx<double>*,      \
-                                   cplx<T>*, hipStream_t);
+#define ANN_GENERIC_INSTANTIATE(V)                                                                                  \
+    template void launch_generic_log_psi<V>(const V&, const uint64_t*, int, int64_t, cplx<double>, cplx<double>*, \
+                                            double*, hipStream_t);                                                 \
+    template void launch_generic_mc<V>(const V&, uint64_t*, int, int, int, int64_t, uint64_t, uint64_t,           \
+                                       uint32_t*, hipStream_t);                                                    \
+    template void launch_generic_local_energy<V>(const V&, const OpView&, const uint64_t*, int64_t,               \
+                                                 cplx<double>*, hipStream_t);                                      \
+    template void launch_generic_ok<V>(const V&, const uint64_t*, int, int64_t, const cplx<double>*,              \
+                                       cplx<typename V::real_t>*, hipStream_t);
 
-ANN_CNN_INSTANTIATE(float)
-ANN_CNN_INSTANTIATE(double)
+ANN_GENERIC_INSTANTIATE(CnnView<float>)
+ANN_GENERIC_INSTANTIATE(CnnView<double>)
+ANN_GENERIC_INSTANTIATE(MlpView<float>)
+ANN_GENERIC_INSTANTIATE(MlpView<double>)
 
 }  // namespace ann

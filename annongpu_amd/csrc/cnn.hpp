@@ -1,6 +1,6 @@
-// cnn.hpp -- device view and launchers of PsiCNN (cnn.hip) and of the
-// materialised-O kernels used by wave functions without a structured O
-// (generic_ok.hip).
+// cnn.hpp -- device views and launchers of the wave functions without a structured
+// O: PsiCNN and multi-layer PsiDeep (cnn.hip), and the materialised-O kernels they
+// share (generic_ok.hip).
 #pragma once
 
 #include "kernels.hpp"
@@ -17,6 +17,7 @@ constexpr int kMaxCnnLayers = 3;
 // neighbour of site x at offset o (include/quantum_state/detail/Convolve.hpp:122-148).
 template <typename T>
 struct CnnView {
+    using real_t = T;
     int N;
     int L;
     int nsym;
@@ -33,25 +34,49 @@ struct CnnView {
     const uint32_t* sym;
 };
 
-// LDS bytes per wavefront
-template <typename T>
-size_t cnn_forward_lds(const CnnView<T>& net);
-template <typename T>
-size_t cnn_ok_lds(const CnnView<T>& net);
+constexpr int kMaxMlpLayers = 4;
 
+// PsiDeep with more than one hidden layer (include/quantum_state/PsiDeep.hpp:171-215,
+// 345-445).  Hidden layer l (0-based) has size[l] units, each fed by conn[l] units
+// of the layer below (the spins for l = 0): pre-activation
+//   z_l[j] = b_l[j] + sum_i W_l[i][j] in_l[cn_l[i][j]],  in_0 = s,  in_l = act_{l-1}(z_{l-1}),
+// log psi = log_prefactor + sum_j f_j act_{L-1}(z_{L-1}[j]), act_0 = log-cosh series,
+// act_{l >= 1} = the layer >= 1 series.  Parameters: [a (N) | per layer b_l, W_l
+// ([i][j])]; b_l of layer l starts at poff[l].  rb/ri (layers l >= 1) list, for each
+// unit i of layer l - 1, the weights i' size[l] + j of layer l that read it (the
+// back-propagation gathers them in a fixed order).
 template <typename T>
-void launch_cnn_log_psi(const CnnView<T>& net, const uint64_t* confs, int words, int64_t count,
-                        cplx<double> log_prefactor, cplx<double>* log_psi, double* weight, hipStream_t stream);
-template <typename T>
-void launch_cnn_mc(const CnnView<T>& net, uint64_t* chain_confs, int words, int num_chains, int move,
-                   int64_t num_proposals, uint64_t counter, uint64_t seed, uint32_t* accepted, hipStream_t stream);
-template <typename T>
-void launch_cnn_local_energy(const CnnView<T>& net, const OpView& op, const uint64_t* confs, int64_t count,
-                             cplx<double>* e_loc, hipStream_t stream);
+struct MlpView {
+    using real_t = T;
+    int N, L, num_params;
+    int size[kMaxMlpLayers], conn[kMaxMlpLayers];
+    int zoff[kMaxMlpLayers + 1];
+    int poff[kMaxMlpLayers];
+    const uint32_t* cn[kMaxMlpLayers];
+    const cplx<T>* W[kMaxMlpLayers];
+    const cplx<T>* b[kMaxMlpLayers];
+    const cplx<T>* f;
+    const int* rb[kMaxMlpLayers];
+    const int* ri[kMaxMlpLayers];
+};
+
+// Kernels of the wave functions without a structured O (V = CnnView<T> or MlpView<T>):
+// one 64-lane wavefront per configuration, a full forward pass per configuration,
+// the layer state in LDS.
+template <typename V>
+void launch_generic_log_psi(const V& net, const uint64_t* confs, int words, int64_t count,
+                            cplx<double> log_prefactor, cplx<double>* log_psi, double* weight, hipStream_t stream);
+template <typename V>
+void launch_generic_mc(const V& net, uint64_t* chain_confs, int words, int num_chains, int move,
+                       int64_t num_proposals, uint64_t counter, uint64_t seed, uint32_t* accepted,
+                       hipStream_t stream);
+template <typename V>
+void launch_generic_local_energy(const V& net, const OpView& op, const uint64_t* confs, int64_t count,
+                                 cplx<double>* e_loc, hipStream_t stream);
 // O[s][k] = d log psi(s) / d param_k - center[k] (center may be null)
-template <typename T>
-void launch_cnn_ok(const CnnView<T>& net, const uint64_t* confs, int words, int64_t count,
-                   const cplx<double>* center, cplx<T>* O, hipStream_t stream);
+template <typename V>
+void launch_generic_ok(const V& net, const uint64_t* confs, int words, int64_t count, const cplx<double>* center,
+                       cplx<typename V::real_t>* O, hipStream_t stream);
 
 // ------------------------------------------------------- materialised O [n][P]
 // u[s] = sum_k O[s][k] v[k]   (fp64 accumulation)

@@ -313,7 +313,14 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void local_energy_k
         f[u] = net.f[lane + 64 * u];
     }
     const bool realf = net.real_f != 0;
-    const cplx<T> L0 = wave_csum(realf ? lane_sum_realf<T, U>(th, f) : lane_sum<T, U>(th, f));
+    // log-cosh of every unit at s: a ratio is summed as per-unit differences
+    // lc(theta'_j) - lc(theta_j), which keeps it accurate to the rounding of those
+    // differences (a difference of the two totals loses the digits the totals share)
+    // (kept in LDS: as registers they would push the kernel past 4 waves per SIMD)
+    __shared__ cplx<T> lc_lds[kWavesPerBlock][U][64];
+    cplx<T>(&lc0)[U][64] = lc_lds[threadIdx.x >> 6];
+#pragma unroll
+    for (int u = 0; u < U; u++) lc0[u][lane] = logcosh0(th[u]);
     double e_re = 0.0, e_im = 0.0;
     // off-diagonal terms wait in lane `pend` (log-ratio and coefficient) until 64 are
     // collected; then every lane evaluates its own exp / sincos once
@@ -376,10 +383,11 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void local_energy_k
                 }
             }
         }
-        const cplx<T> L1 = wave_csum(realf ? lane_sum_realf<T, U>(tn, f) : lane_sum<T, U>(tn, f));
+        const cplx<T> dL = wave_csum(realf ? lane_dsum<T, U, true>(tn, lc0, f, lane)
+                                           : lane_dsum<T, U, false>(tn, lc0, f, lane));
         if (lane == pend) {
-            q_dre = T(double(L1.re) - double(L0.re));
-            q_dim = T(double(L1.im) - double(L0.im));
+            q_dre = dL.re;
+            q_dim = dL.im;
             q_cre = T(c_re);
             q_cim = T(c_im);
         }

@@ -226,6 +226,12 @@ void launch_site_major(const cplx<double>* in, int N, int M, int shift, bool for
 void launch_weighted_sums(const cplx<double>* e_loc, const double* w, int64_t count,
                           double* out4 /* E.re, E.im, E2, sum w */, hipStream_t stream);
 
+// HilbertSpaceDistance per-sample terms: c_omega = w omega, c_ratio = w ratio and
+// sums4 = [sum w omega (re, im), sum w ratio, sum w norm]
+void launch_hsd_terms(const cplx<double>* e_loc, const cplx<double>* lp, const cplx<double>* lpp, const double* w,
+                      int64_t n, bool unitary, cplx<double>* c_omega, cplx<double>* c_ratio, double* sums4,
+                      hipStream_t stream);
+
 // dense Hermitian Gram S = O^H diag(w) O  (O row-major [count][P], complex T)
 template <typename T>
 void launch_gram(const cplx<T>* O, const double* w, int64_t count, int P, cplx<double>* S,
@@ -235,8 +241,10 @@ void launch_gram(const cplx<T>* O, const double* w, int64_t count, int P, cplx<d
 void launch_dense_matvec(const cplx<double>* S, const cplx<double>* x, int P, double shift,
                          cplx<double>* y, hipStream_t stream);
 
-// in-place blocked Cholesky of the lower triangle (fp64 complex); info != 0 on failure
-void cholesky_lower(cplx<double>* A, int P, int* d_info, hipStream_t stream);
+// in-place blocked Cholesky of the lower triangle (fp64 complex, matrix cores); info != 0
+// on failure; work holds cholesky_work_elems() complex values
+size_t cholesky_work_elems();
+void cholesky_lower(cplx<double>* A, int P, int* d_info, cplx<double>* work, hipStream_t stream);
 void cholesky_solve_inplace(const cplx<double>* L, int P, cplx<double>* x, hipStream_t stream);
 
 }  // namespace ann

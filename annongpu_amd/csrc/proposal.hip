@@ -61,7 +61,28 @@ __device__ inline float logits8_reduce(const float (&part)[kMaxBlock]) {
 }
 
 // ------------------------------------------------------------- block sweep
-template <typename T, int U, int WORDS>
+// Hidden unit 4 lane + qq of the proposal network sits in lane `lane`, slot qq, so a
+// lane's four first-layer weights of a site are one 16-byte load and its four
+// second-layer weights one 8-byte load.
+
+// the first-layer columns W1q[site][4 lane .. 4 lane + 3] of a block's sites
+template <int BS>
+__device__ inline void load_block_cols(const BlockNet& bn, const int (&st)[kMaxBlock], int bs, int lane,
+                                       int4 (&wq)[kMaxBlock]) {
+    const int4* W1 = reinterpret_cast<const int4*>(bn.W1q);
+#pragma unroll
+    for (int j = 0; j < kMaxBlock; j++)
+        wq[j] = (BS == kMaxBlock || j < bs) ? W1[int64_t(st[j]) * (kHPad / 4) + lane] : make_int4(0, 0, 0, 0);
+}
+
+// The block of proposal q is drawn from the Philox batch of its 64-proposal group, so
+// the sites and the first-layer columns of proposal q + 1 are loaded while proposal q
+// is evaluated (its second-layer rows are loaded at its start and arrive while the
+// hidden units are formed; prefetching them too raised the registers past the point
+// where the sweep got slower: 1.59 against 1.55 ms for 65 moves of 16,384 chains).
+// BS = 8 (the benchmark's plaquettes) is compiled with the block size fixed; BS = 0
+// reads it at run time.
+template <typename T, int U, int WORDS, int BS>
 __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, BlockNet bn, uint64_t* chain_confs,
                                                              cplx<T>* theta, int num_chains, int64_t num_proposals,
                                                              uint64_t counter, uint32_t key0, uint32_t key1,
@@ -70,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, B
     const int c = blockIdx.x * kWavesPerBlock + uniform_int(threadIdx.x >> 6);
     if (c >= num_chains) return;
     const int N = net.N;
-    const int bs = bn.bs;
+    const int bs = BS == kMaxBlock ? kMaxBlock : bn.bs;
     uint64_t w[WORDS];
 #pragma unroll
     for (int k = 0; k < WORDS; k++) w[k] = uniform_u64(chain_confs[int64_t(c) * WORDS + k]);
@@ -85,69 +106,105 @@ __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, B
         return wave_sum(realf ? lane_re_sum<T, U, true>(z, f) : lane_re_sum<T, U, false>(z, f));
     };
     T cur = log_amp(th);
-    // first-layer pre-activations of the whole configuration
+    // first-layer pre-activations of the whole configuration (4 hidden units per lane)
+    const int4* W1 = reinterpret_cast<const int4*>(bn.W1q);
     int32_t a1[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) a1[q] = bn.b1q[lane + 64 * q];
+    {
+        const int4 b = reinterpret_cast<const int4*>(bn.b1q)[lane];
+        a1[0] = b.x;
+        a1[1] = b.y;
+        a1[2] = b.z;
+        a1[3] = b.w;
+    }
     for (int i = 0; i < N; i++) {
-        const int32_t* col = bn.W1q + int64_t(i) * kHPad + lane;
+        const int4 col = W1[int64_t(i) * (kHPad / 4) + lane];
         const bool up = conf_bit(w, i);
-#pragma unroll
-        for (int q = 0; q < 4; q++) a1[q] += up ? col[64 * q] : -col[64 * q];
+        a1[0] += up ? col.x : -col.x;
+        a1[1] += up ? col.y : -col.y;
+        a1[2] += up ? col.z : -col.z;
+        a1[3] += up ? col.w : -col.w;
     }
     uint32_t acc = 0, nul = 0, rows = 0;
-    uint32_t rx = 0, ry = 0, rz = 0;
-    for (int64_t q = 0; q < num_proposals; q++) {
-        const int ql = int(q & 63);
-        if (ql == 0) {
-            const uint64_t t = counter + uint64_t(q) + uint64_t(lane);
-            const u32x4 rr = philox(uint32_t(c), uint32_t(t), uint32_t(t >> 32), 3u, key0, key1);
-            rx = rr.x;
-            ry = rr.y;
-            rz = rr.z;
-        }
-        const uint32_t r_x = readlane_u32(rx, ql), r_y = readlane_u32(ry, ql), r_z = readlane_u32(rz, ql);
+    // Philox batch of proposals q0 .. q0 + 63 (x: block, y: pattern, z: acceptance)
+    auto batch = [&](int64_t q0, uint32_t& rx, uint32_t& ry, uint32_t& rz) {
+        const uint64_t t = counter + uint64_t(q0) + uint64_t(lane);
+        const u32x4 rr = philox(uint32_t(c), uint32_t(t), uint32_t(t >> 32), 3u, key0, key1);
+        rx = rr.x;
+        ry = rr.y;
+        rz = rr.z;
+    };
+    auto block_sites = [&](uint32_t r_x, int (&st)[kMaxBlock]) {
         const int b = int(range32(r_x, uint32_t(bn.num_blocks)));
         const int* bsites = bn.blocks + int64_t(b) * bs;
-        int st[kMaxBlock];
+#pragma unroll
+        for (int j = 0; j < kMaxBlock; j++) st[j] = (BS == kMaxBlock || j < bs) ? bsites[j] : bsites[0];
+    };
+    uint32_t rx = 0, ry = 0, rz = 0;
+    int st[kMaxBlock];
+    int4 wq[kMaxBlock];
+    if (num_proposals > 0) {
+        batch(0, rx, ry, rz);
+        block_sites(readlane_u32(rx, 0), st);
+        load_block_cols<BS>(bn, st, bs, lane, wq);
+    }
+    for (int64_t q = 0; q < num_proposals; q++) {
+        const int ql = int(q & 63);
+        const uint32_t r_y = readlane_u32(ry, ql), r_z = readlane_u32(rz, ql);
+        // the next proposal's block and columns, in flight while this one is evaluated
+        int nst[kMaxBlock];
+        int4 nwq[kMaxBlock];
+        const bool more = q + 1 < num_proposals;
+        if (more) {
+            if (ql == 63) batch(q + 1, rx, ry, rz);
+            block_sites(readlane_u32(rx, int((q + 1) & 63)), nst);
+            load_block_cols<BS>(bn, nst, bs, lane, nwq);
+        }
+        uint2 w2[kMaxBlock];   // W2h[site][4 lane .. 4 lane + 3] (bf16 pairs)
+        {
+            const uint2* W2 = reinterpret_cast<const uint2*>(bn.W2h);
+#pragma unroll
+            for (int j = 0; j < kMaxBlock; j++)
+                w2[j] = (BS == kMaxBlock || j < bs) ? W2[int64_t(st[j]) * (kHPad / 4) + lane] : make_uint2(0u, 0u);
+        }
+        float b2v[kMaxBlock];
         uint32_t p_old = 0;
 #pragma unroll
         for (int j = 0; j < kMaxBlock; j++) {
-            st[j] = j < bs ? bsites[j] : 0;
-            p_old |= j < bs ? conf_bit(w, st[j]) << j : 0u;
+            const bool in = BS == kMaxBlock || j < bs;
+            b2v[j] = in ? bn.b2[st[j]] : 0.f;
+            p_old |= in ? conf_bit(w, st[j]) << j : 0u;
         }
         const int k = __popc(p_old);
         // hidden units of the masked input
         float hd[4];
-#pragma unroll
-        for (int qq = 0; qq < 4; qq++) {
-            int32_t a = a1[qq];
+        {
+            int32_t a[4] = {a1[0], a1[1], a1[2], a1[3]};
 #pragma unroll
             for (int j = 0; j < kMaxBlock; j++) {
-                if (j < bs) {
-                    const int32_t wv = bn.W1q[int64_t(st[j]) * kHPad + lane + 64 * qq];
-                    a -= (p_old >> j) & 1u ? wv : -wv;
-                }
+                const bool up = (p_old >> j) & 1u;
+                a[0] = up ? a[0] - wq[j].x : a[0] + wq[j].x;
+                a[1] = up ? a[1] - wq[j].y : a[1] + wq[j].y;
+                a[2] = up ? a[2] - wq[j].z : a[2] + wq[j].z;
+                a[3] = up ? a[3] - wq[j].w : a[3] + wq[j].w;
             }
-            hd[qq] = tanh_hw(float(a) * (1.0f / kFixScale));
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) hd[qq] = tanh_hw(float(a[qq]) * (1.0f / kFixScale));
         }
         // logits of the block's sites: lane partials, one transposed wave reduction
         float part[kMaxBlock];
 #pragma unroll
         for (int j = 0; j < kMaxBlock; j++) {
-            part[j] = 0.f;
-            if (j < bs) {
-                const uint16_t* row = bn.W2h + int64_t(st[j]) * kHPad + lane;
-#pragma unroll
-                for (int qq = 0; qq < 4; qq++) part[j] += bf16_to_f32(row[64 * qq]) * hd[qq];
-            }
+            const uint2 v = w2[j];
+            part[j] = __uint_as_float(v.x << 16) * hd[0] + __uint_as_float(v.x & 0xFFFF0000u) * hd[1] +
+                      __uint_as_float(v.y << 16) * hd[2] + __uint_as_float(v.y & 0xFFFF0000u) * hd[3];
         }
         const float lsum8 = logits8_reduce(part);
         float lg[kMaxBlock];
 #pragma unroll
         for (int j = 0; j < kMaxBlock; j++)
-            lg[j] = j < bs ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lsum8), 8 * j)) + bn.b2[st[j]]
-                           : 0.f;
+            lg[j] = (BS == kMaxBlock || j < bs)
+                        ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lsum8), 8 * j)) + b2v[j]
+                        : 0.f;
         // patterns 4 lane + r with k up spins; weights exp(e_p - e_max).  Bits 2..7 of
         // the pattern are the lane's bits 0..5: their energy is shared by the 4 patterns.
         float e[4], wr[4];
@@ -159,11 +216,11 @@ __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, B
         const int pc_hi = __popc(uint32_t(lane));
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const uint32_t p = uint32_t(4 * lane + r);
-            ok[r] = p < (1u << bs) && pc_hi + __popc(uint32_t(r)) == k;
-            const float s = e_hi + ((r & 1) ? lg[0] : 0.f) + ((r & 2) ? lg[1] : 0.f);
-            e[r] = s;
-            if (ok[r]) emax_l = fmaxf(emax_l, s);
+            const uint32_t pt = uint32_t(4 * lane + r);
+            ok[r] = (BS == kMaxBlock || pt < (1u << bs)) && pc_hi + __popc(uint32_t(r)) == k;
+            const float sv = e_hi + ((r & 1) ? lg[0] : 0.f) + ((r & 2) ? lg[1] : 0.f);
+            e[r] = sv;
+            if (ok[r]) emax_l = fmaxf(emax_l, sv);
         }
         const float emax = wave_max(emax_l);
         float lsum = 0.f;
@@ -190,55 +247,75 @@ __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, B
         }
         if (pick < 0) pick = last;
         if (pick < 0) pick = int(p_old);
-        const uint32_t p_new = uint32_t(__builtin_amdgcn_readlane(pick, L)) & ((1u << bs) - 1u);
+        const uint32_t p_new =
+            uint32_t(__builtin_amdgcn_readlane(pick, L)) & (BS == kMaxBlock ? 0xFFu : ((1u << bs) - 1u));
         const uint32_t flips = p_old ^ p_new;
-        if (flips == 0) {   // the current pattern was redrawn: nothing to evaluate, nothing changes
-            nul++;
-            continue;
-        }
-        rows += uint32_t(__popc(flips));
-        float e_old = 0.f, e_new = 0.f;
-#pragma unroll
-        for (int j = 0; j < kMaxBlock; j++) {
-            e_old += (p_old >> j) & 1u ? lg[j] : 0.f;
-            e_new += (p_new >> j) & 1u ? lg[j] : 0.f;
-        }
-        // psi ratio: theta' = theta - 2 s_j W_j over the flipped sites
-        cplx<T> z[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) z[u] = th[u];
-#pragma unroll
-        for (int j = 0; j < kMaxBlock; j++) {
-            if ((flips >> j) & 1u) {
-                const T cj = (p_old >> j) & 1u ? T(-2) : T(2);
-                cplx<T> row[U];
-                load_row<T, U>(row, net.Wd, st[j], net.Mpad, lane);
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    z[u].re += cj * row[u].re;
-                    z[u].im += cj * row[u].im;
-                }
-            }
-        }
-        const T nxt = log_amp(z);
-        const T d = T(2) * (nxt - cur) + T(e_old - e_new);
-        const T uu = (T(r_z) + T(0.5)) * T(2.3283064365386963e-10);
-        const bool accept = d > T(0) || uu <= dexp(d);
-        if (accept) {
-#pragma unroll
-            for (int u = 0; u < U; u++) th[u] = z[u];
-            cur = nxt;
+        if (flips != 0) {
+            float e_old = 0.f, e_new = 0.f;
 #pragma unroll
             for (int j = 0; j < kMaxBlock; j++) {
-                if ((flips >> j) & 1u) {
-                    conf_flip(w, st[j]);
-                    const bool was_up = (p_old >> j) & 1u;
-                    const int32_t* col = bn.W1q + int64_t(st[j]) * kHPad + lane;
+                e_old += (p_old >> j) & 1u ? lg[j] : 0.f;
+                e_new += (p_new >> j) & 1u ? lg[j] : 0.f;
+            }
+            rows += uint32_t(__popc(flips));
+            // psi ratio: theta' = theta - 2 s_j W_j over the flipped sites, whose rows are
+            // loaded two at a time (a move flips an even number of sites: S_z is conserved)
+            cplx<T> z[U];
 #pragma unroll
-                    for (int qq = 0; qq < 4; qq++) a1[qq] += was_up ? -2 * col[64 * qq] : 2 * col[64 * qq];
+            for (int u = 0; u < U; u++) z[u] = th[u];
+            uint32_t fl = flips;
+            while (fl) {
+                const int j0 = int(__builtin_ctz(fl));
+                fl &= fl - 1u;
+                const int j1 = fl ? int(__builtin_ctz(fl)) : j0;
+                const T c1 = fl ? ((p_old >> j1) & 1u ? T(-2) : T(2)) : T(0);
+                fl &= fl ? fl - 1u : 0u;
+                int s0 = st[0], s1 = st[0];
+#pragma unroll
+                for (int j = 0; j < kMaxBlock; j++) {
+                    s0 = j == j0 ? st[j] : s0;
+                    s1 = j == j1 ? st[j] : s1;
+                }
+                const T c0 = (p_old >> j0) & 1u ? T(-2) : T(2);
+                cplx<T> row0[U], row1[U];
+                load_row<T, U>(row0, net.Wd, s0, net.Mpad, lane);
+                load_row<T, U>(row1, net.Wd, s1, net.Mpad, lane);
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    z[u].re += c0 * row0[u].re + c1 * row1[u].re;
+                    z[u].im += c0 * row0[u].im + c1 * row1[u].im;
                 }
             }
-            acc++;
+            const T nxt = log_amp(z);
+            const T d = T(2) * (nxt - cur) + T(e_old - e_new);
+            const T uu = (T(r_z) + T(0.5)) * T(2.3283064365386963e-10);
+            if (d > T(0) || uu <= dexp(d)) {
+#pragma unroll
+                for (int u = 0; u < U; u++) th[u] = z[u];
+                cur = nxt;
+#pragma unroll
+                for (int j = 0; j < kMaxBlock; j++) {
+                    if ((flips >> j) & 1u) {
+                        conf_flip(w, st[j]);
+                        // the site leaves the block's masked sum with its new spin: +-2 W1
+                        const bool was_up = (p_old >> j) & 1u;
+                        a1[0] += was_up ? -2 * wq[j].x : 2 * wq[j].x;
+                        a1[1] += was_up ? -2 * wq[j].y : 2 * wq[j].y;
+                        a1[2] += was_up ? -2 * wq[j].z : 2 * wq[j].z;
+                        a1[3] += was_up ? -2 * wq[j].w : 2 * wq[j].w;
+                    }
+                }
+                acc++;
+            }
+        } else {   // the current pattern was redrawn: nothing to evaluate, nothing changes
+            nul++;
+        }
+        if (more) {
+#pragma unroll
+            for (int j = 0; j < kMaxBlock; j++) {
+                st[j] = nst[j];
+                wq[j] = nwq[j];
+            }
         }
     }
 #pragma unroll
@@ -471,13 +548,21 @@ void launch_block_sweep(const DenseView<T>& net, const BlockNet& bn, uint64_t* c
     dispatch_u(net.Mpad / 64, [&](auto tag) {
         constexpr int U = std::decay_t<decltype(tag)>::value;
         const unsigned grid = grid_waves(num_chains);
-        switch (words) {
-            case 1: block_sweep_kernel<T, U, 1><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
-            case 2: block_sweep_kernel<T, U, 2><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
-            case 3: block_sweep_kernel<T, U, 3><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
-            case 4: block_sweep_kernel<T, U, 4><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
-            default: throw HipError("configurations of 1..4 words are supported");
-        }
+        auto go = [&](auto bs_tag) {
+            constexpr int BS = decltype(bs_tag)::value;
+            switch (words) {
+                case 1: block_sweep_kernel<T, U, 1, BS><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
+                case 2: block_sweep_kernel<T, U, 2, BS><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
+                case 3: block_sweep_kernel<T, U, 3, BS><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
+                case 4: block_sweep_kernel<T, U, 4, BS><<<grid, kBlock, 0, stream>>>(net, bn, chain_confs, theta, num_chains, num_proposals, counter, k0, k1, accepted, bstat); break;
+                default: throw HipError("configurations of 1..4 words are supported");
+            }
+        };
+        // 8-site blocks of the fp32 path with the block size fixed at compile time
+        if (std::is_same<T, float>::value && bn.bs == kMaxBlock)
+            go(std::integral_constant<int, kMaxBlock>{});
+        else
+            go(std::integral_constant<int, 0>{});
     });
     ANN_LAUNCH_CHECK();
 }

@@ -105,6 +105,23 @@ static hipStream_t S() { return Runtime::get().stream(); }
 
 // --------------------------------------------------------------------- Psi
 
+// wave functions without a structured O: f(view) with the PsiCNN or multi-layer PsiDeep
+// view of the wave function's precision
+template <typename F>
+static void with_generic(Psi& psi, F&& f) {
+    if (psi.kind == PSI_CNN) {
+        if (psi.precision == 0)
+            f(psi.cnn64());
+        else
+            f(psi.cnn32());
+    } else {
+        if (psi.precision == 0)
+            f(psi.mlp64());
+        else
+            f(psi.mlp32());
+    }
+}
+
 static int units_per_lane(int M) {
     const int need = (M + 63) / 64;
     for (int u : {1, 2, 4, 7, 8, 16})
@@ -138,6 +155,14 @@ void Psi::finalize() {
         num_params = int(params.size());
     }
     dirty = true;
+}
+
+// a complex host vector into an fp64 and an fp32 device copy
+static void up_both(DevBuf<cplx<double>>& d64, DevBuf<cplx<float>>& d32, const std::vector<cd>& h) {
+    d64.upload(reinterpret_cast<const cplx<double>*>(h.data()), h.size());
+    std::vector<cplx<float>> t(h.size());
+    for (size_t i = 0; i < h.size(); i++) t[i] = {float(h[i].real()), float(h[i].imag())};
+    d32.upload(t.data(), t.size());
 }
 
 void Psi::sync_device() {
@@ -178,6 +203,48 @@ void Psi::sync_device() {
         return;
     }
     if (!dense()) {
+        // multi-layer PsiDeep: every layer's connections, weights and biases, the final
+        // weights and, for layers l >= 1, the weights reading each unit of layer l - 1
+        std::vector<uint32_t> cn;
+        std::vector<cd> wv, bv;
+        std::vector<int> rb, ri;
+        mlp_cn_off.assign(num_layers, 0);
+        mlp_w_off.assign(num_layers, 0);
+        mlp_b_off.assign(num_layers, 0);
+        mlp_rb_off.assign(num_layers, 0);
+        mlp_ri_off.assign(num_layers, 0);
+        for (int l = 0; l < num_layers; l++) {
+            const int Ml = sizes[l], Cl = conns[l];
+            mlp_cn_off[l] = int(cn.size());
+            mlp_w_off[l] = int(wv.size());
+            mlp_b_off[l] = int(bv.size());
+            cn.insert(cn.end(), conn_l[l].begin(), conn_l[l].end());
+            wv.insert(wv.end(), w_l[l].begin(), w_l[l].end());
+            bv.insert(bv.end(), b_l[l].begin(), b_l[l].end());
+            mlp_rb_off[l] = int(rb.size());
+            mlp_ri_off[l] = int(ri.size());
+            if (l > 0) {
+                const int prev = sizes[l - 1];
+                std::vector<std::vector<int>> by_unit(prev);
+                for (int i = 0; i < Cl; i++)
+                    for (int j = 0; j < Ml; j++) by_unit[conn_l[l][size_t(i) * Ml + j]].push_back(i * Ml + j);
+                int e = 0;
+                for (int u = 0; u < prev; u++) {
+                    rb.push_back(e);
+                    ri.insert(ri.end(), by_unit[u].begin(), by_unit[u].end());
+                    e += int(by_unit[u].size());
+                }
+                rb.push_back(e);
+            }
+        }
+        mlp_cn.upload(cn.data(), cn.size());
+        up_both(mlp_w64, mlp_w32, wv);
+        up_both(mlp_b64, mlp_b32, bv);
+        up_both(f64, f32, f);
+        if (rb.empty()) rb.push_back(0);
+        if (ri.empty()) ri.push_back(0);
+        mlp_rb.upload(rb.data(), rb.size());
+        mlp_ri.upload(ri.data(), ri.size());
         dirty = false;
         return;
     }
@@ -273,6 +340,41 @@ static CnnView<T> make_cnn_view(const Psi& q, const cplx<T>* p, const int* nbr, 
     v.nbr = nbr;
     v.sym = sym;
     return v;
+}
+
+template <typename T>
+static MlpView<T> make_mlp_view(const Psi& q, const cplx<T>* w, const cplx<T>* b, const cplx<T>* f) {
+    MlpView<T> v{};
+    v.N = q.N;
+    v.L = q.num_layers;
+    v.num_params = q.num_params;
+    int z = 0, par = q.N;
+    for (int l = 0; l < q.num_layers; l++) {
+        v.size[l] = q.sizes[l];
+        v.conn[l] = q.conns[l];
+        v.zoff[l] = z;
+        v.poff[l] = par;
+        z += q.sizes[l];
+        par += q.sizes[l] + q.sizes[l] * q.conns[l];
+        v.cn[l] = q.mlp_cn.p + q.mlp_cn_off[l];
+        v.W[l] = w + q.mlp_w_off[l];
+        v.b[l] = b + q.mlp_b_off[l];
+        v.rb[l] = q.mlp_rb.p + q.mlp_rb_off[l];
+        v.ri[l] = q.mlp_ri.p + q.mlp_ri_off[l];
+    }
+    v.zoff[q.num_layers] = z;
+    v.f = f;
+    return v;
+}
+
+MlpView<double> Psi::mlp64() {
+    sync_device();
+    return make_mlp_view<double>(*this, mlp_w64.p, mlp_b64.p, f64.p);
+}
+
+MlpView<float> Psi::mlp32() {
+    sync_device();
+    return make_mlp_view<float>(*this, mlp_w32.p, mlp_b32.p, f32.p);
 }
 
 CnnView<double> Psi::cnn64() {
@@ -446,7 +548,7 @@ void compute_theta(Psi& psi, const uint64_t* confs, int64_t count, Samples& out,
 
 static void require_dense(const Psi& psi) {
     if (!psi.dense())
-        throw std::logic_error("this wave function kind is not available on the GPU path yet");
+        throw std::logic_error("this operation needs a PsiRBM or a one-layer PsiDeep (a structured O_k)");
 }
 
 static void prepare_samples(Psi& psi, Samples& s, int64_t count) {
@@ -546,12 +648,11 @@ void Ensemble::settle_acceptance() {
 // log psi (and |psi|^2 weights when w is given) of configurations, any wave function
 void log_psi_of(Psi& psi, const uint64_t* confs, int64_t count, Samples& s, double* w) {
     const cplx<double> lp = {psi.log_prefactor.real(), psi.log_prefactor.imag()};
-    if (psi.kind == PSI_CNN) {
+    if (!psi.dense()) {
         ScopedTimer timer("log_psi");
-        if (psi.precision == 0)
-            launch_cnn_log_psi<double>(psi.cnn64(), confs, psi.words, count, lp, s.log_psi.p, w, S());
-        else
-            launch_cnn_log_psi<float>(psi.cnn32(), confs, psi.words, count, lp, s.log_psi.p, w, S());
+        with_generic(psi, [&](const auto& v) {
+            launch_generic_log_psi(v, confs, psi.words, count, lp, s.log_psi.p, w, S());
+        });
         return;
     }
     require_dense(psi);
@@ -579,9 +680,9 @@ static void sample_list(Psi& psi, Ensemble& ens) {
     log_psi_of(psi, s.confs.p, count, s, nullptr);
 }
 
-// PsiCNN: Metropolis with a full forward pass per proposal (the reference keeps
-// no incremental state for PsiCNN, include/quantum_state/PsiCNN.hpp:177-181)
-static void sample_cnn(Psi& psi, Ensemble& ens) {
+// PsiCNN and multi-layer PsiDeep: Metropolis with a full forward pass per proposal
+// (the reference keeps no incremental state for them, include/quantum_state/PsiCNN.hpp:177-181)
+static void sample_generic(Psi& psi, Ensemble& ens) {
     psi.sync_device();
     if (ens.kind == ENS_LIST) return sample_list(psi, ens);
     Samples& s = ens.samples;
@@ -603,12 +704,9 @@ static void sample_cnn(Psi& psi, Ensemble& ens) {
         const int64_t nprop = (step == 0 ? therm : 0) + int64_t(ens.num_sweeps) * psi.N;
         {
             ScopedTimer timer("mc_sweep");
-            if (psi.precision == 0)
-                launch_cnn_mc<double>(psi.cnn64(), ens.chains.p, words, B, ens.move, nprop, ens.counter, ens.seed,
-                                      ens.acc.p, S());
-            else
-                launch_cnn_mc<float>(psi.cnn32(), ens.chains.p, words, B, ens.move, nprop, ens.counter, ens.seed,
-                                     ens.acc.p, S());
+            with_generic(psi, [&](const auto& v) {
+                launch_generic_mc(v, ens.chains.p, words, B, ens.move, nprop, ens.counter, ens.seed, ens.acc.p, S());
+            });
         }
         ens.counter += uint64_t(nprop);
         ANN_HIP_CHECK(hipMemcpyAsync(s.confs.p + step * B * words, ens.chains.p, sizeof(uint64_t) * B * words,
@@ -621,7 +719,7 @@ static void sample_cnn(Psi& psi, Ensemble& ens) {
 }
 
 void sample(Psi& psi, Ensemble& ens) {
-    if (psi.kind == PSI_CNN) return sample_cnn(psi, ens);
+    if (!psi.dense()) return sample_generic(psi, ens);
     require_dense(psi);
     psi.sync_device();
     if (ens.kind == ENS_LIST) return sample_list(psi, ens);
@@ -725,11 +823,10 @@ void local_energies(Psi& psi, Operator& op, Samples& s) {
     const DeviceOp& d = op.device(psi.words);
     s.e_loc.ensure(s.count);
     ScopedTimer timer("local_energy");
-    if (psi.kind == PSI_CNN) {
-        if (psi.precision == 0)
-            launch_cnn_local_energy<double>(psi.cnn64(), d.view(), s.confs.p, s.count, s.e_loc.p, S());
-        else
-            launch_cnn_local_energy<float>(psi.cnn32(), d.view(), s.confs.p, s.count, s.e_loc.p, S());
+    if (!psi.dense()) {
+        with_generic(psi, [&](const auto& v) {
+            launch_generic_local_energy(v, d.view(), s.confs.p, s.count, s.e_loc.p, S());
+        });
         return;
     }
     if (psi.precision == 0)
@@ -964,14 +1061,21 @@ DevBuf<cplx<T>>& materialised(Tdvp& t) {
         return t.Og32;
 }
 
-// O_k of a PsiCNN for `count` configurations into O [count][P]
+// O_k of a PsiCNN or multi-layer PsiDeep for `count` configurations into O [count][P]
 template <typename T>
-void cnn_ok(Psi& psi, const uint64_t* confs, int64_t count, cplx<T>* O) {
+void generic_ok(Psi& psi, const uint64_t* confs, int64_t count, cplx<T>* O) {
     ScopedTimer timer("o_k");
-    if constexpr (sizeof(T) == 8)
-        launch_cnn_ok<double>(psi.cnn64(), confs, psi.words, count, nullptr, O, S());
-    else
-        launch_cnn_ok<float>(psi.cnn32(), confs, psi.words, count, nullptr, O, S());
+    if constexpr (sizeof(T) == 8) {
+        if (psi.kind == PSI_CNN)
+            launch_generic_ok(psi.cnn64(), confs, psi.words, count, nullptr, O, S());
+        else
+            launch_generic_ok(psi.mlp64(), confs, psi.words, count, nullptr, O, S());
+    } else {
+        if (psi.kind == PSI_CNN)
+            launch_generic_ok(psi.cnn32(), confs, psi.words, count, nullptr, O, S());
+        else
+            launch_generic_ok(psi.mlp32(), confs, psi.words, count, nullptr, O, S());
+    }
 }
 
 // TDVP statistics from a materialised O (TDVP.cu.template:77-126,181-302)
@@ -987,7 +1091,7 @@ void tdvp_eval_generic(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with
     launch_weighted_sums(s.e_loc.p, s.w.p, n, sums.p, S());
     auto& O = materialised<T>(t);
     O.ensure(size_t(n) * P);
-    cnn_ok<T>(psi, s.confs.p, n, O.p);
+    generic_ok<T>(psi, s.confs.p, n, O.p);
     DevBuf<cplx<double>> coef, sq;
     coef.ensure(n);
     sq.ensure(P);
@@ -1180,7 +1284,7 @@ namespace {
 template <typename T>
 void ok_weighted_sum_t(Psi& psi, Samples& s, const cplx<double>* coef, int conj, cplx<double>* out) {
     const int P = psi.num_params;
-    if (psi.kind == PSI_CNN) {
+    if (!psi.dense()) {
         // materialise O in row chunks of at most 256 MiB, accumulating into out
         const int64_t rows = std::max<int64_t>(1, (int64_t(256) << 20) / (int64_t(sizeof(cplx<T>)) * P));
         DevBuf<cplx<T>> O;
@@ -1190,7 +1294,7 @@ void ok_weighted_sum_t(Psi& psi, Samples& s, const cplx<double>* coef, int conj,
         ANN_HIP_CHECK(hipMemsetAsync(out, 0, sizeof(cplx<double>) * P, S()));
         for (int64_t r0 = 0; r0 < s.count; r0 += rows) {
             const int64_t m = std::min(rows, s.count - r0);
-            cnn_ok<T>(psi, s.confs.p + r0 * s.words, m, O.p);
+            generic_ok<T>(psi, s.confs.p + r0 * s.words, m, O.p);
             launch_ok_colsum<T>(O.p, m, P, coef + r0, conj, 1.0, out, part.p, out, S());
         }
         Runtime::get().sync();
@@ -1222,9 +1326,55 @@ void ok_weighted_sum(Psi& psi, Samples& s, const cplx<double>* coef, int conj, c
         ok_weighted_sum_t<float>(psi, s, coef, conj, out);
 }
 
+double hilbert_space_distance(Psi& psi, Psi& psi_prime, Operator& op, bool unitary, Ensemble& ens, double nu,
+                              cd* grad) {
+    if (psi.num_sites != psi_prime.num_sites) throw std::invalid_argument("psi and psi_prime differ in num_sites");
+    sample(psi, ens);
+    Samples& s = ens.samples;
+    local_energies(psi, op, s);
+    const int64_t n = s.count;
+    // log psi' and (for its O_k) the angles of psi' on the same configurations
+    psi_prime.sync_device();
+    Samples sp;
+    prepare_samples(psi_prime, sp, n);
+    ANN_HIP_CHECK(hipMemcpyAsync(sp.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice,
+                                 S()));
+    log_psi_of(psi_prime, sp.confs.p, n, sp, nullptr);
+    DevBuf<cplx<double>> c_omega, c_ratio;
+    DevBuf<double> sums;
+    c_omega.ensure(n);
+    c_ratio.ensure(n);
+    sums.ensure(4);
+    launch_hsd_terms(s.e_loc.p, s.log_psi.p, sp.log_psi.p, s.w.p, n, unitary, c_omega.p, c_ratio.p, sums.p, S());
+    double h[4];
+    ANN_HIP_CHECK(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    const cd omega(h[0], h[1]);
+    const double u = std::norm(omega), v = h[3] * h[2];
+    const double distance = std::sqrt(std::max(1.0 - u / v, 1e-8));
+    if (grad) {
+        const int P = psi_prime.num_params;
+        DevBuf<cplx<double>> a, b;
+        a.ensure(P);
+        b.ensure(P);
+        ok_weighted_sum(psi_prime, sp, c_omega.p, 1, a.p);   // sum w omega conj(O'_k)
+        ok_weighted_sum(psi_prime, sp, c_ratio.p, 1, b.p);   // sum w ratio conj(O'_k)
+        std::vector<cd> ha(P), hb(P);
+        a.download(reinterpret_cast<cplx<double>*>(ha.data()), P);
+        b.download(reinterpret_cast<cplx<double>*>(hb.data()), P);
+        const double prefactor = std::pow(distance, nu);
+        for (int k = 0; k < P; k++) {
+            const cd uk = std::conj(omega) * ha[k];
+            const cd vk = h[3] * hb[k];
+            grad[k] = -(uk * v - u * vk) / (v * v) / prefactor;
+        }
+    }
+    return distance;
+}
+
 void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     if (uint32_t(psi.num_params) != t.P) throw std::invalid_argument("TDVP num_params does not match the wave function");
-    if (psi.kind == PSI_CNN) {
+    if (!psi.dense()) {
         if (psi.precision == 0)
             tdvp_eval_generic<double>(t, op, psi, ens, with_S);
         else
@@ -1315,8 +1465,10 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         DevBuf<cplx<double>> x;
         x.ensure(P);
         if (!t.has_S) throw std::logic_error("Cholesky solve needs TDVP.eval with the S matrix");
-        DevBuf<cplx<double>> A;
+        // the factor and its workspace persist in the TDVP object (4.5 GB at P = 16,704)
+        DevBuf<cplx<double>>& A = t.chol;
         A.ensure(size_t(P) * P);
+        t.chol_work.ensure(cholesky_work_elems());
         DevBuf<int> info;
         info.ensure(1);
         ANN_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), S()));
@@ -1324,7 +1476,7 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         {
             ScopedTimer timer("cholesky");
             launch_add_diag(A.p, P, shift, S());
-            cholesky_lower(A.p, P, info.p, S());
+            cholesky_lower(A.p, P, info.p, t.chol_work.p, S());
             ANN_HIP_CHECK(hipMemcpyAsync(x.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
             cholesky_solve_inplace(A.p, P, x.p, S());
         }

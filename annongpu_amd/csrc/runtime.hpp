@@ -229,6 +229,12 @@ struct Psi {
     DevBuf<cplx<float>> cp32;
     DevBuf<int> nbr_d;
     DevBuf<uint32_t> sym_d;
+    // multi-layer PsiDeep device state (layers concatenated; offsets per layer)
+    DevBuf<uint32_t> mlp_cn;
+    DevBuf<cplx<double>> mlp_w64, mlp_b64;
+    DevBuf<cplx<float>> mlp_w32, mlp_b32;
+    DevBuf<int> mlp_rb, mlp_ri;
+    std::vector<int> mlp_cn_off, mlp_w_off, mlp_b_off, mlp_rb_off, mlp_ri_off;
 
     bool dense() const { return kind == PSI_RBM || (kind == PSI_DEEP && num_layers == 1); }
     void finalize();           // derived sizes after construction
@@ -237,6 +243,8 @@ struct Psi {
     DenseView<float> view32();
     CnnView<double> cnn64();
     CnnView<float> cnn32();
+    MlpView<double> mlp64();
+    MlpView<float> mlp32();
     std::vector<cd> get_params() const;
     void set_params(const cd* p);
     std::unique_ptr<Psi> clone() const;
@@ -345,6 +353,7 @@ struct Tdvp {
     std::vector<cd> O, F;
     bool has_S = false, has_samples = false;
     DevBuf<cplx<double>> S, Obar, Fd;
+    DevBuf<cplx<double>> chol, chol_work;   // Cholesky factor of S + shift and its panel workspace
     // structured O of the last evaluation (dense nets)
     int precision = 0, words = 0, N = 0, M = 0, Mpad = 0, C = 0, layout = 0, npad = 0;
     int64_t count = 0;
@@ -439,6 +448,12 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
                const cd* x0 = nullptr);
 // O_k of every sample of the last evaluation, uncentred, [count][P] in parameter order (host)
 void tdvp_O_samples(Tdvp& t, cd* out);
+
+// HilbertSpaceDistance.distance / gradient (HilbertSpaceDistance.cu.template:113-165):
+// the distance between psi' and U psi on psi's samples; with grad, also its gradient
+// with respect to psi''s parameters (nu: the power of the distance it is divided by)
+double hilbert_space_distance(Psi& psi, Psi& psi_prime, Operator& op, bool unitary, Ensemble& ens, double nu,
+                              cd* grad);
 
 // proposal training: Adam steps on random (sample, block) rows of the given
 // device configurations; returns the loss of the last step

@@ -88,6 +88,26 @@ __device__ inline cplx<T> lane_sum_realf(const cplx<T> (&th)[U], const cplx<T> (
     return s;
 }
 
+// Sum_j f_j (lc(theta'_j) - lc0_j) over this lane's units (lc0[u][lane] = lc(theta_j), in LDS)
+template <typename T, int U, bool REALF>
+__device__ inline cplx<T> lane_dsum(const cplx<T> (&tn)[U], const cplx<T> (&lc0)[U][64], const cplx<T> (&f)[U],
+                                    int lane) {
+    cplx<T> s = {0, 0};
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const cplx<T> l = logcosh0(tn[u]);
+        const cplx<T> l0 = lc0[u][lane];
+        const cplx<T> d = {l.re - l0.re, l.im - l0.im};
+        if constexpr (REALF) {
+            s.re += f[u].re * d.re;
+            s.im += f[u].re * d.im;
+        } else {
+            s = cadd(s, cmul(f[u], d));
+        }
+    }
+    return s;
+}
+
 // eight lane partials summed over the wave in one transposed reduction: lane l ends
 // with the total of part[(l >> 3) & 7].  Lanes l and l + 32 exchange halves
 // (permlane32 swap), then rows 2r and 2r + 1 (permlane16 swap), then lanes l and l ^ 8

@@ -1,16 +1,14 @@
 """ESS/s of one VMC step (sampling + E_loc + O_k + SR solve) on one MI355X:
-learned block proposal (bf16 inference) vs the plain fp32 exchange sampler.
+learned block proposal (bf16 inference) vs the plain fp32 samplers.
 
 Workload: 2D J1 Heisenberg 10x10 periodic (N=100, 600 Pauli strings), complex
 RBM with biases alpha=4 (PsiDeep, M=400, P=40,500, seeded complex Gaussian
-sigma=0.01, final weights 1), 16,384 persistent chains in the S_z=0 sector,
-angles on MFMA (bf16x3 split, fp32 accumulation), fp32 sweeps / E_loc / O_k,
-fp64 reductions, SR by CG on the matrix-free S (fp64 vectors, from zero) to a
-relative residual of 1e-2 (at most 100 iterations), the W parameters
-preconditioned by Kronecker factors of S (spin covariance x hidden-unit
-covariance, blocks of 64 units) and the biases by its diagonal, then the SR
-update.  (The Jacobi-preconditioned CG, --solver cg, reaches only 3-5 % relative
-residual in the same 100 iterations.)
+sigma=0.01, final weights 1), 16,384 persistent chains, angles on MFMA (bf16x3
+split, fp32 accumulation), fp32 sweeps / E_loc / O_k, fp64 reductions, SR by CG
+on the matrix-free S (fp64 vectors, from zero) to the workload's relative residual
+1e-6 (at most 200 iterations), the W parameters preconditioned by Kronecker
+factors of S (spin covariance x hidden-unit covariance, blocks of 64 units) and the
+biases by its diagonal, then the SR update.
 
 Protocol (all on the GPU, untimed unless stated):
   1. the RBM is pre-optimised by 25 SR steps at lr 0.05 with the plain sampler
@@ -18,22 +16,27 @@ Protocol (all on the GPU, untimed unless stated):
      autocorrelation is that of a structured state (E/N about -1.5), not of the
      near-uniform seed.  The run is kept short on purpose: the reference's layer-0
      activation is the series z^2/2 - z^4/12 + z^6/45, which grows without bound,
-     and longer SR runs reach angles where log psi develops spikes (E_loc variance
-     jumps by orders of magnitude, on some seeds within 30 steps); 25 steps were
-     stable on every seed tried, and so were 10 further steps at the timed rate;
+     and longer SR runs reach angles where log psi develops spikes;
   2. training data: num_chains x 64 samples of the plain sampler; the proposal
      network (100 -> 256 -> 100, tanh; 2x4 / 4x2 plaquette blocks) is trained by
      the library's own forward / backward / Adam kernels;
   3. tau_int (Sokal window c = 5, batch means as a cross-check) of E_loc per sweep
-     on 4096 chains x 400 sweeps after 300 plain thermalisation sweeps (the energy of
-     the pre-optimised state relaxes over ~100 sweeps from a random start), for plain
-     exchange sweeps and for learned block sweeps (the sweep count ceil(2 tau) is a
-     step function of the estimate, so it is taken on enough chains not to flip with
-     the noise);
-  4. timed VMC steps of three paths on the same state:
-       plain-config  : 100 exchange sweeps per sample (the workload's definition)
-       plain-matched : ceil(2 tau_plain) exchange sweeps per sample
-       learned       : ceil(2 tau_learned) block sweeps per sample (headline)
+     on 4096 chains x 400 sweeps after 300 plain thermalisation sweeps, for plain
+     exchange sweeps, plain single-flip sweeps (the full spin space) and learned
+     block sweeps;
+  4. timed VMC steps on the same state:
+       plain-config       : 100 exchange sweeps per sample (the workload's definition)
+       plain-matched      : ceil(2 tau) exchange sweeps per sample
+       plain-flip-matched : ceil(2 tau) single-flip sweeps per sample (full space)
+       learned            : ceil(2 tau) block sweeps per sample (headline)
+       learned-rtol1e-2   : the learned path with the SR solve stopped at 1e-2
+  5. energy check: both samplers on the same state until the standard error of the
+     mean (from independent chain means) is below 0.004, and the fp32 device path
+     against the fp64 device path on the same 4,096 samples;
+  6. dense-S leg (1D Heisenberg ring N=64, M=256, P=16,704, 8,192 chains, 64
+     exchange sweeps): S on fp64 matrix cores and the fp64 blocked Cholesky solve;
+  7. CPU baseline: the fp64 oracle (sampling, E_loc, O_k, F and a matrix-free
+     Jacobi CG to the same tolerance) on a bounded sample, at 1 core and all cores.
 ESS per step = Ns * min(1, S_sw / (2 tau_int)).  Prints one JSON line.
 """
 import argparse
@@ -87,11 +90,12 @@ def build_workload(A, L=10, alpha=4, seed=3, precision="fp32"):
     return N, psi, H
 
 
-def cpu_baseline(A, L, alpha, seed, seconds, sweeps):
-    """Oracle (fp64 C restatement of the reference host path, OpenMP over chains)
-    on a bounded sample of the same workload: chains x `sweeps` exchange sweeps,
-    then E_loc and O_k per sample.  Excludes the SR solve (dense S of 40,500^2
-    does not fit a CPU step), which makes the baseline optimistic."""
+def cpu_baseline(A, L, alpha, seed, seconds, legs, tau_p):
+    """The fp64 oracle (a C restatement of the reference host path, OpenMP over
+    chains) timed on a bounded sample of the same workload: `chains` chains x `sweeps`
+    exchange sweeps, then E_loc, O_k and F per sample and the SR solve (matrix-free
+    Jacobi CG, shift 1e-3, relative residual 1e-6, at most 200 iterations).  legs:
+    [(name, sweeps, threads)]; each is sized to about `seconds` from a calibration."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
     N, psi, H = build_workload(A, L, alpha, seed, "fp64")
@@ -107,23 +111,31 @@ def cpu_baseline(A, L, alpha, seed, seconds, sweeps):
             b[k, w] = (mb >> (64 * w)) & 0xFFFFFFFFFFFFFFFF
         cc[k] = coef
     op = orc.OracleOperator(a, b, cc, words)
-    cores = os.cpu_count() or 1
-    # calibrate on 4 chains per core x 10 sweeps, then size the sample to ~`seconds`
-    t0 = time.perf_counter()
-    orc.mc_run(o, 4 * cores, 1, 10, 0, move=orc.MOVE_EXCHANGE, seed=1, num_threads=cores)
-    per_sweep = (time.perf_counter() - t0) / 40          # `cores` chains in parallel
-    chains = int(seconds / (sweeps * per_sweep * 1.2) * cores)
-    chains = max(cores, min(4096, chains // cores * cores))
-    t0 = time.perf_counter()
-    r = orc.mc_run(o, chains, 1, sweeps, 0, move=orc.MOVE_EXCHANGE, seed=2, num_threads=cores)
-    w = np.full(chains, 1.0 / chains)
-    orc.tdvp_accumulate(op, o, r["samples"], w, with_S=False, num_threads=cores)
-    dt = time.perf_counter() - t0
-    return dict(samples_per_s=chains / dt, seconds=dt, chains=chains, cores=cores)
+
+    def step(chains, sweeps, threads):
+        t0 = time.perf_counter()
+        r = orc.mc_run(o, chains, 1, sweeps, 0, move=orc.MOVE_EXCHANGE, seed=2, num_threads=threads)
+        w = np.full(chains, 1.0 / chains)
+        acc = orc.tdvp_accumulate(op, o, r["samples"], w, with_S=False, num_threads=threads)
+        x, it, res = orc.sr_cg_matfree(acc["O_samples"], w, acc["O"], acc["F"], 1e-3, rtol=1e-6, max_iter=200,
+                                       num_threads=threads)
+        return time.perf_counter() - t0, it
+
+    out = {}
+    for name, sweeps, threads in legs:
+        cal = 4 * threads
+        t, _ = step(cal, sweeps, threads)
+        chains = int(seconds / max(t / cal, 1e-9))
+        chains = max(2 * threads, min(4096, chains // threads * threads))
+        dt, it = step(chains, sweeps, threads)
+        ess = chains * min(1.0, sweeps / (2 * tau_p))
+        out[name] = dict(ess_per_s=ess / dt, samples_per_s=chains / dt, seconds=dt, chains=chains, sweeps=sweeps,
+                         threads=threads, cg_iterations=it)
+    return out
 
 
 KERNELS = ("mc_sweep", "block_sweep", "theta", "local_energy", "o_k", "spin_sums", "sr_pack", "sr_ov", "sr_finish",
-           "spin_gemm", "sr_y", "kfac_setup", "kfac_step")
+           "spin_gemm", "sr_y", "kfac_setup", "kfac_step", "gram", "cholesky")
 
 
 class Stepper:
@@ -177,18 +189,105 @@ class Stepper:
 THERM_SWEEPS = 300   # plain sweeps before a chain's first sample (also for learned paths)
 
 
-def measure_tau(A, psi, op, proposal, chains=4096, T=400, seed=23):
-    # 300 plain thermalisation sweeps: from a random S_z = 0 start the energy of this
-    # state relaxes over ~100 sweeps (a series started after 30 sweeps is a transient).
+def measure_tau(A, psi, op, proposal, chains=4096, T=400, seed=23, move="exchange"):
+    # 300 plain thermalisation sweeps: from a random start the energy of this state
+    # relaxes over ~100 sweeps (a series started after 30 sweeps is a transient).
     # The first pass over the persistent chains thermalises and is discarded; the series
     # and the acceptance rate (which counts every proposal of a pass, thermalisation
     # included, as the reference's does) come from the second pass.
-    ens = A.MonteCarloSpins(chains * T, 1, THERM_SWEEPS, chains, move="exchange", seed=seed, persistent=True)
+    ens = A.MonteCarloSpins(chains * T, 1, THERM_SWEEPS, chains, move=move, seed=seed, persistent=True)
     if proposal is not None:
         ens.set_proposal(proposal, plain_sweeps=0)
     A.log_psi(psi, ens)
     el = A.local_energies(op, psi, ens).real.reshape(T, chains)
     return tau_int_sokal(el), tau_batch_means(el), ens.acceptance_rate, el
+
+
+def energy_run(A, psi, op, proposal, chains, target_se, max_seconds, seed, chunk=40):
+    """E on persistent chains (one plain or block sweep between samples) until the
+    standard error of the mean from the independent chain means is below target_se or
+    max_seconds have passed; returns (E, SE, samples, seconds, last chunk's configurations)"""
+    ens = A.MonteCarloSpins(chains * chunk, 1, THERM_SWEEPS, chains, move="exchange", seed=seed, persistent=True)
+    if proposal is not None:
+        ens.set_proposal(proposal, plain_sweeps=0)
+    sums = np.zeros(chains)
+    n = 0
+    t0 = time.perf_counter()
+    se = float("inf")
+    while True:
+        el = A.local_energies(op, psi, ens).real.reshape(chunk, chains)
+        sums += el.sum(axis=0)
+        n += chunk
+        cm = sums / n
+        se = float(cm.std(ddof=1) / math.sqrt(chains))
+        if se < target_se or time.perf_counter() - t0 > max_seconds:
+            break
+    return float(cm.mean()), se, n * chains, time.perf_counter() - t0, ens.last_samples()[:4096]
+
+
+def fp64_check(A, psi, op, confs):
+    """the fp32 device path against the fp64 device path on the same samples"""
+    out = {}
+    res = {}
+    for prec in ("fp32", "fp64"):
+        q = psi.copy()
+        q.precision = prec
+        t = A.TDVP(q.num_params)
+        t.eval_F(op, q, A.SampleList(q.num_sites, confs))
+        res[prec] = (t.E_local_samples, t.E_local, t.F_vector, t.O_k_vector)
+    e32, e64 = res["fp32"], res["fp64"]
+    out["E_loc_max_rel"] = float((np.abs(e32[0] - e64[0]) / np.abs(e64[0])).max())
+    out["E_rel"] = abs(e32[1] - e64[1]) / abs(e64[1])
+    out["F_rel_norm"] = float(np.linalg.norm(e32[2] - e64[2]) / np.linalg.norm(e64[2]))
+    out["O_rel_norm"] = float(np.linalg.norm(e32[3] - e64[3]) / np.linalg.norm(e64[3]))
+    out["samples"] = int(len(confs))
+    return out
+
+
+def dense_S_leg(A, steps=2, warmup=1):
+    """1D Heisenberg ring N=64, PsiDeep M=256 (P=16,704), 8,192 persistent exchange chains,
+    64 sweeps per sample; S on fp64 matrix cores, fp64 blocked Cholesky of S + 1e-3"""
+    N, M, B = 64, 256, 8192
+    psi = A.seeded_rbm_deep(N, M, seed=2, sigma=0.01, final_weight=1.0)
+    psi.precision = "fp32"
+    op = A.Operator(A.hamiltonians.heisenberg_chain(N))
+    ens = A.MonteCarloSpins(B, 64, 64, B, move="exchange", seed=3, persistent=True)
+    tdvp = A.TDVP(psi.num_params)
+    P = psi.num_params
+    params = psi.params.copy()
+
+    def one():
+        tdvp.eval(op, psi, ens)
+        x = tdvp.solve(1e-3, "cholesky")
+        d = 0.02 * x
+        m = np.abs(d).max()
+        if m > 0.02:
+            d *= 0.02 / m
+        params[:] -= d
+        psi.params = params
+        return tdvp.last_solve["residual"]
+
+    for _ in range(warmup):
+        one()
+    A.profiler_reset()
+    A.start_profiling()
+    t0 = time.perf_counter()
+    res = [one() for _ in range(steps)]
+    A.stop_profiling()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    g_ms, g_n = A.profiler_query("gram")
+    c_ms, c_n = A.profiler_query("cholesky")
+    gram_flop = 4.0 * B * P * P            # upper-triangular half of the Hermitian O^H W O (real flops)
+    chol_flop = 4.0 / 3.0 * P ** 3         # complex Cholesky (real flops); the two triangular solves excluded
+    g_avg, c_avg = g_ms / max(g_n, 1), c_ms / max(c_n, 1)
+    return {"workload": "Heisenberg ring N=64 PBC, PsiDeep M=256 (P=16,704), 8192 persistent exchange chains, "
+                        "64 sweeps per sample, fp32 E_loc/O_k, dense S (fp64 matrix cores) + fp64 Cholesky, "
+                        "shift 1e-3",
+            "ms_per_step": ms, "steps": steps, "solve_residual_max": float(max(res)),
+            "gram": {"ms": g_avg, "algorithmic_flop": gram_flop, "achieved_tflops": gram_flop / g_avg * 1e-9,
+                     "peak_tflops": 78.6, "frac": gram_flop / g_avg * 1e-9 / 78.6, "engine": "fp64 MFMA"},
+            "cholesky": {"ms": c_avg, "algorithmic_flop": chol_flop, "achieved_tflops": chol_flop / c_avg * 1e-9,
+                         "peak_tflops": 78.6, "frac": chol_flop / c_avg * 1e-9 / 78.6, "engine": "fp64 MFMA"}}
 
 
 def main():
@@ -203,14 +302,17 @@ def main():
     ap.add_argument("--opt-steps", type=int, default=25)
     ap.add_argument("--opt-lr", type=float, default=0.05)
     ap.add_argument("--train-steps", type=int, default=1500)
-    ap.add_argument("--cg-iters", type=int, default=100)
+    ap.add_argument("--cg-iters", type=int, default=200)
     ap.add_argument("--solver", default="cg_kfac", choices=("cg_kfac", "cg"))
-    ap.add_argument("--rtol", type=float, default=1e-2)
+    ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--warm-solve", action="store_true",
-                    help="start every CG solve from the previous step's solution (measured slower with K-FAC)")
+                    help="start every CG solve from the previous step's solution")
     ap.add_argument("--lr", type=float, default=0.02)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--energy-se", type=float, default=0.004)
+    ap.add_argument("--energy-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the energy check and the dense-S leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X; run with --gpus 1", file=sys.stderr)
@@ -231,8 +333,7 @@ def main():
     t0 = time.perf_counter()
     psi.precision = "fp64"
     opt_ens = A.MonteCarloSpins(B, 10, 30, B, move="exchange", seed=5, persistent=True)
-    # (the stability of this schedule was established with the Jacobi solve below)
-    opt = Stepper(A, psi, op, opt_ens, args.opt_lr, args.cg_iters, shift=1e-2, solver="cg", rtol=1e-4)
+    opt = Stepper(A, psi, op, opt_ens, args.opt_lr, 100, shift=1e-2, solver="cg", rtol=1e-4)
     for _ in range(args.opt_steps):
         opt.step()
     psi.precision = "fp32"
@@ -258,42 +359,63 @@ def main():
     A.stop_profiling()
     train_ms = A.profiler_query("proposal_train")[0]
     t_train = time.perf_counter() - t0
+    del data
     print(f"proposal trained: loss {loss0:.4f} -> {loss:.4f} ({t_train:.1f} s)", file=sys.stderr, flush=True)
 
     # 3. autocorrelation of E_loc, one (plain or block) sweep apart
-    tau_p, tau_p_bm, acc_p, el_p = measure_tau(A, psi, op, None)
-    tau_l, tau_l_bm, acc_l, el_l = measure_tau(A, psi, op, prop)
-    print(f"tau plain {tau_p:.2f} sweeps, tau learned {tau_l:.2f} block sweeps", file=sys.stderr, flush=True)
+    tau_p, tau_p_bm, acc_p, _ = measure_tau(A, psi, op, None)
+    tau_f, tau_f_bm, acc_f, _ = measure_tau(A, psi, op, None, move="flip")
+    tau_l, tau_l_bm, acc_l, _ = measure_tau(A, psi, op, prop)
+    print(f"tau: plain exchange {tau_p:.2f}, plain single-flip {tau_f:.2f} sweeps, learned {tau_l:.2f} block sweeps",
+          file=sys.stderr, flush=True)
 
     # 4. timed paths
     paths = {}
     s_match = max(1, math.ceil(2 * tau_p))
+    s_flip = max(1, math.ceil(2 * tau_f))
     s_learn = max(1, math.ceil(2 * tau_l))
-    for name, sweeps, proposal, tau in (("plain-config", args.sweeps, None, tau_p),
-                                        ("plain-matched", s_match, None, tau_p),
-                                        ("learned", s_learn, prop, tau_l)):
-        ens = A.MonteCarloSpins(B, sweeps, THERM_SWEEPS, B, move="exchange", seed=17, persistent=True)
+    for name, sweeps, proposal, tau, move, rtol, iters in (
+            ("plain-config", args.sweeps, None, tau_p, "exchange", args.rtol, args.cg_iters),
+            ("plain-matched", s_match, None, tau_p, "exchange", args.rtol, args.cg_iters),
+            ("plain-flip-matched", s_flip, None, tau_f, "flip", args.rtol, args.cg_iters),
+            ("learned", s_learn, prop, tau_l, "exchange", args.rtol, args.cg_iters),
+            ("learned-rtol1e-2", s_learn, prop, tau_l, "exchange", 1e-2, 100)):
+        ens = A.MonteCarloSpins(B, sweeps, THERM_SWEEPS, B, move=move, seed=17, persistent=True)
         if proposal is not None:
             ens.set_proposal(proposal, plain_sweeps=0)
-        st = Stepper(A, psi.copy(), op, ens, args.lr, args.cg_iters, solver=args.solver, rtol=args.rtol,
-                     warm=args.warm_solve)
+        st = Stepper(A, psi.copy(), op, ens, args.lr, iters, solver=args.solver, rtol=rtol, warm=args.warm_solve)
         ms, kms = st.timed(args.steps, args.warmup)
         ess = B * min(1.0, sweeps / (2 * tau))
-        paths[name] = dict(sweeps_per_sample=sweeps, ms_per_step=ms, ess_per_step=ess,
+        paths[name] = dict(sweeps_per_sample=sweeps, move=move if proposal is None else "learned block",
+                           sr_rtol=rtol, sr_max_iter=iters, ms_per_step=ms, ess_per_step=ess,
                            ess_per_s=ess / (ms * 1e-3), tau_int=tau, acceptance=ens.acceptance_rate,
                            E_per_site=st.tdvp.E_local.real / N, cg_iterations_mean=float(np.mean(st.iters)),
                            cg_residual_mean=float(np.mean(st.residuals)),
+                           cg_residual_max=float(np.max(st.residuals)),
                            kernel_ms_total={k: v[0] for k, v in kms.items()},
                            kernel_launches={k: v[1] for k, v in kms.items()})
-        print(f"{name}: {ms:.2f} ms/step, ESS/s {paths[name]['ess_per_s']:.0f}", file=sys.stderr, flush=True)
+        if proposal is not None:
+            paths[name]["block_stats_last_step"] = ens.block_stats
+        print(f"{name}: {ms:.2f} ms/step, ESS/s {paths[name]['ess_per_s']:.0f}, CG {np.mean(st.iters):.0f} it "
+              f"to {np.mean(st.residuals):.1e}", file=sys.stderr, flush=True)
     head = paths["learned"]
 
-    # energies of the two samplers on the same state (samples of step 3)
-    def mean_se(el):
-        cm = el.mean(axis=0)
-        return float(cm.mean()), float(cm.std(ddof=1) / math.sqrt(el.shape[1]))
-    Ep, sep = mean_se(el_p)
-    El, sel = mean_se(el_l)
+    # 5. energies of the two samplers on the same state, and fp32 against fp64
+    energy = None
+    dense = None
+    if not args.no_extra:
+        Ep, sep, n_p, t_p, confs = energy_run(A, psi, op, None, B, args.energy_se, args.energy_seconds, 41)
+        El, sel, n_l, t_l, _ = energy_run(A, psi, op, prop, B, args.energy_se, args.energy_seconds, 43)
+        energy = {"E_plain": Ep, "se_plain": sep, "samples_plain": n_p, "E_learned": El, "se_learned": sel,
+                  "samples_learned": n_l, "rel_diff": abs(El - Ep) / abs(Ep),
+                  "rel_se_combined": math.hypot(sep, sel) / abs(Ep),
+                  "fp32_vs_fp64_same_samples": fp64_check(A, psi, op, confs)}
+        print(f"energy: plain {Ep:.4f} +- {sep:.4f}, learned {El:.4f} +- {sel:.4f}", file=sys.stderr, flush=True)
+        # 6. dense S and Cholesky
+        dense = dense_S_leg(A)
+        print(f"dense S leg: {dense['ms_per_step']:.1f} ms/step, gram {dense['gram']['ms']:.1f} ms "
+              f"({dense['gram']['achieved_tflops']:.1f} TFLOP/s), cholesky {dense['cholesky']['ms']:.1f} ms "
+              f"({dense['cholesky']['achieved_tflops']:.1f} TFLOP/s)", file=sys.stderr, flush=True)
 
     # roofline of the headline path's dominant single-kernel launch class (each timer
     # below brackets exactly one kernel launch with hipEvents on the library stream;
@@ -317,20 +439,22 @@ def main():
             achieved = work / (avg_ms * 1e-3) / 1e9
         else:
             if name == "block_sweep":
+                # per proposal: second layer 2 bs H; per proposal that is not a redraw of the
+                # current pattern: log-cosh sum 26 M; per flipped row entering a ratio: 4 M
+                # (device counters of the last timed step, scaled to one launch)
                 bs = blocks.shape[1]
-                moves = paths[path]["sweeps_per_sample"] * math.ceil(N / bs)
-                # per block move: second layer 2 bs H, angle update of <= bs rows 4 bs M, log-cosh sum 26 M
-                work = B * moves * (2.0 * bs * 256 + 4.0 * bs * M + 26.0 * M)
+                st = paths[path]["block_stats_last_step"]
+                work = st["proposals"] * 2.0 * bs * 256 + (st["proposals"] - st["null_moves"]) * 26.0 * M \
+                    + st["rows_evaluated"] * 4.0 * M
             elif name == "mc_sweep":
                 work = B * paths[path]["sweeps_per_sample"] * N * 36.0 * M
             else:   # local_energy: 36 M per distinct off-diagonal configuration (~130 on the 10x10 AFM)
                 work = B * 130 * 36.0 * M
-            r = {"bound": "mfma", "peak": 157.3, "unit": "TFLOP/s",
-                 "engine": "fp32 VALU (MI355X vector fp32 peak 157.3 TFLOP/s = fp32 matrix peak)"}
+            r = {"bound": "valu", "peak": 157.3, "unit": "TFLOP/s", "engine": "fp32 VALU (MI355X vector fp32 peak)"}
             achieved = work / (avg_ms * 1e-3) / 1e12
         traffic = measured_traffic(name) if r["bound"] == "hbm" else None
         return dict(r, achieved=achieved, frac=achieved / r["peak"], traffic=traffic, kernel=name,
-                    avg_launch_ms=avg_ms, launches=launches, algorithmic_bytes=work)
+                    avg_launch_ms=avg_ms, launches=launches, algorithmic_work_per_launch=work)
 
     def measured_traffic(name):
         """HBM bytes per launch of the kernel behind timer `name` on this workload, from the
@@ -349,16 +473,22 @@ def main():
     candidates = ("block_sweep", "sr_ov", "spin_gemm", "local_energy", "mc_sweep")
     dom = max(candidates, key=lambda k: head["kernel_ms_total"][k])
     roof = roofline("learned", dom)
+    extra_roofs = {k: roofline("learned", k) for k in candidates if k != dom and head["kernel_launches"][k] > 0}
     roof_plain = roofline("plain-config", "mc_sweep")
 
     cpu = None
     if not args.no_cpu:
-        c = cpu_baseline(A, args.L, args.alpha, 3, args.cpu_seconds, args.sweeps)
-        ess_frac = min(1.0, args.sweeps / (2 * tau_p))
-        cpu = {"value": c["samples_per_s"] * ess_frac, "unit": "ESS/s", "cores": c["cores"], "kind": "port",
-               "sample": f"{c['chains']} chains x {args.sweeps} exchange sweeps + E_loc + O_k in fp64 "
-                         f"({c['seconds']:.1f} s, OpenMP over chains) on the seeded RBM; SR solve excluded; "
-                         f"ESS fraction of the plain GPU path applied"}
+        cores = min(16, os.cpu_count() or 1)
+        legs = [("all_cores_matched", s_match, cores), ("one_core_matched", s_match, 1),
+                ("all_cores_config", args.sweeps, cores)]
+        c = cpu_baseline(A, args.L, args.alpha, 3, args.cpu_seconds, legs, tau_p)
+        m = c["all_cores_matched"]
+        cpu = {"value": m["ess_per_s"], "unit": "ESS/s", "cores": cores, "kind": "port",
+               "sample": f"{m['chains']} chains x {s_match} exchange sweeps + E_loc + O_k + F in fp64 + matrix-free "
+                         f"Jacobi CG (shift 1e-3, rtol 1e-6, <= 200 it: {m['cg_iterations']}) on the seeded RBM "
+                         f"({m['seconds']:.1f} s, OpenMP over chains, {cores} threads); ESS fraction from the "
+                         f"plain GPU tau",
+               "legs": c}
 
     line = {
         "metric": "ESS/s with E_loc+O_k+SR, RBM N=100 alpha=4: learned-proposal bf16 vs plain fp32",
@@ -367,21 +497,25 @@ def main():
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"Heisenberg {args.L}x{args.L} PBC, complex RBM alpha={args.alpha} "
                                f"(P={psi.num_params}), {B} persistent chains, S_z=0; learned 2x4 block proposal "
-                               f"(MLP {N}-256-{N}, bf16 inference) vs plain exchange sweeps; fp32 sweeps/E_loc/O_k, "
-                               f"bf16x3 angle GEMM, {args.solver} SR to rtol {args.rtol} (fp64)",
-                   "learned_vs_plain_config": head["ess_per_s"] / paths["plain-config"]["ess_per_s"],
+                               f"(MLP {N}-256-{N}, bf16 inference); fp32 sweeps/E_loc/O_k, bf16x3 angle GEMM, "
+                               f"{args.solver} SR to rtol {args.rtol}, <= {args.cg_iters} iterations (fp64)",
                    "learned_vs_plain_matched": head["ess_per_s"] / paths["plain-matched"]["ess_per_s"],
+                   "learned_vs_plain_flip_matched": head["ess_per_s"] / paths["plain-flip-matched"]["ess_per_s"],
+                   "learned_vs_plain_config": head["ess_per_s"] / paths["plain-config"]["ess_per_s"],
                    "paths": paths,
                    "pre_optimisation": {"sr_steps": args.opt_steps, "lr": args.opt_lr, "E_per_site": E_opt,
                                         "var_per_site": var_opt, "seconds": t_opt},
                    "training": {"samples": B * 64, "adam_steps": args.train_steps, "batch": 4096,
                                 "loss_before": loss0, "loss_after": loss, "device_ms": train_ms,
                                 "seconds": t_train},
-                   "tau": {"plain_sweeps": tau_p, "plain_batch_means": tau_p_bm,
+                   "tau": {"plain_exchange_sweeps": tau_p, "plain_exchange_batch_means": tau_p_bm,
+                           "plain_flip_sweeps": tau_f, "plain_flip_batch_means": tau_f_bm,
                            "learned_block_sweeps": tau_l, "learned_batch_means": tau_l_bm,
-                           "acceptance_plain": acc_p, "acceptance_learned": acc_l},
-                   "energy_check": {"E_plain": Ep, "se_plain": sep, "E_learned": El, "se_learned": sel,
-                                    "rel_diff": abs(El - Ep) / abs(Ep)},
+                           "acceptance_plain_exchange": acc_p, "acceptance_plain_flip": acc_f,
+                           "acceptance_learned": acc_l},
+                   "energy_check": energy,
+                   "dense_S": dense,
+                   "roofline_other_kernels": extra_roofs,
                    "roofline_plain_sweep": roof_plain},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -156,6 +156,14 @@ int ann_expectation_gradient(ann_operator op, ann_psi psi, ann_ensemble ens, dou
 int ann_local_energies(ann_operator op, ann_psi psi, ann_ensemble ens, double* e_loc, int64_t capacity);
 
 /* ------------------------------------------------------------------- TDVP */
+/* HilbertSpaceDistance(num_params, gpu) __call__ / gradient (MAIN:436-441;
+ * HilbertSpaceDistance.cu.template:113-165): the distance between psi_prime and U psi
+ * (U = op, unitary or, with is_unitary = 0, the generator exp(E_loc)) on psi's samples;
+ * gradient (psi_prime's num_params, interleaved complex; may be null) with respect to
+ * psi_prime's parameters, divided by distance^nu */
+int ann_hilbert_space_distance(ann_psi psi, ann_psi psi_prime, ann_operator op, int32_t is_unitary, ann_ensemble ens,
+                               double nu, double* gradient, double* distance);
+
 /* TDVP(num_params, gpu) -- network_functions/TDVP.hpp:16-101, MAIN:465-489 */
 int ann_tdvp_create(ann_tdvp* out, uint32_t num_params);
 int ann_tdvp_destroy(ann_tdvp tdvp);

@@ -322,6 +322,23 @@ def cg_solve(S, shift, F, rtol=1e-10, max_iter=1000):
     return x, it.value, rc
 
 
+def sr_cg_matfree(O_samples, weights, O_mean, F, shift, rtol=1e-6, max_iter=200, num_threads=0):
+    """(S + shift) x = F by Jacobi CG on the matrix-free S = sum_s w_s conj(Oc_s) Oc_s^T"""
+    O_samples = np.ascontiguousarray(O_samples, dtype=np.complex128)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    O_mean = np.ascontiguousarray(O_mean, dtype=np.complex128)
+    F = np.ascontiguousarray(F, dtype=np.complex128)
+    ns, Pn = O_samples.shape
+    x = np.zeros(Pn, dtype=np.complex128)
+    res = ctypes.c_double(0.0)
+    f = lib().orc_sr_cg_matfree
+    f.restype = ctypes.c_int32
+    it = f(ctypes.c_int64(ns), ctypes.c_int32(Pn), _ptr(O_samples), _ptr(w), _ptr(O_mean), _ptr(F),
+           ctypes.c_double(shift), ctypes.c_double(rtol), ctypes.c_int32(max_iter), _ptr(x), ctypes.byref(res),
+           ctypes.c_int32(num_threads))
+    return x, int(it), res.value
+
+
 # ------------------------------------------------------------ learned proposal
 # (oracle/proposal_oracle.c: no counterpart in the reference)
 

@@ -694,3 +694,90 @@ int orc_cg_solve(int32_t n, const cd* S, double shift, const cd* F, cd* x,
     free(r); free(p); free(Ap);
     return sqrt(rr) <= rtol * bnorm ? 0 : 1;
 }
+
+/* (S + shift) x = F by Jacobi-preconditioned CG with the matrix-free product
+ * S v = sum_s w_s conj(Oc_s) (Oc_s . v), Oc_s = O_s - <O>, on the stored O_k samples
+ * [ns][P]: the product a caller of the reference's TDVP.S_dot_vector
+ * (source/network_functions/TDVP.cu.template:336-443) iterates on, restated on the
+ * CPU for the baseline and for checks of the device solve.  Returns the iterations;
+ * *res gets the final relative residual of the recursion. */
+static void sv_matfree(int64_t ns, int P, const cd* O, const double* w, const cd* Ob, double shift,
+                       const cd* v, cd* u, cd* y) {
+    cd ov = 0.0;
+    for (int k = 0; k < P; k++) ov += Ob[k] * v[k];
+    #pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < ns; s++) {
+        const cd* row = O + (size_t)s * P;
+        cd acc = 0.0;
+        for (int k = 0; k < P; k++) acc += row[k] * v[k];
+        u[s] = w[s] * (acc - ov);
+    }
+    cd su = 0.0;
+    for (int64_t s = 0; s < ns; s++) su += u[s];
+    #pragma omp parallel
+    {
+        const int nt = omp_get_num_threads(), id = omp_get_thread_num();
+        const int k0 = (int)((int64_t)P * id / nt), k1 = (int)((int64_t)P * (id + 1) / nt);
+        for (int k = k0; k < k1; k++) y[k] = shift * v[k] - conj(Ob[k]) * su;
+        for (int64_t s = 0; s < ns; s++) {
+            const cd* row = O + (size_t)s * P;
+            const cd us = u[s];
+            for (int k = k0; k < k1; k++) y[k] += conj(row[k]) * us;
+        }
+    }
+}
+
+int orc_sr_cg_matfree(int64_t ns, int32_t P, const cd* O_samples, const double* w, const cd* O_mean,
+                      const cd* F, double shift, double rtol, int32_t max_iter, cd* x, double* res,
+                      int32_t num_threads) {
+#ifdef _OPENMP
+    if (num_threads > 0) omp_set_num_threads(num_threads);
+#endif
+    cd* r = (cd*)malloc(sizeof(cd) * P);
+    cd* z = (cd*)malloc(sizeof(cd) * P);
+    cd* p = (cd*)malloc(sizeof(cd) * P);
+    cd* Ap = (cd*)malloc(sizeof(cd) * P);
+    cd* u = (cd*)malloc(sizeof(cd) * (size_t)ns);
+    double* dinv = (double*)malloc(sizeof(double) * P);
+    /* Jacobi: diagonal of S + shift */
+    for (int k = 0; k < P; k++) dinv[k] = shift;
+    for (int64_t s = 0; s < ns; s++)
+        for (int k = 0; k < P; k++) {
+            const cd d = O_samples[(size_t)s * P + k] - O_mean[k];
+            dinv[k] += w[s] * (creal(d) * creal(d) + cimag(d) * cimag(d));
+        }
+    double fn = 0.0, rz = 0.0, rr = 0.0;
+    for (int k = 0; k < P; k++) {
+        dinv[k] = 1.0 / dinv[k];
+        x[k] = 0.0;
+        r[k] = F[k];
+        z[k] = dinv[k] * r[k];
+        p[k] = z[k];
+        fn += creal(F[k] * conj(F[k]));
+        rz += creal(conj(r[k]) * z[k]);
+    }
+    rr = fn;
+    int it = 0;
+    while (it < max_iter && sqrt(rr) > rtol * sqrt(fn)) {
+        sv_matfree(ns, P, O_samples, w, O_mean, shift, p, u, Ap);
+        cd pAp = 0.0;
+        for (int k = 0; k < P; k++) pAp += conj(p[k]) * Ap[k];
+        const double alpha = rz / creal(pAp);
+        double rz_new = 0.0;
+        rr = 0.0;
+        for (int k = 0; k < P; k++) {
+            x[k] += alpha * p[k];
+            r[k] -= alpha * Ap[k];
+            z[k] = dinv[k] * r[k];
+            rz_new += creal(conj(r[k]) * z[k]);
+            rr += creal(r[k] * conj(r[k]));
+        }
+        const double beta = rz_new / rz;
+        for (int k = 0; k < P; k++) p[k] = z[k] + beta * p[k];
+        rz = rz_new;
+        it++;
+    }
+    if (res) *res = sqrt(rr / fn);
+    free(r); free(z); free(p); free(Ap); free(u); free(dinv);
+    return it;
+}

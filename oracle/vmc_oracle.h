@@ -120,6 +120,10 @@ int orc_cholesky_solve(int32_t n, const double complex* S, double shift,
                        const double complex* F, double complex* x);
 int orc_cg_solve(int32_t n, const double complex* S, double shift, const double complex* F,
                  double complex* x, double rtol, int32_t max_iter, int32_t* iterations);
+/* (S + shift) x = F by Jacobi CG on the matrix-free S of stored O_k samples [ns][P] */
+int orc_sr_cg_matfree(int64_t ns, int32_t P, const double complex* O_samples, const double* w,
+                      const double complex* O_mean, const double complex* F, double shift, double rtol,
+                      int32_t max_iter, double complex* x, double* res, int32_t num_threads);
 
 /* learned block proposal (proposal_oracle.c; no counterpart in the reference) */
 void orc_mlp_logits(int32_t N, int32_t H, const double* params, const uint64_t* conf,

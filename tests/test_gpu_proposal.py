@@ -127,7 +127,7 @@ def test_plaquette_block_sampler_preserves_psi_squared(precision, scale):
     network as initialised, scaled to be strongly non-uniform, and trained on plain
     samples; redraws of the current pattern are counted apart from accepted moves"""
     L, N = 4, 16
-    psi = A.seeded_rbm_deep(N, 32, 21, sigma=0.15)
+    psi = A.seeded_rbm_deep(N, 32, 21, sigma=0.1)   # participation ratio ~1,500 of 12,870 states
     psi.precision = precision
     o = orc.deep(N, psi.a, psi.b, psi.connections, psi.W, psi.final_weights, 0.0)
     H = ham.heisenberg_square(L)

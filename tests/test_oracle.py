@@ -410,3 +410,24 @@ def test_mc_counter_continuation():
     r_a = orc.mc_run(psi, 8, 1, 3, 0, seed=5)
     r_b = orc.mc_run(psi, 8, 1, 3, 0, seed=5, counter=3 * N, chain_confs=r_a["chains"])
     assert np.array_equal(r1["samples"][8:], r_b["samples"])
+
+
+def test_matrix_free_sr_cg_matches_dense_solve():
+    """the CPU baseline's matrix-free SR solve equals numpy's dense solve of
+    (Oc^H diag(w) Oc + shift) x = F, Oc = O - <O>"""
+    rng = np.random.default_rng(3)
+    ns, P = 60, 40
+    O = rng.standard_normal((ns, P)) + 1j * rng.standard_normal((ns, P))
+    w = rng.random(ns)
+    w /= w.sum()
+    Ob = w @ O
+    F = rng.standard_normal(P) + 1j * rng.standard_normal(P)
+    Oc = O - Ob
+    S = (Oc.conj().T * w) @ Oc
+    x_ref = np.linalg.solve(S + 0.05 * np.eye(P), F)
+    x, it, res = orc.sr_cg_matfree(O, w, Ob, F, 0.05, rtol=1e-12, max_iter=500)
+    assert res <= 1e-12 and it <= 500
+    assert np.linalg.norm(x - x_ref) < 1e-9 * np.linalg.norm(x_ref)
+    # a dropped centring term would be caught: the uncentred system has another solution
+    S_unc = (O.conj().T * w) @ O
+    assert np.linalg.norm(np.linalg.solve(S_unc + 0.05 * np.eye(P), F) - x_ref) > 1e-3 * np.linalg.norm(x_ref)
