@@ -88,6 +88,7 @@ _SIGNATURES = {
     "ann_ensemble_block_stats": [P, P, P, P],
     "ann_tdvp_get_samples": [P, P, P, P, i64, P, P, P],
     "ann_tdvp_get_O_samples": [P, P, i64],
+    "ann_local_energies_noalloc": [P, P, P, i64, P],
     "ann_hilbert_space_distance": [P, P, P, i32, P, f64, P, P],
 }
 

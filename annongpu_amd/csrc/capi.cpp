@@ -9,6 +9,7 @@
 #include <cmath>
 #include <random>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -671,6 +672,80 @@ int ann_local_energies(ann_operator op, ann_psi psi, ann_ensemble ens, double* e
         ann::local_energies(psi->impl, op->impl, s);
         ann::Runtime::get().sync();
         s.e_loc.download(reinterpret_cast<ann::cplx<double>*>(e_loc), s.count);
+    });
+}
+
+// ---------------------------------------------- local energies without allocation
+
+namespace {
+
+// Angles of the configurations of ann_local_energies_noalloc.  Each GPU the library
+// runs on holds its own copy of this array until the process ends; host code reaches
+// it only through the address hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current device, looked up once per device
+void* scratch_address() {
+    int dev = -1;
+    ANN_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw ann::HipError("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw ann::HipError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+int ann_local_energies_noalloc(ann_operator op, ann_psi psi, const uint64_t* confs, int64_t count, double* e_loc) {
+    return guard([&] {
+        need(op, "op");
+        need(psi, "psi");
+        need(confs, "confs");
+        need(e_loc, "e_loc");
+        auto& q = psi->impl;
+        if (!q.dense()) throw std::logic_error("ann_local_energies_noalloc needs a PsiRBM or one-layer PsiDeep");
+        if (count <= 0) throw std::invalid_argument("count must be positive");
+        const size_t per = size_t(q.Mpad) * (q.precision == 0 ? sizeof(ann::cplx<double>) : sizeof(ann::cplx<float>));
+        if (size_t(count) * per > kNoAllocScratchBytes)
+            throw std::invalid_argument("ann_local_energies_noalloc: count * hidden units exceeds its " +
+                                        std::to_string(kNoAllocScratchBytes >> 20) + " MiB scratch (at most " +
+                                        std::to_string(kNoAllocScratchBytes / per) + " configurations here)");
+        // the parameters and the operator tables must already be on the device: putting
+        // them there allocates, which the ordinary entry points do on first use
+        if (q.dirty || (q.params_dirty && q.params_d.n < size_t(q.num_params)))
+            throw std::logic_error("the wave function's parameters are not on the device yet: make one ordinary "
+                                   "call with it (e.g. ann_log_psi_s) first");
+        if (!op->impl.compiled.count(q.words))
+            throw std::logic_error("the operator's device tables are not built yet: make one ordinary local "
+                                   "energy call with it first");
+        if (op->impl.max_site() >= q.num_sites) throw std::invalid_argument("operator acts on sites beyond num_sites");
+        void* th = scratch_address();
+        const ann::DeviceOp& d = *op->impl.compiled.at(q.words);
+        hipStream_t st = ann::lib_stream();
+        auto* out = reinterpret_cast<ann::cplx<double>*>(e_loc);
+        if (q.precision == 0) {
+            auto v = q.view64();
+            auto* t = static_cast<ann::cplx<double>*>(th);
+            ann::launch_theta_valu<double>(v, confs, q.words, count, t, st);
+            ann::launch_local_energy<double>(v, d.view(), confs, t, count, out, st);
+        } else {
+            auto v = q.view32();
+            auto* t = static_cast<ann::cplx<float>*>(th);
+            ann::launch_theta_mfma(v, q.wfrag.p, q.npad, confs, q.words, count, t, st);
+            ann::launch_local_energy<float>(v, d.view(), confs, t, count, out, st);
+        }
+        ANN_HIP_CHECK(hipStreamSynchronize(st));
     });
 }
 

@@ -154,6 +154,16 @@ int ann_expectation_fluctuation(ann_operator op, ann_psi psi, ann_ensemble ens, 
 int ann_expectation_gradient(ann_operator op, ann_psi psi, ann_ensemble ens, double* gradient, double* value2);
 /* additive: per-sample local energies of the last sampling, [num_steps] */
 int ann_local_energies(ann_operator op, ann_psi psi, ann_ensemble ens, double* e_loc, int64_t capacity);
+/* No counterpart in the reference: for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  E_loc of `count` configurations given
+ * in DEVICE memory, confs[count][words], into DEVICE memory e_loc[count] (interleaved
+ * complex), for a PsiRBM / one-layer PsiDeep.  The angles live in a static 16 MiB device
+ * array: count * padded hidden units * 8 B (fp32 mode; 16 B in fp64 mode) must fit it, else
+ * ANN_E_INVALID.  The net's parameters and the operator's device tables must be on the
+ * device already (one ordinary call with each puts them there), else ANN_E_UNSUPPORTED.
+ * Synchronous on the library stream.  Calls that overlap on one GPU share that array
+ * and race. */
+int ann_local_energies_noalloc(ann_operator op, ann_psi psi, const uint64_t* confs, int64_t count, double* e_loc);
 
 /* ------------------------------------------------------------------- TDVP */
 /* HilbertSpaceDistance(num_params, gpu) __call__ / gradient (MAIN:436-441;
