@@ -84,6 +84,7 @@ _SIGNATURES = {
     "ann_proposal_adam_step": [P, P, f64],
     "ann_proposal_train": [P, P, i32, i32, f64, P],
     "ann_monte_carlo_set_proposal": [P, P, u32],
+    "ann_monte_carlo_set_proposal_mode": [P, i32],
     "ann_sample_list_create": [P, u32, P, i64, i32, P],
     "ann_ensemble_block_stats": [P, P, P, P],
     "ann_tdvp_get_samples": [P, P, P, P, i64, P, P, P],

@@ -556,11 +556,15 @@ class MonteCarloSpins(_Ensemble):
         check(lib.ann_ensemble_block_stats(self._h, ref(p), ref(n), ref(r)))
         return dict(proposals=p.value, null_moves=n.value, rows_evaluated=r.value)
 
-    def set_proposal(self, proposal, plain_sweeps=1):
+    def set_proposal(self, proposal, plain_sweeps=1, exclude_current=False):
         """Attach a trained BlockProposal (None detaches).  Each sample interval then
-        holds num_sweeps block sweeps plus ``plain_sweeps`` plain sweeps."""
+        holds num_sweeps block sweeps plus ``plain_sweeps`` plain sweeps.  With
+        ``exclude_current`` a block move never redraws the current pattern (Metropolized
+        Gibbs form of the proposal, exact under the corrected Metropolis-Hastings ratio)."""
         check(lib.ann_monte_carlo_set_proposal(self._h, proposal._h if proposal is not None else None,
                                                int(plain_sweeps)))
+        check(lib.ann_monte_carlo_set_proposal_mode(self._h, 1 if exclude_current else 0))
+        self.exclude_current = bool(exclude_current)
         self._proposal = proposal      # keeps the network alive while attached
         self.plain_sweeps = int(plain_sweeps)
 

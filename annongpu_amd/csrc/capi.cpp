@@ -1016,4 +1016,12 @@ int ann_monte_carlo_set_proposal(ann_ensemble ens, ann_proposal p, uint32_t plai
     });
 }
 
+int ann_monte_carlo_set_proposal_mode(ann_ensemble ens, int32_t exclude_current) {
+    return guard([&] {
+        need(ens, "ens");
+        if (ens->impl.kind != ann::ENS_MC) throw std::invalid_argument("proposals apply to Monte Carlo ensembles");
+        ens->impl.proposal_exclusive = exclude_current != 0;
+    });
+}
+
 }  // extern "C"

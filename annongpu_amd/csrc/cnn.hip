@@ -321,6 +321,316 @@ __global__ __launch_bounds__(64) void cnn_ok_kernel(CnnView<T> net, const uint64
     }
 }
 
+// ------------------------------------------------ PsiCNN local updates
+// A flip changes the layer-0 pre-activations only at the consumers of the flipped
+// sites (their receptive field), the layer-1 ones only at the consumers of those, and
+// so on; a proposal recomputes exactly those entries from the new inputs (no running
+// sums, so nothing drifts) and the log psi difference is the sum over the last layer's
+// affected sites of the new minus the old per-site output.  State per wavefront (LDS):
+//   A    activations of layers 0 .. L-2 at the current configuration ([C_l][N] each)
+//   Tn   the same for the proposal (valid on the affected sites)
+//   V    last-layer activations of the proposal ([C_{L-1}][N], affected sites)
+//   out  per-site output sum_c act(z_{L-1}[c][x]) at the current configuration, nout
+//        the same for the proposal, lst the affected sites of the layer at hand.
+template <typename T>
+struct CnnInc {
+    cplx<T>* A;
+    cplx<T>* Tn;
+    cplx<T>* V;
+    cplx<T>* out;
+    cplx<T>* nout;
+    int* lst;
+    int aoff[kMaxCnnLayers];
+};
+
+template <typename T>
+__device__ inline CnnInc<T> cnn_inc_layout(const CnnView<T>& net, unsigned char* smem) {
+    CnnInc<T> st;
+    int a = 0;
+    for (int l = 0; l + 1 < net.L; l++) {
+        st.aoff[l] = a;
+        a += net.ch[l] * net.N;
+    }
+    st.A = reinterpret_cast<cplx<T>*>(smem);
+    st.Tn = st.A + a;
+    st.V = st.Tn + a;
+    st.out = st.V + net.ch[net.L - 1] * net.N;
+    st.nout = st.out + net.N;
+    st.lst = reinterpret_cast<int*>(st.nout + net.N);
+    return st;
+}
+
+template <typename T>
+size_t cnn_inc_lds(const CnnView<T>& net) {
+    size_t a = 0;
+    for (int l = 0; l + 1 < net.L; l++) a += size_t(net.ch[l]) * net.N;
+    return sizeof(cplx<T>) * (2 * a + size_t(net.ch[net.L - 1]) * net.N + 2 * size_t(net.N)) + sizeof(int) * net.N;
+}
+
+__device__ inline bool mask_bit(const uint64_t (&m)[kMaxWords], int y) {
+    const uint64_t word = y < 64 ? m[0] : y < 128 ? m[1] : y < 192 ? m[2] : m[3];
+    return (word >> (y & 63)) & 1u;
+}
+
+// the sites of mask m as a list in LDS (in increasing order); returns their number
+__device__ inline int mask_list(const uint64_t (&m)[kMaxWords], int N, int* lst, int lane) {
+    int n = 0;
+#pragma unroll
+    for (int u = 0; u < kMaxWords; u++) {
+        const int x = lane + 64 * u;
+        const bool in = x < N && mask_bit(m, x);
+        const uint64_t b = __ballot(in);
+        const int idx = n + int(__builtin_amdgcn_mbcnt_hi(uint32_t(b >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(b), 0u)));
+        if (in) lst[idx] = x;
+        n += __popcll(b);
+    }
+    __syncthreads();
+    return n;
+}
+
+// full forward at configuration w into the state; returns sum_x out[x] (the network
+// part of log psi before the final factor)
+template <typename T>
+__device__ cplx<T> cnn_inc_init(const CnnView<T>& net, const uint64_t* w, CnnInc<T>& st, int lane) {
+    const int N = net.N;
+    for (int l = 0; l < net.L; l++) {
+        const int prev = l > 0 ? net.ch[l - 1] : 1;
+        const int vol = net.vol[l];
+        const int C = net.ch[l];
+        const bool last = l == net.L - 1;
+        for (int x = lane; x < N; x += 64) {
+            const int cls = int(net.sym[x]);
+            const int* nb = net.nbr + net.nbr_off[l] + int64_t(x) * vol;
+            cplx<T> osum = {0, 0};
+            for (int c = 0; c < C; c++) {
+                cplx<T> z = {0, 0};
+                for (int ci = 0; ci < prev; ci++) {
+                    const cplx<T>* wk = net.params + net.link_off[l] + (int64_t(ci) * C + c) * net.nsym * vol +
+                                        int64_t(cls) * vol;
+                    for (int k = 0; k < vol; k++) {
+                        const cplx<T> a = l == 0 ? layer_input<T>(0, w, st.A, N, 0, nb[k])
+                                                 : st.A[st.aoff[l - 1] + ci * N + nb[k]];
+                        const cplx<T> q = wk[k];
+                        z.re += q.re * a.re - q.im * a.im;
+                        z.im += q.re * a.im + q.im * a.re;
+                    }
+                }
+                const cplx<T> av = act(z, l);
+                if (last)
+                    osum = cadd(osum, av);
+                else
+                    st.A[st.aoff[l] + c * N + x] = av;
+            }
+            if (last) st.out[x] = osum;
+        }
+        __syncthreads();
+    }
+    cplx<T> part = {0, 0};
+    for (int x = lane; x < N; x += 64) part = cadd(part, st.out[x]);
+    return {wave_sum(part.re), wave_sum(part.im)};
+}
+
+// the change of sum_x out[x] when the spins of mask F flip; R[l + 1] receives the
+// sites whose layer-l pre-activations change (for a commit)
+template <typename T>
+__device__ cplx<T> cnn_inc_delta(const CnnView<T>& net, const uint64_t* w, const uint64_t (&F)[kMaxWords],
+                                 CnnInc<T>& st, uint64_t (&R)[kMaxCnnLayers + 1][kMaxWords], int lane) {
+    const int N = net.N;
+    uint64_t wn[kMaxWords];
+#pragma unroll
+    for (int k = 0; k < kMaxWords; k++) {
+        wn[k] = w[k] ^ F[k];
+        R[0][k] = F[k];
+    }
+    int n = 0;
+    for (int l = 0; l < net.L; l++) {
+        // dilate: the consumers of the sites changed in the layer below
+#pragma unroll
+        for (int k = 0; k < kMaxWords; k++) R[l + 1][k] = 0;
+        for (int k = 0; k < kMaxWords; k++) {
+            uint64_t m = R[l][k];
+            while (m) {
+                const int y = 64 * k + int(__builtin_ctzll(m));
+                m &= m - 1;
+                const uint64_t* cy = net.cons + (int64_t(l) * N + y) * kMaxWords;
+#pragma unroll
+                for (int q = 0; q < kMaxWords; q++) R[l + 1][q] |= cy[q];
+            }
+        }
+        n = mask_list(R[l + 1], N, st.lst, lane);
+        const int prev = l > 0 ? net.ch[l - 1] : 1;
+        const int vol = net.vol[l];
+        const int C = net.ch[l];
+        const bool last = l == net.L - 1;
+        for (int e = lane; e < C * n; e += 64) {
+            const int c = e / n, x = st.lst[e % n];
+            const int cls = int(net.sym[x]);
+            const int* nb = net.nbr + net.nbr_off[l] + int64_t(x) * vol;
+            cplx<T> z = {0, 0};
+            for (int ci = 0; ci < prev; ci++) {
+                const cplx<T>* wk = net.params + net.link_off[l] + (int64_t(ci) * C + c) * net.nsym * vol +
+                                    int64_t(cls) * vol;
+                for (int k = 0; k < vol; k++) {
+                    const int y = nb[k];
+                    cplx<T> a;
+                    if (l == 0) {
+                        a = layer_input<T>(0, wn, st.A, N, 0, y);
+                    } else {
+                        const int o = st.aoff[l - 1] + ci * N + y;
+                        a = mask_bit(R[l], y) ? st.Tn[o] : st.A[o];
+                    }
+                    const cplx<T> q = wk[k];
+                    z.re += q.re * a.re - q.im * a.im;
+                    z.im += q.re * a.im + q.im * a.re;
+                }
+            }
+            const cplx<T> av = act(z, l);
+            if (last)
+                st.V[c * N + x] = av;
+            else
+                st.Tn[st.aoff[l] + c * N + x] = av;
+        }
+        __syncthreads();
+    }
+    const int C = net.ch[net.L - 1];
+    cplx<T> part = {0, 0};
+    for (int t = lane; t < n; t += 64) {
+        const int x = st.lst[t];
+        cplx<T> sx = {0, 0};
+        for (int c = 0; c < C; c++) sx = cadd(sx, st.V[c * N + x]);
+        st.nout[x] = sx;
+        part.re += sx.re - st.out[x].re;
+        part.im += sx.im - st.out[x].im;
+    }
+    __syncthreads();
+    return {wave_sum(part.re), wave_sum(part.im)};
+}
+
+// the proposal of the last cnn_inc_delta becomes the current state
+template <typename T>
+__device__ void cnn_inc_commit(const CnnView<T>& net, CnnInc<T>& st, const uint64_t (&R)[kMaxCnnLayers + 1][kMaxWords],
+                               int lane) {
+    const int N = net.N;
+    for (int l = 0; l + 1 < net.L; l++) {
+        const int n = mask_list(R[l + 1], N, st.lst, lane);
+        const int C = net.ch[l];
+        for (int e = lane; e < C * n; e += 64) {
+            const int o = st.aoff[l] + (e / n) * N + st.lst[e % n];
+            st.A[o] = st.Tn[o];
+        }
+        __syncthreads();
+    }
+    for (int x = lane; x < N; x += 64)
+        if (mask_bit(R[net.L], x)) st.out[x] = st.nout[x];
+    __syncthreads();
+}
+
+// Metropolis chains of a PsiCNN with local updates (same proposals and Philox stream as
+// the full-forward sampler)
+template <typename T>
+__global__ __launch_bounds__(64) void cnn_mc_inc_kernel(CnnView<T> net, uint64_t* chain_confs, int words,
+                                                        int num_chains, int move, int64_t num_proposals,
+                                                        uint64_t counter, uint32_t key0, uint32_t key1,
+                                                        uint32_t* accepted) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x;
+    if (c >= num_chains) return;
+    const int N = net.N;
+    CnnInc<T> st = cnn_inc_layout(net, smem);
+    uint64_t w[kMaxWords];
+    load_words(chain_confs, c, words, w);
+    uint64_t msk[kMaxWords];
+    for (int k = 0; k < kMaxWords; k++) {
+        const int bits = N - 64 * k;
+        msk[k] = k >= words ? 0 : bits >= 64 ? ~uint64_t(0) : bits > 0 ? (uint64_t(1) << bits) - 1 : 0;
+    }
+    T cur = cnn_inc_init(net, w, st, lane).re;
+    uint64_t R[kMaxCnnLayers + 1][kMaxWords];
+    uint32_t acc = 0;
+    for (int64_t q = 0; q < num_proposals; q++) {
+        const uint64_t t = counter + uint64_t(q);
+        const u32x4 r = philox(uint32_t(c), uint32_t(t), uint32_t(t >> 32), 0u, key0, key1);
+        const int i = int(range32(r.x, uint32_t(N)));
+        const int si_bit = int((w[i >> 6] >> (i & 63)) & 1u);
+        int kp = -1;
+        if (move == MOVE_EXCHANGE) {
+            uint64_t o[kMaxWords];
+            uint32_t n_opp = 0;
+            for (int k = 0; k < kMaxWords; k++) {
+                o[k] = (si_bit ? ~w[k] : w[k]) & msk[k];
+                n_opp += uint32_t(__popcll(o[k]));
+            }
+            if (n_opp == 0) continue;
+            kp = select_bit(o, words, range32(r.y, n_opp));
+        }
+        uint64_t F[kMaxWords] = {0, 0, 0, 0};
+        F[i >> 6] ^= uint64_t(1) << (i & 63);
+        if (kp >= 0) F[kp >> 6] ^= uint64_t(1) << (kp & 63);
+        const T d = cnn_inc_delta(net, w, F, st, R, lane).re;
+        const T d2 = T(2) * net.final_factor * d;
+        const T uu = (T(r.z) + T(0.5)) * T(2.3283064365386963e-10);
+        if (d2 > T(0) || uu <= dexp(d2)) {
+            for (int k = 0; k < kMaxWords; k++) w[k] ^= F[k];
+            cnn_inc_commit(net, st, R, lane);
+            acc++;
+        }
+    }
+    (void)cur;
+    if (lane == 0) {
+        for (int k = 0; k < words; k++) chain_confs[int64_t(c) * words + k] = w[k];
+        accepted[c] += acc;
+    }
+}
+
+// E_loc of PsiCNN configurations with the psi ratios from local updates
+template <typename T>
+__global__ __launch_bounds__(64) void cnn_local_energy_inc_kernel(CnnView<T> net, OpView op, const uint64_t* confs,
+                                                                  int64_t count, cplx<double>* e_loc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const int words = op.words;
+    CnnInc<T> st = cnn_inc_layout(net, smem);
+    uint64_t R[kMaxCnnLayers + 1][kMaxWords];
+    for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
+        uint64_t w[kMaxWords];
+        load_words(confs, s, words, w);
+        cnn_inc_init(net, w, st, lane);
+        double e_re = 0.0, e_im = 0.0;
+        for (int g = 0; g < op.num_groups; g++) {
+            double c_re = 0.0, c_im = 0.0;
+            for (int n = op.group_begin[g]; n < op.group_begin[g + 1]; n++) {
+                int par = 0;
+                for (int k = 0; k < words; k++) par += __popcll(~w[k] & op.string_zy[int64_t(n) * words + k]);
+                const double sg = (par & 1) ? -1.0 : 1.0;
+                c_re += sg * op.string_coef[2 * n];
+                c_im += sg * op.string_coef[2 * n + 1];
+            }
+            if (c_re == 0.0 && c_im == 0.0) continue;
+            bool diag = true;
+            uint64_t F[kMaxWords];
+            for (int k = 0; k < kMaxWords; k++) {
+                F[k] = k < words ? op.group_flip[int64_t(g) * words + k] : 0;
+                diag = diag && F[k] == 0;
+            }
+            if (diag) {
+                e_re += c_re;
+                e_im += c_im;
+                continue;
+            }
+            const cplx<T> dl = cnn_inc_delta(net, w, F, st, R, lane);
+            const double d_re = double(net.final_factor) * double(dl.re), d_im = double(net.final_factor) * double(dl.im);
+            const double m = exp(d_re);
+            double sn, cs;
+            sincos(d_im, &sn, &cs);
+            e_re += c_re * m * cs - c_im * m * sn;
+            e_im += c_re * m * sn + c_im * m * cs;
+        }
+        if (lane == 0) e_loc[s] = {e_re, e_im};
+        __syncthreads();
+    }
+}
+
 // O_k of a multi-layer PsiDeep by back-propagation (PsiDeep.hpp:345-445); row s of O is
 // written in parameter order (minus center if given).  LDS: Z, Act (forward), D (the
 // derivative of log psi by the pre-activations of every layer).
@@ -428,10 +738,18 @@ void launch_generic_mc(const V& net, uint64_t* chain_confs, int words, int num_c
                        int64_t num_proposals, uint64_t counter, uint64_t seed, uint32_t* accepted,
                        hipStream_t stream) {
     if (num_chains <= 0 || num_proposals <= 0) return;
-    allow_lds(generic_mc_kernel<V>, forward_lds(net));
-    generic_mc_kernel<V><<<unsigned(num_chains), 64, forward_lds(net), stream>>>(
-        net, chain_confs, words, num_chains, move, num_proposals, counter, uint32_t(seed), uint32_t(seed >> 32),
-        accepted);
+    using T = typename V::real_t;
+    if constexpr (std::is_same<V, CnnView<T>>::value) {
+        allow_lds(cnn_mc_inc_kernel<T>, cnn_inc_lds(net));
+        cnn_mc_inc_kernel<T><<<unsigned(num_chains), 64, cnn_inc_lds(net), stream>>>(
+            net, chain_confs, words, num_chains, move, num_proposals, counter, uint32_t(seed), uint32_t(seed >> 32),
+            accepted);
+    } else {
+        allow_lds(generic_mc_kernel<V>, forward_lds(net));
+        generic_mc_kernel<V><<<unsigned(num_chains), 64, forward_lds(net), stream>>>(
+            net, chain_confs, words, num_chains, move, num_proposals, counter, uint32_t(seed), uint32_t(seed >> 32),
+            accepted);
+    }
     ANN_LAUNCH_CHECK();
 }
 
@@ -439,8 +757,16 @@ template <typename V>
 void launch_generic_local_energy(const V& net, const OpView& op, const uint64_t* confs, int64_t count,
                                  cplx<double>* e_loc, hipStream_t stream) {
     if (count <= 0) return;
-    allow_lds(generic_local_energy_kernel<V>, forward_lds(net));
-    generic_local_energy_kernel<V><<<grid_for(count), 64, forward_lds(net), stream>>>(net, op, confs, count, e_loc);
+    using T = typename V::real_t;
+    if constexpr (std::is_same<V, CnnView<T>>::value) {
+        allow_lds(cnn_local_energy_inc_kernel<T>, cnn_inc_lds(net));
+        cnn_local_energy_inc_kernel<T><<<grid_for(count), 64, cnn_inc_lds(net), stream>>>(net, op, confs, count,
+                                                                                           e_loc);
+    } else {
+        allow_lds(generic_local_energy_kernel<V>, forward_lds(net));
+        generic_local_energy_kernel<V><<<grid_for(count), 64, forward_lds(net), stream>>>(net, op, confs, count,
+                                                                                         e_loc);
+    }
     ANN_LAUNCH_CHECK();
 }
 

@@ -32,6 +32,9 @@ struct CnnView {
     const cplx<T>* params;
     const int* nbr;
     const uint32_t* sym;
+    // cons[(l * N + y) * kMaxWords + k]: word k of the mask of the sites x whose layer-l
+    // pre-activations read site y (the consumers of y, for the local updates)
+    const uint64_t* cons;
 };
 
 constexpr int kMaxMlpLayers = 4;

@@ -223,10 +223,13 @@ __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, B
             if (ok[r]) emax_l = fmaxf(emax_l, sv);
         }
         const float emax = wave_max(emax_l);
+        // with bn.exclusive the current pattern is not proposed (Metropolized Gibbs: the
+        // proposal is the conditional law without its current value, corrected below)
         float lsum = 0.f;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            wr[r] = ok[r] ? __expf(e[r] - emax) : 0.f;
+            const bool cur_p = bn.exclusive && uint32_t(4 * lane + r) == p_old;
+            wr[r] = ok[r] && !cur_p ? __expf(e[r] - emax) : 0.f;
             lsum += wr[r];
         }
         const float incl = wave_incl_scan(lsum, lane);
@@ -249,13 +252,20 @@ __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, B
         if (pick < 0) pick = int(p_old);
         const uint32_t p_new =
             uint32_t(__builtin_amdgcn_readlane(pick, L)) & (BS == kMaxBlock ? 0xFFu : ((1u << bs) - 1u));
-        const uint32_t flips = p_old ^ p_new;
+        const uint32_t flips = Z > 0.f ? p_old ^ p_new : 0u;
         if (flips != 0) {
             float e_old = 0.f, e_new = 0.f;
 #pragma unroll
             for (int j = 0; j < kMaxBlock; j++) {
                 e_old += (p_old >> j) & 1u ? lg[j] : 0.f;
                 e_new += (p_new >> j) & 1u ? lg[j] : 0.f;
+            }
+            // log q(old | new) - log q(new | old); exclusive: q(new | old) = w_new / (Z_all - w_old)
+            // with Z (the scan total) = Z_all - w_old, and q(old | new) = w_old / (Z_all - w_new)
+            float lq = e_old - e_new;
+            if (bn.exclusive) {
+                const float w_old = __expf(e_old - emax), w_new = __expf(e_new - emax);
+                lq += __logf(Z) - __logf(Z + w_old - w_new);
             }
             rows += uint32_t(__popc(flips));
             // psi ratio: theta' = theta - 2 s_j W_j over the flipped sites, whose rows are
@@ -287,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void block_sweep_kernel(DenseView<T> net, B
                 }
             }
             const T nxt = log_amp(z);
-            const T d = T(2) * (nxt - cur) + T(e_old - e_new);
+            const T d = T(2) * (nxt - cur) + T(lq);
             const T uu = (T(r_z) + T(0.5)) * T(2.3283064365386963e-10);
             if (d > T(0) || uu <= dexp(d)) {
 #pragma unroll

@@ -26,6 +26,7 @@ struct BlockNet {
     const int32_t* b1q;       // [H]
     const uint16_t* W2h;      // [N][H] second layer, bf16 (row of output o)
     const float* b2;          // [N]
+    int exclusive;            // propose only patterns other than the current one
 };
 
 // quantise the fp32 parameters into the inference form

@@ -198,6 +198,20 @@ void Psi::sync_device() {
             }
         }
         nbr_d.upload(nbr.data(), nbr.size());
+        // consumers of every site in every layer (the receptive fields of the local updates)
+        std::vector<uint64_t> cons(size_t(cnn_layers) * N * kMaxWords, 0);
+        size_t off = 0;
+        for (int l = 0; l < cnn_layers; l++) {
+            const int* cv = &connectivity[3 * l];
+            const int vol = cv[0] * cv[1] * cv[2];
+            for (int x = 0; x < N; x++)
+                for (int k = 0; k < vol; k++) {
+                    const int y = nbr[off + size_t(x) * vol + k];
+                    cons[(size_t(l) * N + y) * kMaxWords + (x >> 6)] |= uint64_t(1) << (x & 63);
+                }
+            off += size_t(N) * vol;
+        }
+        cons_d.upload(cons.data(), cons.size());
         sym_d.upload(sym.data(), sym.size());
         dirty = false;
         return;
@@ -339,6 +353,7 @@ static CnnView<T> make_cnn_view(const Psi& q, const cplx<T>* p, const int* nbr, 
     v.params = p;
     v.nbr = nbr;
     v.sym = sym;
+    v.cons = q.cons_d.p;
     return v;
 }
 
@@ -756,6 +771,7 @@ void sample(Psi& psi, Ensemble& ens) {
     if (prop) {
         if (prop->N != N) throw std::invalid_argument("the proposal was built for another number of sites");
         bn = prop->view();
+        bn.exclusive = ens.proposal_exclusive ? 1 : 0;
         plain_per_step = int64_t(ens.plain_sweeps) * N;
         block_per_step = int64_t(ens.num_sweeps) * ((N + prop->bs - 1) / prop->bs);
         ens.bstat.ensure(size_t(2) * B);
@@ -1754,7 +1770,7 @@ BlockNet Proposal::view() {
         launch_block_net_pack(params.p, N, H, W1q.p, b1q.p, W2h.p, b2.p, S());
         packed = true;
     }
-    return {N, H, bs, num_blocks, blocks.p, W1q.p, b1q.p, W2h.p, b2.p};
+    return {N, H, bs, num_blocks, blocks.p, W1q.p, b1q.p, W2h.p, b2.p, 0};
 }
 
 static double batch_loss(Proposal& p, int Bt) {

@@ -229,6 +229,7 @@ struct Psi {
     DevBuf<cplx<float>> cp32;
     DevBuf<int> nbr_d;
     DevBuf<uint32_t> sym_d;
+    DevBuf<uint64_t> cons_d;
     // multi-layer PsiDeep device state (layers concatenated; offsets per layer)
     DevBuf<uint32_t> mlp_cn;
     DevBuf<cplx<double>> mlp_w64, mlp_b64;
@@ -332,6 +333,7 @@ struct Ensemble {
     // learned block proposal (not owned); block sweeps per sample and plain sweeps per sample
     Proposal* proposal = nullptr;
     uint32_t plain_sweeps = 1;
+    bool proposal_exclusive = false;   // block moves never redraw the current pattern
     // block-move counters of the last call: per chain [null redraws, rows evaluated],
     // summed on the host when asked for, and the block proposals made
     DevBuf<uint32_t> bstat;

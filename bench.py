@@ -19,7 +19,9 @@ Protocol (all on the GPU, untimed unless stated):
      and longer SR runs reach angles where log psi develops spikes;
   2. training data: num_chains x 64 samples of the plain sampler; the proposal
      network (100 -> 256 -> 100, tanh; 2x4 / 4x2 plaquette blocks) is trained by
-     the library's own forward / backward / Adam kernels;
+     the library's own forward / backward / Adam kernels; a block move draws its
+     pattern from the network's conditional law without the current pattern
+     (Metropolized Gibbs, exact under the corrected Metropolis-Hastings ratio);
   3. tau_int (Sokal window c = 5, batch means as a cross-check) of E_loc per sweep
      on 4096 chains x 400 sweeps after 300 plain thermalisation sweeps, for plain
      exchange sweeps, plain single-flip sweeps (the full spin space) and learned
@@ -30,11 +32,14 @@ Protocol (all on the GPU, untimed unless stated):
        plain-flip-matched : ceil(2 tau) single-flip sweeps per sample (full space)
        learned            : ceil(2 tau) block sweeps per sample (headline)
        learned-rtol1e-2   : the learned path with the SR solve stopped at 1e-2
-  5. energy check: both samplers on the same state until the standard error of the
-     mean (from independent chain means) is below 0.004, and the fp32 device path
+  5. energy check: both samplers on the same state (1,000 plain thermalisation
+     sweeps) until the standard error of the mean (from independent chain means) is
+     below 0.004 or 20 s have passed, and the fp32 device path
      against the fp64 device path on the same 4,096 samples;
   6. dense-S leg (1D Heisenberg ring N=64, M=256, P=16,704, 8,192 chains, 64
      exchange sweeps): S on fp64 matrix cores and the fp64 blocked Cholesky solve;
+     CNN quench leg (TFIM 12x12, PsiCNN 2 x 3x3 x 8 channels, 16,384 single-flip
+     chains, RK4 steps of dt = 0.005);
   7. CPU baseline: the fp64 oracle (sampling, E_loc, O_k, F and a matrix-free
      Jacobi CG to the same tolerance) on a bounded sample, at 1 core and all cores.
 ESS per step = Ns * min(1, S_sw / (2 tau_int)).  Prints one JSON line.
@@ -187,6 +192,7 @@ class Stepper:
 
 
 THERM_SWEEPS = 300   # plain sweeps before a chain's first sample (also for learned paths)
+ENERGY_THERM_SWEEPS = 1000   # the same before the energy comparison (its transients must not bias 1e-4)
 
 
 def measure_tau(A, psi, op, proposal, chains=4096, T=400, seed=23, move="exchange"):
@@ -197,7 +203,7 @@ def measure_tau(A, psi, op, proposal, chains=4096, T=400, seed=23, move="exchang
     # included, as the reference's does) come from the second pass.
     ens = A.MonteCarloSpins(chains * T, 1, THERM_SWEEPS, chains, move=move, seed=seed, persistent=True)
     if proposal is not None:
-        ens.set_proposal(proposal, plain_sweeps=0)
+        ens.set_proposal(proposal, plain_sweeps=0, exclude_current=True)
     A.log_psi(psi, ens)
     el = A.local_energies(op, psi, ens).real.reshape(T, chains)
     return tau_int_sokal(el), tau_batch_means(el), ens.acceptance_rate, el
@@ -207,9 +213,10 @@ def energy_run(A, psi, op, proposal, chains, target_se, max_seconds, seed, chunk
     """E on persistent chains (one plain or block sweep between samples) until the
     standard error of the mean from the independent chain means is below target_se or
     max_seconds have passed; returns (E, SE, samples, seconds, last chunk's configurations)"""
-    ens = A.MonteCarloSpins(chains * chunk, 1, THERM_SWEEPS, chains, move="exchange", seed=seed, persistent=True)
+    ens = A.MonteCarloSpins(chains * chunk, 1, ENERGY_THERM_SWEEPS, chains, move="exchange", seed=seed,
+                            persistent=True)
     if proposal is not None:
-        ens.set_proposal(proposal, plain_sweeps=0)
+        ens.set_proposal(proposal, plain_sweeps=0, exclude_current=True)
     sums = np.zeros(chains)
     n = 0
     t0 = time.perf_counter()
@@ -288,6 +295,52 @@ def dense_S_leg(A, steps=2, warmup=1):
                      "peak_tflops": 78.6, "frac": gram_flop / g_avg * 1e-9 / 78.6, "engine": "fp64 MFMA"},
             "cholesky": {"ms": c_avg, "algorithmic_flop": chol_flop, "achieved_tflops": chol_flop / c_avg * 1e-9,
                          "peak_tflops": 78.6, "frac": chol_flop / c_avg * 1e-9 / 78.6, "engine": "fp64 MFMA"}}
+
+
+def cnn_quench_leg(A, steps=2, warmup=1):
+    """TFIM 12x12 (h = 3.04) quench from the near-uniform state: PsiCNN with 2 layers of 3x3
+    periodic kernels, 8 channels (648 parameters, weights complex-Gaussian 1e-3), 16,384
+    persistent single-flip chains (one sweep between samples), RK4 with dt = 0.005 and the
+    fp64 Cholesky solve of S + 1e-3 on every right-hand side"""
+    L = 12
+    N = L * L
+    rng = np.random.default_rng(4)
+    n = 1 * 8 * 9 + 8 * 8 * 9
+    params = 1e-3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    psi = A.PsiCNN([1, L, L], [8, 8], [[1, 3, 3], [1, 3, 3]], np.zeros(N, dtype=np.uint32), params, 1.0, 0.0)
+    psi.precision = "fp32"
+    op = A.Operator(A.hamiltonians.transverse_ising_square(L, 1.0, 3.04))
+    B = 16384
+    ens = A.MonteCarloSpins(B, 1, 50, B, move="flip", seed=3, persistent=True)
+    rk = A.RK4(psi, op, ens, 0.005, shift=1e-3, method="cholesky")
+    for _ in range(warmup):
+        rk.step()
+    A.profiler_reset()
+    A.start_profiling()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rk.step()
+    A.stop_profiling()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    kms = {k: A.profiler_query(k) for k in ("mc_sweep", "local_energy", "o_k", "spin_sums", "gram", "cholesky",
+                                            "log_psi")}
+    sweep_ms, sweep_n = kms["mc_sweep"]
+    # work model of one single-flip proposal with local updates (a flip away from the
+    # lattice edge changes 9 layer-0 and 25 layer-1 sites): complex MACs 9*8*9 + 25*8*8*9,
+    # 8 real flops each
+    work = B * N * 8.0 * (9 * 8 * 9 + 25 * 8 * 8 * 9) * max(sweep_n, 1)
+    E = np.array(rk.energies).real
+    return {"workload": "TFIM 12x12 PBC h=3.04 quench, PsiCNN 2 x (3x3, 8 channels), P=648, 16384 persistent "
+                        "single-flip chains, 1 sweep per sample, RK4 dt=0.005, fp32 sampling/E_loc/O_k, dense S "
+                        "+ fp64 Cholesky (shift 1e-3)",
+            "ms_per_rk4_step": ms, "steps": steps, "tdvp_evaluations_per_step": 4,
+            "kernel_ms_per_step": {k: v[0] / steps for k, v in kms.items()},
+            "kernel_launches": {k: v[1] for k, v in kms.items()},
+            "energy_first_last": [float(E[0]), float(E[-1])],
+            "sweep_roofline": {"bound": "valu", "kernel": "cnn_mc_inc (local updates)",
+                               "achieved_tflops": work / (sweep_ms * 1e-3) / 1e12 if sweep_ms else None,
+                               "peak_tflops": 157.3,
+                               "frac": work / (sweep_ms * 1e-3) / 1e12 / 157.3 if sweep_ms else None}}
 
 
 def main():
@@ -382,7 +435,7 @@ def main():
             ("learned-rtol1e-2", s_learn, prop, tau_l, "exchange", 1e-2, 100)):
         ens = A.MonteCarloSpins(B, sweeps, THERM_SWEEPS, B, move=move, seed=17, persistent=True)
         if proposal is not None:
-            ens.set_proposal(proposal, plain_sweeps=0)
+            ens.set_proposal(proposal, plain_sweeps=0, exclude_current=True)
         st = Stepper(A, psi.copy(), op, ens, args.lr, iters, solver=args.solver, rtol=rtol, warm=args.warm_solve)
         ms, kms = st.timed(args.steps, args.warmup)
         ess = B * min(1.0, sweeps / (2 * tau))
@@ -403,6 +456,7 @@ def main():
     # 5. energies of the two samplers on the same state, and fp32 against fp64
     energy = None
     dense = None
+    cnn = None
     if not args.no_extra:
         Ep, sep, n_p, t_p, confs = energy_run(A, psi, op, None, B, args.energy_se, args.energy_seconds, 41)
         El, sel, n_l, t_l, _ = energy_run(A, psi, op, prop, B, args.energy_se, args.energy_seconds, 43)
@@ -413,6 +467,8 @@ def main():
         print(f"energy: plain {Ep:.4f} +- {sep:.4f}, learned {El:.4f} +- {sel:.4f}", file=sys.stderr, flush=True)
         # 6. dense S and Cholesky
         dense = dense_S_leg(A)
+        cnn = cnn_quench_leg(A)
+        print(f"cnn quench leg: {cnn['ms_per_rk4_step']:.1f} ms per RK4 step", file=sys.stderr, flush=True)
         print(f"dense S leg: {dense['ms_per_step']:.1f} ms/step, gram {dense['gram']['ms']:.1f} ms "
               f"({dense['gram']['achieved_tflops']:.1f} TFLOP/s), cholesky {dense['cholesky']['ms']:.1f} ms "
               f"({dense['cholesky']['achieved_tflops']:.1f} TFLOP/s)", file=sys.stderr, flush=True)
@@ -515,6 +571,7 @@ def main():
                            "acceptance_learned": acc_l},
                    "energy_check": energy,
                    "dense_S": dense,
+                   "cnn_quench": cnn,
                    "roofline_other_kernels": extra_roofs,
                    "roofline_plain_sweep": roof_plain},
         "roofline": roof,

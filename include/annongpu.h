@@ -242,6 +242,11 @@ int ann_proposal_train(ann_proposal p, ann_ensemble ens, int32_t steps, int32_t 
  * each) plus plain_sweeps single-flip/exchange sweeps.  The ensemble does not
  * own p: keep it alive while attached. */
 int ann_monte_carlo_set_proposal(ann_ensemble ens, ann_proposal p, uint32_t plain_sweeps);
+/* additive: with exclude_current != 0 a block move draws its pattern from the network's
+ * conditional law without the current pattern, and the Metropolis-Hastings ratio carries
+ * the matching factor (Z - q(old)) / (Z - q(new)) (a Metropolized Gibbs move); no move
+ * redraws the current configuration */
+int ann_monte_carlo_set_proposal_mode(ann_ensemble ens, int32_t exclude_current);
 
 #ifdef __cplusplus
 }

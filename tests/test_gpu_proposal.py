@@ -89,8 +89,10 @@ def _chain_stats(values, chains):
     return cm.mean(), cm.std(ddof=1) / math.sqrt(chains)
 
 
-@pytest.mark.parametrize("precision,scale", [("fp64", 3.0), ("fp32", 3.0), ("fp32", 0.0), ("fp32", None)])
-def test_block_sampler_preserves_psi_squared(precision, scale):
+@pytest.mark.parametrize("precision,scale,exclusive", [("fp64", 3.0, False), ("fp32", 3.0, False),
+                                                       ("fp32", 0.0, False), ("fp32", None, False),
+                                                       ("fp64", 3.0, True), ("fp32", 0.0, True)])
+def test_block_sampler_preserves_psi_squared(precision, scale, exclusive):
     """block moves alone (no plain sweeps after plain thermalisation) sample |psi|^2
     restricted to S_z = 0; scale 0 is the uniform proposal (proposal ratio 1) and
     scale None the plain exchange sampler (the control)"""
@@ -106,7 +108,7 @@ def test_block_sampler_preserves_psi_squared(precision, scale):
         prop = A.BlockProposal(N, A.chain_blocks(N, 4), hidden=64, seed=5)
         # a strongly non-uniform network: a wrong proposal ratio would bias the result
         prop.params = scale * prop.params
-        ens.set_proposal(prop, plain_sweeps=0)
+        ens.set_proposal(prop, plain_sweeps=0, exclude_current=exclusive)
     el = A.local_energies(A.Operator(H), psi, ens).real
     s = ens.last_samples()[:, 0].astype(np.int64)
     assert np.all([bin(int(x)).count("1") == N // 2 for x in s[:64]])
@@ -118,14 +120,16 @@ def test_block_sampler_preserves_psi_squared(precision, scale):
     assert 0.0 < ens.acceptance_rate < 1.0
 
 
+@pytest.mark.parametrize("exclusive", [False, True])
 @pytest.mark.parametrize("precision,scale", [("fp32", 1.0), ("fp32", 3.0), ("fp64", 3.0), ("fp32", "trained")])
-def test_plaquette_block_sampler_preserves_psi_squared(precision, scale):
+def test_plaquette_block_sampler_preserves_psi_squared(precision, scale, exclusive):
     """the benchmark's block shape: 2x4 and 4x2 plaquettes (8 sites, so every lane of the
     pattern draw holds patterns and the shared high-bit energies span bits 2..7) with the
     benchmark's 256 hidden units, on a 4x4 Heisenberg torus.  E and <s_0 s_1> within 4
     standard errors of exact sums over the 12,870 states of the S_z = 0 sector, for the
     network as initialised, scaled to be strongly non-uniform, and trained on plain
-    samples; redraws of the current pattern are counted apart from accepted moves"""
+    samples; redraws of the current pattern are counted apart from accepted moves.
+    exclusive: the Metropolized Gibbs form (the current pattern is never proposed)"""
     L, N = 4, 16
     psi = A.seeded_rbm_deep(N, 32, 21, sigma=0.1)   # participation ratio ~1,500 of 12,870 states
     psi.precision = precision
@@ -143,7 +147,7 @@ def test_plaquette_block_sampler_preserves_psi_squared(precision, scale):
         prop.params = scale * prop.params
     chains, steps = 2048, 48
     ens = A.MonteCarloSpins(chains * steps, 2, 10, chains, move="exchange", seed=8)
-    ens.set_proposal(prop, plain_sweeps=0)
+    ens.set_proposal(prop, plain_sweeps=0, exclude_current=exclusive)
     el = A.local_energies(A.Operator(H), psi, ens).real
     s = ens.last_samples()[:, 0].astype(np.int64)
     assert np.all([bin(int(x)).count("1") == N // 2 for x in s[:256]])
@@ -155,13 +159,15 @@ def test_plaquette_block_sampler_preserves_psi_squared(precision, scale):
     st = ens.block_stats
     assert st["proposals"] == chains * steps * 2 * 2     # 2 block sweeps of ceil(16 / 8) moves per sample
     assert 0 <= st["null_moves"] < st["proposals"]
+    if exclusive:   # only blocks whose spins are all equal have no other pattern
+        assert st["null_moves"] < 0.05 * st["proposals"]
     moved = ens.acceptance_rate * st["proposals"]
     assert 0.0 < ens.acceptance_rate < 1.0
     # every evaluated proposal flips at least 2 sites (S_z is conserved) and at most 8
     evaluated = st["proposals"] - st["null_moves"]
     assert 2 * evaluated <= st["rows_evaluated"] <= 8 * evaluated
     assert moved <= evaluated
-    print(f"\n{precision} scale {scale}: E {E:.4f} +- {se:.4f} (exact {E_ex:.4f}), acceptance "
+    print(f"\n{precision} scale {scale} exclusive {exclusive}: E {E:.4f} +- {se:.4f} (exact {E_ex:.4f}), acceptance "
           f"{ens.acceptance_rate:.3f}, null redraws {st['null_moves'] / st['proposals']:.3f}, "
           f"rows per evaluated move {st['rows_evaluated'] / max(evaluated, 1):.2f}")
 
