@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libannongpu.so")
 
 ANN_OK, ANN_E_INVALID, ANN_E_DEVICE, ANN_E_UNSUPPORTED, ANN_E_NUMERIC = 0, 1, 2, 3, 4
 FP64, FP32 = 0, 1
-MOVE_FLIP, MOVE_EXCHANGE = 0, 1
+MOVE_FLIP, MOVE_EXCHANGE, MOVE_PAULI = 0, 1, 2
 SOLVE_CHOLESKY, SOLVE_CG, SOLVE_CG_WARM, SOLVE_CG_KFAC, SOLVE_CG_KFAC_WARM = 0, 1, 2, 3, 4
 
 if not os.path.exists(LIB_PATH):
@@ -91,6 +91,22 @@ _SIGNATURES = {
     "ann_tdvp_get_O_samples": [P, P, i64],
     "ann_local_energies_noalloc": [P, P, P, i64, P],
     "ann_hilbert_space_distance": [P, P, P, i32, P, f64, P, P],
+    "ann_psi_create_classical": [P, i32, i32, P, P, P, i32, P, P],
+    "ann_psi_classical_info": [P, P, P, P],
+    "ann_psi_classical_ref": [P, P],
+    "ann_psi_get_ref_log_prefactor": [P, P],
+    "ann_psi_set_ref_log_prefactor": [P, P],
+    "ann_exact_summation_paulis_create": [P, u32],
+    "ann_monte_carlo_paulis_create": [P, u32, u32, u32, u32, u64],
+    "ann_pauli_sample_list_create": [P, u32, P, i64, P],
+    "ann_log_psi_s_pauli": [P, P, P],
+    "ann_psi_O_k_pauli": [P, P, P],
+    "ann_kl_create": [P, u32],
+    "ann_kl_destroy": [P],
+    "ann_kl_eval": [P, P, P, P, f64, f64, i32, P, P, P],
+    "ann_kl_info": [P, P, P],
+    "ann_kl_get_log_psi_scale": [P, P],
+    "ann_kl_set_log_psi_scale": [P, f64],
 }
 
 for _name, _args in _SIGNATURES.items():

@@ -90,6 +90,61 @@ def _conf_words(spins):
     return out
 
 
+class PauliString:
+    """Pauli-string configuration (include/basis/PauliString.hpp:18-278): bit i of ``a``
+    and ``b`` give site i's type a_i + 2 b_i (0 = identity, 1 = X, 2 = Y, 3 = Z); at most
+    64 sites."""
+
+    __slots__ = ("a", "b")
+
+    def __init__(self, a=0, b=0):
+        self.a, self.b = int(a), int(b)
+
+    @staticmethod
+    def enumerate(index):
+        """PauliString.hpp:30-35: a from the even bits of the index, b from the odd bits"""
+        index = int(index)
+        a = b = 0
+        for k in range(32):
+            a |= ((index >> (2 * k)) & 1) << k
+            b |= ((index >> (2 * k + 1)) & 1) << k
+        return PauliString(a, b)
+
+    def __getitem__(self, i):
+        return ((self.a >> i) & 1) | (((self.b >> i) & 1) << 1)
+
+    def set_at(self, i, kind):
+        """PauliString.hpp:37-56"""
+        bit = 1 << i
+        self.a = (self.a & ~bit) | (bit if kind & 1 else 0)
+        self.b = (self.b & ~bit) | (bit if kind & 2 else 0)
+
+    def roll(self, shift, N):
+        shift %= N
+        mask = (1 << N) - 1
+
+        def rot(x):
+            x &= mask
+            return ((x << shift) | (x >> (N - shift))) & mask
+
+        return PauliString(rot(self.a), rot(self.b))
+
+    def array(self, num_sites):
+        return np.array([self[i] for i in range(num_sites)], dtype=np.uint32)
+
+    def words(self):
+        return np.array([self.a, self.b], dtype=np.uint64)
+
+    def __eq__(self, other):
+        return isinstance(other, PauliString) and (self.a, self.b) == (other.a, other.b)
+
+    def __hash__(self):
+        return hash((self.a, self.b))
+
+    def __repr__(self):
+        return f"PauliString({self.a:#x}, {self.b:#x})"
+
+
 # ---------------------------------------------------------- wave functions
 
 class _Psi:
@@ -393,11 +448,12 @@ class PsiCNN(_Psi):
 
     @property
     def num_channels_list(self):
-        return self._channels.copy()
+        # unsigned, as the reference's property (MAIN:180)
+        return self._channels.astype(np.uint32)
 
     @property
     def connectivity_list(self):
-        return self._connectivity.copy()
+        return self._connectivity.astype(np.uint32)
 
     @property
     def symmetry_classes(self):
@@ -438,6 +494,209 @@ class PsiCNN(_Psi):
         return PsiCNN(obj["extent"], obj["num_channels_list"], obj["connectivity_list"], obj["symmetry_classes"],
                       obj["params"], obj["final_factor"], obj["log_prefactor_re"] + 1j * obj["log_prefactor_im"],
                       gpu)
+
+
+class PsiFullyPolarized:
+    """PsiFullyPolarized(num_sites, log_prefactor) -- include/quantum_state/PsiFullyPolarized.hpp:
+    the reference state of the classical networks: log psi = 0 on every configuration (its
+    log_prefactor is stored but, as in the reference, does not enter log psi), no parameters."""
+
+    def __init__(self, num_sites, log_prefactor=0.0):
+        self.num_sites = int(num_sites)
+        self.log_prefactor = complex(log_prefactor)
+        self.num_params = 0
+        self._h = handle()
+        check(lib.ann_psi_create_classical(ref(self._h), self.num_sites, 0, None, None, None, 1, None, None))
+        self._precision = "fp64"
+
+    def __del__(self):
+        if self._h is not None and self._h.value:
+            try:
+                lib.ann_psi_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def params(self):
+        return np.zeros(0, dtype=np.complex128)
+
+    @property
+    def gpu(self):
+        return True
+
+    def copy(self):
+        return PsiFullyPolarized(self.num_sites, self.log_prefactor)
+
+    def __pos__(self):
+        return self.copy()
+
+    def vector(self, ensemble):
+        return psi_vector(self, ensemble)
+
+    def to_json(self):
+        """pyANNonGPU/PsiFullyPolarized.py:4-10"""
+        return dict(type="PsiFullyPolarized", num_sites=self.num_sites, log_prefactor_re=self.log_prefactor.real,
+                    log_prefactor_im=self.log_prefactor.imag)
+
+    @staticmethod
+    def from_json(obj):
+        return PsiFullyPolarized(obj["num_sites"], obj["log_prefactor_re"] + 1j * obj["log_prefactor_im"])
+
+
+class PsiClassical(_Psi):
+    """PsiClassical (include/quantum_state/PsiClassical.hpp:48-160, source/quantum_state/
+    PsiClassical.cu:9-61), the reference's PsiClassicalFP_1/_2 and PsiClassicalANN_1/_2:
+
+        log psi(s) = log_prefactor + sum_n params[n] L_n(s)  [+ log psi_ref(s), order 2],
+
+    L_n(s) = sum over the strings of H_local[n] of c_m <s|P_m|s'> (Operator::fast_local_energy,
+    operator/Operator.hpp:123-135), on spin configurations or Pauli strings.  psi_ref is a
+    PsiFullyPolarized (log psi_ref = 0, no parameters) or a PsiDeep; with order 2 a PsiDeep
+    reference enters log psi and its parameters follow the classical ones.
+    """
+
+    def __init__(self, num_sites, H_local, params, psi_ref, log_prefactor=0.0, gpu=True, order=1):
+        self._num_sites = int(num_sites)
+        ops = [h if isinstance(h, Operator) else Operator(h) for h in H_local]
+        self._H_local = ops
+        p = as_c128(params).ravel()
+        if p.size < len(ops):
+            raise ValueError(f"expected {len(ops)} classical parameters, got {p.size}")
+        p = np.ascontiguousarray(p[:len(ops)])
+        hs = (ctypes.c_void_p * max(len(ops), 1))(*[o._h.value for o in ops])
+        if psi_ref is None:
+            psi_ref = PsiFullyPolarized(num_sites, 0.0)
+        self._ref_template = psi_ref
+        net = None if isinstance(psi_ref, PsiFullyPolarized) else psi_ref
+        if net is not None and not isinstance(net, PsiDeep):
+            raise TypeError("psi_ref must be a PsiFullyPolarized or a PsiDeep")
+        self._precision = "fp64"
+        self._h = handle()
+        check(lib.ann_psi_create_classical(ref(self._h), self._num_sites, len(ops), ctypes.cast(hs, ctypes.c_void_p),
+                                           ptr(p), net._h if net is not None else None, int(order),
+                                           ptr(c2(log_prefactor)), ptr(c2(psi_ref.log_prefactor))))
+
+    @property
+    def num_sites(self):
+        return self._num_sites
+
+    @property
+    def N(self):
+        return self._num_sites
+
+    @property
+    def H_local(self):
+        return list(self._H_local)
+
+    def _cinfo(self):
+        o, n, r = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib.ann_psi_classical_info(self._h, ref(o), ref(n), ref(r)))
+        return o.value, n.value, r.value
+
+    @property
+    def order(self):
+        return self._cinfo()[0]
+
+    @property
+    def ref_log_prefactor(self):
+        out = np.zeros(2)
+        check(lib.ann_psi_get_ref_log_prefactor(self._h, ptr(out)))
+        return complex(out[0], out[1])
+
+    @ref_log_prefactor.setter
+    def ref_log_prefactor(self, value):
+        check(lib.ann_psi_set_ref_log_prefactor(self._h, ptr(c2(value))))
+
+    @property
+    def psi_ref(self):
+        """a copy of the reference (writing to it has no effect, MAIN:236-239)"""
+        t = self._ref_template
+        if isinstance(t, PsiFullyPolarized):
+            return PsiFullyPolarized(self._num_sites, self.ref_log_prefactor)
+        h = handle()
+        check(lib.ann_psi_classical_ref(self._h, ref(h)))
+        c = object.__new__(type(t))
+        c.__dict__.update(t.__dict__)
+        c._h = h
+        c._precision = "fp64"
+        return c
+
+    def update_psi_ref_kernel(self):
+        """MAIN:255; the reference is held by the library and needs no update"""
+
+    def calibrate(self, ensemble):
+        """MAIN:274-281 (MonteCarloSpins): psi_ref.log_prefactor -= log_psi(psi_ref),
+        log_prefactor -= log_psi(psi).  On an exact summation both are normalised instead
+        (and the mean phase of log psi removed), as the other wave functions do."""
+        exact = type(ensemble).__name__.startswith("ExactSummation")
+        if not isinstance(self._ref_template, PsiFullyPolarized):
+            r = self.psi_ref
+            if exact:
+                self.ref_log_prefactor = self.ref_log_prefactor - np.log(r.norm(ensemble))
+            else:
+                self.ref_log_prefactor = self.ref_log_prefactor - log_psi(r, ensemble)
+        if exact:
+            _Psi.calibrate(self, ensemble)
+        else:
+            self.log_prefactor -= log_psi(self, ensemble)
+
+    def to_json(self, ansatz=None):
+        """vCN_to_json (pyANNonGPU/PsiClassical.py:7-25); ansatz defaults to the H_local expressions"""
+        ansatz = [o.expr for o in self._H_local] if ansatz is None else list(ansatz)
+        pref = self.psi_ref
+        obj = dict(type="PsiClassical", num_sites=self.num_sites, order=self.order,
+                   ansatz=[a.to_json() for a in ansatz], params=self.params[:len(self._H_local)],
+                   psi_ref=pref.to_json(), log_prefactor_re=self.log_prefactor.real,
+                   log_prefactor_im=self.log_prefactor.imag)
+        return json.loads(json.dumps(obj, cls=NumpyEncoder))
+
+    @staticmethod
+    def from_json(json_obj, gpu=True):
+        """vCN_from_json (pyANNonGPU/PsiClassical.py:28-67)"""
+        obj = json.loads(json.dumps(json_obj, cls=NumpyEncoder), cls=NumpyDecoder)
+        r = obj["psi_ref"]
+        psi_ref = PsiFullyPolarized.from_json(r) if r["type"] == "PsiFullyPolarized" else PsiDeep.from_json(r, gpu)
+        H_local = [Operator(PauliExpression.from_json(h)) for h in obj["ansatz"]]
+        return PsiClassical(obj["num_sites"], H_local, obj["params"], psi_ref,
+                            obj["log_prefactor_re"] + 1j * obj["log_prefactor_im"], gpu, order=obj["order"])
+
+
+def PsiClassicalFP_1(num_sites, H_local, params, psi_ref, log_prefactor=0.0, gpu=True):
+    return PsiClassical(num_sites, H_local, params, psi_ref, log_prefactor, gpu, order=1)
+
+
+def PsiClassicalFP_2(num_sites, H_local, params, psi_ref, log_prefactor=0.0, gpu=True):
+    return PsiClassical(num_sites, H_local, params, psi_ref, log_prefactor, gpu, order=2)
+
+
+def PsiClassicalANN_1(num_sites, H_local, params, psi_ref, log_prefactor=0.0, gpu=True):
+    return PsiClassical(num_sites, H_local, params, psi_ref, log_prefactor, gpu, order=1)
+
+
+def PsiClassicalANN_2(num_sites, H_local, params, psi_ref, log_prefactor=0.0, gpu=True):
+    return PsiClassical(num_sites, H_local, params, psi_ref, log_prefactor, gpu, order=2)
+
+
+def new_classical_network(num_sites, order, H_local, symmetric=False, distance="max", params=0,
+                          psi_ref="fully polarized", use_super_operator=False, gpu=False):
+    """pyANNonGPU/new_classical_network.py:6-70.  H_local: a PauliExpression (one local
+    operator per term, as iterating the reference's PauliExpression yields) or a list of
+    expressions.  The symmetric (PSI_CLASSICAL_SYMMETRIC) and SuperOperator variants are
+    not built."""
+    assert order in (1, 2)
+    if symmetric or use_super_operator:
+        raise NotImplementedError("the symmetric and SuperOperator classical networks are not built")
+    if isinstance(H_local, PauliExpression):
+        H_local = H_local.term_list()
+    H_local = list(H_local)
+    log_prefactor = np.log(1 / 2 ** (num_sites / 2))
+    if isinstance(params, (int, float)) and params == 0:
+        params = np.zeros(len(H_local), dtype=complex)
+    if isinstance(psi_ref, str):
+        return PsiClassical(num_sites, H_local, params, PsiFullyPolarized(num_sites, log_prefactor), log_prefactor,
+                            gpu, order=order)
+    return PsiClassical(num_sites, H_local, params, psi_ref, 0.0, gpu, order=order)
 
 
 # ------------------------------------------------------------------ operator
@@ -673,6 +932,51 @@ class ExactSummationSpins(_Ensemble):
         self.num_sites = int(num_sites)
 
 
+class ExactSummationPaulis(_Ensemble):
+    """ExactSummationPaulis(num_sites, gpu) -- include/ensembles/ExactSummation.hpp:125: the
+    4^N Pauli strings in PauliString.enumerate order."""
+
+    def __init__(self, num_sites, gpu=True):
+        self._h = handle()
+        check(lib.ann_exact_summation_paulis_create(ref(self._h), int(num_sites)))
+        self.num_sites = int(num_sites)
+
+
+class MonteCarloPaulis(_Ensemble):
+    """MonteCarloPaulis(num_samples, num_sweeps, num_thermalization_sweeps, num_markov_chains, gpu)
+    -- include/ensembles/MonteCarlo.hpp:265-283: Metropolis moves that set a random site to a
+    random Pauli type (policies/Update_Policy.hpp:41-56), num_sweeps x num_sites moves
+    between samples.  Additive: ``seed``."""
+
+    def __init__(self, num_samples, num_sweeps, num_thermalization_sweeps, num_markov_chains, gpu=True, seed=0):
+        self._h = handle()
+        check(lib.ann_monte_carlo_paulis_create(ref(self._h), int(num_samples), int(num_sweeps),
+                                                int(num_thermalization_sweeps), int(num_markov_chains),
+                                                int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    @property
+    def acceptance_rate(self):
+        a, r = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib.ann_ensemble_acceptance(self._h, ref(a), ref(r)))
+        tot = a.value + r.value
+        return a.value / tot if tot else float("nan")
+
+
+class PauliSampleList(_Ensemble):
+    """Additive: an ensemble of given Pauli strings (``PauliString`` objects or rows (a, b))
+    with weights (default 1 / count)."""
+
+    def __init__(self, num_sites, strings, weights=None, gpu=True):
+        rows = [(s.a, s.b) if isinstance(s, PauliString) else (int(s[0]), int(s[1])) for s in strings]
+        arr = np.ascontiguousarray(np.array(rows, dtype=np.uint64).reshape(-1, 2))
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64).ravel()
+        if w is not None and w.size != arr.shape[0]:
+            raise ValueError("one weight per configuration")
+        self._h = handle()
+        check(lib.ann_pauli_sample_list_create(ref(self._h), int(num_sites), ptr(arr), arr.shape[0], ptr(w)))
+        self.num_sites = int(num_sites)
+
+
 class SampleList(_Ensemble):
     """Additive (no counterpart in the reference): an ensemble of given configurations.
 
@@ -704,18 +1008,22 @@ class SampleList(_Ensemble):
 # --------------------------------------------------------- network functions
 
 def log_psi_s(psi, spins):
-    """PsiVector.cu.template:101-133"""
-    c = _conf_words(spins)
+    """PsiVector.cu.template:101-133 (spins or a PauliString)"""
     out = np.zeros(2)
-    check(lib.ann_log_psi_s(psi._h, ptr(c), ptr(out)))
+    if isinstance(spins, PauliString):
+        check(lib.ann_log_psi_s_pauli(psi._h, ptr(spins.words()), ptr(out)))
+    else:
+        check(lib.ann_log_psi_s(psi._h, ptr(_conf_words(spins)), ptr(out)))
     return complex(out[0], out[1])
 
 
 def psi_O_k(psi, spins):
-    """PsiOkVector.cu.template:40-74"""
-    c = _conf_words(spins)
+    """PsiOkVector.cu.template:40-74 (spins or a PauliString)"""
     out = np.zeros(psi.num_params, dtype=np.complex128)
-    check(lib.ann_psi_O_k(psi._h, ptr(c), ptr(out)))
+    if isinstance(spins, PauliString):
+        check(lib.ann_psi_O_k_pauli(psi._h, ptr(spins.words()), ptr(out)))
+    else:
+        check(lib.ann_psi_O_k(psi._h, ptr(_conf_words(spins)), ptr(out)))
     return out
 
 
@@ -835,6 +1143,70 @@ class HilbertSpaceDistance:
         check(lib.ann_hilbert_space_distance(psi._h, psi_prime._h, operator_._h, 1 if is_unitary else 0,
                                              spin_ensemble._h, float(nu), ptr(g), ref(d)))
         return g, d.value
+
+
+class KullbackLeibler:
+    """KullbackLeibler(num_params, gpu) -- network_functions/KullbackLeibler.hpp:14-117, bound at
+    MAIN:445-462.  psi is a classical network, psi_prime the network whose ensemble is
+    sampled and whose parameters the gradient is taken with respect to."""
+
+    def __init__(self, num_params, gpu=True):
+        self.num_params = int(num_params)
+        self._h = handle()
+        check(lib.ann_kl_create(ref(self._h), self.num_params))
+
+    def __del__(self):
+        if self._h is not None and self._h.value:
+            try:
+                lib.ann_kl_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _eval(self, psi, psi_prime, ensemble, threshold, nu, mode):
+        g = np.zeros(psi_prime.num_params, dtype=np.complex128) if mode > 0 else None
+        n = np.zeros(psi_prime.num_params, dtype=np.float64) if mode == 2 else None
+        v = ctypes.c_double()
+        check(lib.ann_kl_eval(self._h, psi._h, psi_prime._h, ensemble._h, float(threshold), float(nu), int(mode),
+                              ptr(g), ptr(n), ref(v)))
+        return g, n, v.value
+
+    def __call__(self, psi, psi_prime, ensemble, threshold=0.0):
+        """KullbackLeibler.cu.template:171-196"""
+        return self._eval(psi, psi_prime, ensemble, threshold, 1.0, 0)[2]
+
+    def gradient(self, psi, psi_prime, ensemble, nu, threshold=0.0):
+        """(gradient, value) -- KullbackLeibler.cu.template:199-242, gradient_py"""
+        g, _, v = self._eval(psi, psi_prime, ensemble, threshold, nu, 1)
+        return g, v
+
+    def gradient_with_noise(self, psi, psi_prime, ensemble, nu, threshold=0.0):
+        """(gradient, noise, value) -- KullbackLeibler.cu.template:245-316"""
+        return self._eval(psi, psi_prime, ensemble, threshold, nu, 2)
+
+    def _info(self):
+        w = ctypes.c_double()
+        m = np.zeros(2)
+        check(lib.ann_kl_info(self._h, ref(w), ptr(m)))
+        return w.value, complex(m[0], m[1])
+
+    @property
+    def total_weight(self):
+        return self._info()[0]
+
+    @property
+    def mean_deviation(self):
+        return self._info()[1]
+
+    @property
+    def log_psi_scale(self):
+        x = ctypes.c_double()
+        check(lib.ann_kl_get_log_psi_scale(self._h, ref(x)))
+        return x.value
+
+    @log_psi_scale.setter
+    def log_psi_scale(self, value):
+        check(lib.ann_kl_set_log_psi_scale(self._h, float(value)))
 
 
 class TDVP:

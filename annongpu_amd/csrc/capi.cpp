@@ -25,6 +25,9 @@ struct ann_ensemble_s {
 struct ann_tdvp_s {
     ann::Tdvp impl;
 };
+struct ann_kl_s {
+    ann::Kl impl;
+};
 struct ann_proposal_s {
     ann::Proposal impl;
 };
@@ -62,21 +65,14 @@ void need(const void* p, const char* what) {
     if (!p) throw std::invalid_argument(std::string("null pointer: ") + what);
 }
 
-ann::Samples& single_sample(ann::Psi& psi, const uint64_t* conf, ann::Samples& s) {
+ann::Samples& single_sample(ann::Psi& psi, const uint64_t* conf, ann::Samples& s, int basis = ann::BASIS_SPINS) {
     psi.sync_device();
-    s.count = 1;
-    s.words = psi.words;
-    s.Mpad = psi.dense() ? psi.Mpad : 0;
-    s.precision = psi.precision;
-    s.confs.upload(conf, psi.words);
-    s.w.ensure(1);
-    s.log_psi.ensure(1);
-    if (psi.dense()) {
-        if (psi.precision == 0)
-            s.th64.ensure(psi.Mpad);
-        else
-            s.th32.ensure(psi.Mpad);
+    ann::prepare_samples(psi, s, 1, basis);
+    if (basis == ann::BASIS_PAULIS) {
+        const uint64_t m = psi.num_sites >= 64 ? ~uint64_t(0) : (uint64_t(1) << psi.num_sites) - 1;
+        if ((conf[0] | conf[1]) & ~m) throw std::invalid_argument("a Pauli string has sites beyond num_sites");
     }
+    s.confs.upload(conf, s.words);
     ann::log_psi_of(psi, s.confs.p, 1, s, nullptr);
     return s;
 }
@@ -181,7 +177,8 @@ int ann_psi_create_deep(ann_psi* out, int32_t num_sites, int32_t N, const double
         need(sizes, "sizes");
         need(conns, "conns");
         if (num_layers < 1 || num_layers > 4) throw std::invalid_argument("PsiDeep supports 1..4 hidden layers");
-        if (N != num_sites) throw std::invalid_argument("only the spin basis (N == num_sites) is supported");
+        if (N != num_sites && !(N == 3 * num_sites && num_sites <= 64))
+            throw std::invalid_argument("N must be num_sites (spins) or 3 num_sites (Pauli strings, <= 64 sites)");
         auto p = new ann_psi_s;
         try {
             auto& q = p->impl;
@@ -268,6 +265,104 @@ int ann_psi_create_cnn(ann_psi* out, const int32_t* extent, int32_t num_layers, 
             throw;
         }
         *out = p;
+    });
+}
+
+int ann_psi_create_classical(ann_psi* out, int32_t num_sites, int32_t num_ops, const ann_operator* ops,
+                             const double* params, ann_psi ref, int32_t order, const double* log_prefactor,
+                             const double* ref_log_prefactor) {
+    return guard([&] {
+        need(out, "out");
+        if (num_sites <= 0 || num_sites > 64 * ann::kMaxWords) throw std::invalid_argument("num_sites must be 1..256");
+        if (num_ops < 0) throw std::invalid_argument("num_ops must not be negative");
+        if (num_ops > 0) {
+            need(ops, "ops");
+            need(params, "params");
+        }
+        if (order != 1 && order != 2) throw std::invalid_argument("order must be 1 or 2");
+        if (ref) {
+            if (ref->impl.kind != ann::PSI_DEEP) throw std::invalid_argument("the network reference must be a PsiDeep");
+            if (ref->impl.num_sites != num_sites) throw std::invalid_argument("psi_ref has another num_sites");
+        }
+        auto p = new ann_psi_s;
+        try {
+            auto& q = p->impl;
+            q.kind = ann::PSI_CLASSICAL;
+            q.num_sites = num_sites;
+            q.N = num_sites;
+            q.cl_order = order;
+            q.cl_begin.push_back(0);
+            for (int n = 0; n < num_ops; n++) {
+                need(ops[n], "ops[n]");
+                const ann::Operator& o = ops[n]->impl;
+                if (o.max_site() >= num_sites) throw std::invalid_argument("H_local acts on sites beyond num_sites");
+                for (int m = 0; m < o.num_strings; m++) {
+                    for (int w = 0; w < ann::kMaxWords; w++) {
+                        q.cl_a.push_back(o.a[size_t(m) * ann::kMaxWords + w]);
+                        q.cl_b.push_back(o.b[size_t(m) * ann::kMaxWords + w]);
+                    }
+                    q.cl_coef.push_back(o.coef[m]);
+                }
+                q.cl_begin.push_back(int(q.cl_coef.size()));
+            }
+            if (num_ops > 0) q.params.assign(C(params), C(params) + num_ops);
+            if (ref) q.ref = ref->impl.clone();
+            q.log_prefactor = log_prefactor ? *C(log_prefactor) : cd(0.0);
+            if (ref_log_prefactor) {
+                q.ref_log_prefactor = *C(ref_log_prefactor);
+                if (q.ref) q.ref->log_prefactor = q.ref_log_prefactor;
+            }
+            q.finalize();
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+int ann_psi_classical_info(ann_psi psi, int32_t* order, int32_t* num_ops, int32_t* has_network_ref) {
+    return guard([&] {
+        need(psi, "psi");
+        const auto& q = psi->impl;
+        if (q.kind != ann::PSI_CLASSICAL) throw std::invalid_argument("not a PsiClassical");
+        if (order) *order = q.cl_order;
+        if (num_ops) *num_ops = int32_t(q.params.size());
+        if (has_network_ref) *has_network_ref = q.ref ? 1 : 0;
+    });
+}
+
+int ann_psi_classical_ref(ann_psi psi, ann_psi* out) {
+    return guard([&] {
+        need(psi, "psi");
+        need(out, "out");
+        const auto& q = psi->impl;
+        if (q.kind != ann::PSI_CLASSICAL || !q.ref) throw std::invalid_argument("no network reference");
+        auto c = q.ref->clone();
+        auto p = new ann_psi_s;
+        p->impl = std::move(*c);
+        *out = p;
+    });
+}
+
+int ann_psi_get_ref_log_prefactor(ann_psi psi, double* value2) {
+    return guard([&] {
+        need(psi, "psi");
+        need(value2, "value");
+        const auto& q = psi->impl;
+        if (q.kind != ann::PSI_CLASSICAL) throw std::invalid_argument("not a PsiClassical");
+        *C(value2) = q.ref ? q.ref->log_prefactor : q.ref_log_prefactor;
+    });
+}
+
+int ann_psi_set_ref_log_prefactor(ann_psi psi, const double* value2) {
+    return guard([&] {
+        need(psi, "psi");
+        need(value2, "value");
+        auto& q = psi->impl;
+        if (q.kind != ann::PSI_CLASSICAL) throw std::invalid_argument("not a PsiClassical");
+        q.ref_log_prefactor = *C(value2);
+        if (q.ref) q.ref->log_prefactor = q.ref_log_prefactor;
     });
 }
 
@@ -443,6 +538,70 @@ int ann_sample_list_create(ann_ensemble* out, uint32_t num_sites, const uint64_t
     });
 }
 
+int ann_exact_summation_paulis_create(ann_ensemble* out, uint32_t num_sites) {
+    return guard([&] {
+        need(out, "out");
+        if (num_sites == 0 || num_sites > 13)
+            throw std::invalid_argument("exact summation over Pauli strings needs 1..13 sites");
+        auto p = new ann_ensemble_s;
+        p->impl.kind = ann::ENS_EXACT;
+        p->impl.basis = ann::BASIS_PAULIS;
+        p->impl.num_sites = num_sites;
+        *out = p;
+    });
+}
+
+int ann_monte_carlo_paulis_create(ann_ensemble* out, uint32_t num_samples, uint32_t num_sweeps,
+                                  uint32_t num_thermalization_sweeps, uint32_t num_markov_chains, uint64_t seed) {
+    return guard([&] {
+        need(out, "out");
+        if (num_markov_chains == 0) throw std::invalid_argument("num_markov_chains must be positive");
+        auto p = new ann_ensemble_s;
+        auto& e = p->impl;
+        e.kind = ann::ENS_MC;
+        e.basis = ann::BASIS_PAULIS;
+        e.num_samples = num_samples;
+        e.num_sweeps = num_sweeps;
+        e.num_therm = num_thermalization_sweeps;
+        e.num_chains = num_markov_chains;
+        e.move = ann::MOVE_PAULI;
+        e.seed = seed;
+        *out = p;
+    });
+}
+
+int ann_pauli_sample_list_create(ann_ensemble* out, uint32_t num_sites, const uint64_t* ab, int64_t count,
+                                 const double* weights) {
+    return guard([&] {
+        need(out, "out");
+        need(ab, "ab");
+        if (num_sites == 0 || num_sites > 64) throw std::invalid_argument("num_sites must be 1..64");
+        if (count <= 0 || count > int64_t(UINT32_MAX)) throw std::invalid_argument("count must be positive");
+        const uint64_t m = num_sites >= 64 ? ~uint64_t(0) : (uint64_t(1) << num_sites) - 1;
+        std::vector<uint64_t> list(size_t(count) * ann::kMaxWords, 0);
+        for (int64_t i = 0; i < count; i++) {
+            const uint64_t a = ab[2 * i], b = ab[2 * i + 1];
+            if ((a | b) & ~m) throw std::invalid_argument("a Pauli string has sites beyond num_sites");
+            list[size_t(i) * ann::kMaxWords] = a;
+            list[size_t(i) * ann::kMaxWords + 1] = b;
+        }
+        std::vector<double> wt(size_t(count), 1.0 / double(count));
+        if (weights)
+            for (int64_t i = 0; i < count; i++) {
+                if (!std::isfinite(weights[i]) || weights[i] < 0.0) throw std::invalid_argument("weights must be >= 0");
+                wt[size_t(i)] = weights[i];
+            }
+        auto p = new ann_ensemble_s;
+        p->impl.kind = ann::ENS_LIST;
+        p->impl.basis = ann::BASIS_PAULIS;
+        p->impl.num_sites = num_sites;
+        p->impl.num_samples = uint32_t(count);
+        p->impl.list_confs = std::move(list);
+        p->impl.list_w = std::move(wt);
+        *out = p;
+    });
+}
+
 int ann_ensemble_destroy(ann_ensemble ens) {
     return guard([&] { delete ens; });
 }
@@ -506,6 +665,31 @@ int ann_psi_O_k(ann_psi psi, const uint64_t* conf, double* out) {
         auto& q = psi->impl;
         ann::Samples s;
         single_sample(q, conf, s);
+        ann::DevBuf<ann::cplx<double>> coef;
+        const ann::cplx<double> one = {1.0, 0.0};
+        coef.upload(&one, 1);
+        structured_sum(q, s, coef.p, 0, C(out));
+    });
+}
+
+int ann_log_psi_s_pauli(ann_psi psi, const uint64_t* ab, double* out2) {
+    return guard([&] {
+        need(psi, "psi");
+        need(ab, "ab");
+        ann::Samples s;
+        single_sample(psi->impl, ab, s, ann::BASIS_PAULIS);
+        auto lp = host_log_psi(s);
+        *C(out2) = lp[0];
+    });
+}
+
+int ann_psi_O_k_pauli(ann_psi psi, const uint64_t* ab, double* out) {
+    return guard([&] {
+        need(psi, "psi");
+        need(ab, "ab");
+        auto& q = psi->impl;
+        ann::Samples s;
+        single_sample(q, ab, s, ann::BASIS_PAULIS);
         ann::DevBuf<ann::cplx<double>> coef;
         const ann::cplx<double> one = {1.0, 0.0};
         coef.upload(&one, 1);
@@ -763,6 +947,58 @@ int ann_hilbert_space_distance(ann_psi psi, ann_psi psi_prime, ann_operator op, 
 }
 
 // ---------------------------------------------------------------------- TDVP
+
+int ann_kl_create(ann_kl* out, uint32_t num_params) {
+    return guard([&] {
+        need(out, "out");
+        auto p = new ann_kl_s;
+        p->impl.P = int(num_params);
+        *out = p;
+    });
+}
+
+int ann_kl_destroy(ann_kl kl) {
+    return guard([&] { delete kl; });
+}
+
+int ann_kl_eval(ann_kl kl, ann_psi psi, ann_psi psi_prime, ann_ensemble ens, double threshold, double nu,
+                int32_t mode, double* gradient, double* noise, double* value) {
+    return guard([&] {
+        need(kl, "kl");
+        need(psi, "psi");
+        need(psi_prime, "psi_prime");
+        need(ens, "ens");
+        need(value, "value");
+        if (mode < 0 || mode > 2) throw std::invalid_argument("mode must be 0, 1 or 2");
+        if (psi->impl.kind != ann::PSI_CLASSICAL) throw std::invalid_argument("psi must be a PsiClassical");
+        *value = ann::kullback_leibler(kl->impl, psi->impl, psi_prime->impl, ens->impl, threshold, mode, nu,
+                                       gradient ? C(gradient) : nullptr, noise);
+    });
+}
+
+int ann_kl_info(ann_kl kl, double* total_weight, double* mean_deviation2) {
+    return guard([&] {
+        need(kl, "kl");
+        if (total_weight) *total_weight = kl->impl.total_weight;
+        if (mean_deviation2) *C(mean_deviation2) = kl->impl.mean_deviation;
+    });
+}
+
+int ann_kl_get_log_psi_scale(ann_kl kl, double* scale) {
+    return guard([&] {
+        need(kl, "kl");
+        need(scale, "scale");
+        *scale = kl->impl.log_psi_scale;
+    });
+}
+
+int ann_kl_set_log_psi_scale(ann_kl kl, double scale) {
+    return guard([&] {
+        need(kl, "kl");
+        if (!std::isfinite(scale)) throw std::invalid_argument("log_psi_scale must be finite");
+        kl->impl.log_psi_scale = scale;
+    });
+}
 
 int ann_tdvp_create(ann_tdvp* out, uint32_t num_params) {
     return guard([&] {

@@ -123,6 +123,100 @@ __device__ inline cplx<T> view_forward(const MlpView<T>& net, const uint64_t* w,
     return mlp_forward(net, w, Z, Z + net.zoff[net.L], lane);
 }
 
+// the 3 N input units of a Pauli-string configuration (a = w[0], b = w[1], N <= 64) as
+// spins: bit 3i + t - 1 set when site i holds Pauli type t = a_i + 2 b_i > 0
+__device__ inline void pauli_units(const uint64_t* w, int N, uint64_t* e) {
+#pragma unroll
+    for (int k = 0; k < kMaxWords; k++) e[k] = 0;
+    for (int i = 0; i < N; i++) {
+        const int t = int((w[0] >> i) & 1u) | (int((w[1] >> i) & 1u) << 1);
+        if (t) {
+            const int u = 3 * i + t - 1;
+            e[u >> 6] |= uint64_t(1) << (u & 63);
+        }
+    }
+}
+
+template <typename V>
+__device__ inline cplx<typename V::real_t> view_forward(const PauliInput<V>& net, const uint64_t* w,
+                                                        unsigned char* smem, int lane) {
+    uint64_t e[kMaxWords];
+    pauli_units(w, net.N, e);
+    return view_forward(net.net, e, smem, lane);
+}
+
+// matrix-element coefficient of string (pa, pb) applied to the Pauli string (xa, xb)
+// (PauliString.hpp:257-276): (-1)^{#sites with a negative Levi-Civita sign}
+// (-i)^{#sites where both are non-trivial and differ}
+__device__ inline cplx<double> pauli_phase(uint64_t pa, uint64_t pb, uint64_t xa, uint64_t xb) {
+    const uint64_t px = pa & ~pb, py = ~pa & pb, pz = pa & pb;
+    const uint64_t xx = xa & ~xb, xy = ~xa & xb, xz = xa & xb;
+    const int neg = __popcll((px & xz) | (py & xx) | (pz & xy)) & 1;
+    const int ne = __popcll((pa | pb) & (xa | xb) & ((pa ^ xa) | (pb ^ xb))) & 3;
+    double re = ne == 0 ? 1.0 : ne == 2 ? -1.0 : 0.0;
+    double im = ne == 1 ? -1.0 : ne == 3 ? 1.0 : 0.0;
+    if (neg) {
+        re = -re;
+        im = -im;
+    }
+    return {re, im};
+}
+
+// L_n(s) of PsiClassical (Operator::fast_local_energy over the strings of H_local[n])
+template <typename T>
+__device__ inline cplx<double> classical_term(const ClassicalView<T>& v, int n, const uint64_t* w) {
+    double re = 0.0, im = 0.0;
+    for (int m = v.op_begin[n]; m < v.op_begin[n + 1]; m++) {
+        cplx<double> f;
+        if (v.basis == BASIS_PAULIS) {
+            f = pauli_phase(v.sa[int64_t(m) * kMaxWords], v.sb[int64_t(m) * kMaxWords], w[0], w[1]);
+        } else {
+            // spin basis: (-1)^{#(Z or Y) on a down spin}; the (-i)^{#Y} is in coef
+            int par = 0;
+#pragma unroll
+            for (int k = 0; k < kMaxWords; k++) par += __popcll(~w[k] & v.sb[int64_t(m) * kMaxWords + k]);
+            f = {(par & 1) ? -1.0 : 1.0, 0.0};
+        }
+        const double cr = v.coef[2 * m], ci = v.coef[2 * m + 1];
+        re += cr * f.re - ci * f.im;
+        im += cr * f.im + ci * f.re;
+    }
+    return {re, im};
+}
+
+template <typename T>
+__device__ inline cplx<T> view_forward(const ClassicalView<T>& v, const uint64_t* w, unsigned char* smem, int lane) {
+    double re = 0.0, im = 0.0;
+    for (int n = lane; n < v.num_ops; n += 64) {
+        const cplx<double> L = classical_term(v, n, w);
+        const double pr = double(v.params[n].re), pi = double(v.params[n].im);
+        re += pr * L.re - pi * L.im;
+        im += pr * L.im + pi * L.re;
+    }
+    cplx<T> r = {T(wave_sum(re)), T(wave_sum(im))};
+    if (v.has_ref) {
+        r = cadd(r, v.ref_lp);
+        if (v.basis == BASIS_PAULIS) {
+            uint64_t e[kMaxWords];
+            pauli_units(w, v.N, e);
+            r = cadd(r, view_forward(v.ref, e, smem, lane));
+        } else {
+            r = cadd(r, view_forward(v.ref, w, smem, lane));
+        }
+    }
+    return r;
+}
+
+// the number of sites a Metropolis move picks from
+template <typename T>
+__device__ inline int view_sites(const CnnView<T>& v) { return v.N; }
+template <typename T>
+__device__ inline int view_sites(const MlpView<T>& v) { return v.N; }
+template <typename V>
+__device__ inline int view_sites(const PauliInput<V>& v) { return v.N; }
+template <typename T>
+__device__ inline int view_sites(const ClassicalView<T>& v) { return v.N; }
+
 // s is uniform over the wavefront, so these become scalar loads
 __device__ inline void load_words(const uint64_t* confs, int64_t s, int words, uint64_t* w) {
     for (int k = 0; k < kMaxWords; k++) w[k] = k < words ? confs[s * words + k] : 0;
@@ -158,7 +252,7 @@ __global__ __launch_bounds__(64) void generic_mc_kernel(V net, uint64_t* chain_c
     const int lane = threadIdx.x;
     const int c = blockIdx.x;
     if (c >= num_chains) return;
-    const int N = net.N;
+    const int N = view_sites(net);
     uint64_t w[kMaxWords];
     load_words(chain_confs, c, words, w);
     uint64_t msk[kMaxWords];
@@ -174,6 +268,24 @@ __global__ __launch_bounds__(64) void generic_mc_kernel(V net, uint64_t* chain_c
         const int i = int(range32(r.x, uint32_t(N)));
         const int si_bit = int((w[i >> 6] >> (i & 63)) & 1u);
         int kp = -1;
+        if (move == MOVE_PAULI) {
+            // site i takes Pauli type r.y >> 30 (possibly its current one)
+            const uint32_t ty = r.y >> 30;
+            const uint64_t bit = uint64_t(1) << i;
+            uint64_t wn[kMaxWords];
+            for (int k = 0; k < kMaxWords; k++) wn[k] = w[k];
+            wn[0] = (w[0] & ~bit) | ((ty & 1u) ? bit : 0);
+            wn[1] = (w[1] & ~bit) | ((ty & 2u) ? bit : 0);
+            const T nxt = view_forward(net, wn, smem, lane).re;
+            const T d2 = T(2) * (nxt - cur);
+            const T uu = (T(r.z) + T(0.5)) * T(2.3283064365386963e-10);
+            if (d2 > T(0) || uu <= dexp(d2)) {
+                for (int k = 0; k < kMaxWords; k++) w[k] = wn[k];
+                cur = nxt;
+                acc++;
+            }
+            continue;
+        }
         if (move == MOVE_EXCHANGE) {
             uint64_t o[kMaxWords];
             uint32_t n_opp = 0;
@@ -244,6 +356,43 @@ __global__ __launch_bounds__(64) void generic_local_energy_kernel(V net, OpView 
             sincos(d_im, &sn, &cs);
             e_re += c_re * m * cs - c_im * m * sn;
             e_im += c_re * m * sn + c_im * m * cs;
+        }
+        if (lane == 0) e_loc[s] = {e_re, e_im};
+        __syncthreads();
+    }
+}
+
+// E_loc on Pauli-string configurations: string n maps x to x' = (a_n ^ x.a, b_n ^ x.b)
+// with the coefficient c_n pauli_phase(string n, x); the identity string is diagonal
+template <typename V>
+__global__ __launch_bounds__(64) void generic_local_energy_pauli_kernel(V net, PauliOpView op, const uint64_t* confs,
+                                                                        int64_t count, cplx<double>* e_loc) {
+    using T = typename V::real_t;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
+        uint64_t w[kMaxWords];
+        load_words(confs, s, 2, w);
+        const cplx<T> L0 = view_forward(net, w, smem, lane);
+        double e_re = 0.0, e_im = 0.0;
+        for (int n = 0; n < op.num_strings; n++) {
+            const uint64_t pa = op.ab[2 * n], pb = op.ab[2 * n + 1];
+            const double cr = op.coef[2 * n], ci = op.coef[2 * n + 1];
+            if ((pa | pb) == 0) {
+                e_re += cr;
+                e_im += ci;
+                continue;
+            }
+            const cplx<double> f = pauli_phase(pa, pb, w[0], w[1]);
+            const double fr = cr * f.re - ci * f.im, fi = cr * f.im + ci * f.re;
+            uint64_t wn[kMaxWords] = {w[0] ^ pa, w[1] ^ pb, 0, 0};
+            const cplx<T> L1 = view_forward(net, wn, smem, lane);
+            const double d_re = double(L1.re) - double(L0.re), d_im = double(L1.im) - double(L0.im);
+            const double m = exp(d_re);
+            double sn, cs;
+            sincos(d_im, &sn, &cs);
+            e_re += fr * m * cs - fi * m * sn;
+            e_im += fr * m * sn + fi * m * cs;
         }
         if (lane == 0) e_loc[s] = {e_re, e_im};
         __syncthreads();
@@ -554,6 +703,24 @@ __global__ __launch_bounds__(64) void cnn_mc_inc_kernel(CnnView<T> net, uint64_t
         const int i = int(range32(r.x, uint32_t(N)));
         const int si_bit = int((w[i >> 6] >> (i & 63)) & 1u);
         int kp = -1;
+        if (move == MOVE_PAULI) {
+            // site i takes Pauli type r.y >> 30 (possibly its current one)
+            const uint32_t ty = r.y >> 30;
+            const uint64_t bit = uint64_t(1) << i;
+            uint64_t wn[kMaxWords];
+            for (int k = 0; k < kMaxWords; k++) wn[k] = w[k];
+            wn[0] = (w[0] & ~bit) | ((ty & 1u) ? bit : 0);
+            wn[1] = (w[1] & ~bit) | ((ty & 2u) ? bit : 0);
+            const T nxt = view_forward(net, wn, smem, lane).re;
+            const T d2 = T(2) * (nxt - cur);
+            const T uu = (T(r.z) + T(0.5)) * T(2.3283064365386963e-10);
+            if (d2 > T(0) || uu <= dexp(d2)) {
+                for (int k = 0; k < kMaxWords; k++) w[k] = wn[k];
+                cur = nxt;
+                acc++;
+            }
+            continue;
+        }
         if (move == MOVE_EXCHANGE) {
             uint64_t o[kMaxWords];
             uint32_t n_opp = 0;
@@ -634,59 +801,112 @@ __global__ __launch_bounds__(64) void cnn_local_energy_inc_kernel(CnnView<T> net
 // O_k of a multi-layer PsiDeep by back-propagation (PsiDeep.hpp:345-445); row s of O is
 // written in parameter order (minus center if given).  LDS: Z, Act (forward), D (the
 // derivative of log psi by the pre-activations of every layer).
+// O_k of a PsiDeep for the spins w into row[0 .. num_params) (minus center), by
+// back-propagation; LDS: Z, Act and D, zoff[L] values each
 template <typename T>
-__global__ __launch_bounds__(64) void mlp_ok_kernel(MlpView<T> net, const uint64_t* confs, int words, int64_t count,
-                                                    const cplx<double>* center, cplx<T>* O) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ void mlp_ok_row(const MlpView<T>& net, const uint64_t* w, unsigned char* smem, cplx<T>* row,
+                           const cplx<double>* center, int lane) {
     cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
     cplx<T>* Act = Z + net.zoff[net.L];
     cplx<T>* D = Act + net.zoff[net.L];
-    const int lane = threadIdx.x;
-    auto put = [&](cplx<T>* row, int k, cplx<T> v) {
+    auto put = [&](int k, cplx<T> v) {
         if (center) {
             v.re -= T(center[k].re);
             v.im -= T(center[k].im);
         }
         row[k] = v;
     };
+    mlp_forward(net, w, Z, Act, lane);
+    auto spin = [&](int i) {
+        const uint64_t word = i < 64 ? w[0] : i < 128 ? w[1] : i < 192 ? w[2] : w[3];
+        return ((word >> (i & 63)) & 1) ? T(1) : T(-1);
+    };
+    for (int i = lane; i < net.N; i += 64) put(i, {spin(i), T(0)});   // a: O = s_i
+    const int last = net.L - 1;
+    for (int j = lane; j < net.size[last]; j += 64)
+        D[net.zoff[last] + j] = cmul(net.f[j], dact(Z[net.zoff[last] + j], last));
+    __syncthreads();
+    for (int l = last; l >= 0; l--) {
+        const int M = net.size[l], C = net.conn[l];
+        for (int j = lane; j < M; j += 64) {
+            const cplx<T> dj = D[net.zoff[l] + j];
+            put(net.poff[l] + j, dj);
+            for (int i = 0; i < C; i++) {
+                const int src = int(net.cn[l][i * M + j]);
+                const cplx<T> in = l == 0 ? cplx<T>{spin(src), T(0)} : Act[net.zoff[l - 1] + src];
+                put(net.poff[l] + M + i * M + j, cmul(dj, in));
+            }
+        }
+        if (l > 0) {
+            // D of layer l - 1: the weights reading each unit, gathered in a fixed order
+            for (int i = lane; i < net.size[l - 1]; i += 64) {
+                cplx<T> acc = {0, 0};
+                for (int e = net.rb[l][i]; e < net.rb[l][i + 1]; e++) {
+                    const int k = net.ri[l][e];
+                    acc = cadd(acc, cmul(net.W[l][k], D[net.zoff[l] + k % M]));
+                }
+                D[net.zoff[l - 1] + i] = cmul(acc, dact(Z[net.zoff[l - 1] + i], l - 1));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void mlp_ok_kernel(MlpView<T> net, const uint64_t* confs, int words, int64_t count,
+                                                    const cplx<double>* center, cplx<T>* O) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
     for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
         uint64_t w[kMaxWords];
         load_words(confs, s, words, w);
-        mlp_forward(net, w, Z, Act, lane);
-        cplx<T>* row = O + s * net.num_params;
-        auto spin = [&](int i) {
-            const uint64_t word = i < 64 ? w[0] : i < 128 ? w[1] : i < 192 ? w[2] : w[3];
-            return ((word >> (i & 63)) & 1) ? T(1) : T(-1);
-        };
-        for (int i = lane; i < net.N; i += 64) put(row, i, {spin(i), T(0)});   // a: O = s_i
-        const int last = net.L - 1;
-        for (int j = lane; j < net.size[last]; j += 64)
-            D[net.zoff[last] + j] = cmul(net.f[j], dact(Z[net.zoff[last] + j], last));
-        __syncthreads();
-        for (int l = last; l >= 0; l--) {
-            const int M = net.size[l], C = net.conn[l];
-            for (int j = lane; j < M; j += 64) {
-                const cplx<T> dj = D[net.zoff[l] + j];
-                put(row, net.poff[l] + j, dj);
-                for (int i = 0; i < C; i++) {
-                    const int src = int(net.cn[l][i * M + j]);
-                    const cplx<T> in = l == 0 ? cplx<T>{spin(src), T(0)} : Act[net.zoff[l - 1] + src];
-                    put(row, net.poff[l] + M + i * M + j, cmul(dj, in));
-                }
+        mlp_ok_row(net, w, smem, O + s * net.num_params, center, lane);
+    }
+}
+
+// the same for Pauli-string configurations read through 3 units per site
+template <typename T>
+__global__ __launch_bounds__(64) void mlp_ok_pauli_kernel(PauliInput<MlpView<T>> pn, const uint64_t* confs,
+                                                          int64_t count, const cplx<double>* center, cplx<T>* O) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
+        uint64_t w[kMaxWords], e[kMaxWords];
+        load_words(confs, s, 2, w);
+        pauli_units(w, pn.N, e);
+        mlp_ok_row(pn.net, e, smem, O + s * pn.num_params, center, lane);
+    }
+}
+
+// O_k of PsiClassical: L_n(s) for n < num_ops, then the reference network's O_k
+template <typename T>
+__global__ __launch_bounds__(64) void classical_ok_kernel(ClassicalView<T> v, const uint64_t* confs, int words,
+                                                          int64_t count, const cplx<double>* center, cplx<T>* O) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
+        uint64_t w[kMaxWords];
+        load_words(confs, s, words, w);
+        cplx<T>* row = O + s * v.num_params;
+        for (int n = lane; n < v.num_ops; n += 64) {
+            cplx<double> L = classical_term(v, n, w);
+            if (center) {
+                L.re -= center[n].re;
+                L.im -= center[n].im;
             }
-            if (l > 0) {
-                // D of layer l - 1: the weights reading each unit, gathered in a fixed order
-                for (int i = lane; i < net.size[l - 1]; i += 64) {
-                    cplx<T> acc = {0, 0};
-                    for (int e = net.rb[l][i]; e < net.rb[l][i + 1]; e++) {
-                        const int k = net.ri[l][e];
-                        acc = cadd(acc, cmul(net.W[l][k], D[net.zoff[l] + k % M]));
-                    }
-                    D[net.zoff[l - 1] + i] = cmul(acc, dact(Z[net.zoff[l - 1] + i], l - 1));
-                }
-            }
-            __syncthreads();
+            row[n] = {T(L.re), T(L.im)};
         }
+        if (v.has_ref) {
+            const cplx<double>* c2 = center ? center + v.num_ops : nullptr;
+            if (v.basis == BASIS_PAULIS) {
+                uint64_t e[kMaxWords];
+                pauli_units(w, v.N, e);
+                mlp_ok_row(v.ref, e, smem, row + v.num_ops, c2, lane);
+            } else {
+                mlp_ok_row(v.ref, w, smem, row + v.num_ops, c2, lane);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -721,6 +941,22 @@ size_t ok_lds(const CnnView<T>& net) {
 template <typename T>
 size_t ok_lds(const MlpView<T>& net) {
     return sizeof(cplx<T>) * 3 * size_t(net.zoff[net.L]);
+}
+template <typename V>
+size_t forward_lds(const PauliInput<V>& net) {
+    return forward_lds(net.net);
+}
+template <typename V>
+size_t ok_lds(const PauliInput<V>& net) {
+    return ok_lds(net.net);
+}
+template <typename T>
+size_t forward_lds(const ClassicalView<T>& v) {
+    return v.has_ref ? forward_lds(v.ref) : 0;
+}
+template <typename T>
+size_t ok_lds(const ClassicalView<T>& v) {
+    return v.has_ref ? ok_lds(v.ref) : 0;
 }
 
 template <typename V>
@@ -778,10 +1014,26 @@ void launch_generic_ok(const V& net, const uint64_t* confs, int words, int64_t c
     if constexpr (std::is_same<V, CnnView<T>>::value) {
         allow_lds(cnn_ok_kernel<T>, ok_lds(net));
         cnn_ok_kernel<T><<<grid_for(count), 64, ok_lds(net), stream>>>(net, confs, words, count, center, O);
-    } else {
+    } else if constexpr (std::is_same<V, MlpView<T>>::value) {
         allow_lds(mlp_ok_kernel<T>, ok_lds(net));
         mlp_ok_kernel<T><<<grid_for(count), 64, ok_lds(net), stream>>>(net, confs, words, count, center, O);
+    } else if constexpr (std::is_same<V, PauliInput<MlpView<T>>>::value) {
+        allow_lds(mlp_ok_pauli_kernel<T>, ok_lds(net));
+        mlp_ok_pauli_kernel<T><<<grid_for(count), 64, ok_lds(net), stream>>>(net, confs, count, center, O);
+    } else {
+        allow_lds(classical_ok_kernel<T>, ok_lds(net));
+        classical_ok_kernel<T><<<grid_for(count), 64, ok_lds(net), stream>>>(net, confs, words, count, center, O);
     }
+    ANN_LAUNCH_CHECK();
+}
+
+template <typename V>
+void launch_generic_local_energy_pauli(const V& net, const PauliOpView& op, const uint64_t* confs, int64_t count,
+                                       cplx<double>* e_loc, hipStream_t stream) {
+    if (count <= 0) return;
+    allow_lds(generic_local_energy_pauli_kernel<V>, forward_lds(net));
+    generic_local_energy_pauli_kernel<V><<<grid_for(count), 64, forward_lds(net), stream>>>(net, op, confs, count,
+                                                                                           e_loc);
     ANN_LAUNCH_CHECK();
 }
 
@@ -799,5 +1051,18 @@ ANN_GENERIC_INSTANTIATE(CnnView<float>)
 ANN_GENERIC_INSTANTIATE(CnnView<double>)
 ANN_GENERIC_INSTANTIATE(MlpView<float>)
 ANN_GENERIC_INSTANTIATE(MlpView<double>)
+ANN_GENERIC_INSTANTIATE(PauliInput<MlpView<float>>)
+ANN_GENERIC_INSTANTIATE(PauliInput<MlpView<double>>)
+ANN_GENERIC_INSTANTIATE(ClassicalView<float>)
+ANN_GENERIC_INSTANTIATE(ClassicalView<double>)
+
+#define ANN_PAULI_INSTANTIATE(V)                                                                          \
+    template void launch_generic_local_energy_pauli<V>(const V&, const PauliOpView&, const uint64_t*, int64_t, \
+                                                       cplx<double>*, hipStream_t);
+
+ANN_PAULI_INSTANTIATE(PauliInput<MlpView<float>>)
+ANN_PAULI_INSTANTIATE(PauliInput<MlpView<double>>)
+ANN_PAULI_INSTANTIATE(ClassicalView<float>)
+ANN_PAULI_INSTANTIATE(ClassicalView<double>)
 
 }  // namespace ann

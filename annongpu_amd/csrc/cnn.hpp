@@ -63,6 +63,42 @@ struct MlpView {
     const int* ri[kMaxMlpLayers];
 };
 
+// A PsiDeep reading Pauli-string configurations (w[0] = a, w[1] = b over N sites)
+// through three input units per site: unit 3i + t - 1 is +1 when site i holds Pauli
+// type t (1 = X, 2 = Y, 3 = Z) and -1 otherwise (PauliString::network_unit_at,
+// include/basis/PauliString.hpp:84-90; PsiDeep.hpp:283-305).
+template <typename V>
+struct PauliInput {
+    using real_t = typename V::real_t;
+    V net;             // the network over 3 N input units
+    int N;             // Pauli sites
+    int num_params;
+};
+
+// PsiClassical (include/quantum_state/PsiClassical.hpp:48-160):
+//   log psi(s) = sum_n params[n] L_n(s) [+ log psi_ref(s) when the reference enters],
+//   L_n(s) = sum over the strings m of H_local[n] of c_m <s|P_m|s'> (Operator::fast_local_energy,
+//   operator/Operator.hpp:123-135), the matrix-element coefficient of PauliString::apply on
+//   spins (PauliString.hpp:242-254) or on Pauli strings (:257-276).
+// O_k = L_k(s) for k < num_ops, then the reference's O_k.  The fully polarised reference
+// (PsiFullyPolarized.hpp:31-40) has log psi_ref = 0 and no parameters: has_ref = 0.
+template <typename T>
+struct ClassicalView {
+    using real_t = T;
+    int N;               // sites
+    int basis;           // Basis
+    int num_ops;
+    int num_params;      // num_ops + the reference's parameters when has_ref
+    int has_ref;         // a network reference enters log psi (order 2)
+    const int* op_begin;     // [num_ops + 1] string ranges
+    const uint64_t* sa;      // [strings][kMaxWords]
+    const uint64_t* sb;      // [strings][kMaxWords]
+    const double* coef;      // [strings][2]: c_m, times (-i)^{#Y of P_m} for the spin basis
+    const cplx<T>* params;   // [num_ops]
+    MlpView<T> ref;          // over N spins, or 3 N units of a Pauli-string configuration
+    cplx<T> ref_lp;          // the reference's log_prefactor
+};
+
 // Kernels of the wave functions without a structured O (V = CnnView<T> or MlpView<T>):
 // one 64-lane wavefront per configuration, a full forward pass per configuration,
 // the layer state in LDS.
@@ -81,13 +117,21 @@ template <typename V>
 void launch_generic_ok(const V& net, const uint64_t* confs, int words, int64_t count, const cplx<double>* center,
                        cplx<typename V::real_t>* O, hipStream_t stream);
 
+// E_loc on Pauli-string configurations (confs [count][2]) for the views that read them
+// (PauliInput<MlpView<T>>, ClassicalView<T> with basis BASIS_PAULIS):
+// E_loc(x) = sum_n c_n phase_n(x) psi(x_n') / psi(x), the identity string adding c_n
+// (operator/Operator.hpp:36-84 with PauliString::apply on Pauli strings)
+template <typename V>
+void launch_generic_local_energy_pauli(const V& net, const PauliOpView& op, const uint64_t* confs, int64_t count,
+                                       cplx<double>* e_loc, hipStream_t stream);
+
 // ------------------------------------------------------- materialised O [n][P]
 // u[s] = sum_k O[s][k] v[k]   (fp64 accumulation)
 template <typename T>
 void launch_ok_rowdot(const cplx<T>* O, int64_t n, int P, const cplx<double>* v, cplx<double>* u,
                       hipStream_t stream);
 // out[k] = sum_s coef[s] * op(O[s][k]) + shift * v[k]; op = identity (mode 0),
-// conjugate (mode 1) or |.|^2 (mode 2, real part only).  partial holds
+// conjugate (mode 1) or |.|^2 (mode 2).  partial holds
 // col_sum_partials(n) * P complex values.
 int64_t col_sum_partials(int64_t n);
 template <typename T>

@@ -34,6 +34,19 @@ __global__ void enumerate_kernel(uint64_t* confs, int64_t first, int64_t count, 
     for (int w = 1; w < words; w++) confs[i * words + w] = 0;
 }
 
+__global__ void enumerate_paulis_kernel(uint64_t* confs, int64_t first, int64_t count) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint64_t x = uint64_t(first + i);
+    uint64_t a = 0, b = 0;
+    for (int k = 0; k < 32; k++) {
+        a |= ((x >> (2 * k)) & 1u) << k;
+        b |= ((x >> (2 * k + 1)) & 1u) << k;
+    }
+    confs[2 * i] = a;
+    confs[2 * i + 1] = b;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void theta_valu_kernel(DenseView<T> net, const uint64_t* confs,
                                                             int words, int64_t count, cplx<T>* theta) {
@@ -675,6 +688,12 @@ void launch_enumerate_confs(uint64_t* confs, int64_t first_index, int64_t count,
                             hipStream_t stream) {
     if (count <= 0) return;
     enumerate_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(confs, first_index, count, words);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_enumerate_paulis(uint64_t* confs, int64_t first_index, int64_t count, hipStream_t stream) {
+    if (count <= 0) return;
+    enumerate_paulis_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(confs, first_index, count);
     ANN_LAUNCH_CHECK();
 }
 

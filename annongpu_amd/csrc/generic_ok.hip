@@ -48,7 +48,9 @@ __global__ __launch_bounds__(256) void ok_colsum_kernel(const cplx<T>* __restric
         const cplx<double> c = coef[s];
         const double ore = double(o.re), oim = mode == 1 ? -double(o.im) : double(o.im);
         if (mode == 2) {
-            re += c.re * (ore * ore + oim * oim);
+            const double m2 = ore * ore + oim * oim;
+            re += c.re * m2;
+            im += c.im * m2;
         } else {
             re += c.re * ore - c.im * oim;
             im += c.re * oim + c.im * ore;

@@ -13,7 +13,13 @@
 
 namespace ann {
 
-enum MoveKind : int { MOVE_FLIP = 0, MOVE_EXCHANGE = 1 };
+// MOVE_PAULI: a random site of a Pauli-string configuration set to a uniformly random
+// Pauli type, the current one included (include/ensembles/policies/Update_Policy.hpp:41-56)
+enum MoveKind : int { MOVE_FLIP = 0, MOVE_EXCHANGE = 1, MOVE_PAULI = 2 };
+// Configurations: spins (bit i = site i up, words = ceil(N / 64)) or Pauli strings of
+// at most 64 sites, stored as the two words (a, b): site i holds I (0, 0), X (1, 0),
+// Y (0, 1) or Z (1, 1) (include/basis/PauliString.hpp:18-72)
+enum Basis : int { BASIS_SPINS = 0, BASIS_PAULIS = 1 };
 enum ParamLayout : int { LAYOUT_RBM = 0, LAYOUT_DEEP1 = 1 };
 
 template <typename T>
@@ -66,11 +72,22 @@ struct OpView {
     const double* string_coef;       // [num_strings][2] coefficient * (-i)^{#Y}
 };
 
+// Pauli-sum operator acting on Pauli-string configurations (string n maps x to
+// (a_n ^ x.a, b_n ^ x.b) with the phase of PauliString::apply, PauliString.hpp:257-276)
+struct PauliOpView {
+    int num_strings;
+    const uint64_t* ab;              // [num_strings][2]: a, b of the string
+    const double* coef;              // [num_strings][2]
+};
+
 // ---------------------------------------------------------------- launches
 // Every wrapper launches on `stream` and throws HipError on a launch failure.
 
 void launch_enumerate_confs(uint64_t* confs, int64_t first_index, int64_t count, int words,
                             hipStream_t stream);
+// Pauli strings by index (PauliString::enumerate, PauliString.hpp:30-35): site i takes
+// bit 2i of the index as its a bit and bit 2i + 1 as its b bit; confs [count][2]
+void launch_enumerate_paulis(uint64_t* confs, int64_t first_index, int64_t count, hipStream_t stream);
 
 template <typename T>
 void launch_theta_valu(const DenseView<T>& net, const uint64_t* confs, int words, int64_t count,
@@ -231,6 +248,13 @@ void launch_weighted_sums(const cplx<double>* e_loc, const double* w, int64_t co
 void launch_hsd_terms(const cplx<double>* e_loc, const cplx<double>* lp, const cplx<double>* lpp, const double* w,
                       int64_t n, bool unitary, cplx<double>* c_omega, cplx<double>* c_ratio, double* sums4,
                       hipStream_t stream);
+
+// KullbackLeibler per-sample terms on psi' samples: c_w = weight, c_d = weight dev mask,
+// c_d2 = weight |dev|^2 mask and sums6 = [sum weight, sum weight (l' - l) (re, im),
+// sum weight dev mask (re, im), sum weight |dev|^2 mask]
+void launch_kl_terms(const cplx<double>* lp, const cplx<double>* lpp, const double* wp, int64_t n, double scale,
+                     cplx<double> last_mean_deviation, double threshold, cplx<double>* c_w, cplx<double>* c_d,
+                     cplx<double>* c_d2, double* sums6, hipStream_t stream);
 
 // dense Hermitian Gram S = O^H diag(w) O  (O row-major [count][P], complex T)
 template <typename T>

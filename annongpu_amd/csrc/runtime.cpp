@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <type_traits>
 
 namespace ann {
 
@@ -105,17 +106,45 @@ static hipStream_t S() { return Runtime::get().stream(); }
 
 // --------------------------------------------------------------------- Psi
 
-// wave functions without a structured O: f(view) with the PsiCNN or multi-layer PsiDeep
-// view of the wave function's precision
+// the wave functions a configuration basis admits: on Pauli strings PsiClassical and a
+// PsiDeep with three input units per site (PsiDeep.hpp:283-305); such a PsiDeep reads no
+// spins
+static void check_basis(const Psi& psi, int basis) {
+    if (basis == BASIS_PAULIS) {
+        if (psi.num_sites > 64) throw std::invalid_argument("Pauli-string configurations hold at most 64 sites");
+        if (psi.kind == PSI_CLASSICAL || psi.pauli_net()) return;
+        throw std::logic_error("on Pauli strings: PsiClassical, or a PsiDeep with 3 input units per site");
+    }
+    if (psi.pauli_net()) throw std::logic_error("this PsiDeep reads Pauli strings (3 input units per site)");
+}
+
+// the kernels without a structured O serve every wave function but a dense net on spins
+static bool generic_path(const Psi& psi, int basis) {
+    return !psi.dense() || basis != BASIS_SPINS || psi.pauli_net();
+}
+
+// f(view) with the generic view of the wave function in its precision and basis
 template <typename F>
-static void with_generic(Psi& psi, F&& f) {
+static void with_generic(Psi& psi, F&& f, int basis = BASIS_SPINS) {
+    check_basis(psi, basis);
+    const bool d = psi.precision == 0;
     if (psi.kind == PSI_CNN) {
-        if (psi.precision == 0)
+        if (d)
             f(psi.cnn64());
         else
             f(psi.cnn32());
+    } else if (psi.kind == PSI_CLASSICAL) {
+        if (d)
+            f(psi.classical64(basis));
+        else
+            f(psi.classical32(basis));
+    } else if (basis == BASIS_PAULIS) {
+        if (d)
+            f(PauliInput<MlpView<double>>{psi.mlp64(), psi.num_sites, psi.num_params});
+        else
+            f(PauliInput<MlpView<float>>{psi.mlp32(), psi.num_sites, psi.num_params});
     } else {
-        if (psi.precision == 0)
+        if (d)
             f(psi.mlp64());
         else
             f(psi.mlp32());
@@ -151,10 +180,13 @@ void Psi::finalize() {
             for (uint32_t c : conn)
                 if (int(c) >= N) throw std::invalid_argument("PsiDeep connection index out of range");
         }
+    } else if (kind == PSI_CLASSICAL) {
+        num_params = int(params.size()) + (ref_enters() ? ref->num_params : 0);
     } else {
         num_params = int(params.size());
     }
     dirty = true;
+    mlp_stale = true;
 }
 
 // a complex host vector into an fp64 and an fp32 device copy
@@ -163,6 +195,50 @@ static void up_both(DevBuf<cplx<double>>& d64, DevBuf<cplx<float>>& d32, const s
     std::vector<cplx<float>> t(h.size());
     for (size_t i = 0; i < h.size(); i++) t[i] = {float(h[i].real()), float(h[i].imag())};
     d32.upload(t.data(), t.size());
+}
+
+// the layer tables of the generic PsiDeep view: every layer's connections, weights and
+// biases and, for layers l >= 1, the weights reading each unit of layer l - 1
+void Psi::upload_mlp_tables() {
+    std::vector<uint32_t> cn;
+    std::vector<cd> wv, bv;
+    std::vector<int> rb, ri;
+    mlp_cn_off.assign(num_layers, 0);
+    mlp_w_off.assign(num_layers, 0);
+    mlp_b_off.assign(num_layers, 0);
+    mlp_rb_off.assign(num_layers, 0);
+    mlp_ri_off.assign(num_layers, 0);
+    for (int l = 0; l < num_layers; l++) {
+        const int Ml = sizes[l], Cl = conns[l];
+        mlp_cn_off[l] = int(cn.size());
+        mlp_w_off[l] = int(wv.size());
+        mlp_b_off[l] = int(bv.size());
+        cn.insert(cn.end(), conn_l[l].begin(), conn_l[l].end());
+        wv.insert(wv.end(), w_l[l].begin(), w_l[l].end());
+        bv.insert(bv.end(), b_l[l].begin(), b_l[l].end());
+        mlp_rb_off[l] = int(rb.size());
+        mlp_ri_off[l] = int(ri.size());
+        if (l > 0) {
+            const int prev = sizes[l - 1];
+            std::vector<std::vector<int>> by_unit(prev);
+            for (int i = 0; i < Cl; i++)
+                for (int j = 0; j < Ml; j++) by_unit[conn_l[l][size_t(i) * Ml + j]].push_back(i * Ml + j);
+            int e = 0;
+            for (int u = 0; u < prev; u++) {
+                rb.push_back(e);
+                ri.insert(ri.end(), by_unit[u].begin(), by_unit[u].end());
+                e += int(by_unit[u].size());
+            }
+            rb.push_back(e);
+        }
+    }
+    mlp_cn.upload(cn.data(), cn.size());
+    up_both(mlp_w64, mlp_w32, wv);
+    up_both(mlp_b64, mlp_b32, bv);
+    if (rb.empty()) rb.push_back(0);
+    if (ri.empty()) ri.push_back(0);
+    mlp_rb.upload(rb.data(), rb.size());
+    mlp_ri.upload(ri.data(), ri.size());
 }
 
 void Psi::sync_device() {
@@ -216,49 +292,40 @@ void Psi::sync_device() {
         dirty = false;
         return;
     }
-    if (!dense()) {
-        // multi-layer PsiDeep: every layer's connections, weights and biases, the final
-        // weights and, for layers l >= 1, the weights reading each unit of layer l - 1
-        std::vector<uint32_t> cn;
-        std::vector<cd> wv, bv;
-        std::vector<int> rb, ri;
-        mlp_cn_off.assign(num_layers, 0);
-        mlp_w_off.assign(num_layers, 0);
-        mlp_b_off.assign(num_layers, 0);
-        mlp_rb_off.assign(num_layers, 0);
-        mlp_ri_off.assign(num_layers, 0);
-        for (int l = 0; l < num_layers; l++) {
-            const int Ml = sizes[l], Cl = conns[l];
-            mlp_cn_off[l] = int(cn.size());
-            mlp_w_off[l] = int(wv.size());
-            mlp_b_off[l] = int(bv.size());
-            cn.insert(cn.end(), conn_l[l].begin(), conn_l[l].end());
-            wv.insert(wv.end(), w_l[l].begin(), w_l[l].end());
-            bv.insert(bv.end(), b_l[l].begin(), b_l[l].end());
-            mlp_rb_off[l] = int(rb.size());
-            mlp_ri_off[l] = int(ri.size());
-            if (l > 0) {
-                const int prev = sizes[l - 1];
-                std::vector<std::vector<int>> by_unit(prev);
-                for (int i = 0; i < Cl; i++)
-                    for (int j = 0; j < Ml; j++) by_unit[conn_l[l][size_t(i) * Ml + j]].push_back(i * Ml + j);
-                int e = 0;
-                for (int u = 0; u < prev; u++) {
-                    rb.push_back(e);
-                    ri.insert(ri.end(), by_unit[u].begin(), by_unit[u].end());
-                    e += int(by_unit[u].size());
-                }
-                rb.push_back(e);
-            }
+    if (kind == PSI_CLASSICAL) {
+        // classical parameters, the strings of every H_local[n] with their coefficients
+        // in both bases, and the reference network
+        up_both(cp64, cp32, params);
+        std::vector<int> begin(cl_begin.begin(), cl_begin.end());
+        cl_begin_d.upload(begin.data(), begin.size());
+        const size_t ns = cl_coef.size();
+        std::vector<uint64_t> a0(cl_a), b0(cl_b);
+        if (a0.empty()) a0.assign(kMaxWords, 0);
+        if (b0.empty()) b0.assign(kMaxWords, 0);
+        cl_a_d.upload(a0.data(), a0.size());
+        cl_b_d.upload(b0.data(), b0.size());
+        std::vector<double> cs(2 * std::max<size_t>(ns, 1), 0.0), cp(2 * std::max<size_t>(ns, 1), 0.0);
+        for (size_t m = 0; m < ns; m++) {
+            // (-i)^{#Y} of the string on spins (PauliString.hpp:189-206)
+            int ny = 0;
+            for (int k = 0; k < kMaxWords; k++)
+                ny += __builtin_popcountll(~cl_a[m * kMaxWords + k] & cl_b[m * kMaxWords + k]);
+            static const cd phase[4] = {cd(1, 0), cd(0, -1), cd(-1, 0), cd(0, 1)};
+            const cd c = cl_coef[m] * phase[ny & 3];
+            cs[2 * m] = c.real();
+            cs[2 * m + 1] = c.imag();
+            cp[2 * m] = cl_coef[m].real();
+            cp[2 * m + 1] = cl_coef[m].imag();
         }
-        mlp_cn.upload(cn.data(), cn.size());
-        up_both(mlp_w64, mlp_w32, wv);
-        up_both(mlp_b64, mlp_b32, bv);
+        cl_coef_s_d.upload(cs.data(), cs.size());
+        cl_coef_p_d.upload(cp.data(), cp.size());
+        dirty = false;
+        return;
+    }
+    if (!dense()) {
+        // multi-layer PsiDeep: every layer's tables, and the final weights
+        upload_mlp_tables();
         up_both(f64, f32, f);
-        if (rb.empty()) rb.push_back(0);
-        if (ri.empty()) ri.push_back(0);
-        mlp_rb.upload(rb.data(), rb.size());
-        mlp_ri.upload(ri.data(), ri.size());
         dirty = false;
         return;
     }
@@ -382,14 +449,63 @@ static MlpView<T> make_mlp_view(const Psi& q, const cplx<T>* w, const cplx<T>* b
     return v;
 }
 
+// a one-layer PsiDeep builds the generic tables on first use after each change (its
+// final weights are the dense net's, zero-padded beyond M)
+static void ensure_mlp(Psi& q) {
+    if (q.kind != PSI_DEEP) throw std::logic_error("not a PsiDeep");
+    q.sync_device();
+    if (q.dense() && q.mlp_stale) {
+        q.upload_mlp_tables();
+        q.mlp_stale = false;
+    }
+}
+
 MlpView<double> Psi::mlp64() {
-    sync_device();
+    ensure_mlp(*this);
     return make_mlp_view<double>(*this, mlp_w64.p, mlp_b64.p, f64.p);
 }
 
 MlpView<float> Psi::mlp32() {
-    sync_device();
+    ensure_mlp(*this);
     return make_mlp_view<float>(*this, mlp_w32.p, mlp_b32.p, f32.p);
+}
+
+template <typename T>
+static ClassicalView<T> make_classical_view(Psi& q, int basis, const cplx<T>* params) {
+    ClassicalView<T> v{};
+    v.N = q.num_sites;
+    v.basis = basis;
+    v.num_ops = int(q.params.size());
+    v.num_params = q.num_params;
+    v.has_ref = q.ref_enters() ? 1 : 0;
+    v.op_begin = q.cl_begin_d.p;
+    v.sa = q.cl_a_d.p;
+    v.sb = q.cl_b_d.p;
+    v.coef = basis == BASIS_PAULIS ? q.cl_coef_p_d.p : q.cl_coef_s_d.p;
+    v.params = params;
+    return v;
+}
+
+ClassicalView<double> Psi::classical64(int basis) {
+    sync_device();
+    auto v = make_classical_view<double>(*this, basis, cp64.p);
+    if (v.has_ref) {
+        ref->precision = precision;
+        v.ref = ref->mlp64();
+        v.ref_lp = {ref->log_prefactor.real(), ref->log_prefactor.imag()};
+    }
+    return v;
+}
+
+ClassicalView<float> Psi::classical32(int basis) {
+    sync_device();
+    auto v = make_classical_view<float>(*this, basis, cp32.p);
+    if (v.has_ref) {
+        ref->precision = precision;
+        v.ref = ref->mlp32();
+        v.ref_lp = {float(ref->log_prefactor.real()), float(ref->log_prefactor.imag())};
+    }
+    return v;
 }
 
 CnnView<double> Psi::cnn64() {
@@ -415,6 +531,10 @@ std::vector<cd> Psi::get_params() const {
         }
     } else {
         p = params;
+        if (ref_enters()) {
+            const auto r = ref->get_params();
+            p.insert(p.end(), r.begin(), r.end());
+        }
     }
     return p;
 }
@@ -436,9 +556,14 @@ void Psi::set_params(const cd* p) {
             W = w_l[0];
             b = b_l[0];
         }
+    } else if (kind == PSI_CLASSICAL) {
+        const size_t nc = params.size();
+        params.assign(p, p + nc);
+        if (ref_enters()) ref->set_params(p + nc);
     } else {
         params.assign(p, p + num_params);
     }
+    mlp_stale = true;
     if (dense() && !dirty) {
         // the previous staged copy must have left the page-locked buffer
         if (stage_ev.e) ANN_HIP_CHECK(hipEventSynchronize(stage_ev.e));
@@ -482,6 +607,13 @@ std::unique_ptr<Psi> Psi::clone() const {
     q->num_sym = num_sym;
     q->params = params;
     q->final_factor = final_factor;
+    q->cl_order = cl_order;
+    q->cl_begin = cl_begin;
+    q->cl_a = cl_a;
+    q->cl_b = cl_b;
+    q->cl_coef = cl_coef;
+    q->ref_log_prefactor = ref_log_prefactor;
+    if (ref) q->ref = ref->clone();
     q->dirty = true;
     return q;
 }
@@ -549,6 +681,25 @@ const DeviceOp& Operator::device(int words) {
     return ref;
 }
 
+const DevicePauliOp& Operator::device_pauli() {
+    if (pauli_compiled) return *pauli_compiled;
+    if (max_site() >= 64) throw std::invalid_argument("Pauli-string configurations hold at most 64 sites");
+    auto d = std::make_unique<DevicePauliOp>();
+    d->num_strings = num_strings;
+    std::vector<uint64_t> ab(2 * size_t(std::max(num_strings, 1)), 0);
+    std::vector<double> c(2 * size_t(std::max(num_strings, 1)), 0.0);
+    for (int n = 0; n < num_strings; n++) {
+        ab[2 * n] = a[size_t(n) * kMaxWords];
+        ab[2 * n + 1] = b[size_t(n) * kMaxWords];
+        c[2 * n] = coef[n].real();
+        c[2 * n + 1] = coef[n].imag();
+    }
+    d->ab.upload(ab.data(), ab.size());
+    d->coef.upload(c.data(), c.size());
+    pauli_compiled = std::move(d);
+    return *pauli_compiled;
+}
+
 // ------------------------------------------------------------------ sampling
 
 void compute_theta(Psi& psi, const uint64_t* confs, int64_t count, Samples& out, int64_t offset) {
@@ -566,16 +717,19 @@ static void require_dense(const Psi& psi) {
         throw std::logic_error("this operation needs a PsiRBM or a one-layer PsiDeep (a structured O_k)");
 }
 
-static void prepare_samples(Psi& psi, Samples& s, int64_t count) {
+void prepare_samples(Psi& psi, Samples& s, int64_t count, int basis) {
+    check_basis(psi, basis);
+    const bool gen = generic_path(psi, basis);
     s.count = count;
     s.w_uniform = false;
-    s.words = psi.words;
-    s.Mpad = psi.dense() ? psi.Mpad : 0;
+    s.basis = basis;
+    s.words = basis == BASIS_PAULIS ? 2 : psi.words;
+    s.Mpad = gen ? 0 : psi.Mpad;
     s.precision = psi.precision;
-    s.confs.ensure(size_t(count) * psi.words);
+    s.confs.ensure(size_t(count) * s.words);
     s.w.ensure(count);
     s.log_psi.ensure(count);
-    if (!psi.dense()) return;
+    if (gen) return;
     if (psi.precision == 0)
         s.th64.ensure(size_t(count) * psi.Mpad);
     else
@@ -585,8 +739,26 @@ static void prepare_samples(Psi& psi, Samples& s, int64_t count) {
 static int64_t exact_count(const Psi& psi, const Ensemble& ens) {
     if (int(ens.num_sites) != psi.num_sites)
         throw std::invalid_argument("ExactSummation num_sites does not match the wave function");
+    if (ens.basis == BASIS_PAULIS) {
+        if (ens.num_sites > 13) throw std::invalid_argument("exact summation over Pauli strings is limited to 13 sites");
+        return int64_t(1) << (2 * ens.num_sites);
+    }
     if (ens.num_sites > 26) throw std::invalid_argument("exact summation is limited to 26 sites");
     return int64_t(1) << ens.num_sites;
+}
+
+// configurations of an exact summation in index order
+static void enumerate_into(const Ensemble& ens, Samples& s, int64_t count) {
+    if (ens.basis == BASIS_PAULIS)
+        launch_enumerate_paulis(s.confs.p, 0, count, S());
+    else
+        launch_enumerate_confs(s.confs.p, 0, count, s.words, S());
+}
+
+// words per chain configuration and the number of sites a move picks from
+static void chain_shape(const Psi& psi, const Ensemble& ens, int* words, int* N) {
+    *words = ens.basis == BASIS_PAULIS ? 2 : psi.words;
+    *N = ens.basis == BASIS_PAULIS ? psi.num_sites : psi.N;
 }
 
 // chain set-up shared by the samplers: (re)initialises the chains unless they
@@ -596,7 +768,8 @@ static int64_t start_chains(const Psi& psi, Ensemble& ens, int64_t* steps) {
     if (B <= 0) throw std::invalid_argument("num_markov_chains must be positive");
     *steps = ens.num_samples / ens.num_chains;
     if (*steps <= 0) throw std::invalid_argument("num_samples must be at least num_markov_chains");
-    const int words = psi.words, N = psi.N;
+    int words = 0, N = 0;
+    chain_shape(psi, ens, &words, &N);
     const bool reinit = !ens.persistent || !ens.chains_valid || ens.chain_words != words || ens.chain_N != N;
     ens.chains.ensure(size_t(B) * words);
     ens.acc.ensure(B);
@@ -613,8 +786,7 @@ static int64_t start_chains(const Psi& psi, Ensemble& ens, int64_t* steps) {
 static void finish_chains(const Psi& psi, Ensemble& ens, int64_t therm, int64_t steps, int64_t per_step) {
     const int B = int(ens.num_chains);
     ens.chains_valid = true;
-    ens.chain_words = psi.words;
-    ens.chain_N = psi.N;
+    chain_shape(psi, ens, &ens.chain_words, &ens.chain_N);
     // no synchronisation here: the counters are read when the acceptance is asked for
     ens.acc_host.ensure(size_t(B));
     ANN_HIP_CHECK(hipMemcpyAsync(ens.acc_host.p, ens.acc.p, sizeof(uint32_t) * B, hipMemcpyDeviceToHost, S()));
@@ -663,11 +835,11 @@ void Ensemble::settle_acceptance() {
 // log psi (and |psi|^2 weights when w is given) of configurations, any wave function
 void log_psi_of(Psi& psi, const uint64_t* confs, int64_t count, Samples& s, double* w) {
     const cplx<double> lp = {psi.log_prefactor.real(), psi.log_prefactor.imag()};
-    if (!psi.dense()) {
+    if (generic_path(psi, s.basis)) {
         ScopedTimer timer("log_psi");
         with_generic(psi, [&](const auto& v) {
-            launch_generic_log_psi(v, confs, psi.words, count, lp, s.log_psi.p, w, S());
-        });
+            launch_generic_log_psi(v, confs, s.words, count, lp, s.log_psi.p, w, S());
+        }, s.basis);
         return;
     }
     require_dense(psi);
@@ -684,8 +856,8 @@ static void sample_list(Psi& psi, Ensemble& ens) {
         throw std::invalid_argument("the sample list was built for another number of sites");
     Samples& s = ens.samples;
     const int64_t count = ens.num_samples;
-    const int words = psi.words;
-    prepare_samples(psi, s, count);
+    prepare_samples(psi, s, count, ens.basis);
+    const int words = s.words;
     std::vector<uint64_t> packed(size_t(count) * words);
     for (int64_t i = 0; i < count; i++)
         for (int w = 0; w < words; w++) packed[size_t(i) * words + w] = ens.list_confs[size_t(i) * kMaxWords + w];
@@ -695,16 +867,19 @@ static void sample_list(Psi& psi, Ensemble& ens) {
     log_psi_of(psi, s.confs.p, count, s, nullptr);
 }
 
-// PsiCNN and multi-layer PsiDeep: Metropolis with a full forward pass per proposal
-// (the reference keeps no incremental state for them, include/quantum_state/PsiCNN.hpp:177-181)
+// PsiCNN, multi-layer PsiDeep, PsiClassical and every wave function on Pauli strings:
+// Metropolis with a full forward pass per proposal (the reference keeps no incremental
+// state for them, include/quantum_state/PsiCNN.hpp:177-181; PsiCNN uses local updates)
 static void sample_generic(Psi& psi, Ensemble& ens) {
+    check_basis(psi, ens.basis);
+    if (psi.kind == PSI_DEEP) ensure_mlp(psi);
     psi.sync_device();
     if (ens.kind == ENS_LIST) return sample_list(psi, ens);
     Samples& s = ens.samples;
     if (ens.kind == ENS_EXACT) {
         const int64_t count = exact_count(psi, ens);
-        prepare_samples(psi, s, count);
-        launch_enumerate_confs(s.confs.p, 0, count, psi.words, S());
+        prepare_samples(psi, s, count, ens.basis);
+        enumerate_into(ens, s, count);
         log_psi_of(psi, s.confs.p, count, s, s.w.p);
         return;
     }
@@ -713,15 +888,16 @@ static void sample_generic(Psi& psi, Ensemble& ens) {
     int64_t steps = 0;
     const int64_t therm = start_chains(psi, ens, &steps);
     const int64_t count = steps * B;
-    prepare_samples(psi, s, count);
-    const int words = psi.words;
+    prepare_samples(psi, s, count, ens.basis);
+    int words = 0, sites = 0;
+    chain_shape(psi, ens, &words, &sites);
     for (int64_t step = 0; step < steps; step++) {
-        const int64_t nprop = (step == 0 ? therm : 0) + int64_t(ens.num_sweeps) * psi.N;
+        const int64_t nprop = (step == 0 ? therm : 0) + int64_t(ens.num_sweeps) * sites;
         {
             ScopedTimer timer("mc_sweep");
             with_generic(psi, [&](const auto& v) {
                 launch_generic_mc(v, ens.chains.p, words, B, ens.move, nprop, ens.counter, ens.seed, ens.acc.p, S());
-            });
+            }, ens.basis);
         }
         ens.counter += uint64_t(nprop);
         ANN_HIP_CHECK(hipMemcpyAsync(s.confs.p + step * B * words, ens.chains.p, sizeof(uint64_t) * B * words,
@@ -730,11 +906,11 @@ static void sample_generic(Psi& psi, Ensemble& ens) {
     launch_fill(s.w.p, 1.0 / double(ens.num_samples), count, S());
     s.w_uniform = true;
     log_psi_of(psi, s.confs.p, count, s, nullptr);
-    finish_chains(psi, ens, therm, steps, int64_t(ens.num_sweeps) * psi.N);
+    finish_chains(psi, ens, therm, steps, int64_t(ens.num_sweeps) * sites);
 }
 
 void sample(Psi& psi, Ensemble& ens) {
-    if (!psi.dense()) return sample_generic(psi, ens);
+    if (generic_path(psi, ens.basis)) return sample_generic(psi, ens);
     require_dense(psi);
     psi.sync_device();
     if (ens.kind == ENS_LIST) return sample_list(psi, ens);
@@ -836,10 +1012,23 @@ void sample(Psi& psi, Ensemble& ens) {
 
 void local_energies(Psi& psi, Operator& op, Samples& s) {
     if (op.max_site() >= psi.num_sites) throw std::invalid_argument("operator acts on sites beyond num_sites");
-    const DeviceOp& d = op.device(psi.words);
     s.e_loc.ensure(s.count);
+    if (s.basis == BASIS_PAULIS) {
+        const DevicePauliOp& d = op.device_pauli();
+        ScopedTimer timer("local_energy");
+        with_generic(psi, [&](const auto& v) {
+            using V = std::decay_t<decltype(v)>;
+            using T = typename V::real_t;
+            if constexpr (std::is_same<V, ClassicalView<T>>::value || std::is_same<V, PauliInput<MlpView<T>>>::value)
+                launch_generic_local_energy_pauli(v, d.view(), s.confs.p, s.count, s.e_loc.p, S());
+            else
+                throw std::logic_error("this wave function does not read Pauli strings");
+        }, s.basis);
+        return;
+    }
+    const DeviceOp& d = op.device(psi.words);
     ScopedTimer timer("local_energy");
-    if (!psi.dense()) {
+    if (generic_path(psi, s.basis)) {
         with_generic(psi, [&](const auto& v) {
             launch_generic_local_energy(v, d.view(), s.confs.p, s.count, s.e_loc.p, S());
         });
@@ -1077,21 +1266,17 @@ DevBuf<cplx<T>>& materialised(Tdvp& t) {
         return t.Og32;
 }
 
-// O_k of a PsiCNN or multi-layer PsiDeep for `count` configurations into O [count][P]
+// O_k of a wave function on the generic path for `count` configurations into O [count][P]
 template <typename T>
-void generic_ok(Psi& psi, const uint64_t* confs, int64_t count, cplx<T>* O) {
+void generic_ok(Psi& psi, const uint64_t* confs, int words, int basis, int64_t count, cplx<T>* O) {
     ScopedTimer timer("o_k");
-    if constexpr (sizeof(T) == 8) {
-        if (psi.kind == PSI_CNN)
-            launch_generic_ok(psi.cnn64(), confs, psi.words, count, nullptr, O, S());
+    with_generic(psi, [&](const auto& v) {
+        using V = std::decay_t<decltype(v)>;
+        if constexpr (std::is_same<typename V::real_t, T>::value)
+            launch_generic_ok(v, confs, words, count, nullptr, O, S());
         else
-            launch_generic_ok(psi.mlp64(), confs, psi.words, count, nullptr, O, S());
-    } else {
-        if (psi.kind == PSI_CNN)
-            launch_generic_ok(psi.cnn32(), confs, psi.words, count, nullptr, O, S());
-        else
-            launch_generic_ok(psi.mlp32(), confs, psi.words, count, nullptr, O, S());
-    }
+            throw std::logic_error("precision mismatch");
+    }, basis);
 }
 
 // TDVP statistics from a materialised O (TDVP.cu.template:77-126,181-302)
@@ -1107,7 +1292,7 @@ void tdvp_eval_generic(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with
     launch_weighted_sums(s.e_loc.p, s.w.p, n, sums.p, S());
     auto& O = materialised<T>(t);
     O.ensure(size_t(n) * P);
-    generic_ok<T>(psi, s.confs.p, n, O.p);
+    generic_ok<T>(psi, s.confs.p, s.words, s.basis, n, O.p);
     DevBuf<cplx<double>> coef, sq;
     coef.ensure(n);
     sq.ensure(P);
@@ -1300,7 +1485,7 @@ namespace {
 template <typename T>
 void ok_weighted_sum_t(Psi& psi, Samples& s, const cplx<double>* coef, int conj, cplx<double>* out) {
     const int P = psi.num_params;
-    if (!psi.dense()) {
+    if (generic_path(psi, s.basis)) {
         // materialise O in row chunks of at most 256 MiB, accumulating into out
         const int64_t rows = std::max<int64_t>(1, (int64_t(256) << 20) / (int64_t(sizeof(cplx<T>)) * P));
         DevBuf<cplx<T>> O;
@@ -1310,7 +1495,7 @@ void ok_weighted_sum_t(Psi& psi, Samples& s, const cplx<double>* coef, int conj,
         ANN_HIP_CHECK(hipMemsetAsync(out, 0, sizeof(cplx<double>) * P, S()));
         for (int64_t r0 = 0; r0 < s.count; r0 += rows) {
             const int64_t m = std::min(rows, s.count - r0);
-            generic_ok<T>(psi, s.confs.p + r0 * s.words, m, O.p);
+            generic_ok<T>(psi, s.confs.p + r0 * s.words, s.words, s.basis, m, O.p);
             launch_ok_colsum<T>(O.p, m, P, coef + r0, conj, 1.0, out, part.p, out, S());
         }
         Runtime::get().sync();
@@ -1342,6 +1527,121 @@ void ok_weighted_sum(Psi& psi, Samples& s, const cplx<double>* coef, int conj, c
         ok_weighted_sum_t<float>(psi, s, coef, conj, out);
 }
 
+namespace {
+
+// out_i[k] = sum_s coef_i[s] op_i(O[s][k]) for several coefficient vectors at once, O
+// materialised in row chunks of at most 256 MiB (op: identity, conjugate or |.|^2)
+template <typename T>
+void ok_column_sums_t(Psi& psi, Samples& s, int nsum, const cplx<double>* const* coef, const int* mode,
+                      cplx<double>* const* out) {
+    const int P = psi.num_params;
+    const bool gen = generic_path(psi, s.basis);
+    DevBuf<cplx<T>> g;
+    DenseView<T> net{};
+    if (!gen) {
+        net = Prec<T>::view(psi);
+        g_factor_padded<T>(net, Prec<T>::th(s).p, s.count, g);
+    }
+    const int64_t rows = std::max<int64_t>(1, (int64_t(256) << 20) / (int64_t(sizeof(cplx<T>)) * P));
+    DevBuf<cplx<T>> O;
+    DevBuf<cplx<double>> part;
+    O.ensure(size_t(std::min(rows, s.count)) * P);
+    part.ensure(size_t(col_sum_partials(std::min(rows, s.count))) * P);
+    for (int i = 0; i < nsum; i++) ANN_HIP_CHECK(hipMemsetAsync(out[i], 0, sizeof(cplx<double>) * P, S()));
+    for (int64_t r0 = 0; r0 < s.count; r0 += rows) {
+        const int64_t m = std::min(rows, s.count - r0);
+        if (gen)
+            generic_ok<T>(psi, s.confs.p + r0 * s.words, s.words, s.basis, m, O.p);
+        else
+            launch_materialize_ok<T>(net, s.confs.p + r0 * s.words, s.words, g.p + r0 * net.Mpad, m, nullptr, O.p,
+                                     S());
+        for (int i = 0; i < nsum; i++)
+            launch_ok_colsum<T>(O.p, m, P, coef[i] + r0, mode[i], 1.0, out[i], part.p, out[i], S());
+    }
+    Runtime::get().sync();
+}
+
+}  // namespace
+
+double kullback_leibler(Kl& kl, Psi& psi, Psi& psi_prime, Ensemble& ens, double threshold, int mode, double nu,
+                        cd* grad, double* noise) {
+    if (psi.num_sites != psi_prime.num_sites) throw std::invalid_argument("psi and psi_prime differ in num_sites");
+    if (mode > 0 && psi_prime.num_params != kl.P)
+        throw std::invalid_argument("KullbackLeibler num_params does not match psi_prime");
+    if (!std::isfinite(threshold)) throw std::invalid_argument("threshold must be finite");
+    // samples of psi' with log psi'; log psi of the classical network on the same configurations
+    sample(psi_prime, ens);
+    Samples& s = ens.samples;
+    const int64_t n = s.count;
+    psi.sync_device();
+    Samples sc;
+    prepare_samples(psi, sc, n, s.basis);
+    ANN_HIP_CHECK(hipMemcpyAsync(sc.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice,
+                                 S()));
+    log_psi_of(psi, sc.confs.p, n, sc, nullptr);
+    DevBuf<cplx<double>> c_w, c_d, c_d2;
+    DevBuf<double> sums;
+    c_w.ensure(n);
+    c_d.ensure(n);
+    c_d2.ensure(n);
+    sums.ensure(6);
+    const cplx<double> last = {kl.last_mean_deviation.real(), kl.last_mean_deviation.imag()};
+    launch_kl_terms(sc.log_psi.p, s.log_psi.p, s.w.p, n, kl.log_psi_scale, last, threshold, c_w.p, c_d.p, c_d2.p,
+                    sums.p, S());
+    double h[6];
+    ANN_HIP_CHECK(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    const double W = h[0];
+    if (!(W > 0.0)) throw std::runtime_error("KullbackLeibler: the samples carry no weight");
+    kl.total_weight = W;
+    kl.mean_deviation = cd(h[1], h[2]) / W;
+    kl.last_mean_deviation = kl.mean_deviation;   // KullbackLeibler.cu.template:191-197
+    const cd d = cd(h[3], h[4]) / W;
+    const double d2 = h[5] / W;
+    const double value = std::sqrt(std::max(1e-8, d2 - std::norm(d)));
+    if (mode == 0) return value;
+    const int P = psi_prime.num_params;
+    const int nsum = mode == 2 ? 7 : 2;
+    std::vector<DevBuf<cplx<double>>> outs(nsum);
+    std::vector<cplx<double>*> op(nsum);
+    for (int i = 0; i < nsum; i++) {
+        outs[i].ensure(P);
+        op[i] = outs[i].p;
+    }
+    // 0: <O>, 1: <dev conj(O)>; noise: 2: <|dev|^2 |O|^2>, 3: <dev O>, 4: <dev |O|^2>,
+    // 5: <|dev|^2 O>, 6: <|O|^2> (all weighted, deviation terms masked)
+    const cplx<double>* coef[7] = {c_w.p, c_d.p, c_d2.p, c_d.p, c_d.p, c_d2.p, c_w.p};
+    const int modes[7] = {0, 1, 2, 0, 2, 0, 2};
+    if (psi_prime.precision == 0)
+        ok_column_sums_t<double>(psi_prime, s, nsum, coef, modes, op.data());
+    else
+        ok_column_sums_t<float>(psi_prime, s, nsum, coef, modes, op.data());
+    std::vector<std::vector<cd>> hs(nsum, std::vector<cd>(P));
+    for (int i = 0; i < nsum; i++) {
+        outs[i].download(reinterpret_cast<cplx<double>*>(hs[i].data()), P);
+        for (cd& x : hs[i]) x /= W;
+    }
+    const double factor = std::pow(value, nu);
+    const double steps = double(ens.num_steps());
+    for (int k = 0; k < P; k++) {
+        const cd Ok = hs[0][k], dOc = hs[1][k];
+        if (grad) grad[k] = (dOc - d * std::conj(Ok)) / factor;
+        if (mode == 2 && noise) {
+            // KullbackLeibler.cu.template:300-312
+            const double d2Ok2 = hs[2][k].real();
+            const cd dOk = hs[3][k], dOk2 = hs[4][k], d2Ok = hs[5][k];
+            const double Ok2 = hs[6][k].real();
+            const double var = d2Ok2 - std::norm(dOc) +
+                               2.0 * (dOk * std::conj(d) * std::conj(Ok) + 2.0 * std::conj(dOc) * d * std::conj(Ok) -
+                                      d2Ok * std::conj(Ok) - dOk2 * std::conj(d))
+                                         .real() +
+                               d2 * std::norm(Ok) + std::norm(d) * Ok2 - 4.0 * std::norm(d) * std::norm(Ok);
+            noise[k] = std::sqrt(var / steps) / factor;
+        }
+    }
+    return value;
+}
+
 double hilbert_space_distance(Psi& psi, Psi& psi_prime, Operator& op, bool unitary, Ensemble& ens, double nu,
                               cd* grad) {
     if (psi.num_sites != psi_prime.num_sites) throw std::invalid_argument("psi and psi_prime differ in num_sites");
@@ -1352,7 +1652,7 @@ double hilbert_space_distance(Psi& psi, Psi& psi_prime, Operator& op, bool unita
     // log psi' and (for its O_k) the angles of psi' on the same configurations
     psi_prime.sync_device();
     Samples sp;
-    prepare_samples(psi_prime, sp, n);
+    prepare_samples(psi_prime, sp, n, s.basis);
     ANN_HIP_CHECK(hipMemcpyAsync(sp.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice,
                                  S()));
     log_psi_of(psi_prime, sp.confs.p, n, sp, nullptr);
@@ -1390,7 +1690,7 @@ double hilbert_space_distance(Psi& psi, Psi& psi_prime, Operator& op, bool unita
 
 void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     if (uint32_t(psi.num_params) != t.P) throw std::invalid_argument("TDVP num_params does not match the wave function");
-    if (!psi.dense()) {
+    if (generic_path(psi, ens.basis)) {
         if (psi.precision == 0)
             tdvp_eval_generic<double>(t, op, psi, ens, with_S);
         else

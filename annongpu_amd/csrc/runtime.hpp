@@ -181,7 +181,7 @@ struct OwnedEvent {
     }
 };
 
-enum PsiKind : int { PSI_RBM = 0, PSI_DEEP = 1, PSI_CNN = 2 };
+enum PsiKind : int { PSI_RBM = 0, PSI_DEEP = 1, PSI_CNN = 2, PSI_CLASSICAL = 3 };
 
 struct Psi {
     int kind = PSI_RBM;
@@ -204,8 +204,21 @@ struct Psi {
     std::vector<int> connectivity;  // [L][3]
     std::vector<uint32_t> sym;
     int num_sym = 1;
-    std::vector<cd> params;  // PsiCNN parameters
+    std::vector<cd> params;  // PsiCNN parameters; the classical parameters of PsiClassical
     double final_factor = 1.0;
+    // PsiClassical: H_local[n] holds the strings cl_begin[n] .. cl_begin[n + 1] - 1 (a and b
+    // words, coefficients).  ref is the network reference (a PsiDeep), null for the fully
+    // polarised one (log psi_ref = 0, no parameters); with cl_order == 2 it enters log psi
+    // and its parameters follow the classical ones.
+    int cl_order = 1;
+    std::vector<int> cl_begin;
+    std::vector<uint64_t> cl_a, cl_b;   // [strings][kMaxWords]
+    std::vector<cd> cl_coef;
+    cd ref_log_prefactor = 0.0;
+    std::unique_ptr<Psi> ref;
+    DevBuf<int> cl_begin_d;
+    DevBuf<uint64_t> cl_a_d, cl_b_d;
+    DevBuf<double> cl_coef_s_d, cl_coef_p_d;   // spin basis (times (-i)^{#Y}), Pauli basis
 
     // device state: dirty asks for the full upload; params_dirty (after the first upload,
     // dense nets) only for the parameters, staged in page-locked memory and repacked
@@ -237,7 +250,11 @@ struct Psi {
     DevBuf<int> mlp_rb, mlp_ri;
     std::vector<int> mlp_cn_off, mlp_w_off, mlp_b_off, mlp_rb_off, mlp_ri_off;
 
+    bool mlp_stale = true;     // the layer tables of a one-layer PsiDeep's generic view
     bool dense() const { return kind == PSI_RBM || (kind == PSI_DEEP && num_layers == 1); }
+    // a PsiDeep over Pauli strings: 3 input units per site
+    bool pauli_net() const { return kind == PSI_DEEP && N == 3 * num_sites; }
+    bool ref_enters() const { return kind == PSI_CLASSICAL && cl_order == 2 && ref != nullptr; }
     void finalize();           // derived sizes after construction
     void sync_device();        // upload parameters when dirty
     DenseView<double> view64();
@@ -246,6 +263,9 @@ struct Psi {
     CnnView<float> cnn32();
     MlpView<double> mlp64();
     MlpView<float> mlp32();
+    ClassicalView<double> classical64(int basis);
+    ClassicalView<float> classical32(int basis);
+    void upload_mlp_tables();
     std::vector<cd> get_params() const;
     void set_params(const cd* p);
     std::unique_ptr<Psi> clone() const;
@@ -262,18 +282,29 @@ struct DeviceOp {
     }
 };
 
+// the strings as they act on Pauli-string configurations (at most 64 sites)
+struct DevicePauliOp {
+    int num_strings = 0;
+    DevBuf<uint64_t> ab;
+    DevBuf<double> coef;
+    PauliOpView view() const { return {num_strings, ab.p, coef.p}; }
+};
+
 struct Operator {
     int num_strings = 0;
     std::vector<uint64_t> a, b;  // [n][kMaxWords]
     std::vector<cd> coef;
     std::map<int, std::unique_ptr<DeviceOp>> compiled;
+    std::unique_ptr<DevicePauliOp> pauli_compiled;
     const DeviceOp& device(int words);
+    const DevicePauliOp& device_pauli();
     int max_site() const;
 };
 
 struct Samples {
     int64_t count = 0;
     int words = 0, Mpad = 0, precision = 0;
+    int basis = BASIS_SPINS;
     DevBuf<uint64_t> confs;
     DevBuf<double> w;
     bool w_uniform = false;      // every sample weighs 1 / count (Monte Carlo)
@@ -312,6 +343,7 @@ enum EnsembleKind : int { ENS_MC = 0, ENS_EXACT = 1, ENS_LIST = 2 };
 
 struct Ensemble {
     int kind = ENS_MC;
+    int basis = BASIS_SPINS;
     uint32_t num_sites = 0;
     uint32_t num_samples = 0, num_sweeps = 0, num_therm = 0, num_chains = 0;
     int move = MOVE_FLIP, total_z = 0;
@@ -344,7 +376,10 @@ struct Ensemble {
     // caller-given configurations [num_samples][kMaxWords] and weights (ENS_LIST)
     std::vector<uint64_t> list_confs;
     std::vector<double> list_w;
-    uint64_t num_steps() const { return kind == ENS_EXACT ? (uint64_t(1) << num_sites) : num_samples; }
+    uint64_t num_steps() const {
+        if (kind != ENS_EXACT) return num_samples;
+        return uint64_t(1) << (basis == BASIS_PAULIS ? 2 * num_sites : num_sites);
+    }
 };
 
 
@@ -436,6 +471,9 @@ struct Tdvp {
 void sample(Psi& psi, Ensemble& ens);
 void compute_theta(Psi& psi, const uint64_t* confs, int64_t count, Samples& out, int64_t offset);
 void log_psi_of(Psi& psi, const uint64_t* confs, int64_t count, Samples& s, double* w);
+// size the buffers of `count` samples of psi in a basis (Basis); throws when psi cannot
+// read that basis
+void prepare_samples(Psi& psi, Samples& s, int64_t count, int basis = BASIS_SPINS);
 // sum_s coef_s op(O_s) in parameter order (op: 0 identity, 1 conjugate), any wave function
 void ok_weighted_sum(Psi& psi, Samples& s, const cplx<double>* coef, int conj, cplx<double>* out);
 void local_energies(Psi& psi, Operator& op, Samples& s);
@@ -454,6 +492,19 @@ void tdvp_O_samples(Tdvp& t, cd* out);
 // HilbertSpaceDistance.distance / gradient (HilbertSpaceDistance.cu.template:113-165):
 // the distance between psi' and U psi on psi's samples; with grad, also its gradient
 // with respect to psi''s parameters (nu: the power of the distance it is divided by)
+// KullbackLeibler (include/network_functions/KullbackLeibler.hpp): psi a classical network,
+// psi_prime the network whose ensemble is sampled; the mean deviation of one call shifts
+// the deviations of the next (KullbackLeibler.cu.template:191-197)
+struct Kl {
+    int P = 0;
+    double log_psi_scale = 1.0;
+    cd last_mean_deviation = 0.0;
+    double total_weight = 0.0;
+    cd mean_deviation = 0.0;
+};
+// mode 0: the value; 1: and the gradient (P values); 2: and the noise estimate
+double kullback_leibler(Kl& kl, Psi& psi, Psi& psi_prime, Ensemble& ens, double threshold, int mode, double nu,
+                        cd* grad, double* noise);
 double hilbert_space_distance(Psi& psi, Psi& psi_prime, Operator& op, bool unitary, Ensemble& ens, double nu,
                               cd* grad);
 

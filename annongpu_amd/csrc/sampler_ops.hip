@@ -14,7 +14,13 @@ __global__ void init_chains_kernel(uint64_t* confs, int num_chains, int words, i
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= num_chains) return;
     uint64_t w[kMaxWords] = {0, 0, 0, 0};
-    if (move == MOVE_FLIP) {
+    if (move == MOVE_PAULI) {
+        // a uniformly random Pauli string on the N sites (Init_Policy.hpp:33-39, PauliString.hpp:58-65)
+        const u32x4 r = philox(uint32_t(c), 0u, call_index, 3u, k0, k1);
+        const uint64_t m = N >= 64 ? ~uint64_t(0) : (uint64_t(1) << N) - 1;
+        w[0] = (uint64_t(r.x) | (uint64_t(r.y) << 32)) & m;
+        w[1] = (uint64_t(r.z) | (uint64_t(r.w) << 32)) & m;
+    } else if (move == MOVE_FLIP) {
         for (int blk = 0; blk < (words + 1) / 2; blk++) {
             const u32x4 r = philox(uint32_t(c), uint32_t(blk), call_index, 1u, k0, k1);
             w[2 * blk] = uint64_t(r.x) | (uint64_t(r.y) << 32);

@@ -115,6 +115,22 @@ class PauliExpression:
             m = max(m, (a | b).bit_length() - 1)
         return m
 
+    def to_json(self):
+        """{"type": "PauliExpression", "terms": [[a, b, re, im], ...]} with the masks a, b as
+        decimal strings (any width)"""
+        return {"type": "PauliExpression",
+                "terms": [[str(a), str(b), c.real, c.imag] for (a, b), c in self.terms.items()]}
+
+    @staticmethod
+    def from_json(obj):
+        if obj.get("type") != "PauliExpression":
+            raise ValueError("not a PauliExpression JSON object")
+        return PauliExpression({(int(a), int(b)): complex(re, im) for a, b, re, im in obj["terms"]})
+
+    def term_list(self):
+        """one single-string PauliExpression per term (iteration order)"""
+        return [PauliExpression({k: v}) for k, v in self.terms.items()]
+
     def roll(self, shift, n):
         """Cyclic shift of every string by `shift` sites on a ring of n sites."""
         mask = (1 << n) - 1

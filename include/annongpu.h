@@ -34,6 +34,7 @@ typedef struct ann_operator_s* ann_operator;
 typedef struct ann_ensemble_s* ann_ensemble;
 typedef struct ann_tdvp_s* ann_tdvp;
 typedef struct ann_proposal_s* ann_proposal;
+typedef struct ann_kl_s* ann_kl;
 
 /* precision of the device path: FP64 everywhere, or FP32 sweeps / E_loc / O_k
  * with the angle GEMM on MFMA (bf16 x3 split of W, fp32 accumulation).  fp64 is
@@ -43,6 +44,7 @@ typedef struct ann_proposal_s* ann_proposal;
 
 #define ANN_MOVE_FLIP 0       /* single spin flip  (policies/Update_Policy.hpp:19-33) */
 #define ANN_MOVE_EXCHANGE 1   /* swap of an anti-parallel pair, conserves S_z          */
+#define ANN_MOVE_PAULI 2      /* a site set to a random Pauli type (Pauli-string ensembles) */
 
 /* ---------------------------------------------------------------- runtime */
 const char* ann_last_error(void);
@@ -75,6 +77,25 @@ int ann_psi_create_deep(ann_psi* out, int32_t num_sites, int32_t N, const double
 int ann_psi_create_cnn(ann_psi* out, const int32_t* extent, int32_t num_layers, const int32_t* channels,
                        const int32_t* connectivity, const uint32_t* symmetry_classes, int32_t num_params,
                        const double* params, double final_factor, const double* log_prefactor);
+/* PsiClassicalFP_1/_2, PsiClassicalANN_1/_2 (num_sites, H_local, params, psi_ref, log_prefactor,
+ * gpu) -- quantum_state/PsiClassical.hpp:48-160,204-225, source/quantum_state/PsiClassical.cu:9-61,
+ * MAIN:208-270.  H_local[n] = ops[n] (copied), params[num_ops] interleaved complex;
+ * log psi(s) = log_prefactor + sum_n params[n] L_n(s), L_n = the matrix-element sum of
+ * H_local[n] (Operator.hpp:123-135).  ref: a PsiDeep (copied; the ANN form) or null for
+ * PsiFullyPolarized (log psi_ref = 0, no parameters, PsiFullyPolarized.hpp:31-40).  With
+ * order 2 and a network reference log psi_ref(s) enters log psi and the reference's
+ * parameters follow the classical ones in params and O_k. */
+int ann_psi_create_classical(ann_psi* out, int32_t num_sites, int32_t num_ops, const ann_operator* ops,
+                             const double* params, ann_psi ref, int32_t order, const double* log_prefactor,
+                             const double* ref_log_prefactor);
+/* order (MAIN:242), len(H_local) (MAIN:221), and whether a network reference is held */
+int ann_psi_classical_info(ann_psi psi, int32_t* order, int32_t* num_ops, int32_t* has_network_ref);
+/* psi_ref (MAIN:236-239): a copy of the network reference (ANN_E_INVALID for the fully
+ * polarised one) */
+int ann_psi_classical_ref(ann_psi psi, ann_psi* out);
+/* ref_log_prefactor property (MAIN:226-234) */
+int ann_psi_get_ref_log_prefactor(ann_psi psi, double* value2);
+int ann_psi_set_ref_log_prefactor(ann_psi psi, const double* value2);
 /* copy() (MAIN:82,127,171) */
 int ann_psi_copy(ann_psi src, ann_psi* out);
 int ann_psi_destroy(ann_psi psi);
@@ -128,11 +149,27 @@ int ann_ensemble_last_samples(ann_ensemble ens, uint64_t* confs, int64_t capacit
 int ann_sample_list_create(ann_ensemble* out, uint32_t num_sites, const uint64_t* confs, int64_t count,
                            int32_t words, const double* weights);
 
+/* ExactSummationPaulis(num_sites, gpu) -- ensembles/ExactSummation.hpp:125, MAIN:402-406:
+ * the 4^N Pauli strings in PauliString::enumerate order (basis/PauliString.hpp:30-35) */
+int ann_exact_summation_paulis_create(ann_ensemble* out, uint32_t num_sites);
+/* MonteCarloPaulis(num_samples, num_sweeps, num_thermalization_sweeps, num_markov_chains,
+ * gpu) -- ensembles/MonteCarlo.hpp:265-283, MAIN:369-380: Metropolis moves that set a
+ * random site to a random Pauli type (policies/Update_Policy.hpp:41-56).  Additive: seed. */
+int ann_monte_carlo_paulis_create(ann_ensemble* out, uint32_t num_samples, uint32_t num_sweeps,
+                                  uint32_t num_thermalization_sweeps, uint32_t num_markov_chains, uint64_t seed);
+/* additive: caller-given Pauli strings ab[count][2] = (a, b) over num_sites <= 64 sites,
+ * with weights (null: 1 / count each) */
+int ann_pauli_sample_list_create(ann_ensemble* out, uint32_t num_sites, const uint64_t* ab, int64_t count,
+                                 const double* weights);
+
 /* ------------------------------------------------------ network functions */
 /* log_psi_s(psi, spins) -- PsiVector.cu.template:101-133, MAIN:495-497 */
 int ann_log_psi_s(ann_psi psi, const uint64_t* conf, double* out2);
 /* psi_O_k(psi, spins) -- PsiOkVector.cu.template:40-74, MAIN:498-500 */
 int ann_psi_O_k(ann_psi psi, const uint64_t* conf, double* out);
+/* log_psi_s / psi_O_k of a Pauli string ab = (a, b) (MAIN:495-500 on the PauliString basis) */
+int ann_log_psi_s_pauli(ann_psi psi, const uint64_t* ab, double* out2);
+int ann_psi_O_k_pauli(ann_psi psi, const uint64_t* ab, double* out);
 /* psi_O_k_vector(psi, exact) = sum_s O_k(s), unweighted -- PsiOkVector.cu.template:8-37 */
 int ann_psi_O_k_vector(ann_psi psi, ann_ensemble ens, double* out);
 /* log_psi(psi, ens) = sum_s w_s log psi(s) -- PsiVector.cu.template:73-99 */
@@ -173,6 +210,24 @@ int ann_local_energies_noalloc(ann_operator op, ann_psi psi, const uint64_t* con
  * psi_prime's parameters, divided by distance^nu */
 int ann_hilbert_space_distance(ann_psi psi, ann_psi psi_prime, ann_operator op, int32_t is_unitary, ann_ensemble ens,
                                double nu, double* gradient, double* distance);
+
+/* KullbackLeibler(num_params, gpu) -- network_functions/KullbackLeibler.hpp:14-117,
+ * KullbackLeibler.cu.template:126-162, MAIN:445-462 */
+int ann_kl_create(ann_kl* out, uint32_t num_params);
+int ann_kl_destroy(ann_kl kl);
+/* __call__ (mode 0), gradient (mode 1), gradient_with_noise (mode 2) --
+ * KullbackLeibler.cu.template:171-323: psi the classical network, psi_prime the network
+ * whose ensemble is sampled; the deviation log psi_prime - log_psi_scale log psi is compared
+ * with the mean deviation of the previous call and counts where its square exceeds
+ * threshold^2.  gradient (num_params interleaved complex, may be null) is with respect to
+ * psi_prime's parameters, divided by value^nu; noise (num_params, mode 2) may be null. */
+int ann_kl_eval(ann_kl kl, ann_psi psi, ann_psi psi_prime, ann_ensemble ens, double threshold, double nu,
+                int32_t mode, double* gradient, double* noise, double* value);
+/* total_weight and mean_deviation of the last call (MAIN:459-460) */
+int ann_kl_info(ann_kl kl, double* total_weight, double* mean_deviation2);
+/* log_psi_scale property (MAIN:461) */
+int ann_kl_get_log_psi_scale(ann_kl kl, double* scale);
+int ann_kl_set_log_psi_scale(ann_kl kl, double scale);
 
 /* TDVP(num_params, gpu) -- network_functions/TDVP.hpp:16-101, MAIN:465-489 */
 int ann_tdvp_create(ann_tdvp* out, uint32_t num_params);
