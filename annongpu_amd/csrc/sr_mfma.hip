@@ -19,9 +19,9 @@
 // 16-column tiles (real parts, imaginary parts).  One wavefront owns one pair,
 // keeps what it multiplies in registers and streams its column of g, so neither
 // kernel touches LDS in its main loop:
-//   ov32_kernel        V fragments of the pair in VGPRs, spins built from the
-//                      configuration words, angles of v contracted with g on the
-//                      fly; per-pair partial sums of O.v.
+//   ov32_kernel        V fragments of the pair in VGPRs (A operand), spins built
+//                      from the configuration words (B operand), angles of v
+//                      contracted with g on the fly; per-pair partial sums of O.v.
 //   spin_gemm32_kernel spins built from a transposed bit array (bit b of
 //                      bits[blk][v] = spin v of sample 64 blk + b), c o g split on
 //                      the fly; the 4 waves of a workgroup share a pair, reduce
@@ -79,14 +79,6 @@ __device__ inline dw4_t spin_frag8(uint32_t x) {
     return a;
 }
 
-__device__ inline float row16_sum_f(float v) {
-    v += dpp_f32<0xB1>(v);
-    v += dpp_f32<0x4E>(v);
-    v += dpp_f32<0x141>(v);
-    v += dpp_f32<0x140>(v);   // row_mirror: every lane of the 16-lane row holds the sum
-    return v;
-}
-
 constexpr int kMaxVpad = 64 * kMaxWords + 16;            // spin rows incl. the bias row
 constexpr int kBitWords = (kMaxVpad + 63) / 64;
 
@@ -120,12 +112,16 @@ __device__ inline void fill_spin_lut(dw4_t* lut) {
 // ------------------------------------------------------------ u = O.v (part)
 // Wave (pair p, slice): the V fragments of the pair (vfrag[((sp KT + kt) NT + 2p + h)
 // 64 + lane], 8 bf16 each: sites 32 kt + 8 (lane/16) + e, unit 16p + lane%16,
-// h = 0 real / 1 imaginary parts) stay in registers; the wave walks the 16-sample
-// groups slice, slice + slices, ...; the next group's configuration words and g
-// are loaded while the current group multiplies (configuration indices clamped
-// to the last sample, g padded to sr_count_pad(count) rows: rows beyond count are
-// computed and never stored).  upart[p][s] = sum over the pair's units of
-// g_sj (theta_v(s)_j + vb_j).
+// h = 0 real / 1 imaginary parts) stay in registers as the A operand (row = unit), the
+// spins of 16 samples are the B operand (column = sample), so the product tile holds
+// theta_v(s)_j with lane l owning sample l%16 at the units 16p + 4(l/16) + r, r < 4.
+// Contracted with g there (one 32-byte load of the 4 units per lane) and summed over
+// the 4 lane rows by two permlane swaps, lanes 0..15 store the 16 samples' partials in
+// one coalesced 128-byte store.  The wave walks the 16-sample groups slice, slice +
+// slices, ...; the next group's configuration words and g are loaded while the current
+// group multiplies (configuration indices clamped to the last sample, g padded to
+// sr_count_pad(count) rows: rows beyond count are computed and never stored).
+// upart[p][s] = sum over the pair's units of g_sj (theta_v(s)_j + vb_j).
 template <int KT>
 __global__ __launch_bounds__(256) void ov32_kernel(const dw4_t* vfrag, int NT, int npairs, int slices,
                                                    const uint64_t* confs, int64_t count, int Mpad, int M,
@@ -139,7 +135,12 @@ __global__ __launch_bounds__(256) void ov32_kernel(const dw4_t* vfrag, int NT, i
     if (wid >= npairs * slices) return;   // whole waves leave after the only barrier
     const int p = wid % npairs, slice = wid / npairs;
     const int q = lane >> 4, col = lane & 15;
-    const int j = 16 * p + col;
+    const int j0 = 16 * p + 4 * q;   // this lane's first unit
+    // prologue loads in the order biases (vb: Mpad values, zero for PsiRBM), V fragments,
+    // first group: each waits only for what was issued before it
+    cplx<double> vbx[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) vbx[r] = vb[j0 + r];
     dw4_t bv[3][KT][2];
 #pragma unroll
     for (int sp = 0; sp < 3; sp++)
@@ -147,52 +148,62 @@ __global__ __launch_bounds__(256) void ov32_kernel(const dw4_t* vfrag, int NT, i
         for (int kt = 0; kt < KT; kt++)
 #pragma unroll
             for (int h = 0; h < 2; h++) bv[sp][kt][h] = vfrag[((sp * KT + kt) * NT + 2 * p + h) * 64 + lane];
-    float bre = 0.f, bim = 0.f;
-    if (vb && j < M) {
-        bre = float(vb[j].re);
-        bim = float(vb[j].im);
-    }
     const int64_t groups = (count + 15) / 16;
     const int64_t last = count - 1;
     uint64_t wn[WORDS];
-    float2 gn[4];
+    float4 gn[2];
     auto load = [&](int64_t grp) {
-        const int64_t sa = grp * 16 + col < last ? grp * 16 + col : last;   // sample of this lane's A row
+        const int64_t sa = grp * 16 + col < last ? grp * 16 + col : last;   // sample of this lane's column
 #pragma unroll
         for (int k = 0; k < WORDS; k++) wn[k] = confs[sa * WORDS + k];
-        const float2* gp = g + (grp * 16 + 4 * q) * Mpad + j;
-#pragma unroll
-        for (int r = 0; r < 4; r++) gn[r] = gp[int64_t(r) * Mpad];
+        const float4* gp = reinterpret_cast<const float4*>(g + (grp * 16 + col) * Mpad + j0);
+        gn[0] = gp[0];
+        gn[1] = gp[1];
     };
     int64_t grp = slice;
     if (grp < groups) load(grp);
+    float bre[4], bim[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        bre[r] = j0 + r < M ? float(vbx[r].re) : 0.f;
+        bim[r] = j0 + r < M ? float(vbx[r].im) : 0.f;
+    }
     for (; grp < groups; grp += slices) {
         uint64_t w[WORDS];
-        float2 gv[4];
 #pragma unroll
         for (int k = 0; k < WORDS; k++) w[k] = wn[k];
-#pragma unroll
-        for (int r = 0; r < 4; r++) gv[r] = gn[r];
+        const float4 g01 = gn[0], g23 = gn[1];
         if (grp + slices < groups) load(grp + slices);
         float4_t are = {0.f, 0.f, 0.f, 0.f}, aim = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < KT; kt++) {
             const int o = 32 * kt + 8 * q;
-            const dw4_t a = lut[uint32_t(w[kt >> 1] >> (o & 63)) & 0xFFu];
+            const dw4_t b = lut[uint32_t(w[kt >> 1] >> (o & 63)) & 0xFFu];
 #pragma unroll
             for (int sp = 0; sp < 3; sp++) {
-                are = mfma32(a, bv[sp][kt][0], are);
-                aim = mfma32(a, bv[sp][kt][1], aim);
+                are = mfma32(bv[sp][kt][0], b, are);
+                aim = mfma32(bv[sp][kt][1], b, aim);
             }
         }
+        const float gx[4] = {g01.x, g01.z, g23.x, g23.z}, gy[4] = {g01.y, g01.w, g23.y, g23.w};
+        float ur = 0.f, ui = 0.f;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const float tr = are[r] + bre, ti = aim[r] + bim;
-            const float ur = row16_sum_f(gv[r].x * tr - gv[r].y * ti);
-            const float ui = row16_sum_f(gv[r].x * ti + gv[r].y * tr);
-            const int64_t s = grp * 16 + 4 * q + r;
-            if (col == 15 && s < count) upart[int64_t(p) * count + s] = float2{ur, ui};
+            const float tr = are[r] + bre[r], ti = aim[r] + bim[r];
+            ur += gx[r] * tr - gy[r] * ti;
+            ui += gx[r] * ti + gy[r] * tr;
         }
+        // sum over the 4 lane rows: lanes l, l ^ 16, l ^ 32, l ^ 48
+        const auto a16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(ur), __float_as_uint(ur), false, false);
+        const auto b16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(ui), __float_as_uint(ui), false, false);
+        ur = __uint_as_float(a16[0]) + __uint_as_float(a16[1]);
+        ui = __uint_as_float(b16[0]) + __uint_as_float(b16[1]);
+        const auto a32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(ur), __float_as_uint(ur), false, false);
+        const auto b32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(ui), __float_as_uint(ui), false, false);
+        ur = __uint_as_float(a32[0]) + __uint_as_float(a32[1]);
+        ui = __uint_as_float(b32[0]) + __uint_as_float(b32[1]);
+        const int64_t s = grp * 16 + col;
+        if (q == 0 && s < count) upart[int64_t(p) * count + s] = float2{ur, ui};
     }
 }
 
@@ -869,7 +880,6 @@ void launch_sr_ov32(const DenseView<float>& net, const uint64_t* confs, int word
     if (count <= 0) return;
     const int KT = sr_kt32(net.N), NT = 2 * net.Mpad / 16;
     if (KT > 8 || words != (KT + 1) / 2) throw std::invalid_argument("configurations of 1..256 sites are supported");
-    const bool deep = net.layout == LAYOUT_DEEP1;
     const int npairs = sr_ov_pairs(net.M);
     const int64_t groups = (count + 15) / 16;
     // about 2048 waves (two per SIMD)
@@ -879,7 +889,7 @@ void launch_sr_ov32(const DenseView<float>& net, const uint64_t* confs, int word
         constexpr int kt = std::decay_t<decltype(tag)>::value;
         ov32_kernel<kt><<<unsigned((waves + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, stream>>>(
             reinterpret_cast<const dw4_t*>(ws.frag), NT, npairs, slices, confs, count, net.Mpad, net.M,
-            reinterpret_cast<const float2*>(g), deep ? ws.vb : nullptr, reinterpret_cast<float2*>(ws.upart));
+            reinterpret_cast<const float2*>(g), ws.vb, reinterpret_cast<float2*>(ws.upart));
     });
     ANN_LAUNCH_CHECK();
 }
