@@ -14,6 +14,7 @@
 //                   proposal re-runs the forward pass, as here)
 #include "cnn.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace ann {
@@ -23,7 +24,7 @@ namespace {
 // input of layer l at site y for channel ci: spins for layer 0, the previous
 // layer's activation otherwise (stored in A)
 template <typename T>
-__device__ inline cplx<T> layer_input(int l, const uint64_t* w, const cplx<T>* A, int N, int ci, int y) {
+__device__ __forceinline__ cplx<T> layer_input(int l, const uint64_t* w, const cplx<T>* A, int N, int ci, int y) {
     if (l == 0) {
         const uint64_t word = y < 64 ? w[0] : y < 128 ? w[1] : y < 192 ? w[2] : w[3];
         return {((word >> (y & 63)) & 1) ? T(1) : T(-1), T(0)};
@@ -35,7 +36,7 @@ __device__ inline cplx<T> layer_input(int l, const uint64_t* w, const cplx<T>* A
 // (A2 is the ping-pong partner), Z (optional) receives the pre-activations of
 // every layer at Z + zoff[l].  Returns the network part of log psi.
 template <typename T>
-__device__ cplx<T> cnn_forward(const CnnView<T>& net, const uint64_t* w, cplx<T>* A, cplx<T>* A2, cplx<T>* Z,
+__device__ __forceinline__ cplx<T> cnn_forward(const CnnView<T>& net, const uint64_t* w, cplx<T>* A, cplx<T>* A2, cplx<T>* Z,
                                int lane) {
     const int N = net.N;
     cplx<T> total = {0, 0};
@@ -80,14 +81,14 @@ __device__ cplx<T> cnn_forward(const CnnView<T>& net, const uint64_t* w, cplx<T>
 }
 
 template <typename T>
-__device__ inline cplx<T> view_forward(const CnnView<T>& net, const uint64_t* w, unsigned char* smem, int lane) {
+__device__ __forceinline__ cplx<T> view_forward(const CnnView<T>& net, const uint64_t* w, unsigned char* smem, int lane) {
     cplx<T>* A = reinterpret_cast<cplx<T>*>(smem);
     return cnn_forward(net, w, A, A + net.max_ch * net.N, static_cast<cplx<T>*>(nullptr), lane);
 }
 
 // PsiDeep forward: Z[zoff[l] + j] = z_l[j], Act[zoff[l] + j] = act_l(z_l[j])
 template <typename T>
-__device__ cplx<T> mlp_forward(const MlpView<T>& net, const uint64_t* w, cplx<T>* Z, cplx<T>* Act, int lane) {
+__device__ __forceinline__ cplx<T> mlp_forward(const MlpView<T>& net, const uint64_t* w, cplx<T>* Z, cplx<T>* Act, int lane) {
     for (int l = 0; l < net.L; l++) {
         const int M = net.size[l], C = net.conn[l];
         for (int j = lane; j < M; j += 64) {
@@ -118,27 +119,24 @@ __device__ cplx<T> mlp_forward(const MlpView<T>& net, const uint64_t* w, cplx<T>
 }
 
 template <typename T>
-__device__ inline cplx<T> view_forward(const MlpView<T>& net, const uint64_t* w, unsigned char* smem, int lane) {
+__device__ __forceinline__ cplx<T> view_forward(const MlpView<T>& net, const uint64_t* w, unsigned char* smem, int lane) {
     cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
     return mlp_forward(net, w, Z, Z + net.zoff[net.L], lane);
 }
 
 // the 3 N input units of a Pauli-string configuration (a = w[0], b = w[1], N <= 64) as
 // spins: bit 3i + t - 1 set when site i holds Pauli type t = a_i + 2 b_i > 0
-__device__ inline void pauli_units(const uint64_t* w, int N, uint64_t* e) {
+__device__ __forceinline__ void pauli_units(const uint64_t* w, int N, uint64_t* e) {
 #pragma unroll
     for (int k = 0; k < kMaxWords; k++) e[k] = 0;
     for (int i = 0; i < N; i++) {
         const int t = int((w[0] >> i) & 1u) | (int((w[1] >> i) & 1u) << 1);
-        if (t) {
-            const int u = 3 * i + t - 1;
-            e[u >> 6] |= uint64_t(1) << (u & 63);
-        }
+        if (t) flip_bit(e, 3 * i + t - 1);   // the bit was clear
     }
 }
 
 template <typename V>
-__device__ inline cplx<typename V::real_t> view_forward(const PauliInput<V>& net, const uint64_t* w,
+__device__ __forceinline__ cplx<typename V::real_t> view_forward(const PauliInput<V>& net, const uint64_t* w,
                                                         unsigned char* smem, int lane) {
     uint64_t e[kMaxWords];
     pauli_units(w, net.N, e);
@@ -148,7 +146,7 @@ __device__ inline cplx<typename V::real_t> view_forward(const PauliInput<V>& net
 // matrix-element coefficient of string (pa, pb) applied to the Pauli string (xa, xb)
 // (PauliString.hpp:257-276): (-1)^{#sites with a negative Levi-Civita sign}
 // (-i)^{#sites where both are non-trivial and differ}
-__device__ inline cplx<double> pauli_phase(uint64_t pa, uint64_t pb, uint64_t xa, uint64_t xb) {
+__device__ __forceinline__ cplx<double> pauli_phase(uint64_t pa, uint64_t pb, uint64_t xa, uint64_t xb) {
     const uint64_t px = pa & ~pb, py = ~pa & pb, pz = pa & pb;
     const uint64_t xx = xa & ~xb, xy = ~xa & xb, xz = xa & xb;
     const int neg = __popcll((px & xz) | (py & xx) | (pz & xy)) & 1;
@@ -164,7 +162,7 @@ __device__ inline cplx<double> pauli_phase(uint64_t pa, uint64_t pb, uint64_t xa
 
 // L_n(s) of PsiClassical (Operator::fast_local_energy over the strings of H_local[n])
 template <typename T>
-__device__ inline cplx<double> classical_term(const ClassicalView<T>& v, int n, const uint64_t* w) {
+__device__ __forceinline__ cplx<double> classical_term(const ClassicalView<T>& v, int n, const uint64_t* w) {
     double re = 0.0, im = 0.0;
     for (int m = v.op_begin[n]; m < v.op_begin[n + 1]; m++) {
         cplx<double> f;
@@ -185,7 +183,7 @@ __device__ inline cplx<double> classical_term(const ClassicalView<T>& v, int n, 
 }
 
 template <typename T>
-__device__ inline cplx<T> view_forward(const ClassicalView<T>& v, const uint64_t* w, unsigned char* smem, int lane) {
+__device__ __forceinline__ cplx<T> view_forward(const ClassicalView<T>& v, const uint64_t* w, unsigned char* smem, int lane) {
     double re = 0.0, im = 0.0;
     for (int n = lane; n < v.num_ops; n += 64) {
         const cplx<double> L = classical_term(v, n, w);
@@ -209,16 +207,16 @@ __device__ inline cplx<T> view_forward(const ClassicalView<T>& v, const uint64_t
 
 // the number of sites a Metropolis move picks from
 template <typename T>
-__device__ inline int view_sites(const CnnView<T>& v) { return v.N; }
+__device__ __forceinline__ int view_sites(const CnnView<T>& v) { return v.N; }
 template <typename T>
-__device__ inline int view_sites(const MlpView<T>& v) { return v.N; }
+__device__ __forceinline__ int view_sites(const MlpView<T>& v) { return v.N; }
 template <typename V>
-__device__ inline int view_sites(const PauliInput<V>& v) { return v.N; }
+__device__ __forceinline__ int view_sites(const PauliInput<V>& v) { return v.N; }
 template <typename T>
-__device__ inline int view_sites(const ClassicalView<T>& v) { return v.N; }
+__device__ __forceinline__ int view_sites(const ClassicalView<T>& v) { return v.N; }
 
 // s is uniform over the wavefront, so these become scalar loads
-__device__ inline void load_words(const uint64_t* confs, int64_t s, int words, uint64_t* w) {
+__device__ __forceinline__ void load_words(const uint64_t* confs, int64_t s, int words, uint64_t* w) {
     for (int k = 0; k < kMaxWords; k++) w[k] = k < words ? confs[s * words + k] : 0;
 }
 
@@ -266,7 +264,7 @@ __global__ __launch_bounds__(64) void generic_mc_kernel(V net, uint64_t* chain_c
         const uint64_t t = counter + uint64_t(q);
         const u32x4 r = philox(uint32_t(c), uint32_t(t), uint32_t(t >> 32), 0u, key0, key1);
         const int i = int(range32(r.x, uint32_t(N)));
-        const int si_bit = int((w[i >> 6] >> (i & 63)) & 1u);
+        const int si_bit = int((word_at(w, i >> 6) >> (i & 63)) & 1u);
         int kp = -1;
         if (move == MOVE_PAULI) {
             // site i takes Pauli type r.y >> 30 (possibly its current one)
@@ -298,8 +296,8 @@ __global__ __launch_bounds__(64) void generic_mc_kernel(V net, uint64_t* chain_c
         }
         uint64_t wn[kMaxWords];
         for (int k = 0; k < kMaxWords; k++) wn[k] = w[k];
-        wn[i >> 6] ^= uint64_t(1) << (i & 63);
-        if (kp >= 0) wn[kp >> 6] ^= uint64_t(1) << (kp & 63);
+        flip_bit(wn, i);
+        if (kp >= 0) flip_bit(wn, kp);
         const T nxt = view_forward(net, wn, smem, lane).re;
         const T d2 = T(2) * (nxt - cur);
         const T uu = (T(r.z) + T(0.5)) * T(2.3283064365386963e-10);
@@ -310,7 +308,9 @@ __global__ __launch_bounds__(64) void generic_mc_kernel(V net, uint64_t* chain_c
         }
     }
     if (lane == 0) {
-        for (int k = 0; k < words; k++) chain_confs[int64_t(c) * words + k] = w[k];
+#pragma unroll
+        for (int k = 0; k < kMaxWords; k++)
+            if (k < words) chain_confs[int64_t(c) * words + k] = w[k];
         accepted[c] += acc;
     }
 }
@@ -331,7 +331,9 @@ __global__ __launch_bounds__(64) void generic_local_energy_kernel(V net, OpView 
             double c_re = 0.0, c_im = 0.0;
             for (int n = op.group_begin[g]; n < op.group_begin[g + 1]; n++) {
                 int par = 0;
-                for (int k = 0; k < words; k++) par += __popcll(~w[k] & op.string_zy[int64_t(n) * words + k]);
+#pragma unroll
+                for (int k = 0; k < kMaxWords; k++)
+                    if (k < words) par += __popcll(~w[k] & op.string_zy[int64_t(n) * words + k]);
                 const double sg = (par & 1) ? -1.0 : 1.0;
                 c_re += sg * op.string_coef[2 * n];
                 c_im += sg * op.string_coef[2 * n + 1];
@@ -493,7 +495,7 @@ struct CnnInc {
 };
 
 template <typename T>
-__device__ inline CnnInc<T> cnn_inc_layout(const CnnView<T>& net, unsigned char* smem) {
+__device__ __forceinline__ CnnInc<T> cnn_inc_layout(const CnnView<T>& net, unsigned char* smem) {
     CnnInc<T> st;
     int a = 0;
     for (int l = 0; l + 1 < net.L; l++) {
@@ -516,13 +518,52 @@ size_t cnn_inc_lds(const CnnView<T>& net) {
     return sizeof(cplx<T>) * (2 * a + size_t(net.ch[net.L - 1]) * net.N + 2 * size_t(net.N)) + sizeof(int) * net.N;
 }
 
-__device__ inline bool mask_bit(const uint64_t (&m)[kMaxWords], int y) {
+// the read-only tables of a PsiCNN (parameters, neighbour tables, symmetry classes) in
+// LDS: every multiply-add of a local update reads a weight and a neighbour index, which
+// from global memory would be a dependent cache round trip each
+__host__ __device__ __forceinline__ int cnn_nbr_elems(int L, const int* nbr_off, const int* vol, int N) {
+    return nbr_off[L - 1] + N * vol[L - 1];
+}
+
+template <typename T>
+size_t cnn_table_lds(const CnnView<T>& net) {
+    const size_t p = (sizeof(cplx<T>) * size_t(net.num_params) + 15) / 16 * 16;
+    const size_t nb = (sizeof(int) * size_t(cnn_nbr_elems(net.L, net.nbr_off, net.vol, net.N)) + 15) / 16 * 16;
+    return p + nb + (sizeof(uint32_t) * size_t(net.N) + 15) / 16 * 16;
+}
+
+// where the local updates read the tables: LDS copies or the view's global arrays (kept
+// apart from the view so that the view stays in the kernel arguments, where its small
+// per-layer arrays are indexed by scalar loads)
+template <typename T>
+struct CnnTables {
+    const cplx<T>* params;
+    const int* nbr;
+    const uint32_t* sym;
+};
+
+// copies the tables to smem and returns where they are
+template <typename T>
+__device__ __forceinline__ CnnTables<T> stage_cnn_tables(const CnnView<T>& net, unsigned char* smem, int lane) {
+    cplx<T>* p = reinterpret_cast<cplx<T>*>(smem);
+    for (int k = lane; k < net.num_params; k += 64) p[k] = net.params[k];
+    int* nb = reinterpret_cast<int*>(smem + (sizeof(cplx<T>) * size_t(net.num_params) + 15) / 16 * 16);
+    const int nnb = cnn_nbr_elems(net.L, net.nbr_off, net.vol, net.N);
+    for (int k = lane; k < nnb; k += 64) nb[k] = net.nbr[k];
+    uint32_t* sy = reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(nb) +
+                                               (sizeof(int) * size_t(nnb) + 15) / 16 * 16);
+    for (int k = lane; k < net.N; k += 64) sy[k] = net.sym[k];
+    __syncthreads();
+    return {p, nb, sy};
+}
+
+__device__ __forceinline__ bool mask_bit(const uint64_t (&m)[kMaxWords], int y) {
     const uint64_t word = y < 64 ? m[0] : y < 128 ? m[1] : y < 192 ? m[2] : m[3];
     return (word >> (y & 63)) & 1u;
 }
 
 // the sites of mask m as a list in LDS (in increasing order); returns their number
-__device__ inline int mask_list(const uint64_t (&m)[kMaxWords], int N, int* lst, int lane) {
+__device__ __forceinline__ int mask_list(const uint64_t (&m)[kMaxWords], int N, int* lst, int lane) {
     int n = 0;
 #pragma unroll
     for (int u = 0; u < kMaxWords; u++) {
@@ -537,38 +578,74 @@ __device__ inline int mask_list(const uint64_t (&m)[kMaxWords], int N, int* lst,
     return n;
 }
 
+// pre-activation of output channel c at site x in layer l: sum over the input channels
+// ci and the kernel offsets k of params[link(l, ci, c, class(x)) + k] times the input at
+// neighbour k of x (the spins w for l = 0, else in[ci N + y]).  With VMAX > 0 (an upper
+// bound of the kernel volume) the neighbour indices sit in registers and the loads of one
+// input channel are issued together; VMAX = 0 is the plain loop for any volume.
+template <typename T, int VMAX>
+__device__ __forceinline__ cplx<T> conv_point(const CnnView<T>& net, const CnnTables<T>& tb, int l, int c, int x,
+                                              const uint64_t* w, const cplx<T>* in) {
+    const int N = net.N;
+    const int prev = l > 0 ? net.ch[l - 1] : 1;
+    const int vol = net.vol[l];
+    const int C = net.ch[l];
+    const int cls = int(tb.sym[x]);
+    const int* nb = tb.nbr + net.nbr_off[l] + int64_t(x) * vol;
+    cplx<T> z = {0, 0};
+    auto input = [&](int ci, int y) -> cplx<T> {
+        return l == 0 ? cplx<T>{((word_at(w, y >> 6) >> (y & 63)) & 1) ? T(1) : T(-1), T(0)} : in[ci * N + y];
+    };
+    if constexpr (VMAX > 0) {
+        int nbk[VMAX];
+#pragma unroll
+        for (int k = 0; k < VMAX; k++) nbk[k] = k < vol ? nb[k] : 0;
+        for (int ci = 0; ci < prev; ci++) {
+            const cplx<T>* wk = tb.params + net.link_off[l] + ((int64_t(ci) * C + c) * net.nsym + cls) * vol;
+#pragma unroll
+            for (int k = 0; k < VMAX; k++) {
+                if (k < vol) {
+                    const cplx<T> a = input(ci, nbk[k]);
+                    const cplx<T> q = wk[k];
+                    z.re += q.re * a.re - q.im * a.im;
+                    z.im += q.re * a.im + q.im * a.re;
+                }
+            }
+        }
+    } else {
+        for (int ci = 0; ci < prev; ci++) {
+            const cplx<T>* wk = tb.params + net.link_off[l] + ((int64_t(ci) * C + c) * net.nsym + cls) * vol;
+            for (int k = 0; k < vol; k++) {
+                const cplx<T> a = input(ci, nb[k]);
+                const cplx<T> q = wk[k];
+                z.re += q.re * a.re - q.im * a.im;
+                z.im += q.re * a.im + q.im * a.re;
+            }
+        }
+    }
+    return z;
+}
+
 // full forward at configuration w into the state; returns sum_x out[x] (the network
 // part of log psi before the final factor)
-template <typename T>
-__device__ cplx<T> cnn_inc_init(const CnnView<T>& net, const uint64_t* w, CnnInc<T>& st, int lane) {
+template <typename T, int VMAX>
+__device__ __forceinline__ cplx<T> cnn_inc_init(const CnnView<T>& net, const CnnTables<T>& tb, const uint64_t* w,
+                                                CnnInc<T>& st, int lane) {
     const int N = net.N;
     for (int l = 0; l < net.L; l++) {
-        const int prev = l > 0 ? net.ch[l - 1] : 1;
-        const int vol = net.vol[l];
         const int C = net.ch[l];
         const bool last = l == net.L - 1;
         for (int x = lane; x < N; x += 64) {
-            const int cls = int(net.sym[x]);
-            const int* nb = net.nbr + net.nbr_off[l] + int64_t(x) * vol;
             cplx<T> osum = {0, 0};
             for (int c = 0; c < C; c++) {
-                cplx<T> z = {0, 0};
-                for (int ci = 0; ci < prev; ci++) {
-                    const cplx<T>* wk = net.params + net.link_off[l] + (int64_t(ci) * C + c) * net.nsym * vol +
-                                        int64_t(cls) * vol;
-                    for (int k = 0; k < vol; k++) {
-                        const cplx<T> a = l == 0 ? layer_input<T>(0, w, st.A, N, 0, nb[k])
-                                                 : st.A[st.aoff[l - 1] + ci * N + nb[k]];
-                        const cplx<T> q = wk[k];
-                        z.re += q.re * a.re - q.im * a.im;
-                        z.im += q.re * a.im + q.im * a.re;
-                    }
-                }
+                const cplx<T> z = conv_point<T, VMAX>(net, tb, l, c, x, w, l > 0 ? st.A + st.aoff[l - 1] : st.A);
                 const cplx<T> av = act(z, l);
-                if (last)
+                if (last) {
                     osum = cadd(osum, av);
-                else
+                } else {
                     st.A[st.aoff[l] + c * N + x] = av;
+                    st.Tn[st.aoff[l] + c * N + x] = av;
+                }
             }
             if (last) st.out[x] = osum;
         }
@@ -580,9 +657,10 @@ __device__ cplx<T> cnn_inc_init(const CnnView<T>& net, const uint64_t* w, CnnInc
 }
 
 // the change of sum_x out[x] when the spins of mask F flip; R[l + 1] receives the
-// sites whose layer-l pre-activations change (for a commit)
-template <typename T>
-__device__ cplx<T> cnn_inc_delta(const CnnView<T>& net, const uint64_t* w, const uint64_t (&F)[kMaxWords],
+// sites whose layer-l pre-activations change (for a commit or a reject)
+template <typename T, int VMAX>
+__device__ __forceinline__ cplx<T> cnn_inc_delta(const CnnView<T>& net, const CnnTables<T>& tb, const uint64_t* w,
+                                                 const uint64_t (&F)[kMaxWords],
                                  CnnInc<T>& st, uint64_t (&R)[kMaxCnnLayers + 1][kMaxWords], int lane) {
     const int N = net.N;
     uint64_t wn[kMaxWords];
@@ -592,10 +670,14 @@ __device__ cplx<T> cnn_inc_delta(const CnnView<T>& net, const uint64_t* w, const
         R[0][k] = F[k];
     }
     int n = 0;
-    for (int l = 0; l < net.L; l++) {
+    // layers unrolled: R stays in registers (no run-time index into it)
+#pragma unroll
+    for (int l = 0; l < kMaxCnnLayers; l++) {
+        if (l >= net.L) break;
         // dilate: the consumers of the sites changed in the layer below
 #pragma unroll
         for (int k = 0; k < kMaxWords; k++) R[l + 1][k] = 0;
+#pragma unroll
         for (int k = 0; k < kMaxWords; k++) {
             uint64_t m = R[l][k];
             while (m) {
@@ -607,32 +689,13 @@ __device__ cplx<T> cnn_inc_delta(const CnnView<T>& net, const uint64_t* w, const
             }
         }
         n = mask_list(R[l + 1], N, st.lst, lane);
-        const int prev = l > 0 ? net.ch[l - 1] : 1;
-        const int vol = net.vol[l];
         const int C = net.ch[l];
         const bool last = l == net.L - 1;
         for (int e = lane; e < C * n; e += 64) {
             const int c = e / n, x = st.lst[e % n];
-            const int cls = int(net.sym[x]);
-            const int* nb = net.nbr + net.nbr_off[l] + int64_t(x) * vol;
-            cplx<T> z = {0, 0};
-            for (int ci = 0; ci < prev; ci++) {
-                const cplx<T>* wk = net.params + net.link_off[l] + (int64_t(ci) * C + c) * net.nsym * vol +
-                                    int64_t(cls) * vol;
-                for (int k = 0; k < vol; k++) {
-                    const int y = nb[k];
-                    cplx<T> a;
-                    if (l == 0) {
-                        a = layer_input<T>(0, wn, st.A, N, 0, y);
-                    } else {
-                        const int o = st.aoff[l - 1] + ci * N + y;
-                        a = mask_bit(R[l], y) ? st.Tn[o] : st.A[o];
-                    }
-                    const cplx<T> q = wk[k];
-                    z.re += q.re * a.re - q.im * a.im;
-                    z.im += q.re * a.im + q.im * a.re;
-                }
-            }
+            // Tn mirrors A except at the sites of this proposal (R[l]), where it holds the
+            // proposal's activations
+            const cplx<T> z = conv_point<T, VMAX>(net, tb, l, c, x, wn, l > 0 ? st.Tn + st.aoff[l - 1] : st.Tn);
             const cplx<T> av = act(z, l);
             if (last)
                 st.V[c * N + x] = av;
@@ -655,12 +718,32 @@ __device__ cplx<T> cnn_inc_delta(const CnnView<T>& net, const uint64_t* w, const
     return {wave_sum(part.re), wave_sum(part.im)};
 }
 
-// the proposal of the last cnn_inc_delta becomes the current state
+// the proposal of the last cnn_inc_delta is dropped: Tn mirrors A again
 template <typename T>
-__device__ void cnn_inc_commit(const CnnView<T>& net, CnnInc<T>& st, const uint64_t (&R)[kMaxCnnLayers + 1][kMaxWords],
+__device__ __forceinline__ void cnn_inc_reject(const CnnView<T>& net, CnnInc<T>& st, const uint64_t (&R)[kMaxCnnLayers + 1][kMaxWords],
                                int lane) {
     const int N = net.N;
-    for (int l = 0; l + 1 < net.L; l++) {
+#pragma unroll
+    for (int l = 0; l + 1 < kMaxCnnLayers; l++) {
+        if (l + 1 >= net.L) break;
+        const int n = mask_list(R[l + 1], N, st.lst, lane);
+        const int C = net.ch[l];
+        for (int e = lane; e < C * n; e += 64) {
+            const int o = st.aoff[l] + (e / n) * N + st.lst[e % n];
+            st.Tn[o] = st.A[o];
+        }
+        __syncthreads();
+    }
+}
+
+// the proposal of the last cnn_inc_delta becomes the current state
+template <typename T>
+__device__ __forceinline__ void cnn_inc_commit(const CnnView<T>& net, CnnInc<T>& st, const uint64_t (&R)[kMaxCnnLayers + 1][kMaxWords],
+                               int lane) {
+    const int N = net.N;
+#pragma unroll
+    for (int l = 0; l + 1 < kMaxCnnLayers; l++) {
+        if (l + 1 >= net.L) break;
         const int n = mask_list(R[l + 1], N, st.lst, lane);
         const int C = net.ch[l];
         for (int e = lane; e < C * n; e += 64) {
@@ -669,24 +752,30 @@ __device__ void cnn_inc_commit(const CnnView<T>& net, CnnInc<T>& st, const uint6
         }
         __syncthreads();
     }
+    uint64_t RL[kMaxWords];   // R[L] without a run-time index
+#pragma unroll
+    for (int k = 0; k < kMaxWords; k++) RL[k] = net.L == 1 ? R[1][k] : net.L == 2 ? R[2][k] : R[3][k];
     for (int x = lane; x < N; x += 64)
-        if (mask_bit(R[net.L], x)) st.out[x] = st.nout[x];
+        if (mask_bit(RL, x)) st.out[x] = st.nout[x];
     __syncthreads();
 }
 
 // Metropolis chains of a PsiCNN with local updates (same proposals and Philox stream as
 // the full-forward sampler)
-template <typename T>
-__global__ __launch_bounds__(64) void cnn_mc_inc_kernel(CnnView<T> net, uint64_t* chain_confs, int words,
+template <typename T, int VMAX>
+__global__ __launch_bounds__(64) void cnn_mc_inc_kernel(CnnView<T> gnet, uint64_t* chain_confs, int words,
                                                         int num_chains, int move, int64_t num_proposals,
                                                         uint64_t counter, uint32_t key0, uint32_t key1,
-                                                        uint32_t* accepted) {
+                                                        uint32_t* accepted, int stage) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
     const int c = blockIdx.x;
-    if (c >= num_chains) return;
+    if (c >= num_chains) return;   // the whole wave: the barriers below see all of it
+    // tables in LDS when they fit (stage = their size in bytes), the state after them
+    const CnnView<T>& net = gnet;
+    const CnnTables<T> tb = stage ? stage_cnn_tables(net, smem, lane) : CnnTables<T>{net.params, net.nbr, net.sym};
     const int N = net.N;
-    CnnInc<T> st = cnn_inc_layout(net, smem);
+    CnnInc<T> st = cnn_inc_layout(net, smem + stage);
     uint64_t w[kMaxWords];
     load_words(chain_confs, c, words, w);
     uint64_t msk[kMaxWords];
@@ -694,33 +783,15 @@ __global__ __launch_bounds__(64) void cnn_mc_inc_kernel(CnnView<T> net, uint64_t
         const int bits = N - 64 * k;
         msk[k] = k >= words ? 0 : bits >= 64 ? ~uint64_t(0) : bits > 0 ? (uint64_t(1) << bits) - 1 : 0;
     }
-    T cur = cnn_inc_init(net, w, st, lane).re;
+    T cur = cnn_inc_init<T, VMAX>(net, tb, w, st, lane).re;
     uint64_t R[kMaxCnnLayers + 1][kMaxWords];
     uint32_t acc = 0;
     for (int64_t q = 0; q < num_proposals; q++) {
         const uint64_t t = counter + uint64_t(q);
         const u32x4 r = philox(uint32_t(c), uint32_t(t), uint32_t(t >> 32), 0u, key0, key1);
         const int i = int(range32(r.x, uint32_t(N)));
-        const int si_bit = int((w[i >> 6] >> (i & 63)) & 1u);
+        const int si_bit = int((word_at(w, i >> 6) >> (i & 63)) & 1u);
         int kp = -1;
-        if (move == MOVE_PAULI) {
-            // site i takes Pauli type r.y >> 30 (possibly its current one)
-            const uint32_t ty = r.y >> 30;
-            const uint64_t bit = uint64_t(1) << i;
-            uint64_t wn[kMaxWords];
-            for (int k = 0; k < kMaxWords; k++) wn[k] = w[k];
-            wn[0] = (w[0] & ~bit) | ((ty & 1u) ? bit : 0);
-            wn[1] = (w[1] & ~bit) | ((ty & 2u) ? bit : 0);
-            const T nxt = view_forward(net, wn, smem, lane).re;
-            const T d2 = T(2) * (nxt - cur);
-            const T uu = (T(r.z) + T(0.5)) * T(2.3283064365386963e-10);
-            if (d2 > T(0) || uu <= dexp(d2)) {
-                for (int k = 0; k < kMaxWords; k++) w[k] = wn[k];
-                cur = nxt;
-                acc++;
-            }
-            continue;
-        }
         if (move == MOVE_EXCHANGE) {
             uint64_t o[kMaxWords];
             uint32_t n_opp = 0;
@@ -732,43 +803,51 @@ __global__ __launch_bounds__(64) void cnn_mc_inc_kernel(CnnView<T> net, uint64_t
             kp = select_bit(o, words, range32(r.y, n_opp));
         }
         uint64_t F[kMaxWords] = {0, 0, 0, 0};
-        F[i >> 6] ^= uint64_t(1) << (i & 63);
-        if (kp >= 0) F[kp >> 6] ^= uint64_t(1) << (kp & 63);
-        const T d = cnn_inc_delta(net, w, F, st, R, lane).re;
+        flip_bit(F, i);
+        if (kp >= 0) flip_bit(F, kp);
+        const T d = cnn_inc_delta<T, VMAX>(net, tb, w, F, st, R, lane).re;
         const T d2 = T(2) * net.final_factor * d;
         const T uu = (T(r.z) + T(0.5)) * T(2.3283064365386963e-10);
         if (d2 > T(0) || uu <= dexp(d2)) {
             for (int k = 0; k < kMaxWords; k++) w[k] ^= F[k];
             cnn_inc_commit(net, st, R, lane);
             acc++;
+        } else {
+            cnn_inc_reject(net, st, R, lane);
         }
     }
     (void)cur;
     if (lane == 0) {
-        for (int k = 0; k < words; k++) chain_confs[int64_t(c) * words + k] = w[k];
+#pragma unroll
+        for (int k = 0; k < kMaxWords; k++)
+            if (k < words) chain_confs[int64_t(c) * words + k] = w[k];
         accepted[c] += acc;
     }
 }
 
 // E_loc of PsiCNN configurations with the psi ratios from local updates
-template <typename T>
-__global__ __launch_bounds__(64) void cnn_local_energy_inc_kernel(CnnView<T> net, OpView op, const uint64_t* confs,
-                                                                  int64_t count, cplx<double>* e_loc) {
+template <typename T, int VMAX>
+__global__ __launch_bounds__(64) void cnn_local_energy_inc_kernel(CnnView<T> gnet, OpView op, const uint64_t* confs,
+                                                                  int64_t count, cplx<double>* e_loc, int stage) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
     const int words = op.words;
-    CnnInc<T> st = cnn_inc_layout(net, smem);
+    const CnnView<T>& net = gnet;
+    const CnnTables<T> tb = stage ? stage_cnn_tables(net, smem, lane) : CnnTables<T>{net.params, net.nbr, net.sym};
+    CnnInc<T> st = cnn_inc_layout(net, smem + stage);
     uint64_t R[kMaxCnnLayers + 1][kMaxWords];
     for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
         uint64_t w[kMaxWords];
         load_words(confs, s, words, w);
-        cnn_inc_init(net, w, st, lane);
+        cnn_inc_init<T, VMAX>(net, tb, w, st, lane);
         double e_re = 0.0, e_im = 0.0;
         for (int g = 0; g < op.num_groups; g++) {
             double c_re = 0.0, c_im = 0.0;
             for (int n = op.group_begin[g]; n < op.group_begin[g + 1]; n++) {
                 int par = 0;
-                for (int k = 0; k < words; k++) par += __popcll(~w[k] & op.string_zy[int64_t(n) * words + k]);
+#pragma unroll
+                for (int k = 0; k < kMaxWords; k++)
+                    if (k < words) par += __popcll(~w[k] & op.string_zy[int64_t(n) * words + k]);
                 const double sg = (par & 1) ? -1.0 : 1.0;
                 c_re += sg * op.string_coef[2 * n];
                 c_im += sg * op.string_coef[2 * n + 1];
@@ -785,7 +864,8 @@ __global__ __launch_bounds__(64) void cnn_local_energy_inc_kernel(CnnView<T> net
                 e_im += c_im;
                 continue;
             }
-            const cplx<T> dl = cnn_inc_delta(net, w, F, st, R, lane);
+            const cplx<T> dl = cnn_inc_delta<T, VMAX>(net, tb, w, F, st, R, lane);
+            cnn_inc_reject(net, st, R, lane);
             const double d_re = double(net.final_factor) * double(dl.re), d_im = double(net.final_factor) * double(dl.im);
             const double m = exp(d_re);
             double sn, cs;
@@ -798,13 +878,12 @@ __global__ __launch_bounds__(64) void cnn_local_energy_inc_kernel(CnnView<T> net
     }
 }
 
-// O_k of a multi-layer PsiDeep by back-propagation (PsiDeep.hpp:345-445); row s of O is
-// written in parameter order (minus center if given).  LDS: Z, Act (forward), D (the
-// derivative of log psi by the pre-activations of every layer).
-// O_k of a PsiDeep for the spins w into row[0 .. num_params) (minus center), by
-// back-propagation; LDS: Z, Act and D, zoff[L] values each
+// O_k of a PsiDeep (PsiDeep.hpp:345-445) for the spins w into row[0 .. num_params) in
+// parameter order (minus center if given), by back-propagation; LDS: Z, Act (forward)
+// and D (the derivative of log psi by the pre-activations of every layer), zoff[L]
+// values each
 template <typename T>
-__device__ void mlp_ok_row(const MlpView<T>& net, const uint64_t* w, unsigned char* smem, cplx<T>* row,
+__device__ __forceinline__ void mlp_ok_row(const MlpView<T>& net, const uint64_t* w, unsigned char* smem, cplx<T>* row,
                            const cplx<double>* center, int lane) {
     cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
     cplx<T>* Act = Z + net.zoff[net.L];
@@ -942,6 +1021,33 @@ template <typename T>
 size_t ok_lds(const MlpView<T>& net) {
     return sizeof(cplx<T>) * 3 * size_t(net.zoff[net.L]);
 }
+template <int V>
+struct VmaxTag {
+    static constexpr int value = V;
+};
+
+// the local-update kernels with the neighbour indices of one site in registers when
+// every layer's kernel volume is at most 9 (3 x 3) or 27 (3 x 3 x 3), else the plain loop
+template <typename T, typename F>
+void dispatch_vmax(const CnnView<T>& net, F&& fn) {
+    int vmax = 0;
+    for (int l = 0; l < net.L; l++) vmax = std::max(vmax, net.vol[l]);
+    if (vmax <= 9)
+        fn(VmaxTag<9>{});
+    else if (vmax <= 27)
+        fn(VmaxTag<27>{});
+    else
+        fn(VmaxTag<0>{});
+}
+
+// bytes of the LDS copy of a PsiCNN's tables ahead of the local-update state, 0 when
+// both would not fit the 64 KiB that leave room for two chains per CU
+template <typename T>
+int cnn_stage_bytes(const CnnView<T>& net) {
+    const size_t t = cnn_table_lds(net);
+    return t + cnn_inc_lds(net) <= 64 * 1024 ? int(t) : 0;
+}
+
 template <typename V>
 size_t forward_lds(const PauliInput<V>& net) {
     return forward_lds(net.net);
@@ -976,10 +1082,14 @@ void launch_generic_mc(const V& net, uint64_t* chain_confs, int words, int num_c
     if (num_chains <= 0 || num_proposals <= 0) return;
     using T = typename V::real_t;
     if constexpr (std::is_same<V, CnnView<T>>::value) {
-        allow_lds(cnn_mc_inc_kernel<T>, cnn_inc_lds(net));
-        cnn_mc_inc_kernel<T><<<unsigned(num_chains), 64, cnn_inc_lds(net), stream>>>(
-            net, chain_confs, words, num_chains, move, num_proposals, counter, uint32_t(seed), uint32_t(seed >> 32),
-            accepted);
+        const int stage = cnn_stage_bytes(net);
+        dispatch_vmax(net, [&](auto vt) {
+            constexpr int VM = decltype(vt)::value;
+            allow_lds(cnn_mc_inc_kernel<T, VM>, cnn_inc_lds(net) + stage);
+            cnn_mc_inc_kernel<T, VM><<<unsigned(num_chains), 64, cnn_inc_lds(net) + stage, stream>>>(
+                net, chain_confs, words, num_chains, move, num_proposals, counter, uint32_t(seed),
+                uint32_t(seed >> 32), accepted, stage);
+        });
     } else {
         allow_lds(generic_mc_kernel<V>, forward_lds(net));
         generic_mc_kernel<V><<<unsigned(num_chains), 64, forward_lds(net), stream>>>(
@@ -995,9 +1105,13 @@ void launch_generic_local_energy(const V& net, const OpView& op, const uint64_t*
     if (count <= 0) return;
     using T = typename V::real_t;
     if constexpr (std::is_same<V, CnnView<T>>::value) {
-        allow_lds(cnn_local_energy_inc_kernel<T>, cnn_inc_lds(net));
-        cnn_local_energy_inc_kernel<T><<<grid_for(count), 64, cnn_inc_lds(net), stream>>>(net, op, confs, count,
-                                                                                           e_loc);
+        const int stage = cnn_stage_bytes(net);
+        dispatch_vmax(net, [&](auto vt) {
+            constexpr int VM = decltype(vt)::value;
+            allow_lds(cnn_local_energy_inc_kernel<T, VM>, cnn_inc_lds(net) + stage);
+            cnn_local_energy_inc_kernel<T, VM><<<grid_for(count), 64, cnn_inc_lds(net) + stage, stream>>>(
+                net, op, confs, count, e_loc, stage);
+        });
     } else {
         allow_lds(generic_local_energy_kernel<V>, forward_lds(net));
         generic_local_energy_kernel<V><<<grid_for(count), 64, forward_lds(net), stream>>>(net, op, confs, count,

@@ -169,12 +169,27 @@ __device__ inline void flip_bit(Conf& c, int i) {
 // index of the rank-th set bit of mask (words of the configuration); defined below
 __device__ inline int select64(uint64_t x, uint32_t rank);
 __device__ inline int select_bit(const uint64_t* m, int words, uint32_t rank) {
-    for (int w = 0; w < words; w++) {
-        const uint32_t c = uint32_t(__popcll(m[w]));
-        if (rank < c) return w * 64 + select64(m[w], rank);
-        rank -= c;
+    // unrolled over the largest word count: a register array is never indexed at run time
+    int pos = -1;
+#pragma unroll
+    for (int w = 0; w < kMaxWords; w++) {
+        const uint32_t c = w < words ? uint32_t(__popcll(m[w])) : 0u;
+        if (pos < 0 && rank < c) pos = w * 64 + select64(m[w], rank);
+        rank = pos < 0 ? rank - c : rank;
     }
-    return -1;
+    return pos;
+}
+
+// word idx of a configuration in registers without a run-time array index
+__device__ inline uint64_t word_at(const uint64_t* m, int idx) {
+    return idx == 0 ? m[0] : idx == 1 ? m[1] : idx == 2 ? m[2] : m[3];
+}
+
+// m ^= bit i, unrolled over the words
+__device__ inline void flip_bit(uint64_t* m, int i) {
+#pragma unroll
+    for (int k = 0; k < kMaxWords; k++)
+        if (k == (i >> 6)) m[k] ^= uint64_t(1) << (i & 63);
 }
 
 // ------------------------------------------------------ wave reductions
