@@ -527,32 +527,33 @@ __host__ __device__ __forceinline__ int cnn_nbr_elems(int L, const int* nbr_off,
 
 template <typename T>
 size_t cnn_table_lds(const CnnView<T>& net) {
+    // neighbour indices and symmetry classes as bytes (at most 256 sites and classes)
     const size_t p = (sizeof(cplx<T>) * size_t(net.num_params) + 15) / 16 * 16;
-    const size_t nb = (sizeof(int) * size_t(cnn_nbr_elems(net.L, net.nbr_off, net.vol, net.N)) + 15) / 16 * 16;
-    return p + nb + (sizeof(uint32_t) * size_t(net.N) + 15) / 16 * 16;
+    const size_t nb = (size_t(cnn_nbr_elems(net.L, net.nbr_off, net.vol, net.N)) + 15) / 16 * 16;
+    return p + nb + (size_t(net.N) + 15) / 16 * 16;
 }
 
 // where the local updates read the tables: LDS copies or the view's global arrays (kept
 // apart from the view so that the view stays in the kernel arguments, where its small
 // per-layer arrays are indexed by scalar loads)
-template <typename T>
+template <typename T, typename NI, typename SI>
 struct CnnTables {
     const cplx<T>* params;
-    const int* nbr;
-    const uint32_t* sym;
+    const NI* nbr;
+    const SI* sym;
 };
 
 // copies the tables to smem and returns where they are
 template <typename T>
-__device__ __forceinline__ CnnTables<T> stage_cnn_tables(const CnnView<T>& net, unsigned char* smem, int lane) {
+__device__ __forceinline__ CnnTables<T, uint8_t, uint8_t> stage_cnn_tables(const CnnView<T>& net,
+                                                                           unsigned char* smem, int lane) {
     cplx<T>* p = reinterpret_cast<cplx<T>*>(smem);
     for (int k = lane; k < net.num_params; k += 64) p[k] = net.params[k];
-    int* nb = reinterpret_cast<int*>(smem + (sizeof(cplx<T>) * size_t(net.num_params) + 15) / 16 * 16);
+    uint8_t* nb = smem + (sizeof(cplx<T>) * size_t(net.num_params) + 15) / 16 * 16;
     const int nnb = cnn_nbr_elems(net.L, net.nbr_off, net.vol, net.N);
-    for (int k = lane; k < nnb; k += 64) nb[k] = net.nbr[k];
-    uint32_t* sy = reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(nb) +
-                                               (sizeof(int) * size_t(nnb) + 15) / 16 * 16);
-    for (int k = lane; k < net.N; k += 64) sy[k] = net.sym[k];
+    for (int k = lane; k < nnb; k += 64) nb[k] = uint8_t(net.nbr[k]);
+    uint8_t* sy = nb + (size_t(nnb) + 15) / 16 * 16;
+    for (int k = lane; k < net.N; k += 64) sy[k] = uint8_t(net.sym[k]);
     __syncthreads();
     return {p, nb, sy};
 }
@@ -583,15 +584,15 @@ __device__ __forceinline__ int mask_list(const uint64_t (&m)[kMaxWords], int N, 
 // neighbour k of x (the spins w for l = 0, else in[ci N + y]).  With VMAX > 0 (an upper
 // bound of the kernel volume) the neighbour indices sit in registers and the loads of one
 // input channel are issued together; VMAX = 0 is the plain loop for any volume.
-template <typename T, int VMAX>
-__device__ __forceinline__ cplx<T> conv_point(const CnnView<T>& net, const CnnTables<T>& tb, int l, int c, int x,
+template <typename T, int VMAX, typename TB>
+__device__ __forceinline__ cplx<T> conv_point(const CnnView<T>& net, const TB& tb, int l, int c, int x,
                                               const uint64_t* w, const cplx<T>* in) {
     const int N = net.N;
     const int prev = l > 0 ? net.ch[l - 1] : 1;
     const int vol = net.vol[l];
     const int C = net.ch[l];
     const int cls = int(tb.sym[x]);
-    const int* nb = tb.nbr + net.nbr_off[l] + int64_t(x) * vol;
+    const auto* nb = tb.nbr + net.nbr_off[l] + int64_t(x) * vol;
     cplx<T> z = {0, 0};
     auto input = [&](int ci, int y) -> cplx<T> {
         return l == 0 ? cplx<T>{((word_at(w, y >> 6) >> (y & 63)) & 1) ? T(1) : T(-1), T(0)} : in[ci * N + y];
@@ -628,8 +629,8 @@ __device__ __forceinline__ cplx<T> conv_point(const CnnView<T>& net, const CnnTa
 
 // full forward at configuration w into the state; returns sum_x out[x] (the network
 // part of log psi before the final factor)
-template <typename T, int VMAX>
-__device__ __forceinline__ cplx<T> cnn_inc_init(const CnnView<T>& net, const CnnTables<T>& tb, const uint64_t* w,
+template <typename T, int VMAX, typename TB>
+__device__ __forceinline__ cplx<T> cnn_inc_init(const CnnView<T>& net, const TB& tb, const uint64_t* w,
                                                 CnnInc<T>& st, int lane) {
     const int N = net.N;
     for (int l = 0; l < net.L; l++) {
@@ -638,7 +639,7 @@ __device__ __forceinline__ cplx<T> cnn_inc_init(const CnnView<T>& net, const Cnn
         for (int x = lane; x < N; x += 64) {
             cplx<T> osum = {0, 0};
             for (int c = 0; c < C; c++) {
-                const cplx<T> z = conv_point<T, VMAX>(net, tb, l, c, x, w, l > 0 ? st.A + st.aoff[l - 1] : st.A);
+                const cplx<T> z = conv_point<T, VMAX, TB>(net, tb, l, c, x, w, l > 0 ? st.A + st.aoff[l - 1] : st.A);
                 const cplx<T> av = act(z, l);
                 if (last) {
                     osum = cadd(osum, av);
@@ -658,8 +659,8 @@ __device__ __forceinline__ cplx<T> cnn_inc_init(const CnnView<T>& net, const Cnn
 
 // the change of sum_x out[x] when the spins of mask F flip; R[l + 1] receives the
 // sites whose layer-l pre-activations change (for a commit or a reject)
-template <typename T, int VMAX>
-__device__ __forceinline__ cplx<T> cnn_inc_delta(const CnnView<T>& net, const CnnTables<T>& tb, const uint64_t* w,
+template <typename T, int VMAX, typename TB>
+__device__ __forceinline__ cplx<T> cnn_inc_delta(const CnnView<T>& net, const TB& tb, const uint64_t* w,
                                                  const uint64_t (&F)[kMaxWords],
                                  CnnInc<T>& st, uint64_t (&R)[kMaxCnnLayers + 1][kMaxWords], int lane) {
     const int N = net.N;
@@ -695,7 +696,7 @@ __device__ __forceinline__ cplx<T> cnn_inc_delta(const CnnView<T>& net, const Cn
             const int c = e / n, x = st.lst[e % n];
             // Tn mirrors A except at the sites of this proposal (R[l]), where it holds the
             // proposal's activations
-            const cplx<T> z = conv_point<T, VMAX>(net, tb, l, c, x, wn, l > 0 ? st.Tn + st.aoff[l - 1] : st.Tn);
+            const cplx<T> z = conv_point<T, VMAX, TB>(net, tb, l, c, x, wn, l > 0 ? st.Tn + st.aoff[l - 1] : st.Tn);
             const cplx<T> av = act(z, l);
             if (last)
                 st.V[c * N + x] = av;
@@ -762,7 +763,7 @@ __device__ __forceinline__ void cnn_inc_commit(const CnnView<T>& net, CnnInc<T>&
 
 // Metropolis chains of a PsiCNN with local updates (same proposals and Philox stream as
 // the full-forward sampler)
-template <typename T, int VMAX>
+template <typename T, int VMAX, bool STAGE>
 __global__ __launch_bounds__(64) void cnn_mc_inc_kernel(CnnView<T> gnet, uint64_t* chain_confs, int words,
                                                         int num_chains, int move, int64_t num_proposals,
                                                         uint64_t counter, uint32_t key0, uint32_t key1,
@@ -773,7 +774,12 @@ __global__ __launch_bounds__(64) void cnn_mc_inc_kernel(CnnView<T> gnet, uint64_
     if (c >= num_chains) return;   // the whole wave: the barriers below see all of it
     // tables in LDS when they fit (stage = their size in bytes), the state after them
     const CnnView<T>& net = gnet;
-    const CnnTables<T> tb = stage ? stage_cnn_tables(net, smem, lane) : CnnTables<T>{net.params, net.nbr, net.sym};
+    using TB = std::conditional_t<STAGE, CnnTables<T, uint8_t, uint8_t>, CnnTables<T, int, uint32_t>>;
+    TB tb;
+    if constexpr (STAGE)
+        tb = stage_cnn_tables(net, smem, lane);
+    else
+        tb = TB{net.params, net.nbr, net.sym};
     const int N = net.N;
     CnnInc<T> st = cnn_inc_layout(net, smem + stage);
     uint64_t w[kMaxWords];
@@ -783,7 +789,7 @@ __global__ __launch_bounds__(64) void cnn_mc_inc_kernel(CnnView<T> gnet, uint64_
         const int bits = N - 64 * k;
         msk[k] = k >= words ? 0 : bits >= 64 ? ~uint64_t(0) : bits > 0 ? (uint64_t(1) << bits) - 1 : 0;
     }
-    T cur = cnn_inc_init<T, VMAX>(net, tb, w, st, lane).re;
+    T cur = cnn_inc_init<T, VMAX, TB>(net, tb, w, st, lane).re;
     uint64_t R[kMaxCnnLayers + 1][kMaxWords];
     uint32_t acc = 0;
     for (int64_t q = 0; q < num_proposals; q++) {
@@ -805,7 +811,7 @@ __global__ __launch_bounds__(64) void cnn_mc_inc_kernel(CnnView<T> gnet, uint64_
         uint64_t F[kMaxWords] = {0, 0, 0, 0};
         flip_bit(F, i);
         if (kp >= 0) flip_bit(F, kp);
-        const T d = cnn_inc_delta<T, VMAX>(net, tb, w, F, st, R, lane).re;
+        const T d = cnn_inc_delta<T, VMAX, TB>(net, tb, w, F, st, R, lane).re;
         const T d2 = T(2) * net.final_factor * d;
         const T uu = (T(r.z) + T(0.5)) * T(2.3283064365386963e-10);
         if (d2 > T(0) || uu <= dexp(d2)) {
@@ -826,20 +832,25 @@ __global__ __launch_bounds__(64) void cnn_mc_inc_kernel(CnnView<T> gnet, uint64_
 }
 
 // E_loc of PsiCNN configurations with the psi ratios from local updates
-template <typename T, int VMAX>
+template <typename T, int VMAX, bool STAGE>
 __global__ __launch_bounds__(64) void cnn_local_energy_inc_kernel(CnnView<T> gnet, OpView op, const uint64_t* confs,
                                                                   int64_t count, cplx<double>* e_loc, int stage) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
     const int words = op.words;
     const CnnView<T>& net = gnet;
-    const CnnTables<T> tb = stage ? stage_cnn_tables(net, smem, lane) : CnnTables<T>{net.params, net.nbr, net.sym};
+    using TB = std::conditional_t<STAGE, CnnTables<T, uint8_t, uint8_t>, CnnTables<T, int, uint32_t>>;
+    TB tb;
+    if constexpr (STAGE)
+        tb = stage_cnn_tables(net, smem, lane);
+    else
+        tb = TB{net.params, net.nbr, net.sym};
     CnnInc<T> st = cnn_inc_layout(net, smem + stage);
     uint64_t R[kMaxCnnLayers + 1][kMaxWords];
     for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
         uint64_t w[kMaxWords];
         load_words(confs, s, words, w);
-        cnn_inc_init<T, VMAX>(net, tb, w, st, lane);
+        cnn_inc_init<T, VMAX, TB>(net, tb, w, st, lane);
         double e_re = 0.0, e_im = 0.0;
         for (int g = 0; g < op.num_groups; g++) {
             double c_re = 0.0, c_im = 0.0;
@@ -864,7 +875,7 @@ __global__ __launch_bounds__(64) void cnn_local_energy_inc_kernel(CnnView<T> gne
                 e_im += c_im;
                 continue;
             }
-            const cplx<T> dl = cnn_inc_delta<T, VMAX>(net, tb, w, F, st, R, lane);
+            const cplx<T> dl = cnn_inc_delta<T, VMAX, TB>(net, tb, w, F, st, R, lane);
             cnn_inc_reject(net, st, R, lane);
             const double d_re = double(net.final_factor) * double(dl.re), d_im = double(net.final_factor) * double(dl.im);
             const double m = exp(d_re);
@@ -1040,12 +1051,13 @@ void dispatch_vmax(const CnnView<T>& net, F&& fn) {
         fn(VmaxTag<0>{});
 }
 
-// bytes of the LDS copy of a PsiCNN's tables ahead of the local-update state, 0 when
-// both would not fit the 64 KiB that leave room for two chains per CU
+// bytes of the LDS copy of a PsiCNN's tables ahead of the local-update state (one chain
+// per workgroup), 0 when they would leave fewer than four chains per CU (the kernels
+// are latency-bound: chains per CU count for more than where the tables live)
 template <typename T>
 int cnn_stage_bytes(const CnnView<T>& net) {
     const size_t t = cnn_table_lds(net);
-    return t + cnn_inc_lds(net) <= 64 * 1024 ? int(t) : 0;
+    return 4 * (t + cnn_inc_lds(net)) <= 160 * 1024 ? int(t) : 0;
 }
 
 template <typename V>
@@ -1085,10 +1097,16 @@ void launch_generic_mc(const V& net, uint64_t* chain_confs, int words, int num_c
         const int stage = cnn_stage_bytes(net);
         dispatch_vmax(net, [&](auto vt) {
             constexpr int VM = decltype(vt)::value;
-            allow_lds(cnn_mc_inc_kernel<T, VM>, cnn_inc_lds(net) + stage);
-            cnn_mc_inc_kernel<T, VM><<<unsigned(num_chains), 64, cnn_inc_lds(net) + stage, stream>>>(
-                net, chain_confs, words, num_chains, move, num_proposals, counter, uint32_t(seed),
-                uint32_t(seed >> 32), accepted, stage);
+            auto run = [&](auto kern) {
+                allow_lds(kern, cnn_inc_lds(net) + stage);
+                kern<<<unsigned(num_chains), 64, cnn_inc_lds(net) + stage, stream>>>(
+                    net, chain_confs, words, num_chains, move, num_proposals, counter, uint32_t(seed),
+                    uint32_t(seed >> 32), accepted, stage);
+            };
+            if (stage)
+                run(cnn_mc_inc_kernel<T, VM, true>);
+            else
+                run(cnn_mc_inc_kernel<T, VM, false>);
         });
     } else {
         allow_lds(generic_mc_kernel<V>, forward_lds(net));
@@ -1108,9 +1126,14 @@ void launch_generic_local_energy(const V& net, const OpView& op, const uint64_t*
         const int stage = cnn_stage_bytes(net);
         dispatch_vmax(net, [&](auto vt) {
             constexpr int VM = decltype(vt)::value;
-            allow_lds(cnn_local_energy_inc_kernel<T, VM>, cnn_inc_lds(net) + stage);
-            cnn_local_energy_inc_kernel<T, VM><<<grid_for(count), 64, cnn_inc_lds(net) + stage, stream>>>(
-                net, op, confs, count, e_loc, stage);
+            auto run = [&](auto kern) {
+                allow_lds(kern, cnn_inc_lds(net) + stage);
+                kern<<<grid_for(count), 64, cnn_inc_lds(net) + stage, stream>>>(net, op, confs, count, e_loc, stage);
+            };
+            if (stage)
+                run(cnn_local_energy_inc_kernel<T, VM, true>);
+            else
+                run(cnn_local_energy_inc_kernel<T, VM, false>);
         });
     } else {
         allow_lds(generic_local_energy_kernel<V>, forward_lds(net));
