@@ -26,8 +26,7 @@ namespace {
 template <typename T>
 __device__ __forceinline__ cplx<T> layer_input(int l, const uint64_t* w, const cplx<T>* A, int N, int ci, int y) {
     if (l == 0) {
-        const uint64_t word = y < 64 ? w[0] : y < 128 ? w[1] : y < 192 ? w[2] : w[3];
-        return {((word >> (y & 63)) & 1) ? T(1) : T(-1), T(0)};
+        return {((word_at(w, y >> 6) >> (y & 63)) & 1) ? T(1) : T(-1), T(0)};
     }
     return A[ci * N + y];
 }
@@ -97,8 +96,7 @@ __device__ __forceinline__ cplx<T> mlp_forward(const MlpView<T>& net, const uint
                 const int src = int(net.cn[l][i * M + j]);
                 const cplx<T> q = net.W[l][i * M + j];
                 if (l == 0) {
-                    const uint64_t word = src < 64 ? w[0] : src < 128 ? w[1] : src < 192 ? w[2] : w[3];
-                    const T sv = ((word >> (src & 63)) & 1) ? T(1) : T(-1);
+                    const T sv = ((word_at(w, src >> 6) >> (src & 63)) & 1) ? T(1) : T(-1);
                     z.re += q.re * sv;
                     z.im += q.im * sv;
                 } else {

@@ -148,12 +148,13 @@ struct Conf {
     uint64_t w[kMaxWords];
 };
 
+// word k of a configuration in registers, picked by masks: a select between loads of
+// the words can be folded into one load at a run-time index, which would move the
+// whole array to scratch memory
 __device__ inline uint64_t word_sel(const Conf& c, int k) {
-    // branch-free select keeps the words in registers (no scratch indexing)
-    uint64_t r = c.w[0];
-    r = k == 1 ? c.w[1] : r;
-    r = k == 2 ? c.w[2] : r;
-    r = k == 3 ? c.w[3] : r;
+    uint64_t r = 0;
+#pragma unroll
+    for (int j = 0; j < kMaxWords; j++) r |= c.w[j] & (uint64_t(0) - uint64_t(k == j));
     return r;
 }
 __device__ inline int spin_bit(const Conf& c, int i) { return int((word_sel(c, i >> 6) >> (i & 63)) & 1u); }
@@ -181,15 +182,19 @@ __device__ inline int select_bit(const uint64_t* m, int words, uint32_t rank) {
 }
 
 // word idx of a configuration in registers without a run-time array index
+// (m has kMaxWords words; picked by masks, as in word_sel)
 __device__ inline uint64_t word_at(const uint64_t* m, int idx) {
-    return idx == 0 ? m[0] : idx == 1 ? m[1] : idx == 2 ? m[2] : m[3];
+    uint64_t r = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxWords; k++) r |= m[k] & (uint64_t(0) - uint64_t(idx == k));
+    return r;
 }
 
 // m ^= bit i, unrolled over the words
 __device__ inline void flip_bit(uint64_t* m, int i) {
+    const uint64_t b = uint64_t(1) << (i & 63);
 #pragma unroll
-    for (int k = 0; k < kMaxWords; k++)
-        if (k == (i >> 6)) m[k] ^= uint64_t(1) << (i & 63);
+    for (int k = 0; k < kMaxWords; k++) m[k] ^= b & (uint64_t(0) - uint64_t(k == (i >> 6)));
 }
 
 // ------------------------------------------------------ wave reductions

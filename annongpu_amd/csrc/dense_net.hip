@@ -361,7 +361,10 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void local_energy_k
         const int b0 = ld_uniform(op.group_begin + g), b1 = ld_uniform(op.group_begin + g + 1);
         for (int n = uniform_int(b0); n < b1; n++) {
             int par = 0;
-            for (int w = 0; w < words; w++) par += __popcll(~conf.w[w] & ld_uniform(op.string_zy + n * words + w));
+            // unrolled over the largest word count: the configuration stays in registers
+#pragma unroll
+            for (int w = 0; w < kMaxWords; w++)
+                if (w < words) par += __popcll(~conf.w[w] & ld_uniform(op.string_zy + n * words + w));
             const double sg = (par & 1) ? -1.0 : 1.0;
             c_re += sg * ld_uniform(op.string_coef + 2 * n);
             c_im += sg * ld_uniform(op.string_coef + 2 * n + 1);
@@ -381,12 +384,16 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void local_energy_k
         cplx<T> tn[U];
 #pragma unroll
         for (int u = 0; u < U; u++) tn[u] = th[u];
-        for (int w = 0; w < words; w++) {
-            uint64_t m = ld_uniform(op.group_flip + g * words + w);
+        // unrolled over the largest word count: the configuration stays in registers
+#pragma unroll
+        for (int w = 0; w < kMaxWords; w++) {
+            uint64_t m = w < words ? ld_uniform(op.group_flip + g * words + w) : 0;
+            const uint64_t cw = conf.w[w];
             while (m) {
-                const int i = w * 64 + __ffsll((unsigned long long)m) - 1;
+                const int b = __ffsll((unsigned long long)m) - 1;
+                const int i = w * 64 + b;
                 m &= m - 1;
-                const T ci = spin_bit(conf, i) ? T(-2) : T(2);
+                const T ci = ((cw >> b) & 1) ? T(-2) : T(2);
                 const cplx<T>* wi = net.Wd + int64_t(i) * net.Mpad + lane;
 #pragma unroll
                 for (int u = 0; u < U; u++) {

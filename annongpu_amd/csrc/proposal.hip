@@ -386,7 +386,12 @@ __global__ void build_inputs_kernel(TrainBatch b, const uint64_t* confs, int wor
     b.Mask[e] = in_block ? 1.f : 0.f;
 }
 
-// C[m][n] = sum_k A(m, k) B(k, n) (+ bias[n], then tanh), fp32; 64x64 tiles, 4x4 per thread
+// C[m][n] = sum_k A(m, k) B(k, n) (+ bias[n], then tanh), fp32 on the matrix cores
+// (v_mfma_f32_16x16x4_f32: fp32 operands, fp32 accumulation).  64x64 tiles staged
+// through LDS 16 k at a time; wave w of the 4 takes the 32x32 quarter (w / 2, w % 2) as
+// 2 x 2 MFMA tiles.  Operand layout of one 16x16x4 product: lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16] and receives D[4 (l / 16) + r][l % 16], r < 4.
+typedef float sg_float4_t __attribute__((ext_vector_type(4)));
 constexpr int kTK = 16;
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void sgemm_kernel(int M, int Nn, int K, const float* __restrict__ A, int lda,
@@ -399,13 +404,15 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int Nn, int K, const 
     C += int64_t(blockIdx.z) * M * ldc;
     __shared__ float As[kTK][64 + 4];
     __shared__ float Bs[kTK][64 + 4];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int kq = lane >> 4, c = lane & 15;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-    float acc[4][4];
+    sg_float4_t acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0.f;
+        for (int j = 0; j < 2; j++) acc[i][j] = sg_float4_t{0.f, 0.f, 0.f, 0.f};
     for (int k0 = kbeg; k0 < kend; k0 += kTK) {
         for (int e = threadIdx.x; e < 64 * kTK; e += 256) {
             int m, k;
@@ -435,33 +442,32 @@ __global__ __launch_bounds__(256) void sgemm_kernel(int M, int Nn, int K, const 
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kTK; k++) {
-            float a[4], bb[4];
+        for (int kk = 0; kk < kTK; kk += 4) {
+            float a[2], b[2];
 #pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = As[k][ty * 4 + i];
+            for (int i = 0; i < 2; i++) a[i] = As[kk + kq][wm + 16 * i + c];
 #pragma unroll
-            for (int j = 0; j < 4; j++) bb[j] = Bs[k][tx * 4 + j];
+            for (int j = 0; j < 2; j++) b[j] = Bs[kk + kq][wn + 16 * j + c];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = fmaf(a[i], bb[j], acc[i][j]);
+                for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int m = m0 + ty * 4 + i;
-        if (m >= M) continue;
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int n = n0 + tx * 4 + j;
-            if (n >= Nn) continue;
-            float v = acc[i][j];
-            if constexpr (EPI >= 1) v += bias[n];
-            if constexpr (EPI == 2) v = tanhf(v);
-            C[int64_t(m) * ldc + n] = v;
-        }
-    }
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + 16 * i + 4 * kq + r, n = n0 + wn + 16 * j + c;
+                if (m >= M || n >= Nn) continue;
+                float v = acc[i][j][r];
+                if constexpr (EPI >= 1) v += bias[n];
+                if constexpr (EPI == 2) v = tanhf(v);
+                C[int64_t(m) * ldc + n] = v;
+            }
 }
 
 template <bool TA, bool TB, int EPI>
