@@ -552,8 +552,10 @@ class PsiClassical(_Psi):
 
     L_n(s) = sum over the strings of H_local[n] of c_m <s|P_m|s'> (Operator::fast_local_energy,
     operator/Operator.hpp:123-135), on spin configurations or Pauli strings.  psi_ref is a
-    PsiFullyPolarized (log psi_ref = 0, no parameters) or a PsiDeep; with order 2 a PsiDeep
-    reference enters log psi and its parameters follow the classical ones.
+    PsiFullyPolarized (log psi_ref = 0, no parameters), a PsiCNN (the reference's
+    PsiClassicalANN, PsiClassical.hpp:245-256; spin configurations) or a PsiDeep (also on
+    Pauli strings); with order 2 a network reference enters log psi and its parameters follow
+    the classical ones.
     """
 
     def __init__(self, num_sites, H_local, params, psi_ref, log_prefactor=0.0, gpu=True, order=1):
@@ -569,8 +571,8 @@ class PsiClassical(_Psi):
             psi_ref = PsiFullyPolarized(num_sites, 0.0)
         self._ref_template = psi_ref
         net = None if isinstance(psi_ref, PsiFullyPolarized) else psi_ref
-        if net is not None and not isinstance(net, PsiDeep):
-            raise TypeError("psi_ref must be a PsiFullyPolarized or a PsiDeep")
+        if net is not None and not isinstance(net, (PsiDeep, PsiCNN)):
+            raise TypeError("psi_ref must be a PsiFullyPolarized, a PsiCNN or a PsiDeep")
         self._precision = "fp64"
         self._h = handle()
         check(lib.ann_psi_create_classical(ref(self._h), self._num_sites, len(ops), ctypes.cast(hs, ctypes.c_void_p),
@@ -656,7 +658,11 @@ class PsiClassical(_Psi):
         """vCN_from_json (pyANNonGPU/PsiClassical.py:28-67)"""
         obj = json.loads(json.dumps(json_obj, cls=NumpyEncoder), cls=NumpyDecoder)
         r = obj["psi_ref"]
-        psi_ref = PsiFullyPolarized.from_json(r) if r["type"] == "PsiFullyPolarized" else PsiDeep.from_json(r, gpu)
+        readers = {"PsiFullyPolarized": lambda o: PsiFullyPolarized.from_json(o),
+                   "PsiDeep": lambda o: PsiDeep.from_json(o, gpu), "PsiCNN": lambda o: PsiCNN.from_json(o, gpu)}
+        if r["type"] not in readers:
+            raise ValueError(f"unknown reference network type {r['type']!r}")
+        psi_ref = readers[r["type"]](r)
         H_local = [Operator(PauliExpression.from_json(h)) for h in obj["ansatz"]]
         return PsiClassical(obj["num_sites"], H_local, obj["params"], psi_ref,
                             obj["log_prefactor_re"] + 1j * obj["log_prefactor_im"], gpu, order=obj["order"])

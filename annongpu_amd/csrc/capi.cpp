@@ -281,7 +281,8 @@ int ann_psi_create_classical(ann_psi* out, int32_t num_sites, int32_t num_ops, c
         }
         if (order != 1 && order != 2) throw std::invalid_argument("order must be 1 or 2");
         if (ref) {
-            if (ref->impl.kind != ann::PSI_DEEP) throw std::invalid_argument("the network reference must be a PsiDeep");
+            if (ref->impl.kind != ann::PSI_DEEP && ref->impl.kind != ann::PSI_CNN)
+                throw std::invalid_argument("the network reference must be a PsiCNN or a PsiDeep");
             if (ref->impl.num_sites != num_sites) throw std::invalid_argument("psi_ref has another num_sites");
         }
         auto p = new ann_psi_s;

@@ -159,8 +159,8 @@ __device__ __forceinline__ cplx<double> pauli_phase(uint64_t pa, uint64_t pb, ui
 }
 
 // L_n(s) of PsiClassical (Operator::fast_local_energy over the strings of H_local[n])
-template <typename T>
-__device__ __forceinline__ cplx<double> classical_term(const ClassicalView<T>& v, int n, const uint64_t* w) {
+template <typename T, typename R>
+__device__ __forceinline__ cplx<double> classical_term(const ClassicalView<T, R>& v, int n, const uint64_t* w) {
     double re = 0.0, im = 0.0;
     for (int m = v.op_begin[n]; m < v.op_begin[n + 1]; m++) {
         cplx<double> f;
@@ -180,8 +180,8 @@ __device__ __forceinline__ cplx<double> classical_term(const ClassicalView<T>& v
     return {re, im};
 }
 
-template <typename T>
-__device__ __forceinline__ cplx<T> view_forward(const ClassicalView<T>& v, const uint64_t* w, unsigned char* smem, int lane) {
+template <typename T, typename R>
+__device__ __forceinline__ cplx<T> view_forward(const ClassicalView<T, R>& v, const uint64_t* w, unsigned char* smem, int lane) {
     double re = 0.0, im = 0.0;
     for (int n = lane; n < v.num_ops; n += 64) {
         const cplx<double> L = classical_term(v, n, w);
@@ -210,8 +210,8 @@ template <typename T>
 __device__ __forceinline__ int view_sites(const MlpView<T>& v) { return v.N; }
 template <typename V>
 __device__ __forceinline__ int view_sites(const PauliInput<V>& v) { return v.N; }
-template <typename T>
-__device__ __forceinline__ int view_sites(const ClassicalView<T>& v) { return v.N; }
+template <typename T, typename R>
+__device__ __forceinline__ int view_sites(const ClassicalView<T, R>& v) { return v.N; }
 
 // s is uniform over the wavefront, so these become scalar loads
 __device__ __forceinline__ void load_words(const uint64_t* confs, int64_t s, int words, uint64_t* w) {
@@ -402,21 +402,16 @@ __global__ __launch_bounds__(64) void generic_local_energy_pauli_kernel(V net, P
 // O_k of PsiCNN by back-propagation; row s of O is written in parameter order
 // (minus center if given).  LDS: Z (all pre-activations), A, A2, G (gradients).
 template <typename T>
-__global__ __launch_bounds__(64) void cnn_ok_kernel(CnnView<T> net, const uint64_t* confs, int words, int64_t count,
-                                                    const cplx<double>* center, cplx<T>* O) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void cnn_ok_row(const CnnView<T>& net, const uint64_t* w, unsigned char* smem, cplx<T>* row,
+                                           const cplx<double>* center, int lane) {
     const int N = net.N;
     cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
     cplx<T>* A = Z + net.zoff[net.L];
     cplx<T>* A2 = A + net.max_ch * N;
     cplx<T>* G = A2 + net.max_ch * N;    // d logpsi / d output activation of the current layer
     cplx<T>* D = G + net.max_ch * N;     // d logpsi / d pre-activation
-    const int lane = threadIdx.x;
-    for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
-        uint64_t w[kMaxWords];
-        load_words(confs, s, words, w);
+    {
         cnn_forward(net, w, A, A2, Z, lane);
-        cplx<T>* row = O + s * net.num_params;
         const int last = net.L - 1;
         for (int e = lane; e < net.ch[last] * N; e += 64) G[e] = {net.final_factor, T(0)};
         __syncthreads();
@@ -467,6 +462,19 @@ __global__ __launch_bounds__(64) void cnn_ok_kernel(CnnView<T> net, const uint64
             }
             __syncthreads();
         }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void cnn_ok_kernel(CnnView<T> net, const uint64_t* confs, int words, int64_t count,
+                                                    const cplx<double>* center, cplx<T>* O) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
+        uint64_t w[kMaxWords];
+        load_words(confs, s, words, w);
+        cnn_ok_row(net, w, smem, O + s * net.num_params, center, lane);
+        __syncthreads();
     }
 }
 
@@ -967,8 +975,20 @@ __global__ __launch_bounds__(64) void mlp_ok_pauli_kernel(PauliInput<MlpView<T>>
 }
 
 // O_k of PsiClassical: L_n(s) for n < num_ops, then the reference network's O_k
+// O_k rows of the reference network (same LDS layout as its own O_k kernel)
 template <typename T>
-__global__ __launch_bounds__(64) void classical_ok_kernel(ClassicalView<T> v, const uint64_t* confs, int words,
+__device__ __forceinline__ void ref_ok_row(const MlpView<T>& net, const uint64_t* w, unsigned char* smem, cplx<T>* row,
+                                           const cplx<double>* center, int lane) {
+    mlp_ok_row(net, w, smem, row, center, lane);
+}
+template <typename T>
+__device__ __forceinline__ void ref_ok_row(const CnnView<T>& net, const uint64_t* w, unsigned char* smem, cplx<T>* row,
+                                           const cplx<double>* center, int lane) {
+    cnn_ok_row(net, w, smem, row, center, lane);
+}
+
+template <typename T, typename R>
+__global__ __launch_bounds__(64) void classical_ok_kernel(ClassicalView<T, R> v, const uint64_t* confs, int words,
                                                           int64_t count, const cplx<double>* center, cplx<T>* O) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
@@ -989,9 +1009,9 @@ __global__ __launch_bounds__(64) void classical_ok_kernel(ClassicalView<T> v, co
             if (v.basis == BASIS_PAULIS) {
                 uint64_t e[kMaxWords];
                 pauli_units(w, v.N, e);
-                mlp_ok_row(v.ref, e, smem, row + v.num_ops, c2, lane);
+                ref_ok_row(v.ref, e, smem, row + v.num_ops, c2, lane);
             } else {
-                mlp_ok_row(v.ref, w, smem, row + v.num_ops, c2, lane);
+                ref_ok_row(v.ref, w, smem, row + v.num_ops, c2, lane);
             }
         }
         __syncthreads();
@@ -1066,12 +1086,12 @@ template <typename V>
 size_t ok_lds(const PauliInput<V>& net) {
     return ok_lds(net.net);
 }
-template <typename T>
-size_t forward_lds(const ClassicalView<T>& v) {
+template <typename T, typename R>
+size_t forward_lds(const ClassicalView<T, R>& v) {
     return v.has_ref ? forward_lds(v.ref) : 0;
 }
-template <typename T>
-size_t ok_lds(const ClassicalView<T>& v) {
+template <typename T, typename R>
+size_t ok_lds(const ClassicalView<T, R>& v) {
     return v.has_ref ? ok_lds(v.ref) : 0;
 }
 
@@ -1156,8 +1176,9 @@ void launch_generic_ok(const V& net, const uint64_t* confs, int words, int64_t c
         allow_lds(mlp_ok_pauli_kernel<T>, ok_lds(net));
         mlp_ok_pauli_kernel<T><<<grid_for(count), 64, ok_lds(net), stream>>>(net, confs, count, center, O);
     } else {
-        allow_lds(classical_ok_kernel<T>, ok_lds(net));
-        classical_ok_kernel<T><<<grid_for(count), 64, ok_lds(net), stream>>>(net, confs, words, count, center, O);
+        using R = decltype(net.ref);
+        allow_lds(classical_ok_kernel<T, R>, ok_lds(net));
+        classical_ok_kernel<T, R><<<grid_for(count), 64, ok_lds(net), stream>>>(net, confs, words, count, center, O);
     }
     ANN_LAUNCH_CHECK();
 }
@@ -1190,6 +1211,10 @@ ANN_GENERIC_INSTANTIATE(PauliInput<MlpView<float>>)
 ANN_GENERIC_INSTANTIATE(PauliInput<MlpView<double>>)
 ANN_GENERIC_INSTANTIATE(ClassicalView<float>)
 ANN_GENERIC_INSTANTIATE(ClassicalView<double>)
+using ClassicalCnn32 = ClassicalView<float, CnnView<float>>;
+using ClassicalCnn64 = ClassicalView<double, CnnView<double>>;
+ANN_GENERIC_INSTANTIATE(ClassicalCnn32)
+ANN_GENERIC_INSTANTIATE(ClassicalCnn64)
 
 #define ANN_PAULI_INSTANTIATE(V)                                                                          \
     template void launch_generic_local_energy_pauli<V>(const V&, const PauliOpView&, const uint64_t*, int64_t, \

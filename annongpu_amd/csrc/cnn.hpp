@@ -82,7 +82,8 @@ struct PauliInput {
 //   spins (PauliString.hpp:242-254) or on Pauli strings (:257-276).
 // O_k = L_k(s) for k < num_ops, then the reference's O_k.  The fully polarised reference
 // (PsiFullyPolarized.hpp:31-40) has log psi_ref = 0 and no parameters: has_ref = 0.
-template <typename T>
+// R: the reference network's view (MlpView<T> for a PsiDeep, CnnView<T> for a PsiCNN)
+template <typename T, typename R = MlpView<T>>
 struct ClassicalView {
     using real_t = T;
     int N;               // sites
@@ -95,7 +96,7 @@ struct ClassicalView {
     const uint64_t* sb;      // [strings][kMaxWords]
     const double* coef;      // [strings][2]: c_m, times (-i)^{#Y of P_m} for the spin basis
     const cplx<T>* params;   // [num_ops]
-    MlpView<T> ref;          // over N spins, or 3 N units of a Pauli-string configuration
+    R ref;                   // over N spins (a PsiDeep also over the 3 N units of a Pauli string)
     cplx<T> ref_lp;          // the reference's log_prefactor
 };
 

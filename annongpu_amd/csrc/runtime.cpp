@@ -112,6 +112,7 @@ static hipStream_t S() { return Runtime::get().stream(); }
 static void check_basis(const Psi& psi, int basis) {
     if (basis == BASIS_PAULIS) {
         if (psi.num_sites > 64) throw std::invalid_argument("Pauli-string configurations hold at most 64 sites");
+        if (psi.cnn_ref()) throw std::logic_error("a PsiClassical with a PsiCNN reference reads spins only");
         if (psi.kind == PSI_CLASSICAL || psi.pauli_net()) return;
         throw std::logic_error("on Pauli strings: PsiClassical, or a PsiDeep with 3 input units per site");
     }
@@ -133,6 +134,11 @@ static void with_generic(Psi& psi, F&& f, int basis = BASIS_SPINS) {
             f(psi.cnn64());
         else
             f(psi.cnn32());
+    } else if (psi.cnn_ref()) {
+        if (d)
+            f(psi.classical_cnn64());
+        else
+            f(psi.classical_cnn32());
     } else if (psi.kind == PSI_CLASSICAL) {
         if (d)
             f(psi.classical64(basis));
@@ -470,9 +476,9 @@ MlpView<float> Psi::mlp32() {
     return make_mlp_view<float>(*this, mlp_w32.p, mlp_b32.p, f32.p);
 }
 
-template <typename T>
-static ClassicalView<T> make_classical_view(Psi& q, int basis, const cplx<T>* params) {
-    ClassicalView<T> v{};
+template <typename T, typename R = MlpView<T>>
+static ClassicalView<T, R> make_classical_view(Psi& q, int basis, const cplx<T>* params) {
+    ClassicalView<T, R> v{};
     v.N = q.num_sites;
     v.basis = basis;
     v.num_ops = int(q.params.size());
@@ -503,6 +509,28 @@ ClassicalView<float> Psi::classical32(int basis) {
     if (v.has_ref) {
         ref->precision = precision;
         v.ref = ref->mlp32();
+        v.ref_lp = {float(ref->log_prefactor.real()), float(ref->log_prefactor.imag())};
+    }
+    return v;
+}
+
+ClassicalView<double, CnnView<double>> Psi::classical_cnn64() {
+    sync_device();
+    auto v = make_classical_view<double, CnnView<double>>(*this, BASIS_SPINS, cp64.p);
+    if (v.has_ref) {
+        ref->precision = precision;
+        v.ref = ref->cnn64();
+        v.ref_lp = {ref->log_prefactor.real(), ref->log_prefactor.imag()};
+    }
+    return v;
+}
+
+ClassicalView<float, CnnView<float>> Psi::classical_cnn32() {
+    sync_device();
+    auto v = make_classical_view<float, CnnView<float>>(*this, BASIS_SPINS, cp32.p);
+    if (v.has_ref) {
+        ref->precision = precision;
+        v.ref = ref->cnn32();
         v.ref_lp = {float(ref->log_prefactor.real()), float(ref->log_prefactor.imag())};
     }
     return v;

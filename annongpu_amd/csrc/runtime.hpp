@@ -265,6 +265,10 @@ struct Psi {
     MlpView<float> mlp32();
     ClassicalView<double> classical64(int basis);
     ClassicalView<float> classical32(int basis);
+    // the same with a PsiCNN reference (spin basis)
+    ClassicalView<double, CnnView<double>> classical_cnn64();
+    ClassicalView<float, CnnView<float>> classical_cnn32();
+    bool cnn_ref() const { return kind == PSI_CLASSICAL && ref && ref->kind == PSI_CNN; }
     void upload_mlp_tables();
     std::vector<cd> get_params() const;
     void set_params(const cd* p);

@@ -81,8 +81,9 @@ int ann_psi_create_cnn(ann_psi* out, const int32_t* extent, int32_t num_layers, 
  * gpu) -- quantum_state/PsiClassical.hpp:48-160,204-225, source/quantum_state/PsiClassical.cu:9-61,
  * MAIN:208-270.  H_local[n] = ops[n] (copied), params[num_ops] interleaved complex;
  * log psi(s) = log_prefactor + sum_n params[n] L_n(s), L_n = the matrix-element sum of
- * H_local[n] (Operator.hpp:123-135).  ref: a PsiDeep (copied; the ANN form) or null for
- * PsiFullyPolarized (log psi_ref = 0, no parameters, PsiFullyPolarized.hpp:31-40).  With
+ * H_local[n] (Operator.hpp:123-135).  ref (copied): a PsiCNN (the ANN form,
+ * PsiClassical.hpp:245-256; spin configurations only), a PsiDeep (also on Pauli strings), or
+ * null for PsiFullyPolarized (log psi_ref = 0, no parameters, PsiFullyPolarized.hpp:31-40).  With
  * order 2 and a network reference log psi_ref(s) enters log psi and the reference's
  * parameters follow the classical ones in params and O_k. */
 int ann_psi_create_classical(ann_psi* out, int32_t num_sites, int32_t num_ops, const ann_operator* ops,

@@ -5,7 +5,9 @@ the fp64 C oracle for the PsiDeep parts:
 - PsiClassical (PsiClassical.hpp:79-141) with the reference's fixtures
   (test/conftest.py:80-118): the fully polarised reference (order 1) and a PsiDeep
   reference (orders 1 and 2, whose parameters follow the classical ones), log psi of every
-  configuration and O_k, on spins and on Pauli strings;
+  configuration and O_k, on spins and on Pauli strings; the PsiCNN reference of the
+  reference's PsiClassicalANN (PsiClassical.hpp:245-256): log psi, O_k, exact TDVP
+  statistics and a Monte Carlo energy on spins;
 - the Pauli-string basis (PauliString.hpp, ExactSummation / MonteCarlo over
   PauliString): a PsiDeep with three input units per site (test/test_Psi.py:80-107),
   apply_operator against the Pauli expansion of H Psi, a Monte Carlo energy within 4
@@ -215,6 +217,99 @@ def test_tdvp_exact_on_pauli_strings(precision):
     assert rel(tdvp.O_k_vector, Ob) < tol
     assert rel(tdvp.F_vector, F) < tol
     assert rel(tdvp.S_matrix, S) < tol
+
+
+# ------------------------------------------- PsiClassical over a PsiCNN reference
+
+def classical_cnn(order, precision="fp64", seed=11):
+    """the reference's PsiClassicalANN (PsiClassical.hpp:245-256: the network reference is a
+    PsiCNN) on a 2x4 periodic lattice, and its oracle: the classical terms from the numpy
+    restatement, the reference network from the C oracle's PsiCNN"""
+    rng = np.random.default_rng(seed)
+    extent, chans, conns = [1, 2, 4], [2, 2], [[1, 2, 3], [1, 1, 3]]
+    N = 8
+    sym = np.zeros(N, dtype=np.uint32)
+    n = 1 * 2 * 6 + 2 * 2 * 3
+    cparams = cgauss(rng, n, 0.2)
+    ref = A.PsiCNN(extent, chans, conns, sym, cparams, 0.9, 0.05 + 0.1j)
+    oref = orc.cnn(extent, chans, conns, sym, cparams, 0.9, 0.05 + 0.1j)
+    H = A.sigma_z(0) * A.sigma_x(1) * A.sigma_z(2) + 1.2 * A.sigma_y(0) + 1.1 * A.sigma_x(0) * A.sigma_x(5)
+    psi = A.new_classical_network(N, order, H, psi_ref=ref)
+    assert psi.num_params == len(psi.H_local) + (n if order == 2 else 0)
+    p = psi.params.copy()
+    p[:len(psi.H_local)] = 0.1 * cgauss(rng, len(psi.H_local), 1.0)
+    psi.params = p
+    psi.precision = precision
+    ocl = co.Classical([list(t) for t in H.term_list()], psi.params[:len(psi.H_local)], psi.log_prefactor,
+                       ref_log_psi=lambda c: orc.log_psi_s(oref, orc.conf_words(c, 4)),
+                       ref_O_k=lambda c: orc.psi_O_k(oref, orc.conf_words(c, 4)), order=order)
+    return N, psi, ocl, oref
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("order", [1, 2])
+def test_classical_cnn_reference_log_psi_and_O_k(order, precision):
+    N, psi, ocl, _ = classical_cnn(order, precision)
+    ex = A.ExactSummationSpins(N)
+    got = A.log_psi_vector(psi, ex)
+    want = np.array([ocl.log_psi(s, "spins") for s in range(2 ** N)])
+    assert np.max(np.abs(got - want)) < TOL[precision] * max(1.0, np.abs(want).max())
+    for s in (0, 3, 77, 2 ** N - 1):
+        assert rel(A.psi_O_k(psi, A.Spins(s, N)), ocl.O_k(s, "spins")) < 10 * TOL[precision]
+    # the reference network travels with the classical one
+    r = psi.psi_ref
+    assert isinstance(r, A.PsiCNN) and np.array_equal(r.params, psi.params[len(psi.H_local):]) == (order == 2)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_classical_cnn_reference_tdvp_exact(precision):
+    N, psi, ocl, _ = classical_cnn(2, precision, seed=12)
+    ex = A.ExactSummationSpins(N)
+    psi.normalize(ex)
+    ocl.lp = complex(psi.log_prefactor)
+    H = A.sigma_x(0) * A.sigma_x(1) + 0.7 * A.sigma_z(2) + 0.3 * A.sigma_y(3) * A.sigma_y(4)
+    tdvp = A.TDVP(psi.num_params)
+    tdvp.eval(A.Operator(H), psi, ex)
+    lps = np.array([ocl.log_psi(s, "spins") for s in range(2 ** N)])
+    w = np.exp(2 * lps.real)
+    w /= w.sum()
+    Hd = expr_dense(H, N)
+    psi_v = np.exp(lps)
+    El = (Hd @ psi_v) / psi_v
+    O = np.array([ocl.O_k(s, "spins") for s in range(2 ** N)])
+    E = w @ El
+    Ob = w @ O
+    Oc = O - Ob[None, :]
+    F = (w * (El - E)) @ O.conj()
+    S = Oc.conj().T @ (w[:, None] * Oc)
+    tol = 100 * TOL[precision]
+    assert abs(tdvp.E_local - E) < tol * abs(E)
+    assert rel(tdvp.O_k_vector, Ob) < tol
+    assert rel(tdvp.F_vector, F) < tol
+    assert rel(tdvp.S_matrix, S) < tol
+
+
+def test_classical_cnn_reference_monte_carlo_energy():
+    N, psi, ocl, _ = classical_cnn(2, "fp64", seed=13)
+    H = A.sigma_z(0) * A.sigma_z(1) + 0.5 * A.sigma_x(2) + 0.4 * A.sigma_x(0) * A.sigma_x(4)
+    lps = np.array([ocl.log_psi(s, "spins") for s in range(2 ** N)])
+    w = np.exp(2 * (lps.real - lps.real.max()))
+    w /= w.sum()
+    psi_v = np.exp(lps - lps.real.max())
+    E_exact = (w @ ((expr_dense(H, N) @ psi_v) / psi_v)).real
+    # one sample per chain after thermalisation: independent samples
+    mc = A.MonteCarloSpins(4096, 50, 40, 4096, seed=5)
+    tdvp = A.TDVP(psi.num_params)
+    tdvp.eval(A.Operator(H), psi, mc)
+    samples = tdvp.E_local_samples.real
+    se = samples.std() / math.sqrt(len(samples))
+    assert abs(tdvp.E_local.real - E_exact) < 4.5 * se, (tdvp.E_local, E_exact, se)
+
+
+def test_classical_cnn_reference_on_pauli_strings_is_refused():
+    N, psi, _, _ = classical_cnn(2)
+    with pytest.raises(Exception):
+        A.log_psi_vector(psi, A.ExactSummationPaulis(N))
 
 
 # ------------------------------------------------------------ KullbackLeibler

@@ -3,8 +3,9 @@ json_numpy.py:5-38) against fixtures that spell it out by hand
 (tests/golden/make_json_fixtures.py): from_json reads them into the numbers they hold,
 and to_json writes back exactly the same objects (dtypes included), so files move
 between the reference and this package in both directions.  The classical network's
-format (pyANNonGPU/PsiClassical.py:7-67) is checked by a round trip; its ansatz entries
-use this package's PauliExpression encoding.  The GPU test evaluates the loaded
+format (pyANNonGPU/PsiClassical.py:7-67) is checked by a round trip, with the fully
+polarised, a PsiDeep and a PsiCNN reference; its ansatz entries use this package's
+PauliExpression encoding.  The GPU test evaluates the loaded
 network against the oracle built from the fixture's numbers.
 """
 import json
@@ -69,6 +70,14 @@ def test_classical_network_round_trip():
     assert c2.num_params == 3 + ref.num_params
     back2 = A.PsiClassical.from_json(c2.to_json())
     assert np.array_equal(back2.params, c2.params) and back2.order == 2
+    # a PsiCNN reference (the reference's PsiClassicalANN)
+    cref = A.PsiCNN.from_json(load("psicnn_reference_format.json"))
+    c3 = A.new_classical_network(cref.num_sites, 2, A.sigma_x(0) + A.sigma_z(1) * A.sigma_z(2), psi_ref=cref)
+    js = c3.to_json()
+    assert js["psi_ref"]["type"] == "PsiCNN"
+    back3 = A.PsiClassical.from_json(json.loads(json.dumps(js)))
+    assert isinstance(back3.psi_ref, A.PsiCNN) and np.array_equal(back3.params, c3.params)
+    assert back3.num_params == 2 + cref.num_params
 
 
 @pytest.mark.gpu
