@@ -65,6 +65,8 @@ _SIGNATURES = {
     "ann_expectation_value": [P, P, P, P],
     "ann_expectation_fluctuation": [P, P, P, P, P],
     "ann_expectation_gradient": [P, P, P, P, P],
+    "ann_expectation_exp_sigma_z": [P, P, P, P],
+    "ann_expectation_value_reweighted": [P, P, P, P, P],
     "ann_local_energies": [P, P, P, P, i64],
     "ann_tdvp_create": [P, u32],
     "ann_tdvp_destroy": [P],

@@ -1108,11 +1108,31 @@ class ExpectationValue:
     def __init__(self, gpu=True):
         pass
 
-    def __call__(self, operator_, psi, ensemble):
+    def __call__(self, operator_, psi, *args):
+        """(operator, psi, ensemble), (operator list, psi, ensemble), or (operator, psi,
+        psi_sampling, ensemble): <operator> in psi from the samples of psi_sampling, weighted
+        by |psi / psi_sampling|^2 (ExpectationValue.hpp:51-59, ExpectationValue.cu.template:
+        130-172; the reference divides by an accumulator it never fills, here the weights'
+        sum)"""
+        if len(args) == 2:
+            psi_sampling, ensemble = args
+            out = np.zeros(2)
+            check(lib.ann_expectation_value_reweighted(operator_._h, psi._h, psi_sampling._h, ensemble._h, ptr(out)))
+            return complex(out[0], out[1])
+        if len(args) != 1:
+            raise TypeError("ExpectationValue(operator, psi, [psi_sampling,] ensemble)")
+        ensemble = args[0]
         if isinstance(operator_, (list, tuple)):
             return np.array([self(op, psi, ensemble) for op in operator_])
         out = np.zeros(2)
         check(lib.ann_expectation_value(operator_._h, psi._h, ensemble._h, ptr(out)))
+        return complex(out[0], out[1])
+
+    def exp_sigma_z(self, operator_, psi, ensemble):
+        """ExpectationValue.cu.template:52-77: the weighted sum over the samples of psi of
+        exp(Operator::fast_local_energy(s)), the matrix-element sum of the operator's strings"""
+        out = np.zeros(2)
+        check(lib.ann_expectation_exp_sigma_z(operator_._h, psi._h, ensemble._h, ptr(out)))
         return complex(out[0], out[1])
 
     def fluctuation(self, operator_, psi, ensemble):
@@ -1126,6 +1146,11 @@ class ExpectationValue:
         out = np.zeros(2)
         check(lib.ann_expectation_gradient(operator_._h, psi._h, ensemble._h, ptr(g), ptr(out)))
         return g, complex(out[0], out[1])
+
+    def gradient_with_noise(self, operator_, psi, ensemble):
+        """MAIN:427 binds this name to ExpectationValue::gradient (the noise variant's body is
+        commented out in ExpectationValue.cu.template:278-337): the same (gradient, value)"""
+        return self.gradient(operator_, psi, ensemble)
 
 
 class HilbertSpaceDistance:

@@ -820,6 +820,28 @@ int ann_expectation_fluctuation(ann_operator op, ann_psi psi, ann_ensemble ens, 
     });
 }
 
+int ann_expectation_exp_sigma_z(ann_operator op, ann_psi psi, ann_ensemble ens, double* value2) {
+    return guard([&] {
+        need(op, "op");
+        need(psi, "psi");
+        need(ens, "ens");
+        need(value2, "value");
+        *C(value2) = ann::expectation_exp_fast(op->impl, psi->impl, ens->impl);
+    });
+}
+
+int ann_expectation_value_reweighted(ann_operator op, ann_psi psi, ann_psi psi_sampling, ann_ensemble ens,
+                                     double* value2) {
+    return guard([&] {
+        need(op, "op");
+        need(psi, "psi");
+        need(psi_sampling, "psi_sampling");
+        need(ens, "ens");
+        need(value2, "value");
+        *C(value2) = ann::expectation_reweighted(op->impl, psi->impl, psi_sampling->impl, ens->impl);
+    });
+}
+
 int ann_expectation_gradient(ann_operator op, ann_psi psi, ann_ensemble ens, double* gradient, double* value2) {
     return guard([&] {
         need(op, "op");

@@ -1032,7 +1032,79 @@ inline unsigned grid_for(int64_t count) {
     return unsigned(g > 0 ? g : 1);
 }
 
+// exp(L(s)) of Operator::fast_local_energy (Operator.hpp:123-135): one thread per
+// configuration; the string tables are read as wave-uniform loads
+__global__ __launch_bounds__(256) void exp_fast_energy_kernel(OpView op, const uint64_t* confs, int64_t count,
+                                                              cplx<double>* out) {
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= count) return;
+    const int words = op.words;
+    uint64_t w[kMaxWords];
+#pragma unroll
+    for (int k = 0; k < kMaxWords; k++) w[k] = k < words ? confs[s * words + k] : 0;
+    double re = 0.0, im = 0.0;
+    const int n_end = op.group_begin[op.num_groups];
+    for (int n = 0; n < n_end; n++) {
+        int par = 0;
+#pragma unroll
+        for (int k = 0; k < kMaxWords; k++)
+            if (k < words) par += __popcll(~w[k] & op.string_zy[int64_t(n) * words + k]);
+        const double sg = (par & 1) ? -1.0 : 1.0;
+        re += sg * op.string_coef[2 * n];
+        im += sg * op.string_coef[2 * n + 1];
+    }
+    const double m = exp(re);
+    double sn, cs;
+    sincos(im, &sn, &cs);
+    out[s] = {m * cs, m * sn};
+}
+
+__global__ __launch_bounds__(256) void exp_fast_energy_pauli_kernel(PauliOpView op, const uint64_t* confs,
+                                                                    int64_t count, cplx<double>* out) {
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= count) return;
+    const uint64_t xa = confs[2 * s], xb = confs[2 * s + 1];
+    double re = 0.0, im = 0.0;
+    for (int n = 0; n < op.num_strings; n++) {
+        const cplx<double> f = pauli_phase(op.ab[2 * n], op.ab[2 * n + 1], xa, xb);
+        const double cr = op.coef[2 * n], ci = op.coef[2 * n + 1];
+        re += cr * f.re - ci * f.im;
+        im += cr * f.im + ci * f.re;
+    }
+    const double m = exp(re);
+    double sn, cs;
+    sincos(im, &sn, &cs);
+    out[s] = {m * cs, m * sn};
+}
+
+__global__ void reweight_kernel(const double* w, const cplx<double>* lp, const cplx<double>* lps, int64_t count,
+                                double* out) {
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s < count) out[s] = w[s] * exp(2.0 * (lp[s].re - lps[s].re));
+}
+
 }  // namespace
+
+void launch_exp_fast_energy(const OpView& op, const uint64_t* confs, int64_t count, cplx<double>* out,
+                            hipStream_t stream) {
+    if (count <= 0) return;
+    exp_fast_energy_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(op, confs, count, out);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_exp_fast_energy_pauli(const PauliOpView& op, const uint64_t* confs, int64_t count, cplx<double>* out,
+                                  hipStream_t stream) {
+    if (count <= 0) return;
+    exp_fast_energy_pauli_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(op, confs, count, out);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_reweight(const double* w, const cplx<double>* lp, const cplx<double>* lp_sampling, int64_t count,
+                     double* out, hipStream_t stream) {
+    if (count <= 0) return;
+    reweight_kernel<<<unsigned((count + 255) / 256), 256, 0, stream>>>(w, lp, lp_sampling, count, out);
+    ANN_LAUNCH_CHECK();
+}
 
 template <typename T>
 size_t forward_lds(const CnnView<T>& net) {

@@ -126,6 +126,17 @@ template <typename V>
 void launch_generic_local_energy_pauli(const V& net, const PauliOpView& op, const uint64_t* confs, int64_t count,
                                        cplx<double>* e_loc, hipStream_t stream);
 
+// out[s] = exp(L(s)), L = the matrix-element sum of the operator's strings at
+// configuration s (Operator::fast_local_energy, operator/Operator.hpp:123-135): on spins
+// from the grouped strings, on Pauli strings (confs [count][2]) from the string phases
+void launch_exp_fast_energy(const OpView& op, const uint64_t* confs, int64_t count, cplx<double>* out,
+                            hipStream_t stream);
+void launch_exp_fast_energy_pauli(const PauliOpView& op, const uint64_t* confs, int64_t count, cplx<double>* out,
+                                  hipStream_t stream);
+// out[s] = w[s] exp(2 (Re lp[s] - Re lp_sampling[s]))
+void launch_reweight(const double* w, const cplx<double>* lp, const cplx<double>* lp_sampling, int64_t count,
+                     double* out, hipStream_t stream);
+
 // ------------------------------------------------------- materialised O [n][P]
 // u[s] = sum_k O[s][k] v[k]   (fp64 accumulation)
 template <typename T>

@@ -1591,6 +1591,51 @@ void ok_column_sums_t(Psi& psi, Samples& s, int nsum, const cplx<double>* const*
 
 }  // namespace
 
+cd expectation_exp_fast(Operator& op, Psi& psi, Ensemble& ens) {
+    if (op.max_site() >= psi.num_sites) throw std::invalid_argument("operator acts on sites beyond num_sites");
+    sample(psi, ens);
+    Samples& s = ens.samples;
+    DevBuf<cplx<double>> e;
+    DevBuf<double> sums;
+    e.ensure(s.count);
+    sums.ensure(4);
+    if (s.basis == BASIS_PAULIS)
+        launch_exp_fast_energy_pauli(op.device_pauli().view(), s.confs.p, s.count, e.p, S());
+    else
+        launch_exp_fast_energy(op.device(s.words).view(), s.confs.p, s.count, e.p, S());
+    launch_weighted_sums(e.p, s.w.p, s.count, sums.p, S());
+    double h[4];
+    ANN_HIP_CHECK(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    return {h[0], h[1]};
+}
+
+cd expectation_reweighted(Operator& op, Psi& psi, Psi& psi_sampling, Ensemble& ens) {
+    if (psi.num_sites != psi_sampling.num_sites) throw std::invalid_argument("psi and psi_sampling differ in num_sites");
+    if (op.max_site() >= psi.num_sites) throw std::invalid_argument("operator acts on sites beyond num_sites");
+    // samples and log psi of psi_sampling; log psi and E_loc of psi on the same configurations
+    sample(psi_sampling, ens);
+    Samples& s = ens.samples;
+    const int64_t n = s.count;
+    psi.sync_device();
+    Samples sc;
+    prepare_samples(psi, sc, n, s.basis);
+    ANN_HIP_CHECK(hipMemcpyAsync(sc.confs.p, s.confs.p, sizeof(uint64_t) * n * s.words, hipMemcpyDeviceToDevice,
+                                 S()));
+    log_psi_of(psi, sc.confs.p, n, sc, nullptr);
+    local_energies(psi, op, sc);
+    DevBuf<double> wr, sums;
+    wr.ensure(n);
+    sums.ensure(4);
+    launch_reweight(s.w.p, sc.log_psi.p, s.log_psi.p, n, wr.p, S());
+    launch_weighted_sums(sc.e_loc.p, wr.p, n, sums.p, S());
+    double h[4];
+    ANN_HIP_CHECK(hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, S()));
+    Runtime::get().sync();
+    if (!(h[3] > 0.0)) throw std::runtime_error("ExpectationValue: the reweighted samples carry no weight");
+    return cd(h[0], h[1]) / h[3];
+}
+
 double kullback_leibler(Kl& kl, Psi& psi, Psi& psi_prime, Ensemble& ens, double threshold, int mode, double nu,
                         cd* grad, double* noise) {
     if (psi.num_sites != psi_prime.num_sites) throw std::invalid_argument("psi and psi_prime differ in num_sites");

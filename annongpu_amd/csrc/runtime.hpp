@@ -481,6 +481,12 @@ void prepare_samples(Psi& psi, Samples& s, int64_t count, int basis = BASIS_SPIN
 // sum_s coef_s op(O_s) in parameter order (op: 0 identity, 1 conjugate), any wave function
 void ok_weighted_sum(Psi& psi, Samples& s, const cplx<double>* coef, int conj, cplx<double>* out);
 void local_energies(Psi& psi, Operator& op, Samples& s);
+// ExpectationValue::exp_sigma_z (network_functions/ExpectationValue.cu.template:52-77): the
+// weighted sum over the samples of psi of exp(Operator::fast_local_energy(s))
+cd expectation_exp_fast(Operator& op, Psi& psi, Ensemble& ens);
+// ExpectationValue(op, psi, psi_sampling, ensemble) (ExpectationValue.cu.template:130-170):
+// <op> in psi from the samples of psi_sampling weighted by |psi / psi_sampling|^2
+cd expectation_reweighted(Operator& op, Psi& psi, Psi& psi_sampling, Ensemble& ens);
 void tdvp_eval(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S);
 // y = (S + shift) v on the device; when the matvec also produced the block partials
 // of v^H y, *ypart / *yparts describe them ([yparts][2]), else they are null / 0

@@ -297,7 +297,7 @@ def dense_S_leg(A, steps=2, warmup=1):
                          "peak_tflops": 78.6, "frac": chol_flop / c_avg * 1e-9 / 78.6, "engine": "fp64 MFMA"}}
 
 
-def cnn_quench_leg(A, steps=2, warmup=1):
+def cnn_quench_leg(A, steps=2, warmup=1, progress=False):
     """TFIM 12x12 (h = 3.04) quench from the near-uniform state: PsiCNN with 2 layers of 3x3
     periodic kernels, 8 channels (648 parameters, weights complex-Gaussian 1e-3), 16,384
     persistent single-flip chains (one sweep between samples), RK4 with dt = 0.005 and the
@@ -318,8 +318,11 @@ def cnn_quench_leg(A, steps=2, warmup=1):
     A.profiler_reset()
     A.start_profiling()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         rk.step()
+        if progress and (k + 1) % 20 == 0:
+            print(f"quench step {k + 1}: t = {rk.t:.3f}, E = {rk.energies[-4].real:.4f}, "
+                  f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     A.stop_profiling()
     ms = (time.perf_counter() - t0) / steps * 1e3
     kms = {k: A.profiler_query(k) for k in ("mc_sweep", "local_energy", "o_k", "spin_sums", "gram", "cholesky",
@@ -330,6 +333,9 @@ def cnn_quench_leg(A, steps=2, warmup=1):
     # 8 real flops each
     work = B * N * 8.0 * (9 * 8 * 9 + 25 * 8 * 8 * 9) * max(sweep_n, 1)
     E = np.array(rk.energies).real
+    # the energy at the first evaluation of every step (4 evaluations per RK4 step): a
+    # real-time evolution conserves it up to the Monte Carlo noise and the shift
+    E_steps = E[::4]
     return {"workload": "TFIM 12x12 PBC h=3.04 quench, PsiCNN 2 x (3x3, 8 channels), P=648, 16384 persistent "
                         "single-flip chains, 1 sweep per sample, RK4 dt=0.005, fp32 sampling/E_loc/O_k, dense S "
                         "+ fp64 Cholesky (shift 1e-3)",
@@ -337,6 +343,8 @@ def cnn_quench_leg(A, steps=2, warmup=1):
             "kernel_ms_per_step": {k: v[0] / steps for k, v in kms.items()},
             "kernel_launches": {k: v[1] for k, v in kms.items()},
             "energy_first_last": [float(E[0]), float(E[-1])],
+            "energy_per_step": [float(x) for x in E_steps],
+            "energy_max_rel_change": float(np.max(np.abs(E_steps - E_steps[0])) / abs(E_steps[0])),
             "sweep_roofline": {"bound": "valu", "kernel": "cnn_mc_inc (local updates)",
                                "achieved_tflops": work / (sweep_ms * 1e-3) / 1e12 if sweep_ms else None,
                                "peak_tflops": 157.3,
@@ -366,6 +374,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the energy check and the dense-S leg")
+    ap.add_argument("--quench-only", type=int, default=0, metavar="STEPS",
+                    help="run only the 12x12 CNN quench leg for STEPS RK4 steps and print its record")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X; run with --gpus 1", file=sys.stderr)
@@ -373,6 +383,9 @@ def main():
 
     import annongpu_amd as A
     A.setDevice(0)
+    if args.quench_only > 0:
+        print(json.dumps(cnn_quench_leg(A, steps=args.quench_only, progress=True)))
+        return
     N, psi, H = build_workload(A, args.L, args.alpha)
     op = A.Operator(H)
     B = args.chains

@@ -190,6 +190,18 @@ int ann_activation_function(const double* x2, int32_t layer, double* out2);
 int ann_expectation_value(ann_operator op, ann_psi psi, ann_ensemble ens, double* out2);
 int ann_expectation_fluctuation(ann_operator op, ann_psi psi, ann_ensemble ens, double* sigma, double* value2);
 int ann_expectation_gradient(ann_operator op, ann_psi psi, ann_ensemble ens, double* gradient, double* value2);
+/* ExpectationValue.exp_sigma_z(op, psi, ensemble) -- ExpectationValue.hpp:27,
+ * ExpectationValue.cu.template:52-77, MAIN:428: the weighted sum over the samples of psi of
+ * exp(fast_local_energy(s)) (Operator.hpp:123-135: the matrix-element sum of op's strings
+ * at s), spin or Pauli-string samples; value2 = complex result */
+int ann_expectation_exp_sigma_z(ann_operator op, ann_psi psi, ann_ensemble ens, double* value2);
+/* ExpectationValue(op, psi, psi_sampling, ensemble) -- ExpectationValue.hpp:32-33,
+ * ExpectationValue.cu.template:130-172, MAIN:431: <op> in psi from the samples of
+ * psi_sampling, each weighted by |psi(s) / psi_sampling(s)|^2, divided by the sum of those
+ * weights.  (The reference divides by an accumulator its kernel never adds to,
+ * ExpectationValue.cu.template:135,155-171; this is the self-normalised estimate it names.) */
+int ann_expectation_value_reweighted(ann_operator op, ann_psi psi, ann_psi psi_sampling, ann_ensemble ens,
+                                     double* value2);
 /* additive: per-sample local energies of the last sampling, [num_steps] */
 int ann_local_energies(ann_operator op, ann_psi psi, ann_ensemble ens, double* e_loc, int64_t capacity);
 /* No counterpart in the reference: for callers that cannot allocate device memory (tight

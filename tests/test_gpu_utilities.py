@@ -3,7 +3,9 @@ psi_O_k_vector (PsiOkVector.cu.template:8-37: unweighted sum of O_k over the
 ensemble), psi_vector and the value of log_psi (PsiVector.cu.template:44-99:
 exp(log psi) per configuration; sum_s w_s log psi(s) with the ensemble's weights,
 the unnormalised |psi|^2 for ExactSummation), calibrate (pyANNonGPU/PsiRBM.py:43-49)
-and the operator-list form of ExpectationValue.__call__ (main.cpp.template:424).
+the operator-list form of ExpectationValue.__call__ (main.cpp.template:424), exp_sigma_z
+(:428) against the dense operator's row sums, the reweighted form (:431) against <H> of the
+wave function it reweights to, and gradient_with_noise (:427).
 """
 import cmath
 import math
@@ -120,3 +122,75 @@ def test_expectation_value_of_operator_list():
         ref = orc.tdvp_accumulate(oracle_operator(h, 1), o, orc.exact_confs(N, 1), w, with_S=False)["E"]
         assert abs(g - ref) < 1e-10 * max(1.0, abs(ref))
     assert cmath.isfinite(got[0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_exp_sigma_z(precision):
+    """ExpectationValue.exp_sigma_z (ExpectationValue.cu.template:52-77): sum_s w_s
+    exp(L(s)) with L(s) the row sum of the operator's dense matrix (the matrix-element sum
+    of its strings at s, Operator.hpp:123-135) and w the normalised |psi|^2"""
+    import classical_oracle as co
+    from helpers import expr_dense
+    N = 6
+    spec, psi = _net(precision, N, seed=21)
+    ex = A.ExactSummationSpins(N)
+    psi.normalize(ex)
+    H = A.sigma_z(0) * A.sigma_z(1) + 0.3 * A.sigma_x(2) - 0.2j * A.sigma_y(3) * A.sigma_x(4) + 0.1 * A.sigma_y(5)
+    got = A.ExpectationValue().exp_sigma_z(A.Operator(H), psi, ex)
+    o = spec.oracle()
+    o.s.log_prefactor = orc._cplx(psi.log_prefactor)
+    w = np.exp(2 * orc.exact_log_psi_vector(o, N).real)
+    L = expr_dense(H, N).sum(axis=1)
+    assert all(co.fast_local_energy(list(H), s, "spins") == pytest.approx(L[s]) for s in range(2 ** N))
+    ref = w @ np.exp(L)
+    assert abs(got - ref) < 10 * TOL[precision] * abs(ref)
+
+
+def test_exp_sigma_z_on_pauli_strings():
+    import classical_oracle as co
+    N = 3
+    H = A.sigma_z(0) * A.sigma_x(1) + 0.4 * A.sigma_y(2) + 0.25 * A.sigma_x(0) * A.sigma_y(1)
+    psi = A.new_classical_network(N, 1, A.sigma_x(0) * A.sigma_z(1) + A.sigma_y(2))
+    psi.params = np.array([0.3 - 0.1j, -0.2 + 0.05j])
+    ex = A.ExactSummationPaulis(N)
+    psi.normalize(ex)
+    got = A.ExpectationValue().exp_sigma_z(A.Operator(H), psi, ex)
+    strings = [co.enumerate_pauli(i) for i in range(4 ** N)]
+    lps = A.log_psi_vector(psi, ex)
+    w = np.exp(2 * lps.real)
+    ref = sum(wi * np.exp(co.fast_local_energy(list(H), x, "paulis")) for wi, x in zip(w, strings))
+    assert abs(got - ref) < 1e-10 * abs(ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_expectation_value_reweighted(precision):
+    """ExpectationValue(op, psi, psi_sampling, ensemble) (ExpectationValue.cu.template:130-172)
+    on an exact summation over psi_sampling: the |psi / psi_sampling|^2 reweighting turns the
+    samples of psi_sampling into those of psi, so the result is <H> in psi exactly"""
+    from helpers import expr_dense
+    N = 6
+    _, psi_s = _net(precision, N, seed=31)
+    H_cl = A.sigma_z(0) * A.sigma_x(1) + 1.2 * A.sigma_y(2) + 0.7 * A.sigma_x(3) * A.sigma_x(4)
+    psi = A.new_classical_network(N, 1, H_cl)
+    psi.params = np.array([0.2 + 0.1j, -0.3j, 0.15])
+    psi.precision = precision
+    ex = A.ExactSummationSpins(N)
+    H = ham.transverse_ising_chain(N, 1.0, 0.7)
+    got = A.ExpectationValue()(A.Operator(H), psi, psi_s, ex)
+    v = A.psi_vector(psi, ex)
+    ref = (v.conj() @ expr_dense(H, N) @ v) / (v.conj() @ v)
+    assert abs(got - ref) < 10 * TOL[precision] * abs(ref)
+    # with psi_sampling = psi the reweighting is the identity
+    same = A.ExpectationValue()(A.Operator(H), psi, psi, ex)
+    psi.normalize(ex)
+    assert abs(same - A.ExpectationValue()(A.Operator(H), psi, ex)) < 10 * TOL[precision] * abs(ref)
+
+
+def test_gradient_with_noise_is_the_gradient():
+    N = 6
+    _, psi = _net("fp64", N, seed=41)
+    ex = A.ExactSummationSpins(N)
+    op = A.Operator(ham.transverse_ising_chain(N, 1.0, 0.5))
+    g1, e1 = A.ExpectationValue().gradient(op, psi, ex)
+    g2, e2 = A.ExpectationValue().gradient_with_noise(op, psi, ex)
+    assert np.array_equal(g1, g2) and e1 == e2
