@@ -125,6 +125,13 @@ void launch_sr_diag(int layout, int N, int M, int Mpad, const double* wsum, cons
                     const cplx<double>* Obar, int P, double* diag, hipStream_t stream);
 void launch_rank1(cplx<double>* S, const cplx<double>* o, int P, double alpha, hipStream_t stream);
 void launch_add_diag(cplx<double>* S, int P, double shift, hipStream_t stream);
+// A <- D^-1/2 A D^-1/2 with D = diag(A) (entries <= 0 left unscaled), dinv[i] = D_ii^-1/2:
+// the Cholesky factor of the scaled matrix solves the same system (x = D^-1/2 y for
+// (D^-1/2 A D^-1/2) y = D^-1/2 b) and is far better conditioned when the parameters'
+// scales differ by orders of magnitude
+void launch_equilibrate(cplx<double>* A, int P, double* dinv, hipStream_t stream);
+// x[i] *= dinv[i]
+void launch_scale_vec(cplx<double>* x, int P, const double* dinv, hipStream_t stream);
 template <typename Tin, typename Tout>
 void launch_convert(const cplx<Tin>* a, cplx<Tout>* b, int64_t n, hipStream_t stream);
 void launch_fill(double* a, double v, int64_t n, hipStream_t stream);

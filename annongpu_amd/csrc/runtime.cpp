@@ -1865,9 +1865,16 @@ int tdvp_solve(Tdvp& t, double shift, int method, double rtol, int max_iter, cd*
         {
             ScopedTimer timer("cholesky");
             launch_add_diag(A.p, P, shift, S());
+            // factor the diagonally scaled matrix (same solution, far smaller condition
+            // number when the O_k of different parameters differ in scale)
+            DevBuf<double>& dinv = t.chol_dinv;
+            dinv.ensure(P);
+            launch_equilibrate(A.p, P, dinv.p, S());
             cholesky_lower(A.p, P, info.p, t.chol_work.p, S());
             ANN_HIP_CHECK(hipMemcpyAsync(x.p, t.Fd.p, sizeof(cplx<double>) * P, hipMemcpyDeviceToDevice, S()));
+            launch_scale_vec(x.p, P, dinv.p, S());
             cholesky_solve_inplace(A.p, P, x.p, S());
+            launch_scale_vec(x.p, P, dinv.p, S());
         }
         int hinfo = 0;
         ANN_HIP_CHECK(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, S()));

@@ -394,7 +394,8 @@ struct Tdvp {
     std::vector<cd> O, F;
     bool has_S = false, has_samples = false;
     DevBuf<cplx<double>> S, Obar, Fd;
-    DevBuf<cplx<double>> chol, chol_work;   // Cholesky factor of S + shift and its panel workspace
+    DevBuf<cplx<double>> chol, chol_work;   // Cholesky factor of the scaled S + shift and its panel workspace
+    DevBuf<double> chol_dinv;               // the diagonal scaling of that factor
     // structured O of the last evaluation (dense nets)
     int precision = 0, words = 0, N = 0, M = 0, Mpad = 0, C = 0, layout = 0, npad = 0;
     int64_t count = 0;

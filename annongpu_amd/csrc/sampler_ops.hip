@@ -290,6 +290,31 @@ __global__ void add_diag_kernel(cplx<double>* S, int P, double shift) {
     if (i < P) S[int64_t(i) * P + i].re += shift;
 }
 
+// symmetric diagonal scaling of a Hermitian matrix with a positive diagonal:
+// dinv[i] = 1 / sqrt(A_ii) (1 where A_ii is not positive), A <- D^-1/2 A D^-1/2
+__global__ void diag_rsqrt_kernel(const cplx<double>* A, int P, double* dinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const double d = A[int64_t(i) * P + i].re;
+    dinv[i] = d > 0.0 ? 1.0 / sqrt(d) : 1.0;
+}
+
+__global__ void scale_sym_kernel(cplx<double>* A, int P, const double* dinv) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= int64_t(P) * P) return;
+    const int i = int(e / P), j = int(e % P);
+    const double f = dinv[i] * dinv[j];
+    A[e].re *= f;
+    A[e].im *= f;
+}
+
+__global__ void scale_vec_kernel(cplx<double>* x, int P, const double* dinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    x[i].re *= dinv[i];
+    x[i].im *= dinv[i];
+}
+
 template <typename Tin, typename Tout>
 __global__ void convert_kernel(const cplx<Tin>* a, cplx<Tout>* b, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -417,6 +442,18 @@ void launch_rank1(cplx<double>* S, const cplx<double>* o, int P, double alpha, h
 
 void launch_add_diag(cplx<double>* S, int P, double shift, hipStream_t stream) {
     add_diag_kernel<<<blocks_for(P), 256, 0, stream>>>(S, P, shift);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_equilibrate(cplx<double>* A, int P, double* dinv, hipStream_t stream) {
+    diag_rsqrt_kernel<<<blocks_for(P), 256, 0, stream>>>(A, P, dinv);
+    ANN_LAUNCH_CHECK();
+    scale_sym_kernel<<<blocks_for(int64_t(P) * P), 256, 0, stream>>>(A, P, dinv);
+    ANN_LAUNCH_CHECK();
+}
+
+void launch_scale_vec(cplx<double>* x, int P, const double* dinv, hipStream_t stream) {
+    scale_vec_kernel<<<blocks_for(P), 256, 0, stream>>>(x, P, dinv);
     ANN_LAUNCH_CHECK();
 }
 
