@@ -297,7 +297,7 @@ def dense_S_leg(A, steps=2, warmup=1):
                          "peak_tflops": 78.6, "frac": chol_flop / c_avg * 1e-9 / 78.6, "engine": "fp64 MFMA"}}
 
 
-def cnn_quench_leg(A, steps=2, warmup=1, progress=False):
+def cnn_quench_leg(A, steps=2, warmup=1, progress=False, precision="fp32", shift=1e-3):
     """TFIM 12x12 (h = 3.04) quench from the near-uniform state: PsiCNN with 2 layers of 3x3
     periodic kernels, 8 channels (648 parameters, weights complex-Gaussian 1e-3), 16,384
     persistent single-flip chains (one sweep between samples), RK4 with dt = 0.005 and the
@@ -308,20 +308,39 @@ def cnn_quench_leg(A, steps=2, warmup=1, progress=False):
     n = 1 * 8 * 9 + 8 * 8 * 9
     params = 1e-3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
     psi = A.PsiCNN([1, L, L], [8, 8], [[1, 3, 3], [1, 3, 3]], np.zeros(N, dtype=np.uint32), params, 1.0, 0.0)
-    psi.precision = "fp32"
+    psi.precision = precision
     op = A.Operator(A.hamiltonians.transverse_ising_square(L, 1.0, 3.04))
     B = 16384
     ens = A.MonteCarloSpins(B, 1, 50, B, move="flip", seed=3, persistent=True)
-    rk = A.RK4(psi, op, ens, 0.005, shift=1e-3, method="cholesky")
+    rk = A.RK4(psi, op, ens, 0.005, shift=shift, method="cholesky")
     for _ in range(warmup):
         rk.step()
     A.profiler_reset()
     A.start_profiling()
     t0 = time.perf_counter()
+    failure = None
+    pnorm = [float(np.linalg.norm(psi.params))]
     for k in range(steps):
-        rk.step()
-        if progress and (k + 1) % 20 == 0:
-            print(f"quench step {k + 1}: t = {rk.t:.3f}, E = {rk.energies[-4].real:.4f}, "
+        try:
+            rk.step()
+            pnorm.append(float(np.linalg.norm(psi.params)))
+        except RuntimeError as e:
+            # the record says where the evolution stopped and what S looked like there
+            S = rk.tdvp.S_matrix
+            ev = np.linalg.eigvalsh(S) if np.all(np.isfinite(S)) else None
+            El = rk.tdvp.E_local_samples
+            failure = {"step": k, "t": rk.t, "error": str(e), "S_finite": bool(np.all(np.isfinite(S))),
+                       "params_finite": bool(np.all(np.isfinite(psi.params))),
+                       "E_loc_samples_nonfinite": int(np.sum(~np.isfinite(El))),
+                       "S_diag_min": float(np.min(S.diagonal().real)), "S_diag_max": float(np.max(S.diagonal().real)),
+                       "S_eig_min": float(ev[0]) if ev is not None else None,
+                       "S_eig_max": float(ev[-1]) if ev is not None else None,
+                       "E_last": complex(rk.tdvp.E_local).__repr__()}
+            print(f"quench stopped at step {k}: {failure}", file=sys.stderr, flush=True)
+            steps = max(k, 1)
+            break
+        if progress and ((k + 1) % 20 == 0 or k + 1 > 100 and (k + 1) % 2 == 0):
+            print(f"quench step {k + 1}: t = {rk.t:.3f}, E = {rk.energies[-4].real:.4f}, |params| = {pnorm[-1]:.4g}, "
                   f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     A.stop_profiling()
     ms = (time.perf_counter() - t0) / steps * 1e3
@@ -345,6 +364,8 @@ def cnn_quench_leg(A, steps=2, warmup=1, progress=False):
             "energy_first_last": [float(E[0]), float(E[-1])],
             "energy_per_step": [float(x) for x in E_steps],
             "energy_max_rel_change": float(np.max(np.abs(E_steps - E_steps[0])) / abs(E_steps[0])),
+            "failure": failure, "precision": precision, "shift": shift,
+            "param_norm_per_step": pnorm,
             "sweep_roofline": {"bound": "valu", "kernel": "cnn_mc_inc (local updates)",
                                "achieved_tflops": work / (sweep_ms * 1e-3) / 1e12 if sweep_ms else None,
                                "peak_tflops": 157.3,
@@ -376,6 +397,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the energy check and the dense-S leg")
     ap.add_argument("--quench-only", type=int, default=0, metavar="STEPS",
                     help="run only the 12x12 CNN quench leg for STEPS RK4 steps and print its record")
+    ap.add_argument("--quench-precision", default="fp32", choices=("fp32", "fp64"))
+    ap.add_argument("--quench-shift", type=float, default=1e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X; run with --gpus 1", file=sys.stderr)
@@ -384,7 +407,8 @@ def main():
     import annongpu_amd as A
     A.setDevice(0)
     if args.quench_only > 0:
-        print(json.dumps(cnn_quench_leg(A, steps=args.quench_only, progress=True)))
+        print(json.dumps(cnn_quench_leg(A, steps=args.quench_only, progress=True, precision=args.quench_precision,
+                                        shift=args.quench_shift)))
         return
     N, psi, H = build_workload(A, args.L, args.alpha)
     op = A.Operator(H)
