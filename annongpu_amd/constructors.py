@@ -65,7 +65,7 @@ def new_deep_neural_network(num_sites, N, M, C, initial_value=(0.01 + 1j * math.
                             gpu=True, noise_modulation="auto", final_weights=10, rng=None):
     """new_neural_network.py:31-131 (one-dimensional layouts)."""
     if isinstance(N, (list, tuple)):
-        raise NotImplementedError("multi-dimensional PsiDeep layouts are not part of this build")
+        raise NotImplementedError("multi-dimensional PsiDeep layouts are not built")
     for n, m, c in zip([N] + M[:-1], M, C):
         assert (m * c) % n == 0
         assert c <= n

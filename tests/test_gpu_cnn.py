@@ -116,7 +116,7 @@ def test_cnn_mc_trajectories_match_oracle(move):
     assert ens.acceptance_rate == pytest.approx(ref["acceptances"] / (ref["acceptances"] + ref["rejections"]))
 
 
-def test_cnn_config4_samples_match_oracle():
+def test_cnn_tfim12x12_samples_match_oracle():
     """TFIM 12x12, 2 conv layers 3x3 x 8 channels (N = 144, 3 words): E_loc and O_k
     of device MC samples equal the oracle's on the same samples."""
     psi, o = make_cnn([1, 12, 12], [(8, [1, 3, 3]), (8, [1, 3, 3])], seed=4, sigma=0.05, precision="fp32")
@@ -221,7 +221,7 @@ def test_cnn_local_update_sampler_matches_full_forward(move):
     assert rel(got, ref["E_loc"]) < 1e-5
 
 
-def test_cnn_config4_rk4_quench_at_size():
+def test_cnn_tfim12x12_rk4_quench_at_size():
     """the quench workload at its size: TFIM 12x12 (h = 3.04) from the near-uniform state
     (all CNN weights ~1e-3, the h = infinity ground state up to that noise), 2 layers of
     3x3 kernels with 8 channels (648 parameters), 16,384 single-flip chains, RK4 with

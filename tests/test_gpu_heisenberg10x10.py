@@ -172,7 +172,7 @@ def test_fp32_error_against_fp64_on_the_same_samples(workload):
     }
     oracle = {"E (rel)": abs(t64.E_local - ref["E"]) / abs(ref["E"]),
               "F (rel norm)": rel(t64.F_vector, ref["F"])}
-    print(f"\nconfig-3 shapes, sigma {wl['sigma']}, {CHAINS} samples: fp32 vs fp64 device")
+    print(f"\n10x10 Heisenberg, M = 400, sigma {wl['sigma']}, {CHAINS} samples: fp32 vs fp64 device")
     for k, val in rows.items():
         print(f"  {k:45s} {val:.3e}")
     for k, val in oracle.items():

@@ -74,7 +74,7 @@ def test_psi_O_k(precision, shape):
         assert rel(got, ref) < 10 * TOL[precision]
 
 
-def _config1(precision, seed=1):
+def _tfim16_rbm(precision, seed=1):
     psi = A.seeded_rbm_deep(16, 32, seed)
     psi.precision = precision
     o = orc.deep(16, psi.a, psi.b, psi.connections, psi.W, psi.final_weights, 0.0)
@@ -82,9 +82,9 @@ def _config1(precision, seed=1):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_exact_summation_config1(precision):
+def test_exact_summation_tfim16_rbm(precision):
     """TFIM ring N=16, complex RBM alpha=2: psi vector, norm, E, sigma, gradient."""
-    psi, o = _config1(precision)
+    psi, o = _tfim16_rbm(precision)
     ens = A.ExactSummationSpins(16)
     H = ham.transverse_ising_chain(16, 1.0, 1.0)
     op = A.Operator(H)
@@ -194,9 +194,9 @@ def test_mc_trajectories_match_oracle(move):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_mc_energy_config1(precision):
+def test_mc_energy_tfim16_rbm(precision):
     """MCMC energy within 3 standard errors of exact summation (TFIM N=16, 1024 chains x 200 sweeps)."""
-    psi, o = _config1(precision, seed=11)
+    psi, o = _tfim16_rbm(precision, seed=11)
     H = ham.transverse_ising_chain(16, 1.0, 1.0)
     op = A.Operator(H)
     ex = A.ExactSummationSpins(16)
