@@ -18,10 +18,12 @@ Runtime& Runtime::get() {
     return rt;
 }
 
-hipStream_t Runtime::side() {
-    stream();
-    if (!side_) ANN_HIP_CHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
-    return side_;
+hipStream_t Runtime::side(int k) {
+    if (k < 0 || k > 1) throw std::logic_error("side stream index out of range");
+    const hipStream_t main = stream();
+    (void)main;
+    if (!side_[k]) ANN_HIP_CHECK(hipStreamCreateWithFlags(&side_[k], hipStreamNonBlocking));
+    return side_[k];
 }
 
 hipStream_t Runtime::stream() {
@@ -47,10 +49,11 @@ void Runtime::set_device(int device) {
         pool_.clear();
         ANN_HIP_CHECK(hipStreamDestroy(stream_));
         stream_ = nullptr;
-        if (side_) {
-            ANN_HIP_CHECK(hipStreamSynchronize(side_));
-            ANN_HIP_CHECK(hipStreamDestroy(side_));
-            side_ = nullptr;
+        for (auto& sd : side_) {
+            if (!sd) continue;
+            ANN_HIP_CHECK(hipStreamSynchronize(sd));
+            ANN_HIP_CHECK(hipStreamDestroy(sd));
+            sd = nullptr;
         }
     }
     device_ = device;
@@ -1219,12 +1222,19 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
     }
     ANN_HIP_CHECK(hipEventRecord(t.fork_ev.get(), S()));
     {
-        SideStream side;
+        SideStream side(0);
         ANN_HIP_CHECK(hipStreamWaitEvent(S(), t.fork_ev.get(), 0));
         structured_sums<T>(t, net, s.confs.p, s.words, g.p, 0, coef.p, n, t.Obar.p, 0.0, nullptr);
         // diagonal of S for the Jacobi preconditioner: sum_s w |g_sj|^2 - |<O_k>|^2
         structured_sums<T>(t, net, s.confs.p, s.words, g.p, 2, coef.p, n, nullptr, 0.0, nullptr);
-        if (pre_kfac) {
+        ANN_HIP_CHECK(hipEventRecord(t.join_ev.get(), S()));
+    }
+    if (pre_kfac) {
+        // the K-FAC set-up on a third stream: its few-block, latency-bound kernels run
+        // beside both the local energies and the wide statistics kernels
+        SideStream side(1);
+        ANN_HIP_CHECK(hipStreamWaitEvent(S(), t.fork_ev.get(), 0));
+        {
             ScopedTimer timer("kfac_setup");
             const SpinPlan plan = spin_plan(t.N, t.M, t.Mpad, n, t.layout == LAYOUT_DEEP1);
             launch_kfac_moments(s.confs.p, s.words, t.spin_bits.p, plan.vpad, s.w_uniform,
@@ -1238,12 +1248,13 @@ void tdvp_eval_t(Tdvp& t, Operator& op, Psi& psi, Ensemble& ens, bool with_S) {
             t.kf_inverted = true;
             t.kf_inv_shift = t.kfac_shift;
         }
-        ANN_HIP_CHECK(hipEventRecord(t.join_ev.get(), S()));
+        ANN_HIP_CHECK(hipEventRecord(t.join2_ev.get(), S()));
     }
     local_energies(psi, op, s);
     // E, E2 and sum w stay on the device until the end of the evaluation
     launch_weighted_sums(s.e_loc.p, s.w.p, n, t.sums.p, S());
     ANN_HIP_CHECK(hipStreamWaitEvent(S(), t.join_ev.get(), 0));
+    if (pre_kfac) ANN_HIP_CHECK(hipStreamWaitEvent(S(), t.join2_ev.get(), 0));
     launch_sr_diag(net.layout, net.N, net.M, net.Mpad, t.sums.p + 3, t.Yb.p, t.Obar.p, P, t.diag.p, S());
     // F = sum w conj(O) E_loc - E conj(<O>) (TDVP.cu.template:300) = sum w (E_loc - E) conj(O)
     launch_weight_coef_centered_dev(s.w.p, s.e_loc.p, t.sums.p, n, coef.p, S());

@@ -68,9 +68,9 @@ class Runtime {
 public:
     static Runtime& get();
     hipStream_t stream();
-    // a second stream (default priority) for work that overlaps the library stream;
-    // while a SideStream guard lives, stream() returns it
-    hipStream_t side();
+    // two more streams (default priority) for work that overlaps the library stream;
+    // while a SideStream guard lives, stream() returns the one it names
+    hipStream_t side(int k = 0);
     hipStream_t override_ = nullptr;
     void set_device(int device);
     void sync();
@@ -85,7 +85,7 @@ public:
 
 private:
     hipStream_t stream_ = nullptr;
-    hipStream_t side_ = nullptr;
+    hipStream_t side_[2] = {nullptr, nullptr};
     int device_ = 0;
     struct Pending {
         std::string name;
@@ -100,7 +100,7 @@ private:
 
 // launches inside the guard go to the side stream (no synchronisation may happen there)
 struct SideStream {
-    SideStream() { Runtime::get().override_ = Runtime::get().side(); }
+    explicit SideStream(int k = 0) { Runtime::get().override_ = Runtime::get().side(k); }
     ~SideStream() { Runtime::get().override_ = nullptr; }
     SideStream(const SideStream&) = delete;
     SideStream& operator=(const SideStream&) = delete;
@@ -435,7 +435,7 @@ struct Tdvp {
     double kfac_shift = 0.0;     // shift of the last K-FAC solve
     bool kf_inverted = false;    // kf_Ci / kf_D hold the inverses for the current samples and kf_inv_shift
     double kf_inv_shift = 0.0;
-    OwnedEvent fork_ev, join_ev; // the evaluation's side-stream fork and join
+    OwnedEvent fork_ev, join_ev, join2_ev;   // the evaluation's side-stream fork and joins
     // E, E2, wsum (sums_host) and O, F (from Obar, Fd) of the last evaluation are read
     // to the host on first use
     bool sums_pending = false, of_pending = false;
