@@ -408,8 +408,10 @@ __device__ __forceinline__ void cnn_ok_row(const CnnView<T>& net, const uint64_t
     cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
     cplx<T>* A = Z + net.zoff[net.L];
     cplx<T>* A2 = A + net.max_ch * N;
-    cplx<T>* G = A2 + net.max_ch * N;    // d logpsi / d output activation of the current layer
-    cplx<T>* D = G + net.max_ch * N;     // d logpsi / d pre-activation
+    // d logpsi / d output activation of the current layer: in the forward pass's second
+    // activation buffer, which is dead once the forward pass is done
+    cplx<T>* G = A2;
+    cplx<T>* D = A2 + net.max_ch * N;    // d logpsi / d pre-activation
     {
         cnn_forward(net, w, A, A2, Z, lane);
         const int last = net.L - 1;
@@ -1116,7 +1118,7 @@ size_t forward_lds(const MlpView<T>& net) {
 }
 template <typename T>
 size_t ok_lds(const CnnView<T>& net) {
-    return sizeof(cplx<T>) * (size_t(net.zoff[net.L]) + 4 * size_t(net.max_ch) * net.N);
+    return sizeof(cplx<T>) * (size_t(net.zoff[net.L]) + 3 * size_t(net.max_ch) * net.N);
 }
 template <typename T>
 size_t ok_lds(const MlpView<T>& net) {
