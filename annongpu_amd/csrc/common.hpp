@@ -15,6 +15,9 @@ namespace ann {
 
 constexpr int kWave = 64;
 constexpr int kMaxWords = 4;   // configurations of up to 256 sites
+// accelerator complex dies (XCDs) of an MI355X, each with its own L2; the dispatcher
+// hands workgroup i of a launch to XCD i mod 8
+constexpr int kXcds = 8;
 
 struct HipError : std::runtime_error {
     using std::runtime_error::runtime_error;

@@ -476,9 +476,16 @@ __global__ __launch_bounds__(kStepThreads) void kfac_step_kernel(KfacStep a, int
     };
     bool frozen = false;
     const int nblk = (M + kDBlock - 1) / kDBlock;
-    const int tasks = (N + kStepRows - 1) / kStepRows * nblk;
-    for (int task = blockIdx.x; task < tasks; task += gridDim.x) {
-        const int v0 = task / nblk * kStepRows, b = task % nblk;
+    const int nsg = (N + kStepRows - 1) / kStepRows;
+    const int tasks = nsg * nblk;
+    // tasks in unit-block-major order, handed out in contiguous runs per XCD (workgroup
+    // i runs on XCD i mod 8): the tasks of one unit block share their columns of r and
+    // Ap, which an XCD's L2 then fetches for about tasks / 8 / nsg blocks instead of all
+    const int xslot = gridDim.x % kXcds == 0
+                          ? int(blockIdx.x % kXcds) * int(gridDim.x / kXcds) + int(blockIdx.x / kXcds)
+                          : int(blockIdx.x);
+    for (int task = xslot; task < tasks; task += gridDim.x) {
+        const int b = task / nsg, v0 = task % nsg * kStepRows;
         const int j = b * kDBlock + lane;
         const bool jok = j < M;
         const int jc = jok ? j : M - 1;
