@@ -506,12 +506,12 @@ __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, 
     };
     auto get_beta = [&]() { return cg ? cb.value(dir) : 0.0; };
     if (b < pack_blocks) {
+        // one thread per (site, unit): it reads the parameter once and writes both the
+        // real-part (column tile 2 (j / 16)) and the imaginary-part (2 (j / 16) + 1) entry
         const int64_t e = int64_t(b) * 256 + threadIdx.x;
-        const int ncol = 2 * net.Mpad;
-        const bool in = e < int64_t(KT) * 32 * ncol;
-        const int vis = in ? int(e / ncol) : 0, col = in ? int(e % ncol) : 0;
-        const int nt = col / 16, cc = col % 16;
-        const int j = (nt / 2) * 16 + cc;
+        const bool in = e < int64_t(KT) * 32 * net.Mpad;
+        const int vis = in ? int(e / net.Mpad) : 0, j = in ? int(e % net.Mpad) : 0;
+        const int nt0 = 2 * (j / 16), cc = j % 16;
         // parameter of (site vis, unit j): -1 none, -2 several connections (summed below)
         int64_t k = -1;
         if (in && vis < net.N && j < net.M) {
@@ -541,14 +541,17 @@ __global__ __launch_bounds__(256) void unpack_pack_kernel(DenseView<float> net, 
             }
         }
         if (!in) return;
-        float r = float((nt & 1) ? im : re);
         const int kt = vis / 32, kk = vis % 32;
         const int lane = (kk / 8) * 16 + cc, el = kk % 8;
 #pragma unroll
-        for (int sp = 0; sp < 3; sp++) {
-            const short h = bf16_bits(r);
-            r -= bf16_value(h);
-            frag[((((int64_t(sp) * KT + kt) * NT + nt) * 64 + lane) * 8) + el] = uint16_t(h);
+        for (int h2 = 0; h2 < 2; h2++) {
+            float r = float(h2 ? im : re);
+#pragma unroll
+            for (int sp = 0; sp < 3; sp++) {
+                const short h = bf16_bits(r);
+                r -= bf16_value(h);
+                frag[((((int64_t(sp) * KT + kt) * NT + nt0 + h2) * 64 + lane) * 8) + el] = uint16_t(h);
+            }
         }
         return;
     }
@@ -867,7 +870,7 @@ void launch_sr_pack(const DenseView<float>& net, const WParamMap& wm, const int*
     if (dir.z && (!dir.p_old || !dir.p_new || !dir.newpart || !dir.oldpart || !dir.done || dir.parts < 1 ||
                   dir.parts > 256))
         throw std::invalid_argument("incomplete CG direction update");
-    const int64_t npack = int64_t(KT) * 32 * 2 * net.Mpad;
+    const int64_t npack = int64_t(KT) * 32 * net.Mpad;   // (site, unit) pairs
     const int pack_blocks = int((npack + 255) / 256);
     const int vab_blocks = (net.Mpad + net.N + 255) / 256;
     unpack_pack_kernel<<<unsigned(pack_blocks + vab_blocks + kDotParts), 256, 0, stream>>>(
