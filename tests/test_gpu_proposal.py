@@ -172,6 +172,40 @@ def test_plaquette_block_sampler_preserves_psi_squared(precision, scale, exclusi
           f"rows per evaluated move {st['rows_evaluated'] / max(evaluated, 1):.2f}")
 
 
+@pytest.mark.parametrize("sampler", ["plain", "learned"])
+def test_persistent_chains_do_not_drift(sampler):
+    """persistent chains run for about 27,000 sweeps: the energy of the first window (10
+    thermalisation sweeps) and of the last 8,000 sweeps both within 4 standard errors of the exact
+    S_z = 0 sum, for the plain exchange sampler and for the trained 2x4-plaquette proposal
+    (Metropolized Gibbs form).  A sampler whose error grows with the length of a run (a
+    random stream reused across calls, drifting angles) would fail the last window."""
+    L, N = 4, 16
+    psi = A.seeded_rbm_deep(N, 32, 21, sigma=0.1)
+    psi.precision = "fp32"
+    o = orc.deep(N, psi.a, psi.b, psi.connections, psi.W, psi.final_weights, 0.0)
+    H = ham.heisenberg_square(L)
+    E_ex, _ = _sector_exact(o, N, H)
+    op = A.Operator(H)
+    chains, steps = 2048, 40
+    ens = A.MonteCarloSpins(chains * steps, 2, 10, chains, move="exchange", seed=12, persistent=True)
+    if sampler == "learned":
+        prop = A.BlockProposal(N, A.rect_blocks(L, (2, 4)), hidden=256, seed=5)
+        data = A.MonteCarloSpins(1024 * 32, 2, 20, 1024, move="exchange", seed=2)
+        A.log_psi(psi, data)
+        prop.fit(data, steps=300, batch=2048, lr=3e-3)
+        ens.set_proposal(prop, plain_sweeps=0, exclude_current=True)
+    early = A.local_energies(op, psi, ens).real
+    E0, se0 = _chain_stats(early, chains)
+    for _ in range(240):   # 80 sweeps per call
+        A.local_energies(op, psi, ens)
+    late = np.concatenate([A.local_energies(op, psi, ens).real for _ in range(100)])
+    E1, se1 = _chain_stats(late, chains)
+    print(f"\n{sampler}: exact {E_ex:.5f}; sweeps 10-90 {E0:.5f} +- {se0:.5f}; "
+          f"sweeps 19,290-27,290 {E1:.5f} +- {se1:.5f}")
+    assert abs(E0 - E_ex) < 4 * se0, (E0, E_ex, se0)
+    assert abs(E1 - E_ex) < 4 * se1, (E1, E_ex, se1)
+
+
 def _optimise_ring(N=12, M=24, steps=50, lr=0.02):
     """imaginary-time SR on the Heisenberg ring with exact sums (the polynomial
     activations diverge if the weights grow much further than 50 such steps take them)"""
