@@ -209,26 +209,37 @@ def measure_tau(A, psi, op, proposal, chains=4096, T=400, seed=23, move="exchang
     return tau_int_sokal(el), tau_batch_means(el), ens.acceptance_rate, el
 
 
-def energy_run(A, psi, op, proposal, chains, target_se, max_seconds, seed, chunk=40):
+def energy_run(A, psi, op, proposal, chains, target_se, max_seconds, seed, chunk=40, halves=None):
     """E on persistent chains (one plain or block sweep between samples) until the
     standard error of the mean from the independent chain means is below target_se or
-    max_seconds have passed; returns (E, SE, samples, seconds, last chunk's configurations)"""
+    max_seconds have passed; returns (E, SE, samples, seconds, last chunk's configurations).
+    With a dict as halves, E and SE of the first and of the second half of the run are
+    stored in it (a mean that still moves between the halves is a chain that is still
+    relaxing)."""
     ens = A.MonteCarloSpins(chains * chunk, 1, ENERGY_THERM_SWEEPS, chains, move="exchange", seed=seed,
                             persistent=True)
     if proposal is not None:
         ens.set_proposal(proposal, plain_sweeps=0, exclude_current=True)
     sums = np.zeros(chains)
     n = 0
+    parts = []
     t0 = time.perf_counter()
     se = float("inf")
     while True:
         el = A.local_energies(op, psi, ens).real.reshape(chunk, chains)
-        sums += el.sum(axis=0)
+        parts.append(el.sum(axis=0))
+        sums += parts[-1]
         n += chunk
         cm = sums / n
         se = float(cm.std(ddof=1) / math.sqrt(chains))
         if se < target_se or time.perf_counter() - t0 > max_seconds:
             break
+    if halves is not None and len(parts) >= 2:
+        h = len(parts) // 2
+        for name, sel in (("first", parts[:h]), ("second", parts[h:])):
+            hm = np.sum(sel, axis=0) / (len(sel) * chunk)
+            halves[name] = {"E": float(hm.mean()), "se": float(hm.std(ddof=1) / math.sqrt(chains)),
+                            "samples_per_chain": len(sel) * chunk}
     return float(cm.mean()), se, n * chains, time.perf_counter() - t0, ens.last_samples()[:4096]
 
 
@@ -495,13 +506,20 @@ def main():
     dense = None
     cnn = None
     if not args.no_extra:
-        Ep, sep, n_p, t_p, confs = energy_run(A, psi, op, None, B, args.energy_se, args.energy_seconds, 41)
-        El, sel, n_l, t_l, _ = energy_run(A, psi, op, prop, B, args.energy_se, args.energy_seconds, 43)
+        hp, hl = {}, {}
+        Ep, sep, n_p, t_p, confs = energy_run(A, psi, op, None, B, args.energy_se, args.energy_seconds, 41,
+                                              halves=hp)
+        El, sel, n_l, t_l, _ = energy_run(A, psi, op, prop, B, args.energy_se, args.energy_seconds, 43, halves=hl)
         energy = {"E_plain": Ep, "se_plain": sep, "samples_plain": n_p, "E_learned": El, "se_learned": sel,
                   "samples_learned": n_l, "rel_diff": abs(El - Ep) / abs(Ep),
                   "rel_se_combined": math.hypot(sep, sel) / abs(Ep),
+                  "halves_plain": hp, "halves_learned": hl,
                   "fp32_vs_fp64_same_samples": fp64_check(A, psi, op, confs)}
         print(f"energy: plain {Ep:.4f} +- {sep:.4f}, learned {El:.4f} +- {sel:.4f}", file=sys.stderr, flush=True)
+        for name, hv in (("plain", hp), ("learned", hl)):
+            if hv:
+                print(f"energy halves ({name}): {hv['first']['E']:.4f} +- {hv['first']['se']:.4f} then "
+                      f"{hv['second']['E']:.4f} +- {hv['second']['se']:.4f}", file=sys.stderr, flush=True)
         # 6. dense S and Cholesky
         dense = dense_S_leg(A)
         cnn = cnn_quench_leg(A)
