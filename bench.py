@@ -394,6 +394,8 @@ def main():
     ap.add_argument("--sweeps", type=int, default=100)
     ap.add_argument("--opt-steps", type=int, default=25)
     ap.add_argument("--opt-lr", type=float, default=0.05)
+    ap.add_argument("--opt-therm", type=int, default=30,
+                    help="thermalisation sweeps of the pre-optimisation's persistent chains")
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--cg-iters", type=int, default=200)
     ap.add_argument("--solver", default="cg_kfac", choices=("cg_kfac", "cg"))
@@ -433,7 +435,7 @@ def main():
     # another state whenever an fp32 kernel changes its summation order
     t0 = time.perf_counter()
     psi.precision = "fp64"
-    opt_ens = A.MonteCarloSpins(B, 10, 30, B, move="exchange", seed=5, persistent=True)
+    opt_ens = A.MonteCarloSpins(B, 10, args.opt_therm, B, move="exchange", seed=5, persistent=True)
     opt = Stepper(A, psi, op, opt_ens, args.opt_lr, 100, shift=1e-2, solver="cg", rtol=1e-4)
     for _ in range(args.opt_steps):
         opt.step()
@@ -614,7 +616,7 @@ def main():
                    "learned_vs_plain_flip_matched": head["ess_per_s"] / paths["plain-flip-matched"]["ess_per_s"],
                    "learned_vs_plain_config": head["ess_per_s"] / paths["plain-config"]["ess_per_s"],
                    "paths": paths,
-                   "pre_optimisation": {"sr_steps": args.opt_steps, "lr": args.opt_lr, "E_per_site": E_opt,
+                   "pre_optimisation": {"sr_steps": args.opt_steps, "lr": args.opt_lr, "therm_sweeps": args.opt_therm, "E_per_site": E_opt,
                                         "var_per_site": var_opt, "seconds": t_opt},
                    "training": {"samples": B * 64, "adam_steps": args.train_steps, "batch": 4096,
                                 "loss_before": loss0, "loss_after": loss, "device_ms": train_ms,
