@@ -1144,8 +1144,10 @@ void dispatch_vmax(const CnnView<T>& net, F&& fn) {
 }
 
 // bytes of the LDS copy of a PsiCNN's tables ahead of the local-update state (one chain
-// per workgroup), 0 when they would leave fewer than four chains per CU (the kernels
-// are latency-bound: chains per CU count for more than where the tables live)
+// per workgroup), 0 when they would leave fewer than four chains per CU.  Above that the
+// copy is kept even where leaving the tables in global memory would fit one chain more:
+// on the 12x12, 2 x 8-channel net, 4 chains per CU with the tables in LDS sample faster
+// than 5 without (each multiply-add reads a weight and a neighbour index)
 template <typename T>
 int cnn_stage_bytes(const CnnView<T>& net) {
     const size_t t = cnn_table_lds(net);
